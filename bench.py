@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X tree-code force pass.
+
+Workload (default): Plummer sphere, N = 1,048,576 (a = 1, M = 1, G = 1,
+r < 10 a), FP32, eps = 0.005 (eps2 = 2.5e-5), theta = 0.5, leaf <= 16, 30-bit
+Morton keys over the bounding cube, leapfrog dt = 5e-4, seed 1, linear-BVH
+solver.  A "step" is Solver::step(): tree build + walk + kick/drift, all on
+the device with the state resident in HBM.
+
+Prints one JSON line: bodies*steps/s over the timed steps, interactions/s,
+the walk kernel's FP32 roofline fraction (HIP events on the system stream),
+the oracle CPU baseline timed on this host, force error against an FP64
+direct sum on a 16,384-body sample and the energy drift over the run.
+"""
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "bodies·steps/s at N=1M θ=0.5; batched-serving p50/p99 latency at fixed force err"
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (packed) peak
+FLOPS_PER_INTERACTION = 20
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(b, args, threads):
+    """The oracle's restatement of the reference CPU path, timed on this host:
+    a full LBVH build plus the walk for a target sample, scaled to N."""
+    import oracle
+
+    n = b.n
+    t0 = time.perf_counter()
+    t = oracle.Tree(1, b.x, b.y, b.z, b.m, theta=args.theta, leaf=args.leaf, bits=args.bits, eps2=args.eps2)
+    t_build = time.perf_counter() - t0
+    sample = min(n, args.cpu_sample)
+    sel = np.linspace(0, n - 1, sample).astype(np.int64)
+    t0 = time.perf_counter()
+    t.accel(b.x[sel], b.y[sel], b.z[sel], threads=threads)
+    t_walk = (time.perf_counter() - t0) * (n / sample)
+    t_update = 0.0
+    x = b.x.copy()
+    t0 = time.perf_counter()
+    x += b.vx * np.float32(args.dt)
+    t_update = (time.perf_counter() - t0) * 6  # v and p, 3 components each
+    step_s = t_build + t_walk + t_update
+    return {
+        "value": n / step_s,
+        "unit": "bodies·steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"one LBVH build over all {n} bodies + the walk for {sample} evenly spaced targets, "
+                  f"scaled by N/sample (build {t_build:.2f} s, walk {t_walk:.2f} s extrapolated)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--leaf", type=int, default=16)
+    ap.add_argument("--bits", type=int, default=30)
+    ap.add_argument("--eps2", type=float, default=2.5e-5)
+    ap.add_argument("--dt", type=float, default=5e-4)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--mode", choices=["fast", "ordered"], default="fast")
+    ap.add_argument("--cpu-sample", type=int, default=131072)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (systems shard by body "
+              "or by system with no communication; see DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    nbg = importlib.import_module("n-body_amd")
+    if nbg.device_count() == 0:
+        raise SystemExit("no GPU visible")
+    ctx = nbg.Context(0)
+    log(f"[bench] generating Plummer N={args.n} seed={args.seed}")
+    b = nbg.plummer(args.n, seed=args.seed)
+
+    p = nbg.default_params()
+    p.solver = nbg.LINEAR_BVH
+    p.theta, p.leaf_size, p.morton_bits = args.theta, args.leaf, args.bits
+    p.eps2, p.dt = args.eps2, args.dt
+    p.sum_mode = nbg.SUM_FAST if args.mode == "fast" else nbg.SUM_ORDERED
+    s = nbg.System(ctx, p, args.n)
+    s.set_state(b)
+
+    # force pass at t = 0 (also the energy diagnostic's reference point)
+    a0 = s.update_accelerations()
+    pp0, np0 = s.counts()
+
+    log(f"[bench] warmup {args.warmup} steps")
+    s.step(args.warmup)
+    s.synchronize()
+    s.set_state(b)  # timed run starts from the initial conditions
+    s.enable_timing(True)
+    log(f"[bench] timing {args.steps} steps")
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    s.synchronize()
+    elapsed = time.perf_counter() - t0
+    phases = s.phase_times()
+    pp, npc = s.counts()
+    s.enable_timing(False)
+
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = args.n * args.steps / elapsed
+    inter = pp + npc
+    walk_ms = phases["walk"]
+    achieved = FLOPS_PER_INTERACTION * inter / (walk_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "bodies·steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plummer-{args.n} linear-bvh step (theta={args.theta}, leaf={args.leaf}, "
+                        f"{args.bits}-bit keys, eps2={args.eps2}, dt={args.dt}, seed={args.seed})",
+            "n_bodies": args.n, "theta": args.theta, "leaf_size": args.leaf, "morton_bits": args.bits,
+            "eps2": args.eps2, "dt": args.dt, "sum_mode": args.mode, "solver": "linear-bvh",
+        },
+        "interactions_per_step": inter,
+        "interactions_per_s": inter / (ms_per_step * 1e-3),
+        "phases_ms": phases,
+        "roofline": {
+            "bound": "valu",
+            "kernel": "k_walk_lbvh",
+            "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": None,
+            "note": f"{FLOPS_PER_INTERACTION} flops per interaction (pp + np, SimpleTrackingRule counting) "
+                    f"/ mean walk-kernel time from HIP events on the system stream",
+        },
+    }
+
+    if not args.no_accuracy:
+        log("[bench] accuracy: FP64 direct sum on 16384 bodies, energy drift")
+        st = s.get_state()
+        # synchronised end state: v(t_N) = v_{N-1/2} + a(x_N) dt / 2
+        end = nbg.Bodies(st.x, st.y, st.z, b.m, st.vx, st.vy, st.vz)
+        s2 = nbg.System(ctx, p, args.n)
+        s2.set_state(end)
+        aN = s2.update_accelerations()
+        s0 = nbg.System(ctx, p, args.n)
+        s0.set_state(b)
+        W0 = s0.potential_f64()
+        WN = s2.potential_f64()
+        m = b.m.astype(np.float64)
+        K0 = 0.5 * np.sum(m * (b.vx.astype(np.float64) ** 2 + b.vy.astype(np.float64) ** 2 + b.vz.astype(np.float64) ** 2))
+        vs = np.stack([end.vx, end.vy, end.vz], 1).astype(np.float64) + aN.astype(np.float64) * (args.dt / 2)
+        KN = 0.5 * np.sum(m * (vs ** 2).sum(1))
+        E0, EN = K0 + W0, KN + WN
+        sub = np.random.default_rng(0).choice(args.n, min(16384, args.n), replace=False)
+        d = ctx.direct_f64(b, b.x[sub], b.y[sub], b.z[sub], 1.0, args.eps2)
+        rel = np.linalg.norm(a0[sub] - d, axis=1) / np.linalg.norm(d, axis=1)
+        out["accuracy"] = {
+            "force_err_vs_fp64_median": float(np.median(rel)),
+            "force_err_vs_fp64_p99": float(np.quantile(rel, 0.99)),
+            "force_err_vs_fp64_max": float(rel.max()),
+            "sample": int(len(sub)),
+            "energy_drift": float(abs(EN - E0) / abs(E0)),
+            "E0": float(E0), "E_end": float(EN),
+        }
+        s0.close()
+        s2.close()
+
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        log(f"[bench] CPU baseline with {threads} threads")
+        cb = cpu_baseline(b, args, threads)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = value / cb["value"]
+    s.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,185 @@
+/*
+ * nbg.h — C ABI of the MI355X tree-code force pass (HIP, gfx950).
+ *
+ * This library is a drop-in for the reference simulator's CPU tree solvers
+ * (JacksonCampolattaro/n-body):  BarnesHutSolver (octree) and LinearBVHSolver,
+ * i.e. ActiveTreeSolver::updateAccelerations() plus the Solver::step()
+ * integrator around it.  Every entry point names the reference interface it
+ * replaces (paths relative to the reference's src/NBody/).
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative nbg_status on error;
+ *     no C++ exception crosses the ABI.  nbg_last_error() returns the message
+ *     of the last failure on the calling thread.
+ *   - Plain pointers and sizes only.  Host arrays are SoA float32, caller
+ *     owned, and copied; device state is owned by the nbg_sys.
+ *   - One HIP stream per system.  Calls on one nbg_sys must be serialised by
+ *     the caller; distinct systems may be driven from different threads.
+ *     Every call makes the system's device current itself (the reference
+ *     drives Solver::step from a background std::thread, Solver.h:155-186).
+ */
+#ifndef NBG_H
+#define NBG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum nbg_status {
+    NBG_OK = 0,
+    NBG_ERR_ARG = -1,       /* bad argument (null pointer, size, parameter range) */
+    NBG_ERR_HIP = -2,       /* HIP runtime error (message in nbg_last_error) */
+    NBG_ERR_NO_DEVICE = -3, /* no GPU / no HIP runtime */
+    NBG_ERR_DEVICE = -4,    /* a kernel reported a failure (e.g. tree deeper than supported) */
+    NBG_ERR_BUDGET = -5,    /* memory budget exhausted (batched / serving forms) */
+    NBG_ERR_STATE = -6      /* call out of order (e.g. counts before a force pass) */
+};
+
+/* Tree solver kind.  BARNES_HUT = BarnesHutSolver (Solvers/BarnesHutSolver.h:14-32,
+ * id "barnes-hut"); LINEAR_BVH = LinearBVHSolver (Solvers/LinearBVHSolver.h:18-28,
+ * id "linear-bvh"). */
+enum nbg_solver_kind { NBG_BARNES_HUT = 0, NBG_LINEAR_BVH = 1 };
+
+/* Per-body sum order.  ORDERED reproduces the reference's arithmetic exactly:
+ * IEEE sqrt/div, no FMA, and the std::transform_reduce bracketing of child
+ * results (Descent/activeTree.h:44-52,66-76).  FAST keeps the reference's
+ * interaction set and order (same opening decisions) but sums each body's
+ * interactions into one accumulator with rsqrt and FMA. */
+enum nbg_sum_mode { NBG_SUM_ORDERED = 0, NBG_SUM_FAST = 1 };
+
+typedef struct nbg_params {
+    int32_t solver;      /* nbg_solver_kind */
+    float theta;         /* opening angle; ActiveTreeSolver default 0.3 (ActiveTreeSolver.h:21) */
+    int32_t leaf_size;   /* max leaf size; Octree/LinearBVH default 64 (Octree.h:168, LinearBVH.h:135) */
+    int32_t max_depth;   /* octree refine depth below the 2 forced levels; default 32 (Octree.h:167) */
+    int32_t morton_bits; /* LBVH key width, 30 or 63; default 63 (MortonSort.h:25) */
+    float G;             /* Gravity G, default 1 (Physics/Rules/Gravity.h:195) */
+    float eps2;          /* Gravity "epsilon", added to r^2; default 1e-5 (Gravity.h:195,207) */
+    float dt;            /* Solver::timeStep(), default 1e-3 (Simulation/Solver.h:34,101) */
+    int32_t sum_mode;    /* nbg_sum_mode */
+} nbg_params;
+
+typedef struct nbg_ctx nbg_ctx;
+typedef struct nbg_sys nbg_sys;
+
+/* Fills p with the reference's defaults (see the field comments). */
+void nbg_params_default(nbg_params *p);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *nbg_last_error(void);
+
+/* Library version string. */
+const char *nbg_version(void);
+
+/* Number of visible HIP devices (0 if none or no runtime). */
+int nbg_device_count(void);
+
+/* A context binds one GPU.  Replaces nothing in the reference (which is
+ * CPU-only); it owns the device selection that Solver::maxThreadCount()
+ * (Solver.h:97-99) expresses for TBB. */
+int nbg_ctx_create(int device, nbg_ctx **out);
+int nbg_ctx_destroy(nbg_ctx *ctx);
+
+/* A system is one Simulation's bodies plus one solver's state
+ * (ActiveTreeSolver(Simulation&, Rule&), Solvers/ActiveTreeSolver.h:25-27).
+ * Capacity n is fixed at creation; device buffers are allocated here. */
+int nbg_sys_create(nbg_ctx *ctx, const nbg_params *p, int64_t n, nbg_sys **out);
+int nbg_sys_destroy(nbg_sys *sys);
+
+/* theta() / maxLeafSize() / maxDepth() / Gravity / timeStep() setters
+ * (ActiveTreeSolver.h:78-80, BarnesHutSolver.h:25-31, Gravity.h:195,
+ * Solver.h:101).  The solver kind and key width may change too. */
+int nbg_sys_set_params(nbg_sys *sys, const nbg_params *p);
+int nbg_sys_get_params(const nbg_sys *sys, nbg_params *p);
+
+/* Copies host SoA state in (the Simulation's Position, Mass and Velocity
+ * components; Physics/Position.h:17-29, Mass.h:10-28, Velocity.h:16-24).
+ * vx/vy/vz may be NULL (velocities set to 0). */
+int nbg_sys_set_state(nbg_sys *sys, const float *x, const float *y, const float *z, const float *m,
+                      const float *vx, const float *vy, const float *vz);
+
+/* Copies device state out; any pointer may be NULL. */
+int nbg_sys_get_state(nbg_sys *sys, float *x, float *y, float *z, float *vx, float *vy, float *vz);
+
+/* ActiveTreeSolver::updateAccelerations() (Solvers/ActiveTreeSolver.h:33-72):
+ * builds the tree from the current positions and writes every body's
+ * acceleration.  ax/ay/az (host) may all be NULL to keep them on the device. */
+int nbg_sys_update_accelerations(nbg_sys *sys, float *ax, float *ay, float *az);
+
+/* Copies the accelerations of the last force pass out (Acceleration component,
+ * Physics/Acceleration.h:16-42). */
+int nbg_sys_get_accelerations(nbg_sys *sys, float *ax, float *ay, float *az);
+
+/* Solver::step() repeated nsteps times (Simulation/Solver.h:103-139):
+ * updateAccelerations, then v = v + a*dt, then p = p + v*dt, on the device,
+ * with no host copies.  Asynchronous: returns after enqueueing. */
+int nbg_sys_step(nbg_sys *sys, int32_t nsteps);
+
+/* Waits for the system's stream and reports any device-side failure. */
+int nbg_sys_synchronize(nbg_sys *sys);
+
+/* Interaction counts of the last force pass as SimpleTrackingRule counts
+ * them (Physics/Rules/SimpleTrackingRule.h:29-45): pp = particle-particle
+ * calls (self pairs included), np = node-particle calls.  64-bit. */
+int nbg_sys_counts(nbg_sys *sys, uint64_t *pp, uint64_t *np);
+
+/* Per-phase device time (ms) of the last force pass, measured with HIP events
+ * on the system's stream when timing is enabled.  Phases, in order:
+ * 0 bbox, 1 keys, 2 sort, 3 gather, 4 topology, 5 summaries, 6 walk,
+ * 7 update.  Returns the number of phases written (<= cap). */
+int nbg_sys_enable_timing(nbg_sys *sys, int32_t on);
+int nbg_sys_phase_times(nbg_sys *sys, float *ms, int32_t cap);
+
+/* ---- parity hooks: read back intermediate results of the last force pass ---- */
+
+/* outerBoundingBox (Physics/Summaries/SummaryType.h:73-81): min xyz, max xyz. */
+int nbg_sys_debug_bbox(nbg_sys *sys, float *out6);
+
+/* LBVH: sorted Morton keys (setMortonCodes, Solvers/MortonSort.h:63-73; keys
+ * widened to 64 bit) and the sort permutation (recursiveParallelMortonSort,
+ * MortonSort.h:141-183), ties by original index.  Octree: keys_or_null is
+ * ignored and perm is the body order of the leaves. */
+int nbg_sys_debug_order(nbg_sys *sys, uint64_t *keys_or_null, uint32_t *perm);
+
+/* Tree topology and summaries.  Call with arrays NULL to get the node count.
+ * Per node: first body (sorted position), body count, leaf flag, parent
+ * (-1 for the root), centre of mass + total mass (CenterOfMassSummary.h:14-63)
+ * and geometry: LBVH bbox min/max (BoundingBoxSummary.h:14-74); octree
+ * centre xyz, side, 0, 0 (Octree.h:73-74). */
+int nbg_sys_debug_tree(nbg_sys *sys, int64_t *num_nodes, int64_t *first, int64_t *count, int32_t *leaf,
+                       int32_t *parent, float *com4, float *geo6);
+
+/* ---- direct summation (NaiveSolver, Solvers/NaiveSolver.h:22-52) ---- */
+
+/* FP64 direct sum on the device: accelerations at nt target points from ns
+ * sources, the reference's softening (eps2 added to r^2) and self-skip by
+ * exact position equality.  Host arrays in/out. */
+int nbg_direct_f64(nbg_ctx *ctx, int64_t ns, const float *sx, const float *sy, const float *sz,
+                   const float *sm, int64_t nt, const float *tx, const float *ty, const float *tz,
+                   double G, double eps2, double *ax, double *ay, double *az);
+
+/* FP64 softened potential energy -sum_{i<j} G m_i m_j / sqrt(r^2 + eps2)
+ * of a system's current device state. */
+int nbg_sys_potential_f64(nbg_sys *sys, double *W);
+
+/* ---- synthetic inputs (host, CPU) ---- */
+
+/* Plummer sphere: scale radius a, total mass M, gravitational constant G,
+ * radius truncated at rmax*a; positions by inverse CDF, velocities by the
+ * Aarseth-Henon-Wielen rejection; std::mt19937_64(seed) with 53-bit
+ * doubles; generated in FP64, rounded to FP32.  m_i = M / n. */
+int nbg_generate_plummer(uint64_t seed, int64_t n, double a, double M, double G, double rmax, float *x,
+                         float *y, float *z, float *m, float *vx, float *vy, float *vz);
+
+/* Uniform cube of side `side` centred at the origin, zero velocity, total
+ * mass M. */
+int nbg_generate_cube(uint64_t seed, int64_t n, double side, double M, float *x, float *y, float *z,
+                      float *m, float *vx, float *vy, float *vz);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBG_H */

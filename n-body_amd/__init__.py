@@ -1,0 +1,467 @@
+"""Python mirror of the reference's Solver API over the nbg C ABI (libnbg.so).
+
+The reference (JacksonCampolattaro/n-body) is C++: a ``Simulation`` holds the
+bodies, a ``Gravity`` rule holds G and epsilon, and a tree solver such as
+``LinearBVHSolver(simulation, rule)`` exposes ``step()``,
+``updateAccelerations()``, ``theta()``, ``maxLeafSize()`` and the static
+``timeStep()`` (src/NBody/Simulation/Solver.h:26-187,
+src/NBody/Simulation/Solvers/ActiveTreeSolver.h:16-82).  This module keeps
+those names so tests and tools read like the reference's own code; every
+call goes to the HIP kernels through ``include/nbg.h``.  There is no CPU
+fallback: if libnbg.so is missing or no GPU is present, calls raise.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnbg.so")
+
+BARNES_HUT = 0
+LINEAR_BVH = 1
+SUM_ORDERED = 0
+SUM_FAST = 1
+
+PHASES = ("bbox", "keys", "sort", "gather", "topology", "summaries", "walk", "update")
+
+# Every symbol include/nbg.h declares.
+EXPORTS = (
+    "nbg_params_default", "nbg_last_error", "nbg_version", "nbg_device_count",
+    "nbg_ctx_create", "nbg_ctx_destroy", "nbg_sys_create", "nbg_sys_destroy",
+    "nbg_sys_set_params", "nbg_sys_get_params", "nbg_sys_set_state", "nbg_sys_get_state",
+    "nbg_sys_update_accelerations", "nbg_sys_get_accelerations", "nbg_sys_step",
+    "nbg_sys_synchronize", "nbg_sys_counts", "nbg_sys_enable_timing", "nbg_sys_phase_times",
+    "nbg_sys_debug_bbox", "nbg_sys_debug_order", "nbg_sys_debug_tree", "nbg_direct_f64",
+    "nbg_sys_potential_f64", "nbg_generate_plummer", "nbg_generate_cube",
+)
+
+
+class NbgError(RuntimeError):
+    """A negative status from the C ABI, with nbg_last_error()'s message."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"nbg error {code}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    """nbg_params (include/nbg.h)."""
+
+    _fields_ = [
+        ("solver", ctypes.c_int32),
+        ("theta", ctypes.c_float),
+        ("leaf_size", ctypes.c_int32),
+        ("max_depth", ctypes.c_int32),
+        ("morton_bits", ctypes.c_int32),
+        ("G", ctypes.c_float),
+        ("eps2", ctypes.c_float),
+        ("dt", ctypes.c_float),
+        ("sum_mode", ctypes.c_int32),
+    ]
+
+
+_lib = None
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+
+
+def lib() -> ctypes.CDLL:
+    """Loads the in-tree libnbg.so (built by __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        sig = {
+            "nbg_params_default": (None, [ctypes.POINTER(Params)]),
+            "nbg_last_error": (ctypes.c_char_p, []),
+            "nbg_version": (ctypes.c_char_p, []),
+            "nbg_device_count": (ctypes.c_int, []),
+            "nbg_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+            "nbg_ctx_destroy": (ctypes.c_int, [_vp]),
+            "nbg_sys_create": (ctypes.c_int, [_vp, ctypes.POINTER(Params), ctypes.c_int64, ctypes.POINTER(_vp)]),
+            "nbg_sys_destroy": (ctypes.c_int, [_vp]),
+            "nbg_sys_set_params": (ctypes.c_int, [_vp, ctypes.POINTER(Params)]),
+            "nbg_sys_get_params": (ctypes.c_int, [_vp, ctypes.POINTER(Params)]),
+            "nbg_sys_set_state": (ctypes.c_int, [_vp] + [_f32p] * 7),
+            "nbg_sys_get_state": (ctypes.c_int, [_vp] + [_f32p] * 6),
+            "nbg_sys_update_accelerations": (ctypes.c_int, [_vp] + [_f32p] * 3),
+            "nbg_sys_get_accelerations": (ctypes.c_int, [_vp] + [_f32p] * 3),
+            "nbg_sys_step": (ctypes.c_int, [_vp, ctypes.c_int32]),
+            "nbg_sys_synchronize": (ctypes.c_int, [_vp]),
+            "nbg_sys_counts": (ctypes.c_int, [_vp, _u64p, _u64p]),
+            "nbg_sys_enable_timing": (ctypes.c_int, [_vp, ctypes.c_int32]),
+            "nbg_sys_phase_times": (ctypes.c_int, [_vp, _f32p, ctypes.c_int32]),
+            "nbg_sys_debug_bbox": (ctypes.c_int, [_vp, _f32p]),
+            "nbg_sys_debug_order": (ctypes.c_int, [_vp, _u64p, _u32p]),
+            "nbg_sys_debug_tree": (ctypes.c_int, [_vp, _i64p, _i64p, _i64p, _i32p, _i32p, _f32p, _f32p]),
+            "nbg_direct_f64": (ctypes.c_int, [_vp, ctypes.c_int64] + [_f32p] * 4 + [ctypes.c_int64] + [_f32p] * 3
+                               + [ctypes.c_double, ctypes.c_double] + [_f64p] * 3),
+            "nbg_sys_potential_f64": (ctypes.c_int, [_vp, _f64p]),
+            "nbg_generate_plummer": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+                                                    ctypes.c_double, ctypes.c_double] + [_f32p] * 7),
+            "nbg_generate_cube": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_double, ctypes.c_double]
+                                  + [_f32p] * 7),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> int:
+    if rc < 0:
+        raise NbgError(rc, lib().nbg_last_error().decode())
+    return rc
+
+
+def _ptr(a, t=_f32p):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def default_params() -> Params:
+    p = Params()
+    lib().nbg_params_default(ctypes.byref(p))
+    return p
+
+
+def device_count() -> int:
+    return lib().nbg_device_count()
+
+
+# ------------------------------------------------------------------ generators
+
+@dataclass
+class Bodies:
+    """SoA FP32 body state: the Simulation's Position / Velocity / Mass."""
+
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    m: np.ndarray
+    vx: np.ndarray
+    vy: np.ndarray
+    vz: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return int(self.x.shape[0])
+
+    def copy(self) -> "Bodies":
+        return Bodies(*(a.copy() for a in (self.x, self.y, self.z, self.m, self.vx, self.vy, self.vz)))
+
+    @staticmethod
+    def empty(n: int) -> "Bodies":
+        return Bodies(*(np.zeros(n, np.float32) for _ in range(7)))
+
+
+def plummer(n: int, seed: int, a: float = 1.0, M: float = 1.0, G: float = 1.0, rmax: float = 10.0) -> Bodies:
+    """Plummer sphere (nbg_generate_plummer)."""
+    b = Bodies.empty(n)
+    _check(lib().nbg_generate_plummer(seed, n, a, M, G, rmax, *[_ptr(v) for v in
+                                                                  (b.x, b.y, b.z, b.m, b.vx, b.vy, b.vz)]))
+    return b
+
+
+def cube(n: int, seed: int, side: float = 2.0, M: float = 1.0) -> Bodies:
+    """Uniform cube at rest (nbg_generate_cube)."""
+    b = Bodies.empty(n)
+    _check(lib().nbg_generate_cube(seed, n, side, M, *[_ptr(v) for v in (b.x, b.y, b.z, b.m, b.vx, b.vy, b.vz)]))
+    return b
+
+
+# ------------------------------------------------------------------ C ABI objects
+
+class Context:
+    """nbg_ctx: one GPU."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        _check(lib().nbg_ctx_create(device, ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().nbg_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def direct_f64(self, src: Bodies, tx, ty, tz, G: float, eps2: float):
+        """FP64 direct-sum accelerations at the target points (nbg_direct_f64)."""
+        tx, ty, tz = _f32(tx), _f32(ty), _f32(tz)
+        nt = tx.shape[0]
+        out = [np.zeros(nt, np.float64) for _ in range(3)]
+        _check(lib().nbg_direct_f64(self.handle, src.n, _ptr(src.x), _ptr(src.y), _ptr(src.z), _ptr(src.m), nt,
+                                    _ptr(tx), _ptr(ty), _ptr(tz), G, eps2, *[_ptr(o, _f64p) for o in out]))
+        return np.stack(out, axis=1)
+
+
+class System:
+    """nbg_sys: one set of bodies plus one tree solver's device state."""
+
+    def __init__(self, ctx: Context, params: Params, n: int):
+        h = _vp()
+        self._ctx = ctx
+        _check(lib().nbg_sys_create(ctx.handle, ctypes.byref(params), n, ctypes.byref(h)))
+        self.handle = h
+        self.n = n
+
+    def close(self):
+        if self.handle:
+            lib().nbg_sys_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self) -> Params:
+        p = Params()
+        _check(lib().nbg_sys_get_params(self.handle, ctypes.byref(p)))
+        return p
+
+    @params.setter
+    def params(self, p: Params):
+        _check(lib().nbg_sys_set_params(self.handle, ctypes.byref(p)))
+
+    def set_state(self, b: Bodies):
+        arrs = [_f32(v) for v in (b.x, b.y, b.z, b.m, b.vx, b.vy, b.vz)]
+        _check(lib().nbg_sys_set_state(self.handle, *[_ptr(a) for a in arrs]))
+
+    def get_state(self) -> Bodies:
+        out = Bodies.empty(self.n)
+        _check(lib().nbg_sys_get_state(self.handle, *[_ptr(v) for v in (out.x, out.y, out.z, out.vx, out.vy, out.vz)]))
+        return out
+
+    def update_accelerations(self) -> np.ndarray:
+        a = [np.zeros(self.n, np.float32) for _ in range(3)]
+        _check(lib().nbg_sys_update_accelerations(self.handle, *[_ptr(v) for v in a]))
+        return np.stack(a, axis=1)
+
+    def accelerations(self) -> np.ndarray:
+        a = [np.zeros(self.n, np.float32) for _ in range(3)]
+        _check(lib().nbg_sys_get_accelerations(self.handle, *[_ptr(v) for v in a]))
+        return np.stack(a, axis=1)
+
+    def step(self, nsteps: int = 1):
+        _check(lib().nbg_sys_step(self.handle, nsteps))
+
+    def synchronize(self):
+        _check(lib().nbg_sys_synchronize(self.handle))
+
+    def counts(self):
+        pp, np_ = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().nbg_sys_counts(self.handle, ctypes.byref(pp), ctypes.byref(np_)))
+        return int(pp.value), int(np_.value)
+
+    def enable_timing(self, on: bool = True):
+        _check(lib().nbg_sys_enable_timing(self.handle, int(on)))
+
+    def phase_times(self) -> dict:
+        ms = np.zeros(len(PHASES), np.float32)
+        k = _check(lib().nbg_sys_phase_times(self.handle, _ptr(ms), len(PHASES)))
+        return {PHASES[i]: float(ms[i]) for i in range(k)}
+
+    def debug_bbox(self) -> np.ndarray:
+        out = np.zeros(6, np.float32)
+        _check(lib().nbg_sys_debug_bbox(self.handle, _ptr(out)))
+        return out
+
+    def debug_order(self):
+        keys = np.zeros(self.n, np.uint64)
+        perm = np.zeros(self.n, np.uint32)
+        _check(lib().nbg_sys_debug_order(self.handle, _ptr(keys, _u64p), _ptr(perm, _u32p)))
+        return keys, perm
+
+    def debug_tree(self) -> dict:
+        nn = ctypes.c_int64()
+        _check(lib().nbg_sys_debug_tree(self.handle, ctypes.byref(nn), None, None, None, None, None, None))
+        k = nn.value
+        first = np.zeros(k, np.int64)
+        count = np.zeros(k, np.int64)
+        leaf = np.zeros(k, np.int32)
+        parent = np.zeros(k, np.int32)
+        com4 = np.zeros((k, 4), np.float32)
+        geo6 = np.zeros((k, 6), np.float32)
+        _check(lib().nbg_sys_debug_tree(self.handle, ctypes.byref(nn), _ptr(first, _i64p), _ptr(count, _i64p),
+                                        _ptr(leaf, _i32p), _ptr(parent, _i32p), _ptr(com4), _ptr(geo6)))
+        return dict(first=first, count=count, leaf=leaf.astype(bool), parent=parent, com4=com4, geo6=geo6)
+
+    def potential_f64(self) -> float:
+        w = ctypes.c_double()
+        _check(lib().nbg_sys_potential_f64(self.handle, ctypes.byref(w)))
+        return float(w.value)
+
+
+# ------------------------------------------------------------------ reference-shaped facade
+
+class Gravity:
+    """Gravity(G, epsilon) rule (src/NBody/Physics/Rules/Gravity.h:185-195).
+
+    As in the reference, ``epsilon`` is added to r^2 (it is a squared
+    softening length)."""
+
+    def __init__(self, G: float = 1.0, epsilon: float = 0.00001):
+        self.G = float(G)
+        self.epsilon = float(epsilon)
+
+    def g(self) -> float:
+        return self.G
+
+
+class Simulation:
+    """The bodies a solver reads and writes (src/NBody/Simulation/Simulation.h:45-227),
+    held as SoA FP32 arrays."""
+
+    def __init__(self, bodies: Bodies):
+        self.bodies = bodies
+        self.acceleration = np.zeros((bodies.n, 3), np.float32)
+
+    def particleCount(self) -> int:
+        return self.bodies.n
+
+
+class _TreeSolver:
+    _kind = LINEAR_BVH
+    _dt = 0.001  # Solver::_dt is static and shared by all solvers (Solver.h:34)
+
+    def __init__(self, simulation: Simulation, rule: Gravity, context: Context | None = None):
+        self._simulation = simulation
+        self._rule = rule
+        self._ctx = context or Context(0)
+        self._params = default_params()
+        self._params.solver = self._kind
+        self._sys: System | None = None
+        self._dirty = True
+
+    # -- reference accessors
+    @staticmethod
+    def timeStep() -> float:
+        return _TreeSolver._dt
+
+    @staticmethod
+    def setTimeStep(dt: float):
+        _TreeSolver._dt = float(dt)
+
+    def theta(self) -> float:
+        return self._params.theta
+
+    def setTheta(self, theta: float):
+        self._params.theta = theta
+
+    def maxLeafSize(self) -> int:
+        return self._params.leaf_size
+
+    def setMaxLeafSize(self, n: int):
+        self._params.leaf_size = n
+
+    def setSumMode(self, mode: int):
+        self._params.sum_mode = mode
+
+    def setMortonBits(self, bits: int):
+        self._params.morton_bits = bits
+
+    def simulation(self) -> Simulation:
+        return self._simulation
+
+    def rule(self) -> Gravity:
+        return self._rule
+
+    # -- device plumbing
+    def _system(self) -> System:
+        p = self._params
+        p.G = self._rule.G
+        p.eps2 = self._rule.epsilon
+        p.dt = _TreeSolver._dt
+        n = self._simulation.bodies.n
+        if self._sys is None or self._sys.n != n:
+            self._sys = System(self._ctx, p, n)
+            self._dirty = True
+        else:
+            self._sys.params = p
+        if self._dirty:
+            self._sys.set_state(self._simulation.bodies)
+            self._dirty = False
+        return self._sys
+
+    def updateAccelerations(self):
+        """ActiveTreeSolver::updateAccelerations (ActiveTreeSolver.h:33-72)."""
+        self._simulation.acceleration = self._system().update_accelerations()
+
+    def step(self):
+        """Solver::step (Solver.h:103-113): accelerations, velocities, positions."""
+        s = self._system()
+        s.step(1)
+        st = s.get_state()
+        b = self._simulation.bodies
+        b.x, b.y, b.z, b.vx, b.vy, b.vz = st.x, st.y, st.z, st.vx, st.vy, st.vz
+        self._simulation.acceleration = s.accelerations()
+
+    def counts(self):
+        return self._system().counts() if self._sys is not None else (0, 0)
+
+
+class LinearBVHSolver(_TreeSolver):
+    """LinearBVHSolver (src/NBody/Simulation/Solvers/LinearBVHSolver.h:18-28)."""
+
+    _kind = LINEAR_BVH
+
+    def id(self) -> str:
+        return "gpu-linear-bvh"
+
+    def name(self) -> str:
+        return "Linear-BVH (MI355X)"
+
+
+class BarnesHutSolver(_TreeSolver):
+    """BarnesHutSolver (src/NBody/Simulation/Solvers/BarnesHutSolver.h:14-32)."""
+
+    _kind = BARNES_HUT
+
+    def id(self) -> str:
+        return "gpu-barnes-hut"
+
+    def name(self) -> str:
+        return "Barnes-Hut (MI355X)"
+
+    def maxDepth(self) -> int:
+        return self._params.max_depth
+
+    def setMaxDepth(self, d: int):
+        self._params.max_depth = d
+
+
+SOLVERS = {"gpu-linear-bvh": LinearBVHSolver, "gpu-barnes-hut": BarnesHutSolver}
+
+
+def select(solver_id: str):
+    """MultiSolver::select(id) (src/NBody/Simulation/MultiSolver.h:79-87)."""
+    try:
+        return SOLVERS[solver_id]
+    except KeyError:
+        raise ValueError(f"unknown solver id {solver_id!r}") from None

@@ -1,0 +1,576 @@
+// C ABI of the MI355X tree-code force pass.  Host code only: it owns the
+// device buffers of each system, enqueues the build / walk / update kernels
+// on the system's stream and reports errors as status codes.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/nbg.h"
+#include "launch.hpp"
+
+using namespace nbg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define NBG_HIP(call)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(NBG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+constexpr int kPhases = 8;
+constexpr int kEventSets = 256;
+
+}  // namespace
+
+struct nbg_ctx {
+    int device = 0;
+};
+
+struct nbg_sys {
+    nbg_ctx *ctx = nullptr;
+    nbg_params prm{};
+    int64_t cap = 0;
+    int64_t n = 0;
+    hipStream_t stream = nullptr;
+    bool have_state = false;
+    bool have_forces = false;
+    bool timing = false;
+    // Timing: one set of kPhases+1 events per force pass, recorded on the
+    // system stream; folded into per-phase sums when the stream is synchronised.
+    std::vector<hipEvent_t> ev;    // kEventSets * (kPhases + 1)
+    int ev_used = 0;               // sets recorded since the last fold
+    double phase_sum[kPhases] = {};
+    int64_t phase_n = 0;           // passes folded into phase_sum
+
+    // state, original order
+    float *x = nullptr, *y = nullptr, *z = nullptr, *m = nullptr;
+    float *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    float *ax = nullptr, *ay = nullptr, *az = nullptr;
+    // build work
+    void *keys = nullptr, *keys_alt = nullptr;
+    uint32_t *idx = nullptr, *idx_alt = nullptr;
+    float4 *bodies = nullptr;
+    int32_t *kfirst = nullptr, *klast = nullptr, *kgamma = nullptr, *kparent = nullptr;
+    uint32_t *ksplit = nullptr, *krank = nullptr;
+    uint32_t *scan_ws = nullptr;
+    uint32_t *sort_status = nullptr;
+    // tree (capacity 2n nodes)
+    int4 *topo = nullptr;
+    float4 *com = nullptr, *bmin = nullptr, *bmax = nullptr, *mac = nullptr;
+    uint32_t *flags = nullptr;
+    // small
+    float *bbox_partials = nullptr, *bbox = nullptr;
+    uint32_t *misc = nullptr;  // [0] bbox counter, [1] err, [2] node count, [4..8) counts (2 x u64)
+
+    uint32_t *err_word() { return misc + 1; }
+    uint32_t *node_count() { return misc + 2; }
+    unsigned long long *counts() { return (unsigned long long *)(misc + 4); }
+};
+
+namespace {
+
+int check_params(const nbg_params *p) {
+    if (!p) return fail(NBG_ERR_ARG, "params is NULL");
+    if (p->solver != NBG_BARNES_HUT && p->solver != NBG_LINEAR_BVH) return fail(NBG_ERR_ARG, "unknown solver kind");
+    if (p->solver == NBG_BARNES_HUT)
+        return fail(NBG_ERR_ARG, "the octree Barnes-Hut solver is not available in this build yet");
+    if (!(p->theta >= 0.0f)) return fail(NBG_ERR_ARG, "theta must be >= 0");
+    if (p->leaf_size < 1) return fail(NBG_ERR_ARG, "leaf_size must be >= 1");
+    if (p->max_depth < 0 || p->max_depth > 32) return fail(NBG_ERR_ARG, "max_depth must be in [0, 32]");
+    if (p->morton_bits != 30 && p->morton_bits != 63) return fail(NBG_ERR_ARG, "morton_bits must be 30 or 63");
+    if (!(p->eps2 >= 0.0f)) return fail(NBG_ERR_ARG, "eps2 must be >= 0");
+    if (p->sum_mode != NBG_SUM_ORDERED && p->sum_mode != NBG_SUM_FAST) return fail(NBG_ERR_ARG, "unknown sum mode");
+    return NBG_OK;
+}
+
+template <typename T>
+int dalloc(T **p, int64_t count) {
+    if (count <= 0) count = 1;
+    NBG_HIP(hipMalloc((void **)p, (size_t)count * sizeof(T)));
+    return NBG_OK;
+}
+
+void free_sys(nbg_sys *s) {
+    void *ptrs[] = {s->x, s->y, s->z, s->m, s->vx, s->vy, s->vz, s->ax, s->ay, s->az, s->keys, s->keys_alt,
+                    s->idx, s->idx_alt, s->bodies, s->kfirst, s->klast, s->kgamma, s->kparent, s->ksplit,
+                    s->krank, s->scan_ws, s->sort_status, s->topo, s->com, s->bmin, s->bmax, s->mac, s->flags,
+                    s->bbox_partials, s->bbox, s->misc};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto &e : s->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+// Records event `phase` of the current pass's set (no-op when timing is off
+// or every set is in use until the next synchronisation).
+void mark(nbg_sys *s, int phase) {
+    if (!s->timing || s->ev_used >= kEventSets) return;
+    (void)hipEventRecord(s->ev[(size_t)s->ev_used * (kPhases + 1) + phase], s->stream);
+    if (phase == kPhases) ++s->ev_used;
+}
+
+// Folds the recorded sets into the per-phase sums (stream must be idle).
+void fold_timing(nbg_sys *s) {
+    for (int k = 0; k < s->ev_used; ++k) {
+        hipEvent_t *e = &s->ev[(size_t)k * (kPhases + 1)];
+        for (int i = 0; i < kPhases; ++i) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, e[i], e[i + 1]) == hipSuccess) s->phase_sum[i] += ms;
+        }
+    }
+    s->phase_n += s->ev_used;
+    s->ev_used = 0;
+}
+
+// ActiveTreeSolver::updateAccelerations (Solvers/ActiveTreeSolver.h:33-72)
+// for the linear BVH: build the tree (LinearBVH::build, Trees/LinearBVH.h:106-133),
+// then walk it for every body.  Enqueue only.
+int enqueue_force_pass(nbg_sys *s) {
+    const int64_t n = s->n;
+    const nbg_params &p = s->prm;
+    hipStream_t st = s->stream;
+    launch_fill_u32(s->misc + 4, 0u, 4, st);  // interaction counters
+    mark(s, 0);
+    launch_bbox(s->x, s->y, s->z, n, s->bbox_partials, s->misc, s->bbox, st);
+    mark(s, 1);
+    launch_morton(s->x, s->y, s->z, n, s->bbox, p.morton_bits, s->keys, s->idx, st);
+    mark(s, 2);
+    launch_radix_sort(s->keys, s->idx, s->keys_alt, s->idx_alt, n, p.morton_bits, s->sort_status, s->err_word(), st);
+    mark(s, 3);
+    launch_gather(s->x, s->y, s->z, s->m, s->idx, n, s->bodies, st);
+    mark(s, 4);
+    launch_lbvh_topology(s->keys, p.morton_bits, n, p.leaf_size, s->kfirst, s->klast, s->kgamma, s->ksplit, s->krank,
+                         s->kparent, s->scan_ws, s->topo, s->node_count(), st);
+    mark(s, 5);
+    launch_lbvh_summaries(s->bodies, n, s->topo, s->node_count(), 2 * n, s->flags, s->com, s->bmin, s->bmax, s->mac,
+                          s->err_word(), st);
+    mark(s, 6);
+    WalkArgs wa;
+    wa.n = n;
+    wa.bodies = s->bodies;
+    wa.perm = s->idx;
+    wa.mac = s->mac;
+    wa.com = s->com;
+    wa.topo = s->topo;
+    wa.theta = p.theta;
+    wa.theta2 = p.theta * p.theta;
+    wa.G = p.G;
+    wa.eps2 = p.eps2;
+    wa.max_nodes = 2 * n;
+    wa.ax = s->ax;
+    wa.ay = s->ay;
+    wa.az = s->az;
+    wa.counts = s->counts();
+    wa.err = s->err_word();
+    launch_walk_lbvh(wa, p.sum_mode == NBG_SUM_ORDERED, p.morton_bits == 30 ? 32 : 64, st);
+    mark(s, 7);
+    NBG_HIP(hipGetLastError());
+    return NBG_OK;
+}
+
+int sync_and_check(nbg_sys *s) {
+    NBG_HIP(hipStreamSynchronize(s->stream));
+    uint32_t err = 0;
+    NBG_HIP(hipMemcpy(&err, s->err_word(), sizeof err, hipMemcpyDeviceToHost));
+    if (err) {
+        uint32_t zero = 0;
+        (void)hipMemcpy(s->err_word(), &zero, sizeof zero, hipMemcpyHostToDevice);
+        if (err & 1u) return fail(NBG_ERR_DEVICE, "radix sort look-back exceeded its spin limit");
+        if (err & 2u) return fail(NBG_ERR_DEVICE, "tree traversal exceeded its stack, depth or index limits");
+        if (err & 4u) return fail(NBG_ERR_DEVICE, "tree summaries found an invalid node index");
+        return fail(NBG_ERR_DEVICE, "device error flag set");
+    }
+    fold_timing(s);
+    return NBG_OK;
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+void nbg_params_default(nbg_params *p) {
+    if (!p) return;
+    p->solver = NBG_LINEAR_BVH;
+    p->theta = 0.3f;
+    p->leaf_size = 64;
+    p->max_depth = 32;
+    p->morton_bits = 63;
+    p->G = 1.0f;
+    p->eps2 = 0.00001f;
+    p->dt = 0.001f;
+    p->sum_mode = NBG_SUM_ORDERED;
+}
+
+const char *nbg_last_error(void) { return g_err.c_str(); }
+
+const char *nbg_version(void) { return "nbg 0.1 (gfx950)"; }
+
+int nbg_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+int nbg_ctx_create(int device, nbg_ctx **out) {
+    if (!out) return fail(NBG_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess || c <= 0) return fail(NBG_ERR_NO_DEVICE, "no HIP device available");
+    if (device < 0 || device >= c) return fail(NBG_ERR_ARG, "device index out of range");
+    NBG_HIP(hipSetDevice(device));
+    nbg_ctx *ctx = new nbg_ctx;
+    ctx->device = device;
+    *out = ctx;
+    return NBG_OK;
+}
+
+int nbg_ctx_destroy(nbg_ctx *ctx) {
+    delete ctx;
+    return NBG_OK;
+}
+
+int nbg_sys_create(nbg_ctx *ctx, const nbg_params *p, int64_t n, nbg_sys **out) {
+    if (!ctx || !out) return fail(NBG_ERR_ARG, "ctx or out is NULL");
+    *out = nullptr;
+    if (int r = check_params(p)) return r;
+    if (n < 0 || n > kMaxBodies) return fail(NBG_ERR_ARG, "body count out of range");
+    DeviceGuard g(ctx->device);
+    nbg_sys *s = new nbg_sys;
+    s->ctx = ctx;
+    s->prm = *p;
+    s->cap = n;
+    s->n = n;
+    int r = NBG_OK;
+    auto A = [&](auto **ptr, int64_t count) {
+        if (r == NBG_OK) r = dalloc(ptr, count);
+    };
+    A(&s->x, n); A(&s->y, n); A(&s->z, n); A(&s->m, n);
+    A(&s->vx, n); A(&s->vy, n); A(&s->vz, n);
+    A(&s->ax, n); A(&s->ay, n); A(&s->az, n);
+    A((uint64_t **)&s->keys, n); A((uint64_t **)&s->keys_alt, n);
+    A(&s->idx, n); A(&s->idx_alt, n);
+    A(&s->bodies, n);
+    A(&s->kfirst, n); A(&s->klast, n); A(&s->kgamma, n); A(&s->kparent, n);
+    A(&s->ksplit, n); A(&s->krank, n);
+    A(&s->scan_ws, 64 + (n + kScanTile - 1) / kScanTile + 64);
+    A(&s->sort_status, sort_status_words(n, 64));
+    A(&s->topo, 2 * n); A(&s->com, 2 * n); A(&s->bmin, 2 * n); A(&s->bmax, 2 * n); A(&s->mac, 2 * n);
+    A(&s->flags, 2 * n);
+    A(&s->bbox_partials, 6 * kBBoxBlocks);
+    A(&s->bbox, 8);
+    A(&s->misc, 16);
+    if (r == NBG_OK) {
+        hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) r = fail(NBG_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    if (r == NBG_OK) {
+        hipError_t e = hipMemsetAsync(s->misc, 0, 16 * sizeof(uint32_t), s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) r = fail(NBG_ERR_HIP, std::string("init: ") + hipGetErrorString(e));
+    }
+    if (r != NBG_OK) {
+        free_sys(s);
+        delete s;
+        return r;
+    }
+    *out = s;
+    return NBG_OK;
+}
+
+int nbg_sys_destroy(nbg_sys *s) {
+    if (!s) return NBG_OK;
+    DeviceGuard g(s->ctx->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    free_sys(s);
+    delete s;
+    return NBG_OK;
+}
+
+int nbg_sys_set_params(nbg_sys *s, const nbg_params *p) {
+    if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
+    if (int r = check_params(p)) return r;
+    s->prm = *p;
+    return NBG_OK;
+}
+
+int nbg_sys_get_params(const nbg_sys *s, nbg_params *p) {
+    if (!s || !p) return fail(NBG_ERR_ARG, "sys or params is NULL");
+    *p = s->prm;
+    return NBG_OK;
+}
+
+int nbg_sys_set_state(nbg_sys *s, const float *x, const float *y, const float *z, const float *m, const float *vx,
+                      const float *vy, const float *vz) {
+    if (!s || !x || !y || !z || !m) return fail(NBG_ERR_ARG, "positions and masses are required");
+    DeviceGuard g(s->ctx->device);
+    const size_t b = (size_t)s->n * sizeof(float);
+    if (s->n > 0) {
+        NBG_HIP(hipMemcpyAsync(s->x, x, b, hipMemcpyHostToDevice, s->stream));
+        NBG_HIP(hipMemcpyAsync(s->y, y, b, hipMemcpyHostToDevice, s->stream));
+        NBG_HIP(hipMemcpyAsync(s->z, z, b, hipMemcpyHostToDevice, s->stream));
+        NBG_HIP(hipMemcpyAsync(s->m, m, b, hipMemcpyHostToDevice, s->stream));
+        float *dv[3] = {s->vx, s->vy, s->vz};
+        const float *hv[3] = {vx, vy, vz};
+        for (int d = 0; d < 3; ++d) {
+            if (hv[d]) NBG_HIP(hipMemcpyAsync(dv[d], hv[d], b, hipMemcpyHostToDevice, s->stream));
+            else launch_fill_u32((uint32_t *)dv[d], 0u, s->n, s->stream);
+        }
+        NBG_HIP(hipStreamSynchronize(s->stream));
+    }
+    s->have_state = true;
+    s->have_forces = false;
+    return NBG_OK;
+}
+
+int nbg_sys_get_state(nbg_sys *s, float *x, float *y, float *z, float *vx, float *vy, float *vz) {
+    if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
+    DeviceGuard g(s->ctx->device);
+    const size_t b = (size_t)s->n * sizeof(float);
+    float *h[6] = {x, y, z, vx, vy, vz};
+    float *d[6] = {s->x, s->y, s->z, s->vx, s->vy, s->vz};
+    NBG_HIP(hipStreamSynchronize(s->stream));
+    for (int i = 0; i < 6; ++i)
+        if (h[i] && s->n > 0) NBG_HIP(hipMemcpy(h[i], d[i], b, hipMemcpyDeviceToHost));
+    return NBG_OK;
+}
+
+int nbg_sys_update_accelerations(nbg_sys *s, float *ax, float *ay, float *az) {
+    if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
+    if (!s->have_state) return fail(NBG_ERR_STATE, "set_state has not been called");
+    DeviceGuard g(s->ctx->device);
+    if (int r = enqueue_force_pass(s)) return r;
+    mark(s, kPhases);  // a force pass alone has an empty update phase
+    if (int r = sync_and_check(s)) return r;
+    s->have_forces = true;
+    if (ax || ay || az) return nbg_sys_get_accelerations(s, ax, ay, az);
+    return NBG_OK;
+}
+
+int nbg_sys_get_accelerations(nbg_sys *s, float *ax, float *ay, float *az) {
+    if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
+    if (!s->have_forces) return fail(NBG_ERR_STATE, "no force pass has run");
+    DeviceGuard g(s->ctx->device);
+    NBG_HIP(hipStreamSynchronize(s->stream));
+    const size_t b = (size_t)s->n * sizeof(float);
+    if (s->n > 0) {
+        if (ax) NBG_HIP(hipMemcpy(ax, s->ax, b, hipMemcpyDeviceToHost));
+        if (ay) NBG_HIP(hipMemcpy(ay, s->ay, b, hipMemcpyDeviceToHost));
+        if (az) NBG_HIP(hipMemcpy(az, s->az, b, hipMemcpyDeviceToHost));
+    }
+    return NBG_OK;
+}
+
+int nbg_sys_step(nbg_sys *s, int32_t nsteps) {
+    if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
+    if (nsteps < 0) return fail(NBG_ERR_ARG, "nsteps must be >= 0");
+    if (!s->have_state) return fail(NBG_ERR_STATE, "set_state has not been called");
+    DeviceGuard g(s->ctx->device);
+    for (int i = 0; i < nsteps; ++i) {
+        if (int r = enqueue_force_pass(s)) return r;
+        launch_update(s->x, s->y, s->z, s->vx, s->vy, s->vz, s->ax, s->ay, s->az, s->prm.dt, s->n, s->stream);
+        mark(s, kPhases);
+    }
+    NBG_HIP(hipGetLastError());
+    if (nsteps > 0) s->have_forces = true;
+    return NBG_OK;
+}
+
+int nbg_sys_synchronize(nbg_sys *s) {
+    if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
+    DeviceGuard g(s->ctx->device);
+    return sync_and_check(s);
+}
+
+int nbg_sys_counts(nbg_sys *s, uint64_t *pp, uint64_t *np) {
+    if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
+    if (!s->have_forces) return fail(NBG_ERR_STATE, "no force pass has run");
+    DeviceGuard g(s->ctx->device);
+    NBG_HIP(hipStreamSynchronize(s->stream));
+    unsigned long long c[2];
+    NBG_HIP(hipMemcpy(c, s->counts(), sizeof c, hipMemcpyDeviceToHost));
+    if (pp) *pp = c[0];
+    if (np) *np = c[1];
+    return NBG_OK;
+}
+
+int nbg_sys_enable_timing(nbg_sys *s, int32_t on) {
+    if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
+    DeviceGuard g(s->ctx->device);
+    if (on && s->ev.empty()) {
+        s->ev.assign((size_t)kEventSets * (kPhases + 1), nullptr);
+        for (auto &e : s->ev)
+            NBG_HIP(hipEventCreate(&e));
+    }
+    NBG_HIP(hipStreamSynchronize(s->stream));
+    s->timing = on != 0;
+    s->ev_used = 0;
+    s->phase_n = 0;
+    for (double &v : s->phase_sum) v = 0.0;
+    return NBG_OK;
+}
+
+// Mean per-phase milliseconds over every pass since timing was enabled.
+int nbg_sys_phase_times(nbg_sys *s, float *ms, int32_t cap) {
+    if (!s || !ms) return fail(NBG_ERR_ARG, "sys or ms is NULL");
+    DeviceGuard g(s->ctx->device);
+    if (int r = sync_and_check(s)) return r;
+    if (!s->timing || s->phase_n == 0) return fail(NBG_ERR_STATE, "timing is not enabled or no pass has run");
+    int k = cap < kPhases ? cap : kPhases;
+    for (int i = 0; i < k; ++i) ms[i] = (float)(s->phase_sum[i] / (double)s->phase_n);
+    return k;
+}
+
+int nbg_sys_debug_bbox(nbg_sys *s, float *out6) {
+    if (!s || !out6) return fail(NBG_ERR_ARG, "sys or out is NULL");
+    if (!s->have_forces) return fail(NBG_ERR_STATE, "no force pass has run");
+    DeviceGuard g(s->ctx->device);
+    NBG_HIP(hipStreamSynchronize(s->stream));
+    NBG_HIP(hipMemcpy(out6, s->bbox, 6 * sizeof(float), hipMemcpyDeviceToHost));
+    return NBG_OK;
+}
+
+int nbg_sys_debug_order(nbg_sys *s, uint64_t *keys, uint32_t *perm) {
+    if (!s || !perm) return fail(NBG_ERR_ARG, "sys or perm is NULL");
+    if (!s->have_forces) return fail(NBG_ERR_STATE, "no force pass has run");
+    DeviceGuard g(s->ctx->device);
+    NBG_HIP(hipStreamSynchronize(s->stream));
+    const int64_t n = s->n;
+    if (n == 0) return NBG_OK;
+    NBG_HIP(hipMemcpy(perm, s->idx, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (keys) {
+        if (s->prm.morton_bits == 30) {
+            std::vector<uint32_t> k32(n);
+            NBG_HIP(hipMemcpy(k32.data(), s->keys, (size_t)n * 4, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; ++i) keys[i] = k32[i];
+        } else {
+            NBG_HIP(hipMemcpy(keys, s->keys, (size_t)n * 8, hipMemcpyDeviceToHost));
+        }
+    }
+    return NBG_OK;
+}
+
+int nbg_sys_debug_tree(nbg_sys *s, int64_t *num_nodes, int64_t *first, int64_t *count, int32_t *leaf,
+                       int32_t *parent, float *com4, float *geo6) {
+    if (!s || !num_nodes) return fail(NBG_ERR_ARG, "sys or num_nodes is NULL");
+    if (!s->have_forces) return fail(NBG_ERR_STATE, "no force pass has run");
+    DeviceGuard g(s->ctx->device);
+    NBG_HIP(hipStreamSynchronize(s->stream));
+    uint32_t nn = 0;
+    NBG_HIP(hipMemcpy(&nn, s->node_count(), 4, hipMemcpyDeviceToHost));
+    *num_nodes = nn;
+    if (!first && !count && !leaf && !parent && !com4 && !geo6) return NBG_OK;
+    std::vector<int4> t(nn);
+    std::vector<float4> c(nn), lo(nn), hi(nn);
+    if (nn) {
+        NBG_HIP(hipMemcpy(t.data(), s->topo, nn * sizeof(int4), hipMemcpyDeviceToHost));
+        NBG_HIP(hipMemcpy(c.data(), s->com, nn * sizeof(float4), hipMemcpyDeviceToHost));
+        NBG_HIP(hipMemcpy(lo.data(), s->bmin, nn * sizeof(float4), hipMemcpyDeviceToHost));
+        NBG_HIP(hipMemcpy(hi.data(), s->bmax, nn * sizeof(float4), hipMemcpyDeviceToHost));
+    }
+    for (uint32_t i = 0; i < nn; ++i) {
+        if (first) first[i] = t[i].x;
+        if (count) count[i] = t[i].y;
+        if (leaf) leaf[i] = t[i].z < 0;
+        if (parent) parent[i] = t[i].w;
+        if (com4) { com4[4 * i] = c[i].x; com4[4 * i + 1] = c[i].y; com4[4 * i + 2] = c[i].z; com4[4 * i + 3] = c[i].w; }
+        if (geo6) {
+            float gg[6] = {lo[i].x, lo[i].y, lo[i].z, hi[i].x, hi[i].y, hi[i].z};
+            std::memcpy(geo6 + 6 * i, gg, sizeof gg);
+        }
+    }
+    return NBG_OK;
+}
+
+int nbg_direct_f64(nbg_ctx *ctx, int64_t ns, const float *sx, const float *sy, const float *sz, const float *sm,
+                   int64_t nt, const float *tx, const float *ty, const float *tz, double G, double eps2, double *ax,
+                   double *ay, double *az) {
+    if (!ctx || ns < 0 || nt < 0) return fail(NBG_ERR_ARG, "bad arguments");
+    if ((ns > 0 && (!sx || !sy || !sz || !sm)) || (nt > 0 && (!tx || !ty || !tz || !ax || !ay || !az)))
+        return fail(NBG_ERR_ARG, "NULL array");
+    if (nt == 0) return NBG_OK;
+    DeviceGuard g(ctx->device);
+    std::vector<float4> hs(ns), ht(nt);
+    for (int64_t i = 0; i < ns; ++i) hs[i] = make_float4(sx[i], sy[i], sz[i], sm[i]);
+    for (int64_t i = 0; i < nt; ++i) ht[i] = make_float4(tx[i], ty[i], tz[i], 0.0f);
+    float4 *ds = nullptr, *dt = nullptr;
+    double *da = nullptr;
+    hipStream_t st = nullptr;
+    int r = NBG_OK;
+    auto chk = [&](hipError_t e, const char *what) {
+        if (r == NBG_OK && e != hipSuccess) r = fail(NBG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    chk(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+    chk(hipMalloc((void **)&ds, (size_t)(ns > 0 ? ns : 1) * sizeof(float4)), "hipMalloc");
+    chk(hipMalloc((void **)&dt, (size_t)nt * sizeof(float4)), "hipMalloc");
+    chk(hipMalloc((void **)&da, (size_t)nt * 3 * sizeof(double)), "hipMalloc");
+    if (ns > 0) chk(hipMemcpyAsync(ds, hs.data(), ns * sizeof(float4), hipMemcpyHostToDevice, st), "H2D");
+    chk(hipMemcpyAsync(dt, ht.data(), nt * sizeof(float4), hipMemcpyHostToDevice, st), "H2D");
+    if (r == NBG_OK) {
+        launch_direct_f64(ds, ns, dt, nt, G, eps2, da, da + nt, da + 2 * nt, st);
+        chk(hipGetLastError(), "launch");
+    }
+    chk(hipStreamSynchronize(st), "sync");
+    std::vector<double> out(3 * nt);
+    chk(hipMemcpy(out.data(), da, 3 * nt * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+    if (r == NBG_OK)
+        for (int64_t i = 0; i < nt; ++i) { ax[i] = out[i]; ay[i] = out[nt + i]; az[i] = out[2 * nt + i]; }
+    if (ds) (void)hipFree(ds);
+    if (dt) (void)hipFree(dt);
+    if (da) (void)hipFree(da);
+    if (st) (void)hipStreamDestroy(st);
+    return r;
+}
+
+int nbg_sys_potential_f64(nbg_sys *s, double *W) {
+    if (!s || !W) return fail(NBG_ERR_ARG, "sys or W is NULL");
+    DeviceGuard g(s->ctx->device);
+    const int64_t n = s->n;
+    *W = 0.0;
+    if (n < 2) return NBG_OK;
+    const int64_t nb = potential_blocks(n);
+    double *part = nullptr;
+    float4 *packed = nullptr;
+    NBG_HIP(hipMalloc((void **)&part, (size_t)nb * sizeof(double)));
+    hipError_t e = hipMalloc((void **)&packed, (size_t)n * sizeof(float4));
+    if (e == hipSuccess) {
+        launch_pack(s->x, s->y, s->z, s->m, n, packed, s->stream);
+        launch_potential_f64(packed, n, s->prm.G, s->prm.eps2, part, s->stream);
+        e = hipStreamSynchronize(s->stream);
+    }
+    std::vector<double> h(nb);
+    if (e == hipSuccess) e = hipMemcpy(h.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(part);
+    if (packed) (void)hipFree(packed);
+    if (e != hipSuccess) return fail(NBG_ERR_HIP, std::string("potential: ") + hipGetErrorString(e));
+    double w = 0.0;
+    for (double v : h) w += v;
+    *W = w;
+    return NBG_OK;
+}
+
+}  // extern "C"

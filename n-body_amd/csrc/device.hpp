@@ -1,0 +1,97 @@
+// Device-side helpers shared by the gfx950 kernels (wave64, CDNA4).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nbg {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+// Lanes strictly below this one.
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const int l = lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+__device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    uint32_t lo = __builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ float readlane_f32(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Relaxed agent-scope atomics on 32-bit status words (look-back protocols).
+__device__ __forceinline__ uint32_t ld_acquire_agent(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed_agent(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Inclusive scan across one wave64.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int l = lane_id();
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, kWave);
+        if (l >= o) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Exclusive scan over a block of NT threads (one value each); `tmp` holds
+// NT/64 words of LDS.  Every thread of the block must call it.
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *tmp, uint32_t *total) {
+    constexpr int W = NT / kWave;
+    const uint32_t inc = wave_incl_scan(v);
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == kWave - 1) tmp[w] = inc;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        uint32_t t = tmp[i];
+        off += (i < w) ? t : 0u;
+        tot += t;
+    }
+    __syncthreads();
+    if (total) *total = tot;
+    return off + inc - v;
+}
+
+// Bit spread of the reference's expandBits10 / expandBits21 magic-constant
+// forms (MortonSort.h:27-44).
+__device__ __forceinline__ uint32_t expand_bits10(uint32_t i) {
+    i = (i * 0x00010001u) & 0xFF0000FFu;
+    i = (i * 0x00000101u) & 0x0F00F00Fu;
+    i = (i * 0x00000011u) & 0xC30C30C3u;
+    i = (i * 0x00000005u) & 0x49249249u;
+    return i;
+}
+
+__device__ __forceinline__ uint64_t expand_bits21(uint64_t i) {
+    i &= 0x1fffffull;
+    i = (i | i << 32) & 0x1f00000000ffffull;
+    i = (i | i << 16) & 0x1f0000ff0000ffull;
+    i = (i | i << 8) & 0x100f00f00f00f00full;
+    i = (i | i << 4) & 0x10c30c30c30c30c3ull;
+    i = (i | i << 2) & 0x1249249249249249ull;
+    return i;
+}
+
+}  // namespace nbg

@@ -1,0 +1,711 @@
+// ORACLE — TEST INFRASTRUCTURE ONLY.
+//
+// A CPU restatement, written from scratch, of the reference simulator's
+// tree-code force pass (JacksonCampolattaro/n-body).  It exists to check the
+// HIP implementation: only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it, and never as the thing measured or shipped.
+//
+// Every function cites the reference file:line whose semantics it restates
+// (paths relative to the reference's src/NBody/).  Arithmetic follows the
+// reference's FP32 operation order exactly (glm scalar paths, no FMA: build
+// with -ffp-contract=off, no -ffast-math), so results are bit-comparable.
+//
+// Where the reference is not deterministic the oracle fixes a rule:
+//   * sort ties (MortonSort.h:75-88,141-183 use std::partition + spreadsort,
+//     both unstable): ties are broken by original body index (stable sort);
+//   * octree partition (Octree.h:33-61 uses std::partition): stable, so the
+//     bodies of a leaf appear in ascending original index.
+// These two orders are the only freedoms; everything else is the reference's.
+//
+// Parity pins (see tests/test_oracle.py): reference KATs gravity.cpp:13-26,
+// centerOfMassSummary.cpp:12-47, boundingBox.cpp:12-32; closed forms for
+// Morton interleave, bbox, COM, integrator; brute force for sort/topology;
+// theta->0 walk == direct sum over leaf order; FP64 direct-sum force error.
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+struct V3 {
+    float x, y, z;
+};
+
+inline V3 v_add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 v_sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 v_mul(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline V3 v_div(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+inline bool v_eq(V3 a, V3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+// glm::dot on vec3: tmp = a*b; tmp.x + tmp.y + tmp.z (left to right).
+inline float v_len2(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+const V3 kZero{0.0f, 0.0f, 0.0f};
+
+struct Params {
+    int solver;       // 0 = Barnes-Hut octree, 1 = linear BVH
+    float theta;
+    int leaf_size;
+    int max_depth;    // octree refine depth below the 2 forced levels
+    int morton_bits;  // 30 or 63
+    float G;
+    float eps2;       // the reference's Gravity "epsilon" (added to r^2)
+    float dt;
+};
+
+struct Counts {
+    uint64_t pp = 0, np = 0;
+};
+
+// Gravity::particleParticle (Physics/Rules/Gravity.h:199-209) with
+// pow<3>(r) = r*(r*(r*1)) (Gravity.h:35-41).
+inline V3 particle_particle(V3 a, float m, V3 p, float G, float eps2) {
+    if (v_eq(a, p)) return kZero;
+    V3 R = v_sub(p, a);
+    float r = std::sqrt(v_len2(R) + eps2);
+    float r3 = r * (r * r);
+    V3 nR{-R.x, -R.y, -R.z};
+    return v_div(v_mul(v_mul(nR, m), G), r3);
+}
+
+// ---------------------------------------------------------------- bbox
+// outerBoundingBox (Physics/Summaries/SummaryType.h:73-81), BoundingBox
+// defaults +-FLT_MAX (Physics/BoundingBox.h:27-37).
+void bbox_of(int64_t n, const float *x, const float *y, const float *z, float out[6]) {
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+    float mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int64_t i = 0; i < n; ++i) {
+        // glm::min(cur, p) = (p < cur) ? p : cur; glm::max(cur, p) = (cur < p) ? p : cur
+        mn[0] = std::min(mn[0], x[i]); mx[0] = std::max(mx[0], x[i]);
+        mn[1] = std::min(mn[1], y[i]); mx[1] = std::max(mx[1], y[i]);
+        mn[2] = std::min(mn[2], z[i]); mx[2] = std::max(mx[2], z[i]);
+    }
+    for (int d = 0; d < 3; ++d) { out[d] = mn[d]; out[3 + d] = mx[d]; }
+}
+
+// ---------------------------------------------------------------- Morton
+// expandBits10 / expandBits21 / toMortonCode / setMortonCodes
+// (Simulation/Solvers/MortonSort.h:27-73).  Written as a per-bit spread,
+// which the tests check against the reference's magic-constant form.
+uint64_t spread_bits(uint64_t v, int nbits) {
+    uint64_t out = 0;
+    for (int b = 0; b < nbits; ++b) out |= ((v >> b) & 1ull) << (3 * b);
+    return out;
+}
+
+uint64_t morton_of(V3 p, const float bb[6], float scale, int bits) {
+    // (position - min) / scale, then * 2^k, clamp to [0, 2^k - 1], truncate.
+    V3 v = v_div(v_sub(p, V3{bb[0], bb[1], bb[2]}), scale);
+    const int per = bits == 30 ? 10 : 21;
+    const float mul = bits == 30 ? 1024.0f : 2097152.0f;
+    const float hi = bits == 30 ? 1023.0f : 2097151.0f;
+    float c[3] = {v.x * mul, v.y * mul, v.z * mul};
+    uint64_t q[3];
+    for (int d = 0; d < 3; ++d) {
+        // glm::clamp(x, lo, hi) = min(max(x, lo), hi)
+        float t = std::min(std::max(c[d], 0.0f), hi);
+        // NaN (a zero-extent box) converts to 0 here; the reference's cast of
+        // NaN is undefined behaviour, so this is the oracle's defined choice.
+        q[d] = (t != t) ? 0 : (bits == 30 ? (uint64_t)(uint32_t)t : (uint64_t)t);
+    }
+    return spread_bits(q[0], per) * 4u + spread_bits(q[1], per) * 2u + spread_bits(q[2], per);
+}
+
+float morton_scale(const float bb[6]) {
+    float dx = bb[3] - bb[0], dy = bb[4] - bb[1], dz = bb[5] - bb[2];
+    return std::max(std::max(dx, dy), dz);
+}
+
+// ---------------------------------------------------------------- LBVH
+struct LNode {
+    int64_t first, count;
+    int32_t child0 = -1, child1 = -1;
+    int32_t depth = 0;
+    bool leaf = true;
+    V3 com{0, 0, 0};
+    float mass = 0;
+    V3 bmin{FLT_MAX, FLT_MAX, FLT_MAX}, bmax{-FLT_MAX, -FLT_MAX, -FLT_MAX};
+};
+
+struct LBVH {
+    Params prm;
+    int64_t n = 0;
+    std::vector<uint64_t> keys;      // sorted keys
+    std::vector<uint32_t> perm;      // sorted position -> original index
+    std::vector<V3> pos;             // sorted positions
+    std::vector<float> mass;         // sorted masses
+    std::vector<LNode> nodes;        // pre-order
+    float bbox[6];
+    int max_depth = 0;
+
+    // CenterOfMassSummary::summarize(entities) (CenterOfMassSummary.h:30-42)
+    // + BoundingBoxSummary leaf loop (BoundingBoxSummary.h:30-57).
+    void summarize_leaf(LNode &nd) {
+        float M = 0.0f;
+        V3 C = kZero;
+        V3 mn{FLT_MAX, FLT_MAX, FLT_MAX}, mx{-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (int64_t i = nd.first; i < nd.first + nd.count; ++i) {
+            float m = mass[i];
+            M += m;
+            C = v_add(C, v_mul(pos[i], m));
+        }
+        C = v_div(C, M);
+        for (int64_t i = nd.first; i < nd.first + nd.count; ++i) {
+            V3 p = pos[i];
+            mn = {std::min(p.x, mn.x), std::min(p.y, mn.y), std::min(p.z, mn.z)};
+            mx = {std::max(p.x, mx.x), std::max(p.y, mx.y), std::max(p.z, mx.z)};
+        }
+        nd.com = C; nd.mass = M; nd.bmin = mn; nd.bmax = mx;
+    }
+
+    // CenterOfMassSummary::summarize(childNodes) (CenterOfMassSummary.h:44-55)
+    // + BoundingBoxSummary child loop (BoundingBoxSummary.h:59-68).
+    void summarize_internal(LNode &nd) {
+        const LNode &a = nodes[nd.child0], &b = nodes[nd.child1];
+        float M = 0.0f;
+        V3 C = kZero;
+        M += a.mass; C = v_add(C, v_mul(a.com, a.mass));
+        M += b.mass; C = v_add(C, v_mul(b.com, b.mass));
+        nd.com = v_div(C, M); nd.mass = M;
+        V3 mn{FLT_MAX, FLT_MAX, FLT_MAX}, mx{-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (const LNode *c : {&a, &b}) {
+            mn = {std::min(c->bmin.x, mn.x), std::min(c->bmin.y, mn.y), std::min(c->bmin.z, mn.z)};
+            mx = {std::max(c->bmax.x, mx.x), std::max(c->bmax.y, mx.y), std::max(c->bmax.z, mx.z)};
+        }
+        nd.bmin = mn; nd.bmax = mx;
+    }
+
+    // LinearBVH::build split criterion (Trees/LinearBVH.h:114-119),
+    // LinearBVHNode::split (LinearBVH.h:47-76), NodeBase::refine
+    // (Trees/NodeBase.h:57-75).  Nodes are appended in pre-order.
+    int32_t build(int64_t first, int64_t count, int depth) {
+        int32_t id = (int32_t)nodes.size();
+        nodes.push_back(LNode{});
+        nodes[id].first = first; nodes[id].count = count; nodes[id].depth = depth;
+        max_depth = std::max(max_depth, depth);
+        const int64_t last = first + count - 1;
+        bool split = count > prm.leaf_size && keys[first] != keys[last];
+        if (!split) {
+            nodes[id].leaf = true;
+            summarize_leaf(nodes[id]);
+            return id;
+        }
+        uint64_t x = keys[first] ^ keys[last];
+        int msb = 63 - __builtin_clzll(x);
+        uint64_t boundary = 1ull << msb;
+        // std::upper_bound with comp(v, e) = (v & key(e)) != 0: first index
+        // whose key has the boundary bit set.
+        int64_t lo = first, hi = last + 1;
+        while (lo < hi) {
+            int64_t mid = lo + (hi - lo) / 2;
+            if ((boundary & keys[mid]) != 0) hi = mid; else lo = mid + 1;
+        }
+        int64_t s = lo;
+        nodes[id].leaf = false;
+        int32_t c0 = build(first, s - first, depth + 1);
+        int32_t c1 = build(s, last + 1 - s, depth + 1);
+        nodes[id].child0 = c0; nodes[id].child1 = c1;
+        summarize_internal(nodes[id]);
+        return id;
+    }
+
+    void construct(const Params &p, int64_t n_, const float *x, const float *y, const float *z,
+                   const float *m) {
+        prm = p; n = n_;
+        bbox_of(n, x, y, z, bbox);
+        float scale = morton_scale(bbox);
+        std::vector<uint64_t> k(n);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) k[i] = morton_of(V3{x[i], y[i], z[i]}, bbox, scale, prm.morton_bits);
+        perm.resize(n);
+        std::iota(perm.begin(), perm.end(), 0u);
+        std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return k[a] < k[b]; });
+        keys.resize(n); pos.resize(n); mass.resize(n);
+        for (int64_t i = 0; i < n; ++i) {
+            keys[i] = k[perm[i]];
+            pos[i] = V3{x[perm[i]], y[perm[i]], z[perm[i]]};
+            mass[i] = m[perm[i]];
+        }
+        nodes.clear();
+        if (n > 0) build(0, n, 0);
+    }
+
+    // DiagonalOverCenterDistance (Descent/DescentCriterion/DiagonalOverCenterDistance.h:18-22):
+    // diagonalLength2 / distance2(center, p) < theta^2, centre = (max+min)/2
+    // (Physics/BoundingBox.h:47,55).
+    bool mac(const LNode &nd, V3 p, float theta2) const {
+        V3 c = v_div(v_add(nd.bmax, nd.bmin), 2.0f);
+        float diag2 = v_len2(v_sub(nd.bmax, nd.bmin));
+        float d2 = v_len2(v_sub(p, c));
+        return (diag2 / d2) < theta2;
+    }
+
+    // Descent::activeTree (Descent/activeTree.h:24-79); children are combined
+    // by std::transform_reduce, which for 2 children is (0 + r0) + r1.
+    V3 walk(int32_t id, V3 p, float theta2, Counts &cnt) const {
+        const LNode &nd = nodes[id];
+        if (nd.count == 0) return kZero;
+        if (mac(nd, p, theta2)) {
+            cnt.np++;
+            return particle_particle(nd.com, nd.mass, p, prm.G, prm.eps2);
+        }
+        if (nd.leaf) {
+            V3 acc = kZero;
+            for (int64_t i = nd.first; i < nd.first + nd.count; ++i) {
+                cnt.pp++;
+                acc = v_add(acc, particle_particle(pos[i], mass[i], p, prm.G, prm.eps2));
+            }
+            return acc;
+        }
+        V3 r0 = walk(nd.child0, p, theta2, cnt);
+        V3 r1 = walk(nd.child1, p, theta2, cnt);
+        return v_add(v_add(kZero, r0), r1);
+    }
+};
+
+// ---------------------------------------------------------------- Octree
+struct ONode {
+    int64_t first = 0, count = 0;
+    int32_t child[8];
+    int32_t depth = 0;
+    bool leaf = true;
+    bool summarized = false;
+    V3 center{0, 0, 0};
+    float side = 1.0f;
+    V3 com{0, 0, 0};
+    float mass = 0;
+};
+
+struct Octree {
+    Params prm;
+    int64_t n = 0;
+    std::vector<uint32_t> idx;   // node contents, permuted in place
+    std::vector<V3> P;           // original-order positions
+    std::vector<float> Mv;       // original-order masses
+    std::vector<ONode> nodes;
+    float bbox[6];
+    int max_depth = 0;
+
+    // OctreeNode::split + partition<2> (Trees/Octree.h:33-61,100-147):
+    // partition on z, then y, then x with p[d] < c[d]; the children array is
+    // x-major (index 4x + 2y + z); child centres c +- (side/2)/2.
+    void split(int32_t id) {
+        ONode nd = nodes[id];
+        auto beg = idx.begin() + nd.first, end = beg + nd.count;
+        V3 c = nd.center;
+        auto less = [&](int d) {
+            return [&, d](uint32_t e) {
+                float v = d == 0 ? P[e].x : (d == 1 ? P[e].y : P[e].z);
+                float cv = d == 0 ? c.x : (d == 1 ? c.y : c.z);
+                return v < cv;
+            };
+        };
+        auto mz = std::stable_partition(beg, end, less(2));
+        auto my0 = std::stable_partition(beg, mz, less(1));
+        auto my1 = std::stable_partition(mz, end, less(1));
+        auto mx00 = std::stable_partition(beg, my0, less(0));
+        auto mx01 = std::stable_partition(my0, mz, less(0));
+        auto mx10 = std::stable_partition(mz, my1, less(0));
+        auto mx11 = std::stable_partition(my1, end, less(0));
+        // spans in memory order (z,y,x): zyx=000,001(x),010(y),011,100(z),...
+        struct Sp { std::vector<uint32_t>::iterator b, e; };
+        Sp s_zyx[8] = {{beg, mx00}, {mx00, my0}, {my0, mx01}, {mx01, mz},
+                       {mz, mx10}, {mx10, my1}, {my1, mx11}, {mx11, end}};
+        float cs = nd.side / 2.0f;
+        float off = cs / 2.0f;
+        int32_t base = (int32_t)nodes.size();
+        for (int k = 0; k < 8; ++k) {
+            int xb = (k >> 2) & 1, yb = (k >> 1) & 1, zb = k & 1;   // child index 4x+2y+z
+            Sp sp = s_zyx[zb * 4 + yb * 2 + xb];
+            ONode ch;
+            ch.first = sp.b - idx.begin();
+            ch.count = sp.e - sp.b;
+            ch.depth = nd.depth + 1;
+            ch.center = v_add(c, V3{xb ? off : -off, yb ? off : -off, zb ? off : -off});
+            ch.side = cs;
+            nodes.push_back(ch);
+        }
+        nodes[id].leaf = false;
+        for (int k = 0; k < 8; ++k) nodes[id].child[k] = base + k;
+        max_depth = std::max(max_depth, nd.depth + 1);
+    }
+
+    // CenterOfMassSummary leaf / child loops (CenterOfMassSummary.h:30-55).
+    void summarize(int32_t id) {
+        ONode &nd = nodes[id];
+        float M = 0.0f;
+        V3 C = kZero;
+        if (nd.leaf) {
+            for (int64_t i = nd.first; i < nd.first + nd.count; ++i) {
+                uint32_t e = idx[i];
+                M += Mv[e];
+                C = v_add(C, v_mul(P[e], Mv[e]));
+            }
+        } else {
+            for (int k = 0; k < 8; ++k) {
+                const ONode &ch = nodes[nd.child[k]];
+                M += ch.mass;
+                C = v_add(C, v_mul(ch.com, ch.mass));
+            }
+        }
+        nodes[id].com = v_div(C, M);
+        nodes[id].mass = M;
+        nodes[id].summarized = true;
+    }
+
+    // NodeBase::refine (Trees/NodeBase.h:57-75): empty nodes return without
+    // a summary (they keep the default zero COM / zero mass).
+    void refine(int32_t id, int depth_left) {
+        if (nodes[id].count == 0) return;
+        if (depth_left > 0 && nodes[id].count > prm.leaf_size) {
+            split(id);
+            for (int k = 0; k < 8; ++k) refine(nodes[id].child[k], depth_left - 1);
+        } else {
+            nodes[id].leaf = true;
+        }
+        summarize(id);
+    }
+
+    // Octree::build (Trees/Octree.h:185-205): root centre min + diag/2,
+    // side = maxSide * 1.1; depthSplit(2) (Trees/Tree.h:172-194) splits the
+    // root and all 8 level-1 nodes unconditionally; level-2 nodes refine with
+    // maxDepth; summarizeTreeTop (Tree.h:196-216) summarises levels 1 and 0.
+    void construct(const Params &p, int64_t n_, const float *x, const float *y, const float *z,
+                   const float *m) {
+        prm = p; n = n_;
+        P.resize(n); Mv.assign(m, m + n);
+        for (int64_t i = 0; i < n; ++i) P[i] = V3{x[i], y[i], z[i]};
+        bbox_of(n, x, y, z, bbox);
+        idx.resize(n);
+        std::iota(idx.begin(), idx.end(), 0u);
+        nodes.clear();
+        ONode root;
+        root.first = 0; root.count = n;
+        V3 mn{bbox[0], bbox[1], bbox[2]}, mx{bbox[3], bbox[4], bbox[5]};
+        V3 diag = v_sub(mx, mn);
+        root.center = v_add(mn, v_div(diag, 2.0f));
+        root.side = std::max({diag.x, diag.y, diag.z}) * 1.1f;
+        nodes.push_back(root);
+        max_depth = 0;
+        split(0);
+        for (int k = 0; k < 8; ++k) split(nodes[0].child[k]);
+        for (int k = 0; k < 8; ++k) {
+            int32_t l1 = nodes[0].child[k];
+            for (int j = 0; j < 8; ++j) refine(nodes[l1].child[j], prm.max_depth);
+        }
+        for (int k = 0; k < 8; ++k) summarize(nodes[0].child[k]);
+        summarize(0);
+    }
+
+    // SideLengthOverDistance (Descent/DescentCriterion/SideLengthOverDistance.h:17-22)
+    // with exclusionRegion(OctreeNode) (Descent/DescentCriterionType.h:81-85)
+    // and half-open BoundingBox::contains (Physics/BoundingBox.h:61-64).
+    bool mac(const ONode &nd, V3 p) const {
+        float dist = std::sqrt(v_len2(v_sub(p, nd.com)));
+        if (!((nd.side / dist) < prm.theta)) return false;
+        V3 lo{nd.center.x - nd.side, nd.center.y - nd.side, nd.center.z - nd.side};
+        V3 hi{nd.center.x + nd.side, nd.center.y + nd.side, nd.center.z + nd.side};
+        bool inside = lo.x <= p.x && lo.y <= p.y && lo.z <= p.z && p.x < hi.x && p.y < hi.y && p.z < hi.z;
+        return !inside;
+    }
+
+    // Descent::activeTree (Descent/activeTree.h:24-79); 8 children combined by
+    // libstdc++ transform_reduce: init + ((r0+r1)+(r2+r3)), then
+    // + ((r4+r5)+(r6+r7)).
+    V3 walk(int32_t id, V3 p, Counts &cnt) const {
+        const ONode &nd = nodes[id];
+        if (nd.count == 0) return kZero;
+        if (mac(nd, p)) {
+            cnt.np++;
+            return particle_particle(nd.com, nd.mass, p, prm.G, prm.eps2);
+        }
+        if (nd.leaf) {
+            V3 acc = kZero;
+            for (int64_t i = nd.first; i < nd.first + nd.count; ++i) {
+                uint32_t e = idx[i];
+                cnt.pp++;
+                acc = v_add(acc, particle_particle(P[e], Mv[e], p, prm.G, prm.eps2));
+            }
+            return acc;
+        }
+        V3 r[8];
+        for (int k = 0; k < 8; ++k) r[k] = walk(nd.child[k], p, cnt);
+        V3 init = kZero;
+        init = v_add(init, v_add(v_add(r[0], r[1]), v_add(r[2], r[3])));
+        init = v_add(init, v_add(v_add(r[4], r[5]), v_add(r[6], r[7])));
+        return init;
+    }
+};
+
+struct Handle {
+    int kind;  // 0 octree, 1 lbvh
+    LBVH *l = nullptr;
+    Octree *o = nullptr;
+    ~Handle() { delete l; delete o; }
+};
+
+Params to_params(const float *fp, const int *ip) {
+    Params p;
+    p.solver = ip[0]; p.leaf_size = ip[1]; p.max_depth = ip[2]; p.morton_bits = ip[3];
+    p.theta = fp[0]; p.G = fp[1]; p.eps2 = fp[2]; p.dt = fp[3];
+    return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Params are passed as two small arrays so the ctypes side stays trivial:
+//   ip = {solver, leaf_size, max_depth, morton_bits}, fp = {theta, G, eps2, dt}.
+
+int orc_bbox(int64_t n, const float *x, const float *y, const float *z, float *out6) {
+    bbox_of(n, x, y, z, out6);
+    return 0;
+}
+
+int orc_morton(int64_t n, const float *x, const float *y, const float *z, int bits, const float *bb,
+               uint64_t *keys) {
+    if (bits != 30 && bits != 63) return -1;
+    float scale = morton_scale(bb);
+    for (int64_t i = 0; i < n; ++i) keys[i] = morton_of(V3{x[i], y[i], z[i]}, bb, scale, bits);
+    return 0;
+}
+
+// Reference form of the interleave (MortonSort.h:27-44), used only by the
+// tests to pin spread_bits.
+uint64_t orc_expand_bits_magic(uint64_t i, int bits) {
+    if (bits == 30) {
+        uint32_t v = (uint32_t)i;
+        v = (v * 0x00010001u) & 0xFF0000FFu;
+        v = (v * 0x00000101u) & 0x0F00F00Fu;
+        v = (v * 0x00000011u) & 0xC30C30C3u;
+        v = (v * 0x00000005u) & 0x49249249u;
+        return v;
+    }
+    i &= 0x1fffff;
+    i = (i | i << 32) & 0x1f00000000ffffull;
+    i = (i | i << 16) & 0x1f0000ff0000ffull;
+    i = (i | i << 8) & 0x100f00f00f00f00full;
+    i = (i | i << 4) & 0x10c30c30c30c30c3ull;
+    i = (i | i << 2) & 0x1249249249249249ull;
+    return i;
+}
+
+uint64_t orc_expand_bits(uint64_t v, int bits) { return spread_bits(v, bits == 30 ? 10 : 21); }
+
+int orc_sort(int64_t n, const uint64_t *keys, uint32_t *perm) {
+    std::iota(perm, perm + n, 0u);
+    std::stable_sort(perm, perm + n, [&](uint32_t a, uint32_t b) { return keys[a] < keys[b]; });
+    return 0;
+}
+
+void *orc_tree_new(const int *ip, const float *fp, int64_t n, const float *x, const float *y,
+                   const float *z, const float *m) {
+    Params p = to_params(fp, ip);
+    Handle *h = new Handle;
+    h->kind = p.solver;
+    if (p.solver == 1) {
+        if (p.morton_bits != 30 && p.morton_bits != 63) { delete h; return nullptr; }
+        h->l = new LBVH;
+        h->l->construct(p, n, x, y, z, m);
+    } else {
+        h->o = new Octree;
+        h->o->construct(p, n, x, y, z, m);
+    }
+    return h;
+}
+
+void orc_tree_free(void *hp) { delete (Handle *)hp; }
+
+int64_t orc_tree_num_nodes(void *hp) {
+    Handle *h = (Handle *)hp;
+    return h->kind == 1 ? (int64_t)h->l->nodes.size() : (int64_t)h->o->nodes.size();
+}
+
+int orc_tree_max_depth(void *hp) {
+    Handle *h = (Handle *)hp;
+    return h->kind == 1 ? h->l->max_depth : h->o->max_depth;
+}
+
+// Nodes in pre-order: first, count, leaf flag, depth, com(x,y,z,mass) and,
+// per kind, bbox (min,max) for LBVH or (centre, side, 0, 0) for the octree.
+int orc_tree_nodes(void *hp, int64_t *first, int64_t *count, int32_t *leaf, int32_t *depth,
+                   float *com4, float *geo6) {
+    Handle *h = (Handle *)hp;
+    if (h->kind == 1) {
+        const auto &N = h->l->nodes;
+        for (size_t i = 0; i < N.size(); ++i) {
+            first[i] = N[i].first; count[i] = N[i].count; leaf[i] = N[i].leaf; depth[i] = N[i].depth;
+            com4[4 * i] = N[i].com.x; com4[4 * i + 1] = N[i].com.y; com4[4 * i + 2] = N[i].com.z;
+            com4[4 * i + 3] = N[i].mass;
+            float g[6] = {N[i].bmin.x, N[i].bmin.y, N[i].bmin.z, N[i].bmax.x, N[i].bmax.y, N[i].bmax.z};
+            std::memcpy(geo6 + 6 * i, g, sizeof g);
+        }
+    } else {
+        const auto &N = h->o->nodes;
+        for (size_t i = 0; i < N.size(); ++i) {
+            first[i] = N[i].first; count[i] = N[i].count; leaf[i] = N[i].leaf; depth[i] = N[i].depth;
+            com4[4 * i] = N[i].com.x; com4[4 * i + 1] = N[i].com.y; com4[4 * i + 2] = N[i].com.z;
+            com4[4 * i + 3] = N[i].mass;
+            float g[6] = {N[i].center.x, N[i].center.y, N[i].center.z, N[i].side, 0.0f, 0.0f};
+            std::memcpy(geo6 + 6 * i, g, sizeof g);
+        }
+    }
+    return 0;
+}
+
+// Body order of the tree (sorted position -> original index) and, for the
+// LBVH, the sorted keys.
+int orc_tree_order(void *hp, uint32_t *perm, uint64_t *keys_or_null) {
+    Handle *h = (Handle *)hp;
+    if (h->kind == 1) {
+        std::copy(h->l->perm.begin(), h->l->perm.end(), perm);
+        if (keys_or_null) std::copy(h->l->keys.begin(), h->l->keys.end(), keys_or_null);
+    } else {
+        std::copy(h->o->idx.begin(), h->o->idx.end(), perm);
+    }
+    return 0;
+}
+
+int orc_tree_bbox(void *hp, float *out6) {
+    Handle *h = (Handle *)hp;
+    std::memcpy(out6, h->kind == 1 ? h->l->bbox : h->o->bbox, 6 * sizeof(float));
+    return 0;
+}
+
+// ActiveTreeSolver::updateAccelerations walk (Solvers/ActiveTreeSolver.h:61-70)
+// for arbitrary target positions; per-target interaction counts (pp, np) as
+// SimpleTrackingRule counts them (Physics/Rules/SimpleTrackingRule.h:29-45).
+int orc_tree_accel(void *hp, int64_t nt, const float *tx, const float *ty, const float *tz, float *ax,
+                   float *ay, float *az, uint64_t *pp_or_null, uint64_t *np_or_null, int threads) {
+    Handle *h = (Handle *)hp;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < nt; ++i) {
+        Counts c;
+        V3 p{tx[i], ty[i], tz[i]}, a;
+        if (h->kind == 1) {
+            float t2 = h->l->prm.theta * h->l->prm.theta;
+            a = h->l->nodes.empty() ? kZero : h->l->walk(0, p, t2, c);
+        } else {
+            a = h->o->nodes.empty() ? kZero : h->o->walk(0, p, c);
+        }
+        ax[i] = a.x; ay[i] = a.y; az[i] = a.z;
+        if (pp_or_null) pp_or_null[i] = c.pp;
+        if (np_or_null) np_or_null[i] = c.np;
+    }
+    return 0;
+}
+
+// Solver::step (Simulation/Solver.h:103-139): accelerations from the tree,
+// then v = v + a*dt, then p = p + v*dt, each a separate pass, no FMA.
+int orc_step(const int *ip, const float *fp, int64_t n, float *x, float *y, float *z, const float *m,
+             float *vx, float *vy, float *vz, float *ax, float *ay, float *az, int nsteps, int threads) {
+    Params p = to_params(fp, ip);
+    for (int s = 0; s < nsteps; ++s) {
+        void *h = orc_tree_new(ip, fp, n, x, y, z, m);
+        if (!h) return -1;
+        orc_tree_accel(h, n, x, y, z, ax, ay, az, nullptr, nullptr, threads);
+        orc_tree_free(h);
+        for (int64_t i = 0; i < n; ++i) {
+            vx[i] = vx[i] + ax[i] * p.dt; vy[i] = vy[i] + ay[i] * p.dt; vz[i] = vz[i] + az[i] * p.dt;
+        }
+        for (int64_t i = 0; i < n; ++i) {
+            x[i] = x[i] + vx[i] * p.dt; y[i] = y[i] + vy[i] * p.dt; z[i] = z[i] + vz[i] * p.dt;
+        }
+    }
+    return 0;
+}
+
+// NaiveSolver::updateAccelerations (Simulation/Solvers/NaiveSolver.h:22-52):
+// FP32, sources summed in index order into a zeroed accumulator.
+int orc_direct_f32(int64_t ns, const float *sx, const float *sy, const float *sz, const float *sm,
+                   int64_t nt, const float *tx, const float *ty, const float *tz, float G, float eps2,
+                   float *ax, float *ay, float *az) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < nt; ++i) {
+        V3 p{tx[i], ty[i], tz[i]}, acc = kZero;
+        for (int64_t j = 0; j < ns; ++j)
+            acc = v_add(acc, particle_particle(V3{sx[j], sy[j], sz[j]}, sm[j], p, G, eps2));
+        ax[i] = acc.x; ay[i] = acc.y; az[i] = acc.z;
+    }
+    return 0;
+}
+
+// FP64 direct sum with the reference's softening (eps2 added to r^2) and
+// self-skip by exact position equality (Gravity.h:203).
+int orc_direct_f64(int64_t ns, const float *sx, const float *sy, const float *sz, const float *sm,
+                   int64_t nt, const float *tx, const float *ty, const float *tz, double G, double eps2,
+                   double *ax, double *ay, double *az) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < nt; ++i) {
+        double px = tx[i], py = ty[i], pz = tz[i], a0 = 0, a1 = 0, a2 = 0;
+        for (int64_t j = 0; j < ns; ++j) {
+            if (sx[j] == tx[i] && sy[j] == ty[i] && sz[j] == tz[i]) continue;
+            double rx = px - sx[j], ry = py - sy[j], rz = pz - sz[j];
+            double r2 = rx * rx + ry * ry + rz * rz + eps2;
+            double inv = 1.0 / (std::sqrt(r2) * r2);
+            double s = G * (double)sm[j] * inv;
+            a0 -= rx * s; a1 -= ry * s; a2 -= rz * s;
+        }
+        ax[i] = a0; ay[i] = a1; az[i] = a2;
+    }
+    return 0;
+}
+
+// Softened potential energy W = -sum_{i<j} G m_i m_j / sqrt(r^2 + eps2), FP64.
+double orc_potential_f64(int64_t n, const float *x, const float *y, const float *z, const float *m,
+                         double G, double eps2) {
+    double W = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : W)
+    for (int64_t i = 0; i < n; ++i) {
+        double wi = 0;
+        for (int64_t j = i + 1; j < n; ++j) {
+            double rx = (double)x[i] - x[j], ry = (double)y[i] - y[j], rz = (double)z[i] - z[j];
+            wi += (double)m[j] / std::sqrt(rx * rx + ry * ry + rz * rz + eps2);
+        }
+        W -= G * (double)m[i] * wi;
+    }
+    return W;
+}
+
+// The reference's P2P rule on one pair (Gravity.h:199-209), for the KATs.
+int orc_particle_particle(const float *a3, float m, const float *p3, float G, float eps2, float *out3) {
+    V3 r = particle_particle(V3{a3[0], a3[1], a3[2]}, m, V3{p3[0], p3[1], p3[2]}, G, eps2);
+    out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
+    return 0;
+}
+
+// BoundingBox box-box intersection (Physics/BoundingBox.h:77-82), for the KAT.
+int orc_boxes_intersect(const float *a6, const float *b6) {
+    for (int d = 0; d < 3; ++d) {
+        float lo = std::max(a6[d], b6[d]);
+        float hi = std::min(a6[3 + d], b6[3 + d]);
+        if (!(hi >= lo)) return 0;
+    }
+    return 1;
+}
+
+// CenterOfMassSummary leaf summary of an explicit body list, for the KAT.
+int orc_com(int64_t n, const float *x, const float *y, const float *z, const float *m, float *out4) {
+    float M = 0.0f;
+    V3 C = kZero;
+    for (int64_t i = 0; i < n; ++i) {
+        M += m[i];
+        C = v_add(C, v_mul(V3{x[i], y[i], z[i]}, m[i]));
+    }
+    C = v_div(C, M);
+    out4[0] = C.x; out4[1] = C.y; out4[2] = C.z; out4[3] = M;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every declared
+symbol, fills the reference's defaults, reports errors without a GPU, and
+its synthetic generators are deterministic and physically right."""
+
+import math
+import os
+import re
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "nbg.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(nbg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol(nbg):
+    L = nbg.lib()
+    declared = _declared_symbols()
+    assert len(declared) >= 20
+    for name in declared:
+        assert hasattr(L, name), f"libnbg.so does not export {name}"
+    assert sorted(nbg.EXPORTS) == declared
+
+
+def test_reference_defaults(nbg):
+    p = nbg.default_params()
+    assert p.solver == nbg.LINEAR_BVH
+    assert math.isclose(p.theta, 0.3, rel_tol=1e-7)          # ActiveTreeSolver.h:21
+    assert p.leaf_size == 64 and p.max_depth == 32          # Octree.h:167-168, LinearBVH.h:135
+    assert p.morton_bits == 63                               # MortonSort.h:25
+    assert p.G == 1.0 and math.isclose(p.eps2, 1e-5, rel_tol=1e-6)  # Gravity.h:195
+    assert math.isclose(p.dt, 1e-3, rel_tol=1e-6)           # Solver.h:34
+
+
+def test_no_device_is_a_clean_error(nbg):
+    if nbg.device_count() > 0:
+        return
+    try:
+        nbg.Context(0)
+    except nbg.NbgError as e:
+        assert e.code == -3 and "no HIP device" in str(e)
+    else:
+        raise AssertionError("expected NbgError")
+
+
+def test_generators_deterministic(nbg):
+    a = nbg.plummer(1000, seed=5)
+    b = nbg.plummer(1000, seed=5)
+    c = nbg.plummer(1000, seed=6)
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.vz, b.vz)
+    assert not np.array_equal(a.x, c.x)
+    q = nbg.cube(500, seed=3, side=2.0)
+    assert np.all(np.abs(q.x) <= 1.0) and np.all(q.vx == 0) and math.isclose(float(q.m.sum()), 1.0, rel_tol=1e-5)
+
+
+def test_plummer_profile(nbg):
+    """Mass inside r follows M r^3 / (r^2 + a^2)^(3/2); no body beyond 10 a;
+    speeds below the local escape speed."""
+    n = 100000
+    b = nbg.plummer(n, seed=1, a=1.0)
+    r = np.sqrt(b.x.astype(np.float64) ** 2 + b.y ** 2 + b.z ** 2)
+    assert r.max() < 10.0
+    trunc = 1000.0 / (101.0 ** 1.5)
+    for rr in (0.5, 1.0, 2.0, 5.0):
+        frac = np.mean(r < rr)
+        exp = (rr ** 3 / (rr ** 2 + 1) ** 1.5) / trunc
+        assert abs(frac - exp) < 0.01
+    v = np.sqrt(b.vx.astype(np.float64) ** 2 + b.vy ** 2 + b.vz ** 2)
+    vesc = np.sqrt(2.0) * (1 + r ** 2) ** -0.25
+    assert np.all(v <= vesc * (1 + 1e-6))
+    # isotropy: mean position and velocity near zero
+    assert abs(b.x.mean()) < 0.05 and abs(b.vx.mean()) < 0.01

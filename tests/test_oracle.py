@@ -1,0 +1,328 @@
+"""Pins the CPU oracle (oracle/oracle.cpp) before it is trusted as the checker.
+
+Each test names what fixes the expected value: a known-answer test from the
+reference's own suite, a closed form, an invariant, or brute force on tiny
+inputs.  Reference tests cited as src/NBody/... paths.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from treecheck import lbvh_check, octree_check, summaries_check
+
+
+def _rand_bodies(n, seed, spread=1.0):
+    r = np.random.default_rng(seed)
+    x, y, z = (r.normal(0, spread, n).astype(np.float32) for _ in range(3))
+    m = r.uniform(0.5, 1.5, n).astype(np.float32)
+    return x, y, z, m
+
+
+# ------------------------------------------------------------------ reference KATs
+
+def test_kat_particle_particle(orc):
+    """src/NBody/Physics/test/gravity.cpp:13-26: pA=(0,0,0), mA=2, pB=(2,0,0),
+    G=2, default epsilon 1e-5  =>  |a - (-1,0,0)| < 5e-4."""
+    a = orc.particle_particle([0, 0, 0], 2.0, [2, 0, 0], 2.0, 1e-5)
+    assert np.linalg.norm(a - np.array([-1, 0, 0], np.float32)) < 5e-4
+    # closed form: -G m R / (R^2 + eps)^(3/2) = -8 / 4.00001^1.5
+    exact = -8.0 / (4.00001 ** 1.5)
+    assert abs(float(a[0]) - exact) < 4 * np.spacing(np.float32(1.0))
+    assert a[1] == 0 and a[2] == 0
+
+
+def test_kat_self_and_coincident_skip(orc):
+    """Gravity.h:203: a body exactly at the passive position contributes 0."""
+    assert np.all(orc.particle_particle([1, 2, 3], 5.0, [1, 2, 3], 1.0, 0.0) == 0)
+
+
+def test_kat_center_of_mass(orc):
+    """src/NBody/Physics/test/centerOfMassSummary.cpp:12-47: unit masses at
+    z = -1 and +1; the monopole at (0, 20, 0) is within 0.5 % of the sum."""
+    c = orc.com(np.zeros(2), np.zeros(2), np.array([-1, 1.0]), np.ones(2))
+    assert np.array_equal(c, np.array([0, 0, 0, 2], np.float32))
+    p = [0, 20, 0]
+    direct = orc.particle_particle([0, 0, -1], 1.0, p, 1.0, 1e-5) + orc.particle_particle([0, 0, 1], 1.0, p, 1.0, 1e-5)
+    approx = orc.particle_particle(c[:3], c[3], p, 1.0, 1e-5)
+    assert np.linalg.norm(direct - approx) / np.linalg.norm(direct) < 0.005
+
+
+def test_kat_box_intersection(orc):
+    """src/NBody/Physics/test/boundingBox.cpp:12-32 truth table."""
+    box = [0, 0, 0, 1, 1, 1]
+    yes = [[0.5, 0.5, 0.5, 1.5, 1.5, 1.5], [-0.5, -0.5, -0.5, 0.5, 0.5, 0.5],
+           [0.25, 0.25, -0.5, 0.75, 0.75, 1.5], [0.25, 0.25, 0.25, 0.75, 0.75, 0.75]]
+    no = [[0, 0, 10, 1, 1, 11], [10, 10, 0, 11, 11, 1]]
+    for b in yes:
+        assert orc.boxes_intersect(box, b)
+    assert orc.boxes_intersect(yes[3], box)
+    for b in no:
+        assert not orc.boxes_intersect(box, b)
+
+
+# ------------------------------------------------------------------ Morton keys
+
+@pytest.mark.parametrize("bits", [30, 63])
+def test_expand_bits_matches_reference_constants(orc, bits):
+    """The oracle's per-bit spread equals the reference's magic-constant
+    expandBits10/21 (MortonSort.h:27-44) on edges and random values."""
+    per = 10 if bits == 30 else 21
+    r = np.random.default_rng(1)
+    vals = [0, 1, 2, (1 << per) - 1, (1 << per) - 2, 1 << (per - 1)] + r.integers(0, 1 << per, 2000).tolist()
+    for v in vals:
+        assert orc.lib().orc_expand_bits(v, bits) == orc.lib().orc_expand_bits_magic(v, bits)
+
+
+def _morton_np(x, y, z, bb, bits):
+    """Closed form of toMortonCode in numpy float32 (IEEE, no FMA)."""
+    mn = bb[:3].astype(np.float32)
+    dims = (bb[3:] - bb[:3]).astype(np.float32)
+    scale = np.float32(max(max(dims[0], dims[1]), dims[2]))
+    per, mul, hi = (10, 1024.0, 1023.0) if bits == 30 else (21, 2097152.0, 2097151.0)
+    key = np.zeros(len(x), dtype=object)
+    for shift, c, m0 in ((2, x, mn[0]), (1, y, mn[1]), (0, z, mn[2])):
+        v = (np.float32(c) - m0) / scale if np.isscalar(c) else (c.astype(np.float32) - m0) / scale
+        v = (v * np.float32(mul)).astype(np.float32)
+        v = np.where(v < 0, np.float32(0), v)
+        v = np.where(np.float32(hi) < v, np.float32(hi), v).astype(np.float32)
+        q = np.where(np.isnan(v), np.float32(0), v).astype(np.uint64)  # zero-extent box: NaN -> 0
+        for b in range(per):
+            key = key + (((q >> np.uint64(b)) & np.uint64(1)).astype(object) << (3 * b + shift))
+    return np.array(key.tolist(), dtype=np.uint64)
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+def test_morton_closed_form(orc, bits):
+    x, y, z, _ = _rand_bodies(3000, 7)
+    bb = orc.bbox(x, y, z)
+    assert np.array_equal(bb[:3], [x.min(), y.min(), z.min()]) and np.array_equal(bb[3:], [x.max(), y.max(), z.max()])
+    k = orc.morton(x, y, z, bits, bb)
+    assert np.array_equal(k, _morton_np(x, y, z, bb, bits))
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+def test_morton_corners_and_axis_order(orc, bits):
+    """Min corner -> 0, max corner -> all ones (clamped), x is the most
+    significant axis of each triplet (4x + 2y + z)."""
+    x = np.array([0, 1, 0, 0, 1], np.float32)
+    y = np.array([0, 1, 0, 1, 0], np.float32)
+    z = np.array([0, 1, 1, 0, 0], np.float32)
+    bb = orc.bbox(x, y, z)
+    k = orc.morton(x, y, z, bits, bb)
+    per = 10 if bits == 30 else 21
+    assert k[0] == 0
+    assert k[1] == (1 << (3 * per)) - 1
+    zbits = sum(1 << (3 * b) for b in range(per))
+    assert k[2] == zbits and k[3] == zbits << 1 and k[4] == zbits << 2
+
+
+# ------------------------------------------------------------------ sort
+
+@pytest.mark.parametrize("n", [0, 1, 2, 1000, 20000])
+def test_sort_is_stable_key_order(orc, n):
+    r = np.random.default_rng(n)
+    keys = r.integers(0, 50, n).astype(np.uint64)  # many ties
+    perm = orc.sort(keys)
+    assert np.array_equal(perm, np.argsort(keys, kind="stable").astype(np.uint32))
+
+
+# ------------------------------------------------------------------ LBVH
+
+@pytest.mark.parametrize("n,leaf,bits", [(1, 16, 63), (2, 1, 63), (3, 1, 30), (64, 4, 63), (1000, 16, 30),
+                                         (1000, 1, 63), (10000, 16, 30), (10000, 64, 63)])
+def test_lbvh_topology_and_summaries(orc, nbg, n, leaf, bits):
+    b = nbg.plummer(n, seed=n)
+    t = orc.Tree(1, b.x, b.y, b.z, b.m, leaf=leaf, bits=bits)
+    keys, perm = t.order()
+    assert np.all(keys[1:] >= keys[:-1])
+    bb = orc.bbox(b.x, b.y, b.z)
+    assert np.array_equal(keys, _morton_np(b.x, b.y, b.z, bb, bits)[perm])
+    nd = t.nodes()
+    lbvh_check(keys, nd["first"], nd["count"], nd["leaf"], leaf)
+    pos = np.stack([b.x, b.y, b.z], 1)[perm]
+    summaries_check(pos, b.m[perm], nd["first"], nd["count"], nd["com4"], nd["geo6"])
+
+
+def test_lbvh_duplicate_keys_make_oversized_leaves(orc):
+    """Leaf rule 'all keys equal' (LinearBVH.h:114-119): 40 coincident bodies
+    stay in one leaf even with leaf size 4."""
+    x = np.concatenate([np.zeros(40), np.linspace(1, 2, 10)]).astype(np.float32)
+    y = np.zeros(50, np.float32)
+    z = np.zeros(50, np.float32)
+    t = orc.Tree(1, x, y, z, np.ones(50, np.float32), leaf=4, bits=30)
+    nd = t.nodes()
+    keys, _ = t.order()
+    lbvh_check(keys, nd["first"], nd["count"], nd["leaf"], 4)
+    assert nd["count"][nd["leaf"]].max() == 40
+
+
+# ------------------------------------------------------------------ octree
+
+@pytest.mark.parametrize("n,leaf,max_depth", [(1, 16, 32), (9, 1, 32), (1000, 16, 32), (5000, 8, 3), (20000, 64, 32)])
+def test_octree_structure(orc, nbg, n, leaf, max_depth):
+    b = nbg.plummer(n, seed=100 + n)
+    t = orc.Tree(0, b.x, b.y, b.z, b.m, leaf=leaf, max_depth=max_depth)
+    _, perm = t.order()
+    nd = t.nodes()
+    pos = np.stack([b.x, b.y, b.z], 1)
+    octree_check(pos, perm, nd, leaf, max_depth)
+    # root cube: centre min + diag/2, side maxSide * 1.1 (Octree.h:189-190)
+    bb = orc.bbox(b.x, b.y, b.z)
+    diag = (bb[3:] - bb[:3]).astype(np.float32)
+    assert np.array_equal(nd["geo6"][0, :3], (bb[:3] + diag / np.float32(2)).astype(np.float32))
+    assert nd["geo6"][0, 3] == np.float32(diag.max() * np.float32(1.1))
+    # leaf contents are in ascending original index (stable partitions)
+    for f, c, lf in zip(nd["first"], nd["count"], nd["leaf"]):
+        if lf and c > 1:
+            assert np.all(np.diff(perm[f:f + c].astype(np.int64)) > 0)
+    # An empty level-1 octant is still split and summarised from 8 empty
+    # children: 0/0 = NaN COM, which poisons the root's COM (Tree.h:196-216,
+    # CenterOfMassSummary.h:44-55).  The root is never accepted by the MAC.
+    lvl1_empty = np.any((nd["depth"] == 1) & (nd["count"] == 0))
+    assert np.isnan(nd["com4"][0, 0]) == bool(lvl1_empty)
+    # summaries: COM/mass of non-empty nodes
+    populated = (nd["count"] > 0) & ~np.isnan(nd["com4"][:, 0])
+    keep = {k: v[populated] for k, v in nd.items()}
+    pos_sorted = pos[perm]
+    # octree nodes index the leaf order, which is contiguous per subtree
+    summaries_check(pos_sorted, b.m[perm], keep["first"], keep["count"], keep["com4"])
+
+
+# ------------------------------------------------------------------ walk
+
+def _py_walk_lbvh(nd, pos, mass, p, theta2, G, eps2):
+    """Brute-force restatement of activeTree over the LBVH in numpy float32,
+    written from the reference text (activeTree.h:24-79, Gravity.h:199-209,
+    DiagonalOverCenterDistance.h:18-22) independently of the oracle."""
+    f32 = np.float32
+    first, count, leaf, depth, com4, geo = nd["first"], nd["count"], nd["leaf"], nd["depth"], nd["com4"], nd["geo6"]
+    kids = {}
+    stack = []
+    for i, d in enumerate(depth.tolist()):
+        while stack and depth[stack[-1]] >= d:
+            stack.pop()
+        if stack:
+            kids.setdefault(stack[-1], []).append(i)
+        stack.append(i)
+
+    def pp(a, m, q):
+        if a[0] == q[0] and a[1] == q[1] and a[2] == q[2]:
+            return np.zeros(3, f32)
+        R = (q - a).astype(f32)
+        r = f32(np.sqrt(f32(f32(f32(R[0] * R[0]) + f32(R[1] * R[1])) + f32(R[2] * R[2])) + f32(eps2)))
+        r3 = f32(r * f32(r * r))
+        return (((-R) * f32(m)).astype(f32) * f32(G)).astype(f32) / r3
+
+    def walk(i):
+        mn, mx = geo[i, :3], geo[i, 3:]
+        c = ((mx + mn) / f32(2)).astype(f32)
+        dg = (mx - mn).astype(f32)
+        diag2 = f32(f32(f32(dg[0] * dg[0]) + f32(dg[1] * dg[1])) + f32(dg[2] * dg[2]))
+        R = (p - c).astype(f32)
+        d2 = f32(f32(f32(R[0] * R[0]) + f32(R[1] * R[1])) + f32(R[2] * R[2]))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            if f32(diag2 / d2) < theta2:
+                return pp(com4[i, :3], com4[i, 3], p)
+        if leaf[i]:
+            acc = np.zeros(3, f32)
+            for j in range(first[i], first[i] + count[i]):
+                acc = (acc + pp(pos[j], mass[j], p)).astype(f32)
+            return acc
+        r0, r1 = walk(kids[i][0]), walk(kids[i][1])
+        return ((np.zeros(3, f32) + r0).astype(f32) + r1).astype(f32)
+
+    return walk(0)
+
+
+@pytest.mark.parametrize("n,leaf,theta", [(3, 1, 0.5), (64, 4, 0.5), (64, 1, 0.9), (200, 8, 0.3)])
+def test_lbvh_walk_bruteforce_bitexact(orc, nbg, n, leaf, theta):
+    b = nbg.plummer(n, seed=3 + n)
+    t = orc.Tree(1, b.x, b.y, b.z, b.m, theta=theta, leaf=leaf, bits=63, eps2=1e-4)
+    keys, perm = t.order()
+    nd = t.nodes()
+    pos = np.stack([b.x, b.y, b.z], 1)
+    acc = t.accel(b.x, b.y, b.z)
+    theta2 = np.float32(np.float32(theta) * np.float32(theta))
+    for i in range(n):
+        ref = _py_walk_lbvh(nd, pos[perm], b.m[perm], pos[i], theta2, 1.0, 1e-4)
+        assert np.array_equal(acc[i], ref), f"body {i}"
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_walk_theta_zero_is_direct_sum(orc, nbg, solver):
+    """theta = 0 never accepts a node, so every body meets every body: the
+    walk equals the direct sum and counts n pp per body, 0 np."""
+    n = 2000
+    b = nbg.plummer(n, seed=11)
+    t = orc.Tree(solver, b.x, b.y, b.z, b.m, theta=0.0, leaf=16, eps2=1e-4)
+    acc, pp, npc = t.accel(b.x, b.y, b.z, counts=True)
+    assert np.all(pp == n) and np.all(npc == 0)
+    ref = orc.direct_f64(b.x, b.y, b.z, b.m, b.x, b.y, b.z, 1.0, 1e-4)
+    err = np.linalg.norm(acc - ref, axis=1)
+    scale = np.linalg.norm(ref, axis=1)
+    assert np.median(err / scale) < 1e-6
+    assert np.all(err <= 1e-4 * scale + 1e-5 * np.sqrt(np.mean(scale ** 2)))
+
+
+@pytest.mark.parametrize("solver,theta,tol", [(0, 0.5, 3e-3), (1, 0.5, 5e-3), (1, 0.3, 1e-3)])
+def test_walk_force_error_vs_fp64(orc, nbg, solver, theta, tol):
+    """Monopole BH error shrinks with theta; median error at the stated
+    bound against the FP64 direct sum."""
+    n = 20000
+    b = nbg.plummer(n, seed=5)
+    t = orc.Tree(solver, b.x, b.y, b.z, b.m, theta=theta, leaf=16, eps2=1e-4)
+    sel = np.arange(0, n, 10)
+    acc = t.accel(b.x[sel], b.y[sel], b.z[sel])
+    ref = orc.direct_f64(b.x, b.y, b.z, b.m, b.x[sel], b.y[sel], b.z[sel], 1.0, 1e-4)
+    rel = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert np.median(rel) < tol
+
+
+def test_interaction_counts_fall_with_theta(orc, nbg):
+    b = nbg.plummer(5000, seed=9)
+    tot = []
+    for th in (0.2, 0.4, 0.8):
+        t = orc.Tree(1, b.x, b.y, b.z, b.m, theta=th, leaf=16)
+        _, pp, npc = t.accel(b.x, b.y, b.z, counts=True)
+        tot.append(int(pp.sum() + npc.sum()))
+    assert tot[0] > tot[1] > tot[2]
+
+
+# ------------------------------------------------------------------ integrator, direct sum, energy
+
+def test_integrator_is_kick_then_drift(orc, nbg):
+    """Solver::step (Solver.h:103-139): v' = v + a dt, p' = p + v' dt in FP32."""
+    b = nbg.plummer(500, seed=2)
+    dt = np.float32(1e-3)
+    t = orc.Tree(1, b.x, b.y, b.z, b.m, theta=0.5, leaf=8, eps2=1e-4)
+    a = t.accel(b.x, b.y, b.z)
+    (x, y, z, vx, vy, vz), _ = orc.step(1, (b.x, b.y, b.z, b.m, b.vx, b.vy, b.vz), 1, theta=0.5, leaf=8, eps2=1e-4,
+                                        dt=1e-3)
+    ev = (b.vx + a[:, 0] * dt).astype(np.float32)
+    assert np.array_equal(vx, ev)
+    assert np.array_equal(x, (b.x + ev * dt).astype(np.float32))
+
+
+def test_direct_f64_matches_numpy(orc, nbg):
+    b = nbg.plummer(300, seed=4)
+    got = orc.direct_f64(b.x, b.y, b.z, b.m, b.x, b.y, b.z, 1.0, 1e-4)
+    P = np.stack([b.x, b.y, b.z], 1).astype(np.float64)
+    R = P[:, None, :] - P[None, :, :]
+    r2 = (R ** 2).sum(-1) + 1e-4
+    w = b.m.astype(np.float64)[None, :] / (np.sqrt(r2) * r2)
+    np.fill_diagonal(w, 0.0)
+    ref = -(R * w[:, :, None]).sum(1)
+    assert np.allclose(got, ref, rtol=1e-12, atol=0)
+
+
+def test_plummer_energy_closed_form(orc, nbg):
+    """Plummer W = -3 pi G M^2 / (32 a) (untruncated; r < 10a keeps 99.9 %
+    of the mass) and virial 2K / |W| ~ 1."""
+    n = 4000
+    b = nbg.plummer(n, seed=21)
+    W = orc.potential_f64(b.x, b.y, b.z, b.m, 1.0, 0.0)
+    K = 0.5 * float(np.sum(b.m.astype(np.float64) * (b.vx.astype(np.float64) ** 2 + b.vy ** 2 + b.vz ** 2)))
+    assert abs(W / (-3 * math.pi / 32) - 1) < 0.06
+    assert abs(2 * K / abs(W) - 1) < 0.08

@@ -1,0 +1,106 @@
+"""Independent checks of tree structure, shared by the oracle and GPU tests.
+
+These do not restate the build algorithms; they verify the properties that
+define the reference's trees:
+  * LBVH (src/NBody/Simulation/Solvers/Trees/LinearBVH.h:47-76,114-119): the
+    root covers every body; a node is a leaf iff it has <= L bodies or equal
+    first/last keys; an internal node has exactly two children that split
+    its range at the first key with the highest differing bit set.
+  * Octree (Trees/Octree.h:33-61,100-147,185-205): levels 0 and 1 always
+    split; deeper nodes split iff > L bodies and depth < 2 + maxDepth; each
+    body of a child lies on the child's side of every parent centre plane
+    (p[d] < c[d] for the lower half).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def lbvh_check(keys_sorted: np.ndarray, first, count, leaf, L: int):
+    n = len(keys_sorted)
+    nodes = {}
+    for f, c, lf in zip(first.tolist(), count.tolist(), np.asarray(leaf).tolist()):
+        assert (f, c) not in nodes, f"duplicate node range {(f, c)}"
+        nodes[(f, c)] = bool(lf)
+    if n == 0:
+        assert not nodes
+        return 0
+    assert (0, n) in nodes, "root must cover all bodies"
+    seen = 0
+    stack = [(0, n)]
+    while stack:
+        f, c = stack.pop()
+        seen += 1
+        k0, k1 = int(keys_sorted[f]), int(keys_sorted[f + c - 1])
+        should_split = c > L and k0 != k1
+        assert nodes[(f, c)] == (not should_split), f"leaf rule violated at {(f, c)}"
+        if not should_split:
+            continue
+        msb = (k0 ^ k1).bit_length() - 1
+        seg = keys_sorted[f:f + c].astype(object)
+        bits = np.array([(int(k) >> msb) & 1 for k in seg])
+        s = int(np.argmax(bits == 1))
+        assert s > 0 and bits[:s].sum() == 0 and bits[s:].all(), "keys not partitioned by the split bit"
+        left, right = (f, s), (f + s, c - s)
+        assert left in nodes and right in nodes, f"children of {(f, c)} missing"
+        stack += [right, left]
+    assert seen == len(nodes), "nodes not reachable from the root"
+    return seen
+
+
+def summaries_check(pos_sorted: np.ndarray, mass_sorted: np.ndarray, first, count, com4, geo6=None,
+                    rel=2e-5):
+    """COM / mass close to their FP64 definitions; bbox exactly min/max."""
+    for i, (f, c) in enumerate(zip(first.tolist(), count.tolist())):
+        if c == 0:
+            continue
+        p = pos_sorted[f:f + c].astype(np.float64)
+        m = mass_sorted[f:f + c].astype(np.float64)
+        M = m.sum()
+        assert abs(com4[i, 3] - M) <= rel * M + 1e-30, f"mass of node {i}"
+        if M > 0:
+            C = (p * m[:, None]).sum(0) / M
+            ext = max(np.abs(p).max(), 1e-30)
+            assert np.all(np.abs(com4[i, :3] - C) <= rel * ext), f"COM of node {i}"
+        if geo6 is not None:
+            assert np.array_equal(geo6[i, :3], pos_sorted[f:f + c].min(0)), f"bbox min of node {i}"
+            assert np.array_equal(geo6[i, 3:], pos_sorted[f:f + c].max(0)), f"bbox max of node {i}"
+
+
+def preorder_parents(depth: np.ndarray) -> np.ndarray:
+    parent = np.full(len(depth), -1, np.int64)
+    stack = []
+    for i, d in enumerate(depth.tolist()):
+        while stack and depth[stack[-1]] >= d:
+            stack.pop()
+        parent[i] = stack[-1] if stack else -1
+        stack.append(i)
+    return parent
+
+
+def octree_check(pos: np.ndarray, perm: np.ndarray, nodes: dict, L: int, max_depth: int):
+    """pos: original-order positions; perm: leaf body order; nodes: pre-order."""
+    depth = nodes["depth"]
+    parent = preorder_parents(depth)
+    first, count, leaf, geo = nodes["first"], nodes["count"], nodes["leaf"], nodes["geo6"]
+    n = len(pos)
+    assert count[0] == n and depth[0] == 0
+    for i in range(len(depth)):
+        d, c = int(depth[i]), int(count[i])
+        if d <= 1:
+            assert not leaf[i], "levels 0 and 1 are always split"
+        elif c > 0:
+            assert bool(leaf[i]) == (not (c > L and d < 2 + max_depth)), f"split rule at node {i}"
+        if i > 0 and c > 0:
+            pc = geo[parent[i], :3]
+            cc = geo[i, :3]
+            body = pos[perm[first[i]:first[i] + c]]
+            for dim in range(3):
+                lower = cc[dim] < pc[dim]
+                side = body[:, dim] < pc[dim]
+                assert np.all(side == lower), f"node {i}: body on the wrong side of its parent's plane"
+    # leaves partition the bodies
+    lc = count[leaf.astype(bool)].sum()
+    assert lc == n, "leaves must hold every body once"
+    assert sorted(perm.tolist()) == list(range(n))
