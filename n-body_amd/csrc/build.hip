@@ -110,9 +110,9 @@ __global__ __launch_bounds__(256) void k_morton(const float *__restrict__ x, con
     const float m1 = (dx < dy) ? dy : dx;   // std::max(std::max(dx, dy), dz)
     const float scale = (m1 < dz) ? dz : m1;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        float vx = __fdiv_rn(x[i] - mnx, scale);
-        float vy = __fdiv_rn(y[i] - mny, scale);
-        float vz = __fdiv_rn(z[i] - mnz, scale);
+        float vx = ieee_div(x[i] - mnx, scale);
+        float vy = ieee_div(y[i] - mny, scale);
+        float vz = ieee_div(z[i] - mnz, scale);
         K key;
         if constexpr (sizeof(K) == 4) {
             uint32_t qx = quantize(vx, 1024.0f, 1023.0f), qy = quantize(vy, 1024.0f, 1023.0f),
@@ -540,7 +540,7 @@ __global__ __launch_bounds__(256) void k_lbvh_summaries(const float4 *__restrict
         xx = fmax_ref(b.x, xx); xy = fmax_ref(b.y, xy); xz = fmax_ref(b.z, xz);
     }
     float4 mn = make_float4(nx, ny, nz, 0.0f), mx = make_float4(xx, xy, xz, 0.0f);
-    com[s] = make_float4(__fdiv_rn(cx, M), __fdiv_rn(cy, M), __fdiv_rn(cz, M), M);
+    com[s] = make_float4(ieee_div(cx, M), ieee_div(cy, M), ieee_div(cz, M), M);
     bmin[s] = mn;
     bmax[s] = mx;
     mac[s] = mac_record(mn, mx);
@@ -567,7 +567,7 @@ __global__ __launch_bounds__(256) void k_lbvh_summaries(const float4 *__restrict
         px = px + b.x * b.w;
         py = py + b.y * b.w;
         pz = pz + b.z * b.w;
-        com[parent] = make_float4(__fdiv_rn(px, Mp), __fdiv_rn(py, Mp), __fdiv_rn(pz, Mp), Mp);
+        com[parent] = make_float4(ieee_div(px, Mp), ieee_div(py, Mp), ieee_div(pz, Mp), Mp);
         float4 pn = make_float4(fmin_ref(an.x, FLT_MAX), fmin_ref(an.y, FLT_MAX), fmin_ref(an.z, FLT_MAX), 0.0f);
         float4 px4 = make_float4(fmax_ref(ax.x, -FLT_MAX), fmax_ref(ax.y, -FLT_MAX), fmax_ref(ax.z, -FLT_MAX), 0.0f);
         pn = make_float4(fmin_ref(bn.x, pn.x), fmin_ref(bn.y, pn.y), fmin_ref(bn.z, pn.z), 0.0f);

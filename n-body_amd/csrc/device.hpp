@@ -16,6 +16,14 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
+// Correctly rounded binary32 square root and division.  hipcc lowers these
+// to v_sqrt_f32 + a +-1 ulp FMA correction and to the v_div_scale /
+// v_div_fmas / v_div_fixup sequence (HIP's default is correctly rounded fp32
+// divide/sqrt).  HIP's __fsqrt_rn is NOT used: without
+// OCML_BASIC_ROUNDED_OPERATIONS it maps to the approximate native sqrt.
+__device__ __forceinline__ float ieee_sqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ float ieee_div(float a, float b) { return a / b; }
+
 __device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 __device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {

@@ -37,11 +37,11 @@ __device__ __forceinline__ void pp_exact(float bx, float by, float bz, float bm,
     }
     const float Rx = px - bx, Ry = py - by, Rz = pz - bz;
     const float r2 = Rx * Rx + Ry * Ry + Rz * Rz;
-    const float r = __fsqrt_rn(r2 + eps2);
+    const float r = ieee_sqrt(r2 + eps2);
     const float r3 = r * (r * r);
-    vx = __fdiv_rn((-Rx * bm) * G, r3);
-    vy = __fdiv_rn((-Ry * bm) * G, r3);
-    vz = __fdiv_rn((-Rz * bm) * G, r3);
+    vx = ieee_div((-Rx * bm) * G, r3);
+    vy = ieee_div((-Ry * bm) * G, r3);
+    vz = ieee_div((-Rz * bm) * G, r3);
 }
 
 // Same interaction, rsqrt + FMA, accumulated in place.  Coincident bodies
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh(WalkArgs a) {
         if (act) {
             const float dx = px - mc.x, dy = py - mc.y, dz = pz - mc.z;
             const float d2 = dx * dx + dy * dy + dz * dz;
-            pass = __fdiv_rn(mc.w, d2) < theta2;
+            pass = ieee_div(mc.w, d2) < theta2;
         }
         if (pass) {
             ++cnp;
