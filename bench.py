@@ -120,6 +120,11 @@ def main():
     phases = s.phase_times()
     pp, npc = s.counts()
     s.enable_timing(False)
+    # schedule statistics from one extra, untimed force pass
+    s.enable_walk_stats(True)
+    s.update_accelerations()
+    ws = s.walk_stats()
+    s.enable_walk_stats(False)
 
     ms_per_step = 1e3 * elapsed / args.steps
     value = args.n * args.steps / elapsed
@@ -148,6 +153,12 @@ def main():
         "interactions_per_step": inter,
         "interactions_per_s": inter / (ms_per_step * 1e-3),
         "phases_ms": phases,
+        "walk_schedule": {
+            "node_visits_per_wave": ws["node_visits"] / max(1, (args.n + 63) // 64),
+            "lane_utilisation_nodes": ws["active_lanes"] / max(1, 64 * ws["node_visits"]),
+            "leaf_visits_per_wave": ws["leaf_visits"] / max(1, (args.n + 63) // 64),
+            "lane_utilisation_leaves": ws["leaf_lanes"] / max(1, 64 * ws["leaf_visits"]),
+        } if ws["node_visits"] else None,
         "roofline": {
             "bound": "valu",
             "kernel": "k_walk_lbvh",

@@ -125,6 +125,14 @@ int nbg_sys_synchronize(nbg_sys *sys);
  * calls (self pairs included), np = node-particle calls.  64-bit. */
 int nbg_sys_counts(nbg_sys *sys, uint64_t *pp, uint64_t *np);
 
+/* Schedule statistics of the last FAST force pass (no reference counterpart;
+ * diagnostics of the wavefront walk): out4 = {wave-level node visits, active
+ * lanes summed over those visits, wave-level leaf visits, lanes interacting
+ * with a leaf summed over leaf visits}.  Counted only while enabled (it
+ * costs scalar instructions in the walk); zero after an ORDERED pass. */
+int nbg_sys_enable_walk_stats(nbg_sys *sys, int32_t on);
+int nbg_sys_walk_stats(nbg_sys *sys, uint64_t *out4);
+
 /* Per-phase device time (ms) of the last force pass, measured with HIP events
  * on the system's stream when timing is enabled.  Phases, in order:
  * 0 bbox, 1 keys, 2 sort, 3 gather, 4 topology, 5 summaries, 6 walk,

@@ -35,7 +35,8 @@ EXPORTS = (
     "nbg_ctx_create", "nbg_ctx_destroy", "nbg_sys_create", "nbg_sys_destroy",
     "nbg_sys_set_params", "nbg_sys_get_params", "nbg_sys_set_state", "nbg_sys_get_state",
     "nbg_sys_update_accelerations", "nbg_sys_get_accelerations", "nbg_sys_step",
-    "nbg_sys_synchronize", "nbg_sys_counts", "nbg_sys_enable_timing", "nbg_sys_phase_times",
+    "nbg_sys_synchronize", "nbg_sys_counts", "nbg_sys_enable_walk_stats", "nbg_sys_walk_stats",
+    "nbg_sys_enable_timing", "nbg_sys_phase_times",
     "nbg_sys_debug_bbox", "nbg_sys_debug_order", "nbg_sys_debug_tree", "nbg_direct_f64",
     "nbg_sys_potential_f64", "nbg_generate_plummer", "nbg_generate_cube",
 )
@@ -101,6 +102,8 @@ def lib() -> ctypes.CDLL:
             "nbg_sys_step": (ctypes.c_int, [_vp, ctypes.c_int32]),
             "nbg_sys_synchronize": (ctypes.c_int, [_vp]),
             "nbg_sys_counts": (ctypes.c_int, [_vp, _u64p, _u64p]),
+            "nbg_sys_walk_stats": (ctypes.c_int, [_vp, _u64p]),
+            "nbg_sys_enable_walk_stats": (ctypes.c_int, [_vp, ctypes.c_int32]),
             "nbg_sys_enable_timing": (ctypes.c_int, [_vp, ctypes.c_int32]),
             "nbg_sys_phase_times": (ctypes.c_int, [_vp, _f32p, ctypes.c_int32]),
             "nbg_sys_debug_bbox": (ctypes.c_int, [_vp, _f32p]),
@@ -278,6 +281,15 @@ class System:
         pp, np_ = ctypes.c_uint64(), ctypes.c_uint64()
         _check(lib().nbg_sys_counts(self.handle, ctypes.byref(pp), ctypes.byref(np_)))
         return int(pp.value), int(np_.value)
+
+    def enable_walk_stats(self, on: bool = True):
+        _check(lib().nbg_sys_enable_walk_stats(self.handle, int(on)))
+
+    def walk_stats(self) -> dict:
+        out = np.zeros(4, np.uint64)
+        _check(lib().nbg_sys_walk_stats(self.handle, _ptr(out, _u64p)))
+        v = [int(x) for x in out]
+        return {"node_visits": v[0], "active_lanes": v[1], "leaf_visits": v[2], "leaf_lanes": v[3]}
 
     def enable_timing(self, on: bool = True):
         _check(lib().nbg_sys_enable_timing(self.handle, int(on)))

@@ -48,6 +48,7 @@ struct nbg_sys {
     bool have_state = false;
     bool have_forces = false;
     bool timing = false;
+    bool walk_stats = false;  // FAST walk counts its schedule statistics
     // Timing: one set of kPhases+1 events per force pass, recorded on the
     // system stream; folded into per-phase sums when the stream is synchronised.
     std::vector<hipEvent_t> ev;    // kEventSets * (kPhases + 1)
@@ -65,15 +66,20 @@ struct nbg_sys {
     float4 *bodies = nullptr;
     int32_t *kfirst = nullptr, *klast = nullptr, *kgamma = nullptr, *kparent = nullptr;
     uint32_t *ksplit = nullptr, *krank = nullptr;
+    int32_t *leaf_start = nullptr;  // leaf slot by first body (n)
+    uint32_t *glist = nullptr;      // straddling parents queued by the summaries (2n + 64)
     uint32_t *scan_ws = nullptr;
     uint32_t *sort_status = nullptr;
     // tree (capacity 2n nodes)
-    int4 *topo = nullptr;
-    float4 *com = nullptr, *bmin = nullptr, *bmax = nullptr, *mac = nullptr;
+    NodeRec *nodes = nullptr;
+    float4 *bmin = nullptr, *bmax = nullptr;
     uint32_t *flags = nullptr;
     // small
     float *bbox_partials = nullptr, *bbox = nullptr;
-    uint32_t *misc = nullptr;  // [0] bbox counter, [1] err, [2] node count, [4..8) counts (2 x u64)
+    // [1] err, [2] node count, [4..16) counts: pp, np, then the FAST walk's
+    // wave-level node visits, active lanes summed over visits, leaf visits
+    // and open lanes summed over leaf visits (6 x u64).
+    uint32_t *misc = nullptr;
 
     uint32_t *err_word() { return misc + 1; }
     uint32_t *node_count() { return misc + 2; }
@@ -106,7 +112,8 @@ int dalloc(T **p, int64_t count) {
 void free_sys(nbg_sys *s) {
     void *ptrs[] = {s->x, s->y, s->z, s->m, s->vx, s->vy, s->vz, s->ax, s->ay, s->az, s->keys, s->keys_alt,
                     s->idx, s->idx_alt, s->bodies, s->kfirst, s->klast, s->kgamma, s->kparent, s->ksplit,
-                    s->krank, s->scan_ws, s->sort_status, s->topo, s->com, s->bmin, s->bmax, s->mac, s->flags,
+                    s->krank, s->leaf_start, s->glist, s->scan_ws, s->sort_status, s->nodes, s->bmin,
+                    s->bmax, s->flags,
                     s->bbox_partials, s->bbox, s->misc};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -143,9 +150,9 @@ int enqueue_force_pass(nbg_sys *s) {
     const int64_t n = s->n;
     const nbg_params &p = s->prm;
     hipStream_t st = s->stream;
-    launch_fill_u32(s->misc + 4, 0u, 4, st);  // interaction counters
+    launch_fill_u32(s->misc + 4, 0u, 12, st);  // interaction counters
     mark(s, 0);
-    launch_bbox(s->x, s->y, s->z, n, s->bbox_partials, s->misc, s->bbox, st);
+    launch_bbox(s->x, s->y, s->z, n, s->bbox_partials, s->bbox, st);
     mark(s, 1);
     launch_morton(s->x, s->y, s->z, n, s->bbox, p.morton_bits, s->keys, s->idx, st);
     mark(s, 2);
@@ -154,20 +161,19 @@ int enqueue_force_pass(nbg_sys *s) {
     launch_gather(s->x, s->y, s->z, s->m, s->idx, n, s->bodies, st);
     mark(s, 4);
     launch_lbvh_topology(s->keys, p.morton_bits, n, p.leaf_size, s->kfirst, s->klast, s->kgamma, s->ksplit, s->krank,
-                         s->kparent, s->scan_ws, s->topo, s->node_count(), st);
+                         s->kparent, s->scan_ws, s->nodes, s->leaf_start, s->node_count(), st);
     mark(s, 5);
-    launch_lbvh_summaries(s->bodies, n, s->topo, s->node_count(), 2 * n, s->flags, s->com, s->bmin, s->bmax, s->mac,
+    launch_lbvh_summaries(s->bodies, n, s->nodes, s->leaf_start, 2 * n, s->flags, s->glist, s->bmin, s->bmax,
                           s->err_word(), st);
     mark(s, 6);
     WalkArgs wa;
     wa.n = n;
     wa.bodies = s->bodies;
     wa.perm = s->idx;
-    wa.mac = s->mac;
-    wa.com = s->com;
-    wa.topo = s->topo;
+    wa.nodes = s->nodes;
     wa.theta = p.theta;
     wa.theta2 = p.theta * p.theta;
+    mac_threshold(p.theta, &wa.mac_mid, &wa.mac_tie);
     wa.G = p.G;
     wa.eps2 = p.eps2;
     wa.max_nodes = 2 * n;
@@ -176,7 +182,7 @@ int enqueue_force_pass(nbg_sys *s) {
     wa.az = s->az;
     wa.counts = s->counts();
     wa.err = s->err_word();
-    launch_walk_lbvh(wa, p.sum_mode == NBG_SUM_ORDERED, p.morton_bits == 30 ? 32 : 64, st);
+    launch_walk_lbvh(wa, p.sum_mode == NBG_SUM_ORDERED, p.morton_bits == 30 ? 32 : 64, s->walk_stats ? 1 : 0, st);
     mark(s, 7);
     NBG_HIP(hipGetLastError());
     return NBG_OK;
@@ -277,19 +283,20 @@ int nbg_sys_create(nbg_ctx *ctx, const nbg_params *p, int64_t n, nbg_sys **out) 
     A(&s->bodies, n);
     A(&s->kfirst, n); A(&s->klast, n); A(&s->kgamma, n); A(&s->kparent, n);
     A(&s->ksplit, n); A(&s->krank, n);
+    A(&s->leaf_start, n); A(&s->glist, 2 * n + 64);
     A(&s->scan_ws, 64 + (n + kScanTile - 1) / kScanTile + 64);
     A(&s->sort_status, sort_status_words(n, 64));
-    A(&s->topo, 2 * n); A(&s->com, 2 * n); A(&s->bmin, 2 * n); A(&s->bmax, 2 * n); A(&s->mac, 2 * n);
+    A(&s->nodes, 2 * n); A(&s->bmin, 2 * n); A(&s->bmax, 2 * n);
     A(&s->flags, 2 * n);
     A(&s->bbox_partials, 6 * kBBoxBlocks);
     A(&s->bbox, 8);
-    A(&s->misc, 16);
+    A(&s->misc, 32);
     if (r == NBG_OK) {
         hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
         if (e != hipSuccess) r = fail(NBG_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
     if (r == NBG_OK) {
-        hipError_t e = hipMemsetAsync(s->misc, 0, 16 * sizeof(uint32_t), s->stream);
+        hipError_t e = hipMemsetAsync(s->misc, 0, 32 * sizeof(uint32_t), s->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
         if (e != hipSuccess) r = fail(NBG_ERR_HIP, std::string("init: ") + hipGetErrorString(e));
     }
@@ -418,6 +425,23 @@ int nbg_sys_counts(nbg_sys *s, uint64_t *pp, uint64_t *np) {
     return NBG_OK;
 }
 
+int nbg_sys_walk_stats(nbg_sys *s, uint64_t *out4) {
+    if (!s || !out4) return fail(NBG_ERR_ARG, "sys or out is NULL");
+    if (!s->have_forces) return fail(NBG_ERR_STATE, "no force pass has run");
+    DeviceGuard g(s->ctx->device);
+    NBG_HIP(hipStreamSynchronize(s->stream));
+    unsigned long long c[6];
+    NBG_HIP(hipMemcpy(c, s->counts(), sizeof c, hipMemcpyDeviceToHost));
+    for (int i = 0; i < 4; ++i) out4[i] = c[2 + i];
+    return NBG_OK;
+}
+
+int nbg_sys_enable_walk_stats(nbg_sys *s, int32_t on) {
+    if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
+    s->walk_stats = on != 0;
+    return NBG_OK;
+}
+
 int nbg_sys_enable_timing(nbg_sys *s, int32_t on) {
     if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
     DeviceGuard g(s->ctx->device);
@@ -484,20 +508,21 @@ int nbg_sys_debug_tree(nbg_sys *s, int64_t *num_nodes, int64_t *first, int64_t *
     NBG_HIP(hipMemcpy(&nn, s->node_count(), 4, hipMemcpyDeviceToHost));
     *num_nodes = nn;
     if (!first && !count && !leaf && !parent && !com4 && !geo6) return NBG_OK;
-    std::vector<int4> t(nn);
-    std::vector<float4> c(nn), lo(nn), hi(nn);
+    std::vector<NodeRec> nd(nn);
+    std::vector<float4> lo(nn), hi(nn);
     if (nn) {
-        NBG_HIP(hipMemcpy(t.data(), s->topo, nn * sizeof(int4), hipMemcpyDeviceToHost));
-        NBG_HIP(hipMemcpy(c.data(), s->com, nn * sizeof(float4), hipMemcpyDeviceToHost));
+        NBG_HIP(hipMemcpy(nd.data(), s->nodes, nn * sizeof(NodeRec), hipMemcpyDeviceToHost));
         NBG_HIP(hipMemcpy(lo.data(), s->bmin, nn * sizeof(float4), hipMemcpyDeviceToHost));
         NBG_HIP(hipMemcpy(hi.data(), s->bmax, nn * sizeof(float4), hipMemcpyDeviceToHost));
     }
     for (uint32_t i = 0; i < nn; ++i) {
-        if (first) first[i] = t[i].x;
-        if (count) count[i] = t[i].y;
-        if (leaf) leaf[i] = t[i].z < 0;
-        if (parent) parent[i] = t[i].w;
-        if (com4) { com4[4 * i] = c[i].x; com4[4 * i + 1] = c[i].y; com4[4 * i + 2] = c[i].z; com4[4 * i + 3] = c[i].w; }
+        const int4 t = nd[i].topo;
+        const float4 c = nd[i].com;
+        if (first) first[i] = t.x;
+        if (count) count[i] = t.y;
+        if (leaf) leaf[i] = t.z < 0;
+        if (parent) parent[i] = t.w;
+        if (com4) { com4[4 * i] = c.x; com4[4 * i + 1] = c.y; com4[4 * i + 2] = c.z; com4[4 * i + 3] = c.w; }
         if (geo6) {
             float gg[6] = {lo[i].x, lo[i].y, lo[i].z, hi[i].x, hi[i].y, hi[i].z};
             std::memcpy(geo6 + 6 * i, gg, sizeof gg);

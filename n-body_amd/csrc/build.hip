@@ -36,8 +36,7 @@ __device__ __forceinline__ float fmin_ref(float a, float b) { return (b < a) ? b
 __device__ __forceinline__ float fmax_ref(float a, float b) { return (a < b) ? b : a; }
 
 __global__ __launch_bounds__(256) void k_bbox(const float *__restrict__ x, const float *__restrict__ y,
-                                              const float *__restrict__ z, int64_t n, float *partials,
-                                              uint32_t *counter, float *out6) {
+                                              const float *__restrict__ z, int64_t n, float *partials) {
     float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         float v[3] = {x[i], y[i], z[i]};
@@ -52,7 +51,6 @@ __global__ __launch_bounds__(256) void k_bbox(const float *__restrict__ x, const
             mx[d] = fmax_ref(mx[d], __shfl_xor(mx[d], o, kWave));
         }
     __shared__ float s_part[4][6];
-    __shared__ bool s_last;
     const int w = threadIdx.x / kWave;
     if (lane_id() == 0)
         for (int d = 0; d < 3; ++d) { s_part[w][d] = mn[d]; s_part[w][3 + d] = mx[d]; }
@@ -63,30 +61,47 @@ __global__ __launch_bounds__(256) void k_bbox(const float *__restrict__ x, const
         for (int ww = 1; ww < 4; ++ww)
             for (int d = 0; d < 3; ++d) { r[d] = fmin_ref(r[d], s_part[ww][d]); r[3 + d] = fmax_ref(r[3 + d], s_part[ww][3 + d]); }
         for (int d = 0; d < 6; ++d) partials[blockIdx.x * 6 + d] = r[d];
-        __threadfence();
-        uint32_t ticket = atomicAdd(counter, 1u);
-        s_last = (ticket == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (s_last && threadIdx.x == 0) {
-        __threadfence();
-        float r[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
-        for (unsigned b = 0; b < gridDim.x; ++b)
-            for (int d = 0; d < 3; ++d) {
-                r[d] = fmin_ref(r[d], __builtin_nontemporal_load(&partials[b * 6 + d]));
-                r[3 + d] = fmax_ref(r[3 + d], __builtin_nontemporal_load(&partials[b * 6 + 3 + d]));
-            }
-        for (int d = 0; d < 6; ++d) out6[d] = r[d];
-        *counter = 0;
     }
 }
 
-void launch_bbox(const float *x, const float *y, const float *z, int64_t n, float *partials, uint32_t *counter,
-                 float *out6, hipStream_t s) {
+// Second stage: one block folds the per-block partials (the kernel boundary
+// makes them visible; no agent-scope fence, which on gfx950 is an L2
+// write-back + invalidate).
+__global__ __launch_bounds__(256) void k_bbox_final(const float *__restrict__ partials, int blocks, float *out6) {
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int b = threadIdx.x; b < blocks; b += blockDim.x)
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = fmin_ref(mn[d], partials[b * 6 + d]);
+            mx[d] = fmax_ref(mx[d], partials[b * 6 + 3 + d]);
+        }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = fmin_ref(mn[d], __shfl_xor(mn[d], o, kWave));
+            mx[d] = fmax_ref(mx[d], __shfl_xor(mx[d], o, kWave));
+        }
+    __shared__ float s_part[4][6];
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0)
+        for (int d = 0; d < 3; ++d) { s_part[w][d] = mn[d]; s_part[w][3 + d] = mx[d]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r[6];
+        for (int d = 0; d < 6; ++d) r[d] = s_part[0][d];
+        for (int ww = 1; ww < 4; ++ww)
+            for (int d = 0; d < 3; ++d) { r[d] = fmin_ref(r[d], s_part[ww][d]); r[3 + d] = fmax_ref(r[3 + d], s_part[ww][3 + d]); }
+        for (int d = 0; d < 6; ++d) out6[d] = r[d];
+    }
+}
+
+void launch_bbox(const float *x, const float *y, const float *z, int64_t n, float *partials, float *out6,
+                 hipStream_t s) {
     int64_t blocks = (n + 255) / 256;
     if (blocks > kBBoxBlocks) blocks = kBBoxBlocks;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_bbox, dim3((unsigned)blocks), dim3(256), 0, s, x, y, z, n, partials, counter, out6);
+    hipLaunchKernelGGL(k_bbox, dim3((unsigned)blocks), dim3(256), 0, s, x, y, z, n, partials);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, s, partials, (int)blocks, out6);
 }
 
 // ------------------------------------------------------------------ Morton
@@ -436,7 +451,8 @@ template <typename K>
 __global__ __launch_bounds__(256) void k_emit(const K *__restrict__ keys, int n, const int32_t *__restrict__ kfirst,
                                               const int32_t *__restrict__ klast, const int32_t *__restrict__ kgamma,
                                               const uint32_t *__restrict__ ksplit, const uint32_t *__restrict__ krank,
-                                              const int32_t *__restrict__ kparent, int4 *__restrict__ topo) {
+                                              const int32_t *__restrict__ kparent, NodeRec *__restrict__ nodes,
+                                              int32_t *__restrict__ leaf_start) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1 || !ksplit[i]) return;
     const int r = (int)krank[i];
@@ -446,22 +462,26 @@ __global__ __launch_bounds__(256) void k_emit(const K *__restrict__ keys, int n,
         const int pk = kparent[i];
         my_slot = 1 + 2 * (int)krank[pk >> 1] + (pk & 1);
     } else {
-        topo[0] = make_int4(first, last - first + 1, 1 + 2 * r, -1);
+        nodes[0].topo = make_int4(first, last - first + 1, 1 + 2 * r, -1);
     }
     const int kl = gamma, kr = gamma + 1;
     const bool l_int = (gamma != first), r_int = (gamma + 1 != last);
     const int lptr = (l_int && ksplit[kl]) ? 1 + 2 * (int)krank[kl] : -1;
     const int rptr = (r_int && ksplit[kr]) ? 1 + 2 * (int)krank[kr] : -1;
-    topo[1 + 2 * r] = make_int4(first, gamma - first + 1, lptr, my_slot);
-    topo[2 + 2 * r] = make_int4(gamma + 1, last - gamma, rptr, my_slot);
+    nodes[1 + 2 * r].topo = make_int4(first, gamma - first + 1, lptr, my_slot);
+    nodes[2 + 2 * r].topo = make_int4(gamma + 1, last - gamma, rptr, my_slot);
+    if (lptr < 0) leaf_start[first] = 1 + 2 * r;
+    if (rptr < 0) leaf_start[gamma + 1] = 2 + 2 * r;
     (void)keys;
 }
 
 template <typename K>
 __global__ void k_root_fix(int n, const uint32_t *__restrict__ ksplit, const uint32_t *__restrict__ total,
-                           int4 *__restrict__ topo, uint32_t *__restrict__ node_count) {
+                           NodeRec *__restrict__ nodes, int32_t *__restrict__ leaf_start,
+                           uint32_t *__restrict__ node_count) {
     if (n <= 1 || !ksplit[0]) {
-        topo[0] = make_int4(0, n, -1, -1);
+        nodes[0].topo = make_int4(0, n, -1, -1);
+        if (n > 0) leaf_start[0] = 0;
         *node_count = n > 0 ? 1u : 0u;
     } else {
         *node_count = 1u + 2u * *total;
@@ -470,10 +490,11 @@ __global__ void k_root_fix(int n, const uint32_t *__restrict__ ksplit, const uin
 
 void launch_lbvh_topology(const void *keys, int bits, int64_t n64, int leaf_size, int32_t *kfirst, int32_t *klast,
                           int32_t *kgamma, uint32_t *ksplit, uint32_t *krank, int32_t *kparent, uint32_t *scan_ws,
-                          int4 *topo, uint32_t *node_count, hipStream_t s) {
+                          NodeRec *nodes, int32_t *leaf_start, uint32_t *node_count, hipStream_t s) {
     const int n = (int)n64;
     uint32_t *total = scan_ws;
     uint32_t *sums = scan_ws + 64;
+    launch_fill_u32((uint32_t *)leaf_start, 0xFFFFFFFFu, n, s);
     if (n >= 2) {
         const int m = n - 1;
         const unsigned blocks = (unsigned)((m + 255) / 256);
@@ -489,24 +510,32 @@ void launch_lbvh_topology(const void *keys, int bits, int64_t n64, int leaf_size
         hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)nt), dim3(kScanBlock), 0, s, ksplit, (int64_t)m, sums, krank);
         if (bits == 30)
             hipLaunchKernelGGL(k_emit<uint32_t>, dim3(blocks), dim3(256), 0, s, (const uint32_t *)keys, n, kfirst, klast,
-                               kgamma, ksplit, krank, kparent, topo);
+                               kgamma, ksplit, krank, kparent, nodes, leaf_start);
         else
             hipLaunchKernelGGL(k_emit<uint64_t>, dim3(blocks), dim3(256), 0, s, (const uint64_t *)keys, n, kfirst, klast,
-                               kgamma, ksplit, krank, kparent, topo);
+                               kgamma, ksplit, krank, kparent, nodes, leaf_start);
     }
-    hipLaunchKernelGGL(k_root_fix<uint32_t>, dim3(1), dim3(1), 0, s, n, ksplit, total, topo, node_count);
+    hipLaunchKernelGGL(k_root_fix<uint32_t>, dim3(1), dim3(1), 0, s, n, ksplit, total, nodes, leaf_start, node_count);
 }
 
 // ------------------------------------------------------------------ summaries
-// NodeBase::summarize (Trees/NodeBase.h:43-55) bottom-up: one thread per
-// leaf computes CenterOfMassSummary's sequential leaf sums
-// (Summaries/CenterOfMassSummary.h:30-42) and the leaf bounding box
-// (Summaries/BoundingBoxSummary.h:30-57), then climbs; at each parent the
-// second arriving child (atomic counter) combines both children
-// (CenterOfMassSummary.h:44-55, BoundingBoxSummary.h:59-68).  The combine
-// reads only finished children, so the result is independent of arrival
-// order.  The MAC record is (bbox centre (max+min)/2, |max-min|^2)
-// (Physics/BoundingBox.h:47,55).
+// NodeBase::summarize (Trees/NodeBase.h:43-55) bottom-up.  A leaf takes
+// CenterOfMassSummary's sequential sums over its bodies in sort order
+// (Summaries/CenterOfMassSummary.h:30-42) and its bounding box
+// (Summaries/BoundingBoxSummary.h:30-57); an internal node combines its two
+// children (CenterOfMassSummary.h:44-55, BoundingBoxSummary.h:59-68).  The
+// combine reads only finished children, so the result does not depend on
+// which child finishes first.  The MAC record is (bbox centre (max+min)/2,
+// |max-min|^2) (Physics/BoundingBox.h:47,55).
+//
+// Scheduling: each workgroup owns kSummBodies consecutive bodies.  Its
+// threads summarise the leaves starting in that range and climb with
+// per-node arrival counters (the second arrival combines) for as long as
+// the parent's range stays inside the workgroup's range, synchronising with
+// workgroup-scope acquire/release only.  Parents that straddle two ranges
+// are queued and finished by a single workgroup in a second launch.  No
+// agent-scope fence is used: on gfx950 that is an L2 write-back and
+// invalidate across the XCDs.
 __device__ __forceinline__ float4 mac_record(float4 mn, float4 mx) {
     const float cx = (mx.x + mn.x) / 2.0f, cy = (mx.y + mn.y) / 2.0f, cz = (mx.z + mn.z) / 2.0f;
     const float dx = mx.x - mn.x, dy = mx.y - mn.y, dz = mx.z - mn.z;
@@ -514,17 +543,34 @@ __device__ __forceinline__ float4 mac_record(float4 mn, float4 mx) {
     return make_float4(cx, cy, cz, d2);
 }
 
-__global__ __launch_bounds__(256) void k_lbvh_summaries(const float4 *__restrict__ bodies, int64_t n,
-                                                        const int4 *__restrict__ topo,
-                                                        const uint32_t *__restrict__ node_count, uint32_t *flags,
-                                                        float4 *com, float4 *bmin, float4 *bmax, float4 *mac,
-                                                        uint32_t *err) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nn = (int)*node_count;
-    if (s >= nn) return;
-    const int4 t = topo[s];
-    if (t.z >= 0) return;  // internal nodes are finished by the climb
-    if (t.x < 0 || t.y < 0 || (int64_t)t.x + t.y > n) { atomicOr(err, 4u); return; }
+__device__ __forceinline__ uint32_t arrive_wg(uint32_t *flag) {
+    return __hip_atomic_fetch_add(flag, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ void summarize_pair(int p, int c0, NodeRec *nodes, float4 *bmin, float4 *bmax) {
+    const float4 a = nodes[c0].com;
+    const float4 b = nodes[c0 + 1].com;
+    const float4 an = bmin[c0], bn = bmin[c0 + 1];
+    const float4 ax = bmax[c0], bx = bmax[c0 + 1];
+    float Mp = 0.0f;
+    Mp += a.w;
+    float px = 0.0f + a.x * a.w, py = 0.0f + a.y * a.w, pz = 0.0f + a.z * a.w;
+    Mp += b.w;
+    px = px + b.x * b.w;
+    py = py + b.y * b.w;
+    pz = pz + b.z * b.w;
+    nodes[p].com = make_float4(ieee_div(px, Mp), ieee_div(py, Mp), ieee_div(pz, Mp), Mp);
+    float4 pn = make_float4(fmin_ref(an.x, FLT_MAX), fmin_ref(an.y, FLT_MAX), fmin_ref(an.z, FLT_MAX), 0.0f);
+    float4 px4 = make_float4(fmax_ref(ax.x, -FLT_MAX), fmax_ref(ax.y, -FLT_MAX), fmax_ref(ax.z, -FLT_MAX), 0.0f);
+    pn = make_float4(fmin_ref(bn.x, pn.x), fmin_ref(bn.y, pn.y), fmin_ref(bn.z, pn.z), 0.0f);
+    px4 = make_float4(fmax_ref(bx.x, px4.x), fmax_ref(bx.y, px4.y), fmax_ref(bx.z, px4.z), 0.0f);
+    bmin[p] = pn;
+    bmax[p] = px4;
+    nodes[p].mac = mac_record(pn, px4);
+}
+
+__device__ void summarize_leaf(int s, int4 t, const float4 *__restrict__ bodies, NodeRec *nodes, float4 *bmin,
+                               float4 *bmax) {
     float M = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
     float nx = FLT_MAX, ny = FLT_MAX, nz = FLT_MAX, xx = -FLT_MAX, xy = -FLT_MAX, xz = -FLT_MAX;
     for (int i = t.x; i < t.x + t.y; ++i) {
@@ -540,54 +586,72 @@ __global__ __launch_bounds__(256) void k_lbvh_summaries(const float4 *__restrict
         xx = fmax_ref(b.x, xx); xy = fmax_ref(b.y, xy); xz = fmax_ref(b.z, xz);
     }
     float4 mn = make_float4(nx, ny, nz, 0.0f), mx = make_float4(xx, xy, xz, 0.0f);
-    com[s] = make_float4(ieee_div(cx, M), ieee_div(cy, M), ieee_div(cz, M), M);
+    nodes[s].com = make_float4(ieee_div(cx, M), ieee_div(cy, M), ieee_div(cz, M), M);
     bmin[s] = mn;
     bmax[s] = mx;
-    mac[s] = mac_record(mn, mx);
-    int cur = s;
-    int parent = t.w;
-    while (parent >= 0) {
-        if (parent >= nn) { atomicOr(err, 4u); return; }
-        __threadfence();
-        const uint32_t old = atomicAdd(&flags[parent], 1u);
-        if (old == 0) return;
-        __threadfence();
-        // The agent-scope fence above orders these loads after the sibling's
-        // published summary (and invalidates this CU's vector L1).
-        const int c0 = topo[parent].z;
-        if (c0 < 0 || c0 + 1 >= nn) { atomicOr(err, 4u); return; }
-        const float4 a = com[c0];
-        const float4 b = com[c0 + 1];
-        const float4 an = bmin[c0], bn = bmin[c0 + 1];
-        const float4 ax = bmax[c0], bx = bmax[c0 + 1];
-        float Mp = 0.0f;
-        Mp += a.w;
-        float px = 0.0f + a.x * a.w, py = 0.0f + a.y * a.w, pz = 0.0f + a.z * a.w;
-        Mp += b.w;
-        px = px + b.x * b.w;
-        py = py + b.y * b.w;
-        pz = pz + b.z * b.w;
-        com[parent] = make_float4(ieee_div(px, Mp), ieee_div(py, Mp), ieee_div(pz, Mp), Mp);
-        float4 pn = make_float4(fmin_ref(an.x, FLT_MAX), fmin_ref(an.y, FLT_MAX), fmin_ref(an.z, FLT_MAX), 0.0f);
-        float4 px4 = make_float4(fmax_ref(ax.x, -FLT_MAX), fmax_ref(ax.y, -FLT_MAX), fmax_ref(ax.z, -FLT_MAX), 0.0f);
-        pn = make_float4(fmin_ref(bn.x, pn.x), fmin_ref(bn.y, pn.y), fmin_ref(bn.z, pn.z), 0.0f);
-        px4 = make_float4(fmax_ref(bx.x, px4.x), fmax_ref(bx.y, px4.y), fmax_ref(bx.z, px4.z), 0.0f);
-        bmin[parent] = pn;
-        bmax[parent] = px4;
-        mac[parent] = mac_record(pn, px4);
-        cur = parent;
-        parent = topo[cur].w;
+    nodes[s].mac = mac_record(mn, mx);
+}
+
+__global__ __launch_bounds__(256) void k_summ_local(const float4 *__restrict__ bodies, int64_t n,
+                                                    NodeRec *nodes,
+                                                    const int32_t *__restrict__ leaf_start, int64_t max_nodes,
+                                                    uint32_t *flags, uint32_t *glist, float4 *bmin,
+                                                    float4 *bmax, uint32_t *err) {
+    const int64_t lo = (int64_t)blockIdx.x * kSummBodies, hi = lo + kSummBodies;
+    const int64_t end = hi < n ? hi : n;
+    for (int64_t i = lo + threadIdx.x; i < end; i += blockDim.x) {
+        const int s = leaf_start[i];
+        if (s < 0) continue;
+        if (s >= max_nodes) { atomicOr(err, 4u); continue; }
+        const int4 t = nodes[s].topo;
+        if (t.x != (int)i || t.y < 1 || (int64_t)t.x + t.y > n || t.z >= 0) { atomicOr(err, 4u); continue; }
+        summarize_leaf(s, t, bodies, nodes, bmin, bmax);
+        int parent = t.w;
+        while (parent >= 0) {
+            if (parent >= max_nodes) { atomicOr(err, 4u); break; }
+            const int4 tp = nodes[parent].topo;
+            const bool local = tp.x >= lo && (int64_t)tp.x + tp.y <= hi;
+            if (!local) {
+                const uint32_t k = atomicAdd(&glist[0], 1u);
+                if (k < 2 * n) glist[64 + k] = (uint32_t)parent;
+                else atomicOr(err, 4u);
+                break;
+            }
+            if (arrive_wg(&flags[parent]) == 0) break;
+            if (tp.z < 0 || tp.z + 1 >= max_nodes) { atomicOr(err, 4u); break; }
+            summarize_pair(parent, tp.z, nodes, bmin, bmax);
+            parent = tp.w;
+        }
     }
 }
 
-void launch_lbvh_summaries(const float4 *bodies, int64_t n, const int4 *topo, const uint32_t *node_count,
-                           int64_t max_nodes, uint32_t *flags, float4 *com, float4 *bmin, float4 *bmax, float4 *mac,
+__global__ __launch_bounds__(1024) void k_summ_global(NodeRec *nodes, int64_t max_nodes, uint32_t *flags,
+                                                      const uint32_t *glist, float4 *bmin, float4 *bmax,
+                                                      uint32_t *err) {
+    const uint32_t cnt = glist[0];
+    for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
+        int parent = (int)glist[64 + j];
+        while (parent >= 0) {
+            if (parent >= max_nodes) { atomicOr(err, 4u); break; }
+            const int4 tp = nodes[parent].topo;
+            if (arrive_wg(&flags[parent]) == 0) break;
+            if (tp.z < 0 || tp.z + 1 >= max_nodes) { atomicOr(err, 4u); break; }
+            summarize_pair(parent, tp.z, nodes, bmin, bmax);
+            parent = tp.w;
+        }
+    }
+}
+
+void launch_lbvh_summaries(const float4 *bodies, int64_t n, NodeRec *nodes, const int32_t *leaf_start,
+                           int64_t max_nodes, uint32_t *flags, uint32_t *glist, float4 *bmin, float4 *bmax,
                            uint32_t *err, hipStream_t s) {
-    if (max_nodes <= 0) return;
+    if (n <= 0) return;
     launch_fill_u32(flags, 0u, max_nodes, s);
-    const unsigned blocks = (unsigned)((max_nodes + 255) / 256);
-    hipLaunchKernelGGL(k_lbvh_summaries, dim3(blocks), dim3(256), 0, s, bodies, n, topo, node_count, flags, com, bmin,
-                       bmax, mac, err);
+    launch_fill_u32(glist, 0u, 64, s);
+    const unsigned blocks = (unsigned)((n + kSummBodies - 1) / kSummBodies);
+    hipLaunchKernelGGL(k_summ_local, dim3(blocks), dim3(256), 0, s, bodies, n, nodes, leaf_start, max_nodes, flags,
+                       glist, bmin, bmax, err);
+    hipLaunchKernelGGL(k_summ_global, dim3(1), dim3(1024), 0, s, nodes, max_nodes, flags, glist, bmin, bmax, err);
 }
 
 }  // namespace nbg

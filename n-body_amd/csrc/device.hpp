@@ -24,6 +24,29 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 __device__ __forceinline__ float ieee_sqrt(float x) { return __builtin_sqrtf(x); }
 __device__ __forceinline__ float ieee_div(float a, float b) { return a / b; }
 
+// Exact form of the reference's LBVH opening test
+//   RN(diag2 / d2) < T,   T = RN(theta * theta)   (DiagonalOverCenterDistance.h:18-22)
+// without a division.  RN is monotone, so RN(q) < T iff q < mid, where mid is
+// the midpoint between T and its predecessor, or q == mid and round-to-even
+// picks the predecessor (T has an odd significand).  With q = diag2 / d2 and
+// d2 > 0 this is diag2 < mid * d2 (or ==), and mid * d2 (25 x 24 significant
+// bits) is exact in FP64, so the comparison is exact.  d2 == 0 gives q = inf
+// or NaN, never accepted.  mid and tie come from mac_threshold() on the host.
+__device__ __forceinline__ bool mac_lbvh_exact(float diag2, float d2, double mid, bool tie) {
+    const double l = (double)diag2, r = mid * (double)d2;
+    return tie ? ((l < r) || (l == r && d2 > 0.0f && l < __builtin_huge_val())) : (l < r);
+}
+
+// Workgroup -> target-range mapping that keeps each XCD on one contiguous
+// eighth of the Morton order (workgroups are dispatched round-robin over the
+// 8 XCDs, and each XCD has its own L2).  A bijection on [0, grid).
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ int64_t xcd_block(unsigned b, unsigned grid) {
+    const unsigned per = grid / kXcds, rem = grid % kXcds;
+    const unsigned x = b % kXcds, k = b / kXcds;
+    return x < rem ? (int64_t)x * (per + 1) + k : (int64_t)rem * (per + 1) + (int64_t)(x - rem) * per + k;
+}
+
 __device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 __device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {

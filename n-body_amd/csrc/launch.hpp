@@ -31,13 +31,27 @@ constexpr int64_t kMaxBodies = (int64_t)1 << 29;
 // Per-wave traversal stack entries.
 constexpr int kStackLBVH = 192;
 
+
+
+// One tree node as the walk reads it (48 bytes; the two children of a node
+// are adjacent, so a sibling pair is 96 contiguous bytes):
+//   mac  = LBVH: bbox centre xyz, squared bbox diagonal;
+//   com  = centre of mass xyz, total mass;
+//   topo = first body (sort order), body count, first child (-1 for a
+//          leaf; the second child is first child + 1), parent (-1 root).
+struct NodeRec {
+    float4 mac;
+    float4 com;
+    int4 topo;
+};
+
 void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s);
 
 // outerBoundingBox: writes min xyz, max xyz to out6.  partials holds
-// 6 * kBBoxBlocks floats; counter is one zeroed word (left zeroed on exit).
+// 6 * kBBoxBlocks floats.
 constexpr int kBBoxBlocks = 512;
-void launch_bbox(const float *x, const float *y, const float *z, int64_t n, float *partials,
-                 uint32_t *counter, float *out6, hipStream_t s);
+void launch_bbox(const float *x, const float *y, const float *z, int64_t n, float *partials, float *out6,
+                 hipStream_t s);
 
 // Morton keys (bits = 30 -> u32 keys, 63 -> u64 keys) and iota payload.
 void launch_morton(const float *x, const float *y, const float *z, int64_t n, const float *bbox6, int bits,
@@ -59,24 +73,41 @@ void launch_gather(const float *x, const float *y, const float *z, const float *
 // node_count receives the number of nodes.
 void launch_lbvh_topology(const void *keys, int bits, int64_t n, int leaf_size, int32_t *kfirst,
                           int32_t *klast, int32_t *kgamma, uint32_t *ksplit, uint32_t *krank,
-                          int32_t *kparent, uint32_t *scan_ws, int4 *topo, uint32_t *node_count,
-                          hipStream_t s);
+                          int32_t *kparent, uint32_t *scan_ws, NodeRec *nodes, int32_t *leaf_start,
+                          uint32_t *node_count, hipStream_t s);
 
 // Bottom-up LBVH summaries with atomic arrival flags: COM, mass, bbox and the
-// MAC record (bbox centre, squared diagonal).  max_nodes bounds the launch.
-void launch_lbvh_summaries(const float4 *bodies, int64_t n, const int4 *topo, const uint32_t *node_count,
-                           int64_t max_nodes, uint32_t *flags, float4 *com, float4 *bmin, float4 *bmax,
-                           float4 *mac, uint32_t *err, hipStream_t s);
+// MAC record (bbox centre, squared diagonal).  leaf_start[i] holds the leaf
+// slot whose first body is i (else -1; written by the topology pass);
+// flags holds max_nodes words; glist holds 2n + 64 words.
+constexpr int kSummBodies = 4096;   // bodies per workgroup in the local pass
+void launch_lbvh_summaries(const float4 *bodies, int64_t n, NodeRec *nodes, const int32_t *leaf_start,
+                           int64_t max_nodes, uint32_t *flags, uint32_t *glist, float4 *bmin, float4 *bmax,
+                           uint32_t *err, hipStream_t s);
+
+// Host side of mac_lbvh_exact: the FP64 midpoint below T = RN(theta*theta)
+// and whether a tie rounds down (T has an odd significand).
+inline void mac_threshold(float theta, double *mid, int *tie) {
+    const float T = theta * theta;
+    if (!(T > 0.0f)) { *mid = -1.0; *tie = 0; return; }
+    uint32_t bits;
+    __builtin_memcpy(&bits, &T, 4);
+    const uint32_t pbits = bits - 1;  // predecessor of a positive finite float
+    float P;
+    __builtin_memcpy(&P, &pbits, 4);
+    *mid = ((double)P + (double)T) * 0.5;
+    *tie = (int)(bits & 1u);
+}
 
 struct WalkArgs {
     int64_t n;
     const float4 *bodies;   // sorted (x, y, z, m)
     const uint32_t *perm;   // sorted position -> original index
-    const float4 *mac;      // per node MAC record
-    const float4 *com;      // per node (com, mass)
-    const int4 *topo;       // per node (first, count, child, parent)
+    const NodeRec *nodes;   // tree nodes
     float theta;            // octree MAC
     float theta2;           // LBVH MAC: theta * theta (float)
+    double mac_mid;         // LBVH MAC threshold (mac_threshold)
+    int mac_tie;
     float G, eps2;
     int64_t max_nodes;      // node array capacity (index guard)
     float *ax, *ay, *az;    // original order
@@ -85,8 +116,9 @@ struct WalkArgs {
 };
 
 // Barnes-Hut traversal over the LBVH.  ordered != 0 selects the
-// bit-exact reference arithmetic.
-void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, hipStream_t s);
+// bit-exact reference arithmetic; diag != 0 makes the FAST walk also count
+// its schedule statistics (counts[2..5]).
+void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, int diag, hipStream_t s);
 
 // v = v + a*dt; p = p + v*dt (original order).
 void launch_update(float *x, float *y, float *z, float *vx, float *vy, float *vz, const float *ax,

@@ -1,23 +1,29 @@
 // Barnes-Hut traversal for gfx950.
 //
 // One wavefront walks the tree for 64 spatially adjacent targets (consecutive
-// in Morton order).  The wave keeps one DFS stack in LDS whose entries carry
-// a 64-bit lane mask: at each node every lane in the mask evaluates its own
-// opening criterion, lanes that accept the node take the monopole
-// interaction, and the lanes that reject it (a ballot) are the mask pushed
-// with the children.  Each lane therefore visits exactly the nodes the
-// reference's per-body recursion visits, in the same order
-// (Descent/activeTree.h:24-79).  Leaf bodies are loaded once per leaf, one
-// per lane (coalesced), and broadcast to the lanes with v_readlane.
+// in Morton order).  At every node, each lane in the wave's active mask
+// evaluates its own opening criterion; lanes that accept the node take its
+// monopole, lanes that reject a leaf interact with its bodies, and lanes that
+// reject an internal node (a ballot) form the mask with which its children
+// are visited.  Each lane therefore meets exactly the nodes and bodies the
+// reference's per-body recursion meets (Descent/activeTree.h:24-79); only the
+// schedule is shared.  Node records and leaf bodies are wave-uniform and are
+// read with scalar loads.
 //
-// ORDERED arithmetic reproduces the reference bit for bit: IEEE sqrt and
-// division in Gravity::particleParticle's order (Physics/Rules/Gravity.h:199-209),
-// leaf sums sequential from zero (activeTree.h:44-52) and child results
-// bracketed as std::transform_reduce does for two children, (0 + r0) + r1
-// (activeTree.h:66-76).  Each lane's pending partial sums live in LDS, one
-// slot per tree depth (the stack depth is wave-uniform).  FAST arithmetic
-// keeps the same opening decisions (the MAC is still evaluated exactly) and
-// order, but accumulates with rsqrt and FMA into one register triple.
+// ORDERED (bit-exact): depth-first in child order with completion markers on
+// an LDS stack, IEEE sqrt/division in Gravity::particleParticle's order
+// (Physics/Rules/Gravity.h:199-209), leaf sums sequential from zero
+// (activeTree.h:44-52) and child results bracketed as libstdc++'s
+// std::transform_reduce does for two children, (0 + r0) + r1
+// (activeTree.h:66-76).  Each lane's pending partial sum lives in LDS, one
+// slot per tree depth (the depth is wave-uniform).
+//
+// FAST: the same opening decisions (the MAC is still exact) and so the same
+// interaction set, evaluated two siblings per step.  All control flow is on
+// wave-uniform ballots; the lanes that take an interaction are selected by
+// turning the ballot back into an execution mask (inverse ballot), so no
+// per-lane predicate is computed in vector registers.  Interactions are
+// accumulated with rsqrt and FMA into one register triple in traversal order.
 
 #include "device.hpp"
 #include "launch.hpp"
@@ -25,7 +31,8 @@
 namespace nbg {
 
 constexpr int kMetaComplete = 0x10000;
-// Upper bound on stack pops per wave; a valid tree needs at most ~3 per node.
+constexpr int kStackFast = 96;
+// Upper bound on loop iterations per wave; a valid tree needs at most ~3 per node.
 constexpr uint32_t kMaxWalkIters = 1u << 28;
 
 // Gravity::particleParticle, exact reference arithmetic.
@@ -44,44 +51,43 @@ __device__ __forceinline__ void pp_exact(float bx, float by, float bz, float bm,
     vz = ieee_div((-Rz * bm) * G, r3);
 }
 
-// Same interaction, rsqrt + FMA, accumulated in place.  Coincident bodies
-// contribute exactly zero when eps2 > 0 (d = 0); the guard covers eps2 = 0.
-__device__ __forceinline__ void pp_fast(float bx, float by, float bz, float bmG, float px, float py, float pz,
-                                        float eps2, float &rx, float &ry, float &rz) {
-    const float dx = bx - px, dy = by - py, dz = bz - pz;
-    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
-    float inv = __builtin_amdgcn_rsqf(r2);
-    inv = r2 > 0.0f ? inv : 0.0f;
-    const float s = bmG * (inv * inv * inv);
-    rx = __builtin_fmaf(dx, s, rx);
-    ry = __builtin_fmaf(dy, s, ry);
-    rz = __builtin_fmaf(dz, s, rz);
+// DiagonalOverCenterDistance (DiagonalOverCenterDistance.h:18-22) for one
+// lane: squared distance to the bbox centre in glm's order, then the exact
+// comparison of mac_lbvh_exact with the tie rule fixed at compile time.
+template <bool TIE>
+__device__ __forceinline__ bool lbvh_accepts(float4 mc, float px, float py, float pz, double mid) {
+    const float dx = px - mc.x, dy = py - mc.y, dz = pz - mc.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const double l = (double)mc.w, r = mid * (double)d2;
+    if (TIE) return l <= r && r > 0.0 && l < __builtin_huge_val();
+    return l < r;
 }
 
-template <bool ORDERED, int MAXD, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh(WalkArgs a) {
-    __shared__ int s_node[WAVES][kStackLBVH];
-    __shared__ int s_meta[WAVES][kStackLBVH];
-    __shared__ uint64_t s_mask[WAVES][kStackLBVH];
-    __shared__ float s_save[ORDERED ? WAVES * MAXD * 3 * kWave : 1];
+// ---------------------------------------------------------------- ORDERED
+template <int MAXD>
+__global__ __launch_bounds__(64) void k_walk_lbvh_ordered(WalkArgs a) {
+    __shared__ int s_node[kStackLBVH];
+    __shared__ int s_meta[kStackLBVH];
+    __shared__ uint64_t s_mask[kStackLBVH];
+    __shared__ float s_save[MAXD * 3 * kWave];
 
-    const int lane = lane_id(), wv = threadIdx.x / kWave;
-    const int64_t tgt = ((int64_t)blockIdx.x * WAVES + wv) * kWave + lane;
+    const float4 *__restrict__ bodies = a.bodies;
+    const NodeRec *__restrict__ nodes = a.nodes;
+    const int lane = lane_id();
+    const int64_t tgt = xcd_block(blockIdx.x, gridDim.x) * kWave + lane;
     const bool valid = tgt < a.n;
     const uint64_t vmask = __ballot(valid);
     if (vmask == 0) return;
 
-    const float4 me = valid ? a.bodies[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 me = valid ? bodies[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const float px = me.x, py = me.y, pz = me.z;
     float rx = 0.0f, ry = 0.0f, rz = 0.0f;
     uint32_t cpp = 0, cnp = 0;
-    int *stn = s_node[wv];
-    int *stm = s_meta[wv];
-    uint64_t *stk = s_mask[wv];
-    float *sv = ORDERED ? &s_save[wv * MAXD * 3 * kWave] : nullptr;
-    const float G = a.G, eps2 = a.eps2, theta2 = a.theta2;
+    const float G = a.G, eps2 = a.eps2;
+    const double mid = a.mac_mid;
+    const bool tie = a.mac_tie != 0;
 
-    if (lane == 0) { stn[0] = 0; stm[0] = 0; stk[0] = vmask; }
+    if (lane == 0) { s_node[0] = 0; s_meta[0] = 0; s_mask[0] = vmask; }
     int sp = 1;
     bool fail = false;
     uint32_t iters = 0;
@@ -89,88 +95,61 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh(WalkArgs a) {
         if (++iters > kMaxWalkIters) { fail = true; break; }
         --sp;
         __builtin_amdgcn_wave_barrier();
-        const int node = uniform_i32(stn[sp]);
-        const int meta = uniform_i32(stm[sp]);
-        const uint64_t mask = uniform_u64(stk[sp]);
+        const int node = uniform_i32(s_node[sp]);
+        const int meta = uniform_i32(s_meta[sp]);
+        const uint64_t mask = uniform_u64(s_mask[sp]);
         const bool act = (mask >> lane) & 1ull;
         const int depth = meta & 0xFFFF;
-        if (ORDERED && (meta & kMetaComplete)) {
+        if (meta & kMetaComplete) {
             if (act) {
                 const float vx = rx, vy = ry, vz = rz;
-                rx = sv[(depth * 3 + 0) * kWave + lane] + vx;
-                ry = sv[(depth * 3 + 1) * kWave + lane] + vy;
-                rz = sv[(depth * 3 + 2) * kWave + lane] + vz;
+                rx = s_save[(depth * 3 + 0) * kWave + lane] + vx;
+                ry = s_save[(depth * 3 + 1) * kWave + lane] + vy;
+                rz = s_save[(depth * 3 + 2) * kWave + lane] + vz;
             }
             continue;
         }
         if (node < 0 || node >= a.max_nodes) { fail = true; break; }
-        const float4 mc = a.mac[node];
-        const float4 cm = a.com[node];
-        const int4 tp = a.topo[node];
-        // DiagonalOverCenterDistance (DiagonalOverCenterDistance.h:18-22).
-        bool pass = false;
-        if (act) {
-            const float dx = px - mc.x, dy = py - mc.y, dz = pz - mc.z;
-            const float d2 = dx * dx + dy * dy + dz * dz;
-            pass = ieee_div(mc.w, d2) < theta2;
-        }
+        const NodeRec nd = nodes[node];
+        const bool pass = act && (tie ? lbvh_accepts<true>(nd.mac, px, py, pz, mid)
+                                      : lbvh_accepts<false>(nd.mac, px, py, pz, mid));
         if (pass) {
             ++cnp;
-            if (ORDERED) {
-                float vx, vy, vz;
-                pp_exact(cm.x, cm.y, cm.z, cm.w, px, py, pz, G, eps2, vx, vy, vz);
-                rx = rx + vx; ry = ry + vy; rz = rz + vz;
-            } else {
-                pp_fast(cm.x, cm.y, cm.z, cm.w * G, px, py, pz, eps2, rx, ry, rz);
-            }
+            float vx, vy, vz;
+            pp_exact(nd.com.x, nd.com.y, nd.com.z, nd.com.w, px, py, pz, G, eps2, vx, vy, vz);
+            rx = rx + vx; ry = ry + vy; rz = rz + vz;
         }
         const uint64_t open = mask & ~__ballot(pass);
         if (open == 0) continue;
         const bool op = (open >> lane) & 1ull;
-        if (tp.z < 0) {
-            // Leaf: every rejecting lane interacts with every body in order.
-            const int first = tp.x, cnt = tp.y;
+        if (nd.topo.z < 0) {
+            const int first = nd.topo.x, cnt = nd.topo.y;
             if (first < 0 || cnt < 0 || (int64_t)first + cnt > a.n) { fail = true; break; }
-            float lx = 0.0f, ly = 0.0f, lz = 0.0f;
-            for (int base = 0; base < cnt; base += kWave) {
-                const int k = (cnt - base) < kWave ? (cnt - base) : kWave;
-                float4 b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (lane < k) b = a.bodies[first + base + lane];
-                for (int j = 0; j < k; ++j) {
-                    const float bx = readlane_f32(b.x, j), by = readlane_f32(b.y, j);
-                    const float bz = readlane_f32(b.z, j), bm = readlane_f32(b.w, j);
-                    if (op) {
-                        if (ORDERED) {
-                            float vx, vy, vz;
-                            pp_exact(bx, by, bz, bm, px, py, pz, G, eps2, vx, vy, vz);
-                            lx = lx + vx; ly = ly + vy; lz = lz + vz;
-                        } else {
-                            pp_fast(bx, by, bz, bm * G, px, py, pz, eps2, rx, ry, rz);
-                        }
-                    }
-                }
-            }
             if (op) {
+                float lx = 0.0f, ly = 0.0f, lz = 0.0f;
+                for (int j = 0; j < cnt; ++j) {
+                    const float4 b = bodies[first + j];
+                    float vx, vy, vz;
+                    pp_exact(b.x, b.y, b.z, b.w, px, py, pz, G, eps2, vx, vy, vz);
+                    lx = lx + vx; ly = ly + vy; lz = lz + vz;
+                }
                 cpp += (uint32_t)cnt;
-                if (ORDERED) { rx = rx + lx; ry = ry + ly; rz = rz + lz; }
+                rx = rx + lx; ry = ry + ly; rz = rz + lz;
             }
         } else {
-            if (sp + 3 > kStackLBVH || (ORDERED && depth >= MAXD)) { fail = true; break; }
-            if (ORDERED) {
-                if (op) {
-                    sv[(depth * 3 + 0) * kWave + lane] = rx;
-                    sv[(depth * 3 + 1) * kWave + lane] = ry;
-                    sv[(depth * 3 + 2) * kWave + lane] = rz;
-                    rx = 0.0f; ry = 0.0f; rz = 0.0f;
-                }
-                if (lane == 0) { stn[sp] = node; stm[sp] = depth | kMetaComplete; stk[sp] = open; }
-                ++sp;
+            if (sp + 3 > kStackLBVH || depth >= MAXD) { fail = true; break; }
+            if (op) {
+                s_save[(depth * 3 + 0) * kWave + lane] = rx;
+                s_save[(depth * 3 + 1) * kWave + lane] = ry;
+                s_save[(depth * 3 + 2) * kWave + lane] = rz;
+                rx = 0.0f; ry = 0.0f; rz = 0.0f;
             }
             if (lane == 0) {
-                stn[sp] = tp.z + 1; stm[sp] = depth + 1; stk[sp] = open;
-                stn[sp + 1] = tp.z; stm[sp + 1] = depth + 1; stk[sp + 1] = open;
+                s_node[sp] = node; s_meta[sp] = depth | kMetaComplete; s_mask[sp] = open;
+                s_node[sp + 1] = nd.topo.z + 1; s_meta[sp + 1] = depth + 1; s_mask[sp + 1] = open;
+                s_node[sp + 2] = nd.topo.z; s_meta[sp + 2] = depth + 1; s_mask[sp + 2] = open;
             }
-            sp += 2;
+            sp += 3;
         }
     }
     if (fail && lane == 0) atomicOr(a.err, 2u);
@@ -185,18 +164,180 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh(WalkArgs a) {
     }
 }
 
-void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, hipStream_t s) {
+// ---------------------------------------------------------------- FAST
+struct FastWave {
+    float px, py, pz;          // this lane's target
+    float rx, ry, rz;          // this lane's acceleration
+    float G;                   // per lane, so G * (scalar mass) is one VALU op
+    uint64_t pp, np;           // wave totals (interaction counts)
+    uint64_t visits, lanes, leaf_visits, leaf_lanes;  // wave-uniform diagnostics
+};
+
+// Monopole (or body) interaction with rsqrt + FMA: s = m * (G * inv^3).
+// eps2 > 0 (SOFT) makes r2 > 0 for every pair, coincident ones included
+// (their d = 0 contributes exactly zero); otherwise r2 = 0 is guarded.
+template <bool SOFT>
+__device__ __forceinline__ void interact(float bx, float by, float bz, float bm, float eps2, FastWave &W) {
+    const float dx = bx - W.px, dy = by - W.py, dz = bz - W.pz;
+    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+    float inv = __builtin_amdgcn_rsqf(r2);
+    if (!SOFT) inv = r2 > 0.0f ? inv : 0.0f;
+    const float s = bm * ((inv * inv) * (inv * W.G));
+    W.rx = __builtin_fmaf(dx, s, W.rx);
+    W.ry = __builtin_fmaf(dy, s, W.ry);
+    W.rz = __builtin_fmaf(dz, s, W.rz);
+}
+
+// The lanes in `take` interact with the `cnt` bodies starting at `first`.
+template <bool SOFT>
+__device__ __forceinline__ void leaf_bodies(const float4 *__restrict__ bodies, int first, int cnt, uint64_t take,
+                                            float eps2, FastWave &W) {
+    if (__builtin_amdgcn_inverse_ballot_w64(take)) {
+        int j = 0;
+        for (; j + 4 <= cnt; j += 4) {
+            const float4 b0 = bodies[first + j], b1 = bodies[first + j + 1];
+            const float4 b2 = bodies[first + j + 2], b3 = bodies[first + j + 3];
+            interact<SOFT>(b0.x, b0.y, b0.z, b0.w, eps2, W);
+            interact<SOFT>(b1.x, b1.y, b1.z, b1.w, eps2, W);
+            interact<SOFT>(b2.x, b2.y, b2.z, b2.w, eps2, W);
+            interact<SOFT>(b3.x, b3.y, b3.z, b3.w, eps2, W);
+        }
+        for (; j < cnt; ++j) {
+            const float4 b = bodies[first + j];
+            interact<SOFT>(b.x, b.y, b.z, b.w, eps2, W);
+        }
+    }
+}
+
+// Visits one node for the lanes in `mask` (wave-uniform); returns the lanes
+// that open it when it is internal, 0 when it is a leaf or all accept.
+template <bool TIE, bool SOFT, bool DIAG>
+__device__ __forceinline__ uint64_t visit(const NodeRec &nd, uint64_t mask, const float4 *__restrict__ bodies,
+                                          int64_t n, float eps2, double mid, FastWave &W, bool &fail) {
+    const uint64_t acc = __ballot(lbvh_accepts<TIE>(nd.mac, W.px, W.py, W.pz, mid)) & mask;
+    if (DIAG) { W.visits += 1; W.lanes += __popcll(mask); }
+    if (acc) {
+        W.np += __popcll(acc);
+        if (__builtin_amdgcn_inverse_ballot_w64(acc)) interact<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, eps2, W);
+    }
+    const uint64_t open = mask & ~acc;
+    if (open == 0 || nd.topo.z >= 0) return open;
+    const int first = nd.topo.x, cnt = nd.topo.y;
+    if (first < 0 || cnt < 0 || (int64_t)first + cnt > n) { fail = true; return 0; }
+    if (DIAG) { W.leaf_visits += 1; W.leaf_lanes += __popcll(open); }
+    W.pp += (uint64_t)__popcll(open) * (uint64_t)cnt;
+    leaf_bodies<SOFT>(bodies, first, cnt, open, eps2, W);
+    return 0;
+}
+
+template <int WAVES, bool TIE, bool SOFT, bool DIAG>
+__global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
+    __shared__ int s_node[WAVES][kStackFast];
+    __shared__ uint64_t s_mask[WAVES][kStackFast];
+
+    const float4 *__restrict__ bodies = a.bodies;
+    const NodeRec *__restrict__ nodes = a.nodes;
+    const int lane = lane_id(), wv = threadIdx.x / kWave;
+    const int64_t tgt = (xcd_block(blockIdx.x, gridDim.x) * WAVES + wv) * kWave + lane;
+    const bool valid = tgt < a.n;
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0) return;
+
+    FastWave W;
+    const float4 me = valid ? bodies[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    W.px = me.x; W.py = me.y; W.pz = me.z;
+    W.rx = 0.0f; W.ry = 0.0f; W.rz = 0.0f;
+    W.pp = 0; W.np = 0;
+    W.visits = 0; W.lanes = 0; W.leaf_visits = 0; W.leaf_lanes = 0;
+    // A VGPR copy of G (non-volatile asm: no memory effects, so the node and
+    // body loads below stay scalar).
+    asm("v_mov_b32 %0, %1" : "=v"(W.G) : "s"(a.G));
+    const float eps2 = a.eps2;
+    const double mid = a.mac_mid;
+    const int64_t n = a.n;
+    const uint32_t last_pair = (uint32_t)(a.max_nodes - 1);
+    bool fail = false;
+
+    // The root is visited on its own; afterwards every step visits the two
+    // children (c, c + 1) of a node that the lanes in `mask` opened.
+    const NodeRec root = nodes[0];
+    uint64_t mask = visit<TIE, SOFT, DIAG>(root, vmask, bodies, n, eps2, mid, W, fail);
+    int c = root.topo.z;
+    int sp = 0;
+    uint32_t iters = 0;
+    while (true) {
+        if (mask == 0) {
+            if (sp == 0) break;
+            --sp;
+            __builtin_amdgcn_wave_barrier();
+            c = uniform_i32(s_node[wv][sp]);
+            mask = uniform_u64(s_mask[wv][sp]);
+        }
+        c = uniform_i32(c);  // keeps the node loads scalar
+        if (++iters > kMaxWalkIters || (uint32_t)c >= last_pair) { fail = true; break; }
+        const NodeRec r0 = nodes[c], r1 = nodes[c + 1];
+        const uint64_t o0 = visit<TIE, SOFT, DIAG>(r0, mask, bodies, n, eps2, mid, W, fail);
+        const uint64_t o1 = visit<TIE, SOFT, DIAG>(r1, mask, bodies, n, eps2, mid, W, fail);
+        if (fail) break;
+        if (o0 != 0) {
+            if (o1 != 0) {
+                if (sp >= kStackFast) { fail = true; break; }
+                if (lane == 0) { s_node[wv][sp] = r1.topo.z; s_mask[wv][sp] = o1; }
+                __builtin_amdgcn_wave_barrier();
+                ++sp;
+            }
+            c = r0.topo.z;
+            mask = o0;
+        } else if (o1 != 0) {
+            c = r1.topo.z;
+            mask = o1;
+        } else {
+            mask = 0;
+        }
+    }
+    if (fail && lane == 0) atomicOr(a.err, 2u);
+    if (valid) {
+        const uint32_t o = a.perm[tgt];
+        if ((int64_t)o < n) { a.ax[o] = W.rx; a.ay[o] = W.ry; a.az[o] = W.rz; }
+    }
+    if (lane == 0) {
+        atomicAdd(&a.counts[0], (unsigned long long)W.pp);
+        atomicAdd(&a.counts[1], (unsigned long long)W.np);
+        if (DIAG) {
+            atomicAdd(&a.counts[2], (unsigned long long)W.visits);
+            atomicAdd(&a.counts[3], (unsigned long long)W.lanes);
+            atomicAdd(&a.counts[4], (unsigned long long)W.leaf_visits);
+            atomicAdd(&a.counts[5], (unsigned long long)W.leaf_lanes);
+        }
+    }
+}
+
+template <bool DIAG>
+static void launch_fast(const WalkArgs &a, hipStream_t s) {
+    constexpr int W = 4;
+    const unsigned blocks = (unsigned)((a.n + 64 * W - 1) / (64 * W));
+    const bool soft = a.eps2 > 0.0f;
+    if (a.mac_tie && soft)
+        hipLaunchKernelGGL((k_walk_lbvh_fast<W, true, true, DIAG>), dim3(blocks), dim3(64 * W), 0, s, a);
+    else if (a.mac_tie)
+        hipLaunchKernelGGL((k_walk_lbvh_fast<W, true, false, DIAG>), dim3(blocks), dim3(64 * W), 0, s, a);
+    else if (soft)
+        hipLaunchKernelGGL((k_walk_lbvh_fast<W, false, true, DIAG>), dim3(blocks), dim3(64 * W), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_walk_lbvh_fast<W, false, false, DIAG>), dim3(blocks), dim3(64 * W), 0, s, a);
+}
+
+void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, int diag, hipStream_t s) {
     if (a.n <= 0) return;
     if (!ordered) {
-        constexpr int W = 4;
-        const unsigned blocks = (unsigned)((a.n + 64 * W - 1) / (64 * W));
-        hipLaunchKernelGGL((k_walk_lbvh<false, 1, W>), dim3(blocks), dim3(64 * W), 0, s, a);
+        if (diag) launch_fast<true>(a, s);
+        else launch_fast<false>(a, s);
     } else {
         const unsigned blocks = (unsigned)((a.n + 63) / 64);
         if (max_depth <= 32)
-            hipLaunchKernelGGL((k_walk_lbvh<true, 32, 1>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((k_walk_lbvh_ordered<32>), dim3(blocks), dim3(64), 0, s, a);
         else
-            hipLaunchKernelGGL((k_walk_lbvh<true, 64, 1>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((k_walk_lbvh_ordered<64>), dim3(blocks), dim3(64), 0, s, a);
     }
 }
 
