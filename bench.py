@@ -110,6 +110,7 @@ def main():
     s.step(args.warmup)
     s.synchronize()
     s.set_state(b)  # timed run starts from the initial conditions
+    s.enable_counts(False)  # production steps do not count interactions
     s.enable_timing(True)
     log(f"[bench] timing {args.steps} steps")
     s.synchronize()
@@ -118,11 +119,13 @@ def main():
     s.synchronize()
     elapsed = time.perf_counter() - t0
     phases = s.phase_times()
-    pp, npc = s.counts()
     s.enable_timing(False)
-    # schedule statistics from one extra, untimed force pass
+    # interaction counts and schedule statistics from one extra, untimed force
+    # pass on the final state of the timed run
+    s.enable_counts(True)
     s.enable_walk_stats(True)
     s.update_accelerations()
+    pp, npc = s.counts()
     ws = s.walk_stats()
     s.enable_walk_stats(False)
 

@@ -122,8 +122,12 @@ int nbg_sys_synchronize(nbg_sys *sys);
 
 /* Interaction counts of the last force pass as SimpleTrackingRule counts
  * them (Physics/Rules/SimpleTrackingRule.h:29-45): pp = particle-particle
- * calls (self pairs included), np = node-particle calls.  64-bit. */
+ * calls (self pairs included), np = node-particle calls.  64-bit.  The
+ * reference counts with a separate rule; here counting is a switch of the
+ * walk (on by default; FAST passes skip it while it is off, ORDERED passes
+ * always count). */
 int nbg_sys_counts(nbg_sys *sys, uint64_t *pp, uint64_t *np);
+int nbg_sys_enable_counts(nbg_sys *sys, int32_t on);
 
 /* Schedule statistics of the last FAST force pass (no reference counterpart;
  * diagnostics of the wavefront walk): out4 = {wave-level node visits, active

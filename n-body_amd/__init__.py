@@ -35,7 +35,8 @@ EXPORTS = (
     "nbg_ctx_create", "nbg_ctx_destroy", "nbg_sys_create", "nbg_sys_destroy",
     "nbg_sys_set_params", "nbg_sys_get_params", "nbg_sys_set_state", "nbg_sys_get_state",
     "nbg_sys_update_accelerations", "nbg_sys_get_accelerations", "nbg_sys_step",
-    "nbg_sys_synchronize", "nbg_sys_counts", "nbg_sys_enable_walk_stats", "nbg_sys_walk_stats",
+    "nbg_sys_synchronize", "nbg_sys_counts", "nbg_sys_enable_counts", "nbg_sys_enable_walk_stats",
+    "nbg_sys_walk_stats",
     "nbg_sys_enable_timing", "nbg_sys_phase_times",
     "nbg_sys_debug_bbox", "nbg_sys_debug_order", "nbg_sys_debug_tree", "nbg_direct_f64",
     "nbg_sys_potential_f64", "nbg_generate_plummer", "nbg_generate_cube",
@@ -104,6 +105,7 @@ def lib() -> ctypes.CDLL:
             "nbg_sys_counts": (ctypes.c_int, [_vp, _u64p, _u64p]),
             "nbg_sys_walk_stats": (ctypes.c_int, [_vp, _u64p]),
             "nbg_sys_enable_walk_stats": (ctypes.c_int, [_vp, ctypes.c_int32]),
+            "nbg_sys_enable_counts": (ctypes.c_int, [_vp, ctypes.c_int32]),
             "nbg_sys_enable_timing": (ctypes.c_int, [_vp, ctypes.c_int32]),
             "nbg_sys_phase_times": (ctypes.c_int, [_vp, _f32p, ctypes.c_int32]),
             "nbg_sys_debug_bbox": (ctypes.c_int, [_vp, _f32p]),
@@ -281,6 +283,9 @@ class System:
         pp, np_ = ctypes.c_uint64(), ctypes.c_uint64()
         _check(lib().nbg_sys_counts(self.handle, ctypes.byref(pp), ctypes.byref(np_)))
         return int(pp.value), int(np_.value)
+
+    def enable_counts(self, on: bool = True):
+        _check(lib().nbg_sys_enable_counts(self.handle, int(on)))
 
     def enable_walk_stats(self, on: bool = True):
         _check(lib().nbg_sys_enable_walk_stats(self.handle, int(on)))

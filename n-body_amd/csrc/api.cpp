@@ -49,6 +49,7 @@ struct nbg_sys {
     bool have_forces = false;
     bool timing = false;
     bool walk_stats = false;  // FAST walk counts its schedule statistics
+    bool counting = true;     // FAST walk counts interactions (SimpleTrackingRule)
     // Timing: one set of kPhases+1 events per force pass, recorded on the
     // system stream; folded into per-phase sums when the stream is synchronised.
     std::vector<hipEvent_t> ev;    // kEventSets * (kPhases + 1)
@@ -164,7 +165,7 @@ int enqueue_force_pass(nbg_sys *s) {
                          s->kparent, s->scan_ws, s->nodes, s->leaf_start, s->node_count(), st);
     mark(s, 5);
     launch_lbvh_summaries(s->bodies, n, s->nodes, s->leaf_start, 2 * n, s->flags, s->glist, s->bmin, s->bmax,
-                          s->err_word(), st);
+                          p.theta, s->err_word(), st);
     mark(s, 6);
     WalkArgs wa;
     wa.n = n;
@@ -182,7 +183,8 @@ int enqueue_force_pass(nbg_sys *s) {
     wa.az = s->az;
     wa.counts = s->counts();
     wa.err = s->err_word();
-    launch_walk_lbvh(wa, p.sum_mode == NBG_SUM_ORDERED, p.morton_bits == 30 ? 32 : 64, s->walk_stats ? 1 : 0, st);
+    const int stats = s->walk_stats ? 2 : (s->counting ? 1 : 0);
+    launch_walk_lbvh(wa, p.sum_mode == NBG_SUM_ORDERED, p.morton_bits == 30 ? 32 : 64, stats, st);
     mark(s, 7);
     NBG_HIP(hipGetLastError());
     return NBG_OK;
@@ -433,6 +435,12 @@ int nbg_sys_walk_stats(nbg_sys *s, uint64_t *out4) {
     unsigned long long c[6];
     NBG_HIP(hipMemcpy(c, s->counts(), sizeof c, hipMemcpyDeviceToHost));
     for (int i = 0; i < 4; ++i) out4[i] = c[2 + i];
+    return NBG_OK;
+}
+
+int nbg_sys_enable_counts(nbg_sys *s, int32_t on) {
+    if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
+    s->counting = on != 0;
     return NBG_OK;
 }
 

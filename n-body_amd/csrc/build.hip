@@ -536,18 +536,25 @@ void launch_lbvh_topology(const void *keys, int bits, int64_t n64, int leaf_size
 // are queued and finished by a single workgroup in a second launch.  No
 // agent-scope fence is used: on gfx950 that is an L2 write-back and
 // invalidate across the XCDs.
-__device__ __forceinline__ float4 mac_record(float4 mn, float4 mx) {
+// The MAC record: bbox centre and the squared-distance threshold above which
+// a target accepts the node (mac_lbvh_threshold of the squared diagonal).
+struct MacParam {
+    double mid;
+    int tie;
+};
+
+__device__ __forceinline__ float4 mac_record(float4 mn, float4 mx, MacParam mp) {
     const float cx = (mx.x + mn.x) / 2.0f, cy = (mx.y + mn.y) / 2.0f, cz = (mx.z + mn.z) / 2.0f;
     const float dx = mx.x - mn.x, dy = mx.y - mn.y, dz = mx.z - mn.z;
-    const float d2 = dx * dx + dy * dy + dz * dz;
-    return make_float4(cx, cy, cz, d2);
+    const float diag2 = dx * dx + dy * dy + dz * dz;
+    return make_float4(cx, cy, cz, mac_lbvh_threshold(diag2, mp.mid, mp.tie != 0));
 }
 
 __device__ __forceinline__ uint32_t arrive_wg(uint32_t *flag) {
     return __hip_atomic_fetch_add(flag, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-__device__ void summarize_pair(int p, int c0, NodeRec *nodes, float4 *bmin, float4 *bmax) {
+__device__ void summarize_pair(int p, int c0, NodeRec *nodes, float4 *bmin, float4 *bmax, MacParam mp) {
     const float4 a = nodes[c0].com;
     const float4 b = nodes[c0 + 1].com;
     const float4 an = bmin[c0], bn = bmin[c0 + 1];
@@ -566,11 +573,11 @@ __device__ void summarize_pair(int p, int c0, NodeRec *nodes, float4 *bmin, floa
     px4 = make_float4(fmax_ref(bx.x, px4.x), fmax_ref(bx.y, px4.y), fmax_ref(bx.z, px4.z), 0.0f);
     bmin[p] = pn;
     bmax[p] = px4;
-    nodes[p].mac = mac_record(pn, px4);
+    nodes[p].mac = mac_record(pn, px4, mp);
 }
 
 __device__ void summarize_leaf(int s, int4 t, const float4 *__restrict__ bodies, NodeRec *nodes, float4 *bmin,
-                               float4 *bmax) {
+                               float4 *bmax, MacParam mp) {
     float M = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
     float nx = FLT_MAX, ny = FLT_MAX, nz = FLT_MAX, xx = -FLT_MAX, xy = -FLT_MAX, xz = -FLT_MAX;
     for (int i = t.x; i < t.x + t.y; ++i) {
@@ -589,14 +596,14 @@ __device__ void summarize_leaf(int s, int4 t, const float4 *__restrict__ bodies,
     nodes[s].com = make_float4(ieee_div(cx, M), ieee_div(cy, M), ieee_div(cz, M), M);
     bmin[s] = mn;
     bmax[s] = mx;
-    nodes[s].mac = mac_record(mn, mx);
+    nodes[s].mac = mac_record(mn, mx, mp);
 }
 
 __global__ __launch_bounds__(256) void k_summ_local(const float4 *__restrict__ bodies, int64_t n,
                                                     NodeRec *nodes,
                                                     const int32_t *__restrict__ leaf_start, int64_t max_nodes,
                                                     uint32_t *flags, uint32_t *glist, float4 *bmin,
-                                                    float4 *bmax, uint32_t *err) {
+                                                    float4 *bmax, MacParam mp, uint32_t *err) {
     const int64_t lo = (int64_t)blockIdx.x * kSummBodies, hi = lo + kSummBodies;
     const int64_t end = hi < n ? hi : n;
     for (int64_t i = lo + threadIdx.x; i < end; i += blockDim.x) {
@@ -605,7 +612,7 @@ __global__ __launch_bounds__(256) void k_summ_local(const float4 *__restrict__ b
         if (s >= max_nodes) { atomicOr(err, 4u); continue; }
         const int4 t = nodes[s].topo;
         if (t.x != (int)i || t.y < 1 || (int64_t)t.x + t.y > n || t.z >= 0) { atomicOr(err, 4u); continue; }
-        summarize_leaf(s, t, bodies, nodes, bmin, bmax);
+        summarize_leaf(s, t, bodies, nodes, bmin, bmax, mp);
         int parent = t.w;
         while (parent >= 0) {
             if (parent >= max_nodes) { atomicOr(err, 4u); break; }
@@ -619,7 +626,7 @@ __global__ __launch_bounds__(256) void k_summ_local(const float4 *__restrict__ b
             }
             if (arrive_wg(&flags[parent]) == 0) break;
             if (tp.z < 0 || tp.z + 1 >= max_nodes) { atomicOr(err, 4u); break; }
-            summarize_pair(parent, tp.z, nodes, bmin, bmax);
+            summarize_pair(parent, tp.z, nodes, bmin, bmax, mp);
             parent = tp.w;
         }
     }
@@ -627,7 +634,7 @@ __global__ __launch_bounds__(256) void k_summ_local(const float4 *__restrict__ b
 
 __global__ __launch_bounds__(1024) void k_summ_global(NodeRec *nodes, int64_t max_nodes, uint32_t *flags,
                                                       const uint32_t *glist, float4 *bmin, float4 *bmax,
-                                                      uint32_t *err) {
+                                                      MacParam mp, uint32_t *err) {
     const uint32_t cnt = glist[0];
     for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
         int parent = (int)glist[64 + j];
@@ -636,7 +643,7 @@ __global__ __launch_bounds__(1024) void k_summ_global(NodeRec *nodes, int64_t ma
             const int4 tp = nodes[parent].topo;
             if (arrive_wg(&flags[parent]) == 0) break;
             if (tp.z < 0 || tp.z + 1 >= max_nodes) { atomicOr(err, 4u); break; }
-            summarize_pair(parent, tp.z, nodes, bmin, bmax);
+            summarize_pair(parent, tp.z, nodes, bmin, bmax, mp);
             parent = tp.w;
         }
     }
@@ -644,14 +651,17 @@ __global__ __launch_bounds__(1024) void k_summ_global(NodeRec *nodes, int64_t ma
 
 void launch_lbvh_summaries(const float4 *bodies, int64_t n, NodeRec *nodes, const int32_t *leaf_start,
                            int64_t max_nodes, uint32_t *flags, uint32_t *glist, float4 *bmin, float4 *bmax,
-                           uint32_t *err, hipStream_t s) {
+                           float theta, uint32_t *err, hipStream_t s) {
     if (n <= 0) return;
+    MacParam mp;
+    mac_threshold(theta, &mp.mid, &mp.tie);
     launch_fill_u32(flags, 0u, max_nodes, s);
     launch_fill_u32(glist, 0u, 64, s);
     const unsigned blocks = (unsigned)((n + kSummBodies - 1) / kSummBodies);
     hipLaunchKernelGGL(k_summ_local, dim3(blocks), dim3(256), 0, s, bodies, n, nodes, leaf_start, max_nodes, flags,
-                       glist, bmin, bmax, err);
-    hipLaunchKernelGGL(k_summ_global, dim3(1), dim3(1024), 0, s, nodes, max_nodes, flags, glist, bmin, bmax, err);
+                       glist, bmin, bmax, mp, err);
+    hipLaunchKernelGGL(k_summ_global, dim3(1), dim3(1024), 0, s, nodes, max_nodes, flags, glist, bmin, bmax, mp,
+                       err);
 }
 
 }  // namespace nbg

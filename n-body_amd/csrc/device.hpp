@@ -37,6 +37,25 @@ __device__ __forceinline__ bool mac_lbvh_exact(float diag2, float d2, double mid
     return tie ? ((l < r) || (l == r && d2 > 0.0f && l < __builtin_huge_val())) : (l < r);
 }
 
+// Per-node form of the same test: the reference accepts iff d2 > thr, where
+// thr is the largest float f >= 0 with f * mid <= diag2 (f * mid < diag2 when
+// a tie rounds down).  For float d2, "d2 > tau = diag2 / mid" is equivalent
+// to "d2 > largest float <= tau"; every product f * mid is exact in FP64.
+// mid <= 0 (theta = 0) never accepts; a NaN diag2 never accepts.
+__device__ __forceinline__ float mac_lbvh_threshold(float diag2, double mid, bool tie) {
+    if (!(mid > 0.0) || diag2 == __builtin_huge_valf()) return __builtin_huge_valf();
+    if (diag2 != diag2) return diag2;
+    const double D = (double)diag2;
+    float f = (float)(D / mid);  // within an ulp of diag2 / mid, >= 0
+    auto ok = [&](float v) { const double p = (double)v * mid; return tie ? (p < D) : (p <= D); };
+    // neighbours of a non-negative finite float by its bit pattern
+    auto up = [](float v) { return __uint_as_float(__float_as_uint(v) + 1u); };
+    auto down = [](float v) { return __uint_as_float(__float_as_uint(v) - 1u); };
+    while (f > 0.0f && !ok(f)) f = down(f);
+    while (f < __FLT_MAX__ && ok(up(f))) f = up(f);
+    return f > 0.0f ? f : 0.0f;
+}
+
 // Workgroup -> target-range mapping that keeps each XCD on one contiguous
 // eighth of the Morton order (workgroups are dispatched round-robin over the
 // 8 XCDs, and each XCD has its own L2).  A bijection on [0, grid).

@@ -83,7 +83,7 @@ void launch_lbvh_topology(const void *keys, int bits, int64_t n, int leaf_size, 
 constexpr int kSummBodies = 4096;   // bodies per workgroup in the local pass
 void launch_lbvh_summaries(const float4 *bodies, int64_t n, NodeRec *nodes, const int32_t *leaf_start,
                            int64_t max_nodes, uint32_t *flags, uint32_t *glist, float4 *bmin, float4 *bmax,
-                           uint32_t *err, hipStream_t s);
+                           float theta, uint32_t *err, hipStream_t s);
 
 // Host side of mac_lbvh_exact: the FP64 midpoint below T = RN(theta*theta)
 // and whether a tie rounds down (T has an odd significand).

@@ -52,15 +52,13 @@ __device__ __forceinline__ void pp_exact(float bx, float by, float bz, float bm,
 }
 
 // DiagonalOverCenterDistance (DiagonalOverCenterDistance.h:18-22) for one
-// lane: squared distance to the bbox centre in glm's order, then the exact
-// comparison of mac_lbvh_exact with the tie rule fixed at compile time.
-template <bool TIE>
-__device__ __forceinline__ bool lbvh_accepts(float4 mc, float px, float py, float pz, double mid) {
+// lane: the squared distance to the bbox centre in glm's order, compared
+// with the node's threshold (mac_lbvh_threshold, computed with the node's
+// summary), which is exactly equivalent to diag2 / d2 < theta^2 in FP32.
+__device__ __forceinline__ bool lbvh_accepts(float4 mc, float px, float py, float pz) {
     const float dx = px - mc.x, dy = py - mc.y, dz = pz - mc.z;
     const float d2 = dx * dx + dy * dy + dz * dz;
-    const double l = (double)mc.w, r = mid * (double)d2;
-    if (TIE) return l <= r && r > 0.0 && l < __builtin_huge_val();
-    return l < r;
+    return d2 > mc.w;
 }
 
 // ---------------------------------------------------------------- ORDERED
@@ -84,8 +82,6 @@ __global__ __launch_bounds__(64) void k_walk_lbvh_ordered(WalkArgs a) {
     float rx = 0.0f, ry = 0.0f, rz = 0.0f;
     uint32_t cpp = 0, cnp = 0;
     const float G = a.G, eps2 = a.eps2;
-    const double mid = a.mac_mid;
-    const bool tie = a.mac_tie != 0;
 
     if (lane == 0) { s_node[0] = 0; s_meta[0] = 0; s_mask[0] = vmask; }
     int sp = 1;
@@ -111,8 +107,7 @@ __global__ __launch_bounds__(64) void k_walk_lbvh_ordered(WalkArgs a) {
         }
         if (node < 0 || node >= a.max_nodes) { fail = true; break; }
         const NodeRec nd = nodes[node];
-        const bool pass = act && (tie ? lbvh_accepts<true>(nd.mac, px, py, pz, mid)
-                                      : lbvh_accepts<false>(nd.mac, px, py, pz, mid));
+        const bool pass = act && lbvh_accepts(nd.mac, px, py, pz);
         if (pass) {
             ++cnp;
             float vx, vy, vz;
@@ -211,26 +206,26 @@ __device__ __forceinline__ void leaf_bodies(const float4 *__restrict__ bodies, i
 
 // Visits one node for the lanes in `mask` (wave-uniform); returns the lanes
 // that open it when it is internal, 0 when it is a leaf or all accept.
-template <bool TIE, bool SOFT, bool DIAG>
+template <bool SOFT, int STATS>
 __device__ __forceinline__ uint64_t visit(const NodeRec &nd, uint64_t mask, const float4 *__restrict__ bodies,
-                                          int64_t n, float eps2, double mid, FastWave &W, bool &fail) {
-    const uint64_t acc = __ballot(lbvh_accepts<TIE>(nd.mac, W.px, W.py, W.pz, mid)) & mask;
-    if (DIAG) { W.visits += 1; W.lanes += __popcll(mask); }
+                                          int64_t n, float eps2, FastWave &W, bool &fail) {
+    const uint64_t acc = __ballot(lbvh_accepts(nd.mac, W.px, W.py, W.pz)) & mask;
+    if (STATS >= 2) { W.visits += 1; W.lanes += __popcll(mask); }
     if (acc) {
-        W.np += __popcll(acc);
+        if (STATS >= 1) W.np += __popcll(acc);
         if (__builtin_amdgcn_inverse_ballot_w64(acc)) interact<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, eps2, W);
     }
     const uint64_t open = mask & ~acc;
     if (open == 0 || nd.topo.z >= 0) return open;
     const int first = nd.topo.x, cnt = nd.topo.y;
     if (first < 0 || cnt < 0 || (int64_t)first + cnt > n) { fail = true; return 0; }
-    if (DIAG) { W.leaf_visits += 1; W.leaf_lanes += __popcll(open); }
-    W.pp += (uint64_t)__popcll(open) * (uint64_t)cnt;
+    if (STATS >= 2) { W.leaf_visits += 1; W.leaf_lanes += __popcll(open); }
+    if (STATS >= 1) W.pp += (uint64_t)__popcll(open) * (uint64_t)cnt;
     leaf_bodies<SOFT>(bodies, first, cnt, open, eps2, W);
     return 0;
 }
 
-template <int WAVES, bool TIE, bool SOFT, bool DIAG>
+template <int WAVES, bool SOFT, int STATS>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     __shared__ int s_node[WAVES][kStackFast];
     __shared__ uint64_t s_mask[WAVES][kStackFast];
@@ -253,7 +248,6 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     // body loads below stay scalar).
     asm("v_mov_b32 %0, %1" : "=v"(W.G) : "s"(a.G));
     const float eps2 = a.eps2;
-    const double mid = a.mac_mid;
     const int64_t n = a.n;
     const uint32_t last_pair = (uint32_t)(a.max_nodes - 1);
     bool fail = false;
@@ -261,7 +255,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     // The root is visited on its own; afterwards every step visits the two
     // children (c, c + 1) of a node that the lanes in `mask` opened.
     const NodeRec root = nodes[0];
-    uint64_t mask = visit<TIE, SOFT, DIAG>(root, vmask, bodies, n, eps2, mid, W, fail);
+    uint64_t mask = visit<SOFT, STATS>(root, vmask, bodies, n, eps2, W, fail);
     int c = root.topo.z;
     int sp = 0;
     uint32_t iters = 0;
@@ -276,8 +270,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
         c = uniform_i32(c);  // keeps the node loads scalar
         if (++iters > kMaxWalkIters || (uint32_t)c >= last_pair) { fail = true; break; }
         const NodeRec r0 = nodes[c], r1 = nodes[c + 1];
-        const uint64_t o0 = visit<TIE, SOFT, DIAG>(r0, mask, bodies, n, eps2, mid, W, fail);
-        const uint64_t o1 = visit<TIE, SOFT, DIAG>(r1, mask, bodies, n, eps2, mid, W, fail);
+        const uint64_t o0 = visit<SOFT, STATS>(r0, mask, bodies, n, eps2, W, fail);
+        const uint64_t o1 = visit<SOFT, STATS>(r1, mask, bodies, n, eps2, W, fail);
         if (fail) break;
         if (o0 != 0) {
             if (o1 != 0) {
@@ -303,7 +297,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     if (lane == 0) {
         atomicAdd(&a.counts[0], (unsigned long long)W.pp);
         atomicAdd(&a.counts[1], (unsigned long long)W.np);
-        if (DIAG) {
+        if (STATS >= 2) {
             atomicAdd(&a.counts[2], (unsigned long long)W.visits);
             atomicAdd(&a.counts[3], (unsigned long long)W.lanes);
             atomicAdd(&a.counts[4], (unsigned long long)W.leaf_visits);
@@ -312,26 +306,22 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     }
 }
 
-template <bool DIAG>
+template <int STATS>
 static void launch_fast(const WalkArgs &a, hipStream_t s) {
     constexpr int W = 4;
     const unsigned blocks = (unsigned)((a.n + 64 * W - 1) / (64 * W));
-    const bool soft = a.eps2 > 0.0f;
-    if (a.mac_tie && soft)
-        hipLaunchKernelGGL((k_walk_lbvh_fast<W, true, true, DIAG>), dim3(blocks), dim3(64 * W), 0, s, a);
-    else if (a.mac_tie)
-        hipLaunchKernelGGL((k_walk_lbvh_fast<W, true, false, DIAG>), dim3(blocks), dim3(64 * W), 0, s, a);
-    else if (soft)
-        hipLaunchKernelGGL((k_walk_lbvh_fast<W, false, true, DIAG>), dim3(blocks), dim3(64 * W), 0, s, a);
+    if (a.eps2 > 0.0f)
+        hipLaunchKernelGGL((k_walk_lbvh_fast<W, true, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
     else
-        hipLaunchKernelGGL((k_walk_lbvh_fast<W, false, false, DIAG>), dim3(blocks), dim3(64 * W), 0, s, a);
+        hipLaunchKernelGGL((k_walk_lbvh_fast<W, false, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
 }
 
-void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, int diag, hipStream_t s) {
+void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, int stats, hipStream_t s) {
     if (a.n <= 0) return;
     if (!ordered) {
-        if (diag) launch_fast<true>(a, s);
-        else launch_fast<false>(a, s);
+        if (stats >= 2) launch_fast<2>(a, s);
+        else if (stats == 1) launch_fast<1>(a, s);
+        else launch_fast<0>(a, s);
     } else {
         const unsigned blocks = (unsigned)((a.n + 63) / 64);
         if (max_depth <= 32)
