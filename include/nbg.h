@@ -45,8 +45,9 @@ enum nbg_solver_kind { NBG_BARNES_HUT = 0, NBG_LINEAR_BVH = 1 };
 /* Per-body sum order.  ORDERED reproduces the reference's arithmetic exactly:
  * IEEE sqrt/div, no FMA, and the std::transform_reduce bracketing of child
  * results (Descent/activeTree.h:44-52,66-76).  FAST keeps the reference's
- * interaction set and order (same opening decisions) but sums each body's
- * interactions into one accumulator with rsqrt and FMA. */
+ * interaction set (the same, exact opening decisions) but sums each body's
+ * interactions into one accumulator, in the walk's own order, with rsqrt
+ * and FMA. */
 enum nbg_sum_mode { NBG_SUM_ORDERED = 0, NBG_SUM_FAST = 1 };
 
 typedef struct nbg_params {
@@ -139,7 +140,7 @@ int nbg_sys_walk_stats(nbg_sys *sys, uint64_t *out4);
 
 /* Per-phase device time (ms) of the last force pass, measured with HIP events
  * on the system's stream when timing is enabled.  Phases, in order:
- * 0 bbox, 1 keys, 2 sort, 3 gather, 4 topology, 5 summaries, 6 walk,
+ * 0 bbox, 1 keys, 2 sort, 3 topology, 4 gather, 5 summaries, 6 walk,
  * 7 update.  Returns the number of phases written (<= cap). */
 int nbg_sys_enable_timing(nbg_sys *sys, int32_t on);
 int nbg_sys_phase_times(nbg_sys *sys, float *ms, int32_t cap);

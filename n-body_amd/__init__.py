@@ -27,7 +27,7 @@ LINEAR_BVH = 1
 SUM_ORDERED = 0
 SUM_FAST = 1
 
-PHASES = ("bbox", "keys", "sort", "gather", "topology", "summaries", "walk", "update")
+PHASES = ("bbox", "keys", "sort", "topology", "gather", "summaries", "walk", "update")
 
 # Every symbol include/nbg.h declares.
 EXPORTS = (

@@ -75,6 +75,8 @@ struct nbg_sys {
     NodeRec *nodes = nullptr;
     float4 *bmin = nullptr, *bmax = nullptr;
     uint32_t *flags = nullptr;
+    // octree work and nodes (allocated when the Barnes-Hut solver is selected)
+    OctBuffers ob;
     // small
     float *bbox_partials = nullptr, *bbox = nullptr;
     // [1] err, [2] node count, [4..16) counts: pp, np, then the FAST walk's
@@ -92,8 +94,6 @@ namespace {
 int check_params(const nbg_params *p) {
     if (!p) return fail(NBG_ERR_ARG, "params is NULL");
     if (p->solver != NBG_BARNES_HUT && p->solver != NBG_LINEAR_BVH) return fail(NBG_ERR_ARG, "unknown solver kind");
-    if (p->solver == NBG_BARNES_HUT)
-        return fail(NBG_ERR_ARG, "the octree Barnes-Hut solver is not available in this build yet");
     if (!(p->theta >= 0.0f)) return fail(NBG_ERR_ARG, "theta must be >= 0");
     if (p->leaf_size < 1) return fail(NBG_ERR_ARG, "leaf_size must be >= 1");
     if (p->max_depth < 0 || p->max_depth > 32) return fail(NBG_ERR_ARG, "max_depth must be in [0, 32]");
@@ -110,7 +110,48 @@ int dalloc(T **p, int64_t count) {
     return NBG_OK;
 }
 
+void free_oct(OctBuffers &b) {
+    void *ptrs[] = {b.key_hi, b.key_lo, b.lo_sorted, b.nown, b.nbase, b.nfirst, b.ncount, b.nparent, b.nmeta,
+                    b.occ, b.cb, b.flags, b.glist, b.scan_ws, b.nodes, b.centre, b.thr, b.misc};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    b = OctBuffers{};
+}
+
+// Allocates the octree buffers for the system's size and parameters (no-op
+// when the current ones are large enough).
+int ensure_octree(nbg_sys *s) {
+    if (s->prm.solver != NBG_BARNES_HUT) return NBG_OK;
+    const int64_t n = s->n;
+    const int64_t cap = octree_capacity(n, s->prm.leaf_size, s->prm.max_depth);
+    if (cap >= ((int64_t)1 << 28))
+        return fail(NBG_ERR_ARG, "octree node bound exceeds 2^28 slots; raise leaf_size or lower n");
+    if (s->ob.cap >= cap && s->ob.n == n) return NBG_OK;
+    if (s->stream) NBG_HIP(hipStreamSynchronize(s->stream));
+    free_oct(s->ob);
+    OctBuffers &b = s->ob;
+    int r = NBG_OK;
+    auto A = [&](auto **ptr, int64_t count) {
+        if (r == NBG_OK) r = dalloc(ptr, count);
+    };
+    A(&b.key_hi, n); A(&b.key_lo, n); A(&b.lo_sorted, n);
+    A(&b.nown, n); A(&b.nbase, n);
+    A(&b.nfirst, cap); A(&b.ncount, cap); A(&b.nparent, cap); A(&b.nmeta, cap);
+    A(&b.occ, cap); A(&b.cb, cap); A(&b.flags, cap); A(&b.glist, cap + 64);
+    A(&b.scan_ws, 128 + (cap + kScanTile - 1) / kScanTile);
+    A(&b.nodes, cap); A(&b.centre, cap); A(&b.thr, 64); A(&b.misc, 64);
+    if (r != NBG_OK) {
+        free_oct(b);
+        return r;
+    }
+    b.cap = cap;
+    b.n = n;
+    NBG_HIP(hipMemset(b.misc, 0, 64 * sizeof(uint32_t)));
+    return NBG_OK;
+}
+
 void free_sys(nbg_sys *s) {
+    free_oct(s->ob);
     void *ptrs[] = {s->x, s->y, s->z, s->m, s->vx, s->vy, s->vz, s->ax, s->ay, s->az, s->keys, s->keys_alt,
                     s->idx, s->idx_alt, s->bodies, s->kfirst, s->klast, s->kgamma, s->kparent, s->ksplit,
                     s->krank, s->leaf_start, s->glist, s->scan_ws, s->sort_status, s->nodes, s->bmin,
@@ -144,47 +185,72 @@ void fold_timing(nbg_sys *s) {
     s->ev_used = 0;
 }
 
-// ActiveTreeSolver::updateAccelerations (Solvers/ActiveTreeSolver.h:33-72)
-// for the linear BVH: build the tree (LinearBVH::build, Trees/LinearBVH.h:106-133),
-// then walk it for every body.  Enqueue only.
+// ActiveTreeSolver::updateAccelerations (Solvers/ActiveTreeSolver.h:33-72):
+// build the tree (LinearBVH::build, Trees/LinearBVH.h:106-133, or
+// Octree::build, Trees/Octree.h:185-205), then walk it for every body.
+// Enqueue only.
 int enqueue_force_pass(nbg_sys *s) {
     const int64_t n = s->n;
     const nbg_params &p = s->prm;
     hipStream_t st = s->stream;
+    const bool octree = p.solver == NBG_BARNES_HUT;
+    if (octree)
+        if (int r = ensure_octree(s)) return r;
     launch_fill_u32(s->misc + 4, 0u, 12, st);  // interaction counters
     mark(s, 0);
     launch_bbox(s->x, s->y, s->z, n, s->bbox_partials, s->bbox, st);
     mark(s, 1);
-    launch_morton(s->x, s->y, s->z, n, s->bbox, p.morton_bits, s->keys, s->idx, st);
-    mark(s, 2);
-    launch_radix_sort(s->keys, s->idx, s->keys_alt, s->idx_alt, n, p.morton_bits, s->sort_status, s->err_word(), st);
-    mark(s, 3);
-    launch_gather(s->x, s->y, s->z, s->m, s->idx, n, s->bodies, st);
-    mark(s, 4);
-    launch_lbvh_topology(s->keys, p.morton_bits, n, p.leaf_size, s->kfirst, s->klast, s->kgamma, s->ksplit, s->krank,
-                         s->kparent, s->scan_ws, s->nodes, s->leaf_start, s->node_count(), st);
-    mark(s, 5);
-    launch_lbvh_summaries(s->bodies, n, s->nodes, s->leaf_start, 2 * n, s->flags, s->glist, s->bmin, s->bmax,
-                          p.theta, s->err_word(), st);
+    if (octree) {
+        OctBuffers &ob = s->ob;
+        uint64_t *keys = (uint64_t *)s->keys, *keys_alt = (uint64_t *)s->keys_alt;
+        launch_octree_keys(s->x, s->y, s->z, n, s->bbox, p.max_depth, ob, keys, s->idx, s->idx_alt, st);
+        mark(s, 2);
+        launch_octree_sort(n, p.max_depth, ob, keys, s->idx, keys_alt, s->idx_alt, s->sort_status, s->err_word(), st);
+        mark(s, 3);
+        launch_octree_topology(n, s->bbox, p.leaf_size, p.max_depth, p.theta, ob, keys, s->idx_alt, s->idx,
+                               s->leaf_start, s->err_word(), st);
+        mark(s, 4);
+        launch_gather(s->x, s->y, s->z, s->m, s->idx, n, s->bodies, st);
+        mark(s, 5);
+        launch_octree_summaries(s->bodies, n, ob, s->leaf_start, s->err_word(), st);
+    } else {
+        launch_morton(s->x, s->y, s->z, n, s->bbox, p.morton_bits, s->keys, s->idx, st);
+        mark(s, 2);
+        launch_radix_sort(s->keys, s->idx, s->keys_alt, s->idx_alt, n, p.morton_bits, s->sort_status, s->err_word(),
+                          st);
+        mark(s, 3);
+        launch_lbvh_topology(s->keys, p.morton_bits, n, p.leaf_size, s->kfirst, s->klast, s->kgamma, s->ksplit,
+                             s->krank, s->kparent, s->scan_ws, s->nodes, s->leaf_start, s->node_count(), st);
+        mark(s, 4);
+        launch_gather(s->x, s->y, s->z, s->m, s->idx, n, s->bodies, st);
+        mark(s, 5);
+        launch_lbvh_summaries(s->bodies, n, s->nodes, s->leaf_start, 2 * n, s->flags, s->glist, s->bmin, s->bmax,
+                              p.theta, s->err_word(), st);
+    }
     mark(s, 6);
     WalkArgs wa;
     wa.n = n;
     wa.bodies = s->bodies;
     wa.perm = s->idx;
     wa.nodes = s->nodes;
+    wa.onodes = s->ob.nodes;
+    wa.levels = p.max_depth + 2;
     wa.theta = p.theta;
     wa.theta2 = p.theta * p.theta;
     mac_threshold(p.theta, &wa.mac_mid, &wa.mac_tie);
     wa.G = p.G;
     wa.eps2 = p.eps2;
-    wa.max_nodes = 2 * n;
+    wa.max_nodes = octree ? s->ob.cap : 2 * n;
     wa.ax = s->ax;
     wa.ay = s->ay;
     wa.az = s->az;
     wa.counts = s->counts();
     wa.err = s->err_word();
     const int stats = s->walk_stats ? 2 : (s->counting ? 1 : 0);
-    launch_walk_lbvh(wa, p.sum_mode == NBG_SUM_ORDERED, p.morton_bits == 30 ? 32 : 64, stats, st);
+    if (octree)
+        launch_walk_oct(wa, p.sum_mode == NBG_SUM_ORDERED, stats, st);
+    else
+        launch_walk_lbvh(wa, p.sum_mode == NBG_SUM_ORDERED, p.morton_bits == 30 ? 32 : 64, stats, st);
     mark(s, 7);
     NBG_HIP(hipGetLastError());
     return NBG_OK;
@@ -200,6 +266,7 @@ int sync_and_check(nbg_sys *s) {
         if (err & 1u) return fail(NBG_ERR_DEVICE, "radix sort look-back exceeded its spin limit");
         if (err & 2u) return fail(NBG_ERR_DEVICE, "tree traversal exceeded its stack, depth or index limits");
         if (err & 4u) return fail(NBG_ERR_DEVICE, "tree summaries found an invalid node index");
+        if (err & 8u) return fail(NBG_ERR_DEVICE, "octree build found an inconsistent node index");
         return fail(NBG_ERR_DEVICE, "device error flag set");
     }
     fold_timing(s);
@@ -302,6 +369,7 @@ int nbg_sys_create(nbg_ctx *ctx, const nbg_params *p, int64_t n, nbg_sys **out) 
         if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
         if (e != hipSuccess) r = fail(NBG_ERR_HIP, std::string("init: ") + hipGetErrorString(e));
     }
+    if (r == NBG_OK) r = ensure_octree(s);
     if (r != NBG_OK) {
         free_sys(s);
         delete s;
@@ -323,7 +391,13 @@ int nbg_sys_destroy(nbg_sys *s) {
 int nbg_sys_set_params(nbg_sys *s, const nbg_params *p) {
     if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
     if (int r = check_params(p)) return r;
+    DeviceGuard g(s->ctx->device);
+    const nbg_params old = s->prm;
     s->prm = *p;
+    if (int r = ensure_octree(s)) {
+        s->prm = old;
+        return r;
+    }
     return NBG_OK;
 }
 
@@ -513,9 +587,33 @@ int nbg_sys_debug_tree(nbg_sys *s, int64_t *num_nodes, int64_t *first, int64_t *
     DeviceGuard g(s->ctx->device);
     NBG_HIP(hipStreamSynchronize(s->stream));
     uint32_t nn = 0;
-    NBG_HIP(hipMemcpy(&nn, s->node_count(), 4, hipMemcpyDeviceToHost));
+    const bool octree = s->prm.solver == NBG_BARNES_HUT;
+    if (octree && s->n > 0) NBG_HIP(hipMemcpy(&nn, s->ob.misc, 4, hipMemcpyDeviceToHost));
+    else if (!octree) NBG_HIP(hipMemcpy(&nn, s->node_count(), 4, hipMemcpyDeviceToHost));
     *num_nodes = nn;
     if (!first && !count && !leaf && !parent && !com4 && !geo6) return NBG_OK;
+    if (octree) {
+        std::vector<OctRec> nd(nn);
+        std::vector<float4> cs(nn);
+        if (nn) {
+            NBG_HIP(hipMemcpy(nd.data(), s->ob.nodes, nn * sizeof(OctRec), hipMemcpyDeviceToHost));
+            NBG_HIP(hipMemcpy(cs.data(), s->ob.centre, nn * sizeof(float4), hipMemcpyDeviceToHost));
+        }
+        for (uint32_t i = 0; i < nn; ++i) {
+            const int4 t = nd[i].topo;
+            const float4 c = nd[i].com;
+            if (first) first[i] = t.x;
+            if (count) count[i] = t.y;
+            if (leaf) leaf[i] = t.z < 0;
+            if (parent) parent[i] = t.w;
+            if (com4) { com4[4 * i] = c.x; com4[4 * i + 1] = c.y; com4[4 * i + 2] = c.z; com4[4 * i + 3] = c.w; }
+            if (geo6) {
+                float gg[6] = {cs[i].x, cs[i].y, cs[i].z, cs[i].w, 0.0f, 0.0f};
+                std::memcpy(geo6 + 6 * i, gg, sizeof gg);
+            }
+        }
+        return NBG_OK;
+    }
     std::vector<NodeRec> nd(nn);
     std::vector<float4> lo(nn), hi(nn);
     if (nn) {

@@ -300,9 +300,8 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K *__restrict__ k
 }
 
 template <typename K>
-static void radix_sort_impl(K *keys, uint32_t *idx, K *keys_alt, uint32_t *idx_alt, int64_t n, int bits,
+static void radix_sort_impl(K *keys, uint32_t *idx, K *keys_alt, uint32_t *idx_alt, int64_t n, int passes,
                             uint32_t *status, uint32_t *err, hipStream_t s) {
-    const int passes = sort_passes(bits);
     const int64_t tiles = sort_tiles(n);
     uint32_t *ghist = status;                       // passes * 256
     uint32_t *counters = status + 8 * kRadix;       // passes
@@ -325,9 +324,16 @@ void launch_radix_sort(void *keys, uint32_t *idx, void *keys_alt, uint32_t *idx_
                        uint32_t *status, uint32_t *err, hipStream_t s) {
     if (n <= 1) return;
     if (bits == 30)
-        radix_sort_impl<uint32_t>((uint32_t *)keys, idx, (uint32_t *)keys_alt, idx_alt, n, 32, status, err, s);
+        radix_sort_impl<uint32_t>((uint32_t *)keys, idx, (uint32_t *)keys_alt, idx_alt, n, 4, status, err, s);
     else
-        radix_sort_impl<uint64_t>((uint64_t *)keys, idx, (uint64_t *)keys_alt, idx_alt, n, 64, status, err, s);
+        radix_sort_impl<uint64_t>((uint64_t *)keys, idx, (uint64_t *)keys_alt, idx_alt, n, 8, status, err, s);
+}
+
+bool launch_radix_sort_u64(uint64_t *keys, uint32_t *idx, uint64_t *keys_alt, uint32_t *idx_alt, int64_t n,
+                           int passes, uint32_t *status, uint32_t *err, hipStream_t s) {
+    if (n <= 1 || passes <= 0) return false;
+    radix_sort_impl<uint64_t>(keys, idx, keys_alt, idx_alt, n, passes, status, err, s);
+    return (passes & 1) != 0;
 }
 
 // ------------------------------------------------------------------ gather
@@ -425,9 +431,9 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_top(uint32_t *sums, int64_t
     if (threadIdx.x == 0) *total = carry;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const uint32_t *__restrict__ in, int64_t n,
-                                                           const uint32_t *__restrict__ sums,
-                                                           uint32_t *__restrict__ out) {
+// in may equal out: every thread reads its items before it writes them.
+__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const uint32_t *in, int64_t n,
+                                                           const uint32_t *__restrict__ sums, uint32_t *out) {
     __shared__ uint32_t s_tmp[kScanBlock / kWave];
     const int64_t base = blockIdx.x * (int64_t)kScanTile;
     // Blocked arrangement: thread t owns items [t*I, t*I + I).
@@ -445,6 +451,19 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const uint32_t *__res
         if (i < n) out[i] = run;
         run += v[j];
     }
+}
+
+void launch_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *ws, uint32_t *total, hipStream_t s) {
+    uint32_t *tot = total ? total : ws;
+    if (n <= 0) {
+        launch_fill_u32(tot, 0u, 1, s);
+        return;
+    }
+    uint32_t *sums = ws + 64;
+    const int64_t nt = (n + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)nt), dim3(kScanBlock), 0, s, in, n, sums);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, s, sums, nt, tot);
+    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)nt), dim3(kScanBlock), 0, s, in, n, sums, out);
 }
 
 template <typename K>

@@ -45,7 +45,84 @@ struct NodeRec {
     int4 topo;
 };
 
+// One octree node as the walk reads it (64 bytes).  The non-empty children
+// of a node are contiguous, in the reference's child order (octant
+// 4x + 2y + z; empty octants have no record):
+//   com  = centre of mass xyz, total mass;
+//   lo   = cell centre - side (xyz), w = squared-distance opening threshold;
+//   hi   = cell centre + side (xyz), w = bits of (octant | nchild << 8 |
+//          level << 16);
+//   topo = first body, body count, first child (-1 for a leaf), parent.
+struct OctRec {
+    float4 com;
+    float4 lo;
+    float4 hi;
+    int4 topo;
+};
+
+__host__ __device__ inline int oct_nchild(const OctRec &r) { return (int)((__builtin_bit_cast(uint32_t, r.hi.w) >> 8) & 0xFFu); }
+__host__ __device__ inline int oct_octant(const OctRec &r) { return (int)(__builtin_bit_cast(uint32_t, r.hi.w) & 0xFFu); }
+__host__ __device__ inline int oct_level(const OctRec &r) { return (int)(__builtin_bit_cast(uint32_t, r.hi.w) >> 16); }
+
+// Octree key geometry: the octant of every level is a 3-bit digit; levels
+// 0..20 live in key_hi bits 62..0, levels 21..33 in key_lo bits 38..0.
+constexpr int kOctHiLevels = 21;
+constexpr int kOctLevels = 34;   // max_depth 32 below the two forced levels
+
+// Upper bound on the number of non-empty octree nodes: every body is in one
+// leaf, and on each level 2..levels-1 an internal node holds > leaf bodies.
+inline int64_t octree_capacity(int64_t n, int leaf, int max_depth) {
+    const int64_t levels = (int64_t)max_depth + 2;
+    return 9 + n + (levels - 2) * (n / ((int64_t)leaf + 1) + 1);
+}
+
+// Work buffers of the octree build (capacity `cap` nodes, n bodies).
+struct OctBuffers {
+    int64_t cap = 0, n = 0;
+    uint64_t *key_hi = nullptr, *key_lo = nullptr, *lo_sorted = nullptr;  // n each
+    uint32_t *nown = nullptr;      // n: nodes whose range starts at a position
+    uint32_t *nbase = nullptr;     // n: exclusive scan of nown (pre-order node ids)
+    int32_t *nfirst = nullptr, *ncount = nullptr, *nparent = nullptr;  // cap, by node id
+    uint32_t *nmeta = nullptr;     // cap: level | octant << 8 | leaf << 12
+    uint32_t *occ = nullptr;       // cap: occupied-child octant mask
+    uint32_t *cb = nullptr;        // cap: child counts, then their exclusive scan
+    uint32_t *flags = nullptr;     // cap: summary arrival counters
+    uint32_t *glist = nullptr;     // cap + 64: straddling parents queued by the summaries
+    uint32_t *scan_ws = nullptr;   // 128 + cap / kScanTile
+    OctRec *nodes = nullptr;       // cap, by slot
+    float4 *centre = nullptr;      // cap, by slot: cell centre xyz, side
+    float *thr = nullptr;          // 64: opening threshold per level
+    uint32_t *misc = nullptr;      // 64: [0] node total, [1] child total
+};
+
+// Octree build (Octree::build, Trees/Octree.h:185-205) in three stages, each
+// enqueue-only.  keys: per-body octant path on the reference's float centre
+// recurrence.  sort: stable LSD radix sort by path (ties by original index);
+// idx_alt receives the key order.  topology: cells from the sorted keys,
+// node records, leaf contents reordered by original index into idx (the
+// final body order), leaf_start.  Summaries run after the body gather.
+void launch_octree_keys(const float *x, const float *y, const float *z, int64_t n, const float *bbox6,
+                        int max_depth, OctBuffers &ob, uint64_t *keys, uint32_t *idx, uint32_t *idx_alt,
+                        hipStream_t s);
+void launch_octree_sort(int64_t n, int max_depth, OctBuffers &ob, uint64_t *keys, uint32_t *idx,
+                        uint64_t *keys_alt, uint32_t *idx_alt, uint32_t *sort_status, uint32_t *err,
+                        hipStream_t s);
+void launch_octree_topology(int64_t n, const float *bbox6, int leaf_size, int max_depth, float theta,
+                            OctBuffers &ob, const uint64_t *keys, const uint32_t *idx_alt, uint32_t *idx,
+                            int32_t *leaf_start, uint32_t *err, hipStream_t s);
+void launch_octree_summaries(const float4 *bodies, int64_t n, OctBuffers &ob, const int32_t *leaf_start,
+                             uint32_t *err, hipStream_t s);
+
 void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s);
+
+// Exclusive scan of n u32 values into out (in may equal out); total (if not
+// null) receives the sum.  ws holds 64 + ceil(n / kScanTile) words.
+void launch_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *ws, uint32_t *total, hipStream_t s);
+
+// Stable radix sort of u64 keys + u32 payload over the low 8 * passes bits.
+// Returns true when the result is in the _alt buffers (odd passes).
+bool launch_radix_sort_u64(uint64_t *keys, uint32_t *idx, uint64_t *keys_alt, uint32_t *idx_alt, int64_t n,
+                           int passes, uint32_t *status, uint32_t *err, hipStream_t s);
 
 // outerBoundingBox: writes min xyz, max xyz to out6.  partials holds
 // 6 * kBBoxBlocks floats.
@@ -103,7 +180,9 @@ struct WalkArgs {
     int64_t n;
     const float4 *bodies;   // sorted (x, y, z, m)
     const uint32_t *perm;   // sorted position -> original index
-    const NodeRec *nodes;   // tree nodes
+    const NodeRec *nodes;   // LBVH nodes
+    const OctRec *onodes;   // octree nodes
+    int levels;             // octree: max_depth + 2
     float theta;            // octree MAC
     float theta2;           // LBVH MAC: theta * theta (float)
     double mac_mid;         // LBVH MAC threshold (mac_threshold)
@@ -119,6 +198,9 @@ struct WalkArgs {
 // bit-exact reference arithmetic; diag != 0 makes the FAST walk also count
 // its schedule statistics (counts[2..5]).
 void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, int diag, hipStream_t s);
+
+// Barnes-Hut traversal over the octree (same modes and statistics).
+void launch_walk_oct(const WalkArgs &a, int ordered, int diag, hipStream_t s);
 
 // v = v + a*dt; p = p + v*dt (original order).
 void launch_update(float *x, float *y, float *z, float *vx, float *vy, float *vz, const float *ax,

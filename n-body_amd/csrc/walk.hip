@@ -306,6 +306,293 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     }
 }
 
+// ================================================================ octree
+// SideLengthOverDistance with the octree exclusion region
+// (Descent/DescentCriterion/SideLengthOverDistance.h:17-22,
+// Descent/DescentCriterionType.h:81-85): the distance to the centre of mass
+// in glm's order against the level's threshold (side / sqrt(d2) < theta is
+// monotone in d2), and the target outside [c - side, c + side).
+__device__ __forceinline__ bool oct_accepts(const OctRec &r, float px, float py, float pz) {
+    const float dx = px - r.com.x, dy = py - r.com.y, dz = pz - r.com.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const bool inside = r.lo.x <= px && r.lo.y <= py && r.lo.z <= pz && px < r.hi.x && py < r.hi.y && pz < r.hi.z;
+    return d2 > r.lo.w && !inside;
+}
+
+// ORDERED: libstdc++'s std::transform_reduce over the 8 children of an
+// octree node (activeTree.h:66-76) computes init + ((r0 + r1) + (r2 + r3)),
+// then + ((r4 + r5) + (r6 + r7)), with init = 0 and r = 0 for an empty
+// child.  Per lane and per tree level the walk keeps that bracket's state
+// in LDS (R: the running init, A: the open pair sum, B: the pending third
+// term) and applies octant q's result with oct_step.
+template <int MAXL>
+__device__ __forceinline__ void oct_step(float *st, int q, bool act, float vx, float vy, float vz) {
+    if (!act) return;
+    float *R = st, *A = st + 3 * kWave, *B = st + 6 * kWave;
+    const float v[3] = {vx, vy, vz};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int o = c * kWave;
+        switch (q & 3) {
+        case 0: A[o] = v[c]; break;
+        case 1: A[o] = A[o] + v[c]; break;
+        case 2: B[o] = v[c]; break;
+        default: {
+            const float a = A[o] + (B[o] + v[c]);
+            A[o] = a;
+            R[o] = q == 3 ? 0.0f + a : R[o] + a;
+        }
+        }
+    }
+}
+
+// Evaluates node `nd` for the lanes in O: accepting lanes take its monopole
+// and leaf lanes their leaf sum as this node's result r; returns the lanes
+// that open it when it is internal.
+__device__ __forceinline__ uint64_t oct_eval_ordered(const OctRec &nd, uint64_t O, int lane, float px, float py,
+                                                     float pz, float G, float eps2, const float4 *__restrict__ bodies,
+                                                     int64_t n, float &rx, float &ry, float &rz, uint32_t &cpp,
+                                                     uint32_t &cnp, bool &fail) {
+    const bool act = (O >> lane) & 1ull;
+    const bool pass = act && oct_accepts(nd, px, py, pz);
+    if (pass) {
+        ++cnp;
+        pp_exact(nd.com.x, nd.com.y, nd.com.z, nd.com.w, px, py, pz, G, eps2, rx, ry, rz);
+    }
+    const uint64_t open = O & ~__ballot(pass);
+    if (open == 0) return 0;
+    if (nd.topo.z >= 0) return open;
+    const int first = nd.topo.x, cnt = nd.topo.y;
+    if (first < 0 || cnt < 0 || (int64_t)first + cnt > n) { fail = true; return 0; }
+    if ((open >> lane) & 1ull) {
+        float lx = 0.0f, ly = 0.0f, lz = 0.0f;
+        for (int j = 0; j < cnt; ++j) {
+            const float4 b = bodies[first + j];
+            float vx, vy, vz;
+            pp_exact(b.x, b.y, b.z, b.w, px, py, pz, G, eps2, vx, vy, vz);
+            lx = lx + vx; ly = ly + vy; lz = lz + vz;
+        }
+        cpp += (uint32_t)cnt;
+        rx = lx; ry = ly; rz = lz;
+    }
+    return 0;
+}
+
+template <int MAXL>
+__global__ __launch_bounds__(64) void k_walk_oct_ordered(WalkArgs a) {
+    __shared__ int s_node[MAXL];
+    __shared__ int s_pos[MAXL];          // next child index | next octant << 8
+    __shared__ uint64_t s_mask[MAXL];
+    __shared__ float s_st[MAXL * 9 * kWave];
+
+    const float4 *__restrict__ bodies = a.bodies;
+    const OctRec *__restrict__ nodes = a.onodes;
+    const int lane = lane_id();
+    const int64_t tgt = xcd_block(blockIdx.x, gridDim.x) * kWave + lane;
+    const bool valid = tgt < a.n;
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0) return;
+
+    const float4 me = valid ? bodies[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float px = me.x, py = me.y, pz = me.z;
+    const float G = a.G, eps2 = a.eps2;
+    float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+    uint32_t cpp = 0, cnp = 0;
+    bool fail = false;
+
+    int sp = 0;
+    {
+        const OctRec root = nodes[0];
+        const uint64_t O = oct_eval_ordered(root, vmask, lane, px, py, pz, G, eps2, bodies, a.n, rx, ry, rz, cpp, cnp,
+                                            fail);
+        if (O) {
+            if (lane == 0) { s_node[0] = 0; s_pos[0] = 0; s_mask[0] = O; }
+            sp = 1;
+        }
+    }
+    // each frame is visited nchild + 1 <= 9 times, each node opened once
+    const int64_t max_iters = 9 * a.max_nodes + 64;
+    int64_t iters = 0;
+    while (sp > 0 && !fail) {
+        if (++iters > max_iters) { fail = true; break; }
+        __builtin_amdgcn_wave_barrier();
+        const int top = sp - 1;
+        const int X = uniform_i32(s_node[top]);
+        const int pos = uniform_i32(s_pos[top]);
+        const uint64_t O = uniform_u64(s_mask[top]);
+        const bool act = (O >> lane) & 1ull;
+        if (X < 0 || X >= a.max_nodes) { fail = true; break; }
+        const OctRec xr = nodes[X];
+        const int cb = xr.topo.z, nch = oct_nchild(xr);
+        if (cb < 0 || nch < 1 || nch > 8 || (int64_t)cb + nch > a.max_nodes) { fail = true; break; }
+        float *st = s_st + top * 9 * kWave + lane;
+        int j = pos & 0xFF, q = pos >> 8;
+        if (j > 0) {
+            const int k = oct_octant(nodes[cb + j - 1]);
+            for (; q < k; ++q) oct_step<MAXL>(st, q, act, 0.0f, 0.0f, 0.0f);
+            oct_step<MAXL>(st, k, act, rx, ry, rz);
+            q = k + 1;
+        }
+        if (j == nch) {
+            for (; q < 8; ++q) oct_step<MAXL>(st, q, act, 0.0f, 0.0f, 0.0f);
+            if (act) { rx = st[0]; ry = st[kWave]; rz = st[2 * kWave]; }
+            sp = top;
+            continue;
+        }
+        if (lane == 0) s_pos[top] = (j + 1) | (q << 8);
+        const OctRec cr = nodes[cb + j];
+        const uint64_t open = oct_eval_ordered(cr, O, lane, px, py, pz, G, eps2, bodies, a.n, rx, ry, rz, cpp, cnp,
+                                               fail);
+        if (open) {
+            if (sp >= MAXL) { fail = true; break; }
+            if (lane == 0) { s_node[sp] = cb + j; s_pos[sp] = 0; s_mask[sp] = open; }
+            ++sp;
+        }
+    }
+    if (fail && lane == 0) atomicOr(a.err, 2u);
+    if (valid) {
+        const uint32_t o = a.perm[tgt];
+        if ((int64_t)o < a.n) { a.ax[o] = rx; a.ay[o] = ry; a.az[o] = rz; }
+    }
+    const uint32_t wpp = wave_sum(valid ? cpp : 0u), wnp = wave_sum(valid ? cnp : 0u);
+    if (lane == 0) {
+        atomicAdd(&a.counts[0], (unsigned long long)wpp);
+        atomicAdd(&a.counts[1], (unsigned long long)wnp);
+    }
+}
+
+// FAST over the octree: the children of an opened node are contiguous and
+// visited in one group; opened internal children are pushed as groups.
+template <bool SOFT, int STATS>
+__device__ __forceinline__ uint64_t visit_oct(const OctRec &nd, uint64_t mask, const float4 *__restrict__ bodies,
+                                              int64_t n, float eps2, FastWave &W, bool &fail) {
+    const uint64_t acc = __ballot(oct_accepts(nd, W.px, W.py, W.pz)) & mask;
+    if (STATS >= 2) { W.visits += 1; W.lanes += __popcll(mask); }
+    if (acc) {
+        if (STATS >= 1) W.np += __popcll(acc);
+        if (__builtin_amdgcn_inverse_ballot_w64(acc)) interact<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, eps2, W);
+    }
+    const uint64_t open = mask & ~acc;
+    if (open == 0 || nd.topo.z >= 0) return open;
+    const int first = nd.topo.x, cnt = nd.topo.y;
+    if (first < 0 || cnt < 0 || (int64_t)first + cnt > n) { fail = true; return 0; }
+    if (STATS >= 2) { W.leaf_visits += 1; W.leaf_lanes += __popcll(open); }
+    if (STATS >= 1) W.pp += (uint64_t)__popcll(open) * (uint64_t)cnt;
+    leaf_bodies<SOFT>(bodies, first, cnt, open, eps2, W);
+    return 0;
+}
+
+constexpr int kStackOct = 256;
+
+template <int WAVES, bool SOFT, int STATS>
+__global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
+    __shared__ int s_grp[WAVES][kStackOct];       // first child | nchild << 28
+    __shared__ uint64_t s_mask[WAVES][kStackOct];
+
+    const float4 *__restrict__ bodies = a.bodies;
+    const OctRec *__restrict__ nodes = a.onodes;
+    const int lane = lane_id(), wv = threadIdx.x / kWave;
+    const int64_t tgt = (xcd_block(blockIdx.x, gridDim.x) * WAVES + wv) * kWave + lane;
+    const bool valid = tgt < a.n;
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0) return;
+
+    FastWave W;
+    const float4 me = valid ? bodies[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    W.px = me.x; W.py = me.y; W.pz = me.z;
+    W.rx = 0.0f; W.ry = 0.0f; W.rz = 0.0f;
+    W.pp = 0; W.np = 0;
+    W.visits = 0; W.lanes = 0; W.leaf_visits = 0; W.leaf_lanes = 0;
+    asm("v_mov_b32 %0, %1" : "=v"(W.G) : "s"(a.G));
+    const float eps2 = a.eps2;
+    const int64_t n = a.n;
+    bool fail = false;
+
+    const OctRec root = nodes[0];
+    uint64_t mask = visit_oct<SOFT, STATS>(root, vmask, bodies, n, eps2, W, fail);
+    int c = root.topo.z, nch = oct_nchild(root);
+    int sp = 0;
+    const int64_t max_iters = a.max_nodes + 64;  // each node's group is visited at most once
+    int64_t iters = 0;
+    while (!fail) {
+        if (mask == 0) {
+            if (sp == 0) break;
+            --sp;
+            __builtin_amdgcn_wave_barrier();
+            const int g = uniform_i32(s_grp[wv][sp]);
+            c = g & 0x0FFFFFFF;
+            nch = (int)((uint32_t)g >> 28);
+            mask = uniform_u64(s_mask[wv][sp]);
+        }
+        c = uniform_i32(c);
+        nch = uniform_i32(nch);
+        if (++iters > max_iters || c < 0 || nch < 1 || nch > 8 || (int64_t)c + nch > a.max_nodes) {
+            fail = true;
+            break;
+        }
+        uint64_t nm = 0;
+        int nc = 0, nn = 0;
+        for (int k = 0; k < nch; ++k) {
+            const OctRec r = nodes[c + k];
+            const uint64_t o = visit_oct<SOFT, STATS>(r, mask, bodies, n, eps2, W, fail);
+            if (o) {
+                if (nm) {
+                    if (sp >= kStackOct) { fail = true; break; }
+                    if (lane == 0) { s_grp[wv][sp] = nc | (nn << 28); s_mask[wv][sp] = nm; }
+                    __builtin_amdgcn_wave_barrier();
+                    ++sp;
+                }
+                nm = o;
+                nc = r.topo.z;
+                nn = oct_nchild(r);
+            }
+        }
+        mask = nm;
+        c = nc;
+        nch = nn;
+    }
+    if (fail && lane == 0) atomicOr(a.err, 2u);
+    if (valid) {
+        const uint32_t o = a.perm[tgt];
+        if ((int64_t)o < n) { a.ax[o] = W.rx; a.ay[o] = W.ry; a.az[o] = W.rz; }
+    }
+    if (lane == 0) {
+        atomicAdd(&a.counts[0], (unsigned long long)W.pp);
+        atomicAdd(&a.counts[1], (unsigned long long)W.np);
+        if (STATS >= 2) {
+            atomicAdd(&a.counts[2], (unsigned long long)W.visits);
+            atomicAdd(&a.counts[3], (unsigned long long)W.lanes);
+            atomicAdd(&a.counts[4], (unsigned long long)W.leaf_visits);
+            atomicAdd(&a.counts[5], (unsigned long long)W.leaf_lanes);
+        }
+    }
+}
+
+template <int STATS>
+static void launch_oct_fast(const WalkArgs &a, hipStream_t s) {
+    constexpr int W = 4;
+    const unsigned blocks = (unsigned)((a.n + 64 * W - 1) / (64 * W));
+    if (a.eps2 > 0.0f)
+        hipLaunchKernelGGL((k_walk_oct_fast<W, true, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_walk_oct_fast<W, false, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
+}
+
+void launch_walk_oct(const WalkArgs &a, int ordered, int stats, hipStream_t s) {
+    if (a.n <= 0) return;
+    if (!ordered) {
+        if (stats >= 2) launch_oct_fast<2>(a, s);
+        else if (stats == 1) launch_oct_fast<1>(a, s);
+        else launch_oct_fast<0>(a, s);
+    } else {
+        const unsigned blocks = (unsigned)((a.n + 63) / 64);
+        if (a.levels <= 18)
+            hipLaunchKernelGGL((k_walk_oct_ordered<18>), dim3(blocks), dim3(64), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_walk_oct_ordered<kOctLevels>), dim3(blocks), dim3(64), 0, s, a);
+    }
+}
+
 template <int STATS>
 static void launch_fast(const WalkArgs &a, hipStream_t s) {
     constexpr int W = 4;
