@@ -2,15 +2,17 @@
 """Benchmark of the MI355X tree-code force pass.
 
 Workload (default): Plummer sphere, N = 1,048,576 (a = 1, M = 1, G = 1,
-r < 10 a), FP32, eps = 0.005 (eps2 = 2.5e-5), theta = 0.5, leaf <= 16, 30-bit
-Morton keys over the bounding cube, leapfrog dt = 5e-4, seed 1, linear-BVH
-solver.  A "step" is Solver::step(): tree build + walk + kick/drift, all on
-the device with the state resident in HBM.
+r < 10 a), FP32, eps = 0.005 (eps2 = 2.5e-5), theta = 0.5, leaf <= 16,
+leapfrog dt = 5e-4, seed 1.  A "step" is Solver::step(): tree build + walk +
+kick/drift, all on the device with the state resident in HBM.
 
-Prints one JSON line: bodies*steps/s over the timed steps, interactions/s,
-the walk kernel's FP32 roofline fraction (HIP events on the system stream),
-the oracle CPU baseline timed on this host, force error against an FP64
-direct sum on a 16,384-body sample and the energy drift over the run.
+The headline line is the linear-BVH solver with 30-bit Morton keys over the
+bounding cube; the same run measures the octree Barnes-Hut solver on the
+same bodies and reports it in the "barnes_hut" block.  Each solver carries:
+bodies*steps/s over the timed steps, interactions/s, the walk kernel's FP32
+roofline fraction (HIP events on the system stream), the force error against
+an FP64 direct sum on a 16,384-body sample, the energy drift over the run and
+the oracle's CPU time for the same solver on this host.
 """
 
 import argparse
@@ -28,91 +30,81 @@ sys.path.insert(0, ROOT)
 METRIC = "bodies·steps/s at N=1M θ=0.5; batched-serving p50/p99 latency at fixed force err"
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (packed) peak
 FLOPS_PER_INTERACTION = 20
+SOLVER_NAMES = {0: "barnes-hut", 1: "linear-bvh"}
+WALK_KERNELS = {0: "k_walk_oct_fast", 1: "k_walk_lbvh_fast"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "walk_traffic_plummer1M.txt")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(b, args, threads):
-    """The oracle's restatement of the reference CPU path, timed on this host:
-    a full LBVH build plus the walk for a target sample, scaled to N."""
+def load_traffic():
+    """HBM bytes per walk launch measured with rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE (scaled by a copy kernel of known size), kept as text beside
+    the profiles; keyed by kernel name."""
+    out = {}
+    if not os.path.exists(TRAFFIC_FILE):
+        return out
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 2 and not line.startswith("#"):
+                try:
+                    out[parts[0]] = float(parts[1])
+                except ValueError:
+                    pass
+    return out
+
+
+def cpu_baseline(b, args, solver, threads):
+    """The oracle's restatement of the reference CPU path for `solver`, timed
+    on this host: a full tree build plus the walk for a target sample,
+    scaled to N, plus the kick/drift passes."""
     import oracle
 
     n = b.n
     t0 = time.perf_counter()
-    t = oracle.Tree(1, b.x, b.y, b.z, b.m, theta=args.theta, leaf=args.leaf, bits=args.bits, eps2=args.eps2)
+    t = oracle.Tree(solver, b.x, b.y, b.z, b.m, theta=args.theta, leaf=args.leaf, bits=args.bits, eps2=args.eps2)
     t_build = time.perf_counter() - t0
     sample = min(n, args.cpu_sample)
     sel = np.linspace(0, n - 1, sample).astype(np.int64)
     t0 = time.perf_counter()
     t.accel(b.x[sel], b.y[sel], b.z[sel], threads=threads)
     t_walk = (time.perf_counter() - t0) * (n / sample)
-    t_update = 0.0
     x = b.x.copy()
     t0 = time.perf_counter()
     x += b.vx * np.float32(args.dt)
     t_update = (time.perf_counter() - t0) * 6  # v and p, 3 components each
+    del t
     step_s = t_build + t_walk + t_update
     return {
         "value": n / step_s,
         "unit": "bodies·steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"one LBVH build over all {n} bodies + the walk for {sample} evenly spaced targets, "
-                  f"scaled by N/sample (build {t_build:.2f} s, walk {t_walk:.2f} s extrapolated)",
+        "sample": f"oracle {SOLVER_NAMES[solver]}: one build over all {n} bodies + the walk for {sample} evenly "
+                  f"spaced targets, scaled by N/sample (build {t_build:.2f} s, walk {t_walk:.2f} s extrapolated)",
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--theta", type=float, default=0.5)
-    ap.add_argument("--leaf", type=int, default=16)
-    ap.add_argument("--bits", type=int, default=30)
-    ap.add_argument("--eps2", type=float, default=2.5e-5)
-    ap.add_argument("--dt", type=float, default=5e-4)
-    ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--mode", choices=["fast", "ordered"], default="fast")
-    ap.add_argument("--cpu-sample", type=int, default=131072)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-accuracy", action="store_true")
-    args = ap.parse_args()
-
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (systems shard by body "
-              "or by system with no communication; see DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
-        sys.exit(2)
-
-    nbg = importlib.import_module("n-body_amd")
-    if nbg.device_count() == 0:
-        raise SystemExit("no GPU visible")
-    ctx = nbg.Context(0)
-    log(f"[bench] generating Plummer N={args.n} seed={args.seed}")
-    b = nbg.plummer(args.n, seed=args.seed)
-
+def measure(nbg, ctx, b, args, solver):
     p = nbg.default_params()
-    p.solver = nbg.LINEAR_BVH
+    p.solver = solver
     p.theta, p.leaf_size, p.morton_bits = args.theta, args.leaf, args.bits
     p.eps2, p.dt = args.eps2, args.dt
     p.sum_mode = nbg.SUM_FAST if args.mode == "fast" else nbg.SUM_ORDERED
     s = nbg.System(ctx, p, args.n)
     s.set_state(b)
+    a0 = s.update_accelerations()  # force pass at t = 0
 
-    # force pass at t = 0 (also the energy diagnostic's reference point)
-    a0 = s.update_accelerations()
-    pp0, np0 = s.counts()
-
-    log(f"[bench] warmup {args.warmup} steps")
+    log(f"[bench] {SOLVER_NAMES[solver]}: warmup {args.warmup} steps")
     s.step(args.warmup)
     s.synchronize()
     s.set_state(b)  # timed run starts from the initial conditions
     s.enable_counts(False)  # production steps do not count interactions
     s.enable_timing(True)
-    log(f"[bench] timing {args.steps} steps")
+    log(f"[bench] {SOLVER_NAMES[solver]}: timing {args.steps} steps")
     s.synchronize()
     t0 = time.perf_counter()
     s.step(args.steps)
@@ -130,89 +122,158 @@ def main():
     s.enable_walk_stats(False)
 
     ms_per_step = 1e3 * elapsed / args.steps
-    value = args.n * args.steps / elapsed
     inter = pp + npc
     walk_ms = phases["walk"]
     achieved = FLOPS_PER_INTERACTION * inter / (walk_ms * 1e-3) / 1e12
+    waves = max(1, (args.n + 63) // 64)
+    traffic = load_traffic().get(WALK_KERNELS[solver])
     out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "bodies·steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "value": args.n * args.steps / elapsed,
         "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic",
-        "config": {
-            "workload": f"plummer-{args.n} linear-bvh step (theta={args.theta}, leaf={args.leaf}, "
-                        f"{args.bits}-bit keys, eps2={args.eps2}, dt={args.dt}, seed={args.seed})",
-            "n_bodies": args.n, "theta": args.theta, "leaf_size": args.leaf, "morton_bits": args.bits,
-            "eps2": args.eps2, "dt": args.dt, "sum_mode": args.mode, "solver": "linear-bvh",
-        },
         "interactions_per_step": inter,
         "interactions_per_s": inter / (ms_per_step * 1e-3),
         "phases_ms": phases,
         "walk_schedule": {
-            "node_visits_per_wave": ws["node_visits"] / max(1, (args.n + 63) // 64),
+            "node_visits_per_wave": ws["node_visits"] / waves,
             "lane_utilisation_nodes": ws["active_lanes"] / max(1, 64 * ws["node_visits"]),
-            "leaf_visits_per_wave": ws["leaf_visits"] / max(1, (args.n + 63) // 64),
+            "leaf_visits_per_wave": ws["leaf_visits"] / waves,
             "lane_utilisation_leaves": ws["leaf_lanes"] / max(1, 64 * ws["leaf_visits"]),
         } if ws["node_visits"] else None,
         "roofline": {
             "bound": "valu",
-            "kernel": "k_walk_lbvh",
+            "kernel": WALK_KERNELS[solver],
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
             "note": f"{FLOPS_PER_INTERACTION} flops per interaction (pp + np, SimpleTrackingRule counting) "
-                    f"/ mean walk-kernel time from HIP events on the system stream",
+                    f"/ mean walk-kernel time from HIP events on the system stream; traffic = HBM bytes per "
+                    f"walk launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated",
         },
     }
-
     if not args.no_accuracy:
-        log("[bench] accuracy: FP64 direct sum on 16384 bodies, energy drift")
+        log(f"[bench] {SOLVER_NAMES[solver]}: accuracy (FP64 direct sum on 16384 bodies, energy drift)")
         st = s.get_state()
         # synchronised end state: v(t_N) = v_{N-1/2} + a(x_N) dt / 2
         end = nbg.Bodies(st.x, st.y, st.z, b.m, st.vx, st.vy, st.vz)
         s2 = nbg.System(ctx, p, args.n)
         s2.set_state(end)
         aN = s2.update_accelerations()
-        s0 = nbg.System(ctx, p, args.n)
-        s0.set_state(b)
-        W0 = s0.potential_f64()
         WN = s2.potential_f64()
         m = b.m.astype(np.float64)
-        K0 = 0.5 * np.sum(m * (b.vx.astype(np.float64) ** 2 + b.vy.astype(np.float64) ** 2 + b.vz.astype(np.float64) ** 2))
+        K0 = 0.5 * np.sum(m * (b.vx.astype(np.float64) ** 2 + b.vy.astype(np.float64) ** 2
+                               + b.vz.astype(np.float64) ** 2))
         vs = np.stack([end.vx, end.vy, end.vz], 1).astype(np.float64) + aN.astype(np.float64) * (args.dt / 2)
         KN = 0.5 * np.sum(m * (vs ** 2).sum(1))
-        E0, EN = K0 + W0, KN + WN
-        sub = np.random.default_rng(0).choice(args.n, min(16384, args.n), replace=False)
-        d = ctx.direct_f64(b, b.x[sub], b.y[sub], b.z[sub], 1.0, args.eps2)
-        rel = np.linalg.norm(a0[sub] - d, axis=1) / np.linalg.norm(d, axis=1)
+        E0, EN = K0 + args.W0, KN + WN
+        rel = np.linalg.norm(a0[args.sub] - args.direct, axis=1) / np.linalg.norm(args.direct, axis=1)
         out["accuracy"] = {
             "force_err_vs_fp64_median": float(np.median(rel)),
             "force_err_vs_fp64_p99": float(np.quantile(rel, 0.99)),
             "force_err_vs_fp64_max": float(rel.max()),
-            "sample": int(len(sub)),
+            "sample": int(len(args.sub)),
             "energy_drift": float(abs(EN - E0) / abs(E0)),
             "E0": float(E0), "E_end": float(EN),
         }
-        s0.close()
         s2.close()
-
-    if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-        log(f"[bench] CPU baseline with {threads} threads")
-        cb = cpu_baseline(b, args, threads)
-        out["cpu_baseline"] = cb
-        out["gpu_over_cpu"] = value / cb["value"]
     s.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--leaf", type=int, default=16)
+    ap.add_argument("--bits", type=int, default=30)
+    ap.add_argument("--eps2", type=float, default=2.5e-5)
+    ap.add_argument("--dt", type=float, default=5e-4)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--mode", choices=["fast", "ordered"], default="fast")
+    ap.add_argument("--solvers", default="linear-bvh,barnes-hut",
+                    help="comma list; the first is the headline line")
+    ap.add_argument("--cpu-sample", type=int, default=131072)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device-to-device tensor copy three times (the known-size kernel "
+                         "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
+    args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": src.numel() * 4, "bytes_written": dst.numel() * 4}))
+        return
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (systems shard by body "
+              "or by system with no communication; see DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    ids = {v: k for k, v in SOLVER_NAMES.items()}
+    solvers = [ids[x.strip()] for x in args.solvers.split(",") if x.strip()]
+
+    nbg = importlib.import_module("n-body_amd")
+    if nbg.device_count() == 0:
+        raise SystemExit("no GPU visible")
+    ctx = nbg.Context(0)
+    log(f"[bench] generating Plummer N={args.n} seed={args.seed}")
+    b = nbg.plummer(args.n, seed=args.seed)
+    if not args.no_accuracy:
+        log("[bench] FP64 references: direct sum on a 16384-body sample, initial potential")
+        args.sub = np.random.default_rng(0).choice(args.n, min(16384, args.n), replace=False)
+        args.direct = ctx.direct_f64(b, b.x[args.sub], b.y[args.sub], b.z[args.sub], 1.0, args.eps2)
+        p0 = nbg.default_params()
+        p0.eps2 = args.eps2
+        s0 = nbg.System(ctx, p0, args.n)
+        s0.set_state(b)
+        args.W0 = s0.potential_f64()
+        s0.close()
+
+    res = {sv: measure(nbg, ctx, b, args, sv) for sv in solvers}
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    if not args.no_cpu:
+        for sv in solvers:
+            log(f"[bench] CPU baseline ({SOLVER_NAMES[sv]}) with {threads} threads")
+            cb = cpu_baseline(b, args, sv, threads)
+            res[sv]["cpu_baseline"] = cb
+            res[sv]["gpu_over_cpu"] = res[sv]["value"] / cb["value"]
+
+    head = res[solvers[0]]
+    out = {
+        "metric": METRIC,
+        "value": head["value"],
+        "unit": "bodies·steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plummer-{args.n} {SOLVER_NAMES[solvers[0]]} step (theta={args.theta}, leaf={args.leaf}, "
+                        f"{args.bits}-bit keys, eps2={args.eps2}, dt={args.dt}, seed={args.seed})",
+            "n_bodies": args.n, "theta": args.theta, "leaf_size": args.leaf, "morton_bits": args.bits,
+            "eps2": args.eps2, "dt": args.dt, "sum_mode": args.mode, "solver": SOLVER_NAMES[solvers[0]],
+        },
+    }
+    for k, v in head.items():
+        if k not in ("value", "ms_per_step"):
+            out[k] = v
+    for sv in solvers[1:]:
+        out[SOLVER_NAMES[sv].replace("-", "_")] = res[sv]
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -49,6 +49,7 @@ def lib() -> ctypes.CDLL:
             "orc_tree_num_nodes": (ctypes.c_int64, [_vp]),
             "orc_tree_max_depth": (ctypes.c_int, [_vp]),
             "orc_tree_nodes": (ctypes.c_int, [_vp, _i64p, _i64p, _i32p, _i32p, fp, fp]),
+            "orc_tree_parents": (ctypes.c_int, [_vp, _i32p]),
             "orc_tree_order": (ctypes.c_int, [_vp, _u32p, _u64p]),
             "orc_tree_bbox": (ctypes.c_int, [_vp, fp]),
             "orc_tree_accel": (ctypes.c_int, [_vp, ctypes.c_int64, fp, fp, fp, fp, fp, fp, _u64p, _u64p,
@@ -136,7 +137,10 @@ class Tree:
         geo6 = np.zeros((k, 6), np.float32)
         lib().orc_tree_nodes(self.h, _p(first, _i64p), _p(count, _i64p), _p(leaf, _i32p), _p(depth, _i32p),
                              _p(com4), _p(geo6))
-        return dict(first=first, count=count, leaf=leaf.astype(bool), depth=depth, com4=com4, geo6=geo6)
+        parent = np.zeros(k, np.int32)
+        lib().orc_tree_parents(self.h, _p(parent, _i32p))
+        return dict(first=first, count=count, leaf=leaf.astype(bool), depth=depth, com4=com4, geo6=geo6,
+                    parent=parent)
 
     def max_depth(self):
         return lib().orc_tree_max_depth(self.h)

@@ -561,6 +561,24 @@ int orc_tree_nodes(void *hp, int64_t *first, int64_t *count, int32_t *leaf, int3
     return 0;
 }
 
+// Parent index of every node (-1 for the root), from the child links.
+int orc_tree_parents(void *hp, int32_t *parent) {
+    Handle *h = (Handle *)hp;
+    if (h->kind == 1) {
+        const auto &N = h->l->nodes;
+        for (size_t i = 0; i < N.size(); ++i) parent[i] = -1;
+        for (size_t i = 0; i < N.size(); ++i)
+            if (!N[i].leaf) { parent[N[i].child0] = (int32_t)i; parent[N[i].child1] = (int32_t)i; }
+    } else {
+        const auto &N = h->o->nodes;
+        for (size_t i = 0; i < N.size(); ++i) parent[i] = -1;
+        for (size_t i = 0; i < N.size(); ++i)
+            if (!N[i].leaf)
+                for (int k = 0; k < 8; ++k) parent[N[i].child[k]] = (int32_t)i;
+    }
+    return 0;
+}
+
 // Body order of the tree (sorted position -> original index) and, for the
 // LBVH, the sorted keys.
 int orc_tree_order(void *hp, uint32_t *perm, uint64_t *keys_or_null) {

@@ -250,6 +250,72 @@ def test_lbvh_walk_bruteforce_bitexact(orc, nbg, n, leaf, theta):
         assert np.array_equal(acc[i], ref), f"body {i}"
 
 
+def _py_walk_octree(nd, pos, mass, p, theta, G, eps2):
+    """Brute-force restatement of activeTree over the octree in numpy float32
+    from the reference text (activeTree.h:24-79 with libstdc++'s 4-way
+    transform_reduce grouping over 8 children, SideLengthOverDistance.h:17-22,
+    DescentCriterionType.h:81-85, BoundingBox.h:61-64), independent of the
+    oracle's walk; it reads only the oracle's tree."""
+    f32 = np.float32
+    first, count, leaf, com4, geo, parent = (nd["first"], nd["count"], nd["leaf"], nd["com4"], nd["geo6"],
+                                            nd["parent"])
+    kids = {}
+    for i, q in enumerate(parent.tolist()):
+        if q >= 0:
+            kids.setdefault(q, []).append(i)
+    z = np.zeros(3, f32)
+
+    def pp(a, m, q):
+        if a[0] == q[0] and a[1] == q[1] and a[2] == q[2]:
+            return z.copy()
+        R = (q - a).astype(f32)
+        r = f32(np.sqrt(f32(f32(f32(R[0] * R[0]) + f32(R[1] * R[1])) + f32(R[2] * R[2])) + f32(eps2)))
+        r3 = f32(r * f32(r * r))
+        return (((-R) * f32(m)).astype(f32) * f32(G)).astype(f32) / r3
+
+    def add(a, b):
+        return (a + b).astype(f32)
+
+    def walk(i):
+        if count[i] == 0:
+            return z.copy()
+        c, side = geo[i, :3], f32(geo[i, 3])
+        R = (p - com4[i, :3]).astype(f32)
+        dist = f32(np.sqrt(f32(f32(f32(R[0] * R[0]) + f32(R[1] * R[1])) + f32(R[2] * R[2]))))
+        lo, hi = (c - side).astype(f32), (c + side).astype(f32)
+        inside = bool(np.all(lo <= p) and np.all(p < hi))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            if f32(side / dist) < f32(theta) and not inside:
+                return pp(com4[i, :3], com4[i, 3], p)
+        if leaf[i]:
+            acc = z.copy()
+            for j in range(first[i], first[i] + count[i]):
+                acc = add(acc, pp(pos[j], mass[j], p))
+            return acc
+        ks = kids[i]
+        assert len(ks) == 8
+        r = [walk(k) for k in ks]
+        init = add(z, add(add(r[0], r[1]), add(r[2], r[3])))
+        return add(init, add(add(r[4], r[5]), add(r[6], r[7])))
+
+    return walk(0)
+
+
+@pytest.mark.parametrize("n,leaf,theta", [(3, 1, 0.5), (64, 4, 0.5), (64, 1, 0.9), (300, 8, 0.3), (300, 2, 1.5)])
+def test_octree_walk_bruteforce_bitexact(orc, nbg, n, leaf, theta):
+    b = nbg.plummer(n, seed=5 + n)
+    t = orc.Tree(0, b.x, b.y, b.z, b.m, theta=theta, leaf=leaf, eps2=1e-4)
+    _, perm = t.order()
+    nd = t.nodes()
+    pos = np.stack([b.x, b.y, b.z], 1)
+    acc, pp, npc = t.accel(b.x, b.y, b.z, counts=True)
+    if theta >= 0.9:
+        assert npc.sum() > 0, "the case must exercise accepted nodes"
+    for i in range(n):
+        ref = _py_walk_octree(nd, pos[perm], b.m[perm], pos[i], theta, 1.0, 1e-4)
+        assert np.array_equal(acc[i], ref), f"body {i}"
+
+
 @pytest.mark.parametrize("solver", [0, 1])
 def test_walk_theta_zero_is_direct_sum(orc, nbg, solver):
     """theta = 0 never accepts a node, so every body meets every body: the

@@ -80,18 +80,26 @@ def preorder_parents(depth: np.ndarray) -> np.ndarray:
 
 
 def octree_check(pos: np.ndarray, perm: np.ndarray, nodes: dict, L: int, max_depth: int):
-    """pos: original-order positions; perm: leaf body order; nodes: pre-order."""
+    """pos: original-order positions; perm: leaf body order; nodes: with
+    explicit parent links (index 0 is the root)."""
     depth = nodes["depth"]
-    parent = preorder_parents(depth)
+    parent = nodes["parent"]
     first, count, leaf, geo = nodes["first"], nodes["count"], nodes["leaf"], nodes["geo6"]
     n = len(pos)
-    assert count[0] == n and depth[0] == 0
+    assert count[0] == n and depth[0] == 0 and parent[0] == -1
+    nkids = np.bincount(parent[1:], minlength=len(depth))
     for i in range(len(depth)):
         d, c = int(depth[i]), int(count[i])
         if d <= 1:
             assert not leaf[i], "levels 0 and 1 are always split"
         elif c > 0:
             assert bool(leaf[i]) == (not (c > L and d < 2 + max_depth)), f"split rule at node {i}"
+        # a split creates all 8 children, empty ones included (Trees/Octree.h:100-147)
+        assert nkids[i] == (0 if leaf[i] else 8), f"node {i} has {nkids[i]} children"
+        if i > 0:
+            p = int(parent[i])
+            assert depth[p] == d - 1, f"depth of node {i}"
+            assert geo[i, 3] == np.float32(geo[p, 3] / np.float32(2)), f"side of node {i}"
         if i > 0 and c > 0:
             pc = geo[parent[i], :3]
             cc = geo[i, :3]
@@ -100,6 +108,10 @@ def octree_check(pos: np.ndarray, perm: np.ndarray, nodes: dict, L: int, max_dep
                 lower = cc[dim] < pc[dim]
                 side = body[:, dim] < pc[dim]
                 assert np.all(side == lower), f"node {i}: body on the wrong side of its parent's plane"
+    # a parent's count is the sum of its children's
+    sums = np.bincount(parent[1:], weights=count[1:], minlength=len(depth))
+    inner = ~leaf.astype(bool)
+    assert np.array_equal(sums[inner].astype(np.int64), count[inner]), "child counts must sum to the parent's"
     # leaves partition the bodies
     lc = count[leaf.astype(bool)].sum()
     assert lc == n, "leaves must hold every body once"
