@@ -113,6 +113,17 @@ int nbg_sys_update_accelerations(nbg_sys *sys, float *ax, float *ay, float *az);
  * Physics/Acceleration.h:16-42). */
 int nbg_sys_get_accelerations(nbg_sys *sys, float *ax, float *ay, float *az);
 
+/* Accelerations at nt passive points (bodies without Mass): the reference
+ * builds its tree from the actives, view<const Position, const Mass>, and walks
+ * every body with an Acceleration, view<const Position, Acceleration>
+ * (Solvers/ActiveTreeSolver.h:33-72; Simulation.cpp:208-211 gives massless
+ * scenario bodies no Mass).  Uses the tree of the system's current state
+ * (building it first if the state or parameters changed since the last force
+ * pass) and writes ax/ay/az in the targets' order.  Counts (nbg_sys_counts)
+ * then describe this walk.  Host arrays; blocking. */
+int nbg_sys_accelerations_at(nbg_sys *sys, int64_t nt, const float *tx, const float *ty, const float *tz,
+                             float *ax, float *ay, float *az);
+
 /* Solver::step() repeated nsteps times (Simulation/Solver.h:103-139):
  * updateAccelerations, then v = v + a*dt, then p = p + v*dt, on the device,
  * with no host copies.  Asynchronous: returns after enqueueing. */

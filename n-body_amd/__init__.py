@@ -34,7 +34,7 @@ EXPORTS = (
     "nbg_params_default", "nbg_last_error", "nbg_version", "nbg_device_count",
     "nbg_ctx_create", "nbg_ctx_destroy", "nbg_sys_create", "nbg_sys_destroy",
     "nbg_sys_set_params", "nbg_sys_get_params", "nbg_sys_set_state", "nbg_sys_get_state",
-    "nbg_sys_update_accelerations", "nbg_sys_get_accelerations", "nbg_sys_step",
+    "nbg_sys_update_accelerations", "nbg_sys_get_accelerations", "nbg_sys_accelerations_at", "nbg_sys_step",
     "nbg_sys_synchronize", "nbg_sys_counts", "nbg_sys_enable_counts", "nbg_sys_enable_walk_stats",
     "nbg_sys_walk_stats",
     "nbg_sys_enable_timing", "nbg_sys_phase_times",
@@ -100,6 +100,7 @@ def lib() -> ctypes.CDLL:
             "nbg_sys_get_state": (ctypes.c_int, [_vp] + [_f32p] * 6),
             "nbg_sys_update_accelerations": (ctypes.c_int, [_vp] + [_f32p] * 3),
             "nbg_sys_get_accelerations": (ctypes.c_int, [_vp] + [_f32p] * 3),
+            "nbg_sys_accelerations_at": (ctypes.c_int, [_vp, ctypes.c_int64] + [_f32p] * 6),
             "nbg_sys_step": (ctypes.c_int, [_vp, ctypes.c_int32]),
             "nbg_sys_synchronize": (ctypes.c_int, [_vp]),
             "nbg_sys_counts": (ctypes.c_int, [_vp, _u64p, _u64p]),
@@ -271,6 +272,15 @@ class System:
     def accelerations(self) -> np.ndarray:
         a = [np.zeros(self.n, np.float32) for _ in range(3)]
         _check(lib().nbg_sys_get_accelerations(self.handle, *[_ptr(v) for v in a]))
+        return np.stack(a, axis=1)
+
+    def accelerations_at(self, tx, ty, tz) -> np.ndarray:
+        """Accelerations at passive points (bodies without Mass) from the
+        tree of the current state."""
+        t = [_f32(v) for v in (tx, ty, tz)]
+        nt = len(t[0])
+        a = [np.zeros(nt, np.float32) for _ in range(3)]
+        _check(lib().nbg_sys_accelerations_at(self.handle, nt, *[_ptr(v) for v in t], *[_ptr(v) for v in a]))
         return np.stack(a, axis=1)
 
     def step(self, nsteps: int = 1):

@@ -50,6 +50,7 @@ struct nbg_sys {
     bool timing = false;
     bool walk_stats = false;  // FAST walk counts its schedule statistics
     bool counting = true;     // FAST walk counts interactions (SimpleTrackingRule)
+    bool tree_valid = false;  // the tree matches the current state and parameters
     // Timing: one set of kPhases+1 events per force pass, recorded on the
     // system stream; folded into per-phase sums when the stream is synchronised.
     std::vector<hipEvent_t> ev;    // kEventSets * (kPhases + 1)
@@ -185,6 +186,9 @@ void fold_timing(nbg_sys *s) {
     s->ev_used = 0;
 }
 
+WalkArgs walk_args(nbg_sys *s);
+void launch_walk(nbg_sys *s, const WalkArgs &wa, int stats);
+
 // ActiveTreeSolver::updateAccelerations (Solvers/ActiveTreeSolver.h:33-72):
 // build the tree (LinearBVH::build, Trees/LinearBVH.h:106-133, or
 // Octree::build, Trees/Octree.h:185-205), then walk it for every body.
@@ -228,9 +232,25 @@ int enqueue_force_pass(nbg_sys *s) {
                               p.theta, s->err_word(), st);
     }
     mark(s, 6);
+    WalkArgs wa = walk_args(s);
+    const int stats = s->walk_stats ? 2 : (s->counting ? 1 : 0);
+    launch_walk(s, wa, stats);
+    mark(s, 7);
+    NBG_HIP(hipGetLastError());
+    s->tree_valid = true;
+    return NBG_OK;
+}
+
+// The walk's arguments for the system's own bodies as targets.
+WalkArgs walk_args(nbg_sys *s) {
+    const int64_t n = s->n;
+    const nbg_params &p = s->prm;
+    const bool octree = p.solver == NBG_BARNES_HUT;
     WalkArgs wa;
     wa.n = n;
     wa.bodies = s->bodies;
+    wa.nt = n;
+    wa.targets = s->bodies;
     wa.perm = s->idx;
     wa.nodes = s->nodes;
     wa.onodes = s->ob.nodes;
@@ -246,14 +266,15 @@ int enqueue_force_pass(nbg_sys *s) {
     wa.az = s->az;
     wa.counts = s->counts();
     wa.err = s->err_word();
-    const int stats = s->walk_stats ? 2 : (s->counting ? 1 : 0);
-    if (octree)
-        launch_walk_oct(wa, p.sum_mode == NBG_SUM_ORDERED, stats, st);
+    return wa;
+}
+
+void launch_walk(nbg_sys *s, const WalkArgs &wa, int stats) {
+    const nbg_params &p = s->prm;
+    if (p.solver == NBG_BARNES_HUT)
+        launch_walk_oct(wa, p.sum_mode == NBG_SUM_ORDERED, stats, s->stream);
     else
-        launch_walk_lbvh(wa, p.sum_mode == NBG_SUM_ORDERED, p.morton_bits == 30 ? 32 : 64, stats, st);
-    mark(s, 7);
-    NBG_HIP(hipGetLastError());
-    return NBG_OK;
+        launch_walk_lbvh(wa, p.sum_mode == NBG_SUM_ORDERED, p.morton_bits == 30 ? 32 : 64, stats, s->stream);
 }
 
 int sync_and_check(nbg_sys *s) {
@@ -398,6 +419,7 @@ int nbg_sys_set_params(nbg_sys *s, const nbg_params *p) {
         s->prm = old;
         return r;
     }
+    s->tree_valid = false;
     return NBG_OK;
 }
 
@@ -427,6 +449,7 @@ int nbg_sys_set_state(nbg_sys *s, const float *x, const float *y, const float *z
     }
     s->have_state = true;
     s->have_forces = false;
+    s->tree_valid = false;
     return NBG_OK;
 }
 
@@ -468,6 +491,74 @@ int nbg_sys_get_accelerations(nbg_sys *s, float *ax, float *ay, float *az) {
     return NBG_OK;
 }
 
+int nbg_sys_accelerations_at(nbg_sys *s, int64_t nt, const float *tx, const float *ty, const float *tz, float *ax,
+                             float *ay, float *az) {
+    if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
+    if (nt < 0 || nt > kMaxBodies) return fail(NBG_ERR_ARG, "target count out of range");
+    if (nt > 0 && (!tx || !ty || !tz || !ax || !ay || !az)) return fail(NBG_ERR_ARG, "NULL array");
+    if (!s->have_state) return fail(NBG_ERR_STATE, "set_state has not been called");
+    if (nt == 0) return NBG_OK;
+    DeviceGuard g(s->ctx->device);
+    if (s->n == 0) {  // no actives: every passive acceleration is zero
+        for (int64_t i = 0; i < nt; ++i) ax[i] = ay[i] = az[i] = 0.0f;
+        return NBG_OK;
+    }
+    if (!s->tree_valid) {
+        // builds the tree (and, as a side effect, the bodies' own accelerations)
+        if (int r = enqueue_force_pass(s)) return r;
+        mark(s, kPhases);
+        if (int r = sync_and_check(s)) return r;
+        s->have_forces = true;
+    }
+    // Targets are walked in Morton order over the system's bounding box
+    // (63-bit keys, clamped outside it) so that a wavefront's targets are
+    // spatially adjacent; results return in the caller's order.
+    hipStream_t st = s->stream;
+    float *t3 = nullptr, *a3 = nullptr;
+    uint64_t *k2 = nullptr;
+    uint32_t *i2 = nullptr, *status = nullptr;
+    float4 *t4 = nullptr;
+    int r = NBG_OK;
+    auto A = [&](auto **ptr, int64_t count) {
+        if (r == NBG_OK) r = dalloc(ptr, count);
+    };
+    A(&t3, 3 * nt); A(&a3, 3 * nt); A(&k2, 2 * nt); A(&i2, 2 * nt); A(&t4, nt);
+    A(&status, sort_status_words(nt, 64));
+    if (r == NBG_OK) {
+        auto up = [&](float *d, const float *h) {
+            hipError_t e = hipMemcpyAsync(d, h, (size_t)nt * 4, hipMemcpyHostToDevice, st);
+            if (e != hipSuccess && r == NBG_OK) r = fail(NBG_ERR_HIP, std::string("H2D: ") + hipGetErrorString(e));
+        };
+        up(t3, tx); up(t3 + nt, ty); up(t3 + 2 * nt, tz);
+    }
+    if (r == NBG_OK) {
+        launch_morton(t3, t3 + nt, t3 + 2 * nt, nt, s->bbox, 63, k2, i2, st);
+        launch_radix_sort(k2, i2, k2 + nt, i2 + nt, nt, 63, status, s->err_word(), st);
+        launch_gather(t3, t3 + nt, t3 + 2 * nt, t3, i2, nt, t4, st);
+        launch_fill_u32(s->misc + 4, 0u, 12, st);
+        WalkArgs wa = walk_args(s);
+        wa.nt = nt;
+        wa.targets = t4;
+        wa.perm = i2;
+        wa.ax = a3;
+        wa.ay = a3 + nt;
+        wa.az = a3 + 2 * nt;
+        launch_walk(s, wa, s->counting || s->prm.sum_mode == NBG_SUM_ORDERED ? 1 : 0);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) r = fail(NBG_ERR_HIP, std::string("walk launch: ") + hipGetErrorString(e));
+    }
+    if (r == NBG_OK) r = sync_and_check(s);
+    if (r == NBG_OK) {
+        hipError_t e = hipMemcpy(ax, a3, (size_t)nt * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(ay, a3 + nt, (size_t)nt * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(az, a3 + 2 * nt, (size_t)nt * 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) r = fail(NBG_ERR_HIP, std::string("D2H: ") + hipGetErrorString(e));
+    }
+    for (void *p : {(void *)t3, (void *)a3, (void *)k2, (void *)i2, (void *)t4, (void *)status})
+        if (p) (void)hipFree(p);
+    return r;
+}
+
 int nbg_sys_step(nbg_sys *s, int32_t nsteps) {
     if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
     if (nsteps < 0) return fail(NBG_ERR_ARG, "nsteps must be >= 0");
@@ -477,6 +568,7 @@ int nbg_sys_step(nbg_sys *s, int32_t nsteps) {
         if (int r = enqueue_force_pass(s)) return r;
         launch_update(s->x, s->y, s->z, s->vx, s->vy, s->vz, s->ax, s->ay, s->az, s->prm.dt, s->n, s->stream);
         mark(s, kPhases);
+        s->tree_valid = false;  // positions moved after the tree was built
     }
     NBG_HIP(hipGetLastError());
     if (nsteps > 0) s->have_forces = true;

@@ -177,9 +177,11 @@ inline void mac_threshold(float theta, double *mid, int *tie) {
 }
 
 struct WalkArgs {
-    int64_t n;
+    int64_t n;              // bodies in the tree
     const float4 *bodies;   // sorted (x, y, z, m)
-    const uint32_t *perm;   // sorted position -> original index
+    int64_t nt;             // targets walked (the bodies themselves, or passive points)
+    const float4 *targets;  // targets in walk order (xyz)
+    const uint32_t *perm;   // walk position -> output index (< nt)
     const NodeRec *nodes;   // LBVH nodes
     const OctRec *onodes;   // octree nodes
     int levels;             // octree: max_depth + 2
@@ -189,7 +191,7 @@ struct WalkArgs {
     int mac_tie;
     float G, eps2;
     int64_t max_nodes;      // node array capacity (index guard)
-    float *ax, *ay, *az;    // original order
+    float *ax, *ay, *az;    // output order (nt each)
     unsigned long long *counts;  // [pp, np]
     uint32_t *err;
 };

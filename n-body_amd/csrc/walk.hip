@@ -73,11 +73,11 @@ __global__ __launch_bounds__(64) void k_walk_lbvh_ordered(WalkArgs a) {
     const NodeRec *__restrict__ nodes = a.nodes;
     const int lane = lane_id();
     const int64_t tgt = xcd_block(blockIdx.x, gridDim.x) * kWave + lane;
-    const bool valid = tgt < a.n;
+    const bool valid = tgt < a.nt;
     const uint64_t vmask = __ballot(valid);
     if (vmask == 0) return;
 
-    const float4 me = valid ? bodies[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const float px = me.x, py = me.y, pz = me.z;
     float rx = 0.0f, ry = 0.0f, rz = 0.0f;
     uint32_t cpp = 0, cnp = 0;
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(64) void k_walk_lbvh_ordered(WalkArgs a) {
     if (fail && lane == 0) atomicOr(a.err, 2u);
     if (valid) {
         const uint32_t o = a.perm[tgt];
-        if ((int64_t)o < a.n) { a.ax[o] = rx; a.ay[o] = ry; a.az[o] = rz; }
+        if ((int64_t)o < a.nt) { a.ax[o] = rx; a.ay[o] = ry; a.az[o] = rz; }
     }
     const uint32_t wpp = wave_sum(valid ? cpp : 0u), wnp = wave_sum(valid ? cnp : 0u);
     if (lane == 0) {
@@ -234,12 +234,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     const NodeRec *__restrict__ nodes = a.nodes;
     const int lane = lane_id(), wv = threadIdx.x / kWave;
     const int64_t tgt = (xcd_block(blockIdx.x, gridDim.x) * WAVES + wv) * kWave + lane;
-    const bool valid = tgt < a.n;
+    const bool valid = tgt < a.nt;
     const uint64_t vmask = __ballot(valid);
     if (vmask == 0) return;
 
     FastWave W;
-    const float4 me = valid ? bodies[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     W.px = me.x; W.py = me.y; W.pz = me.z;
     W.rx = 0.0f; W.ry = 0.0f; W.rz = 0.0f;
     W.pp = 0; W.np = 0;
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     if (fail && lane == 0) atomicOr(a.err, 2u);
     if (valid) {
         const uint32_t o = a.perm[tgt];
-        if ((int64_t)o < n) { a.ax[o] = W.rx; a.ay[o] = W.ry; a.az[o] = W.rz; }
+        if ((int64_t)o < a.nt) { a.ax[o] = W.rx; a.ay[o] = W.ry; a.az[o] = W.rz; }
     }
     if (lane == 0) {
         atomicAdd(&a.counts[0], (unsigned long long)W.pp);
@@ -389,11 +389,11 @@ __global__ __launch_bounds__(64) void k_walk_oct_ordered(WalkArgs a) {
     const OctRec *__restrict__ nodes = a.onodes;
     const int lane = lane_id();
     const int64_t tgt = xcd_block(blockIdx.x, gridDim.x) * kWave + lane;
-    const bool valid = tgt < a.n;
+    const bool valid = tgt < a.nt;
     const uint64_t vmask = __ballot(valid);
     if (vmask == 0) return;
 
-    const float4 me = valid ? bodies[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const float px = me.x, py = me.y, pz = me.z;
     const float G = a.G, eps2 = a.eps2;
     float rx = 0.0f, ry = 0.0f, rz = 0.0f;
@@ -452,7 +452,7 @@ __global__ __launch_bounds__(64) void k_walk_oct_ordered(WalkArgs a) {
     if (fail && lane == 0) atomicOr(a.err, 2u);
     if (valid) {
         const uint32_t o = a.perm[tgt];
-        if ((int64_t)o < a.n) { a.ax[o] = rx; a.ay[o] = ry; a.az[o] = rz; }
+        if ((int64_t)o < a.nt) { a.ax[o] = rx; a.ay[o] = ry; a.az[o] = rz; }
     }
     const uint32_t wpp = wave_sum(valid ? cpp : 0u), wnp = wave_sum(valid ? cnp : 0u);
     if (lane == 0) {
@@ -493,12 +493,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
     const OctRec *__restrict__ nodes = a.onodes;
     const int lane = lane_id(), wv = threadIdx.x / kWave;
     const int64_t tgt = (xcd_block(blockIdx.x, gridDim.x) * WAVES + wv) * kWave + lane;
-    const bool valid = tgt < a.n;
+    const bool valid = tgt < a.nt;
     const uint64_t vmask = __ballot(valid);
     if (vmask == 0) return;
 
     FastWave W;
-    const float4 me = valid ? bodies[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     W.px = me.x; W.py = me.y; W.pz = me.z;
     W.rx = 0.0f; W.ry = 0.0f; W.rz = 0.0f;
     W.pp = 0; W.np = 0;
@@ -554,7 +554,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
     if (fail && lane == 0) atomicOr(a.err, 2u);
     if (valid) {
         const uint32_t o = a.perm[tgt];
-        if ((int64_t)o < n) { a.ax[o] = W.rx; a.ay[o] = W.ry; a.az[o] = W.rz; }
+        if ((int64_t)o < a.nt) { a.ax[o] = W.rx; a.ay[o] = W.ry; a.az[o] = W.rz; }
     }
     if (lane == 0) {
         atomicAdd(&a.counts[0], (unsigned long long)W.pp);
@@ -571,7 +571,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
 template <int STATS>
 static void launch_oct_fast(const WalkArgs &a, hipStream_t s) {
     constexpr int W = 4;
-    const unsigned blocks = (unsigned)((a.n + 64 * W - 1) / (64 * W));
+    const unsigned blocks = (unsigned)((a.nt + 64 * W - 1) / (64 * W));
     if (a.eps2 > 0.0f)
         hipLaunchKernelGGL((k_walk_oct_fast<W, true, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
     else
@@ -579,13 +579,13 @@ static void launch_oct_fast(const WalkArgs &a, hipStream_t s) {
 }
 
 void launch_walk_oct(const WalkArgs &a, int ordered, int stats, hipStream_t s) {
-    if (a.n <= 0) return;
+    if (a.n <= 0 || a.nt <= 0) return;
     if (!ordered) {
         if (stats >= 2) launch_oct_fast<2>(a, s);
         else if (stats == 1) launch_oct_fast<1>(a, s);
         else launch_oct_fast<0>(a, s);
     } else {
-        const unsigned blocks = (unsigned)((a.n + 63) / 64);
+        const unsigned blocks = (unsigned)((a.nt + 63) / 64);
         if (a.levels <= 18)
             hipLaunchKernelGGL((k_walk_oct_ordered<18>), dim3(blocks), dim3(64), 0, s, a);
         else
@@ -596,7 +596,7 @@ void launch_walk_oct(const WalkArgs &a, int ordered, int stats, hipStream_t s) {
 template <int STATS>
 static void launch_fast(const WalkArgs &a, hipStream_t s) {
     constexpr int W = 4;
-    const unsigned blocks = (unsigned)((a.n + 64 * W - 1) / (64 * W));
+    const unsigned blocks = (unsigned)((a.nt + 64 * W - 1) / (64 * W));
     if (a.eps2 > 0.0f)
         hipLaunchKernelGGL((k_walk_lbvh_fast<W, true, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
     else
@@ -604,13 +604,13 @@ static void launch_fast(const WalkArgs &a, hipStream_t s) {
 }
 
 void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, int stats, hipStream_t s) {
-    if (a.n <= 0) return;
+    if (a.n <= 0 || a.nt <= 0) return;
     if (!ordered) {
         if (stats >= 2) launch_fast<2>(a, s);
         else if (stats == 1) launch_fast<1>(a, s);
         else launch_fast<0>(a, s);
     } else {
-        const unsigned blocks = (unsigned)((a.n + 63) / 64);
+        const unsigned blocks = (unsigned)((a.nt + 63) / 64);
         if (max_depth <= 32)
             hipLaunchKernelGGL((k_walk_lbvh_ordered<32>), dim3(blocks), dim3(64), 0, s, a);
         else

@@ -1,0 +1,44 @@
+"""The C++ facade (n-body_amd/host/nbody_gpu.hpp) driven the way a reference
+caller drives Solver<Gravity>: select(id), theta()/maxLeafSize()/timeStep()
+accessors, updateAccelerations() and step().  Its results must be the
+oracle's, bit for bit, in ORDERED mode."""
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PROG = os.path.join(ROOT, "n-body_amd", "_build", "facade_check")
+
+
+def _read(path, n):
+    raw = open(path, "rb").read()
+    assert int(np.frombuffer(raw[:8], np.int64)[0]) == n
+    f = np.frombuffer(raw[8:], np.float32)
+    assert len(f) == 13 * n
+    init = f[:4 * n].reshape(4, n)
+    acc = f[4 * n:7 * n].reshape(3, n).T
+    after = f[7 * n:].reshape(6, n)
+    return init, acc, after
+
+
+@pytest.mark.parametrize("sid,solver", [("gpu-barnes-hut", 0), ("gpu-linear-bvh", 1)])
+def test_facade_matches_oracle(nbg, orc, gpu_ctx, tmp_path, sid, solver):
+    n = 2000
+    out = tmp_path / "fc.bin"
+    r = subprocess.run([PROG, sid, str(n), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    (x, y, z, m), acc, after = _read(out, n)
+    t = orc.Tree(solver, x, y, z, m, theta=0.5, leaf=16, bits=63, eps2=1e-4)
+    assert np.array_equal(acc, t.accel(x, y, z))
+    # the program's initial state comes from the same generator call
+    b = nbg.plummer(n, seed=7)
+    assert np.array_equal(b.x, x) and np.array_equal(b.m, m)
+    (px, py, pz, pvx, pvy, pvz), _ = orc.step(solver, (b.x, b.y, b.z, b.m, b.vx, b.vy, b.vz), 2, theta=0.5,
+                                              leaf=16, bits=63, eps2=1e-4, dt=1e-3)
+    for g, ref in zip(after, (px, py, pz, pvx, pvy, pvz)):
+        assert np.array_equal(g, ref)

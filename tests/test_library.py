@@ -74,3 +74,17 @@ def test_plummer_profile(nbg):
     assert np.all(v <= vesc * (1 + 1e-6))
     # isotropy: mean position and velocity near zero
     assert abs(b.x.mean()) < 0.05 and abs(b.vx.mean()) < 0.01
+
+
+def test_facade_program_built_and_reports_missing_gpu(nbg):
+    """The C++ facade check program links against the library; on a machine
+    without a GPU it exits with status 3 before touching the device."""
+    import subprocess
+
+    prog = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "n-body_amd", "_build",
+                        "facade_check")
+    assert os.path.exists(prog), "build() must compile the facade check program"
+    if nbg.device_count() > 0:
+        pytest.skip("a GPU is present: the GPU tests run the program")
+    r = subprocess.run([prog, "gpu-barnes-hut", "16", os.devnull], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 3, r.stderr
