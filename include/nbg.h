@@ -188,6 +188,67 @@ int nbg_direct_f64(nbg_ctx *ctx, int64_t ns, const float *sx, const float *sy, c
  * of a system's current device state. */
 int nbg_sys_potential_f64(nbg_sys *sys, double *W);
 
+/* ---- batched forest: many independent systems stepped together ----
+ *
+ * No reference counterpart (the reference steps one Simulation).  Each system
+ * is a linear-BVH Solver (Solvers/LinearBVHSolver.h:18-28) with its own
+ * theta, leaf size, G, eps2 and dt; every pass builds all systems' trees in
+ * one segmented build and walks them in one launch.  A system's results
+ * are the same bits it gets alone in an nbg_sys with the same parameters
+ * (30-bit Morton keys, the batch's sum mode).  Systems can join and leave
+ * between steps (continuous batching); state stays in HBM. */
+typedef struct nbg_batch nbg_batch;
+
+/* HBM bytes a batch of this capacity allocates (for admission budgets). */
+int64_t nbg_batch_bytes(int64_t max_bodies, int32_t max_systems);
+
+/* Capacity is fixed at creation (NBG_ERR_BUDGET if the device cannot hold it). */
+int nbg_batch_create(nbg_ctx *ctx, int64_t max_bodies, int32_t max_systems, int32_t sum_mode, nbg_batch **out);
+int nbg_batch_destroy(nbg_batch *b);
+
+/* Adds a system of n >= 1 bodies (host SoA copied in; vx/vy/vz may be NULL).
+ * p->solver must be NBG_LINEAR_BVH and p->morton_bits 30.  *id receives the
+ * system's handle.  NBG_ERR_BUDGET when the batch is full. */
+int nbg_batch_add(nbg_batch *b, const nbg_params *p, int64_t n, const float *x, const float *y, const float *z,
+                  const float *m, const float *vx, const float *vy, const float *vz, int32_t *id);
+
+/* Removes a system; the others keep their state (moved on the device). */
+int nbg_batch_remove(nbg_batch *b, int32_t id);
+
+/* Changes one system's parameters (same constraints as nbg_batch_add). */
+int nbg_batch_set_params(nbg_batch *b, int32_t id, const nbg_params *p);
+
+/* Number of systems and bodies currently in the batch. */
+int nbg_batch_info(nbg_batch *b, int32_t *nsys, int64_t *nbodies);
+
+/* Copies one system's state out (any pointer may be NULL); accelerations are
+ * those of the last pass and need a pass since the last membership change. */
+int nbg_batch_get(nbg_batch *b, int32_t id, float *x, float *y, float *z, float *vx, float *vy, float *vz,
+                  float *ax, float *ay, float *az);
+
+/* Force pass for every system (ActiveTreeSolver::updateAccelerations each). */
+int nbg_batch_update_accelerations(nbg_batch *b);
+
+/* nsteps Solver::step()s of every system; asynchronous. */
+int nbg_batch_step(nbg_batch *b, int32_t nsteps);
+int nbg_batch_synchronize(nbg_batch *b);
+
+/* Interaction totals of the last pass over all systems (SimpleTrackingRule
+ * counting; a switch as for nbg_sys). */
+int nbg_batch_counts(nbg_batch *b, uint64_t *pp, uint64_t *np);
+int nbg_batch_enable_counts(nbg_batch *b, int32_t on);
+
+/* Replays each step from a captured HIP graph (captured again after any
+ * membership or parameter change). */
+int nbg_batch_use_graph(nbg_batch *b, int32_t on);
+
+/* Leaf-size autotuner: times `repeats` force passes of the current batch for
+ * each candidate leaf size (state untouched), then sets every system's leaf
+ * size to the fastest.  *best receives it; ms_out[k] (optional) the mean
+ * pass time of candidate k. */
+int nbg_batch_autotune(nbg_batch *b, const int32_t *leaf_sizes, int32_t k, int32_t repeats, int32_t *best,
+                       float *ms_out);
+
 /* ---- synthetic inputs (host, CPU) ---- */
 
 /* Plummer sphere: scale radius a, total mass M, gravitational constant G,

@@ -40,6 +40,10 @@ EXPORTS = (
     "nbg_sys_enable_timing", "nbg_sys_phase_times",
     "nbg_sys_debug_bbox", "nbg_sys_debug_order", "nbg_sys_debug_tree", "nbg_direct_f64",
     "nbg_sys_potential_f64", "nbg_generate_plummer", "nbg_generate_cube",
+    "nbg_batch_bytes", "nbg_batch_create", "nbg_batch_destroy", "nbg_batch_add", "nbg_batch_remove",
+    "nbg_batch_set_params", "nbg_batch_info", "nbg_batch_get", "nbg_batch_update_accelerations", "nbg_batch_step",
+    "nbg_batch_synchronize", "nbg_batch_counts", "nbg_batch_enable_counts", "nbg_batch_use_graph",
+    "nbg_batch_autotune",
 )
 
 
@@ -101,6 +105,24 @@ def lib() -> ctypes.CDLL:
             "nbg_sys_update_accelerations": (ctypes.c_int, [_vp] + [_f32p] * 3),
             "nbg_sys_get_accelerations": (ctypes.c_int, [_vp] + [_f32p] * 3),
             "nbg_sys_accelerations_at": (ctypes.c_int, [_vp, ctypes.c_int64] + [_f32p] * 6),
+            "nbg_batch_bytes": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_int32]),
+            "nbg_batch_create": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                                ctypes.POINTER(_vp)]),
+            "nbg_batch_destroy": (ctypes.c_int, [_vp]),
+            "nbg_batch_add": (ctypes.c_int, [_vp, ctypes.POINTER(Params), ctypes.c_int64] + [_f32p] * 7
+                              + [ctypes.POINTER(ctypes.c_int32)]),
+            "nbg_batch_remove": (ctypes.c_int, [_vp, ctypes.c_int32]),
+            "nbg_batch_set_params": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(Params)]),
+            "nbg_batch_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)]),
+            "nbg_batch_get": (ctypes.c_int, [_vp, ctypes.c_int32] + [_f32p] * 9),
+            "nbg_batch_update_accelerations": (ctypes.c_int, [_vp]),
+            "nbg_batch_step": (ctypes.c_int, [_vp, ctypes.c_int32]),
+            "nbg_batch_synchronize": (ctypes.c_int, [_vp]),
+            "nbg_batch_counts": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+            "nbg_batch_enable_counts": (ctypes.c_int, [_vp, ctypes.c_int32]),
+            "nbg_batch_use_graph": (ctypes.c_int, [_vp, ctypes.c_int32]),
+            "nbg_batch_autotune": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, ctypes.c_int32,
+                                                  ctypes.POINTER(ctypes.c_int32), _f32p]),
             "nbg_sys_step": (ctypes.c_int, [_vp, ctypes.c_int32]),
             "nbg_sys_synchronize": (ctypes.c_int, [_vp]),
             "nbg_sys_counts": (ctypes.c_int, [_vp, _u64p, _u64p]),
@@ -343,6 +365,90 @@ class System:
         w = ctypes.c_double()
         _check(lib().nbg_sys_potential_f64(self.handle, ctypes.byref(w)))
         return float(w.value)
+
+
+# ------------------------------------------------------------------ batched forest
+
+class Batch:
+    """nbg_batch: independent linear-BVH systems stepped together in one
+    segmented build and one walk launch per step."""
+
+    def __init__(self, ctx: Context, max_bodies: int, max_systems: int, sum_mode: int = SUM_FAST):
+        h = _vp()
+        _check(lib().nbg_batch_create(ctx.handle, max_bodies, max_systems, sum_mode, ctypes.byref(h)))
+        self.handle = h
+        self._n = {}
+
+    @staticmethod
+    def bytes_for(max_bodies: int, max_systems: int) -> int:
+        return int(lib().nbg_batch_bytes(max_bodies, max_systems))
+
+    def close(self):
+        if self.handle:
+            lib().nbg_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, params: Params, b: Bodies) -> int:
+        arrs = [_f32(v) for v in (b.x, b.y, b.z, b.m, b.vx, b.vy, b.vz)]
+        sid = ctypes.c_int32()
+        _check(lib().nbg_batch_add(self.handle, ctypes.byref(params), b.n, *[_ptr(a) for a in arrs],
+                                   ctypes.byref(sid)))
+        self._n[sid.value] = b.n
+        return sid.value
+
+    def remove(self, sid: int):
+        _check(lib().nbg_batch_remove(self.handle, sid))
+        del self._n[sid]
+
+    def set_params(self, sid: int, params: Params):
+        _check(lib().nbg_batch_set_params(self.handle, sid, ctypes.byref(params)))
+
+    def info(self):
+        ns, nb = ctypes.c_int32(), ctypes.c_int64()
+        _check(lib().nbg_batch_info(self.handle, ctypes.byref(ns), ctypes.byref(nb)))
+        return ns.value, nb.value
+
+    def get(self, sid: int, accelerations: bool = False):
+        n = self._n[sid]
+        out = Bodies.empty(n)
+        acc = [np.zeros(n, np.float32) for _ in range(3)] if accelerations else [None] * 3
+        _check(lib().nbg_batch_get(self.handle, sid, *[_ptr(v) for v in (out.x, out.y, out.z, out.vx, out.vy, out.vz)],
+                                   *[_ptr(v) if v is not None else None for v in acc]))
+        return (out, np.stack(acc, axis=1)) if accelerations else out
+
+    def update_accelerations(self):
+        _check(lib().nbg_batch_update_accelerations(self.handle))
+
+    def step(self, nsteps: int = 1):
+        _check(lib().nbg_batch_step(self.handle, nsteps))
+
+    def synchronize(self):
+        _check(lib().nbg_batch_synchronize(self.handle))
+
+    def counts(self):
+        pp, np_ = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().nbg_batch_counts(self.handle, ctypes.byref(pp), ctypes.byref(np_)))
+        return int(pp.value), int(np_.value)
+
+    def enable_counts(self, on: bool = True):
+        _check(lib().nbg_batch_enable_counts(self.handle, int(on)))
+
+    def use_graph(self, on: bool = True):
+        _check(lib().nbg_batch_use_graph(self.handle, int(on)))
+
+    def autotune(self, leaf_sizes=(8, 16, 32, 64), repeats: int = 2):
+        ls = np.asarray(leaf_sizes, np.int32)
+        ms = np.zeros(len(ls), np.float32)
+        best = ctypes.c_int32()
+        _check(lib().nbg_batch_autotune(self.handle, ls.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(ls),
+                                        repeats, ctypes.byref(best), _ptr(ms)))
+        return best.value, {int(a): float(t) for a, t in zip(ls, ms)}
 
 
 # ------------------------------------------------------------------ reference-shaped facade

@@ -10,34 +10,17 @@
 #include <vector>
 
 #include "../../include/nbg.h"
+#include "host_util.hpp"
 #include "launch.hpp"
 
 using namespace nbg;
 
 namespace {
 
-thread_local std::string g_err;
-
-int fail(int code, const std::string &msg) {
-    g_err = msg;
-    return code;
-}
-
-#define NBG_HIP(call)                                                                             \
-    do {                                                                                          \
-        hipError_t e_ = (call);                                                                   \
-        if (e_ != hipSuccess)                                                                     \
-            return fail(NBG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));          \
-    } while (0)
-
 constexpr int kPhases = 8;
 constexpr int kEventSets = 256;
 
 }  // namespace
-
-struct nbg_ctx {
-    int device = 0;
-};
 
 struct nbg_sys {
     nbg_ctx *ctx = nullptr;
@@ -101,13 +84,6 @@ int check_params(const nbg_params *p) {
     if (p->morton_bits != 30 && p->morton_bits != 63) return fail(NBG_ERR_ARG, "morton_bits must be 30 or 63");
     if (!(p->eps2 >= 0.0f)) return fail(NBG_ERR_ARG, "eps2 must be >= 0");
     if (p->sum_mode != NBG_SUM_ORDERED && p->sum_mode != NBG_SUM_FAST) return fail(NBG_ERR_ARG, "unknown sum mode");
-    return NBG_OK;
-}
-
-template <typename T>
-int dalloc(T **p, int64_t count) {
-    if (count <= 0) count = 1;
-    NBG_HIP(hipMalloc((void **)p, (size_t)count * sizeof(T)));
     return NBG_OK;
 }
 
@@ -293,17 +269,6 @@ int sync_and_check(nbg_sys *s) {
     fold_timing(s);
     return NBG_OK;
 }
-
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev) {
-        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-        (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard() {
-        if (prev >= 0) (void)hipSetDevice(prev);
-    }
-};
 
 }  // namespace
 

@@ -115,30 +115,34 @@ __device__ __forceinline__ uint32_t quantize(float v, float mul, float hi) {
     return (t != t) ? 0u : (uint32_t)t;
 }
 
+// Morton key of one point over the bounding cube of bb (min xyz, max xyz).
+template <typename K>
+__device__ __forceinline__ K morton_key(float px, float py, float pz, const float *__restrict__ bb) {
+    const float mnx = bb[0], mny = bb[1], mnz = bb[2];
+    const float dx = bb[3] - mnx, dy = bb[4] - mny, dz = bb[5] - mnz;
+    const float m1 = (dx < dy) ? dy : dx;   // std::max(std::max(dx, dy), dz)
+    const float scale = (m1 < dz) ? dz : m1;
+    const float vx = ieee_div(px - mnx, scale);
+    const float vy = ieee_div(py - mny, scale);
+    const float vz = ieee_div(pz - mnz, scale);
+    if constexpr (sizeof(K) == 4) {
+        uint32_t qx = quantize(vx, 1024.0f, 1023.0f), qy = quantize(vy, 1024.0f, 1023.0f),
+                 qz = quantize(vz, 1024.0f, 1023.0f);
+        return expand_bits10(qx) * 4u + expand_bits10(qy) * 2u + expand_bits10(qz);
+    } else {
+        uint64_t qx = quantize(vx, 2097152.0f, 2097151.0f), qy = quantize(vy, 2097152.0f, 2097151.0f),
+                 qz = quantize(vz, 2097152.0f, 2097151.0f);
+        return expand_bits21(qx) * 4u + expand_bits21(qy) * 2u + expand_bits21(qz);
+    }
+}
+
 template <typename K>
 __global__ __launch_bounds__(256) void k_morton(const float *__restrict__ x, const float *__restrict__ y,
                                                 const float *__restrict__ z, int64_t n,
                                                 const float *__restrict__ bb, K *__restrict__ keys,
                                                 uint32_t *__restrict__ idx) {
-    const float mnx = bb[0], mny = bb[1], mnz = bb[2];
-    const float dx = bb[3] - mnx, dy = bb[4] - mny, dz = bb[5] - mnz;
-    const float m1 = (dx < dy) ? dy : dx;   // std::max(std::max(dx, dy), dz)
-    const float scale = (m1 < dz) ? dz : m1;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        float vx = ieee_div(x[i] - mnx, scale);
-        float vy = ieee_div(y[i] - mny, scale);
-        float vz = ieee_div(z[i] - mnz, scale);
-        K key;
-        if constexpr (sizeof(K) == 4) {
-            uint32_t qx = quantize(vx, 1024.0f, 1023.0f), qy = quantize(vy, 1024.0f, 1023.0f),
-                     qz = quantize(vz, 1024.0f, 1023.0f);
-            key = expand_bits10(qx) * 4u + expand_bits10(qy) * 2u + expand_bits10(qz);
-        } else {
-            uint64_t qx = quantize(vx, 2097152.0f, 2097151.0f), qy = quantize(vy, 2097152.0f, 2097151.0f),
-                     qz = quantize(vz, 2097152.0f, 2097151.0f);
-            key = expand_bits21(qx) * 4u + expand_bits21(qy) * 2u + expand_bits21(qz);
-        }
-        keys[i] = key;
+        keys[i] = morton_key<K>(x[i], y[i], z[i], bb);
         idx[i] = (uint32_t)i;
     }
 }
@@ -372,10 +376,13 @@ __device__ __forceinline__ int kdelta(const K *__restrict__ keys, int n, int i, 
     else return __clz((uint32_t)(a ^ b));
 }
 
+// Forest (sys_shift > 0): keys carry the system index above bit sys_shift; a
+// node whose range spans systems always splits, and a node inside system s
+// uses that system's leaf size leaf_tab[s].
 template <typename K>
 __global__ __launch_bounds__(256) void k_karras(const K *__restrict__ keys, int n, int leaf, int32_t *kfirst,
                                                 int32_t *klast, int32_t *kgamma, uint32_t *ksplit,
-                                                int32_t *kparent) {
+                                                int32_t *kparent, int sys_shift, const int32_t *__restrict__ leaf_tab) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     const int d = (kdelta(keys, n, i, i + 1) - kdelta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -397,7 +404,14 @@ __global__ __launch_bounds__(256) void k_karras(const K *__restrict__ keys, int 
     kfirst[i] = first;
     klast[i] = last;
     kgamma[i] = gamma;
-    ksplit[i] = (last - first + 1 > leaf && keys[first] != keys[last]) ? 1u : 0u;
+    int L = leaf;
+    bool span = false;
+    if (sys_shift > 0) {
+        const uint64_t s0 = (uint64_t)keys[first] >> sys_shift, s1 = (uint64_t)keys[last] >> sys_shift;
+        span = s0 != s1;
+        L = leaf_tab[s0];
+    }
+    ksplit[i] = ((span || last - first + 1 > L) && keys[first] != keys[last]) ? 1u : 0u;
     if (gamma != first) kparent[gamma] = 2 * i;
     if (gamma + 1 != last) kparent[gamma + 1] = 2 * i + 1;
 }
@@ -509,7 +523,8 @@ __global__ void k_root_fix(int n, const uint32_t *__restrict__ ksplit, const uin
 
 void launch_lbvh_topology(const void *keys, int bits, int64_t n64, int leaf_size, int32_t *kfirst, int32_t *klast,
                           int32_t *kgamma, uint32_t *ksplit, uint32_t *krank, int32_t *kparent, uint32_t *scan_ws,
-                          NodeRec *nodes, int32_t *leaf_start, uint32_t *node_count, hipStream_t s) {
+                          NodeRec *nodes, int32_t *leaf_start, uint32_t *node_count, hipStream_t s, int sys_shift,
+                          const int32_t *leaf_tab) {
     const int n = (int)n64;
     uint32_t *total = scan_ws;
     uint32_t *sums = scan_ws + 64;
@@ -519,10 +534,10 @@ void launch_lbvh_topology(const void *keys, int bits, int64_t n64, int leaf_size
         const unsigned blocks = (unsigned)((m + 255) / 256);
         if (bits == 30)
             hipLaunchKernelGGL(k_karras<uint32_t>, dim3(blocks), dim3(256), 0, s, (const uint32_t *)keys, n, leaf_size,
-                               kfirst, klast, kgamma, ksplit, kparent);
+                               kfirst, klast, kgamma, ksplit, kparent, 0, (const int32_t *)nullptr);
         else
             hipLaunchKernelGGL(k_karras<uint64_t>, dim3(blocks), dim3(256), 0, s, (const uint64_t *)keys, n, leaf_size,
-                               kfirst, klast, kgamma, ksplit, kparent);
+                               kfirst, klast, kgamma, ksplit, kparent, sys_shift, leaf_tab);
         const int64_t nt = (m + kScanTile - 1) / kScanTile;
         hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)nt), dim3(kScanBlock), 0, s, ksplit, (int64_t)m, sums);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, s, sums, nt, total);
@@ -557,9 +572,14 @@ void launch_lbvh_topology(const void *keys, int bits, int64_t n64, int leaf_size
 // invalidate across the XCDs.
 // The MAC record: bbox centre and the squared-distance threshold above which
 // a target accepts the node (mac_lbvh_threshold of the squared diagonal).
-struct MacParam {
-    double mid;
-    int tie;
+// Per-node choice of the MAC parameters: one set, or per system of a forest
+// (the system index is the key prefix of the node's first body).
+struct MacSel {
+    MacParam one;
+    const MacParam *tab;
+    const uint64_t *keys;
+    int shift;
+    __device__ __forceinline__ MacParam at(int first) const { return tab ? tab[keys[first] >> shift] : one; }
 };
 
 __device__ __forceinline__ float4 mac_record(float4 mn, float4 mx, MacParam mp) {
@@ -622,7 +642,7 @@ __global__ __launch_bounds__(256) void k_summ_local(const float4 *__restrict__ b
                                                     NodeRec *nodes,
                                                     const int32_t *__restrict__ leaf_start, int64_t max_nodes,
                                                     uint32_t *flags, uint32_t *glist, float4 *bmin,
-                                                    float4 *bmax, MacParam mp, uint32_t *err) {
+                                                    float4 *bmax, MacSel ms, uint32_t *err) {
     const int64_t lo = (int64_t)blockIdx.x * kSummBodies, hi = lo + kSummBodies;
     const int64_t end = hi < n ? hi : n;
     for (int64_t i = lo + threadIdx.x; i < end; i += blockDim.x) {
@@ -631,7 +651,7 @@ __global__ __launch_bounds__(256) void k_summ_local(const float4 *__restrict__ b
         if (s >= max_nodes) { atomicOr(err, 4u); continue; }
         const int4 t = nodes[s].topo;
         if (t.x != (int)i || t.y < 1 || (int64_t)t.x + t.y > n || t.z >= 0) { atomicOr(err, 4u); continue; }
-        summarize_leaf(s, t, bodies, nodes, bmin, bmax, mp);
+        summarize_leaf(s, t, bodies, nodes, bmin, bmax, ms.at(t.x));
         int parent = t.w;
         while (parent >= 0) {
             if (parent >= max_nodes) { atomicOr(err, 4u); break; }
@@ -645,7 +665,7 @@ __global__ __launch_bounds__(256) void k_summ_local(const float4 *__restrict__ b
             }
             if (arrive_wg(&flags[parent]) == 0) break;
             if (tp.z < 0 || tp.z + 1 >= max_nodes) { atomicOr(err, 4u); break; }
-            summarize_pair(parent, tp.z, nodes, bmin, bmax, mp);
+            summarize_pair(parent, tp.z, nodes, bmin, bmax, ms.at(tp.x));
             parent = tp.w;
         }
     }
@@ -653,7 +673,7 @@ __global__ __launch_bounds__(256) void k_summ_local(const float4 *__restrict__ b
 
 __global__ __launch_bounds__(1024) void k_summ_global(NodeRec *nodes, int64_t max_nodes, uint32_t *flags,
                                                       const uint32_t *glist, float4 *bmin, float4 *bmax,
-                                                      MacParam mp, uint32_t *err) {
+                                                      MacSel ms, uint32_t *err) {
     const uint32_t cnt = glist[0];
     for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
         int parent = (int)glist[64 + j];
@@ -662,7 +682,7 @@ __global__ __launch_bounds__(1024) void k_summ_global(NodeRec *nodes, int64_t ma
             const int4 tp = nodes[parent].topo;
             if (arrive_wg(&flags[parent]) == 0) break;
             if (tp.z < 0 || tp.z + 1 >= max_nodes) { atomicOr(err, 4u); break; }
-            summarize_pair(parent, tp.z, nodes, bmin, bmax, mp);
+            summarize_pair(parent, tp.z, nodes, bmin, bmax, ms.at(tp.x));
             parent = tp.w;
         }
     }
@@ -670,17 +690,117 @@ __global__ __launch_bounds__(1024) void k_summ_global(NodeRec *nodes, int64_t ma
 
 void launch_lbvh_summaries(const float4 *bodies, int64_t n, NodeRec *nodes, const int32_t *leaf_start,
                            int64_t max_nodes, uint32_t *flags, uint32_t *glist, float4 *bmin, float4 *bmax,
-                           float theta, uint32_t *err, hipStream_t s) {
+                           float theta, uint32_t *err, hipStream_t s, const MacParam *mp_tab,
+                           const uint64_t *keys, int sys_shift) {
     if (n <= 0) return;
-    MacParam mp;
-    mac_threshold(theta, &mp.mid, &mp.tie);
+    MacSel ms;
+    mac_threshold(theta, &ms.one.mid, &ms.one.tie);
+    ms.tab = mp_tab;
+    ms.keys = keys;
+    ms.shift = sys_shift;
     launch_fill_u32(flags, 0u, max_nodes, s);
     launch_fill_u32(glist, 0u, 64, s);
     const unsigned blocks = (unsigned)((n + kSummBodies - 1) / kSummBodies);
     hipLaunchKernelGGL(k_summ_local, dim3(blocks), dim3(256), 0, s, bodies, n, nodes, leaf_start, max_nodes, flags,
-                       glist, bmin, bmax, mp, err);
-    hipLaunchKernelGGL(k_summ_global, dim3(1), dim3(1024), 0, s, nodes, max_nodes, flags, glist, bmin, bmax, mp,
+                       glist, bmin, bmax, ms, err);
+    hipLaunchKernelGGL(k_summ_global, dim3(1), dim3(1024), 0, s, nodes, max_nodes, flags, glist, bmin, bmax, ms,
                        err);
+}
+
+// ------------------------------------------------------------------ forest
+// Independent systems s = 0..S-1 own the ranges [sys_first[s], sys_first[s+1])
+// of the concatenated state.  Each gets its own bounding box and 30-bit Morton
+// codes over its own bounding cube; the system index sits above the code
+// (key = s << 30 | code), so one sort orders every system and, with nodes
+// that span systems always split, the radix tree over the concatenation holds
+// each system's reference tree as the subtree whose range is that system.
+
+__global__ __launch_bounds__(256) void k_seg_bbox(const float *__restrict__ x, const float *__restrict__ y,
+                                                  const float *__restrict__ z, const int64_t *__restrict__ sys_first,
+                                                  float *bbox6) {
+    const int sys = blockIdx.x;
+    const int64_t lo = sys_first[sys], hi = sys_first[sys + 1];
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        float v[3] = {x[i], y[i], z[i]};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { mn[d] = fmin_ref(mn[d], v[d]); mx[d] = fmax_ref(mx[d], v[d]); }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = fmin_ref(mn[d], __shfl_xor(mn[d], o, kWave));
+            mx[d] = fmax_ref(mx[d], __shfl_xor(mx[d], o, kWave));
+        }
+    __shared__ float s_part[4][6];
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0)
+        for (int d = 0; d < 3; ++d) { s_part[w][d] = mn[d]; s_part[w][3 + d] = mx[d]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r[6];
+        for (int d = 0; d < 6; ++d) r[d] = s_part[0][d];
+        for (int ww = 1; ww < 4; ++ww)
+            for (int d = 0; d < 3; ++d) { r[d] = fmin_ref(r[d], s_part[ww][d]); r[3 + d] = fmax_ref(r[3 + d], s_part[ww][3 + d]); }
+        for (int d = 0; d < 6; ++d) bbox6[sys * 6 + d] = r[d];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_forest_keys(const float *__restrict__ x, const float *__restrict__ y,
+                                                     const float *__restrict__ z, int64_t n,
+                                                     const uint32_t *__restrict__ bsys,
+                                                     const float *__restrict__ bbox6, uint64_t *__restrict__ keys,
+                                                     uint32_t *__restrict__ idx) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t sys = bsys[i];
+        keys[i] = ((uint64_t)sys << 30) | (uint64_t)morton_key<uint32_t>(x[i], y[i], z[i], bbox6 + 6 * sys);
+        idx[i] = (uint32_t)i;
+    }
+}
+
+// A node is the root of system s iff its range lies inside s and its
+// parent's range does not.
+__global__ __launch_bounds__(256) void k_forest_roots(const NodeRec *__restrict__ nodes,
+                                                      const uint32_t *__restrict__ node_count, int64_t max_nodes,
+                                                      const uint64_t *__restrict__ keys, int64_t n, int nsys,
+                                                      int32_t *sys_root, uint32_t *err) {
+    const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (slot >= *node_count || slot >= max_nodes) return;
+    const int4 t = nodes[slot].topo;
+    if (t.x < 0 || t.y < 1 || (int64_t)t.x + t.y > n) { atomicOr(err, 4u); return; }
+    const uint64_t s0 = keys[t.x] >> 30, s1 = keys[t.x + t.y - 1] >> 30;
+    if (s0 != s1) return;
+    if (t.w >= 0) {
+        if ((int64_t)t.w >= max_nodes) { atomicOr(err, 4u); return; }
+        const int4 tp = nodes[t.w].topo;
+        if (tp.x < 0 || tp.y < 1 || (int64_t)tp.x + tp.y > n) { atomicOr(err, 4u); return; }
+        if ((keys[tp.x] >> 30) == (keys[tp.x + tp.y - 1] >> 30)) return;
+    }
+    if (s0 < (uint64_t)nsys) sys_root[s0] = (int32_t)slot;
+}
+
+void launch_forest_build(const float *x, const float *y, const float *z, int64_t n, int nsys,
+                         const int64_t *sys_first, const uint32_t *bsys, float *bbox6, uint64_t *keys, uint32_t *idx,
+                         uint64_t *keys_alt, uint32_t *idx_alt, uint32_t *sort_status, uint32_t *err, hipStream_t s) {
+    if (n <= 0 || nsys <= 0) return;
+    hipLaunchKernelGGL(k_seg_bbox, dim3((unsigned)nsys), dim3(256), 0, s, x, y, z, sys_first, bbox6);
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_forest_keys, dim3((unsigned)blocks), dim3(256), 0, s, x, y, z, n, bsys, bbox6, keys, idx);
+    int sbits = 0;
+    while ((1ll << sbits) < nsys) ++sbits;
+    int passes = (30 + sbits + 7) / 8;
+    if (passes & 1) ++passes;  // an even pass count leaves the result in keys / idx
+    launch_radix_sort_u64(keys, idx, keys_alt, idx_alt, n, passes, sort_status, err, s);
+}
+
+void launch_forest_roots(const NodeRec *nodes, const uint32_t *node_count, int64_t max_nodes, const uint64_t *keys,
+                         int64_t n, int nsys, int32_t *sys_root, uint32_t *err, hipStream_t s) {
+    if (n <= 0 || nsys <= 0) return;
+    launch_fill_u32((uint32_t *)sys_root, 0xFFFFFFFFu, nsys, s);
+    hipLaunchKernelGGL(k_forest_roots, dim3((unsigned)((max_nodes + 255) / 256)), dim3(256), 0, s, nodes, node_count,
+                       max_nodes, keys, n, nsys, sys_root, err);
 }
 
 }  // namespace nbg

@@ -151,16 +151,36 @@ void launch_gather(const float *x, const float *y, const float *z, const float *
 void launch_lbvh_topology(const void *keys, int bits, int64_t n, int leaf_size, int32_t *kfirst,
                           int32_t *klast, int32_t *kgamma, uint32_t *ksplit, uint32_t *krank,
                           int32_t *kparent, uint32_t *scan_ws, NodeRec *nodes, int32_t *leaf_start,
-                          uint32_t *node_count, hipStream_t s);
+                          uint32_t *node_count, hipStream_t s, int sys_shift = 0,
+                          const int32_t *leaf_tab = nullptr);
 
 // Bottom-up LBVH summaries with atomic arrival flags: COM, mass, bbox and the
 // MAC record (bbox centre, squared diagonal).  leaf_start[i] holds the leaf
 // slot whose first body is i (else -1; written by the topology pass);
 // flags holds max_nodes words; glist holds 2n + 64 words.
 constexpr int kSummBodies = 4096;   // bodies per workgroup in the local pass
+struct MacParam {       // mac_threshold's output for one theta
+    double mid;
+    int tie;
+};
+// mp_tab (forest): per-system MAC parameters, indexed by keys[first] >> sys_shift.
 void launch_lbvh_summaries(const float4 *bodies, int64_t n, NodeRec *nodes, const int32_t *leaf_start,
                            int64_t max_nodes, uint32_t *flags, uint32_t *glist, float4 *bmin, float4 *bmax,
-                           float theta, uint32_t *err, hipStream_t s);
+                           float theta, uint32_t *err, hipStream_t s, const MacParam *mp_tab = nullptr,
+                           const uint64_t *keys = nullptr, int sys_shift = 0);
+
+// Forest of independent systems (30-bit codes, u64 keys = system << 30 | code):
+// per-system bounding boxes (bbox6: 6 floats per system), keys, and one sort.
+void launch_forest_build(const float *x, const float *y, const float *z, int64_t n, int nsys,
+                         const int64_t *sys_first, const uint32_t *bsys, float *bbox6, uint64_t *keys, uint32_t *idx,
+                         uint64_t *keys_alt, uint32_t *idx_alt, uint32_t *sort_status, uint32_t *err, hipStream_t s);
+// sys_root[s] = slot of the node whose range is exactly system s.
+void launch_forest_roots(const NodeRec *nodes, const uint32_t *node_count, int64_t max_nodes, const uint64_t *keys,
+                         int64_t n, int nsys, int32_t *sys_root, uint32_t *err, hipStream_t s);
+// Forest update: dt of body i is sys_prm[bsys[i]].w.
+void launch_update_forest(float *x, float *y, float *z, float *vx, float *vy, float *vz, const float *ax,
+                          const float *ay, const float *az, const uint32_t *bsys, const float4 *sys_prm, int64_t n,
+                          hipStream_t s);
 
 // Host side of mac_lbvh_exact: the FP64 midpoint below T = RN(theta*theta)
 // and whether a tie rounds down (T has an odd significand).
@@ -194,6 +214,16 @@ struct WalkArgs {
     float *ax, *ay, *az;    // output order (nt each)
     unsigned long long *counts;  // [pp, np]
     uint32_t *err;
+    // Forest walk (nsys > 0): waves are aligned to systems.  Wave w serves
+    // system s with wave_first[s] <= w < wave_first[s+1], targets
+    // sys_first[s] + 64 (w - wave_first[s]) + lane below sys_first[s+1],
+    // starts at node sys_root[s] and uses G = sys_prm[s].x, eps2 = sys_prm[s].y.
+    int nsys = 0;
+    int64_t nwaves = 0;
+    const int64_t *wave_first = nullptr;
+    const int64_t *sys_first = nullptr;
+    const int32_t *sys_root = nullptr;
+    const float4 *sys_prm = nullptr;
 };
 
 // Barnes-Hut traversal over the LBVH.  ordered != 0 selects the

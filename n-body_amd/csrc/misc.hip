@@ -30,6 +30,69 @@ void launch_update(float *x, float *y, float *z, float *vx, float *vy, float *vz
     hipLaunchKernelGGL(k_update, dim3((unsigned)blocks), dim3(256), 0, s, x, y, z, vx, vy, vz, ax, ay, az, dt, n);
 }
 
+// The same update for a forest: each body takes its system's time step.
+__global__ __launch_bounds__(256) void k_update_forest(float *__restrict__ x, float *__restrict__ y,
+                                                       float *__restrict__ z, float *__restrict__ vx,
+                                                       float *__restrict__ vy, float *__restrict__ vz,
+                                                       const float *__restrict__ ax, const float *__restrict__ ay,
+                                                       const float *__restrict__ az,
+                                                       const uint32_t *__restrict__ bsys,
+                                                       const float4 *__restrict__ sys_prm, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float dt = sys_prm[bsys[i]].w;
+        const float ux = vx[i] + ax[i] * dt, uy = vy[i] + ay[i] * dt, uz = vz[i] + az[i] * dt;
+        vx[i] = ux; vy[i] = uy; vz[i] = uz;
+        x[i] = x[i] + ux * dt;
+        y[i] = y[i] + uy * dt;
+        z[i] = z[i] + uz * dt;
+    }
+}
+
+void launch_update_forest(float *x, float *y, float *z, float *vx, float *vy, float *vz, const float *ax,
+                          const float *ay, const float *az, const uint32_t *bsys, const float4 *sys_prm, int64_t n,
+                          hipStream_t s) {
+    if (n <= 0) return;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_update_forest, dim3((unsigned)blocks), dim3(256), 0, s, x, y, z, vx, vy, vz, ax, ay, az, bsys,
+                       sys_prm, n);
+}
+
+// Forest compaction after a system leaves: new position i of system s
+// takes old position old_first[s] + (i - new_first[s]), for up to 8 arrays.
+struct ArrayPtrs {
+    float *p[8];
+};
+
+__global__ __launch_bounds__(256) void k_forest_compact(ArrayPtrs dst, ArrayPtrs src, int arrays, int64_t n, int nsys,
+                                                        const int64_t *__restrict__ new_first,
+                                                        const int64_t *__restrict__ old_first) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = nsys;  // new_first[lo] <= i < new_first[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (new_first[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        const int64_t j = old_first[lo] + (i - new_first[lo]);
+        for (int k = 0; k < arrays; ++k) dst.p[k][i] = src.p[k][j];
+    }
+}
+
+void launch_forest_compact(float *const *dst, const float *const *src, int arrays, int64_t n, int nsys,
+                           const int64_t *new_first, const int64_t *old_first, hipStream_t s) {
+    if (n <= 0 || nsys <= 0 || arrays <= 0) return;
+    ArrayPtrs d{}, r{};
+    for (int k = 0; k < arrays && k < 8; ++k) {
+        d.p[k] = dst[k];
+        r.p[k] = const_cast<float *>(src[k]);
+    }
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_forest_compact, dim3((unsigned)blocks), dim3(256), 0, s, d, r, arrays < 8 ? arrays : 8, n,
+                       nsys, new_first, old_first);
+}
+
 // FP64 direct sum (NaiveSolver generalised, Solvers/NaiveSolver.h:22-52):
 // one thread per target, sources staged through LDS.  Sources at exactly the
 // target position are skipped, as Gravity::particleParticle does

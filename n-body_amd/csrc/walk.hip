@@ -35,6 +35,39 @@ constexpr int kStackFast = 96;
 // Upper bound on loop iterations per wave; a valid tree needs at most ~3 per node.
 constexpr uint32_t kMaxWalkIters = 1u << 28;
 
+// The targets, root and rule of wave w: the whole tree for one system, or
+// the system that owns wave w in a forest (all wave-uniform).
+struct WaveTask {
+    int64_t base, end;
+    int root;
+    float G, eps2;
+};
+
+__device__ __forceinline__ WaveTask wave_task(const WalkArgs &a, int64_t w) {
+    WaveTask t;
+    if (a.nsys <= 0) {
+        t.base = w * kWave; t.end = a.nt; t.root = 0; t.G = a.G; t.eps2 = a.eps2;
+        return t;
+    }
+    if (w >= a.nwaves) {
+        t.base = 0; t.end = 0; t.root = 0; t.G = a.G; t.eps2 = a.eps2;
+        return t;
+    }
+    int lo = 0, hi = a.nsys;  // wave_first[lo] <= w < wave_first[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a.wave_first[mid] <= w) lo = mid;
+        else hi = mid;
+    }
+    t.base = a.sys_first[lo] + (w - a.wave_first[lo]) * kWave;
+    t.end = a.sys_first[lo + 1];
+    t.root = a.sys_root[lo];
+    const float4 q = a.sys_prm[lo];
+    t.G = q.x;
+    t.eps2 = q.y;
+    return t;
+}
+
 // Gravity::particleParticle, exact reference arithmetic.
 __device__ __forceinline__ void pp_exact(float bx, float by, float bz, float bm, float px, float py, float pz,
                                          float G, float eps2, float &vx, float &vy, float &vz) {
@@ -72,8 +105,9 @@ __global__ __launch_bounds__(64) void k_walk_lbvh_ordered(WalkArgs a) {
     const float4 *__restrict__ bodies = a.bodies;
     const NodeRec *__restrict__ nodes = a.nodes;
     const int lane = lane_id();
-    const int64_t tgt = xcd_block(blockIdx.x, gridDim.x) * kWave + lane;
-    const bool valid = tgt < a.nt;
+    const WaveTask T = wave_task(a, xcd_block(blockIdx.x, gridDim.x));
+    const int64_t tgt = T.base + lane;
+    const bool valid = tgt < T.end;
     const uint64_t vmask = __ballot(valid);
     if (vmask == 0) return;
 
@@ -81,9 +115,9 @@ __global__ __launch_bounds__(64) void k_walk_lbvh_ordered(WalkArgs a) {
     const float px = me.x, py = me.y, pz = me.z;
     float rx = 0.0f, ry = 0.0f, rz = 0.0f;
     uint32_t cpp = 0, cnp = 0;
-    const float G = a.G, eps2 = a.eps2;
+    const float G = T.G, eps2 = T.eps2;
 
-    if (lane == 0) { s_node[0] = 0; s_meta[0] = 0; s_mask[0] = vmask; }
+    if (lane == 0) { s_node[0] = T.root; s_meta[0] = 0; s_mask[0] = vmask; }
     int sp = 1;
     bool fail = false;
     uint32_t iters = 0;
@@ -233,8 +267,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     const float4 *__restrict__ bodies = a.bodies;
     const NodeRec *__restrict__ nodes = a.nodes;
     const int lane = lane_id(), wv = threadIdx.x / kWave;
-    const int64_t tgt = (xcd_block(blockIdx.x, gridDim.x) * WAVES + wv) * kWave + lane;
-    const bool valid = tgt < a.nt;
+    const WaveTask T = wave_task(a, xcd_block(blockIdx.x, gridDim.x) * WAVES + wv);
+    const int64_t tgt = T.base + lane;
+    const bool valid = tgt < T.end;
     const uint64_t vmask = __ballot(valid);
     if (vmask == 0) return;
 
@@ -246,15 +281,19 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     W.visits = 0; W.lanes = 0; W.leaf_visits = 0; W.leaf_lanes = 0;
     // A VGPR copy of G (non-volatile asm: no memory effects, so the node and
     // body loads below stay scalar).
-    asm("v_mov_b32 %0, %1" : "=v"(W.G) : "s"(a.G));
-    const float eps2 = a.eps2;
+    asm("v_mov_b32 %0, %1" : "=v"(W.G) : "s"(T.G));
+    const float eps2 = T.eps2;
     const int64_t n = a.n;
     const uint32_t last_pair = (uint32_t)(a.max_nodes - 1);
     bool fail = false;
 
     // The root is visited on its own; afterwards every step visits the two
     // children (c, c + 1) of a node that the lanes in `mask` opened.
-    const NodeRec root = nodes[0];
+    if (T.root < 0 || T.root >= a.max_nodes) {
+        if (lane == 0) atomicOr(a.err, 2u);
+        return;
+    }
+    const NodeRec root = nodes[T.root];
     uint64_t mask = visit<SOFT, STATS>(root, vmask, bodies, n, eps2, W, fail);
     int c = root.topo.z;
     int sp = 0;
@@ -593,10 +632,13 @@ void launch_walk_oct(const WalkArgs &a, int ordered, int stats, hipStream_t s) {
     }
 }
 
+// Waves the walk launches: one per 64 targets, or the forest's aligned waves.
+static int64_t walk_waves(const WalkArgs &a) { return a.nsys > 0 ? a.nwaves : (a.nt + kWave - 1) / kWave; }
+
 template <int STATS>
 static void launch_fast(const WalkArgs &a, hipStream_t s) {
     constexpr int W = 4;
-    const unsigned blocks = (unsigned)((a.nt + 64 * W - 1) / (64 * W));
+    const unsigned blocks = (unsigned)((walk_waves(a) + W - 1) / W);
     if (a.eps2 > 0.0f)
         hipLaunchKernelGGL((k_walk_lbvh_fast<W, true, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
     else
@@ -610,7 +652,7 @@ void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, int stats, 
         else if (stats == 1) launch_fast<1>(a, s);
         else launch_fast<0>(a, s);
     } else {
-        const unsigned blocks = (unsigned)((a.nt + 63) / 64);
+        const unsigned blocks = (unsigned)walk_waves(a);
         if (max_depth <= 32)
             hipLaunchKernelGGL((k_walk_lbvh_ordered<32>), dim3(blocks), dim3(64), 0, s, a);
         else

@@ -1,0 +1,154 @@
+"""Batched forest: independent linear-BVH systems built and walked together.
+
+Every system's result must be the bits it gets alone (an nbg_sys with the
+same parameters, 30-bit keys, same sum mode) — in both sum modes, because
+each wavefront serves one system and each system's subtree, summaries and
+traversal order are its own — and, in ORDERED mode, the oracle's bits."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(nbg, theta=0.5, leaf=16, eps2=1e-4, G=1.0, dt=1e-3, mode=1):
+    p = nbg.default_params()
+    p.solver, p.morton_bits = nbg.LINEAR_BVH, 30
+    p.theta, p.leaf_size, p.eps2, p.G, p.dt, p.sum_mode = theta, leaf, eps2, G, dt, mode
+    return p
+
+
+def _systems(nbg):
+    """Uneven sizes (wave-boundary cases included) and per-system parameters."""
+    spec = [(1, 0.5, 16, 1e-4, 1.0, 1e-3), (63, 0.5, 8, 1e-4, 1.0, 1e-3), (64, 0.3, 16, 1e-4, 2.0, 5e-4),
+            (65, 0.7, 4, 1e-3, 1.0, 1e-3), (1000, 0.5, 16, 1e-4, 1.0, 1e-3), (5003, 0.5, 32, 2.5e-5, 0.5, 2e-3),
+            (20000, 0.4, 16, 1e-4, 1.0, 1e-3), (3, 0.5, 1, 0.0, 1.0, 1e-3)]
+    out = []
+    for k, (n, th, lf, e2, G, dt) in enumerate(spec):
+        out.append((nbg.plummer(n, seed=500 + k, a=1.0 + 0.1 * k), dict(theta=th, leaf=lf, eps2=e2, G=G, dt=dt)))
+    return out
+
+
+def _alone(nbg, ctx, b, kw, mode, steps=0):
+    s = nbg.System(ctx, _params(nbg, mode=mode, **kw), b.n)
+    s.set_state(b)
+    if steps:
+        s.step(steps)
+        s.synchronize()
+        st = s.get_state()
+        a = s.accelerations()
+    else:
+        a = s.update_accelerations()
+        st = None
+    c = s.counts()
+    s.close()
+    return a, st, c
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_forest_pass_equals_alone_and_oracle(nbg, orc, gpu_ctx, mode):
+    systems = _systems(nbg)
+    B = nbg.Batch(gpu_ctx, 40000, 16, sum_mode=mode)
+    ids = [B.add(_params(nbg, mode=mode, **kw), b) for b, kw in systems]
+    B.update_accelerations()
+    tot = [0, 0]
+    for sid, (b, kw) in zip(ids, systems):
+        print(f"[forest pass] mode={mode} n={b.n} {kw}", flush=True)
+        _, acc = B.get(sid, accelerations=True)
+        a1, _, c = _alone(nbg, gpu_ctx, b, kw, mode)
+        tot[0] += c[0]
+        tot[1] += c[1]
+        assert np.array_equal(acc.view(np.uint32), a1.view(np.uint32)), f"n={b.n}: batch differs from alone"
+        if mode == 0:
+            t = orc.Tree(1, b.x, b.y, b.z, b.m, theta=kw["theta"], leaf=kw["leaf"], bits=30, G=kw["G"],
+                         eps2=kw["eps2"])
+            assert np.array_equal(acc, t.accel(b.x, b.y, b.z)), f"n={b.n}: batch differs from the oracle"
+    assert B.counts() == tuple(tot)
+    B.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_forest_membership_changes(nbg, gpu_ctx, mode):
+    """Systems join and leave between steps; the others' trajectories are
+    unaffected (same bits as alone with the same number of steps)."""
+    systems = _systems(nbg)
+    B = nbg.Batch(gpu_ctx, 60000, 16, sum_mode=mode)
+    ids = [B.add(_params(nbg, mode=mode, **kw), b) for b, kw in systems[:6]]
+    steps = {sid: 0 for sid in ids}
+    data = {sid: systems[k] for k, sid in enumerate(ids)}
+    B.step(3)
+    for sid in ids:
+        steps[sid] += 3
+    B.remove(ids[2])           # a middle system leaves
+    del steps[ids[2]]
+    B.remove(ids[0])           # and the first
+    del steps[ids[0]]
+    late = B.add(_params(nbg, mode=mode, **systems[6][1]), systems[6][0])  # a new one joins
+    steps[late] = 0
+    data[late] = systems[6]
+    B.step(2)
+    for sid in steps:
+        steps[sid] += 2
+    B.synchronize()
+    assert B.info() == (5, sum(data[s][0].n for s in steps))
+    for sid, k in steps.items():
+        b, kw = data[sid]
+        st = B.get(sid)
+        _, ref, _ = _alone(nbg, gpu_ctx, b, kw, mode, steps=k)
+        for g, r in zip((st.x, st.y, st.z, st.vx, st.vy, st.vz), (ref.x, ref.y, ref.z, ref.vx, ref.vy, ref.vz)):
+            assert np.array_equal(g, r), f"system {sid} (n={b.n}) after {k} steps"
+    B.close()
+
+
+def test_forest_graph_replay_equals_eager(nbg, gpu_ctx):
+    systems = _systems(nbg)[:6]
+    runs = []
+    for graph in (False, True):
+        B = nbg.Batch(gpu_ctx, 40000, 8)
+        ids = [B.add(_params(nbg, **kw), b) for b, kw in systems]
+        B.use_graph(graph)
+        snaps = []
+        for _ in range(4):     # replays with host synchronisation in between
+            B.step(1)
+            B.synchronize()
+            snaps.append([B.get(sid).x.copy() for sid in ids])
+        runs.append(snaps)
+        B.close()
+    for r_eager, r_graph in zip(*runs):
+        for a, b in zip(r_eager, r_graph):
+            assert np.array_equal(a, b)
+
+
+def test_forest_autotune(nbg, gpu_ctx):
+    systems = _systems(nbg)
+    B = nbg.Batch(gpu_ctx, 40000, 16)
+    ids = [B.add(_params(nbg, **kw), b) for b, kw in systems]
+    before = [B.get(sid).x.copy() for sid in ids]
+    best, ms = B.autotune((8, 16, 32, 64), repeats=2)
+    print(f"[forest autotune] best leaf {best}, ms {ms}", flush=True)
+    assert best in (8, 16, 32, 64) and all(v > 0 for v in ms.values())
+    for sid, x0 in zip(ids, before):
+        assert np.array_equal(B.get(sid).x, x0), "autotuning must not move bodies"
+    B.update_accelerations()
+    b, kw = systems[4]
+    _, acc = B.get(ids[4], accelerations=True)
+    kw = dict(kw, leaf=best)
+    a1, _, _ = _alone(nbg, gpu_ctx, b, kw, 1)
+    assert np.array_equal(acc, a1)
+    B.close()
+
+
+def test_forest_capacity_and_arguments(nbg, gpu_ctx):
+    B = nbg.Batch(gpu_ctx, 100, 2)
+    b = nbg.plummer(60, seed=1)
+    B.add(_params(nbg), b)
+    with pytest.raises(nbg.NbgError):
+        B.add(_params(nbg), b)            # 120 > 100 bodies
+    p = _params(nbg)
+    p.solver = nbg.BARNES_HUT
+    with pytest.raises(nbg.NbgError):
+        B.add(p, nbg.plummer(10, seed=2))  # octree systems are not batched
+    B.add(_params(nbg), nbg.plummer(10, seed=3))
+    with pytest.raises(nbg.NbgError):
+        B.add(_params(nbg), nbg.plummer(10, seed=4))  # 3 > 2 systems
+    B.close()

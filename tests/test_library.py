@@ -88,3 +88,9 @@ def test_facade_program_built_and_reports_missing_gpu(nbg):
         pytest.skip("a GPU is present: the GPU tests run the program")
     r = subprocess.run([prog, "gpu-barnes-hut", "16", os.devnull], capture_output=True, text=True, timeout=60)
     assert r.returncode == 3, r.stderr
+
+
+def test_batch_bytes_grow_with_capacity(nbg):
+    a = nbg.Batch.bytes_for(1 << 20, 4096)
+    b = nbg.Batch.bytes_for(1 << 21, 4096)
+    assert 200 * (1 << 20) < a < b < 2 * a + (1 << 20)
