@@ -181,6 +181,185 @@ def measure(nbg, ctx, b, args, solver):
     return out
 
 
+def splitmix64(x):
+    """SplitMix64 output function (Steele, Lea, Flood 2014) on a 64-bit state."""
+    m = (1 << 64) - 1
+    z = (x + 0x9E3779B97F4A7C15) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    return z ^ (z >> 31)
+
+
+def ensemble(nbg, nsys, nmin, nmax, seed):
+    """Batched-ensemble inputs: N_i = floor(exp U[ln nmin, ln nmax]), scale
+    radius a_i ~ U[0.5, 2], M = 1, per-system seed SplitMix64(seed + i)."""
+    rng = np.random.default_rng(seed)
+    ns = np.floor(np.exp(rng.uniform(np.log(nmin), np.log(nmax), nsys))).astype(np.int64)
+    aa = rng.uniform(0.5, 2.0, nsys)
+    return [(nbg.plummer(int(n), seed=splitmix64(seed + i), a=float(a)), float(a)) for i, (n, a) in
+            enumerate(zip(ns, aa))]
+
+
+def bench_forest(nbg, ctx, args):
+    """Batched ensemble: many independent Plummer systems stepped together
+    (one build + one walk launch per step, replayed from a HIP graph), leaf
+    size picked by the autotuner, against the same systems stepped one at a
+    time through nbg_sys (a sample, scaled by bodies)."""
+    log(f"[bench] forest: generating {args.forest_systems} systems")
+    systems = ensemble(nbg, args.forest_systems, 256, 65536, args.forest_seed)
+    total = sum(b.n for b, _ in systems)
+    p = nbg.default_params()
+    p.solver, p.morton_bits, p.theta, p.leaf_size = nbg.LINEAR_BVH, 30, 0.5, 16
+    p.eps2, p.dt, p.sum_mode = 1e-4, 1e-3, nbg.SUM_FAST
+    B = nbg.Batch(ctx, total, len(systems), sum_mode=nbg.SUM_FAST)
+    for b, _ in systems:
+        B.add(p, b)
+    B.enable_counts(False)
+    log(f"[bench] forest: {len(systems)} systems, {total} bodies; autotuning leaf size")
+    best, tune_ms = B.autotune((8, 16, 32, 64), repeats=2)
+    p.leaf_size = best
+    B.use_graph(True)
+    B.step(1)            # capture + warm-up
+    B.synchronize()
+    t0 = time.perf_counter()
+    B.step(args.forest_steps)
+    B.synchronize()
+    el = time.perf_counter() - t0
+    B.enable_counts(True)
+    B.update_accelerations()
+    pp, npc = B.counts()
+    B.close()
+    value = total * args.forest_steps / el
+    # one system at a time through nbg_sys, on a sample of the same systems
+    sample = np.random.default_rng(1).choice(len(systems), min(args.forest_alone_sample, len(systems)),
+                                             replace=False)
+    t_alone, bodies_alone = 0.0, 0
+    for k in sample:
+        b, _ = systems[int(k)]
+        s = nbg.System(ctx, p, b.n)
+        s.set_state(b)
+        s.enable_counts(False)
+        s.step(1)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(args.forest_steps)
+        s.synchronize()
+        t_alone += time.perf_counter() - t0
+        bodies_alone += b.n
+        s.close()
+    alone = bodies_alone * args.forest_steps / t_alone
+    return {
+        "workload": f"{len(systems)} Plummer systems, N log-uniform in [256, 65536] (total {total}), a ~ U[0.5, 2], "
+                    f"M = 1, eps2 = 1e-4, theta = 0.5, dt = 1e-3, {args.forest_steps} steps, linear BVH 30-bit keys, "
+                    f"FAST sums, seed {args.forest_seed}",
+        "value": value,
+        "unit": "bodies·steps/s",
+        "ms_per_step": 1e3 * el / args.forest_steps,
+        "systems": len(systems),
+        "bodies": total,
+        "leaf_size_autotuned": best,
+        "autotune_ms_per_pass": tune_ms,
+        "interactions_per_step": pp + npc,
+        "interactions_per_s": (pp + npc) * args.forest_steps / el,
+        "roofline_frac_of_fp32_peak": FLOPS_PER_INTERACTION * (pp + npc) * args.forest_steps / el / 1e12
+                                      / FP32_PEAK_TFLOPS,
+        "one_at_a_time": {"value": alone, "unit": "bodies·steps/s",
+                          "sample": f"{len(sample)} of the systems stepped alone through nbg_sys "
+                                    f"(same parameters and leaf size), bodies-weighted"},
+        "batched_over_one_at_a_time": value / alone,
+    }
+
+
+def bench_serving(nbg, ctx, args, capacity):
+    """Serving: a Poisson stream of jobs (Plummer sphere or uniform cube, N
+    log-uniform in [1024, 2097152], steps uniform in [10, 500], theta in
+    {0.3, 0.5, 0.7}, eps2 = 1e-4) submitted at their arrival times to a
+    continuous-batching server with a 200 GB HBM budget.  The arrival rate
+    offers `serve_load` x `capacity` bodies*steps/s.  Reports latency
+    percentiles and aggregate throughput, and the same jobs' throughput when
+    run one at a time through nbg_sys (a sample, steps capped, scaled)."""
+    rng = np.random.default_rng(args.serve_seed)
+    specs = []
+    for k in range(args.serve_jobs):
+        n = int(np.floor(np.exp(rng.uniform(np.log(1024), np.log(2097152)))))
+        kind = "plummer" if rng.random() < 0.5 else "cube"
+        steps = int(rng.integers(10, 501))
+        theta = float(rng.choice([0.3, 0.5, 0.7]))
+        specs.append((kind, n, steps, theta, splitmix64(args.serve_seed * 1000003 + k)))
+    work = np.array([n * st for _, n, st, _, _ in specs], np.float64)
+    rate = args.serve_load * capacity / work.mean()
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(specs)))
+    log(f"[bench] serving: {len(specs)} jobs, {work.sum():.3g} body-steps, arrival rate {rate:.3g}/s")
+    bodies = [nbg.plummer(n, seed=sd) if kind == "plummer" else nbg.cube(n, seed=sd, side=2.0)
+              for kind, n, _, _, sd in specs]
+
+    def params(theta):
+        p = nbg.default_params()
+        p.solver, p.morton_bits, p.theta, p.leaf_size = nbg.LINEAR_BVH, 30, theta, 16
+        p.eps2, p.dt, p.sum_mode = 1e-4, 1e-3, nbg.SUM_FAST
+        return p
+
+    srv = nbg.Server(ctx, 200 * 10 ** 9, max_systems=4096, sum_mode=nbg.SUM_FAST)
+    cap_bodies, _ = srv.capacity()
+    ids = []
+    t0 = time.perf_counter()
+    for (kind, n, steps, theta, _), b, t_arr in zip(specs, bodies, arrivals):
+        now = time.perf_counter() - t0
+        if t_arr > now:
+            time.sleep(t_arr - now)
+        ids.append(srv.submit(params(theta), b, steps))
+    lat, dev, que = [], [], []
+    for j in ids:
+        _, t = srv.wait(j)
+        lat.append(t["latency"])
+        dev.append(t["device"])
+        que.append(t["queued"])
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    served = work.sum() / wall
+    # one job at a time through nbg_sys: a sample, at most 20 timed steps each
+    sample = np.random.default_rng(2).choice(len(specs), min(args.serve_alone_sample, len(specs)), replace=False)
+    t_tot, w_tot = 0.0, 0.0
+    for k in sample:
+        kind, n, steps, theta, _ = specs[int(k)]
+        s = nbg.System(ctx, params(theta), n)
+        s.set_state(bodies[int(k)])
+        s.enable_counts(False)
+        s.step(1)
+        s.synchronize()
+        k_steps = min(steps, 20)
+        t1 = time.perf_counter()
+        s.step(k_steps)
+        s.synchronize()
+        t_tot += (time.perf_counter() - t1) * steps / k_steps
+        w_tot += n * steps
+        s.close()
+    alone = w_tot / t_tot
+    lat = np.array(lat)
+    return {
+        "workload": f"{len(specs)} jobs (Plummer or cube side 2, N log-uniform [1024, 2097152], steps U[10, 500], "
+                    f"theta in {{0.3, 0.5, 0.7}}, eps2 1e-4, dt 1e-3, linear BVH 30-bit, FAST), Poisson arrivals at "
+                    f"{rate:.3g}/s = {args.serve_load} x {capacity:.3g} bodies*steps/s, 200 GB HBM budget "
+                    f"({cap_bodies} bodies), seed {args.serve_seed}",
+        "value": served,
+        "unit": "bodies·steps/s",
+        "latency_p50_s": float(np.quantile(lat, 0.5)),
+        "latency_p99_s": float(np.quantile(lat, 0.99)),
+        "queue_p50_s": float(np.quantile(que, 0.5)),
+        "queue_p99_s": float(np.quantile(que, 0.99)),
+        "wall_s": wall,
+        "device_busy_s": st["busy_s"],
+        "forest_steps": int(st["steps_run"]),
+        "peak_bodies_resident": int(st["peak_bodies"]),
+        "peak_systems_resident": int(st["peak_systems"]),
+        "one_at_a_time": {"value": alone, "unit": "bodies·steps/s",
+                          "sample": f"{len(sample)} of the jobs run alone through nbg_sys, at most 20 timed steps "
+                                    f"each, scaled to their step counts"},
+        "served_over_one_at_a_time": served / alone,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,6 +378,16 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-forest", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--forest-systems", type=int, default=4096)
+    ap.add_argument("--forest-steps", type=int, default=20)
+    ap.add_argument("--forest-seed", type=int, default=7)
+    ap.add_argument("--forest-alone-sample", type=int, default=64)
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--serve-jobs", type=int, default=100)
+    ap.add_argument("--serve-load", type=float, default=0.8)
+    ap.add_argument("--serve-seed", type=int, default=11)
+    ap.add_argument("--serve-alone-sample", type=int, default=12)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device tensor copy three times (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -248,6 +437,21 @@ def main():
             res[sv]["cpu_baseline"] = cb
             res[sv]["gpu_over_cpu"] = res[sv]["value"] / cb["value"]
 
+    forest = None
+    if not args.no_forest:
+        try:
+            forest = bench_forest(nbg, ctx, args)
+        except Exception as e:  # reported, not fatal for the headline line
+            forest = {"error": f"{type(e).__name__}: {e}"}
+
+    serving = None
+    if not args.no_serving:
+        try:
+            cap = forest["value"] if forest and "value" in forest else 1e8
+            serving = bench_serving(nbg, ctx, args, cap)
+        except Exception as e:
+            serving = {"error": f"{type(e).__name__}: {e}"}
+
     head = res[solvers[0]]
     out = {
         "metric": METRIC,
@@ -274,6 +478,10 @@ def main():
             out[k] = v
     for sv in solvers[1:]:
         out[SOLVER_NAMES[sv].replace("-", "_")] = res[sv]
+    if forest is not None:
+        out["batched_ensemble"] = forest
+    if serving is not None:
+        out["serving"] = serving
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -249,6 +249,41 @@ int nbg_batch_use_graph(nbg_batch *b, int32_t on);
 int nbg_batch_autotune(nbg_batch *b, const int32_t *leaf_sizes, int32_t k, int32_t repeats, int32_t *best,
                        float *ms_out);
 
+/* ---- serving: continuous batching over a forest ----
+ *
+ * No reference counterpart.  Jobs (a linear-BVH system, 30-bit keys, plus a
+ * step count) are submitted from any thread; a worker thread admits them in
+ * arrival order whenever the forest's capacity allows, steps all admitted
+ * systems together, and retires each when its step count is reached.  The
+ * forest's capacity is the largest that fits hbm_budget_bytes
+ * (nbg_batch_bytes).  A job's result is the same bits as the same system
+ * stepped alone in an nbg_sys. */
+typedef struct nbg_server nbg_server;
+
+int nbg_server_create(nbg_ctx *ctx, int64_t hbm_budget_bytes, int32_t max_systems, int32_t sum_mode,
+                      nbg_server **out);
+/* Stops the worker (jobs still queued or running are abandoned). */
+int nbg_server_destroy(nbg_server *s);
+int nbg_server_capacity(nbg_server *s, int64_t *max_bodies, int32_t *max_systems);
+
+/* Queues a job (host arrays copied; vx/vy/vz may be NULL); *job receives its
+ * id.  NBG_ERR_BUDGET if the job alone exceeds the capacity. */
+int nbg_server_submit(nbg_server *s, const nbg_params *p, int64_t n, const float *x, const float *y, const float *z,
+                      const float *m, const float *vx, const float *vy, const float *vz, int32_t nsteps,
+                      int64_t *job);
+
+/* Waits for a job (timeout_s < 0: no limit; NBG_ERR_STATE on time-out), then
+ * copies its final state out (any pointer may be NULL) and its times in
+ * seconds: times3 = {queued, on the device, submit-to-done latency}.  A job's
+ * result can be collected once. */
+int nbg_server_wait(nbg_server *s, int64_t job, double timeout_s, float *x, float *y, float *z, float *vx, float *vy,
+                    float *vz, double *times3);
+
+/* out8 = {jobs finished, forest steps run, body-steps, device-busy seconds,
+ * seconds since creation, peak bodies resident, peak systems resident, jobs
+ * waiting}. */
+int nbg_server_stats(nbg_server *s, double *out8);
+
 /* ---- synthetic inputs (host, CPU) ---- */
 
 /* Plummer sphere: scale radius a, total mass M, gravitational constant G,

@@ -44,6 +44,8 @@ EXPORTS = (
     "nbg_batch_set_params", "nbg_batch_info", "nbg_batch_get", "nbg_batch_update_accelerations", "nbg_batch_step",
     "nbg_batch_synchronize", "nbg_batch_counts", "nbg_batch_enable_counts", "nbg_batch_use_graph",
     "nbg_batch_autotune",
+    "nbg_server_create", "nbg_server_destroy", "nbg_server_capacity", "nbg_server_submit", "nbg_server_wait",
+    "nbg_server_stats",
 )
 
 
@@ -123,6 +125,16 @@ def lib() -> ctypes.CDLL:
             "nbg_batch_use_graph": (ctypes.c_int, [_vp, ctypes.c_int32]),
             "nbg_batch_autotune": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, ctypes.c_int32,
                                                   ctypes.POINTER(ctypes.c_int32), _f32p]),
+            "nbg_server_create": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                                 ctypes.POINTER(_vp)]),
+            "nbg_server_destroy": (ctypes.c_int, [_vp]),
+            "nbg_server_capacity": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64),
+                                                   ctypes.POINTER(ctypes.c_int32)]),
+            "nbg_server_submit": (ctypes.c_int, [_vp, ctypes.POINTER(Params), ctypes.c_int64] + [_f32p] * 7
+                                  + [ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
+            "nbg_server_wait": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_double] + [_f32p] * 6
+                                + [ctypes.POINTER(ctypes.c_double)]),
+            "nbg_server_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
             "nbg_sys_step": (ctypes.c_int, [_vp, ctypes.c_int32]),
             "nbg_sys_synchronize": (ctypes.c_int, [_vp]),
             "nbg_sys_counts": (ctypes.c_int, [_vp, _u64p, _u64p]),
@@ -449,6 +461,57 @@ class Batch:
         _check(lib().nbg_batch_autotune(self.handle, ls.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(ls),
                                         repeats, ctypes.byref(best), _ptr(ms)))
         return best.value, {int(a): float(t) for a, t in zip(ls, ms)}
+
+
+class Server:
+    """nbg_server: continuous batching of submitted jobs over one forest,
+    within an HBM byte budget."""
+
+    def __init__(self, ctx: Context, hbm_budget_bytes: int, max_systems: int = 4096, sum_mode: int = SUM_FAST):
+        h = _vp()
+        _check(lib().nbg_server_create(ctx.handle, int(hbm_budget_bytes), max_systems, sum_mode, ctypes.byref(h)))
+        self.handle = h
+        self._n = {}
+
+    def close(self):
+        if self.handle:
+            lib().nbg_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def capacity(self):
+        nb, ns = ctypes.c_int64(), ctypes.c_int32()
+        _check(lib().nbg_server_capacity(self.handle, ctypes.byref(nb), ctypes.byref(ns)))
+        return nb.value, ns.value
+
+    def submit(self, params: Params, b: Bodies, nsteps: int) -> int:
+        arrs = [_f32(v) for v in (b.x, b.y, b.z, b.m, b.vx, b.vy, b.vz)]
+        job = ctypes.c_int64()
+        _check(lib().nbg_server_submit(self.handle, ctypes.byref(params), b.n, *[_ptr(a) for a in arrs], nsteps,
+                                       ctypes.byref(job)))
+        self._n[job.value] = b.n
+        return job.value
+
+    def wait(self, job: int, timeout: float = -1.0):
+        """(final Bodies without masses, {queued, device, latency} seconds)."""
+        n = self._n[job]
+        out = Bodies.empty(n)
+        t = (ctypes.c_double * 3)()
+        _check(lib().nbg_server_wait(self.handle, job, float(timeout),
+                                     *[_ptr(v) for v in (out.x, out.y, out.z, out.vx, out.vy, out.vz)], t))
+        del self._n[job]
+        return out, {"queued": t[0], "device": t[1], "latency": t[2]}
+
+    def stats(self) -> dict:
+        o = (ctypes.c_double * 8)()
+        _check(lib().nbg_server_stats(self.handle, o))
+        keys = ("jobs_done", "steps_run", "body_steps", "busy_s", "wall_s", "peak_bodies", "peak_systems", "waiting")
+        return {k: float(v) for k, v in zip(keys, o)}
 
 
 # ------------------------------------------------------------------ reference-shaped facade

@@ -1,0 +1,288 @@
+// Continuous-batching server over one batched forest (host code only).
+//
+// Clients submit jobs (a system plus a step count) from any thread.  A
+// worker thread owns the forest: before every step it admits waiting jobs in
+// arrival order while the forest's body and system capacity (sized from an
+// HBM byte budget) allow, steps every admitted system once, and retires the
+// systems that have reached their step count (final state copied to host
+// memory, slot freed).  A job therefore runs alongside whatever else is on
+// the device, and its result is the same bits it gets alone.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/nbg.h"
+#include "host_util.hpp"
+#include "launch.hpp"
+
+using namespace nbg;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Job {
+    int64_t id = 0;
+    nbg_params p{};
+    int64_t n = 0;
+    int32_t steps = 0, remaining = 0;
+    std::vector<float> st[7];   // x y z m vx vy vz in; x y z vx vy vz out
+    int32_t sys = -1;           // forest handle while admitted
+    Clock::time_point t_submit, t_admit, t_done;
+    bool done = false;
+    int status = NBG_OK;
+    std::string error;
+};
+
+double secs(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+}  // namespace
+
+struct nbg_server {
+    nbg_ctx *ctx = nullptr;
+    nbg_batch *batch = nullptr;
+    int64_t cap_n = 0;
+    int32_t cap_s = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Job>> pending;
+    std::vector<std::shared_ptr<Job>> active;
+    std::map<int64_t, std::shared_ptr<Job>> jobs;
+    int64_t next_id = 1;
+    int64_t used_n = 0;
+    bool stop = false;
+    std::thread worker;
+    // statistics
+    Clock::time_point t_start;
+    double busy_s = 0.0;
+    int64_t steps_run = 0, jobs_done = 0;
+    double body_steps = 0.0;
+    int64_t peak_bodies = 0, peak_systems = 0;
+    std::string worker_error;
+};
+
+namespace {
+
+void worker_loop(nbg_server *S) {
+    DeviceGuard g(S->ctx->device);
+    for (;;) {
+        std::vector<std::shared_ptr<Job>> admit;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty() || !S->active.empty(); });
+            if (S->stop) return;
+            // first come, first served: the queue head waits until it fits
+            while (!S->pending.empty()) {
+                auto &j = S->pending.front();
+                if (S->used_n + j->n > S->cap_n || (int64_t)S->active.size() + (int64_t)admit.size() >= S->cap_s)
+                    break;
+                S->used_n += j->n;
+                admit.push_back(j);
+                S->pending.pop_front();
+            }
+        }
+        for (auto &j : admit) {
+            int32_t sid = -1;
+            const int r = nbg_batch_add(S->batch, &j->p, j->n, j->st[0].data(), j->st[1].data(), j->st[2].data(),
+                                        j->st[3].data(), j->st[4].data(), j->st[5].data(), j->st[6].data(), &sid);
+            std::lock_guard<std::mutex> lk(S->mu);
+            j->t_admit = Clock::now();
+            if (r != NBG_OK) {
+                j->status = r;
+                j->error = nbg_last_error();
+                j->done = true;
+                j->t_done = j->t_admit;
+                S->used_n -= j->n;
+                S->cv_done.notify_all();
+                continue;
+            }
+            j->sys = sid;
+            S->active.push_back(j);
+        }
+        int64_t nb = 0;
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            if (S->active.empty()) continue;
+            for (auto &j : S->active) nb += j->n;
+            if (nb > S->peak_bodies) S->peak_bodies = nb;
+            if ((int64_t)S->active.size() > S->peak_systems) S->peak_systems = (int64_t)S->active.size();
+        }
+        const auto t0 = Clock::now();
+        int r = nbg_batch_step(S->batch, 1);
+        if (r == NBG_OK) r = nbg_batch_synchronize(S->batch);
+        const auto t1 = Clock::now();
+        std::vector<std::shared_ptr<Job>> finished, keep;
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->busy_s += secs(t0, t1);
+            S->steps_run += 1;
+            S->body_steps += (double)nb;
+            if (r != NBG_OK) S->worker_error = nbg_last_error();
+            for (auto &j : S->active) {
+                if (r != NBG_OK) {
+                    j->status = r;
+                    j->error = S->worker_error;
+                    finished.push_back(j);
+                    continue;
+                }
+                if (--j->remaining <= 0) finished.push_back(j);
+                else keep.push_back(j);
+            }
+            S->active = keep;
+        }
+        for (auto &j : finished) {
+            if (j->status == NBG_OK) {
+                // out: x y z into st[0..2], vx vy vz into st[4..6]
+                const int rr = nbg_batch_get(S->batch, j->sys, j->st[0].data(), j->st[1].data(), j->st[2].data(),
+                                             j->st[4].data(), j->st[5].data(), j->st[6].data(), nullptr, nullptr,
+                                             nullptr);
+                if (rr != NBG_OK) { j->status = rr; j->error = nbg_last_error(); }
+            }
+            (void)nbg_batch_remove(S->batch, j->sys);
+            std::lock_guard<std::mutex> lk(S->mu);
+            j->t_done = Clock::now();
+            j->done = true;
+            S->used_n -= j->n;
+            S->jobs_done += 1;
+            S->cv_done.notify_all();
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int nbg_server_create(nbg_ctx *ctx, int64_t hbm_budget_bytes, int32_t max_systems, int32_t sum_mode,
+                      nbg_server **out) {
+    if (!ctx || !out) return fail(NBG_ERR_ARG, "ctx or out is NULL");
+    *out = nullptr;
+    if (max_systems < 1) return fail(NBG_ERR_ARG, "max_systems must be >= 1");
+    // largest body capacity whose forest fits the budget
+    const int64_t fixed = nbg_batch_bytes(1, max_systems);
+    const int64_t per = (nbg_batch_bytes(1 << 20, max_systems) - fixed) / ((1 << 20) - 1) + 1;
+    int64_t cap = hbm_budget_bytes > fixed ? (hbm_budget_bytes - fixed) / per : 0;
+    if (cap > kMaxBodies) cap = kMaxBodies;
+    while (cap > 0 && nbg_batch_bytes(cap, max_systems) > hbm_budget_bytes) cap -= 1 + cap / 1000;
+    if (cap < 1) return fail(NBG_ERR_BUDGET, "the HBM budget does not hold a single body");
+    nbg_server *S = new nbg_server;
+    S->ctx = ctx;
+    S->cap_n = cap;
+    S->cap_s = max_systems;
+    if (int r = nbg_batch_create(ctx, cap, max_systems, sum_mode, &S->batch)) {
+        delete S;
+        return r;
+    }
+    nbg_batch_enable_counts(S->batch, 0);
+    S->t_start = Clock::now();
+    S->worker = std::thread(worker_loop, S);
+    *out = S;
+    return NBG_OK;
+}
+
+int nbg_server_destroy(nbg_server *S) {
+    if (!S) return NBG_OK;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    if (S->worker.joinable()) S->worker.join();
+    nbg_batch_destroy(S->batch);
+    delete S;
+    return NBG_OK;
+}
+
+int nbg_server_capacity(nbg_server *S, int64_t *max_bodies, int32_t *max_systems) {
+    if (!S) return fail(NBG_ERR_ARG, "server is NULL");
+    if (max_bodies) *max_bodies = S->cap_n;
+    if (max_systems) *max_systems = S->cap_s;
+    return NBG_OK;
+}
+
+int nbg_server_submit(nbg_server *S, const nbg_params *p, int64_t n, const float *x, const float *y, const float *z,
+                      const float *m, const float *vx, const float *vy, const float *vz, int32_t nsteps,
+                      int64_t *job) {
+    if (!S || !p || !job) return fail(NBG_ERR_ARG, "server, params or job is NULL");
+    if (n < 1 || !x || !y || !z || !m) return fail(NBG_ERR_ARG, "a job needs n >= 1 positions and masses");
+    if (nsteps < 1) return fail(NBG_ERR_ARG, "a job needs at least one step");
+    if (p->solver != NBG_LINEAR_BVH || p->morton_bits != 30)
+        return fail(NBG_ERR_ARG, "served jobs run the linear-BVH solver with 30-bit keys");
+    if (n > S->cap_n) return fail(NBG_ERR_BUDGET, "the job is larger than the server's HBM budget holds");
+    auto j = std::make_shared<Job>();
+    j->p = *p;
+    j->n = n;
+    j->steps = j->remaining = nsteps;
+    const float *src[7] = {x, y, z, m, vx, vy, vz};
+    for (int k = 0; k < 7; ++k) {
+        if (src[k]) j->st[k].assign(src[k], src[k] + n);
+        else j->st[k].assign((size_t)n, 0.0f);
+    }
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (!S->worker_error.empty()) return fail(NBG_ERR_DEVICE, "server stopped: " + S->worker_error);
+        j->id = S->next_id++;
+        j->t_submit = Clock::now();
+        S->jobs[j->id] = j;
+        S->pending.push_back(j);
+        *job = j->id;
+    }
+    S->cv_work.notify_all();
+    return NBG_OK;
+}
+
+int nbg_server_wait(nbg_server *S, int64_t job, double timeout_s, float *x, float *y, float *z, float *vx, float *vy,
+                    float *vz, double *times3) {
+    if (!S) return fail(NBG_ERR_ARG, "server is NULL");
+    std::shared_ptr<Job> j;
+    {
+        std::unique_lock<std::mutex> lk(S->mu);
+        auto it = S->jobs.find(job);
+        if (it == S->jobs.end()) return fail(NBG_ERR_ARG, "unknown job id");
+        j = it->second;
+        auto ready = [&] { return j->done; };
+        if (timeout_s < 0) S->cv_done.wait(lk, ready);
+        else if (!S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready))
+            return fail(NBG_ERR_STATE, "job not finished yet");
+        S->jobs.erase(job);
+    }
+    if (j->status != NBG_OK) return fail(j->status, "job failed: " + j->error);
+    float *dst[6] = {x, y, z, vx, vy, vz};
+    const int map[6] = {0, 1, 2, 4, 5, 6};
+    for (int k = 0; k < 6; ++k)
+        if (dst[k]) std::copy(j->st[map[k]].begin(), j->st[map[k]].end(), dst[k]);
+    if (times3) {
+        times3[0] = secs(j->t_submit, j->t_admit);  // queueing
+        times3[1] = secs(j->t_admit, j->t_done);    // on the device
+        times3[2] = secs(j->t_submit, j->t_done);   // latency
+    }
+    return NBG_OK;
+}
+
+// out8: jobs finished, forest steps run, body-steps, device-busy seconds,
+// seconds since creation, peak bodies, peak systems, jobs waiting.
+int nbg_server_stats(nbg_server *S, double *out8) {
+    if (!S || !out8) return fail(NBG_ERR_ARG, "server or out is NULL");
+    std::lock_guard<std::mutex> lk(S->mu);
+    out8[0] = (double)S->jobs_done;
+    out8[1] = (double)S->steps_run;
+    out8[2] = S->body_steps;
+    out8[3] = S->busy_s;
+    out8[4] = secs(S->t_start, Clock::now());
+    out8[5] = (double)S->peak_bodies;
+    out8[6] = (double)S->peak_systems;
+    out8[7] = (double)S->pending.size();
+    return NBG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+"""Continuous-batching server: jobs arriving while others run are admitted
+as capacity frees up, run together in one forest, and each result is the
+same bits as the same system stepped alone (nbg_sys, same parameters)."""
+
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(nbg, theta, leaf=16, eps2=1e-4, dt=1e-3, mode=1):
+    p = nbg.default_params()
+    p.solver, p.morton_bits = nbg.LINEAR_BVH, 30
+    p.theta, p.leaf_size, p.eps2, p.dt, p.sum_mode = theta, leaf, eps2, dt, mode
+    return p
+
+
+def _jobs(nbg, count, nmin, nmax, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        if k % 2:
+            b = nbg.cube(n, seed=900 + k, side=2.0)
+        else:
+            b = nbg.plummer(n, seed=900 + k)
+        theta = float(rng.choice([0.3, 0.5, 0.7]))
+        steps = int(rng.integers(2, 13))
+        out.append((b, theta, steps))
+    return out
+
+
+def _alone(nbg, ctx, b, theta, steps, mode):
+    s = nbg.System(ctx, _params(nbg, theta, mode=mode), b.n)
+    s.set_state(b)
+    s.step(steps)
+    s.synchronize()
+    st = s.get_state()
+    s.close()
+    return st
+
+
+@pytest.mark.parametrize("mode", [1, 0])
+def test_serving_results_equal_alone(nbg, gpu_ctx, mode):
+    jobs = _jobs(nbg, 12, 200, 15000, 3)
+    budget = nbg.Batch.bytes_for(40000, 64)
+    srv = nbg.Server(gpu_ctx, budget, max_systems=64, sum_mode=mode)
+    cap, _ = srv.capacity()
+    assert 39000 <= cap <= 40000
+    ids = []
+    for b, theta, steps in jobs:
+        ids.append(srv.submit(_params(nbg, theta, mode=mode), b, steps))
+        time.sleep(0.002)  # later jobs arrive while earlier ones run
+    results = [srv.wait(j, timeout=120.0) for j in ids]
+    st = srv.stats()
+    print(f"[serve] mode={mode} stats {st}", flush=True)
+    assert st["jobs_done"] == len(jobs) and st["peak_bodies"] <= cap and st["peak_systems"] >= 2
+    for (b, theta, steps), (out, t) in zip(jobs, results):
+        ref = _alone(nbg, gpu_ctx, b, theta, steps, mode)
+        for g, r in zip((out.x, out.y, out.z, out.vx, out.vy, out.vz), (ref.x, ref.y, ref.z, ref.vx, ref.vy, ref.vz)):
+            assert np.array_equal(g, r), f"job n={b.n} theta={theta} steps={steps}"
+        assert t["latency"] >= t["device"] >= 0.0
+    srv.close()
+
+
+def test_serving_budget_admission(nbg, gpu_ctx):
+    """Jobs that do not fit together wait for capacity; a job larger than the
+    whole budget is refused at submission."""
+    budget = nbg.Batch.bytes_for(5000, 8)
+    srv = nbg.Server(gpu_ctx, budget, max_systems=8)
+    cap, _ = srv.capacity()
+    with pytest.raises(nbg.NbgError):
+        srv.submit(_params(nbg, 0.5), nbg.plummer(cap + 1, seed=1), 2)
+    jobs = [(nbg.plummer(3000, seed=10 + k), 0.5, 3) for k in range(5)]   # 15000 bodies > capacity
+    ids = [srv.submit(_params(nbg, th), b, s) for b, th, s in jobs]
+    times = [srv.wait(j, timeout=120.0)[1] for j in ids]
+    st = srv.stats()
+    assert st["jobs_done"] == 5 and st["peak_bodies"] <= cap and st["peak_systems"] == 1
+    assert max(t["queued"] for t in times) > 0.0
+    srv.close()
