@@ -202,19 +202,34 @@ struct FastWave {
     uint64_t visits, lanes, leaf_visits, leaf_lanes;  // wave-uniform diagnostics
 };
 
-// Monopole (or body) interaction with rsqrt + FMA: s = m * (G * inv^3).
-// eps2 > 0 (SOFT) makes r2 > 0 for every pair, coincident ones included
-// (their d = 0 contributes exactly zero); otherwise r2 = 0 is guarded.
+// Monopole (or body) interaction with rsqrt + FMA: s = m * inv^3; the walk
+// multiplies the sum by G once at the end.  eps2 > 0 (SOFT) makes r2 > 0 for
+// every pair, coincident ones included (their d = 0 contributes exactly
+// zero); otherwise r2 = 0 is guarded.
 template <bool SOFT>
 __device__ __forceinline__ void interact(float bx, float by, float bz, float bm, float eps2, FastWave &W) {
     const float dx = bx - W.px, dy = by - W.py, dz = bz - W.pz;
     const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
     float inv = __builtin_amdgcn_rsqf(r2);
     if (!SOFT) inv = r2 > 0.0f ? inv : 0.0f;
-    const float s = bm * ((inv * inv) * (inv * W.G));
+    const float s = bm * (inv * (inv * inv));
     W.rx = __builtin_fmaf(dx, s, W.rx);
     W.ry = __builtin_fmaf(dy, s, W.ry);
     W.rz = __builtin_fmaf(dz, s, W.rz);
+}
+
+// The same with the target-minus-source differences e = p - b and their
+// squared length d2 already computed (by the octree MAC).
+template <bool SOFT>
+__device__ __forceinline__ void interact_from(float ex, float ey, float ez, float d2, float bm, float eps2,
+                                              FastWave &W) {
+    const float r2 = d2 + eps2;
+    float inv = __builtin_amdgcn_rsqf(r2);
+    if (!SOFT) inv = r2 > 0.0f ? inv : 0.0f;
+    const float s = bm * (inv * (inv * inv));
+    W.rx = __builtin_fmaf(-ex, s, W.rx);
+    W.ry = __builtin_fmaf(-ey, s, W.ry);
+    W.rz = __builtin_fmaf(-ez, s, W.rz);
 }
 
 // The lanes in `take` interact with the `cnt` bodies starting at `first`.
@@ -281,7 +296,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     W.visits = 0; W.lanes = 0; W.leaf_visits = 0; W.leaf_lanes = 0;
     // A VGPR copy of G (non-volatile asm: no memory effects, so the node and
     // body loads below stay scalar).
-    asm("v_mov_b32 %0, %1" : "=v"(W.G) : "s"(T.G));
+    W.G = T.G;
     const float eps2 = T.eps2;
     const int64_t n = a.n;
     const uint32_t last_pair = (uint32_t)(a.max_nodes - 1);
@@ -331,7 +346,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     if (fail && lane == 0) atomicOr(a.err, 2u);
     if (valid) {
         const uint32_t o = a.perm[tgt];
-        if ((int64_t)o < a.nt) { a.ax[o] = W.rx; a.ay[o] = W.ry; a.az[o] = W.rz; }
+        if ((int64_t)o < a.nt) { a.ax[o] = W.rx * W.G; a.ay[o] = W.ry * W.G; a.az[o] = W.rz * W.G; }
     }
     if (lane == 0) {
         atomicAdd(&a.counts[0], (unsigned long long)W.pp);
@@ -502,14 +517,31 @@ __global__ __launch_bounds__(64) void k_walk_oct_ordered(WalkArgs a) {
 
 // FAST over the octree: the children of an opened node are contiguous and
 // visited in one group; opened internal children are pushed as groups.
+// The exclusion region [c - side, c + side) lies within sqrt(3) side of the
+// cell centre and the centre of mass within sqrt(3)/2 side of it, so a target
+// with d2 > near2 = (3 side)^2 is outside the region: the six comparisons are
+// needed only for lanes that pass the distance test with d2 <= near2.
 template <bool SOFT, int STATS>
 __device__ __forceinline__ uint64_t visit_oct(const OctRec &nd, uint64_t mask, const float4 *__restrict__ bodies,
-                                              int64_t n, float eps2, FastWave &W, bool &fail) {
-    const uint64_t acc = __ballot(oct_accepts(nd, W.px, W.py, W.pz)) & mask;
+                                              int64_t n, float eps2, const float *__restrict__ near2, FastWave &W,
+                                              bool &fail) {
+    const float ex = W.px - nd.com.x, ey = W.py - nd.com.y, ez = W.pz - nd.com.z;
+    const float d2 = ex * ex + ey * ey + ez * ez;  // glm::distance's order: the MAC stays exact
+    const bool pass = d2 > nd.lo.w;
+    uint64_t acc = __ballot(pass) & mask;
     if (STATS >= 2) { W.visits += 1; W.lanes += __popcll(mask); }
     if (acc) {
-        if (STATS >= 1) W.np += __popcll(acc);
-        if (__builtin_amdgcn_inverse_ballot_w64(acc)) interact<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, eps2, W);
+        const float u2 = near2[oct_level(nd)];
+        const uint64_t close = __ballot(pass && !(d2 > u2)) & acc;
+        if (close) {
+            const bool inside = nd.lo.x <= W.px && nd.lo.y <= W.py && nd.lo.z <= W.pz && W.px < nd.hi.x &&
+                                W.py < nd.hi.y && W.pz < nd.hi.z;
+            acc &= ~(__ballot(inside) & close);
+        }
+        if (acc) {
+            if (STATS >= 1) W.np += __popcll(acc);
+            if (__builtin_amdgcn_inverse_ballot_w64(acc)) interact_from<SOFT>(ex, ey, ez, d2, nd.com.w, eps2, W);
+        }
     }
     const uint64_t open = mask & ~acc;
     if (open == 0 || nd.topo.z >= 0) return open;
@@ -542,13 +574,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
     W.rx = 0.0f; W.ry = 0.0f; W.rz = 0.0f;
     W.pp = 0; W.np = 0;
     W.visits = 0; W.lanes = 0; W.leaf_visits = 0; W.leaf_lanes = 0;
-    asm("v_mov_b32 %0, %1" : "=v"(W.G) : "s"(a.G));
+    W.G = a.G;
     const float eps2 = a.eps2;
     const int64_t n = a.n;
     bool fail = false;
 
+    const float *__restrict__ near2 = a.oct_near2;
     const OctRec root = nodes[0];
-    uint64_t mask = visit_oct<SOFT, STATS>(root, vmask, bodies, n, eps2, W, fail);
+    uint64_t mask = visit_oct<SOFT, STATS>(root, vmask, bodies, n, eps2, near2, W, fail);
     int c = root.topo.z, nch = oct_nchild(root);
     int sp = 0;
     const int64_t max_iters = a.max_nodes + 64;  // each node's group is visited at most once
@@ -573,7 +606,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
         int nc = 0, nn = 0;
         for (int k = 0; k < nch; ++k) {
             const OctRec r = nodes[c + k];
-            const uint64_t o = visit_oct<SOFT, STATS>(r, mask, bodies, n, eps2, W, fail);
+            const uint64_t o = visit_oct<SOFT, STATS>(r, mask, bodies, n, eps2, near2, W, fail);
             if (o) {
                 if (nm) {
                     if (sp >= kStackOct) { fail = true; break; }
@@ -593,7 +626,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
     if (fail && lane == 0) atomicOr(a.err, 2u);
     if (valid) {
         const uint32_t o = a.perm[tgt];
-        if ((int64_t)o < a.nt) { a.ax[o] = W.rx; a.ay[o] = W.ry; a.az[o] = W.rz; }
+        if ((int64_t)o < a.nt) { a.ax[o] = W.rx * W.G; a.ay[o] = W.ry * W.G; a.az[o] = W.rz * W.G; }
     }
     if (lane == 0) {
         atomicAdd(&a.counts[0], (unsigned long long)W.pp);
