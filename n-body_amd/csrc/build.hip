@@ -638,7 +638,7 @@ __device__ void summarize_leaf(int s, int4 t, const float4 *__restrict__ bodies,
     nodes[s].mac = mac_record(mn, mx, mp);
 }
 
-__global__ __launch_bounds__(256) void k_summ_local(const float4 *__restrict__ bodies, int64_t n,
+__global__ __launch_bounds__(kSummThreads) void k_summ_local(const float4 *__restrict__ bodies, int64_t n,
                                                     NodeRec *nodes,
                                                     const int32_t *__restrict__ leaf_start, int64_t max_nodes,
                                                     uint32_t *flags, uint32_t *glist, float4 *bmin,
@@ -701,7 +701,7 @@ void launch_lbvh_summaries(const float4 *bodies, int64_t n, NodeRec *nodes, cons
     launch_fill_u32(flags, 0u, max_nodes, s);
     launch_fill_u32(glist, 0u, 64, s);
     const unsigned blocks = (unsigned)((n + kSummBodies - 1) / kSummBodies);
-    hipLaunchKernelGGL(k_summ_local, dim3(blocks), dim3(256), 0, s, bodies, n, nodes, leaf_start, max_nodes, flags,
+    hipLaunchKernelGGL(k_summ_local, dim3(blocks), dim3(kSummThreads), 0, s, bodies, n, nodes, leaf_start, max_nodes, flags,
                        glist, bmin, bmax, ms, err);
     hipLaunchKernelGGL(k_summ_global, dim3(1), dim3(1024), 0, s, nodes, max_nodes, flags, glist, bmin, bmax, ms,
                        err);

@@ -159,6 +159,7 @@ void launch_lbvh_topology(const void *keys, int bits, int64_t n, int leaf_size, 
 // slot whose first body is i (else -1; written by the topology pass);
 // flags holds max_nodes words; glist holds 2n + 64 words.
 constexpr int kSummBodies = 4096;   // bodies per workgroup in the local pass
+constexpr int kSummThreads = 1024;  // threads per workgroup in the local pass (latency hiding)
 struct MacParam {       // mac_threshold's output for one theta
     double mid;
     int tie;

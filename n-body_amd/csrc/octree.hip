@@ -404,7 +404,7 @@ __device__ __forceinline__ void oct_summ_inner(int p, const OctRec &r, OctRec *n
     nodes[p].com = com;
 }
 
-__global__ __launch_bounds__(256) void k_oct_summ_local(const float4 *__restrict__ bodies, int64_t n,
+__global__ __launch_bounds__(kSummThreads) void k_oct_summ_local(const float4 *__restrict__ bodies, int64_t n,
                                                         OctRec *nodes, const int32_t *__restrict__ leaf_start,
                                                         const uint32_t *__restrict__ misc, uint32_t *flags,
                                                         uint32_t *glist, int64_t glist_cap, uint32_t *err) {
@@ -538,7 +538,7 @@ void launch_octree_summaries(const float4 *bodies, int64_t n, OctBuffers &ob, co
     if (n <= 0) return;
     launch_fill_u32(ob.flags, 0u, ob.cap, s);
     launch_fill_u32(ob.glist, 0u, 64, s);
-    hipLaunchKernelGGL(k_oct_summ_local, dim3(blocks_for(n, kSummBodies)), dim3(256), 0, s, bodies, n, ob.nodes,
+    hipLaunchKernelGGL(k_oct_summ_local, dim3(blocks_for(n, kSummBodies)), dim3(kSummThreads), 0, s, bodies, n, ob.nodes,
                        leaf_start, ob.misc, ob.flags, ob.glist, ob.cap, err);
     hipLaunchKernelGGL(k_oct_summ_global, dim3(1), dim3(1024), 0, s, ob.nodes, ob.misc, ob.flags, ob.glist, ob.cap,
                        err);

@@ -267,7 +267,7 @@ __device__ __forceinline__ uint64_t visit(const NodeRec &nd, uint64_t mask, cons
     const uint64_t open = mask & ~acc;
     if (open == 0 || nd.topo.z >= 0) return open;
     const int first = nd.topo.x, cnt = nd.topo.y;
-    if (first < 0 || cnt < 0 || (int64_t)first + cnt > n) { fail = true; return 0; }
+    if (first < 0 || cnt < 0 || (uint32_t)(first + cnt) > (uint32_t)n) { fail = true; return 0; }  // n < 2^29: 32-bit, scalar
     if (STATS >= 2) { W.leaf_visits += 1; W.leaf_lanes += __popcll(open); }
     if (STATS >= 1) W.pp += (uint64_t)__popcll(open) * (uint64_t)cnt;
     leaf_bodies<SOFT>(bodies, first, cnt, open, eps2, W);
@@ -314,6 +314,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     int sp = 0;
     uint32_t iters = 0;
     while (true) {
+        // wave-uniform by construction; readfirstlane tells the compiler so
+        // and keeps the traversal state and its mask arithmetic scalar
+        mask = uniform_u64(mask);
+        sp = uniform_i32(sp);
         if (mask == 0) {
             if (sp == 0) break;
             --sp;
@@ -546,7 +550,7 @@ __device__ __forceinline__ uint64_t visit_oct(const OctRec &nd, uint64_t mask, c
     const uint64_t open = mask & ~acc;
     if (open == 0 || nd.topo.z >= 0) return open;
     const int first = nd.topo.x, cnt = nd.topo.y;
-    if (first < 0 || cnt < 0 || (int64_t)first + cnt > n) { fail = true; return 0; }
+    if (first < 0 || cnt < 0 || (uint32_t)(first + cnt) > (uint32_t)n) { fail = true; return 0; }
     if (STATS >= 2) { W.leaf_visits += 1; W.leaf_lanes += __popcll(open); }
     if (STATS >= 1) W.pp += (uint64_t)__popcll(open) * (uint64_t)cnt;
     leaf_bodies<SOFT>(bodies, first, cnt, open, eps2, W);
@@ -587,6 +591,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
     const int64_t max_iters = a.max_nodes + 64;  // each node's group is visited at most once
     int64_t iters = 0;
     while (!fail) {
+        mask = uniform_u64(mask);
+        sp = uniform_i32(sp);
         if (mask == 0) {
             if (sp == 0) break;
             --sp;
