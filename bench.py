@@ -270,14 +270,15 @@ def bench_forest(nbg, ctx, args):
     }
 
 
-def bench_serving(nbg, ctx, args, capacity):
+def bench_serving(nbg, ctx, args):
     """Serving: a Poisson stream of jobs (Plummer sphere or uniform cube, N
     log-uniform in [1024, 2097152], steps uniform in [10, 500], theta in
     {0.3, 0.5, 0.7}, eps2 = 1e-4) submitted at their arrival times to a
     continuous-batching server with a 200 GB HBM budget.  The arrival rate
-    offers `serve_load` x `capacity` bodies*steps/s.  Reports latency
-    percentiles and aggregate throughput, and the same jobs' throughput when
-    run one at a time through nbg_sys (a sample, steps capped, scaled)."""
+    offers `serve_load` x the batched capacity measured on a sample of the
+    same jobs.  Reports latency percentiles, aggregate throughput, and the
+    same jobs run one at a time through nbg_sys (a sample, steps capped,
+    scaled)."""
     rng = np.random.default_rng(args.serve_seed)
     specs = []
     for k in range(args.serve_jobs):
@@ -287,9 +288,6 @@ def bench_serving(nbg, ctx, args, capacity):
         theta = float(rng.choice([0.3, 0.5, 0.7]))
         specs.append((kind, n, steps, theta, splitmix64(args.serve_seed * 1000003 + k)))
     work = np.array([n * st for _, n, st, _, _ in specs], np.float64)
-    rate = args.serve_load * capacity / work.mean()
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(specs)))
-    log(f"[bench] serving: {len(specs)} jobs, {work.sum():.3g} body-steps, arrival rate {rate:.3g}/s")
     bodies = [nbg.plummer(n, seed=sd) if kind == "plummer" else nbg.cube(n, seed=sd, side=2.0)
               for kind, n, _, _, sd in specs]
 
@@ -299,26 +297,8 @@ def bench_serving(nbg, ctx, args, capacity):
         p.eps2, p.dt, p.sum_mode = 1e-4, 1e-3, nbg.SUM_FAST
         return p
 
-    srv = nbg.Server(ctx, 200 * 10 ** 9, max_systems=4096, sum_mode=nbg.SUM_FAST)
-    cap_bodies, _ = srv.capacity()
-    ids = []
-    t0 = time.perf_counter()
-    for (kind, n, steps, theta, _), b, t_arr in zip(specs, bodies, arrivals):
-        now = time.perf_counter() - t0
-        if t_arr > now:
-            time.sleep(t_arr - now)
-        ids.append(srv.submit(params(theta), b, steps))
-    lat, dev, que = [], [], []
-    for j in ids:
-        _, t = srv.wait(j)
-        lat.append(t["latency"])
-        dev.append(t["device"])
-        que.append(t["queued"])
-    wall = time.perf_counter() - t0
-    st = srv.stats()
-    srv.close()
-    served = work.sum() / wall
-    # one job at a time through nbg_sys: a sample, at most 20 timed steps each
+    # one job at a time through nbg_sys, and the same sample batched, to
+    # calibrate the offered load (at most 10 timed steps each)
     sample = np.random.default_rng(2).choice(len(specs), min(args.serve_alone_sample, len(specs)), replace=False)
     t_tot, w_tot = 0.0, 0.0
     for k in sample:
@@ -328,7 +308,7 @@ def bench_serving(nbg, ctx, args, capacity):
         s.enable_counts(False)
         s.step(1)
         s.synchronize()
-        k_steps = min(steps, 20)
+        k_steps = min(steps, 10)
         t1 = time.perf_counter()
         s.step(k_steps)
         s.synchronize()
@@ -336,12 +316,60 @@ def bench_serving(nbg, ctx, args, capacity):
         w_tot += n * steps
         s.close()
     alone = w_tot / t_tot
+    nb = sum(specs[int(k)][1] for k in sample)
+    B = nbg.Batch(ctx, nb, len(sample))
+    for k in sample:
+        B.add(params(specs[int(k)][3]), bodies[int(k)])
+    B.enable_counts(False)
+    B.step(1)
+    B.synchronize()
+    t1 = time.perf_counter()
+    B.step(5)
+    B.synchronize()
+    capacity = nb * 5 / (time.perf_counter() - t1)
+    B.close()
+    rate = args.serve_load * capacity / work.mean()
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(specs)))
+    log(f"[bench] serving: {len(specs)} jobs, {work.sum():.3g} body-steps, batched capacity {capacity:.3g}/s, "
+        f"arrival rate {rate:.3g}/s (~{arrivals[-1]:.0f} s of arrivals)")
+
+    srv = nbg.Server(ctx, 200 * 10 ** 9, max_systems=4096, sum_mode=nbg.SUM_FAST)
+    cap_bodies, _ = srv.capacity()
+    ids = []
+    t0 = time.perf_counter()
+    for k, ((kind, n, steps, theta, _), b, t_arr) in enumerate(zip(specs, bodies, arrivals)):
+        now = time.perf_counter() - t0
+        if t_arr > now:
+            time.sleep(t_arr - now)
+        ids.append(srv.submit(params(theta), b, steps))
+        if (k + 1) % 10 == 0:
+            st = srv.stats()
+            log(f"[bench] serving: t={time.perf_counter() - t0:.1f}s submitted {k + 1}, done {int(st['jobs_done'])}, "
+                f"waiting {int(st['waiting'])}")
+    lat, que = [], []
+    for k, j in enumerate(ids):
+        while True:
+            try:
+                _, t = srv.wait(j, timeout=30.0)
+                break
+            except nbg.NbgError as e:
+                if "not finished" not in str(e):
+                    raise
+                st = srv.stats()
+                log(f"[bench] serving: t={time.perf_counter() - t0:.1f}s waiting for job {k}; "
+                    f"done {int(st['jobs_done'])}, steps {int(st['steps_run'])}")
+        lat.append(t["latency"])
+        que.append(t["queued"])
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    served = work.sum() / wall
     lat = np.array(lat)
     return {
         "workload": f"{len(specs)} jobs (Plummer or cube side 2, N log-uniform [1024, 2097152], steps U[10, 500], "
                     f"theta in {{0.3, 0.5, 0.7}}, eps2 1e-4, dt 1e-3, linear BVH 30-bit, FAST), Poisson arrivals at "
-                    f"{rate:.3g}/s = {args.serve_load} x {capacity:.3g} bodies*steps/s, 200 GB HBM budget "
-                    f"({cap_bodies} bodies), seed {args.serve_seed}",
+                    f"{rate:.3g}/s = {args.serve_load} x the batched capacity {capacity:.3g} bodies*steps/s measured "
+                    f"on {len(sample)} of the jobs, 200 GB HBM budget ({cap_bodies} bodies), seed {args.serve_seed}",
         "value": served,
         "unit": "bodies·steps/s",
         "latency_p50_s": float(np.quantile(lat, 0.5)),
@@ -353,9 +381,13 @@ def bench_serving(nbg, ctx, args, capacity):
         "forest_steps": int(st["steps_run"]),
         "peak_bodies_resident": int(st["peak_bodies"]),
         "peak_systems_resident": int(st["peak_systems"]),
+        "device_busy_throughput": work.sum() / st["busy_s"],
         "one_at_a_time": {"value": alone, "unit": "bodies·steps/s",
-                          "sample": f"{len(sample)} of the jobs run alone through nbg_sys, at most 20 timed steps "
+                          "sample": f"{len(sample)} of the jobs run alone through nbg_sys, at most 10 timed steps "
                                     f"each, scaled to their step counts"},
+        "batched_same_sample": {"value": capacity, "unit": "bodies·steps/s",
+                                "sample": f"the same {len(sample)} jobs stepped together in one forest (5 steps)"},
+        "batched_over_one_at_a_time": capacity / alone,
         "served_over_one_at_a_time": served / alone,
     }
 
@@ -384,10 +416,10 @@ def main():
     ap.add_argument("--forest-seed", type=int, default=7)
     ap.add_argument("--forest-alone-sample", type=int, default=64)
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
-    ap.add_argument("--serve-jobs", type=int, default=100)
+    ap.add_argument("--serve-jobs", type=int, default=60)
     ap.add_argument("--serve-load", type=float, default=0.8)
     ap.add_argument("--serve-seed", type=int, default=11)
-    ap.add_argument("--serve-alone-sample", type=int, default=12)
+    ap.add_argument("--serve-alone-sample", type=int, default=16)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device tensor copy three times (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -447,8 +479,7 @@ def main():
     serving = None
     if not args.no_serving:
         try:
-            cap = forest["value"] if forest and "value" in forest else 1e8
-            serving = bench_serving(nbg, ctx, args, cap)
+            serving = bench_serving(nbg, ctx, args)
         except Exception as e:
             serving = {"error": f"{type(e).__name__}: {e}"}
 

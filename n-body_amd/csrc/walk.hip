@@ -43,9 +43,10 @@ struct WaveTask {
     float G, eps2;
 };
 
+template <bool FOREST>
 __device__ __forceinline__ WaveTask wave_task(const WalkArgs &a, int64_t w) {
     WaveTask t;
-    if (a.nsys <= 0) {
+    if (!FOREST) {
         t.base = w * kWave; t.end = a.nt; t.root = 0; t.G = a.G; t.eps2 = a.eps2;
         return t;
     }
@@ -95,7 +96,7 @@ __device__ __forceinline__ bool lbvh_accepts(float4 mc, float px, float py, floa
 }
 
 // ---------------------------------------------------------------- ORDERED
-template <int MAXD>
+template <int MAXD, bool FOREST>
 __global__ __launch_bounds__(64) void k_walk_lbvh_ordered(WalkArgs a) {
     __shared__ int s_node[kStackLBVH];
     __shared__ int s_meta[kStackLBVH];
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(64) void k_walk_lbvh_ordered(WalkArgs a) {
     const float4 *__restrict__ bodies = a.bodies;
     const NodeRec *__restrict__ nodes = a.nodes;
     const int lane = lane_id();
-    const WaveTask T = wave_task(a, xcd_block(blockIdx.x, gridDim.x));
+    const WaveTask T = wave_task<FOREST>(a, xcd_block(blockIdx.x, gridDim.x));
     const int64_t tgt = T.base + lane;
     const bool valid = tgt < T.end;
     const uint64_t vmask = __ballot(valid);
@@ -274,15 +275,16 @@ __device__ __forceinline__ uint64_t visit(const NodeRec &nd, uint64_t mask, cons
     return 0;
 }
 
-template <int WAVES, bool SOFT, int STATS>
+template <int WAVES, bool SOFT, int STATS, bool FOREST>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     __shared__ int s_node[WAVES][kStackFast];
     __shared__ uint64_t s_mask[WAVES][kStackFast];
 
     const float4 *__restrict__ bodies = a.bodies;
     const NodeRec *__restrict__ nodes = a.nodes;
-    const int lane = lane_id(), wv = threadIdx.x / kWave;
-    const WaveTask T = wave_task(a, xcd_block(blockIdx.x, gridDim.x) * WAVES + wv);
+    // the wave index is uniform; readfirstlane makes the per-wave task scalar
+    const int lane = lane_id(), wv = uniform_i32(threadIdx.x / kWave);
+    const WaveTask T = wave_task<FOREST>(a, xcd_block(blockIdx.x, gridDim.x) * WAVES + wv);
     const int64_t tgt = T.base + lane;
     const bool valid = tgt < T.end;
     const uint64_t vmask = __ballot(valid);
@@ -566,7 +568,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
 
     const float4 *__restrict__ bodies = a.bodies;
     const OctRec *__restrict__ nodes = a.onodes;
-    const int lane = lane_id(), wv = threadIdx.x / kWave;
+    const int lane = lane_id(), wv = uniform_i32(threadIdx.x / kWave);
     const int64_t tgt = (xcd_block(blockIdx.x, gridDim.x) * WAVES + wv) * kWave + lane;
     const bool valid = tgt < a.nt;
     const uint64_t vmask = __ballot(valid);
@@ -674,14 +676,20 @@ void launch_walk_oct(const WalkArgs &a, int ordered, int stats, hipStream_t s) {
 // Waves the walk launches: one per 64 targets, or the forest's aligned waves.
 static int64_t walk_waves(const WalkArgs &a) { return a.nsys > 0 ? a.nwaves : (a.nt + kWave - 1) / kWave; }
 
-template <int STATS>
-static void launch_fast(const WalkArgs &a, hipStream_t s) {
+template <int STATS, bool FOREST>
+static void launch_fast_f(const WalkArgs &a, hipStream_t s) {
     constexpr int W = 4;
     const unsigned blocks = (unsigned)((walk_waves(a) + W - 1) / W);
     if (a.eps2 > 0.0f)
-        hipLaunchKernelGGL((k_walk_lbvh_fast<W, true, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
+        hipLaunchKernelGGL((k_walk_lbvh_fast<W, true, STATS, FOREST>), dim3(blocks), dim3(64 * W), 0, s, a);
     else
-        hipLaunchKernelGGL((k_walk_lbvh_fast<W, false, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
+        hipLaunchKernelGGL((k_walk_lbvh_fast<W, false, STATS, FOREST>), dim3(blocks), dim3(64 * W), 0, s, a);
+}
+
+template <int STATS>
+static void launch_fast(const WalkArgs &a, hipStream_t s) {
+    if (a.nsys > 0) launch_fast_f<STATS, true>(a, s);
+    else launch_fast_f<STATS, false>(a, s);
 }
 
 void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, int stats, hipStream_t s) {
@@ -692,10 +700,12 @@ void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, int stats, 
         else launch_fast<0>(a, s);
     } else {
         const unsigned blocks = (unsigned)walk_waves(a);
-        if (max_depth <= 32)
-            hipLaunchKernelGGL((k_walk_lbvh_ordered<32>), dim3(blocks), dim3(64), 0, s, a);
+        if (a.nsys > 0)
+            hipLaunchKernelGGL((k_walk_lbvh_ordered<32, true>), dim3(blocks), dim3(64), 0, s, a);
+        else if (max_depth <= 32)
+            hipLaunchKernelGGL((k_walk_lbvh_ordered<32, false>), dim3(blocks), dim3(64), 0, s, a);
         else
-            hipLaunchKernelGGL((k_walk_lbvh_ordered<64>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((k_walk_lbvh_ordered<64, false>), dim3(blocks), dim3(64), 0, s, a);
     }
 }
 
