@@ -116,7 +116,7 @@ int ensure_octree(nbg_sys *s) {
     A(&b.nfirst, cap); A(&b.ncount, cap); A(&b.nparent, cap); A(&b.nmeta, cap);
     A(&b.occ, cap); A(&b.cb, cap); A(&b.flags, cap); A(&b.glist, cap + 64);
     A(&b.scan_ws, 128 + (cap + kScanTile - 1) / kScanTile);
-    A(&b.nodes, cap); A(&b.centre, cap); A(&b.thr, 128); A(&b.misc, 64);
+    A(&b.nodes, cap); A(&b.centre, cap); A(&b.thr, 64); A(&b.misc, 64);
     if (r != NBG_OK) {
         free_oct(b);
         return r;
@@ -230,7 +230,6 @@ WalkArgs walk_args(nbg_sys *s) {
     wa.perm = s->idx;
     wa.nodes = s->nodes;
     wa.onodes = s->ob.nodes;
-    wa.oct_near2 = s->ob.thr ? s->ob.thr + 64 : nullptr;
     wa.levels = p.max_depth + 2;
     wa.theta = p.theta;
     wa.theta2 = p.theta * p.theta;

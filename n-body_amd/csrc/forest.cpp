@@ -192,7 +192,6 @@ void enqueue_pass(nbg_batch *b) {
     wa.perm = b->idx;
     wa.nodes = b->nodes;
     wa.onodes = nullptr;
-    wa.oct_near2 = nullptr;
     wa.levels = 0;
     wa.theta = 0.0f;
     wa.theta2 = 0.0f;

@@ -91,7 +91,7 @@ struct OctBuffers {
     uint32_t *scan_ws = nullptr;   // 128 + cap / kScanTile
     OctRec *nodes = nullptr;       // cap, by slot
     float4 *centre = nullptr;      // cap, by slot: cell centre xyz, side
-    float *thr = nullptr;          // 128: opening threshold per level, then (3 side)^2 per level
+    float *thr = nullptr;          // 64: opening threshold per level
     uint32_t *misc = nullptr;      // 64: [0] node total, [1] child total
 };
 
@@ -205,7 +205,6 @@ struct WalkArgs {
     const uint32_t *perm;   // walk position -> output index (< nt)
     const NodeRec *nodes;   // LBVH nodes
     const OctRec *onodes;   // octree nodes
-    const float *oct_near2; // octree: per level, (3 side)^2 (see visit_oct)
     int levels;             // octree: max_depth + 2
     float theta;            // octree MAC
     float theta2;           // LBVH MAC: theta * theta (float)

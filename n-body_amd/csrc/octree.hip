@@ -271,8 +271,6 @@ __global__ void k_oct_levels(const float *__restrict__ bb, int levels, float the
     oct_root(bb, cx, cy, cz, side);
     for (int l = 0; l < L; ++l) side = side / 2.0f;
     thr[L] = oct_threshold(side, theta);
-    const float s3 = 3.0f * side;
-    thr[64 + L] = s3 * s3;
 }
 
 // Slot of node id: its parent's first-child slot plus the number of

@@ -219,20 +219,6 @@ __device__ __forceinline__ void interact(float bx, float by, float bz, float bm,
     W.rz = __builtin_fmaf(dz, s, W.rz);
 }
 
-// The same with the target-minus-source differences e = p - b and their
-// squared length d2 already computed (by the octree MAC).
-template <bool SOFT>
-__device__ __forceinline__ void interact_from(float ex, float ey, float ez, float d2, float bm, float eps2,
-                                              FastWave &W) {
-    const float r2 = d2 + eps2;
-    float inv = __builtin_amdgcn_rsqf(r2);
-    if (!SOFT) inv = r2 > 0.0f ? inv : 0.0f;
-    const float s = bm * (inv * (inv * inv));
-    W.rx = __builtin_fmaf(-ex, s, W.rx);
-    W.ry = __builtin_fmaf(-ey, s, W.ry);
-    W.rz = __builtin_fmaf(-ez, s, W.rz);
-}
-
 // The lanes in `take` interact with the `cnt` bodies starting at `first`.
 template <bool SOFT>
 __device__ __forceinline__ void leaf_bodies(const float4 *__restrict__ bodies, int first, int cnt, uint64_t take,
@@ -275,8 +261,9 @@ __device__ __forceinline__ uint64_t visit(const NodeRec &nd, uint64_t mask, cons
     return 0;
 }
 
-template <int WAVES, bool SOFT, int STATS, bool FOREST>
-__global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
+// Forest: the same traversal for the wave's system (see wave_task).
+template <int WAVES, bool SOFT, int STATS>
+__global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_forest(WalkArgs a) {
     __shared__ int s_node[WAVES][kStackFast];
     __shared__ uint64_t s_mask[WAVES][kStackFast];
 
@@ -284,7 +271,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     const NodeRec *__restrict__ nodes = a.nodes;
     // the wave index is uniform; readfirstlane makes the per-wave task scalar
     const int lane = lane_id(), wv = uniform_i32(threadIdx.x / kWave);
-    const WaveTask T = wave_task<FOREST>(a, xcd_block(blockIdx.x, gridDim.x) * WAVES + wv);
+    const WaveTask T = wave_task<true>(a, xcd_block(blockIdx.x, gridDim.x) * WAVES + wv);
     const int64_t tgt = T.base + lane;
     const bool valid = tgt < T.end;
     const uint64_t vmask = __ballot(valid);
@@ -523,36 +510,21 @@ __global__ __launch_bounds__(64) void k_walk_oct_ordered(WalkArgs a) {
 
 // FAST over the octree: the children of an opened node are contiguous and
 // visited in one group; opened internal children are pushed as groups.
-// The exclusion region [c - side, c + side) lies within sqrt(3) side of the
-// cell centre and the centre of mass within sqrt(3)/2 side of it, so a target
-// with d2 > near2 = (3 side)^2 is outside the region: the six comparisons are
-// needed only for lanes that pass the distance test with d2 <= near2.
+// FAST over the octree: the children of an opened node are contiguous and
+// visited in one group; opened internal children are pushed as groups.
 template <bool SOFT, int STATS>
 __device__ __forceinline__ uint64_t visit_oct(const OctRec &nd, uint64_t mask, const float4 *__restrict__ bodies,
-                                              int64_t n, float eps2, const float *__restrict__ near2, FastWave &W,
-                                              bool &fail) {
-    const float ex = W.px - nd.com.x, ey = W.py - nd.com.y, ez = W.pz - nd.com.z;
-    const float d2 = ex * ex + ey * ey + ez * ez;  // glm::distance's order: the MAC stays exact
-    const bool pass = d2 > nd.lo.w;
-    uint64_t acc = __ballot(pass) & mask;
+                                              int64_t n, float eps2, FastWave &W, bool &fail) {
+    const uint64_t acc = __ballot(oct_accepts(nd, W.px, W.py, W.pz)) & mask;
     if (STATS >= 2) { W.visits += 1; W.lanes += __popcll(mask); }
     if (acc) {
-        const float u2 = near2[oct_level(nd)];
-        const uint64_t close = __ballot(pass && !(d2 > u2)) & acc;
-        if (close) {
-            const bool inside = nd.lo.x <= W.px && nd.lo.y <= W.py && nd.lo.z <= W.pz && W.px < nd.hi.x &&
-                                W.py < nd.hi.y && W.pz < nd.hi.z;
-            acc &= ~(__ballot(inside) & close);
-        }
-        if (acc) {
-            if (STATS >= 1) W.np += __popcll(acc);
-            if (__builtin_amdgcn_inverse_ballot_w64(acc)) interact_from<SOFT>(ex, ey, ez, d2, nd.com.w, eps2, W);
-        }
+        if (STATS >= 1) W.np += __popcll(acc);
+        if (__builtin_amdgcn_inverse_ballot_w64(acc)) interact<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, eps2, W);
     }
     const uint64_t open = mask & ~acc;
     if (open == 0 || nd.topo.z >= 0) return open;
     const int first = nd.topo.x, cnt = nd.topo.y;
-    if (first < 0 || cnt < 0 || (uint32_t)(first + cnt) > (uint32_t)n) { fail = true; return 0; }
+    if (first < 0 || cnt < 0 || (int64_t)first + cnt > n) { fail = true; return 0; }
     if (STATS >= 2) { W.leaf_visits += 1; W.leaf_lanes += __popcll(open); }
     if (STATS >= 1) W.pp += (uint64_t)__popcll(open) * (uint64_t)cnt;
     leaf_bodies<SOFT>(bodies, first, cnt, open, eps2, W);
@@ -568,7 +540,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
 
     const float4 *__restrict__ bodies = a.bodies;
     const OctRec *__restrict__ nodes = a.onodes;
-    const int lane = lane_id(), wv = uniform_i32(threadIdx.x / kWave);
+    const int lane = lane_id(), wv = threadIdx.x / kWave;
     const int64_t tgt = (xcd_block(blockIdx.x, gridDim.x) * WAVES + wv) * kWave + lane;
     const bool valid = tgt < a.nt;
     const uint64_t vmask = __ballot(valid);
@@ -585,16 +557,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
     const int64_t n = a.n;
     bool fail = false;
 
-    const float *__restrict__ near2 = a.oct_near2;
     const OctRec root = nodes[0];
-    uint64_t mask = visit_oct<SOFT, STATS>(root, vmask, bodies, n, eps2, near2, W, fail);
+    uint64_t mask = visit_oct<SOFT, STATS>(root, vmask, bodies, n, eps2, W, fail);
     int c = root.topo.z, nch = oct_nchild(root);
     int sp = 0;
     const int64_t max_iters = a.max_nodes + 64;  // each node's group is visited at most once
     int64_t iters = 0;
     while (!fail) {
-        mask = uniform_u64(mask);
-        sp = uniform_i32(sp);
         if (mask == 0) {
             if (sp == 0) break;
             --sp;
@@ -614,7 +583,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
         int nc = 0, nn = 0;
         for (int k = 0; k < nch; ++k) {
             const OctRec r = nodes[c + k];
-            const uint64_t o = visit_oct<SOFT, STATS>(r, mask, bodies, n, eps2, near2, W, fail);
+            const uint64_t o = visit_oct<SOFT, STATS>(r, mask, bodies, n, eps2, W, fail);
             if (o) {
                 if (nm) {
                     if (sp >= kStackOct) { fail = true; break; }
@@ -673,23 +642,103 @@ void launch_walk_oct(const WalkArgs &a, int ordered, int stats, hipStream_t s) {
     }
 }
 
+// Single system: wave w walks targets 64 w .. 64 w + 63 from the root.
+template <int WAVES, bool SOFT, int STATS>
+__global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
+    __shared__ int s_node[WAVES][kStackFast];
+    __shared__ uint64_t s_mask[WAVES][kStackFast];
+
+    const float4 *__restrict__ bodies = a.bodies;
+    const NodeRec *__restrict__ nodes = a.nodes;
+    const int lane = lane_id(), wv = threadIdx.x / kWave;
+    const int64_t tgt = (xcd_block(blockIdx.x, gridDim.x) * WAVES + wv) * kWave + lane;
+    const bool valid = tgt < a.nt;
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0) return;
+
+    FastWave W;
+    const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    W.px = me.x; W.py = me.y; W.pz = me.z;
+    W.rx = 0.0f; W.ry = 0.0f; W.rz = 0.0f;
+    W.pp = 0; W.np = 0;
+    W.visits = 0; W.lanes = 0; W.leaf_visits = 0; W.leaf_lanes = 0;
+    // A VGPR copy of G (non-volatile asm: no memory effects, so the node and
+    // body loads below stay scalar).
+    asm("v_mov_b32 %0, %1" : "=v"(W.G) : "s"(a.G));
+    const float eps2 = a.eps2;
+    const int64_t n = a.n;
+    const uint32_t last_pair = (uint32_t)(a.max_nodes - 1);
+    bool fail = false;
+
+    // The root is visited on its own; afterwards every step visits the two
+    // children (c, c + 1) of a node that the lanes in `mask` opened.
+    const NodeRec root = nodes[0];
+    uint64_t mask = visit<SOFT, STATS>(root, vmask, bodies, n, eps2, W, fail);
+    int c = root.topo.z;
+    int sp = 0;
+    uint32_t iters = 0;
+    while (true) {
+        if (mask == 0) {
+            if (sp == 0) break;
+            --sp;
+            __builtin_amdgcn_wave_barrier();
+            c = uniform_i32(s_node[wv][sp]);
+            mask = uniform_u64(s_mask[wv][sp]);
+        }
+        c = uniform_i32(c);  // keeps the node loads scalar
+        if (++iters > kMaxWalkIters || (uint32_t)c >= last_pair) { fail = true; break; }
+        const NodeRec r0 = nodes[c], r1 = nodes[c + 1];
+        const uint64_t o0 = visit<SOFT, STATS>(r0, mask, bodies, n, eps2, W, fail);
+        const uint64_t o1 = visit<SOFT, STATS>(r1, mask, bodies, n, eps2, W, fail);
+        if (fail) break;
+        if (o0 != 0) {
+            if (o1 != 0) {
+                if (sp >= kStackFast) { fail = true; break; }
+                if (lane == 0) { s_node[wv][sp] = r1.topo.z; s_mask[wv][sp] = o1; }
+                __builtin_amdgcn_wave_barrier();
+                ++sp;
+            }
+            c = r0.topo.z;
+            mask = o0;
+        } else if (o1 != 0) {
+            c = r1.topo.z;
+            mask = o1;
+        } else {
+            mask = 0;
+        }
+    }
+    if (fail && lane == 0) atomicOr(a.err, 2u);
+    if (valid) {
+        const uint32_t o = a.perm[tgt];
+        if ((int64_t)o < a.nt) { a.ax[o] = W.rx * W.G; a.ay[o] = W.ry * W.G; a.az[o] = W.rz * W.G; }
+    }
+    if (lane == 0) {
+        atomicAdd(&a.counts[0], (unsigned long long)W.pp);
+        atomicAdd(&a.counts[1], (unsigned long long)W.np);
+        if (STATS >= 2) {
+            atomicAdd(&a.counts[2], (unsigned long long)W.visits);
+            atomicAdd(&a.counts[3], (unsigned long long)W.lanes);
+            atomicAdd(&a.counts[4], (unsigned long long)W.leaf_visits);
+            atomicAdd(&a.counts[5], (unsigned long long)W.leaf_lanes);
+        }
+    }
+}
+
 // Waves the walk launches: one per 64 targets, or the forest's aligned waves.
 static int64_t walk_waves(const WalkArgs &a) { return a.nsys > 0 ? a.nwaves : (a.nt + kWave - 1) / kWave; }
 
-template <int STATS, bool FOREST>
-static void launch_fast_f(const WalkArgs &a, hipStream_t s) {
-    constexpr int W = 4;
-    const unsigned blocks = (unsigned)((walk_waves(a) + W - 1) / W);
-    if (a.eps2 > 0.0f)
-        hipLaunchKernelGGL((k_walk_lbvh_fast<W, true, STATS, FOREST>), dim3(blocks), dim3(64 * W), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_walk_lbvh_fast<W, false, STATS, FOREST>), dim3(blocks), dim3(64 * W), 0, s, a);
-}
-
 template <int STATS>
 static void launch_fast(const WalkArgs &a, hipStream_t s) {
-    if (a.nsys > 0) launch_fast_f<STATS, true>(a, s);
-    else launch_fast_f<STATS, false>(a, s);
+    constexpr int W = 4;
+    const unsigned blocks = (unsigned)((walk_waves(a) + W - 1) / W);
+    const bool soft = a.eps2 > 0.0f;
+    if (a.nsys > 0) {
+        if (soft) hipLaunchKernelGGL((k_walk_lbvh_forest<W, true, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
+        else hipLaunchKernelGGL((k_walk_lbvh_forest<W, false, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
+    } else {
+        if (soft) hipLaunchKernelGGL((k_walk_lbvh_fast<W, true, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
+        else hipLaunchKernelGGL((k_walk_lbvh_fast<W, false, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
+    }
 }
 
 void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, int stats, hipStream_t s) {
