@@ -580,6 +580,11 @@ struct MacSel {
     const uint64_t *keys;
     int shift;
     __device__ __forceinline__ MacParam at(int first) const { return tab ? tab[keys[first] >> shift] : one; }
+    // Forest: a node whose range spans systems needs no summary (every walk
+    // starts at a system's root), so climbs stop below it.
+    __device__ __forceinline__ bool spans(int4 t) const {
+        return tab && (keys[t.x] >> shift) != (keys[t.x + t.y - 1] >> shift);
+    }
 };
 
 __device__ __forceinline__ float4 mac_record(float4 mn, float4 mx, MacParam mp) {
@@ -656,6 +661,8 @@ __global__ __launch_bounds__(kSummThreads) void k_summ_local(const float4 *__res
         while (parent >= 0) {
             if (parent >= max_nodes) { atomicOr(err, 4u); break; }
             const int4 tp = nodes[parent].topo;
+            if (tp.x < 0 || tp.y < 1 || (int64_t)tp.x + tp.y > n) { atomicOr(err, 4u); break; }
+            if (ms.spans(tp)) break;
             const bool local = tp.x >= lo && (int64_t)tp.x + tp.y <= hi;
             if (!local) {
                 const uint32_t k = atomicAdd(&glist[0], 1u);
@@ -671,15 +678,17 @@ __global__ __launch_bounds__(kSummThreads) void k_summ_local(const float4 *__res
     }
 }
 
-__global__ __launch_bounds__(1024) void k_summ_global(NodeRec *nodes, int64_t max_nodes, uint32_t *flags,
-                                                      const uint32_t *glist, float4 *bmin, float4 *bmax,
-                                                      MacSel ms, uint32_t *err) {
+__global__ __launch_bounds__(1024) void k_summ_global(NodeRec *nodes, int64_t n, int64_t max_nodes,
+                                                      uint32_t *flags, const uint32_t *glist, float4 *bmin,
+                                                      float4 *bmax, MacSel ms, uint32_t *err) {
     const uint32_t cnt = glist[0];
     for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
         int parent = (int)glist[64 + j];
         while (parent >= 0) {
             if (parent >= max_nodes) { atomicOr(err, 4u); break; }
             const int4 tp = nodes[parent].topo;
+            if (tp.x < 0 || tp.y < 1 || (int64_t)tp.x + tp.y > n) { atomicOr(err, 4u); break; }
+            if (ms.spans(tp)) break;
             if (arrive_wg(&flags[parent]) == 0) break;
             if (tp.z < 0 || tp.z + 1 >= max_nodes) { atomicOr(err, 4u); break; }
             summarize_pair(parent, tp.z, nodes, bmin, bmax, ms.at(tp.x));
@@ -703,7 +712,7 @@ void launch_lbvh_summaries(const float4 *bodies, int64_t n, NodeRec *nodes, cons
     const unsigned blocks = (unsigned)((n + kSummBodies - 1) / kSummBodies);
     hipLaunchKernelGGL(k_summ_local, dim3(blocks), dim3(kSummThreads), 0, s, bodies, n, nodes, leaf_start, max_nodes, flags,
                        glist, bmin, bmax, ms, err);
-    hipLaunchKernelGGL(k_summ_global, dim3(1), dim3(1024), 0, s, nodes, max_nodes, flags, glist, bmin, bmax, ms,
+    hipLaunchKernelGGL(k_summ_global, dim3(1), dim3(1024), 0, s, nodes, n, max_nodes, flags, glist, bmin, bmax, ms,
                        err);
 }
 
