@@ -37,10 +37,12 @@ enum nbg_status {
     NBG_ERR_STATE = -6      /* call out of order (e.g. counts before a force pass) */
 };
 
-/* Tree solver kind.  BARNES_HUT = BarnesHutSolver (Solvers/BarnesHutSolver.h:14-32,
+/* Solver kind.  BARNES_HUT = BarnesHutSolver (Solvers/BarnesHutSolver.h:14-32,
  * id "barnes-hut"); LINEAR_BVH = LinearBVHSolver (Solvers/LinearBVHSolver.h:18-28,
- * id "linear-bvh"). */
-enum nbg_solver_kind { NBG_BARNES_HUT = 0, NBG_LINEAR_BVH = 1 };
+ * id "linear-bvh"); NAIVE = NaiveSolver (Solvers/NaiveSolver.h:22-52, id "naive"):
+ * direct summation over every body in body order, no tree (theta, leaf_size,
+ * max_depth and morton_bits are ignored). */
+enum nbg_solver_kind { NBG_BARNES_HUT = 0, NBG_LINEAR_BVH = 1, NBG_NAIVE = 2 };
 
 /* Per-body sum order.  ORDERED reproduces the reference's arithmetic exactly:
  * IEEE sqrt/div, no FMA, and the std::transform_reduce bracketing of child

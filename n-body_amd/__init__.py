@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(_HERE, "libnbg.so")
 
 BARNES_HUT = 0
 LINEAR_BVH = 1
+NAIVE = 2
 SUM_ORDERED = 0
 SUM_FAST = 1
 
@@ -652,7 +653,20 @@ class BarnesHutSolver(_TreeSolver):
         self._params.max_depth = d
 
 
-SOLVERS = {"gpu-linear-bvh": LinearBVHSolver, "gpu-barnes-hut": BarnesHutSolver}
+class NaiveSolver(_TreeSolver):
+    """NaiveSolver (src/NBody/Simulation/Solvers/NaiveSolver.h:14-52): direct
+    summation over every body, no tree."""
+
+    _kind = NAIVE
+
+    def id(self) -> str:
+        return "gpu-naive"
+
+    def name(self) -> str:
+        return "Naive (MI355X)"
+
+
+SOLVERS = {"gpu-linear-bvh": LinearBVHSolver, "gpu-barnes-hut": BarnesHutSolver, "gpu-naive": NaiveSolver}
 
 
 def select(solver_id: str):

@@ -77,7 +77,8 @@ namespace {
 
 int check_params(const nbg_params *p) {
     if (!p) return fail(NBG_ERR_ARG, "params is NULL");
-    if (p->solver != NBG_BARNES_HUT && p->solver != NBG_LINEAR_BVH) return fail(NBG_ERR_ARG, "unknown solver kind");
+    if (p->solver != NBG_BARNES_HUT && p->solver != NBG_LINEAR_BVH && p->solver != NBG_NAIVE)
+        return fail(NBG_ERR_ARG, "unknown solver kind");
     if (!(p->theta >= 0.0f)) return fail(NBG_ERR_ARG, "theta must be >= 0");
     if (p->leaf_size < 1) return fail(NBG_ERR_ARG, "leaf_size must be >= 1");
     if (p->max_depth < 0 || p->max_depth > 32) return fail(NBG_ERR_ARG, "max_depth must be in [0, 32]");
@@ -177,6 +178,23 @@ int enqueue_force_pass(nbg_sys *s) {
     if (octree)
         if (int r = ensure_octree(s)) return r;
     launch_fill_u32(s->misc + 4, 0u, 12, st);  // interaction counters
+    if (p.solver == NBG_NAIVE) {
+        // NaiveSolver: no tree; every body sums every body in body order.
+        // SimpleTrackingRule counts n calls per body, np = 0.
+        for (int k = 0; k <= 5; ++k) mark(s, k);
+        launch_fill_u32(s->node_count(), 0u, 1, st);
+        launch_pack(s->x, s->y, s->z, s->m, n, s->bodies, st);
+        const uint64_t pp = (uint64_t)n * (uint64_t)n;
+        launch_fill_u32(s->misc + 4, (uint32_t)pp, 1, st);
+        launch_fill_u32(s->misc + 5, (uint32_t)(pp >> 32), 1, st);
+        mark(s, 6);
+        launch_direct_f32(s->bodies, n, s->bodies, n, nullptr, p.G, p.eps2, p.sum_mode == NBG_SUM_ORDERED, s->ax,
+                          s->ay, s->az, st);
+        mark(s, 7);
+        NBG_HIP(hipGetLastError());
+        s->tree_valid = true;
+        return NBG_OK;
+    }
     mark(s, 0);
     launch_bbox(s->x, s->y, s->z, n, s->bbox_partials, s->bbox, st);
     mark(s, 1);
@@ -247,6 +265,14 @@ WalkArgs walk_args(nbg_sys *s) {
 
 void launch_walk(nbg_sys *s, const WalkArgs &wa, int stats) {
     const nbg_params &p = s->prm;
+    if (p.solver == NBG_NAIVE) {
+        launch_direct_f32(s->bodies, s->n, wa.targets, wa.nt, wa.perm, p.G, p.eps2, p.sum_mode == NBG_SUM_ORDERED,
+                          wa.ax, wa.ay, wa.az, s->stream);
+        const uint64_t pp = (uint64_t)s->n * (uint64_t)wa.nt;
+        launch_fill_u32(s->misc + 4, (uint32_t)pp, 1, s->stream);
+        launch_fill_u32(s->misc + 5, (uint32_t)(pp >> 32), 1, s->stream);
+        return;
+    }
     if (p.solver == NBG_BARNES_HUT)
         launch_walk_oct(wa, p.sum_mode == NBG_SUM_ORDERED, stats, s->stream);
     else
@@ -620,6 +646,7 @@ int nbg_sys_debug_bbox(nbg_sys *s, float *out6) {
 int nbg_sys_debug_order(nbg_sys *s, uint64_t *keys, uint32_t *perm) {
     if (!s || !perm) return fail(NBG_ERR_ARG, "sys or perm is NULL");
     if (!s->have_forces) return fail(NBG_ERR_STATE, "no force pass has run");
+    if (s->prm.solver == NBG_NAIVE) return fail(NBG_ERR_STATE, "the naive solver builds no tree");
     DeviceGuard g(s->ctx->device);
     NBG_HIP(hipStreamSynchronize(s->stream));
     const int64_t n = s->n;

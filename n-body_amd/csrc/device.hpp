@@ -24,6 +24,25 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 __device__ __forceinline__ float ieee_sqrt(float x) { return __builtin_sqrtf(x); }
 __device__ __forceinline__ float ieee_div(float a, float b) { return a / b; }
 
+// Gravity::particleParticle (Physics/Rules/Gravity.h:199-209) with the
+// reference's FP32 operation order: R = p - a, r = sqrt(R.R + eps2),
+// -R m G / (r (r r)); zero when the positions are equal.  Compile without
+// FMA contraction.
+__device__ __forceinline__ void pp_exact(float bx, float by, float bz, float bm, float px, float py, float pz,
+                                         float G, float eps2, float &vx, float &vy, float &vz) {
+    if (bx == px && by == py && bz == pz) {
+        vx = 0.0f; vy = 0.0f; vz = 0.0f;
+        return;
+    }
+    const float Rx = px - bx, Ry = py - by, Rz = pz - bz;
+    const float r2 = Rx * Rx + Ry * Ry + Rz * Rz;
+    const float r = ieee_sqrt(r2 + eps2);
+    const float r3 = r * (r * r);
+    vx = ieee_div((-Rx * bm) * G, r3);
+    vy = ieee_div((-Ry * bm) * G, r3);
+    vz = ieee_div((-Rz * bm) * G, r3);
+}
+
 // Exact form of the reference's LBVH opening test
 //   RN(diag2 / d2) < T,   T = RN(theta * theta)   (DiagonalOverCenterDistance.h:18-22)
 // without a division.  RN is monotone, so RN(q) < T iff q < mid, where mid is

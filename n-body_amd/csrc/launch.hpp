@@ -239,6 +239,12 @@ void launch_walk_oct(const WalkArgs &a, int ordered, int diag, hipStream_t s);
 void launch_update(float *x, float *y, float *z, float *vx, float *vy, float *vz, const float *ax,
                    const float *ay, const float *az, float dt, int64_t n, hipStream_t s);
 
+// NaiveSolver in FP32: every target sums every source in order (ORDERED:
+// the reference's exact arithmetic).  Result of target i goes to
+// out_index[i] (or i when out_index is null).
+void launch_direct_f32(const float4 *src, int64_t ns, const float4 *tgt, int64_t nt, const uint32_t *out_index,
+                       float G, float eps2, int ordered, float *ax, float *ay, float *az, hipStream_t s);
+
 // FP64 direct sum at targets (float4 sources x,y,z,m; targets float4 x,y,z).
 void launch_direct_f64(const float4 *src, int64_t ns, const float4 *tgt, int64_t nt, double G, double eps2,
                        double *ax, double *ay, double *az, hipStream_t s);

@@ -93,6 +93,66 @@ void launch_forest_compact(float *const *dst, const float *const *src, int array
                        nsys, new_first, old_first);
 }
 
+// NaiveSolver::updateAccelerations (Solvers/NaiveSolver.h:22-52) in FP32:
+// each target sums every source in source order, starting from zero.
+// ORDERED uses Gravity::particleParticle's exact arithmetic (bit-exact with
+// the reference's sequential sum); FAST uses rsqrt + FMA with G applied
+// once at the end.  Sources are staged through LDS 256 at a time.
+template <bool ORDERED, bool SOFT>
+__global__ __launch_bounds__(256) void k_direct_f32(const float4 *__restrict__ src, int64_t ns,
+                                                    const float4 *__restrict__ tgt, int64_t nt,
+                                                    const uint32_t *__restrict__ out_index, float G, float eps2,
+                                                    float *__restrict__ ax, float *__restrict__ ay,
+                                                    float *__restrict__ az) {
+    __shared__ float4 s[256];
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const float4 p = i < nt ? tgt[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+    for (int64_t base = 0; base < ns; base += 256) {
+        __syncthreads();
+        if (base + threadIdx.x < ns) s[threadIdx.x] = src[base + threadIdx.x];
+        __syncthreads();
+        const int lim = (ns - base) < 256 ? (int)(ns - base) : 256;
+        for (int j = 0; j < lim; ++j) {
+            const float4 b = s[j];
+            if (ORDERED) {
+                float vx, vy, vz;
+                pp_exact(b.x, b.y, b.z, b.w, p.x, p.y, p.z, G, eps2, vx, vy, vz);
+                rx = rx + vx; ry = ry + vy; rz = rz + vz;
+            } else {
+                const float dx = b.x - p.x, dy = b.y - p.y, dz = b.z - p.z;
+                const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+                float inv = __builtin_amdgcn_rsqf(r2);
+                if (!SOFT) inv = r2 > 0.0f ? inv : 0.0f;
+                const float sc = b.w * (inv * (inv * inv));
+                rx = __builtin_fmaf(dx, sc, rx);
+                ry = __builtin_fmaf(dy, sc, ry);
+                rz = __builtin_fmaf(dz, sc, rz);
+            }
+        }
+    }
+    if (!ORDERED) { rx *= G; ry *= G; rz *= G; }
+    if (i < nt) {
+        const int64_t o = out_index ? (int64_t)out_index[i] : i;
+        if (o < nt) { ax[o] = rx; ay[o] = ry; az[o] = rz; }
+    }
+}
+
+void launch_direct_f32(const float4 *src, int64_t ns, const float4 *tgt, int64_t nt, const uint32_t *out_index,
+                       float G, float eps2, int ordered, float *ax, float *ay, float *az, hipStream_t s) {
+    if (nt <= 0) return;
+    const unsigned blocks = (unsigned)((nt + 255) / 256);
+    if (ordered)
+        hipLaunchKernelGGL((k_direct_f32<true, true>), dim3(blocks), dim3(256), 0, s, src, ns, tgt, nt, out_index, G,
+                           eps2, ax, ay, az);
+    else if (eps2 > 0.0f)
+        hipLaunchKernelGGL((k_direct_f32<false, true>), dim3(blocks), dim3(256), 0, s, src, ns, tgt, nt, out_index,
+                           G, eps2, ax, ay, az);
+    else
+        hipLaunchKernelGGL((k_direct_f32<false, false>), dim3(blocks), dim3(256), 0, s, src, ns, tgt, nt, out_index,
+                           G, eps2, ax, ay, az);
+}
+
 // FP64 direct sum (NaiveSolver generalised, Solvers/NaiveSolver.h:22-52):
 // one thread per target, sources staged through LDS.  Sources at exactly the
 // target position are skipped, as Gravity::particleParticle does

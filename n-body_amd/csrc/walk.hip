@@ -69,21 +69,6 @@ __device__ __forceinline__ WaveTask wave_task(const WalkArgs &a, int64_t w) {
     return t;
 }
 
-// Gravity::particleParticle, exact reference arithmetic.
-__device__ __forceinline__ void pp_exact(float bx, float by, float bz, float bm, float px, float py, float pz,
-                                         float G, float eps2, float &vx, float &vy, float &vz) {
-    if (bx == px && by == py && bz == pz) {
-        vx = 0.0f; vy = 0.0f; vz = 0.0f;
-        return;
-    }
-    const float Rx = px - bx, Ry = py - by, Rz = pz - bz;
-    const float r2 = Rx * Rx + Ry * Ry + Rz * Rz;
-    const float r = ieee_sqrt(r2 + eps2);
-    const float r3 = r * (r * r);
-    vx = ieee_div((-Rx * bm) * G, r3);
-    vy = ieee_div((-Ry * bm) * G, r3);
-    vz = ieee_div((-Rz * bm) * G, r3);
-}
 
 // DiagonalOverCenterDistance (DiagonalOverCenterDistance.h:18-22) for one
 // lane: the squared distance to the bbox centre in glm's order, compared

@@ -216,8 +216,17 @@ public:
     int &mortonBits() { return prm_.morton_bits; }
 };
 
+// NaiveSolver (Solvers/NaiveSolver.h:14-52): direct summation, no tree.
+class GpuNaiveSolver : public Solver {
+public:
+    GpuNaiveSolver(Bodies &sim, Gravity &rule, const Device &dev = default_device())
+        : Solver(sim, rule, NBG_NAIVE, dev) {}
+    std::string id() override { return "gpu-naive"; }
+    std::string name() override { return "Naive (MI355X)"; }
+};
+
 // MultiSolver::select(id) (Simulation/Solvers/MultiSolver.h:79-87) for the
-// two device solvers.
+// device tree solvers.
 inline std::unique_ptr<ActiveTreeSolver> select(const std::string &id, Bodies &sim, Gravity &rule) {
     if (id == "gpu-barnes-hut") return std::make_unique<GpuBarnesHutSolver>(sim, rule);
     if (id == "gpu-linear-bvh") return std::make_unique<GpuLinearBVHSolver>(sim, rule);

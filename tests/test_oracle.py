@@ -392,3 +392,25 @@ def test_plummer_energy_closed_form(orc, nbg):
     K = 0.5 * float(np.sum(b.m.astype(np.float64) * (b.vx.astype(np.float64) ** 2 + b.vy ** 2 + b.vz ** 2)))
     assert abs(W / (-3 * math.pi / 32) - 1) < 0.06
     assert abs(2 * K / abs(W) - 1) < 0.08
+
+
+def test_direct_f32_is_the_sequential_reference_sum(orc, nbg):
+    """NaiveSolver (NaiveSolver.h:22-52): acceleration reset to zero, then
+    += Gravity::particleParticle over the actives in order; restated here in
+    numpy float32 independently of the oracle."""
+    b = nbg.plummer(40, seed=12)
+    f32 = np.float32
+    got = orc.direct_f32(b.x, b.y, b.z, b.m, b.x, b.y, b.z, 1.0, 1e-4)
+    for i in range(b.n):
+        p = np.array([b.x[i], b.y[i], b.z[i]], f32)
+        acc = np.zeros(3, f32)
+        for j in range(b.n):
+            a = np.array([b.x[j], b.y[j], b.z[j]], f32)
+            if np.array_equal(a, p):
+                r = np.zeros(3, f32)
+            else:
+                R = (p - a).astype(f32)
+                rr = f32(np.sqrt(f32(f32(f32(R[0] * R[0]) + f32(R[1] * R[1])) + f32(R[2] * R[2])) + f32(1e-4)))
+                r = ((((-R) * b.m[j]).astype(f32) * f32(1.0)).astype(f32) / f32(rr * f32(rr * rr))).astype(f32)
+            acc = (acc + r).astype(f32)
+        assert np.array_equal(got[i], acc), i
