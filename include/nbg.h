@@ -41,8 +41,19 @@ enum nbg_status {
  * id "barnes-hut"); LINEAR_BVH = LinearBVHSolver (Solvers/LinearBVHSolver.h:18-28,
  * id "linear-bvh"); NAIVE = NaiveSolver (Solvers/NaiveSolver.h:22-52, id "naive"):
  * direct summation over every body in body order, no tree (theta, leaf_size,
- * max_depth and morton_bits are ignored). */
-enum nbg_solver_kind { NBG_BARNES_HUT = 0, NBG_LINEAR_BVH = 1, NBG_NAIVE = 2 };
+ * max_depth and morton_bits are ignored).  The QUADRUPOLE kinds are the
+ * same trees whose nodes also carry the raw second mass moment
+ * (MultipoleMassSummary<2>, Physics/Summaries/MultipoleMassSummary.h) and
+ * interact through Gravity::nodeParticle<2> (Physics/Rules/Gravity.h:238-259):
+ * QuadrupoleBarnesHutSolver (BarnesHutSolver.h:34-66, id "barnes-hut-4p")
+ * and QuadrupoleLinearBVHSolver (LinearBVHSolver.h:30-53, id "linear-bvh-4p"). */
+enum nbg_solver_kind {
+    NBG_BARNES_HUT = 0,
+    NBG_LINEAR_BVH = 1,
+    NBG_NAIVE = 2,
+    NBG_BARNES_HUT_QUADRUPOLE = 3,
+    NBG_LINEAR_BVH_QUADRUPOLE = 4
+};
 
 /* Per-body sum order.  ORDERED reproduces the reference's arithmetic exactly:
  * IEEE sqrt/div, no FMA, and the std::transform_reduce bracketing of child
@@ -176,6 +187,12 @@ int nbg_sys_debug_order(nbg_sys *sys, uint64_t *keys_or_null, uint32_t *perm);
  * centre xyz, side, 0, 0 (Octree.h:73-74). */
 int nbg_sys_debug_tree(nbg_sys *sys, int64_t *num_nodes, int64_t *first, int64_t *count, int32_t *leaf,
                        int32_t *parent, float *com4, float *geo6);
+
+/* Quadrupole solvers: each node's raw second moment sum m (x - c)(x - c)^T
+ * about its centre of mass as (xx, xy, xz, yy, yz, zz)
+ * (MultipoleMassSummary::moment(), Summaries/MultipoleMassSummary.h:99-101),
+ * in nbg_sys_debug_tree's node order; q6 holds cap nodes. */
+int nbg_sys_debug_quads(nbg_sys *sys, int64_t cap, float *q6);
 
 /* ---- direct summation (NaiveSolver, Solvers/NaiveSolver.h:22-52) ---- */
 

@@ -25,6 +25,8 @@ LIB_PATH = os.path.join(_HERE, "libnbg.so")
 BARNES_HUT = 0
 LINEAR_BVH = 1
 NAIVE = 2
+BARNES_HUT_QUADRUPOLE = 3
+LINEAR_BVH_QUADRUPOLE = 4
 SUM_ORDERED = 0
 SUM_FAST = 1
 
@@ -39,7 +41,7 @@ EXPORTS = (
     "nbg_sys_synchronize", "nbg_sys_counts", "nbg_sys_enable_counts", "nbg_sys_enable_walk_stats",
     "nbg_sys_walk_stats",
     "nbg_sys_enable_timing", "nbg_sys_phase_times",
-    "nbg_sys_debug_bbox", "nbg_sys_debug_order", "nbg_sys_debug_tree", "nbg_direct_f64",
+    "nbg_sys_debug_bbox", "nbg_sys_debug_order", "nbg_sys_debug_tree", "nbg_sys_debug_quads", "nbg_direct_f64",
     "nbg_sys_potential_f64", "nbg_generate_plummer", "nbg_generate_cube",
     "nbg_batch_bytes", "nbg_batch_create", "nbg_batch_destroy", "nbg_batch_add", "nbg_batch_remove",
     "nbg_batch_set_params", "nbg_batch_info", "nbg_batch_get", "nbg_batch_update_accelerations", "nbg_batch_step",
@@ -147,6 +149,7 @@ def lib() -> ctypes.CDLL:
             "nbg_sys_debug_bbox": (ctypes.c_int, [_vp, _f32p]),
             "nbg_sys_debug_order": (ctypes.c_int, [_vp, _u64p, _u32p]),
             "nbg_sys_debug_tree": (ctypes.c_int, [_vp, _i64p, _i64p, _i64p, _i32p, _i32p, _f32p, _f32p]),
+            "nbg_sys_debug_quads": (ctypes.c_int, [_vp, ctypes.c_int64, _f32p]),
             "nbg_direct_f64": (ctypes.c_int, [_vp, ctypes.c_int64] + [_f32p] * 4 + [ctypes.c_int64] + [_f32p] * 3
                                + [ctypes.c_double, ctypes.c_double] + [_f64p] * 3),
             "nbg_sys_potential_f64": (ctypes.c_int, [_vp, _f64p]),
@@ -373,6 +376,14 @@ class System:
         _check(lib().nbg_sys_debug_tree(self.handle, ctypes.byref(nn), _ptr(first, _i64p), _ptr(count, _i64p),
                                         _ptr(leaf, _i32p), _ptr(parent, _i32p), _ptr(com4), _ptr(geo6)))
         return dict(first=first, count=count, leaf=leaf.astype(bool), parent=parent, com4=com4, geo6=geo6)
+
+    def debug_quads(self) -> np.ndarray:
+        """Per-node second moments (xx, xy, xz, yy, yz, zz) in debug_tree's order."""
+        nn = ctypes.c_int64()
+        _check(lib().nbg_sys_debug_tree(self.handle, ctypes.byref(nn), None, None, None, None, None, None))
+        q = np.zeros((nn.value, 6), np.float32)
+        _check(lib().nbg_sys_debug_quads(self.handle, nn.value, _ptr(q)))
+        return q
 
     def potential_f64(self) -> float:
         w = ctypes.c_double()
@@ -653,6 +664,32 @@ class BarnesHutSolver(_TreeSolver):
         self._params.max_depth = d
 
 
+class QuadrupoleLinearBVHSolver(_TreeSolver):
+    """QuadrupoleLinearBVHSolver = MultipoleLinearBVHSolver<2>
+    (src/NBody/Simulation/Solvers/LinearBVHSolver.h:30-53)."""
+
+    _kind = LINEAR_BVH_QUADRUPOLE
+
+    def id(self) -> str:
+        return "gpu-linear-bvh-4p"
+
+    def name(self) -> str:
+        return "Linear BVH (Quadrupole, MI355X)"
+
+
+class QuadrupoleBarnesHutSolver(BarnesHutSolver):
+    """QuadrupoleBarnesHutSolver = MultipoleBarnesHutSolver<2>
+    (src/NBody/Simulation/Solvers/BarnesHutSolver.h:34-66)."""
+
+    _kind = BARNES_HUT_QUADRUPOLE
+
+    def id(self) -> str:
+        return "gpu-barnes-hut-4p"
+
+    def name(self) -> str:
+        return "Barnes-Hut (Quadrupole, MI355X)"
+
+
 class NaiveSolver(_TreeSolver):
     """NaiveSolver (src/NBody/Simulation/Solvers/NaiveSolver.h:14-52): direct
     summation over every body, no tree."""
@@ -666,7 +703,8 @@ class NaiveSolver(_TreeSolver):
         return "Naive (MI355X)"
 
 
-SOLVERS = {"gpu-linear-bvh": LinearBVHSolver, "gpu-barnes-hut": BarnesHutSolver, "gpu-naive": NaiveSolver}
+SOLVERS = {"gpu-linear-bvh": LinearBVHSolver, "gpu-barnes-hut": BarnesHutSolver, "gpu-naive": NaiveSolver,
+           "gpu-linear-bvh-4p": QuadrupoleLinearBVHSolver, "gpu-barnes-hut-4p": QuadrupoleBarnesHutSolver}
 
 
 def select(solver_id: str):

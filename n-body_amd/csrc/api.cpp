@@ -18,6 +18,9 @@ using namespace nbg;
 namespace {
 
 constexpr int kPhases = 8;
+
+bool is_octree(int solver) { return solver == NBG_BARNES_HUT || solver == NBG_BARNES_HUT_QUADRUPOLE; }
+bool is_quad(int solver) { return solver == NBG_BARNES_HUT_QUADRUPOLE || solver == NBG_LINEAR_BVH_QUADRUPOLE; }
 constexpr int kEventSets = 256;
 
 }  // namespace
@@ -61,6 +64,9 @@ struct nbg_sys {
     uint32_t *flags = nullptr;
     // octree work and nodes (allocated when the Barnes-Hut solver is selected)
     OctBuffers ob;
+    // second moments, 2 float4 per node slot (allocated for the quadrupole solvers)
+    float4 *quad = nullptr;
+    int64_t quad_slots = 0;
     // small
     float *bbox_partials = nullptr, *bbox = nullptr;
     // [1] err, [2] node count, [4..16) counts: pp, np, then the FAST walk's
@@ -77,7 +83,7 @@ namespace {
 
 int check_params(const nbg_params *p) {
     if (!p) return fail(NBG_ERR_ARG, "params is NULL");
-    if (p->solver != NBG_BARNES_HUT && p->solver != NBG_LINEAR_BVH && p->solver != NBG_NAIVE)
+    if (p->solver < NBG_BARNES_HUT || p->solver > NBG_LINEAR_BVH_QUADRUPOLE)
         return fail(NBG_ERR_ARG, "unknown solver kind");
     if (!(p->theta >= 0.0f)) return fail(NBG_ERR_ARG, "theta must be >= 0");
     if (p->leaf_size < 1) return fail(NBG_ERR_ARG, "leaf_size must be >= 1");
@@ -99,7 +105,7 @@ void free_oct(OctBuffers &b) {
 // Allocates the octree buffers for the system's size and parameters (no-op
 // when the current ones are large enough).
 int ensure_octree(nbg_sys *s) {
-    if (s->prm.solver != NBG_BARNES_HUT) return NBG_OK;
+    if (!is_octree(s->prm.solver)) return NBG_OK;
     const int64_t n = s->n;
     const int64_t cap = octree_capacity(n, s->prm.leaf_size, s->prm.max_depth);
     if (cap >= ((int64_t)1 << 28))
@@ -128,8 +134,29 @@ int ensure_octree(nbg_sys *s) {
     return NBG_OK;
 }
 
+// Allocates the quadrupole moment array for the current tree's node slots
+// (no-op for monopole solvers or when the current array is large enough).
+int ensure_quad(nbg_sys *s) {
+    if (!is_quad(s->prm.solver)) return NBG_OK;
+    const int64_t slots = is_octree(s->prm.solver) ? s->ob.cap : 2 * s->cap;
+    if (s->quad_slots >= slots) return NBG_OK;
+    if (s->stream) NBG_HIP(hipStreamSynchronize(s->stream));
+    if (s->quad) (void)hipFree(s->quad);
+    s->quad = nullptr;
+    s->quad_slots = 0;
+    if (int r = dalloc(&s->quad, 2 * (slots > 0 ? slots : 1))) return r;
+    s->quad_slots = slots;
+    return NBG_OK;
+}
+
+int ensure_buffers(nbg_sys *s) {
+    if (int r = ensure_octree(s)) return r;
+    return ensure_quad(s);
+}
+
 void free_sys(nbg_sys *s) {
     free_oct(s->ob);
+    if (s->quad) (void)hipFree(s->quad);
     void *ptrs[] = {s->x, s->y, s->z, s->m, s->vx, s->vy, s->vz, s->ax, s->ay, s->az, s->keys, s->keys_alt,
                     s->idx, s->idx_alt, s->bodies, s->kfirst, s->klast, s->kgamma, s->kparent, s->ksplit,
                     s->krank, s->leaf_start, s->glist, s->scan_ws, s->sort_status, s->nodes, s->bmin,
@@ -174,9 +201,9 @@ int enqueue_force_pass(nbg_sys *s) {
     const int64_t n = s->n;
     const nbg_params &p = s->prm;
     hipStream_t st = s->stream;
-    const bool octree = p.solver == NBG_BARNES_HUT;
-    if (octree)
-        if (int r = ensure_octree(s)) return r;
+    const bool octree = is_octree(p.solver);
+    if (int r = ensure_buffers(s)) return r;
+    float4 *quad = is_quad(p.solver) ? s->quad : nullptr;
     launch_fill_u32(s->misc + 4, 0u, 12, st);  // interaction counters
     if (p.solver == NBG_NAIVE) {
         // NaiveSolver: no tree; every body sums every body in body order.
@@ -210,7 +237,7 @@ int enqueue_force_pass(nbg_sys *s) {
         mark(s, 4);
         launch_gather(s->x, s->y, s->z, s->m, s->idx, n, s->bodies, st);
         mark(s, 5);
-        launch_octree_summaries(s->bodies, n, ob, s->leaf_start, s->err_word(), st);
+        launch_octree_summaries(s->bodies, n, ob, s->leaf_start, s->err_word(), st, quad);
     } else {
         launch_morton(s->x, s->y, s->z, n, s->bbox, p.morton_bits, s->keys, s->idx, st);
         mark(s, 2);
@@ -223,7 +250,7 @@ int enqueue_force_pass(nbg_sys *s) {
         launch_gather(s->x, s->y, s->z, s->m, s->idx, n, s->bodies, st);
         mark(s, 5);
         launch_lbvh_summaries(s->bodies, n, s->nodes, s->leaf_start, 2 * n, s->flags, s->glist, s->bmin, s->bmax,
-                              p.theta, s->err_word(), st);
+                              p.theta, s->err_word(), st, nullptr, nullptr, 0, quad);
     }
     mark(s, 6);
     WalkArgs wa = walk_args(s);
@@ -239,7 +266,7 @@ int enqueue_force_pass(nbg_sys *s) {
 WalkArgs walk_args(nbg_sys *s) {
     const int64_t n = s->n;
     const nbg_params &p = s->prm;
-    const bool octree = p.solver == NBG_BARNES_HUT;
+    const bool octree = is_octree(p.solver);
     WalkArgs wa;
     wa.n = n;
     wa.bodies = s->bodies;
@@ -260,6 +287,7 @@ WalkArgs walk_args(nbg_sys *s) {
     wa.az = s->az;
     wa.counts = s->counts();
     wa.err = s->err_word();
+    wa.quad = is_quad(p.solver) ? s->quad : nullptr;
     return wa;
 }
 
@@ -273,7 +301,7 @@ void launch_walk(nbg_sys *s, const WalkArgs &wa, int stats) {
         launch_fill_u32(s->misc + 5, (uint32_t)(pp >> 32), 1, s->stream);
         return;
     }
-    if (p.solver == NBG_BARNES_HUT)
+    if (is_octree(p.solver))
         launch_walk_oct(wa, p.sum_mode == NBG_SUM_ORDERED, stats, s->stream);
     else
         launch_walk_lbvh(wa, p.sum_mode == NBG_SUM_ORDERED, p.morton_bits == 30 ? 32 : 64, stats, s->stream);
@@ -381,7 +409,7 @@ int nbg_sys_create(nbg_ctx *ctx, const nbg_params *p, int64_t n, nbg_sys **out) 
         if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
         if (e != hipSuccess) r = fail(NBG_ERR_HIP, std::string("init: ") + hipGetErrorString(e));
     }
-    if (r == NBG_OK) r = ensure_octree(s);
+    if (r == NBG_OK) r = ensure_buffers(s);
     if (r != NBG_OK) {
         free_sys(s);
         delete s;
@@ -406,7 +434,7 @@ int nbg_sys_set_params(nbg_sys *s, const nbg_params *p) {
     DeviceGuard g(s->ctx->device);
     const nbg_params old = s->prm;
     s->prm = *p;
-    if (int r = ensure_octree(s)) {
+    if (int r = ensure_buffers(s)) {
         s->prm = old;
         return r;
     }
@@ -671,7 +699,7 @@ int nbg_sys_debug_tree(nbg_sys *s, int64_t *num_nodes, int64_t *first, int64_t *
     DeviceGuard g(s->ctx->device);
     NBG_HIP(hipStreamSynchronize(s->stream));
     uint32_t nn = 0;
-    const bool octree = s->prm.solver == NBG_BARNES_HUT;
+    const bool octree = is_octree(s->prm.solver);
     if (octree && s->n > 0) NBG_HIP(hipMemcpy(&nn, s->ob.misc, 4, hipMemcpyDeviceToHost));
     else if (!octree) NBG_HIP(hipMemcpy(&nn, s->node_count(), 4, hipMemcpyDeviceToHost));
     *num_nodes = nn;
@@ -717,6 +745,24 @@ int nbg_sys_debug_tree(nbg_sys *s, int64_t *num_nodes, int64_t *first, int64_t *
             float gg[6] = {lo[i].x, lo[i].y, lo[i].z, hi[i].x, hi[i].y, hi[i].z};
             std::memcpy(geo6 + 6 * i, gg, sizeof gg);
         }
+    }
+    return NBG_OK;
+}
+
+int nbg_sys_debug_quads(nbg_sys *s, int64_t cap, float *q6) {
+    if (!s || !q6) return fail(NBG_ERR_ARG, "sys or q6 is NULL");
+    if (!is_quad(s->prm.solver)) return fail(NBG_ERR_STATE, "the solver carries no quadrupole moments");
+    if (!s->have_forces) return fail(NBG_ERR_STATE, "no force pass has run");
+    int64_t nn = 0;
+    if (int r = nbg_sys_debug_tree(s, &nn, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr)) return r;
+    if (cap < nn) return fail(NBG_ERR_ARG, "q6 holds fewer nodes than the tree has");
+    if (nn == 0) return NBG_OK;
+    std::vector<float4> h((size_t)(2 * nn));
+    NBG_HIP(hipMemcpy(h.data(), s->quad, h.size() * sizeof(float4), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nn; ++i) {
+        const float4 a = h[2 * i], b = h[2 * i + 1];
+        const float v[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
+        std::memcpy(q6 + 6 * i, v, sizeof v);
     }
     return NBG_OK;
 }

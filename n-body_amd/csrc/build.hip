@@ -598,6 +598,25 @@ __device__ __forceinline__ uint32_t arrive_wg(uint32_t *flag) {
     return __hip_atomic_fetch_add(flag, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// MultipoleMassSummary::summarize(childNodes) (MultipoleMassSummary.h:54-65)
+// for two children: (0 + t0) + t1 with t = outer(c_child - c, c_child) m_child
+// + Q_child.
+__device__ __forceinline__ void quad_pair(int p, int c0, const NodeRec *nodes, float4 *quad) {
+    const float4 c = nodes[p].com;
+    float Q[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const float4 a = nodes[c0 + k].com;
+        const float4 q0 = quad[2 * (c0 + k)], q1 = quad[2 * (c0 + k) + 1];
+        const float cq[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+        float t[6];
+        quad_term(a.x - c.x, a.y - c.y, a.z - c.z, a.w, t);
+#pragma unroll
+        for (int e = 0; e < 6; ++e) Q[e] = Q[e] + (t[e] + cq[e]);
+    }
+    quad_store(quad, p, Q);
+}
+
 __device__ void summarize_pair(int p, int c0, NodeRec *nodes, float4 *bmin, float4 *bmax, MacParam mp) {
     const float4 a = nodes[c0].com;
     const float4 b = nodes[c0 + 1].com;
@@ -643,11 +662,30 @@ __device__ void summarize_leaf(int s, int4 t, const float4 *__restrict__ bodies,
     nodes[s].mac = mac_record(mn, mx, mp);
 }
 
+// Leaf and pair summaries plus, for a quadrupole tree (quad != null), the
+// second moments.
+__device__ __forceinline__ void summarize_leaf_q(int s, int4 t, const float4 *__restrict__ bodies, NodeRec *nodes,
+                                                 float4 *bmin, float4 *bmax, MacParam mp, float4 *quad) {
+    summarize_leaf(s, t, bodies, nodes, bmin, bmax, mp);
+    if (quad) {
+        const float4 c = nodes[s].com;
+        float Q[6];
+        quad_bodies(bodies, t.x, t.y, c.x, c.y, c.z, Q);
+        quad_store(quad, s, Q);
+    }
+}
+
+__device__ __forceinline__ void summarize_pair_q(int p, int c0, NodeRec *nodes, float4 *bmin, float4 *bmax,
+                                                 MacParam mp, float4 *quad) {
+    summarize_pair(p, c0, nodes, bmin, bmax, mp);
+    if (quad) quad_pair(p, c0, nodes, quad);
+}
+
 __global__ __launch_bounds__(kSummThreads) void k_summ_local(const float4 *__restrict__ bodies, int64_t n,
                                                     NodeRec *nodes,
                                                     const int32_t *__restrict__ leaf_start, int64_t max_nodes,
                                                     uint32_t *flags, uint32_t *glist, float4 *bmin,
-                                                    float4 *bmax, MacSel ms, uint32_t *err) {
+                                                    float4 *bmax, MacSel ms, float4 *quad, uint32_t *err) {
     const int64_t lo = (int64_t)blockIdx.x * kSummBodies, hi = lo + kSummBodies;
     const int64_t end = hi < n ? hi : n;
     for (int64_t i = lo + threadIdx.x; i < end; i += blockDim.x) {
@@ -656,7 +694,7 @@ __global__ __launch_bounds__(kSummThreads) void k_summ_local(const float4 *__res
         if (s >= max_nodes) { atomicOr(err, 4u); continue; }
         const int4 t = nodes[s].topo;
         if (t.x != (int)i || t.y < 1 || (int64_t)t.x + t.y > n || t.z >= 0) { atomicOr(err, 4u); continue; }
-        summarize_leaf(s, t, bodies, nodes, bmin, bmax, ms.at(t.x));
+        summarize_leaf_q(s, t, bodies, nodes, bmin, bmax, ms.at(t.x), quad);
         int parent = t.w;
         while (parent >= 0) {
             if (parent >= max_nodes) { atomicOr(err, 4u); break; }
@@ -672,7 +710,7 @@ __global__ __launch_bounds__(kSummThreads) void k_summ_local(const float4 *__res
             }
             if (arrive_wg(&flags[parent]) == 0) break;
             if (tp.z < 0 || tp.z + 1 >= max_nodes) { atomicOr(err, 4u); break; }
-            summarize_pair(parent, tp.z, nodes, bmin, bmax, ms.at(tp.x));
+            summarize_pair_q(parent, tp.z, nodes, bmin, bmax, ms.at(tp.x), quad);
             parent = tp.w;
         }
     }
@@ -680,7 +718,7 @@ __global__ __launch_bounds__(kSummThreads) void k_summ_local(const float4 *__res
 
 __global__ __launch_bounds__(1024) void k_summ_global(NodeRec *nodes, int64_t n, int64_t max_nodes,
                                                       uint32_t *flags, const uint32_t *glist, float4 *bmin,
-                                                      float4 *bmax, MacSel ms, uint32_t *err) {
+                                                      float4 *bmax, MacSel ms, float4 *quad, uint32_t *err) {
     const uint32_t cnt = glist[0];
     for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
         int parent = (int)glist[64 + j];
@@ -691,7 +729,7 @@ __global__ __launch_bounds__(1024) void k_summ_global(NodeRec *nodes, int64_t n,
             if (ms.spans(tp)) break;
             if (arrive_wg(&flags[parent]) == 0) break;
             if (tp.z < 0 || tp.z + 1 >= max_nodes) { atomicOr(err, 4u); break; }
-            summarize_pair(parent, tp.z, nodes, bmin, bmax, ms.at(tp.x));
+            summarize_pair_q(parent, tp.z, nodes, bmin, bmax, ms.at(tp.x), quad);
             parent = tp.w;
         }
     }
@@ -700,7 +738,7 @@ __global__ __launch_bounds__(1024) void k_summ_global(NodeRec *nodes, int64_t n,
 void launch_lbvh_summaries(const float4 *bodies, int64_t n, NodeRec *nodes, const int32_t *leaf_start,
                            int64_t max_nodes, uint32_t *flags, uint32_t *glist, float4 *bmin, float4 *bmax,
                            float theta, uint32_t *err, hipStream_t s, const MacParam *mp_tab,
-                           const uint64_t *keys, int sys_shift) {
+                           const uint64_t *keys, int sys_shift, float4 *quad) {
     if (n <= 0) return;
     MacSel ms;
     mac_threshold(theta, &ms.one.mid, &ms.one.tie);
@@ -711,9 +749,9 @@ void launch_lbvh_summaries(const float4 *bodies, int64_t n, NodeRec *nodes, cons
     launch_fill_u32(glist, 0u, 64, s);
     const unsigned blocks = (unsigned)((n + kSummBodies - 1) / kSummBodies);
     hipLaunchKernelGGL(k_summ_local, dim3(blocks), dim3(kSummThreads), 0, s, bodies, n, nodes, leaf_start, max_nodes, flags,
-                       glist, bmin, bmax, ms, err);
+                       glist, bmin, bmax, ms, quad, err);
     hipLaunchKernelGGL(k_summ_global, dim3(1), dim3(1024), 0, s, nodes, n, max_nodes, flags, glist, bmin, bmax, ms,
-                       err);
+                       quad, err);
 }
 
 // ------------------------------------------------------------------ forest

@@ -43,6 +43,109 @@ __device__ __forceinline__ void pp_exact(float bx, float by, float bz, float bm,
     vz = ieee_div((-Rz * bm) * G, r3);
 }
 
+// ---------------------------------------------------------------- quadrupole
+// Second moments are stored per node slot as two float4:
+//   q[2 s] = (xx, xy, xz, yy),  q[2 s + 1] = (yz, zz, xx + yy + zz, 0).
+// The entries are MultipoleMoment<2>'s raw sum m (x - c)(x - c)^T about the
+// node's centre of mass c (Physics/MultipoleMoment.h:20-55), in its storage
+// order; the trace is for the fast evaluation only.
+
+// One term outerProduct(o, o) * m (SymmetricTensor3.h:531-536, Multipole.h:96-101).
+__device__ __forceinline__ void quad_term(float ox, float oy, float oz, float m, float t[6]) {
+    t[0] = (ox * ox) * m; t[1] = (ox * oy) * m; t[2] = (ox * oz) * m;
+    t[3] = (oy * oy) * m; t[4] = (oy * oz) * m; t[5] = (oz * oz) * m;
+}
+
+__device__ __forceinline__ void quad_store(float4 *q, int64_t s, const float Q[6]) {
+    q[2 * s] = make_float4(Q[0], Q[1], Q[2], Q[3]);
+    q[2 * s + 1] = make_float4(Q[4], Q[5], (Q[0] + Q[3]) + Q[5], 0.0f);
+}
+
+// MultipoleMassSummary::summarize(entities) (MultipoleMassSummary.h:39-51):
+// libstdc++'s transform_reduce over the bodies, 0 + ((t0 + t1) + (t2 + t3))
+// per block of four, then the rest one at a time; t = outer(x - c, x - c) m.
+__device__ __forceinline__ void quad_bodies(const float4 *__restrict__ bodies, int first, int cnt, float cx,
+                                            float cy, float cz, float Q[6]) {
+#pragma unroll
+    for (int e = 0; e < 6; ++e) Q[e] = 0.0f;
+    int i = 0;
+    for (; cnt - i >= 4; i += 4) {
+        float t[4][6];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float4 b = bodies[first + i + k];
+            quad_term(b.x - cx, b.y - cy, b.z - cz, b.w, t[k]);
+        }
+#pragma unroll
+        for (int e = 0; e < 6; ++e) Q[e] = Q[e] + ((t[0][e] + t[1][e]) + (t[2][e] + t[3][e]));
+    }
+    for (; i < cnt; ++i) {
+        const float4 b = bodies[first + i];
+        float t[6];
+        quad_term(b.x - cx, b.y - cy, b.z - cz, b.w, t);
+#pragma unroll
+        for (int e = 0; e < 6; ++e) Q[e] = Q[e] + t[e];
+    }
+}
+
+// Gravity::nodeParticle<2> (Physics/Rules/Gravity.h:238-259) in the
+// reference's FP32 operation order: R = p - c, r = sqrt(R.R + eps2),
+// dPhi = (R * -M) / (r (r r)) + (D<3>(R, r) . Q) / 2, result G * dPhi.
+// D<3> (Gravity.h:43-93): entry (a,b,c) = R_a (R_b R_c) * g3 + B * g2 with
+// B = 3 R_a on (a,a,a), the odd index's component on (a,a,b) / (a,b,b) and
+// 0 on (x,y,z); f2 = 3 / r^3, f3 = -15 / r^4, g2 = f2 / r^2, g3 = f3 / r^3.
+// The contraction (SymmetricTensor3.h:266-276) adds D_ijk Q_jk into
+// component i over (j, k) in lexicographic order from zero.  Compile
+// without FMA contraction.
+__device__ __forceinline__ void np_quad_exact(float cx, float cy, float cz, float M, float4 q0, float4 q1, float px,
+                                              float py, float pz, float G, float eps2, float &vx, float &vy,
+                                              float &vz) {
+    if (cx == px && cy == py && cz == pz) {
+        vx = 0.0f; vy = 0.0f; vz = 0.0f;
+        return;
+    }
+    const float R[3] = {px - cx, py - cy, pz - cz};
+    const float r = ieee_sqrt((R[0] * R[0] + R[1] * R[1] + R[2] * R[2]) + eps2);
+    const float r3 = r * (r * r);
+    const float f2 = ieee_div(3.0f, r3), f3 = ieee_div(-15.0f, r * r3);
+    const float g2 = ieee_div(f2, r * r), g3 = ieee_div(f3, r3);
+    // unique entries: xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz
+    constexpr int kA[10] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 2};
+    constexpr int kB[10] = {0, 0, 0, 1, 1, 2, 1, 1, 2, 2};
+    constexpr int kC[10] = {0, 1, 2, 1, 2, 2, 1, 2, 2, 2};
+    float D[10];
+#pragma unroll
+    for (int u = 0; u < 10; ++u) {
+        const int a = kA[u], b = kB[u], c = kC[u];
+        const float cp = R[a] * (R[b] * R[c]);
+        float B;
+        if (a == b && b == c) B = 0.0f + R[a] * 3.0f;
+        else if (a != b && b != c) B = 0.0f + 0.0f * 3.0f;
+        else B = (a == b ? R[c] : R[a]) + 0.0f * 3.0f;
+        D[u] = cp * g3 + B * g2;
+    }
+    // sorted (i, j, k) -> unique index; (j, k) -> moment index
+    constexpr int kU3[3][3][3] = {{{0, 1, 2}, {1, 3, 4}, {2, 4, 5}},
+                                  {{1, 3, 4}, {3, 6, 7}, {4, 7, 8}},
+                                  {{2, 4, 5}, {4, 7, 8}, {5, 8, 9}}};
+    constexpr int kU2[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    const float Q[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+    float h[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc = acc + D[kU3[i][j][k]] * Q[kU2[j][k]];
+        h[i] = acc;
+    }
+    const float dx = ieee_div(R[0] * -M, r3) + h[0] / 2.0f;
+    const float dy = ieee_div(R[1] * -M, r3) + h[1] / 2.0f;
+    const float dz = ieee_div(R[2] * -M, r3) + h[2] / 2.0f;
+    vx = G * dx; vy = G * dy; vz = G * dz;
+}
+
 // Exact form of the reference's LBVH opening test
 //   RN(diag2 / d2) < T,   T = RN(theta * theta)   (DiagonalOverCenterDistance.h:18-22)
 // without a division.  RN is monotone, so RN(q) < T iff q < mid, where mid is

@@ -111,7 +111,7 @@ void launch_octree_topology(int64_t n, const float *bbox6, int leaf_size, int ma
                             OctBuffers &ob, const uint64_t *keys, const uint32_t *idx_alt, uint32_t *idx,
                             int32_t *leaf_start, uint32_t *err, hipStream_t s);
 void launch_octree_summaries(const float4 *bodies, int64_t n, OctBuffers &ob, const int32_t *leaf_start,
-                             uint32_t *err, hipStream_t s);
+                             uint32_t *err, hipStream_t s, float4 *quad = nullptr);
 
 void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s);
 
@@ -165,10 +165,12 @@ struct MacParam {       // mac_threshold's output for one theta
     int tie;
 };
 // mp_tab (forest): per-system MAC parameters, indexed by keys[first] >> sys_shift.
+// quad (quadrupole trees): 2 * max_nodes float4 of second moments (layout in
+// device.hpp), or null.
 void launch_lbvh_summaries(const float4 *bodies, int64_t n, NodeRec *nodes, const int32_t *leaf_start,
                            int64_t max_nodes, uint32_t *flags, uint32_t *glist, float4 *bmin, float4 *bmax,
                            float theta, uint32_t *err, hipStream_t s, const MacParam *mp_tab = nullptr,
-                           const uint64_t *keys = nullptr, int sys_shift = 0);
+                           const uint64_t *keys = nullptr, int sys_shift = 0, float4 *quad = nullptr);
 
 // Forest of independent systems (30-bit codes, u64 keys = system << 30 | code):
 // per-system bounding boxes (bbox6: 6 floats per system), keys, and one sort.
@@ -225,6 +227,9 @@ struct WalkArgs {
     const int64_t *sys_first = nullptr;
     const int32_t *sys_root = nullptr;
     const float4 *sys_prm = nullptr;
+    // Quadrupole trees: second moments per node slot (2 float4, layout in
+    // device.hpp); null for monopole trees.
+    const float4 *quad = nullptr;
 };
 
 // Barnes-Hut traversal over the LBVH.  ordered != 0 selects the

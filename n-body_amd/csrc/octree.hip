@@ -372,7 +372,8 @@ __device__ __forceinline__ uint32_t arrive(uint32_t *flag) {
     return __hip_atomic_fetch_add(flag, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-__device__ __forceinline__ void oct_summ_leaf(int s, int4 t, const float4 *__restrict__ bodies, OctRec *nodes) {
+__device__ __forceinline__ void oct_summ_leaf(int s, int4 t, const float4 *__restrict__ bodies, OctRec *nodes,
+                                              float4 *quad) {
     float M = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
     for (int i = t.x; i < t.x + t.y; ++i) {
         const float4 b = bodies[i];
@@ -382,9 +383,50 @@ __device__ __forceinline__ void oct_summ_leaf(int s, int4 t, const float4 *__res
         cz = cz + b.z * b.w;
     }
     nodes[s].com = make_float4(ieee_div(cx, M), ieee_div(cy, M), ieee_div(cz, M), M);
+    if (quad) {
+        float Q[6];
+        quad_bodies(bodies, t.x, t.y, ieee_div(cx, M), ieee_div(cy, M), ieee_div(cz, M), Q);
+        quad_store(quad, s, Q);
+    }
 }
 
-__device__ __forceinline__ void oct_summ_inner(int p, const OctRec &r, OctRec *nodes) {
+// MultipoleMassSummary::summarize(childNodes) (MultipoleMassSummary.h:54-65)
+// over the 8 octants: 0 + ((t0 + t1) + (t2 + t3)), then + ((t4 + t5) +
+// (t6 + t7)).  An empty octant's term, outer(0 - c, 0 - c) * 0 + 0, is +0
+// for a finite c, and is used as such.
+__device__ __forceinline__ void oct_quad_inner(int p, const OctRec &r, const OctRec *nodes, float4 *quad,
+                                               float4 com) {
+    const int c0 = r.topo.z, nch = oct_nchild(r);
+    float t[8][6];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+#pragma unroll
+        for (int e = 0; e < 6; ++e) t[q][e] = 0.0f;
+    for (int k = 0; k < nch; ++k) {
+        const OctRec &ch = nodes[c0 + k];
+        const float4 a = ch.com;
+        const float4 q0 = quad[2 * (c0 + k)], q1 = quad[2 * (c0 + k) + 1];
+        const float cq[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+        float u[6];
+        quad_term(a.x - com.x, a.y - com.y, a.z - com.z, a.w, u);
+        const int o = oct_octant(ch) & 7;
+        // a dynamic octant index into t would spill it to scratch
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (q == o)
+#pragma unroll
+                for (int e = 0; e < 6; ++e) t[q][e] = u[e] + cq[e];
+    }
+    float Q[6];
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+        Q[e] = 0.0f + ((t[0][e] + t[1][e]) + (t[2][e] + t[3][e]));
+        Q[e] = Q[e] + ((t[4][e] + t[5][e]) + (t[6][e] + t[7][e]));
+    }
+    quad_store(quad, p, Q);
+}
+
+__device__ __forceinline__ void oct_summ_inner(int p, const OctRec &r, OctRec *nodes, float4 *quad) {
     const int c0 = r.topo.z, nch = oct_nchild(r);
     float M = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
     for (int k = 0; k < nch; ++k) {
@@ -400,12 +442,14 @@ __device__ __forceinline__ void oct_summ_inner(int p, const OctRec &r, OctRec *n
         com.x = q; com.y = q; com.z = q;
     }
     nodes[p].com = com;
+    if (quad) oct_quad_inner(p, r, nodes, quad, com);
 }
 
 __global__ __launch_bounds__(kSummThreads) void k_oct_summ_local(const float4 *__restrict__ bodies, int64_t n,
                                                         OctRec *nodes, const int32_t *__restrict__ leaf_start,
                                                         const uint32_t *__restrict__ misc, uint32_t *flags,
-                                                        uint32_t *glist, int64_t glist_cap, uint32_t *err) {
+                                                        uint32_t *glist, int64_t glist_cap, float4 *quad,
+                                                        uint32_t *err) {
     const int64_t lo = (int64_t)blockIdx.x * kSummBodies, hi = lo + kSummBodies;
     const int64_t end = hi < n ? hi : n;
     const int64_t T = misc[0];
@@ -415,7 +459,7 @@ __global__ __launch_bounds__(kSummThreads) void k_oct_summ_local(const float4 *_
         if (s >= T) { atomicOr(err, 4u); continue; }
         const int4 t = nodes[s].topo;
         if (t.x != (int)i || t.y < 1 || (int64_t)t.x + t.y > n || t.z >= 0) { atomicOr(err, 4u); continue; }
-        oct_summ_leaf(s, t, bodies, nodes);
+        oct_summ_leaf(s, t, bodies, nodes, quad);
         int parent = t.w;
         for (int hops = 0; parent >= 0; ++hops) {
             // parents sit in lower slots than their children; a valid tree
@@ -432,7 +476,7 @@ __global__ __launch_bounds__(kSummThreads) void k_oct_summ_local(const float4 *_
             }
             if (arrive(&flags[parent]) + 1u < (uint32_t)oct_nchild(r)) break;
             if (tp.z <= parent || tp.z + oct_nchild(r) > T || tp.w >= parent) { atomicOr(err, 4u); break; }
-            oct_summ_inner(parent, r, nodes);
+            oct_summ_inner(parent, r, nodes, quad);
             parent = tp.w;
         }
     }
@@ -440,7 +484,7 @@ __global__ __launch_bounds__(kSummThreads) void k_oct_summ_local(const float4 *_
 
 __global__ __launch_bounds__(1024) void k_oct_summ_global(OctRec *nodes, const uint32_t *__restrict__ misc,
                                                           uint32_t *flags, const uint32_t *glist,
-                                                          int64_t glist_cap, uint32_t *err) {
+                                                          int64_t glist_cap, float4 *quad, uint32_t *err) {
     const int64_t T = misc[0];
     const uint32_t cnt = glist[0] < glist_cap ? glist[0] : (uint32_t)glist_cap;
     for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
@@ -453,7 +497,7 @@ __global__ __launch_bounds__(1024) void k_oct_summ_global(OctRec *nodes, const u
                 atomicOr(err, 4u);
                 break;
             }
-            oct_summ_inner(parent, r, nodes);
+            oct_summ_inner(parent, r, nodes, quad);
             parent = r.topo.w;
         }
     }
@@ -532,14 +576,14 @@ void launch_octree_topology(int64_t n, const float *bbox6, int leaf_size, int ma
 }
 
 void launch_octree_summaries(const float4 *bodies, int64_t n, OctBuffers &ob, const int32_t *leaf_start,
-                             uint32_t *err, hipStream_t s) {
+                             uint32_t *err, hipStream_t s, float4 *quad) {
     if (n <= 0) return;
     launch_fill_u32(ob.flags, 0u, ob.cap, s);
     launch_fill_u32(ob.glist, 0u, 64, s);
     hipLaunchKernelGGL(k_oct_summ_local, dim3(blocks_for(n, kSummBodies)), dim3(kSummThreads), 0, s, bodies, n, ob.nodes,
-                       leaf_start, ob.misc, ob.flags, ob.glist, ob.cap, err);
+                       leaf_start, ob.misc, ob.flags, ob.glist, ob.cap, quad, err);
     hipLaunchKernelGGL(k_oct_summ_global, dim3(1), dim3(1024), 0, s, ob.nodes, ob.misc, ob.flags, ob.glist, ob.cap,
-                       err);
+                       quad, err);
 }
 
 }  // namespace nbg

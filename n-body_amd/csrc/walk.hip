@@ -81,7 +81,7 @@ __device__ __forceinline__ bool lbvh_accepts(float4 mc, float px, float py, floa
 }
 
 // ---------------------------------------------------------------- ORDERED
-template <int MAXD, bool FOREST>
+template <int MAXD, bool FOREST, bool QUAD>
 __global__ __launch_bounds__(64) void k_walk_lbvh_ordered(WalkArgs a) {
     __shared__ int s_node[kStackLBVH];
     __shared__ int s_meta[kStackLBVH];
@@ -131,7 +131,11 @@ __global__ __launch_bounds__(64) void k_walk_lbvh_ordered(WalkArgs a) {
         if (pass) {
             ++cnp;
             float vx, vy, vz;
-            pp_exact(nd.com.x, nd.com.y, nd.com.z, nd.com.w, px, py, pz, G, eps2, vx, vy, vz);
+            if (QUAD)
+                np_quad_exact(nd.com.x, nd.com.y, nd.com.z, nd.com.w, a.quad[2 * node], a.quad[2 * node + 1], px, py,
+                              pz, G, eps2, vx, vy, vz);
+            else
+                pp_exact(nd.com.x, nd.com.y, nd.com.z, nd.com.w, px, py, pz, G, eps2, vx, vy, vz);
             rx = rx + vx; ry = ry + vy; rz = rz + vz;
         }
         const uint64_t open = mask & ~__ballot(pass);
@@ -204,6 +208,32 @@ __device__ __forceinline__ void interact(float bx, float by, float bz, float bm,
     W.rz = __builtin_fmaf(dz, s, W.rz);
 }
 
+// Quadrupole node interaction (Gravity::nodeParticle<2>, Gravity.h:238-259)
+// with rsqrt + FMA.  With d = c - p (the reference's R = -d), the reference's
+// R (-M) / r^3 + D3(R) . Q / 2 expands to
+//   d (M r^-3 + 7.5 r^-7 d.Q.d - 1.5 r^-5 tr Q) - 3 r^-5 Q d,
+// r^2 = d.d + eps2; G is applied once at the end of the walk.  d = 0 (the
+// target at the node's COM) contributes exactly zero.
+template <bool SOFT>
+__device__ __forceinline__ void interact_quad(float bx, float by, float bz, float bm, float4 q0, float4 q1,
+                                              float eps2, FastWave &W) {
+    const float dx = bx - W.px, dy = by - W.py, dz = bz - W.pz;
+    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+    float inv = __builtin_amdgcn_rsqf(r2);
+    if (!SOFT) inv = r2 > 0.0f ? inv : 0.0f;
+    const float inv2 = inv * inv, inv3 = inv * inv2, inv5 = inv3 * inv2;
+    // Q d with Q = [[xx xy xz] [xy yy yz] [xz yz zz]] = [[q0.x q0.y q0.z] [q0.y q0.w q1.x] [q0.z q1.x q1.y]]
+    const float qx = __builtin_fmaf(q0.x, dx, __builtin_fmaf(q0.y, dy, q0.z * dz));
+    const float qy = __builtin_fmaf(q0.y, dx, __builtin_fmaf(q0.w, dy, q1.x * dz));
+    const float qz = __builtin_fmaf(q0.z, dx, __builtin_fmaf(q1.x, dy, q1.y * dz));
+    const float dqd = __builtin_fmaf(dx, qx, __builtin_fmaf(dy, qy, dz * qz));
+    const float s = __builtin_fmaf(bm, inv3, inv5 * __builtin_fmaf(7.5f * inv2, dqd, -1.5f * q1.z));
+    const float t = -3.0f * inv5;
+    W.rx = __builtin_fmaf(dx, s, __builtin_fmaf(t, qx, W.rx));
+    W.ry = __builtin_fmaf(dy, s, __builtin_fmaf(t, qy, W.ry));
+    W.rz = __builtin_fmaf(dz, s, __builtin_fmaf(t, qz, W.rz));
+}
+
 // The lanes in `take` interact with the `cnt` bodies starting at `first`.
 template <bool SOFT>
 __device__ __forceinline__ void leaf_bodies(const float4 *__restrict__ bodies, int first, int cnt, uint64_t take,
@@ -227,14 +257,22 @@ __device__ __forceinline__ void leaf_bodies(const float4 *__restrict__ bodies, i
 
 // Visits one node for the lanes in `mask` (wave-uniform); returns the lanes
 // that open it when it is internal, 0 when it is a leaf or all accept.
-template <bool SOFT, int STATS>
-__device__ __forceinline__ uint64_t visit(const NodeRec &nd, uint64_t mask, const float4 *__restrict__ bodies,
-                                          int64_t n, float eps2, FastWave &W, bool &fail) {
+// qn: the node's two moment records (QUAD only; uniform).
+template <bool SOFT, int STATS, bool QUAD>
+__device__ __forceinline__ uint64_t visit(const NodeRec &nd, const float4 *__restrict__ qn, uint64_t mask,
+                                          const float4 *__restrict__ bodies, int64_t n, float eps2, FastWave &W,
+                                          bool &fail) {
     const uint64_t acc = __ballot(lbvh_accepts(nd.mac, W.px, W.py, W.pz)) & mask;
     if (STATS >= 2) { W.visits += 1; W.lanes += __popcll(mask); }
     if (acc) {
         if (STATS >= 1) W.np += __popcll(acc);
-        if (__builtin_amdgcn_inverse_ballot_w64(acc)) interact<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, eps2, W);
+        if (QUAD) {
+            const float4 q0 = qn[0], q1 = qn[1];
+            if (__builtin_amdgcn_inverse_ballot_w64(acc))
+                interact_quad<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, q0, q1, eps2, W);
+        } else if (__builtin_amdgcn_inverse_ballot_w64(acc)) {
+            interact<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, eps2, W);
+        }
     }
     const uint64_t open = mask & ~acc;
     if (open == 0 || nd.topo.z >= 0) return open;
@@ -283,7 +321,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_forest(WalkArgs a) {
         return;
     }
     const NodeRec root = nodes[T.root];
-    uint64_t mask = visit<SOFT, STATS>(root, vmask, bodies, n, eps2, W, fail);
+    uint64_t mask = visit<SOFT, STATS, false>(root, nullptr, vmask, bodies, n, eps2, W, fail);
     int c = root.topo.z;
     int sp = 0;
     uint32_t iters = 0;
@@ -302,8 +340,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_forest(WalkArgs a) {
         c = uniform_i32(c);  // keeps the node loads scalar
         if (++iters > kMaxWalkIters || (uint32_t)c >= last_pair) { fail = true; break; }
         const NodeRec r0 = nodes[c], r1 = nodes[c + 1];
-        const uint64_t o0 = visit<SOFT, STATS>(r0, mask, bodies, n, eps2, W, fail);
-        const uint64_t o1 = visit<SOFT, STATS>(r1, mask, bodies, n, eps2, W, fail);
+        const uint64_t o0 = visit<SOFT, STATS, false>(r0, nullptr, mask, bodies, n, eps2, W, fail);
+        const uint64_t o1 = visit<SOFT, STATS, false>(r1, nullptr, mask, bodies, n, eps2, W, fail);
         if (fail) break;
         if (o0 != 0) {
             if (o1 != 0) {
@@ -381,15 +419,20 @@ __device__ __forceinline__ void oct_step(float *st, int q, bool act, float vx, f
 // Evaluates node `nd` for the lanes in O: accepting lanes take its monopole
 // and leaf lanes their leaf sum as this node's result r; returns the lanes
 // that open it when it is internal.
-__device__ __forceinline__ uint64_t oct_eval_ordered(const OctRec &nd, uint64_t O, int lane, float px, float py,
-                                                     float pz, float G, float eps2, const float4 *__restrict__ bodies,
-                                                     int64_t n, float &rx, float &ry, float &rz, uint32_t &cpp,
-                                                     uint32_t &cnp, bool &fail) {
+// qn: the node's two moment records (QUAD only).
+template <bool QUAD>
+__device__ __forceinline__ uint64_t oct_eval_ordered(const OctRec &nd, const float4 *__restrict__ qn, uint64_t O,
+                                                     int lane, float px, float py, float pz, float G, float eps2,
+                                                     const float4 *__restrict__ bodies, int64_t n, float &rx,
+                                                     float &ry, float &rz, uint32_t &cpp, uint32_t &cnp, bool &fail) {
     const bool act = (O >> lane) & 1ull;
     const bool pass = act && oct_accepts(nd, px, py, pz);
     if (pass) {
         ++cnp;
-        pp_exact(nd.com.x, nd.com.y, nd.com.z, nd.com.w, px, py, pz, G, eps2, rx, ry, rz);
+        if (QUAD)
+            np_quad_exact(nd.com.x, nd.com.y, nd.com.z, nd.com.w, qn[0], qn[1], px, py, pz, G, eps2, rx, ry, rz);
+        else
+            pp_exact(nd.com.x, nd.com.y, nd.com.z, nd.com.w, px, py, pz, G, eps2, rx, ry, rz);
     }
     const uint64_t open = O & ~__ballot(pass);
     if (open == 0) return 0;
@@ -410,7 +453,7 @@ __device__ __forceinline__ uint64_t oct_eval_ordered(const OctRec &nd, uint64_t 
     return 0;
 }
 
-template <int MAXL>
+template <int MAXL, bool QUAD>
 __global__ __launch_bounds__(64) void k_walk_oct_ordered(WalkArgs a) {
     __shared__ int s_node[MAXL];
     __shared__ int s_pos[MAXL];          // next child index | next octant << 8
@@ -435,8 +478,8 @@ __global__ __launch_bounds__(64) void k_walk_oct_ordered(WalkArgs a) {
     int sp = 0;
     {
         const OctRec root = nodes[0];
-        const uint64_t O = oct_eval_ordered(root, vmask, lane, px, py, pz, G, eps2, bodies, a.n, rx, ry, rz, cpp, cnp,
-                                            fail);
+        const uint64_t O = oct_eval_ordered<QUAD>(root, a.quad, vmask, lane, px, py, pz, G, eps2, bodies, a.n, rx,
+                                                  ry, rz, cpp, cnp, fail);
         if (O) {
             if (lane == 0) { s_node[0] = 0; s_pos[0] = 0; s_mask[0] = O; }
             sp = 1;
@@ -473,8 +516,8 @@ __global__ __launch_bounds__(64) void k_walk_oct_ordered(WalkArgs a) {
         }
         if (lane == 0) s_pos[top] = (j + 1) | (q << 8);
         const OctRec cr = nodes[cb + j];
-        const uint64_t open = oct_eval_ordered(cr, O, lane, px, py, pz, G, eps2, bodies, a.n, rx, ry, rz, cpp, cnp,
-                                               fail);
+        const uint64_t open = oct_eval_ordered<QUAD>(cr, a.quad + 2 * (cb + j), O, lane, px, py, pz, G, eps2, bodies,
+                                                     a.n, rx, ry, rz, cpp, cnp, fail);
         if (open) {
             if (sp >= MAXL) { fail = true; break; }
             if (lane == 0) { s_node[sp] = cb + j; s_pos[sp] = 0; s_mask[sp] = open; }
@@ -497,14 +540,21 @@ __global__ __launch_bounds__(64) void k_walk_oct_ordered(WalkArgs a) {
 // visited in one group; opened internal children are pushed as groups.
 // FAST over the octree: the children of an opened node are contiguous and
 // visited in one group; opened internal children are pushed as groups.
-template <bool SOFT, int STATS>
-__device__ __forceinline__ uint64_t visit_oct(const OctRec &nd, uint64_t mask, const float4 *__restrict__ bodies,
-                                              int64_t n, float eps2, FastWave &W, bool &fail) {
+template <bool SOFT, int STATS, bool QUAD>
+__device__ __forceinline__ uint64_t visit_oct(const OctRec &nd, const float4 *__restrict__ qn, uint64_t mask,
+                                              const float4 *__restrict__ bodies, int64_t n, float eps2, FastWave &W,
+                                              bool &fail) {
     const uint64_t acc = __ballot(oct_accepts(nd, W.px, W.py, W.pz)) & mask;
     if (STATS >= 2) { W.visits += 1; W.lanes += __popcll(mask); }
     if (acc) {
         if (STATS >= 1) W.np += __popcll(acc);
-        if (__builtin_amdgcn_inverse_ballot_w64(acc)) interact<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, eps2, W);
+        if (QUAD) {
+            const float4 q0 = qn[0], q1 = qn[1];
+            if (__builtin_amdgcn_inverse_ballot_w64(acc))
+                interact_quad<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, q0, q1, eps2, W);
+        } else if (__builtin_amdgcn_inverse_ballot_w64(acc)) {
+            interact<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, eps2, W);
+        }
     }
     const uint64_t open = mask & ~acc;
     if (open == 0 || nd.topo.z >= 0) return open;
@@ -518,7 +568,7 @@ __device__ __forceinline__ uint64_t visit_oct(const OctRec &nd, uint64_t mask, c
 
 constexpr int kStackOct = 256;
 
-template <int WAVES, bool SOFT, int STATS>
+template <int WAVES, bool SOFT, int STATS, bool QUAD>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
     __shared__ int s_grp[WAVES][kStackOct];       // first child | nchild << 28
     __shared__ uint64_t s_mask[WAVES][kStackOct];
@@ -543,7 +593,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
     bool fail = false;
 
     const OctRec root = nodes[0];
-    uint64_t mask = visit_oct<SOFT, STATS>(root, vmask, bodies, n, eps2, W, fail);
+    uint64_t mask = visit_oct<SOFT, STATS, QUAD>(root, a.quad, vmask, bodies, n, eps2, W, fail);
     int c = root.topo.z, nch = oct_nchild(root);
     int sp = 0;
     const int64_t max_iters = a.max_nodes + 64;  // each node's group is visited at most once
@@ -568,7 +618,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
         int nc = 0, nn = 0;
         for (int k = 0; k < nch; ++k) {
             const OctRec r = nodes[c + k];
-            const uint64_t o = visit_oct<SOFT, STATS>(r, mask, bodies, n, eps2, W, fail);
+            const uint64_t o = visit_oct<SOFT, STATS, QUAD>(r, a.quad + 2 * (c + k), mask, bodies, n, eps2, W, fail);
             if (o) {
                 if (nm) {
                     if (sp >= kStackOct) { fail = true; break; }
@@ -602,14 +652,20 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
     }
 }
 
-template <int STATS>
-static void launch_oct_fast(const WalkArgs &a, hipStream_t s) {
+template <int STATS, bool QUAD>
+static void launch_oct_fast_q(const WalkArgs &a, hipStream_t s) {
     constexpr int W = 4;
     const unsigned blocks = (unsigned)((a.nt + 64 * W - 1) / (64 * W));
     if (a.eps2 > 0.0f)
-        hipLaunchKernelGGL((k_walk_oct_fast<W, true, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
+        hipLaunchKernelGGL((k_walk_oct_fast<W, true, STATS, QUAD>), dim3(blocks), dim3(64 * W), 0, s, a);
     else
-        hipLaunchKernelGGL((k_walk_oct_fast<W, false, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
+        hipLaunchKernelGGL((k_walk_oct_fast<W, false, STATS, QUAD>), dim3(blocks), dim3(64 * W), 0, s, a);
+}
+
+template <int STATS>
+static void launch_oct_fast(const WalkArgs &a, hipStream_t s) {
+    if (a.quad) launch_oct_fast_q<STATS, true>(a, s);
+    else launch_oct_fast_q<STATS, false>(a, s);
 }
 
 void launch_walk_oct(const WalkArgs &a, int ordered, int stats, hipStream_t s) {
@@ -620,15 +676,18 @@ void launch_walk_oct(const WalkArgs &a, int ordered, int stats, hipStream_t s) {
         else launch_oct_fast<0>(a, s);
     } else {
         const unsigned blocks = (unsigned)((a.nt + 63) / 64);
-        if (a.levels <= 18)
-            hipLaunchKernelGGL((k_walk_oct_ordered<18>), dim3(blocks), dim3(64), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_walk_oct_ordered<kOctLevels>), dim3(blocks), dim3(64), 0, s, a);
+        if (a.quad) {
+            if (a.levels <= 18) hipLaunchKernelGGL((k_walk_oct_ordered<18, true>), dim3(blocks), dim3(64), 0, s, a);
+            else hipLaunchKernelGGL((k_walk_oct_ordered<kOctLevels, true>), dim3(blocks), dim3(64), 0, s, a);
+        } else {
+            if (a.levels <= 18) hipLaunchKernelGGL((k_walk_oct_ordered<18, false>), dim3(blocks), dim3(64), 0, s, a);
+            else hipLaunchKernelGGL((k_walk_oct_ordered<kOctLevels, false>), dim3(blocks), dim3(64), 0, s, a);
+        }
     }
 }
 
 // Single system: wave w walks targets 64 w .. 64 w + 63 from the root.
-template <int WAVES, bool SOFT, int STATS>
+template <int WAVES, bool SOFT, int STATS, bool QUAD>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     __shared__ int s_node[WAVES][kStackFast];
     __shared__ uint64_t s_mask[WAVES][kStackFast];
@@ -658,7 +717,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     // The root is visited on its own; afterwards every step visits the two
     // children (c, c + 1) of a node that the lanes in `mask` opened.
     const NodeRec root = nodes[0];
-    uint64_t mask = visit<SOFT, STATS>(root, vmask, bodies, n, eps2, W, fail);
+    uint64_t mask = visit<SOFT, STATS, QUAD>(root, a.quad, vmask, bodies, n, eps2, W, fail);
     int c = root.topo.z;
     int sp = 0;
     uint32_t iters = 0;
@@ -673,8 +732,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
         c = uniform_i32(c);  // keeps the node loads scalar
         if (++iters > kMaxWalkIters || (uint32_t)c >= last_pair) { fail = true; break; }
         const NodeRec r0 = nodes[c], r1 = nodes[c + 1];
-        const uint64_t o0 = visit<SOFT, STATS>(r0, mask, bodies, n, eps2, W, fail);
-        const uint64_t o1 = visit<SOFT, STATS>(r1, mask, bodies, n, eps2, W, fail);
+        const uint64_t o0 = visit<SOFT, STATS, QUAD>(r0, a.quad + 2 * c, mask, bodies, n, eps2, W, fail);
+        const uint64_t o1 = visit<SOFT, STATS, QUAD>(r1, a.quad + 2 * c + 2, mask, bodies, n, eps2, W, fail);
         if (fail) break;
         if (o0 != 0) {
             if (o1 != 0) {
@@ -712,6 +771,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
 // Waves the walk launches: one per 64 targets, or the forest's aligned waves.
 static int64_t walk_waves(const WalkArgs &a) { return a.nsys > 0 ? a.nwaves : (a.nt + kWave - 1) / kWave; }
 
+template <int STATS, bool QUAD>
+static void launch_fast_single(const WalkArgs &a, unsigned blocks, hipStream_t s) {
+    constexpr int W = 4;
+    if (a.eps2 > 0.0f) hipLaunchKernelGGL((k_walk_lbvh_fast<W, true, STATS, QUAD>), dim3(blocks), dim3(64 * W), 0, s, a);
+    else hipLaunchKernelGGL((k_walk_lbvh_fast<W, false, STATS, QUAD>), dim3(blocks), dim3(64 * W), 0, s, a);
+}
+
 template <int STATS>
 static void launch_fast(const WalkArgs &a, hipStream_t s) {
     constexpr int W = 4;
@@ -720,9 +786,10 @@ static void launch_fast(const WalkArgs &a, hipStream_t s) {
     if (a.nsys > 0) {
         if (soft) hipLaunchKernelGGL((k_walk_lbvh_forest<W, true, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
         else hipLaunchKernelGGL((k_walk_lbvh_forest<W, false, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
+    } else if (a.quad) {
+        launch_fast_single<STATS, true>(a, blocks, s);
     } else {
-        if (soft) hipLaunchKernelGGL((k_walk_lbvh_fast<W, true, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
-        else hipLaunchKernelGGL((k_walk_lbvh_fast<W, false, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
+        launch_fast_single<STATS, false>(a, blocks, s);
     }
 }
 
@@ -735,11 +802,15 @@ void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, int stats, 
     } else {
         const unsigned blocks = (unsigned)walk_waves(a);
         if (a.nsys > 0)
-            hipLaunchKernelGGL((k_walk_lbvh_ordered<32, true>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((k_walk_lbvh_ordered<32, true, false>), dim3(blocks), dim3(64), 0, s, a);
+        else if (a.quad && max_depth <= 32)
+            hipLaunchKernelGGL((k_walk_lbvh_ordered<32, false, true>), dim3(blocks), dim3(64), 0, s, a);
+        else if (a.quad)
+            hipLaunchKernelGGL((k_walk_lbvh_ordered<64, false, true>), dim3(blocks), dim3(64), 0, s, a);
         else if (max_depth <= 32)
-            hipLaunchKernelGGL((k_walk_lbvh_ordered<32, false>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((k_walk_lbvh_ordered<32, false, false>), dim3(blocks), dim3(64), 0, s, a);
         else
-            hipLaunchKernelGGL((k_walk_lbvh_ordered<64, false>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((k_walk_lbvh_ordered<64, false, false>), dim3(blocks), dim3(64), 0, s, a);
     }
 }
 

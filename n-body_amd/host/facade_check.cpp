@@ -36,6 +36,8 @@ int main(int argc, char **argv) {
         solver->sumMode() = NBG_SUM_ORDERED;
         if (auto *bh = dynamic_cast<nbody_gpu::GpuBarnesHutSolver *>(solver.get())) bh->maxLeafSize() = 16;
         if (auto *lb = dynamic_cast<nbody_gpu::GpuLinearBVHSolver *>(solver.get())) lb->maxLeafSize() = 16;
+        if (auto *qb = dynamic_cast<nbody_gpu::GpuQuadrupoleBarnesHutSolver *>(solver.get())) qb->maxLeafSize() = 16;
+        if (auto *ql = dynamic_cast<nbody_gpu::GpuQuadrupoleLinearBVHSolver *>(solver.get())) ql->maxLeafSize() = 16;
         std::FILE *f = std::fopen(argv[3], "wb");
         if (!f) return 2;
         std::fwrite(&n, sizeof n, 1, f);

@@ -216,6 +216,30 @@ public:
     int &mortonBits() { return prm_.morton_bits; }
 };
 
+// QuadrupoleBarnesHutSolver = MultipoleBarnesHutSolver<2>
+// (Solvers/BarnesHutSolver.h:34-66): the octree with second moments.
+class GpuQuadrupoleBarnesHutSolver : public ActiveTreeSolver {
+public:
+    GpuQuadrupoleBarnesHutSolver(Bodies &sim, Gravity &rule, const Device &dev = default_device())
+        : ActiveTreeSolver(sim, rule, NBG_BARNES_HUT_QUADRUPOLE, dev) {}
+    std::string id() override { return "gpu-barnes-hut-4p"; }
+    std::string name() override { return "Barnes-Hut (Quadrupole, MI355X)"; }
+    int &maxDepth() { return prm_.max_depth; }
+    int &maxLeafSize() { return prm_.leaf_size; }
+};
+
+// QuadrupoleLinearBVHSolver = MultipoleLinearBVHSolver<2>
+// (Solvers/LinearBVHSolver.h:30-53): the LBVH with second moments.
+class GpuQuadrupoleLinearBVHSolver : public ActiveTreeSolver {
+public:
+    GpuQuadrupoleLinearBVHSolver(Bodies &sim, Gravity &rule, const Device &dev = default_device())
+        : ActiveTreeSolver(sim, rule, NBG_LINEAR_BVH_QUADRUPOLE, dev) {}
+    std::string id() override { return "gpu-linear-bvh-4p"; }
+    std::string name() override { return "Linear BVH (Quadrupole, MI355X)"; }
+    int &maxLeafSize() { return prm_.leaf_size; }
+    int &mortonBits() { return prm_.morton_bits; }
+};
+
 // NaiveSolver (Solvers/NaiveSolver.h:14-52): direct summation, no tree.
 class GpuNaiveSolver : public Solver {
 public:
@@ -230,6 +254,8 @@ public:
 inline std::unique_ptr<ActiveTreeSolver> select(const std::string &id, Bodies &sim, Gravity &rule) {
     if (id == "gpu-barnes-hut") return std::make_unique<GpuBarnesHutSolver>(sim, rule);
     if (id == "gpu-linear-bvh") return std::make_unique<GpuLinearBVHSolver>(sim, rule);
+    if (id == "gpu-barnes-hut-4p") return std::make_unique<GpuQuadrupoleBarnesHutSolver>(sim, rule);
+    if (id == "gpu-linear-bvh-4p") return std::make_unique<GpuQuadrupoleLinearBVHSolver>(sim, rule);
     throw Error(NBG_ERR_ARG, "unknown solver id " + id);
 }
 

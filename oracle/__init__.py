@@ -50,6 +50,9 @@ def lib() -> ctypes.CDLL:
             "orc_tree_max_depth": (ctypes.c_int, [_vp]),
             "orc_tree_nodes": (ctypes.c_int, [_vp, _i64p, _i64p, _i32p, _i32p, fp, fp]),
             "orc_tree_parents": (ctypes.c_int, [_vp, _i32p]),
+            "orc_tree_quads": (ctypes.c_int, [_vp, fp]),
+            "orc_node_particle_quad": (ctypes.c_int, [fp, ctypes.c_float, fp, fp, ctypes.c_float, ctypes.c_float,
+                                                      fp]),
             "orc_tree_order": (ctypes.c_int, [_vp, _u32p, _u64p]),
             "orc_tree_bbox": (ctypes.c_int, [_vp, fp]),
             "orc_tree_accel": (ctypes.c_int, [_vp, ctypes.c_int64, fp, fp, fp, fp, fp, fp, _u64p, _u64p,
@@ -111,7 +114,8 @@ def sort(keys):
 
 
 class Tree:
-    """A reference tree (octree for solver 0, LBVH for solver 1) over bodies."""
+    """A reference tree over bodies: solver 0 octree, 1 LBVH, 3 / 4 their
+    quadrupole variants (nodes also carry second moments)."""
 
     def __init__(self, solver, x, y, z, m, theta=0.5, leaf=16, max_depth=32, bits=63, G=1.0, eps2=1e-5, dt=1e-3):
         self._arrs = [_f32(a) for a in (x, y, z, m)]
@@ -141,6 +145,13 @@ class Tree:
         lib().orc_tree_parents(self.h, _p(parent, _i32p))
         return dict(first=first, count=count, leaf=leaf.astype(bool), depth=depth, com4=com4, geo6=geo6,
                     parent=parent)
+
+    def quads(self):
+        """Per-node second moments (xx, xy, xz, yy, yz, zz), pre-order."""
+        k = lib().orc_tree_num_nodes(self.h)
+        q = np.zeros((k, 6), np.float32)
+        lib().orc_tree_quads(self.h, _p(q))
+        return q
 
     def max_depth(self):
         return lib().orc_tree_max_depth(self.h)
@@ -202,6 +213,13 @@ def potential_f64(x, y, z, m, G, eps2):
 def particle_particle(a3, m, p3, G, eps2):
     out = np.zeros(3, np.float32)
     lib().orc_particle_particle(_p(_f32(a3)), m, _p(_f32(p3)), G, eps2, _p(out))
+    return out
+
+
+def node_particle_quad(c3, M, q6, p3, G, eps2):
+    """Gravity::nodeParticle<2> for one node (COM c3, mass M, moment q6) and target p3."""
+    out = np.zeros(3, np.float32)
+    lib().orc_node_particle_quad(_p(_f32(c3)), M, _p(_f32(q6)), _p(_f32(p3)), G, eps2, _p(out))
     return out
 
 

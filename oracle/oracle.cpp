@@ -58,6 +58,7 @@ struct Params {
     float G;
     float eps2;       // the reference's Gravity "epsilon" (added to r^2)
     float dt;
+    int order = 1;    // 1: monopole; 2: quadrupole (MultipoleMassSummary<2>)
 };
 
 struct Counts {
@@ -73,6 +74,94 @@ inline V3 particle_particle(V3 a, float m, V3 p, float G, float eps2) {
     float r3 = r * (r * r);
     V3 nR{-R.x, -R.y, -R.z};
     return v_div(v_mul(v_mul(nR, m), G), r3);
+}
+
+// ---------------------------------------------------------------- quadrupole
+// MultipoleMoment<2> (Physics/MultipoleMoment.h:20-55): the raw second
+// moment sum m (x - c)(x - c)^T about the centre of mass c, stored as the
+// 6 entries xx, xy, xz, yy, yz, zz (not made traceless: enforceTraceless is
+// commented out in Summaries/MultipoleMassSummary.h:50,89).
+struct Quad {
+    float q[6] = {0, 0, 0, 0, 0, 0};
+};
+
+inline Quad quad_of(V3 o, float m) {  // outerProduct(o, o) * mass
+    Quad r;
+    r.q[0] = (o.x * o.x) * m; r.q[1] = (o.x * o.y) * m; r.q[2] = (o.x * o.z) * m;
+    r.q[3] = (o.y * o.y) * m; r.q[4] = (o.y * o.z) * m; r.q[5] = (o.z * o.z) * m;
+    return r;
+}
+
+inline Quad quad_add(const Quad &a, const Quad &b) {
+    Quad r;
+    for (int k = 0; k < 6; ++k) r.q[k] = a.q[k] + b.q[k];
+    return r;
+}
+
+// libstdc++ std::transform_reduce grouping over a sequence of terms:
+// init + ((t0 + t1) + (t2 + t3)) per block of four, then the rest one by one.
+template <typename F>
+Quad quad_reduce(int64_t count, F term) {
+    Quad init;
+    int64_t i = 0;
+    for (; count - i >= 4; i += 4)
+        init = quad_add(init, quad_add(quad_add(term(i), term(i + 1)), quad_add(term(i + 2), term(i + 3))));
+    for (; i < count; ++i) init = quad_add(init, term(i));
+    return init;
+}
+
+// Gravity::nodeParticle<2> (Physics/Rules/Gravity.h:238-259) in the
+// reference's FP32 operation order:
+//   R = p - c, r = sqrt(R.R + eps2), dPhi = (R * -M) / (r (r r)),
+//   dPhi += (D<3>(R, r) . Q) / 2, result G * dPhi.
+// D<3> (Gravity.h:43-93) holds the 10 unique entries of the third derivative
+// of 1/r, A + B with A = g3 * R(x)R(x)R (each entry R_a (R_b R_c), a right
+// fold, SymmetricTensor3.h:106-113,471-476) and B = g2 * (R_a on the entries
+// (a,b,b) and (b,b,a) with one odd index a, 3 R_a on (a,a,a), 0 on (x,y,z));
+// f2 = 3 / r^3, f3 = -15 / r^4, g2 = f2 / r^2, g3 = f3 / r^3 with
+// pow<k>(r) = r (r (... r)).  The contraction (SymmetricTensor3.h:266-276)
+// runs over all 27 index triples (i, j, k) in lexicographic order and adds
+// D_ijk Q_jk into component i, starting from zero.
+inline V3 node_particle_quad(V3 c, float M, const Quad &Q, V3 p, float G, float eps2) {
+    if (v_eq(c, p)) return kZero;
+    const V3 R = v_sub(p, c);
+    const float r = std::sqrt(v_len2(R) + eps2);
+    V3 dphi = v_div(v_mul(R, -M), r * (r * r));
+    const float f2 = 3.0f / (r * (r * r)), f3 = -15.0f / (r * (r * (r * r)));
+    const float g2 = f2 / (r * r), g3 = f3 / (r * (r * r));
+    const float Rv[3] = {R.x, R.y, R.z};
+    // unique entries in the reference's storage order: xxx xxy xxz xyy xyz
+    // xzz yyy yyz yzz zzz
+    static const int kDims[10][3] = {{0, 0, 0}, {0, 0, 1}, {0, 0, 2}, {0, 1, 1}, {0, 1, 2},
+                                     {0, 2, 2}, {1, 1, 1}, {1, 1, 2}, {1, 2, 2}, {2, 2, 2}};
+    float D[10];
+    for (int u = 0; u < 10; ++u) {
+        const int a = kDims[u][0], b = kDims[u][1], d = kDims[u][2];
+        const float cp = Rv[a] * (Rv[b] * Rv[d]);
+        float B;
+        if (a == b && b == d) B = 0.0f + Rv[a] * 3.0f;
+        else if (a != b && b != d) B = 0.0f + 0.0f * 3.0f;
+        else B = (a == b ? Rv[d] : Rv[a]) + 0.0f * 3.0f;
+        D[u] = cp * g3 + B * g2;
+    }
+    auto u3 = [](int i, int j, int k) {
+        int t[3] = {i, j, k};
+        std::sort(t, t + 3);
+        for (int u = 0; u < 10; ++u)
+            if (kDims[u][0] == t[0] && kDims[u][1] == t[1] && kDims[u][2] == t[2]) return u;
+        return -1;
+    };
+    auto u2 = [](int j, int k) {
+        if (j > k) std::swap(j, k);
+        static const int kIdx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+        return kIdx[j][k];
+    };
+    float h[3] = {0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) h[i] += D[u3(i, j, k)] * Q.q[u2(j, k)];
+    dphi = v_add(dphi, V3{h[0] / 2.0f, h[1] / 2.0f, h[2] / 2.0f});
+    return V3{G * dphi.x, G * dphi.y, G * dphi.z};
 }
 
 // ---------------------------------------------------------------- bbox
@@ -132,6 +221,7 @@ struct LNode {
     V3 com{0, 0, 0};
     float mass = 0;
     V3 bmin{FLT_MAX, FLT_MAX, FLT_MAX}, bmax{-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    Quad quad;
 };
 
 struct LBVH {
@@ -163,6 +253,11 @@ struct LBVH {
             mx = {std::max(p.x, mx.x), std::max(p.y, mx.y), std::max(p.z, mx.z)};
         }
         nd.com = C; nd.mass = M; nd.bmin = mn; nd.bmax = mx;
+        // MultipoleMassSummary::summarize(entities) (MultipoleMassSummary.h:39-51)
+        if (prm.order == 2)
+            nd.quad = quad_reduce(nd.count, [&](int64_t k) {
+                return quad_of(v_sub(pos[nd.first + k], C), mass[nd.first + k]);
+            });
     }
 
     // CenterOfMassSummary::summarize(childNodes) (CenterOfMassSummary.h:44-55)
@@ -180,6 +275,13 @@ struct LBVH {
             mx = {std::max(c->bmax.x, mx.x), std::max(c->bmax.y, mx.y), std::max(c->bmax.z, mx.z)};
         }
         nd.bmin = mn; nd.bmax = mx;
+        // MultipoleMassSummary::summarize(childNodes) (MultipoleMassSummary.h:54-65)
+        if (prm.order == 2) {
+            const LNode *ch[2] = {&a, &b};
+            nd.quad = quad_reduce(2, [&](int64_t k) {
+                return quad_add(quad_of(v_sub(ch[k]->com, nd.com), ch[k]->mass), ch[k]->quad);
+            });
+        }
     }
 
     // LinearBVH::build split criterion (Trees/LinearBVH.h:114-119),
@@ -254,6 +356,7 @@ struct LBVH {
         if (nd.count == 0) return kZero;
         if (mac(nd, p, theta2)) {
             cnt.np++;
+            if (prm.order == 2) return node_particle_quad(nd.com, nd.mass, nd.quad, p, prm.G, prm.eps2);
             return particle_particle(nd.com, nd.mass, p, prm.G, prm.eps2);
         }
         if (nd.leaf) {
@@ -281,6 +384,7 @@ struct ONode {
     float side = 1.0f;
     V3 com{0, 0, 0};
     float mass = 0;
+    Quad quad;
 };
 
 struct Octree {
@@ -358,6 +462,22 @@ struct Octree {
         nodes[id].com = v_div(C, M);
         nodes[id].mass = M;
         nodes[id].summarized = true;
+        // MultipoleMassSummary (MultipoleMassSummary.h:39-65); an empty child
+        // keeps the default summary (zero COM, mass and moment).
+        if (prm.order == 2) {
+            const V3 Cn = nodes[id].com;
+            const ONode &me = nodes[id];
+            if (me.leaf)
+                nodes[id].quad = quad_reduce(me.count, [&](int64_t k) {
+                    const uint32_t e = idx[me.first + k];
+                    return quad_of(v_sub(P[e], Cn), Mv[e]);
+                });
+            else
+                nodes[id].quad = quad_reduce(8, [&](int64_t k) {
+                    const ONode &ch = nodes[me.child[k]];
+                    return quad_add(quad_of(v_sub(ch.com, Cn), ch.mass), ch.quad);
+                });
+        }
     }
 
     // NodeBase::refine (Trees/NodeBase.h:57-75): empty nodes return without
@@ -424,6 +544,7 @@ struct Octree {
         if (nd.count == 0) return kZero;
         if (mac(nd, p)) {
             cnt.np++;
+            if (prm.order == 2) return node_particle_quad(nd.com, nd.mass, nd.quad, p, prm.G, prm.eps2);
             return particle_particle(nd.com, nd.mass, p, prm.G, prm.eps2);
         }
         if (nd.leaf) {
@@ -454,6 +575,9 @@ struct Handle {
 Params to_params(const float *fp, const int *ip) {
     Params p;
     p.solver = ip[0]; p.leaf_size = ip[1]; p.max_depth = ip[2]; p.morton_bits = ip[3];
+    // 3 / 4: the quadrupole variants of the octree / LBVH solvers
+    // (QuadrupoleBarnesHutSolver, QuadrupoleLinearBVHSolver)
+    if (p.solver == 3 || p.solver == 4) { p.solver -= 3; p.order = 2; }
     p.theta = fp[0]; p.G = fp[1]; p.eps2 = fp[2]; p.dt = fp[3];
     return p;
 }
@@ -511,6 +635,7 @@ void *orc_tree_new(const int *ip, const float *fp, int64_t n, const float *x, co
     Params p = to_params(fp, ip);
     Handle *h = new Handle;
     h->kind = p.solver;
+    if (p.solver != 0 && p.solver != 1) { delete h; return nullptr; }
     if (p.solver == 1) {
         if (p.morton_bits != 30 && p.morton_bits != 63) { delete h; return nullptr; }
         h->l = new LBVH;
@@ -558,6 +683,27 @@ int orc_tree_nodes(void *hp, int64_t *first, int64_t *count, int32_t *leaf, int3
             std::memcpy(geo6 + 6 * i, g, sizeof g);
         }
     }
+    return 0;
+}
+
+// Second moments of every node in pre-order (xx xy xz yy yz zz; zeros for
+// a monopole tree and for empty octree nodes).
+int orc_tree_quads(void *hp, float *q6) {
+    Handle *h = (Handle *)hp;
+    if (h->kind == 1)
+        for (size_t i = 0; i < h->l->nodes.size(); ++i) std::memcpy(q6 + 6 * i, h->l->nodes[i].quad.q, 6 * sizeof(float));
+    else
+        for (size_t i = 0; i < h->o->nodes.size(); ++i) std::memcpy(q6 + 6 * i, h->o->nodes[i].quad.q, 6 * sizeof(float));
+    return 0;
+}
+
+// Gravity::nodeParticle<2> for one node and one target (test pin).
+int orc_node_particle_quad(const float *c3, float M, const float *q6, const float *p3, float G, float eps2,
+                           float *out3) {
+    Quad Q;
+    std::memcpy(Q.q, q6, sizeof Q.q);
+    const V3 r = node_particle_quad(V3{c3[0], c3[1], c3[2]}, M, Q, V3{p3[0], p3[1], p3[2]}, G, eps2);
+    out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
     return 0;
 }
 

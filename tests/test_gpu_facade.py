@@ -26,7 +26,8 @@ def _read(path, n):
     return init, acc, after
 
 
-@pytest.mark.parametrize("sid,solver", [("gpu-barnes-hut", 0), ("gpu-linear-bvh", 1)])
+@pytest.mark.parametrize("sid,solver", [("gpu-barnes-hut", 0), ("gpu-linear-bvh", 1), ("gpu-barnes-hut-4p", 3),
+                                        ("gpu-linear-bvh-4p", 4)])
 def test_facade_matches_oracle(nbg, orc, gpu_ctx, tmp_path, sid, solver):
     n = 2000
     out = tmp_path / "fc.bin"

@@ -356,6 +356,151 @@ def test_interaction_counts_fall_with_theta(orc, nbg):
     assert tot[0] > tot[1] > tot[2]
 
 
+# ------------------------------------------------------------------ quadrupole (solvers 3, 4)
+
+def _moment_f64(pos, m):
+    """FP64 centre of mass and raw second moment sum m (x - c)(x - c)^T."""
+    pos = np.asarray(pos, np.float64)
+    m = np.asarray(m, np.float64)
+    c = (pos * m[:, None]).sum(0) / m.sum()
+    o = pos - c
+    return c, np.einsum("k,ki,kj->ij", m, o, o)
+
+
+def _q6(Q):
+    return np.array([Q[0, 0], Q[0, 1], Q[0, 2], Q[1, 1], Q[1, 2], Q[2, 2]])
+
+
+def test_kat_quadrupole_three_bodies(orc):
+    """src/NBody/Physics/test/multipoleMomentSummary.cpp:12-67: masses 1, 1, 2
+    at (0,0,-1), (0,0,1), (0,3,1), target (0,10,0).  Closed form: COM
+    (0, 1.5, 0.5) and moment yy = 9, yz = 3, zz = 3 (every product exact in
+    FP32); the quadrupole force is nearer the direct sum than the monopole
+    and within the test's absolute bound 0.005."""
+    x = np.array([0, 0, 0], np.float32)
+    y = np.array([0, 0, 3], np.float32)
+    z = np.array([-1, 1, 1], np.float32)
+    m = np.array([1, 1, 2], np.float32)
+    t = orc.Tree(4, x, y, z, m, theta=0.5, leaf=16, bits=30, eps2=1e-5)
+    nd = t.nodes()
+    assert len(nd["first"]) == 1 and nd["leaf"][0]
+    assert np.array_equal(nd["com4"][0], np.array([0, 1.5, 0.5, 4], np.float32))
+    assert np.array_equal(t.quads()[0], np.array([0, 0, 0, 9, 3, 3], np.float32))
+    acc, pp, npc = t.accel([0.0], [10.0], [0.0], counts=True)
+    assert npc[0] == 1 and pp[0] == 0          # diag2 / d2 = 13 / 72.25 < 0.25: the root is accepted
+    direct = orc.direct_f64(x, y, z, m, [0.0], [10.0], [0.0], 1.0, 1e-5)[0]
+    mono = orc.particle_particle([0, 1.5, 0.5], 4.0, [0, 10, 0], 1.0, 1e-5)
+    err_q = np.linalg.norm(acc[0] - direct)
+    err_m = np.linalg.norm(mono - direct)
+    assert err_q < 0.25 * err_m
+    assert err_q <= 0.005                      # the reference test's absolute bound
+
+
+def test_kat_quadrupole_real_scenario(orc):
+    """src/NBody/Physics/test/gravity.cpp:122-153 (its four masses and target):
+    the node's quadrupole approximation beats the monopole and stays within
+    5% of the direct sum."""
+    P = np.array([[12.237058, 7.9972196, 2.7898774], [15.70352, 12.367721, 3.9934757],
+                  [12.150897, 13.606152, 3.4104824], [16.167948, 7.70833, 6.0922756]], np.float32)
+    m = np.array([0.0570347, 0.112188, 0.371711, 0.144094], np.float32)
+    p = np.array([0.41169, 1.1282315, 19.398197], np.float32)
+    c, Q = _moment_f64(P, m)
+    direct = orc.direct_f64(P[:, 0], P[:, 1], P[:, 2], m, [p[0]], [p[1]], [p[2]], 1.0, 1e-5)[0]
+    quad = orc.node_particle_quad(c, float(m.sum()), _q6(Q), p, 1.0, 1e-5)
+    mono = orc.particle_particle(c, float(m.sum()), p, 1.0, 1e-5)
+    eq = np.linalg.norm(quad - direct) / np.linalg.norm(direct)
+    em = np.linalg.norm(mono - direct) / np.linalg.norm(direct)
+    assert eq < em and eq < 0.05
+
+
+def test_quadrupole_isotropic_moment_adds_nothing(orc):
+    """Closed form: D3 contracted with the identity is g3 R r^2 + 5 g2 R,
+    which vanishes at eps2 = 0 (1/r is harmonic), so Q = s I gives the
+    monopole force; a traceless moment does not."""
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        c = rng.normal(0, 1, 3).astype(np.float32)
+        p = (c + rng.normal(0, 4, 3)).astype(np.float32)
+        s = float(rng.uniform(0.1, 2.0))
+        iso = orc.node_particle_quad(c, 3.0, [s, 0, 0, s, 0, s], p, 1.5, 0.0)
+        mono = orc.particle_particle(c, 3.0, p, 1.5, 0.0)
+        assert np.allclose(iso, mono, rtol=2e-5, atol=1e-7 * np.linalg.norm(mono))
+        tl = orc.node_particle_quad(c, 3.0, [s, 0, 0, -s, 0, 0], p, 1.5, 0.0)
+        assert np.linalg.norm(tl - mono) > 1e-4 * np.linalg.norm(mono) * s / np.dot(p - c, p - c)
+
+
+def test_quadrupole_is_second_order_taylor(orc):
+    """The node-particle rule is the second-order expansion of the exact sum
+    about the COM: at eps2 = 0 halving the cluster size h cuts the quadrupole
+    error ~8x (next term h^3), the monopole error ~4x (h^2).  A wrong sign or
+    factor on either contraction term leaves an h^2 error and fails."""
+    rng = np.random.default_rng(8)
+    base = rng.normal(0, 1, (12, 3))
+    m = rng.uniform(0.5, 1.5, 12).astype(np.float32)
+    p = np.array([0.3, -0.2, 1.0]) * 4.0
+    errs_q, errs_m = [], []
+    for h in (0.2, 0.1, 0.05):
+        P = (base * h).astype(np.float32)
+        c, Q = _moment_f64(P, m)
+        direct = orc.direct_f64(P[:, 0], P[:, 1], P[:, 2], m, [p[0]], [p[1]], [p[2]], 1.0, 0.0)[0]
+        q = orc.node_particle_quad(c, float(m.astype(np.float64).sum()), _q6(Q), p, 1.0, 0.0)
+        mo = orc.particle_particle(c, float(m.astype(np.float64).sum()), p, 1.0, 0.0)
+        errs_q.append(np.linalg.norm(q - direct))
+        errs_m.append(np.linalg.norm(mo - direct))
+    for k in range(2):
+        assert 3.0 < errs_m[k] / errs_m[k + 1] < 5.5
+        assert errs_q[k] / errs_q[k + 1] > 6.0
+    assert errs_q[-1] < 0.1 * errs_m[-1]
+
+
+@pytest.mark.parametrize("solver,leaf", [(3, 8), (4, 8), (4, 1)])
+def test_quadrupole_node_moments_vs_fp64(orc, nbg, solver, leaf):
+    """Every node's moment (leaf sums about the leaf COM, internal nodes by
+    the parallel-axis combine of their children) equals the FP64 second
+    moment of the node's bodies about its COM."""
+    b = nbg.plummer(3000, seed=21)
+    t = orc.Tree(solver, b.x, b.y, b.z, b.m, theta=0.5, leaf=leaf)
+    nd = t.nodes()
+    q = t.quads()
+    _, perm = t.order()
+    pos = np.stack([b.x, b.y, b.z], 1)
+    checked = 0
+    for i in range(len(nd["first"])):
+        f, k = int(nd["first"][i]), int(nd["count"][i])
+        if k == 0:
+            assert np.all(q[i] == 0)
+            continue
+        sel = perm[f:f + k]
+        c, Q = _moment_f64(pos[sel], b.m[sel])
+        ref = _q6(Q)
+        # FP32 offsets carry ~1 ulp of the positions (a one-body node's COM
+        # need not round back to the body)
+        ulp = 2.0 ** -23 * float(np.abs(pos[sel]).max())
+        tol = 2e-4 * np.abs(ref).max() + 10.0 * float(b.m[sel].sum()) * ulp * ulp
+        assert np.all(np.abs(q[i] - ref) <= tol), (i, q[i], ref)
+        checked += 1
+    assert checked > 100
+
+
+@pytest.mark.parametrize("mono,quad,theta", [(0, 3, 0.5), (1, 4, 0.5), (1, 4, 0.7)])
+def test_quadrupole_walk_same_decisions_smaller_error(orc, nbg, mono, quad, theta):
+    """The quadrupole solvers take the monopole solvers' opening decisions
+    (same MAC, same tree: equal per-body counts) and cut the median force
+    error against the FP64 direct sum."""
+    n = 20000
+    b = nbg.plummer(n, seed=5)
+    sel = np.arange(0, n, 10)
+    ref = orc.direct_f64(b.x, b.y, b.z, b.m, b.x[sel], b.y[sel], b.z[sel], 1.0, 1e-4)
+    out = {}
+    for s in (mono, quad):
+        t = orc.Tree(s, b.x, b.y, b.z, b.m, theta=theta, leaf=16, eps2=1e-4)
+        acc, pp, npc = t.accel(b.x[sel], b.y[sel], b.z[sel], counts=True)
+        rel = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        out[s] = (np.median(rel), pp, npc)
+    assert np.array_equal(out[mono][1], out[quad][1]) and np.array_equal(out[mono][2], out[quad][2])
+    assert out[quad][0] < 0.5 * out[mono][0]
+
+
 # ------------------------------------------------------------------ integrator, direct sum, energy
 
 def test_integrator_is_kick_then_drift(orc, nbg):
