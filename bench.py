@@ -8,7 +8,10 @@ kick/drift, all on the device with the state resident in HBM.
 
 The headline line is the linear-BVH solver with 30-bit Morton keys over the
 bounding cube; the same run measures the octree Barnes-Hut solver on the
-same bodies and reports it in the "barnes_hut" block.  Each solver carries:
+same bodies and reports it in the "barnes_hut" block, and the quadrupole
+variants of both trees ("linear_bvh_4p", "barnes_hut_4p") at the opening
+angles the reference's benchmark tunes them to (src/Benchmark/profile.cpp:
+47-53: 0.72578126 and 0.41289064).  Each solver carries:
 bodies*steps/s over the timed steps, interactions/s, the walk kernel's FP32
 roofline fraction (HIP events on the system stream), the force error against
 an FP64 direct sum on a 16,384-body sample, the energy drift over the run and
@@ -30,8 +33,12 @@ sys.path.insert(0, ROOT)
 METRIC = "bodies·steps/s at N=1M θ=0.5; batched-serving p50/p99 latency at fixed force err"
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (packed) peak
 FLOPS_PER_INTERACTION = 20
-SOLVER_NAMES = {0: "barnes-hut", 1: "linear-bvh"}
-WALK_KERNELS = {0: "k_walk_oct_fast", 1: "k_walk_lbvh_fast"}
+SOLVER_NAMES = {0: "barnes-hut", 1: "linear-bvh", 3: "barnes-hut-4p", 4: "linear-bvh-4p"}
+WALK_KERNELS = {0: "k_walk_oct_fast", 1: "k_walk_lbvh_fast", 3: "k_walk_oct_fast", 4: "k_walk_lbvh_fast"}
+# FP32 operations of one quadrupole node interaction in the FAST walk
+# (difference, r^2, rsqrt and powers, Q d, d.Q.d, the two coefficients and
+# the 6 accumulating FMAs); monopole node and body interactions count 20.
+FLOPS_PER_QUAD_NODE = 50
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "walk_traffic_plummer1M.txt")
 
 
@@ -42,7 +49,7 @@ def log(msg):
 def load_traffic():
     """HBM bytes per walk launch measured with rocprofv3 --pmc FETCH_SIZE and
     WRITE_SIZE (scaled by a copy kernel of known size), kept as text beside
-    the profiles; keyed by kernel name."""
+    the profiles; keyed by solver name."""
     out = {}
     if not os.path.exists(TRAFFIC_FILE):
         return out
@@ -57,6 +64,10 @@ def load_traffic():
     return out
 
 
+def solver_theta(args, solver):
+    return {3: args.theta_4p_bh, 4: args.theta_4p_lbvh}.get(solver, args.theta)
+
+
 def cpu_baseline(b, args, solver, threads):
     """The oracle's restatement of the reference CPU path for `solver`, timed
     on this host: a full tree build plus the walk for a target sample,
@@ -65,7 +76,8 @@ def cpu_baseline(b, args, solver, threads):
 
     n = b.n
     t0 = time.perf_counter()
-    t = oracle.Tree(solver, b.x, b.y, b.z, b.m, theta=args.theta, leaf=args.leaf, bits=args.bits, eps2=args.eps2)
+    t = oracle.Tree(solver, b.x, b.y, b.z, b.m, theta=solver_theta(args, solver), leaf=args.leaf, bits=args.bits,
+                    eps2=args.eps2)
     t_build = time.perf_counter() - t0
     sample = min(n, args.cpu_sample)
     sel = np.linspace(0, n - 1, sample).astype(np.int64)
@@ -91,7 +103,7 @@ def cpu_baseline(b, args, solver, threads):
 def measure(nbg, ctx, b, args, solver):
     p = nbg.default_params()
     p.solver = solver
-    p.theta, p.leaf_size, p.morton_bits = args.theta, args.leaf, args.bits
+    p.theta, p.leaf_size, p.morton_bits = solver_theta(args, solver), args.leaf, args.bits
     p.eps2, p.dt = args.eps2, args.dt
     p.sum_mode = nbg.SUM_FAST if args.mode == "fast" else nbg.SUM_ORDERED
     s = nbg.System(ctx, p, args.n)
@@ -124,12 +136,15 @@ def measure(nbg, ctx, b, args, solver):
     ms_per_step = 1e3 * elapsed / args.steps
     inter = pp + npc
     walk_ms = phases["walk"]
-    achieved = FLOPS_PER_INTERACTION * inter / (walk_ms * 1e-3) / 1e12
+    quad = solver in (3, 4)
+    flops = FLOPS_PER_INTERACTION * pp + (FLOPS_PER_QUAD_NODE if quad else FLOPS_PER_INTERACTION) * npc
+    achieved = flops / (walk_ms * 1e-3) / 1e12
     waves = max(1, (args.n + 63) // 64)
-    traffic = load_traffic().get(WALK_KERNELS[solver])
+    traffic = load_traffic().get(SOLVER_NAMES[solver])
     out = {
         "value": args.n * args.steps / elapsed,
         "ms_per_step": ms_per_step,
+        "theta": p.theta,
         "interactions_per_step": inter,
         "interactions_per_s": inter / (ms_per_step * 1e-3),
         "phases_ms": phases,
@@ -147,9 +162,11 @@ def measure(nbg, ctx, b, args, solver):
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": traffic,
-            "note": f"{FLOPS_PER_INTERACTION} flops per interaction (pp + np, SimpleTrackingRule counting) "
-                    f"/ mean walk-kernel time from HIP events on the system stream; traffic = HBM bytes per "
-                    f"walk launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated",
+            "note": (f"{FLOPS_PER_INTERACTION} flops per body interaction and "
+                     f"{FLOPS_PER_QUAD_NODE if quad else FLOPS_PER_INTERACTION} per node interaction "
+                     f"(SimpleTrackingRule counting) / mean walk-kernel time from HIP events on the system "
+                     f"stream; traffic = HBM bytes per walk launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                     f"copy-kernel calibrated"),
         },
     }
     if not args.no_accuracy:
@@ -405,8 +422,12 @@ def main():
     ap.add_argument("--dt", type=float, default=5e-4)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--mode", choices=["fast", "ordered"], default="fast")
-    ap.add_argument("--solvers", default="linear-bvh,barnes-hut",
+    ap.add_argument("--solvers", default="linear-bvh,barnes-hut,linear-bvh-4p,barnes-hut-4p",
                     help="comma list; the first is the headline line")
+    ap.add_argument("--theta-4p-lbvh", type=float, default=0.72578126,
+                    help="quadrupole LBVH opening angle (the reference benchmark's tuned value)")
+    ap.add_argument("--theta-4p-bh", type=float, default=0.41289064,
+                    help="quadrupole octree opening angle (the reference benchmark's tuned value)")
     ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")

@@ -1,6 +1,7 @@
 """Scenario files in the reference's JSON format (Simulation.cpp:154-228),
 with the reference's own scenarios/threebody.json and scenarios/test.json as
-fixtures (tests/golden/scenarios/)."""
+fixtures (tests/golden/scenarios/), and its tipsy reader
+(Simulation.cpp:273-395)."""
 
 import importlib
 import json
@@ -48,6 +49,75 @@ def test_passive_rule_and_round_trip(scn):
     assert s.has_position.tolist() == [True, True, True, True, False]
     r = scn.loads(scn.dumps(s))
     for k in ("position", "velocity", "mass", "has_position", "has_velocity", "active", "passive", "acceleration"):
+        assert np.array_equal(getattr(r, k), getattr(s, k)), k
+
+
+# ------------------------------------------------------------------ tipsy
+# No tipsy file ships with the reference (its benchmark reads LOW.bin from a
+# separate scenarios repository), so these images are written here from the
+# record layout Simulation.cpp:273-395 reads.
+
+def _tipsy(gas, dark, stars, big=False, pad=False, ndim=3, t=0.5):
+    o = ">" if big else "<"
+    n = len(gas) + len(dark) + len(stars)
+    head = np.array([t], o + "f8").tobytes() + np.array([n, ndim, len(gas), len(dark), len(stars)], o + "u4").tobytes()
+    if pad:
+        head += b"\0\0\0\0"
+    recs = [np.asarray(a, np.float32).astype(o + "f4").tobytes() for a in (gas, dark, stars) if len(a)]
+    return head + b"".join(recs)
+
+
+def _records(rng, k, words):
+    r = rng.normal(0, 1, (k, words)).astype(np.float32)
+    r[:, 0] = rng.uniform(0.1, 2.0, k)   # masses
+    return r
+
+
+@pytest.mark.parametrize("big,pad", [(False, False), (True, False), (False, True), (True, True)])
+def test_tipsy_stars_loaded_gas_and_dark_skipped(scn, big, pad):
+    """With stars present the gas and the dark matter are skipped; every
+    star is active and passive with its mass, position and velocity."""
+    rng = np.random.default_rng(1)
+    gas, dark, stars = _records(rng, 5, 12), _records(rng, 7, 9), _records(rng, 11, 11)
+    s = scn.from_tipsy(_tipsy(gas, dark, stars, big=big, pad=pad))
+    assert s.n == 11 and s.active.all() and s.passive.all()
+    assert np.array_equal(s.mass, stars[:, 0])
+    assert np.array_equal(s.position, stars[:, 1:4]) and np.array_equal(s.velocity, stars[:, 4:7])
+    assert np.all(s.acceleration == 0)
+    r = s.extra[3]["sphere"]["radius"]
+    assert abs(r - float(stars[3, 0]) ** (1 / 3)) < 1e-6
+    c = s.extra[0]["color"]
+    v = stars[0, 4:7]
+    assert np.allclose([c["r"], c["b"], c["g"]], np.abs(v) / np.linalg.norm(v), rtol=1e-6)
+
+
+def test_tipsy_gas_loaded_without_stars(scn):
+    rng = np.random.default_rng(2)
+    gas, dark = _records(rng, 6, 12), _records(rng, 4, 9)
+    s = scn.from_tipsy(_tipsy(gas, dark, np.zeros((0, 11), np.float32)))
+    assert s.n == 6 and np.array_equal(s.mass, gas[:, 0]) and np.array_equal(s.position, gas[:, 1:4])
+    assert all(e["color"] == {"r": 1.0, "g": 1.0, "b": 1.0} for e in s.extra)
+
+
+def test_tipsy_header_and_size_checks(scn):
+    rng = np.random.default_rng(3)
+    img = _tipsy([], _records(rng, 2, 9), _records(rng, 3, 11), big=True, t=1.25)
+    assert scn.tipsy_header(img) == (1.25, 5, 3, 0, 2, 3, True)
+    with pytest.raises(scn.TipsyError):
+        scn.from_tipsy(img[:-4])          # truncated: size does not match the header
+    with pytest.raises(scn.TipsyError):
+        scn.from_tipsy(_tipsy([], [], _records(rng, 1, 11), ndim=7))
+    empty = scn.from_tipsy(_tipsy([], [], []))
+    assert empty.n == 0
+
+
+def test_tipsy_file_round_trip_to_json(scn, tmp_path):
+    rng = np.random.default_rng(4)
+    p = tmp_path / "x.bin"
+    p.write_bytes(_tipsy([], [], _records(rng, 8, 11), pad=True))
+    s = scn.load_tipsy(str(p))
+    r = scn.loads(scn.dumps(s))
+    for k in ("position", "velocity", "mass", "active", "passive"):
         assert np.array_equal(getattr(r, k), getattr(s, k)), k
 
 
