@@ -519,9 +519,11 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": f"plummer-{args.n} {SOLVER_NAMES[solvers[0]]} step (theta={args.theta}, leaf={args.leaf}, "
+            "workload": f"plummer-{args.n} {SOLVER_NAMES[solvers[0]]} step "
+                        f"(theta={solver_theta(args, solvers[0]):.8g}, leaf={args.leaf}, "
                         f"{args.bits}-bit keys, eps2={args.eps2}, dt={args.dt}, seed={args.seed})",
-            "n_bodies": args.n, "theta": args.theta, "leaf_size": args.leaf, "morton_bits": args.bits,
+            "n_bodies": args.n, "theta": solver_theta(args, solvers[0]), "leaf_size": args.leaf,
+            "morton_bits": args.bits,
             "eps2": args.eps2, "dt": args.dt, "sum_mode": args.mode, "solver": SOLVER_NAMES[solvers[0]],
         },
     }
