@@ -264,20 +264,23 @@ __device__ __forceinline__ uint64_t visit(const NodeRec &nd, const float4 *__res
                                           bool &fail) {
     const uint64_t acc = __ballot(lbvh_accepts(nd.mac, W.px, W.py, W.pz)) & mask;
     if (STATS >= 2) { W.visits += 1; W.lanes += __popcll(mask); }
-    if (acc) {
-        if (STATS >= 1) W.np += __popcll(acc);
-        if (QUAD) {
+    if (STATS >= 1) W.np += __popcll(acc);
+    if (QUAD) {
+        if (acc) {
             const float4 q0 = qn[0], q1 = qn[1];
             if (__builtin_amdgcn_inverse_ballot_w64(acc))
                 interact_quad<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, q0, q1, eps2, W);
-        } else if (__builtin_amdgcn_inverse_ballot_w64(acc)) {
-            interact<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, eps2, W);
         }
+    } else if (__builtin_amdgcn_inverse_ballot_w64(acc)) {
+        interact<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, eps2, W);
     }
     const uint64_t open = mask & ~acc;
-    if (open == 0 || nd.topo.z >= 0) return open;
+    if (nd.topo.z >= 0) return open;
+    if (open == 0) return 0;
     const int first = nd.topo.x, cnt = nd.topo.y;
-    if (first < 0 || cnt < 0 || (uint32_t)(first + cnt) > (uint32_t)n) { fail = true; return 0; }  // n < 2^29: 32-bit, scalar
+    // first in [0, n) and cnt in [0, n - first] (n < 2^29: 32-bit unsigned, two scalar tests)
+    if (__builtin_expect((uint32_t)first >= (uint32_t)n, 0)) { fail = true; return 0; }
+    if (__builtin_expect((uint32_t)cnt > (uint32_t)n - (uint32_t)first, 0)) { fail = true; return 0; }
     if (STATS >= 2) { W.leaf_visits += 1; W.leaf_lanes += __popcll(open); }
     if (STATS >= 1) W.pp += (uint64_t)__popcll(open) * (uint64_t)cnt;
     leaf_bodies<SOFT>(bodies, first, cnt, open, eps2, W);
@@ -338,11 +341,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_forest(WalkArgs a) {
             mask = uniform_u64(s_mask[wv][sp]);
         }
         c = uniform_i32(c);  // keeps the node loads scalar
-        if (++iters > kMaxWalkIters || (uint32_t)c >= last_pair) { fail = true; break; }
+        if (__builtin_expect((uint32_t)c >= last_pair, 0)) { fail = true; break; }
+        if (__builtin_expect(++iters > kMaxWalkIters, 0)) { fail = true; break; }
         const NodeRec r0 = nodes[c], r1 = nodes[c + 1];
         const uint64_t o0 = visit<SOFT, STATS, false>(r0, nullptr, mask, bodies, n, eps2, W, fail);
         const uint64_t o1 = visit<SOFT, STATS, false>(r1, nullptr, mask, bodies, n, eps2, W, fail);
-        if (fail) break;
+        // a failed visit opens nothing, so the walk still ends; fail is
+        // reported after the loop
         if (o0 != 0) {
             if (o1 != 0) {
                 if (sp >= kStackFast) { fail = true; break; }
@@ -546,20 +551,22 @@ __device__ __forceinline__ uint64_t visit_oct(const OctRec &nd, const float4 *__
                                               bool &fail) {
     const uint64_t acc = __ballot(oct_accepts(nd, W.px, W.py, W.pz)) & mask;
     if (STATS >= 2) { W.visits += 1; W.lanes += __popcll(mask); }
-    if (acc) {
-        if (STATS >= 1) W.np += __popcll(acc);
-        if (QUAD) {
+    if (STATS >= 1) W.np += __popcll(acc);
+    if (QUAD) {
+        if (acc) {
             const float4 q0 = qn[0], q1 = qn[1];
             if (__builtin_amdgcn_inverse_ballot_w64(acc))
                 interact_quad<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, q0, q1, eps2, W);
-        } else if (__builtin_amdgcn_inverse_ballot_w64(acc)) {
-            interact<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, eps2, W);
         }
+    } else if (__builtin_amdgcn_inverse_ballot_w64(acc)) {
+        interact<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, eps2, W);
     }
     const uint64_t open = mask & ~acc;
-    if (open == 0 || nd.topo.z >= 0) return open;
+    if (nd.topo.z >= 0) return open;
+    if (open == 0) return 0;
     const int first = nd.topo.x, cnt = nd.topo.y;
-    if (first < 0 || cnt < 0 || (int64_t)first + cnt > n) { fail = true; return 0; }
+    if (__builtin_expect((uint64_t)(uint32_t)first >= (uint64_t)n, 0)) { fail = true; return 0; }
+    if (__builtin_expect((uint64_t)(uint32_t)cnt > (uint64_t)n - (uint32_t)first, 0)) { fail = true; return 0; }
     if (STATS >= 2) { W.leaf_visits += 1; W.leaf_lanes += __popcll(open); }
     if (STATS >= 1) W.pp += (uint64_t)__popcll(open) * (uint64_t)cnt;
     leaf_bodies<SOFT>(bodies, first, cnt, open, eps2, W);
@@ -730,11 +737,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
             mask = uniform_u64(s_mask[wv][sp]);
         }
         c = uniform_i32(c);  // keeps the node loads scalar
-        if (++iters > kMaxWalkIters || (uint32_t)c >= last_pair) { fail = true; break; }
+        if (__builtin_expect((uint32_t)c >= last_pair, 0)) { fail = true; break; }
+        if (__builtin_expect(++iters > kMaxWalkIters, 0)) { fail = true; break; }
         const NodeRec r0 = nodes[c], r1 = nodes[c + 1];
         const uint64_t o0 = visit<SOFT, STATS, QUAD>(r0, a.quad + 2 * c, mask, bodies, n, eps2, W, fail);
         const uint64_t o1 = visit<SOFT, STATS, QUAD>(r1, a.quad + 2 * c + 2, mask, bodies, n, eps2, W, fail);
-        if (fail) break;
+        // a failed visit opens nothing, so the walk still ends; fail is
+        // reported after the loop
         if (o0 != 0) {
             if (o1 != 0) {
                 if (sp >= kStackFast) { fail = true; break; }
