@@ -8,7 +8,11 @@ of the reference's scenarios/threebody.json (read from the reference checkout
 when it is present; its particles' mass, position and velocity are stored in
 the fixture).
 
-    python tests/golden/make_golden.py      # rewrites tests/golden/*.npz
+    python tests/golden/make_golden.py            # rewrites tests/golden/*.npz
+    python tests/golden/make_golden.py NAME ...   # rewrites the named cases only
+
+Solvers 3 and 4 are the quadrupole octree / linear BVH; their fixtures also
+hold every node's second moment (quad6).
 """
 
 import hashlib
@@ -50,6 +54,10 @@ CASES = [
     ("octree_n1000", 0, 1000, 16, 63, 0.5, 1e-4),
     ("lbvh63_threebody", 1, "threebody", 64, 63, 0.3, 1e-5),
     ("octree_threebody", 0, "threebody", 64, 63, 0.3, 1e-5),
+    ("lbvh30q_n64", 4, 64, 4, 30, 0.5, 1e-4),
+    ("lbvh63q_n1000", 4, 1000, 16, 63, 0.72578126, 1e-4),
+    ("octreeq_n1000", 3, 1000, 16, 63, 0.41289064, 1e-4),
+    ("octreeq_threebody", 3, "threebody", 64, 63, 0.3, 1e-5),
 ]
 
 
@@ -69,7 +77,10 @@ def main():
 
     nbg = importlib.import_module("n-body_amd")
     index = {}
+    only = set(sys.argv[1:])
     for name, solver, n, leaf, bits, theta, eps2 in CASES:
+        if only and name not in only:
+            continue
         if n == "threebody" and not os.path.exists(SCENARIO):
             print(f"skipping {name}: {SCENARIO} not present")
             continue
@@ -82,9 +93,10 @@ def main():
         np.savez_compressed(
             path, solver=solver, leaf=leaf, bits=bits, theta=np.float32(theta), eps2=np.float32(eps2),
             x=b.x, y=b.y, z=b.z, m=b.m, vx=b.vx, vy=b.vy, vz=b.vz,
-            bbox=t.bbox(), perm=perm, keys=keys if solver == 1 else np.zeros(0, np.uint64),
+            bbox=t.bbox(), perm=perm, keys=keys if solver in (1, 4) else np.zeros(0, np.uint64),
             first=nd["first"], count=nd["count"], leafflag=nd["leaf"], depth=nd["depth"], parent=nd["parent"],
-            com4=nd["com4"], geo6=nd["geo6"], acc=acc, pp=pp, np_=npc)
+            com4=nd["com4"], geo6=nd["geo6"], acc=acc, pp=pp, np_=npc,
+            **({"quad6": t.quads()} if solver in (3, 4) else {}))
         index[name] = hashlib.sha256(open(path, "rb").read()).hexdigest()
     old = {}
     if os.path.exists(os.path.join(HERE, "SHA256SUMS.json")):

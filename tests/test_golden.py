@@ -1,6 +1,6 @@
 """Golden fixtures (tests/golden/*.npz, written by tests/golden/make_golden.py):
-small Plummer spheres and the reference's three-body scenario, both solvers,
-both key widths.  The CPU tests check the files' hashes and that the oracle
+small Plummer spheres and the reference's three-body scenario, both solvers
+and their quadrupole variants, both key widths.  The CPU tests check the files' hashes and that the oracle
 still reproduces every array; the GPU tests check the device's ORDERED pass
 against the stored arrays without the oracle."""
 
@@ -38,9 +38,13 @@ def test_oracle_reproduces_fixture(orc, name):
     keys, perm = t.order()
     assert np.array_equal(t.bbox(), d["bbox"])
     assert np.array_equal(perm, d["perm"])
-    if solver == 1:
+    if solver in (1, 4):
         assert np.array_equal(keys, d["keys"])
     nd = t.nodes()
+    if "quad6" in d:
+        q = t.quads()
+        assert np.array_equal(np.isnan(q), np.isnan(d["quad6"]))
+        assert np.array_equal(np.nan_to_num(q), np.nan_to_num(d["quad6"]))
     for k, fk in (("first", "first"), ("count", "count"), ("leaf", "leafflag"), ("depth", "depth"),
                   ("parent", "parent"), ("geo6", "geo6")):
         assert np.array_equal(nd[k], d[fk]), k
@@ -66,6 +70,6 @@ def test_device_reproduces_fixture(nbg, gpu_ctx, name):
     assert s.counts() == (int(d["pp"].sum()), int(d["np_"].sum()))
     assert np.array_equal(s.debug_bbox(), d["bbox"])
     keys, perm = s.debug_order()
-    if solver == 1:
+    if solver in (1, 4):
         assert np.array_equal(keys, d["keys"]) and np.array_equal(perm, d["perm"])
     s.close()
