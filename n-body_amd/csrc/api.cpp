@@ -55,7 +55,7 @@ struct nbg_sys {
     int32_t *kfirst = nullptr, *klast = nullptr, *kgamma = nullptr, *kparent = nullptr;
     uint32_t *ksplit = nullptr, *krank = nullptr;
     int32_t *leaf_start = nullptr;  // leaf slot by first body (n)
-    uint32_t *glist = nullptr;      // straddling parents queued by the summaries (2n + 64)
+    uint32_t *glist = nullptr;      // straddling parents queued by the summaries (2 queues of 2n + 64)
     uint32_t *scan_ws = nullptr;
     uint32_t *sort_status = nullptr;
     // tree (capacity 2n nodes)
@@ -121,7 +121,7 @@ int ensure_octree(nbg_sys *s) {
     A(&b.key_hi, n); A(&b.key_lo, n); A(&b.lo_sorted, n);
     A(&b.nown, n); A(&b.nbase, n);
     A(&b.nfirst, cap); A(&b.ncount, cap); A(&b.nparent, cap); A(&b.nmeta, cap);
-    A(&b.occ, cap); A(&b.cb, cap); A(&b.flags, cap); A(&b.glist, cap + 64);
+    A(&b.occ, cap); A(&b.cb, cap); A(&b.flags, cap); A(&b.glist, 2 * (cap + 64));
     A(&b.scan_ws, 128 + (cap + kScanTile - 1) / kScanTile);
     A(&b.nodes, cap); A(&b.centre, cap); A(&b.thr, 64); A(&b.misc, 64);
     if (r != NBG_OK) {
@@ -392,7 +392,7 @@ int nbg_sys_create(nbg_ctx *ctx, const nbg_params *p, int64_t n, nbg_sys **out) 
     A(&s->bodies, n);
     A(&s->kfirst, n); A(&s->klast, n); A(&s->kgamma, n); A(&s->kparent, n);
     A(&s->ksplit, n); A(&s->krank, n);
-    A(&s->leaf_start, n); A(&s->glist, 2 * n + 64);
+    A(&s->leaf_start, n); A(&s->glist, 2 * (2 * n + 64));
     A(&s->scan_ws, 64 + (n + kScanTile - 1) / kScanTile + 64);
     A(&s->sort_status, sort_status_words(n, 64));
     A(&s->nodes, 2 * n); A(&s->bmin, 2 * n); A(&s->bmax, 2 * n);

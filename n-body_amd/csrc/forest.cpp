@@ -96,7 +96,7 @@ int64_t bytes_for(int64_t cap_n, int64_t cap_s) {
     b += 4 * n;                                               // bsys
     b += 2 * 8 * n + 2 * 4 * n + 16 * n;                      // keys, idx, bodies
     b += 6 * 4 * n + 4 * n;                                   // Karras scratch, leaf_start
-    b += 4 * (2 * n + 64) + 4 * (128 + n / kScanTile);        // glist, scan_ws
+    b += 4 * 2 * (2 * n + 64) + 4 * (128 + n / kScanTile);    // glist (two queues), scan_ws
     b += 4 * sort_status_words(n, 64);                        // sort status
     b += (int64_t)sizeof(NodeRec) * 2 * n + 2 * 16 * 2 * n + 4 * 2 * n;  // nodes, bmin/bmax, flags
     b += 3 * 8 * (s + 1) + 2 * 4 * s + 16 * s + (int64_t)sizeof(MacParam) * s + 24 * s + 4 * 32;
@@ -282,7 +282,7 @@ int nbg_batch_create(nbg_ctx *ctx, int64_t max_bodies, int32_t max_systems, int3
     A(&b->bsys, n);
     A(&b->keys, n); A(&b->keys_alt, n); A(&b->idx, n); A(&b->idx_alt, n); A(&b->bodies, n);
     A(&b->kfirst, n); A(&b->klast, n); A(&b->kgamma, n); A(&b->kparent, n); A(&b->ksplit, n); A(&b->krank, n);
-    A(&b->leaf_start, n); A(&b->glist, 2 * n + 64); A(&b->scan_ws, 128 + n / kScanTile);
+    A(&b->leaf_start, n); A(&b->glist, 2 * (2 * n + 64)); A(&b->scan_ws, 128 + n / kScanTile);
     A(&b->sort_status, sort_status_words(n, 64));
     A(&b->nodes, 2 * n); A(&b->bmin, 2 * n); A(&b->bmax, 2 * n); A(&b->flags, 2 * n);
     A(&b->sys_first, S + 1); A(&b->wave_first, S + 1); A(&b->old_first, S + 1);

@@ -87,7 +87,7 @@ struct OctBuffers {
     uint32_t *occ = nullptr;       // cap: occupied-child octant mask
     uint32_t *cb = nullptr;        // cap: child counts, then their exclusive scan
     uint32_t *flags = nullptr;     // cap: summary arrival counters
-    uint32_t *glist = nullptr;     // cap + 64: straddling parents queued by the summaries
+    uint32_t *glist = nullptr;     // 2 * (cap + 64): two queues of straddling parents (summaries)
     uint32_t *scan_ws = nullptr;   // 128 + cap / kScanTile
     OctRec *nodes = nullptr;       // cap, by slot
     float4 *centre = nullptr;      // cap, by slot: cell centre xyz, side
@@ -157,8 +157,10 @@ void launch_lbvh_topology(const void *keys, int bits, int64_t n, int leaf_size, 
 // Bottom-up LBVH summaries with atomic arrival flags: COM, mass, bbox and the
 // MAC record (bbox centre, squared diagonal).  leaf_start[i] holds the leaf
 // slot whose first body is i (else -1; written by the topology pass);
-// flags holds max_nodes words; glist holds 2n + 64 words.
+// flags holds max_nodes words; glist holds 2 * (2n + 64) words (two queues).
 constexpr int kSummBodies = 4096;   // bodies per workgroup in the local pass
+constexpr int kSummMidBodies = kSummBodies * 64;  // bodies per workgroup in the middle pass
+constexpr unsigned kSummMidMinBlocks = 8;         // fewer middle workgroups: no middle pass
 constexpr int kSummThreads = 1024;  // threads per workgroup in the local pass (latency hiding)
 struct MacParam {       // mac_threshold's output for one theta
     double mid;

@@ -482,6 +482,39 @@ __global__ __launch_bounds__(kSummThreads) void k_oct_summ_local(const float4 *_
     }
 }
 
+// Middle pass (as the linear BVH's): workgroup g owns kSummMidBodies bodies
+// and the queued parents whose first body lies there, climbs while the node
+// stays inside, and forwards a parent that leaves the range in place of the
+// arrival.
+__global__ __launch_bounds__(1024) void k_oct_summ_mid(OctRec *nodes, const uint32_t *__restrict__ misc,
+                                                       uint32_t *flags, const uint32_t *gin, uint32_t *gout,
+                                                       int64_t qcap, float4 *quad, uint32_t *err) {
+    const int64_t T = misc[0];
+    const int64_t lo = (int64_t)blockIdx.x * kSummMidBodies, hi = lo + kSummMidBodies;
+    const uint32_t cnt = gin[0] < qcap ? gin[0] : (uint32_t)qcap;
+    for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
+        int parent = (int)gin[64 + j];
+        if (parent < 0 || parent >= T) { atomicOr(err, 4u); continue; }
+        const int first = nodes[parent].topo.x;
+        if (first < lo || first >= hi) continue;  // another workgroup's entry
+        for (int hops = 0; parent >= 0; ++hops) {
+            if (parent >= T || hops > kOctLevels) { atomicOr(err, 4u); break; }
+            const OctRec r = nodes[parent];
+            const int4 tp = r.topo;
+            if (!(tp.x >= lo && (int64_t)tp.x + tp.y <= hi)) {
+                const uint32_t k = atomicAdd(&gout[0], 1u);
+                if (k < qcap) gout[64 + k] = (uint32_t)parent;
+                else atomicOr(err, 4u);
+                break;
+            }
+            if (arrive(&flags[parent]) + 1u < (uint32_t)oct_nchild(r)) break;
+            if (tp.z <= parent || tp.z + oct_nchild(r) > T || tp.w >= parent) { atomicOr(err, 4u); break; }
+            oct_summ_inner(parent, r, nodes, quad);
+            parent = tp.w;
+        }
+    }
+}
+
 __global__ __launch_bounds__(1024) void k_oct_summ_global(OctRec *nodes, const uint32_t *__restrict__ misc,
                                                           uint32_t *flags, const uint32_t *glist,
                                                           int64_t glist_cap, float4 *quad, uint32_t *err) {
@@ -578,12 +611,20 @@ void launch_octree_topology(int64_t n, const float *bbox6, int leaf_size, int ma
 void launch_octree_summaries(const float4 *bodies, int64_t n, OctBuffers &ob, const int32_t *leaf_start,
                              uint32_t *err, hipStream_t s, float4 *quad) {
     if (n <= 0) return;
+    // queue layout: local -> middle at glist, middle -> global at glist + cap + 64
+    uint32_t *gmid = ob.glist + ob.cap + 64;
     launch_fill_u32(ob.flags, 0u, ob.cap, s);
     launch_fill_u32(ob.glist, 0u, 64, s);
+    launch_fill_u32(gmid, 0u, 64, s);
     hipLaunchKernelGGL(k_oct_summ_local, dim3(blocks_for(n, kSummBodies)), dim3(kSummThreads), 0, s, bodies, n, ob.nodes,
                        leaf_start, ob.misc, ob.flags, ob.glist, ob.cap, quad, err);
-    hipLaunchKernelGGL(k_oct_summ_global, dim3(1), dim3(1024), 0, s, ob.nodes, ob.misc, ob.flags, ob.glist, ob.cap,
-                       quad, err);
+    const unsigned mblocks = blocks_for(n, kSummMidBodies);
+    const bool mid = mblocks >= kSummMidMinBlocks;  // see launch_lbvh_summaries
+    if (mid)
+        hipLaunchKernelGGL(k_oct_summ_mid, dim3(mblocks), dim3(1024), 0, s, ob.nodes, ob.misc, ob.flags, ob.glist, gmid,
+                           ob.cap, quad, err);
+    hipLaunchKernelGGL(k_oct_summ_global, dim3(1), dim3(1024), 0, s, ob.nodes, ob.misc, ob.flags,
+                       mid ? gmid : ob.glist, ob.cap, quad, err);
 }
 
 }  // namespace nbg

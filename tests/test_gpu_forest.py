@@ -152,3 +152,24 @@ def test_forest_capacity_and_arguments(nbg, gpu_ctx):
     with pytest.raises(nbg.NbgError):
         B.add(_params(nbg), nbg.plummer(10, seed=4))  # 3 > 2 systems
     B.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_forest_large_equals_alone(nbg, gpu_ctx, mode):
+    """A forest above 2M bodies (the summaries then run their middle,
+    many-workgroup pass, whose ranges cut through systems): every system
+    still gets its bits alone."""
+    sizes = [300000, 70001, 450000, 129, 500000, 260000, 333333, 70000, 65, 300000]
+    systems = [(nbg.plummer(n, seed=900 + k, a=0.7 + 0.2 * k), dict(theta=0.5, leaf=16, eps2=1e-4, G=1.0, dt=1e-3))
+               for k, n in enumerate(sizes)]
+    total = sum(b.n for b, _ in systems)
+    assert total > 8 * 4096 * 64
+    B = nbg.Batch(gpu_ctx, total, len(systems), sum_mode=mode)
+    ids = [B.add(_params(nbg, mode=mode, **kw), b) for b, kw in systems]
+    B.update_accelerations()
+    for sid, (b, kw) in zip(ids, systems):
+        print(f"[forest large] mode={mode} n={b.n}", flush=True)
+        _, acc = B.get(sid, accelerations=True)
+        a1, _, _ = _alone(nbg, gpu_ctx, b, kw, mode)
+        assert np.array_equal(acc.view(np.uint32), a1.view(np.uint32))
+    B.close()

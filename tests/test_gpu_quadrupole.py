@@ -226,3 +226,20 @@ def test_quad_solver_switch(nbg, orc, gpu_ctx):
         acc = s.update_accelerations()
         t = orc.Tree(solver, b.x, b.y, b.z, b.m, theta=0.5, leaf=16, eps2=1e-4)
         assert np.array_equal(_bits(acc), _bits(t.accel(b.x, b.y, b.z))), f"solver {solver}"
+
+
+@pytest.mark.parametrize("solver", [1, 0, 4, 3])
+def test_large_system_summaries_bitexact(nbg, orc, gpu_ctx, solver):
+    """Above ~1.8M bodies the summaries add a many-workgroup middle pass;
+    ORDERED accelerations of a sample stay the oracle's bits (every node
+    summary, moment and MAC record on the sample's paths included)."""
+    n = 2200000
+    b = nbg.plummer(n, seed=77)
+    print(f"[large summaries] solver={solver} n={n}", flush=True)
+    theta = {1: 0.5, 0: 0.5, 4: 0.72578126, 3: 0.41289064}[solver]
+    s, acc = _gpu_pass(nbg, gpu_ctx, b, solver, leaf=16, theta=theta)
+    t = orc.Tree(solver, b.x, b.y, b.z, b.m, theta=theta, leaf=16, eps2=1e-4)
+    sel = np.random.default_rng(5).choice(n, 3000, replace=False)
+    ref = t.accel(b.x[sel], b.y[sel], b.z[sel])
+    assert np.array_equal(_bits(acc[sel]), _bits(ref))
+    s.close()
