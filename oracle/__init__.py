@@ -51,6 +51,9 @@ def lib() -> ctypes.CDLL:
             "orc_tree_nodes": (ctypes.c_int, [_vp, _i64p, _i64p, _i32p, _i32p, fp, fp]),
             "orc_tree_parents": (ctypes.c_int, [_vp, _i32p]),
             "orc_tree_quads": (ctypes.c_int, [_vp, fp]),
+            "orc_fmm_accel": (ctypes.c_int, [_vp, fp, fp, fp, _u64p, _f64p]),
+            "orc_fmm_starts": (ctypes.c_int64, [_vp, _i32p, ctypes.c_int64]),
+            "orc_fmm_p2l": (ctypes.c_int, [fp, ctypes.c_float, fp, fp, ctypes.c_float, ctypes.c_float, fp]),
             "orc_node_particle_quad": (ctypes.c_int, [fp, ctypes.c_float, fp, fp, ctypes.c_float, ctypes.c_float,
                                                       fp]),
             "orc_tree_order": (ctypes.c_int, [_vp, _u32p, _u64p]),
@@ -115,7 +118,8 @@ def sort(keys):
 
 class Tree:
     """A reference tree over bodies: solver 0 octree, 1 LBVH, 3 / 4 their
-    quadrupole variants (nodes also carry second moments)."""
+    quadrupole variants (nodes also carry second moments), 5 the octree of
+    the implicit FMM (fmm_accel)."""
 
     def __init__(self, solver, x, y, z, m, theta=0.5, leaf=16, max_depth=32, bits=63, G=1.0, eps2=1e-5, dt=1e-3):
         self._arrs = [_f32(a) for a in (x, y, z, m)]
@@ -152,6 +156,31 @@ class Tree:
         q = np.zeros((k, 6), np.float32)
         lib().orc_tree_quads(self.h, _p(q))
         return q
+
+    def fmm_accel(self, counts=False):
+        """Implicit FMM accelerations of every body (original order) and the
+        counts (pp, np, pn, nn)."""
+        a = [np.zeros(self.n, np.float32) for _ in range(3)]
+        c = np.zeros(4, np.uint64)
+        rc = lib().orc_fmm_accel(self.h, *[_p(v) for v in a], _p(c, _u64p), None)
+        if rc != 0:
+            raise ValueError("not an FMM tree")
+        acc = np.stack(a, axis=1)
+        return (acc, tuple(int(v) for v in c)) if counts else acc
+
+    def fmm_mass_received(self):
+        """Per body, the FP64 sum of the masses of every source the implicit
+        FMM's traversal reaches it with (each source once: the total mass)."""
+        out = np.zeros(self.n, np.float64)
+        c = np.zeros(4, np.uint64)
+        if lib().orc_fmm_accel(self.h, None, None, None, _p(c, _u64p), _p(out, _f64p)) != 0:
+            raise ValueError("not an FMM tree")
+        return out
+
+    def fmm_starts(self):
+        out = np.zeros(512, np.int32)
+        k = lib().orc_fmm_starts(self.h, _p(out, _i32p), 512)
+        return out[:k]
 
     def max_depth(self):
         return lib().orc_tree_max_depth(self.h)
@@ -220,6 +249,13 @@ def node_particle_quad(c3, M, q6, p3, G, eps2):
     """Gravity::nodeParticle<2> for one node (COM c3, mass M, moment q6) and target p3."""
     out = np.zeros(3, np.float32)
     lib().orc_node_particle_quad(_p(_f32(c3)), M, _p(_f32(q6)), _p(_f32(p3)), G, eps2, _p(out))
+    return out
+
+
+def fmm_p2l(s3, m, c3, p3, G, eps2):
+    """One source's local field at c3 evaluated at p3 (implicit FMM expansion)."""
+    out = np.zeros(3, np.float32)
+    lib().orc_fmm_p2l(_p(_f32(s3)), m, _p(_f32(c3)), _p(_f32(p3)), G, eps2, _p(out))
     return out
 
 

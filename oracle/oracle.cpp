@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <deque>
 #include <numeric>
 #include <vector>
 
@@ -565,8 +566,273 @@ struct Octree {
     }
 };
 
+// ---------------------------------------------------------------- implicit FMM
+// MultipoleImplicitFMMSolver<1> (Solvers/ImplicitFMMSolver.h:14-33): an
+// ImplicitDualTraversalSolver (Solvers/ImplicitDualTraversalSolver.h:17-70)
+// over one octree whose nodes carry a centre of mass (active side) and imply
+// a MultipoleAccelerationSummary<2> local field (passive side:
+// Summaries/EmptySummary.h:37-60), with SideLengthOverDistance at theta
+// (default 0.4).
+
+// MultipoleAcceleration<2> (Physics/MultipoleAcceleration.h:14-90): a vector
+// and the symmetric 3x3 tensor of its gradient (xx xy xz yy yz zz).  `mass`
+// is used only by the accounting mode (mass received per target).
+struct Local {
+    V3 a = kZero;
+    float T[6] = {0, 0, 0, 0, 0, 0};
+    double mass = 0.0;
+};
+
+// SymmetricTensor3<2> * vec3 (Physics/SymmetricTensor3.h:235-247).
+inline V3 t_dot(const float *T, V3 o) {
+    return V3{(T[0] * o.x + T[1] * o.y) + T[2] * o.z, (T[1] * o.x + T[3] * o.y) + T[4] * o.z,
+              (T[2] * o.x + T[4] * o.y) + T[5] * o.z};
+}
+// MultipoleAcceleration::at and ::translated (MultipoleAcceleration.h:33-70).
+inline V3 local_at(const Local &L, V3 o) { return v_add(L.a, t_dot(L.T, o)); }
+inline Local local_translated(const Local &L, V3 o) {
+    Local t = L;
+    t.a = v_add(t.a, t_dot(L.T, o));
+    return t;
+}
+
+struct FMMCounts {
+    uint64_t pp = 0, np = 0, pn = 0, nn = 0;  // SimpleTrackingRule's four counters (empty sources excluded)
+};
+
+struct FMM {
+    const Octree &T;
+    float theta;
+    bool account;
+    std::vector<V3> acc;       // by original body index
+    std::vector<double> recv;   // accounting mode: mass received by each body
+    std::vector<double> nmass;  // accounting mode: FP64 mass of every node's bodies
+    FMMCounts cnt;
+
+    FMM(const Octree &t, bool acc_mode) : T(t), theta(t.prm.theta), account(acc_mode) {
+        acc.assign(T.n, kZero);
+        recv.assign(T.n, 0.0);
+        if (account) {
+            nmass.assign(T.nodes.size(), 0.0);
+            for (size_t i = 0; i < T.nodes.size(); ++i)
+                for (int64_t j = T.nodes[i].first; j < T.nodes[i].first + T.nodes[i].count; ++j)
+                    nmass[i] += (double)T.Mv[T.idx[j]];
+        }
+    }
+
+    const ONode &nd(int32_t i) const { return T.nodes[i]; }
+    V3 pos(uint32_t e) const { return T.P[e]; }
+
+    // SideLengthOverDistance, node-node (SideLengthOverDistance.h:33-56):
+    // 0 = Approximate, else DescendActiveNode (1) | DescendPassiveNode (2).
+    // Exclusion regions are [c - side, c + side] (DescentCriterionType.h:81-85),
+    // closed-interval overlap (BoundingBox.h:77-82).
+    int recommend(int32_t a, int32_t p) const {
+        const ONode &A = nd(a), &Pn = nd(p);
+        const float sa2 = A.side * A.side, sp2 = Pn.side * Pn.side;
+        const float d2 = v_len2(v_sub(Pn.center, A.com));
+        const float big = sa2 < sp2 ? sp2 : sa2;
+        if (((4.0f * big) / d2) < (theta * theta)) {
+            const V3 pl{Pn.center.x - Pn.side, Pn.center.y - Pn.side, Pn.center.z - Pn.side};
+            const V3 ph{Pn.center.x + Pn.side, Pn.center.y + Pn.side, Pn.center.z + Pn.side};
+            const V3 al{A.center.x - A.side, A.center.y - A.side, A.center.z - A.side};
+            const V3 ah{A.center.x + A.side, A.center.y + A.side, A.center.z + A.side};
+            const bool meet = std::min(ph.x, ah.x) >= std::max(pl.x, al.x) && std::min(ph.y, ah.y) >= std::max(pl.y, al.y) &&
+                              std::min(ph.z, ah.z) >= std::max(pl.z, al.z);
+            if (!meet) return 0;
+        }
+        int r = (sa2 <= sp2 * 4.0f) ? 2 : 0;
+        r |= (sp2 <= sa2 * 4.0f) ? 1 : 0;
+        return r;
+    }
+
+    // SideLengthOverDistance, particle-node (SideLengthOverDistance.h:25-29):
+    // 2 side / |x - c| < theta and x outside the cell [c - side/2, c + side/2).
+    bool particle_far(V3 x, int32_t p) const {
+        const ONode &Pn = nd(p);
+        const float dist = std::sqrt(v_len2(v_sub(x, Pn.center)));
+        if (!(((2.0f * Pn.side) / dist) < theta)) return false;
+        const float h = Pn.side / 2.0f;
+        const bool in = Pn.center.x - h <= x.x && Pn.center.y - h <= x.y && Pn.center.z - h <= x.z &&
+                        x.x < Pn.center.x + h && x.y < Pn.center.y + h && x.z < Pn.center.z + h;
+        return !in;
+    }
+
+    // Gravity::particleNode into a MultipoleAccelerationSummary<2>
+    // (Physics/Rules/Gravity.h:281-304): R = c - s, r = sqrt(R.R + eps2),
+    // a -= (R / r^3) m G, T += D<2>(R, r) m G with D<2> = g2 R(x)R + g1 I,
+    // g1 = (-1 / r^2) / r, g2 = (3 / r^3) / r^2 (Gravity.h:43-58).
+    void to_local(V3 s, float m, V3 c, Local &L, double exact_mass) const {
+        if (account) { L.mass += exact_mass; return; }
+        if (v_eq(s, c)) return;
+        const float G = T.prm.G;
+        const V3 R = v_sub(c, s);
+        const float r = std::sqrt(v_len2(R) + T.prm.eps2);
+        const float r3 = r * (r * r);
+        L.a = v_sub(L.a, v_mul(v_mul(v_div(R, r3), m), G));
+        const float g1 = (-1.0f / (r * r)) / r, g2 = (3.0f / (r * (r * r))) / (r * r);
+        const float Rv[3] = {R.x, R.y, R.z};
+        static const int kI[6] = {0, 0, 0, 1, 1, 2}, kJ[6] = {0, 1, 2, 1, 2, 2};
+        for (int k = 0; k < 6; ++k) {
+            const float D = Rv[kI[k]] * Rv[kJ[k]] * g2 + (kI[k] == kJ[k] ? 1.0f : 0.0f) * g1;
+            L.T[k] = L.T[k] + (D * m) * G;
+        }
+    }
+
+    // Descent::activeTree from node a for body q (counts as SimpleTrackingRule).
+    void walk_into(int32_t a, uint32_t q) {
+        if (account) { recv[q] += walk_mass(a, pos(q)); return; }
+        Counts c;
+        acc[q] = v_add(acc[q], T.walk(a, pos(q), c));
+        cnt.pp += c.pp;
+        cnt.np += c.np;
+    }
+    double walk_mass(int32_t id, V3 p) {
+        const ONode &n = nd(id);
+        if (n.count == 0) return 0.0;
+        if (T.mac(n, p)) { cnt.np++; return nmass[id]; }
+        if (n.leaf) {
+            double s = 0.0;
+            for (int64_t i = n.first; i < n.first + n.count; ++i) { cnt.pp++; s += T.Mv[T.idx[i]]; }
+            return s;
+        }
+        double s = 0.0;
+        for (int k = 0; k < 8; ++k) s += walk_mass(n.child[k], p);
+        return s;
+    }
+    void p2p(uint32_t e, uint32_t q) {
+        cnt.pp++;
+        if (account) { recv[q] += T.Mv[e]; return; }
+        acc[q] = v_add(acc[q], particle_particle(pos(e), T.Mv[e], pos(q), T.prm.G, T.prm.eps2));
+    }
+    void apply_local(const Local &L, uint32_t q, V3 c) {
+        if (account) { recv[q] += L.mass; return; }
+        acc[q] = v_add(acc[q], local_at(L, v_sub(pos(q), c)));
+    }
+
+    // Descent::passiveTreeImplicitField (Descent/passiveTreeImplicitField.h:13-78)
+    // over the entities E[b, e) (reordered in place by std::partition, as the
+    // reference's span is).
+    void passive_field(std::vector<uint32_t> &E, int64_t b, int64_t e, int32_t p, Local L) {
+        const ONode &Pn = nd(p);
+        if (Pn.count == 0) return;
+        auto mid = std::partition(E.begin() + b, E.begin() + e, [&](uint32_t x) { return particle_far(pos(x), p); });
+        const int64_t m = mid - E.begin();
+        for (int64_t i = b; i < m; ++i) {
+            cnt.pn++;
+            to_local(pos(E[i]), T.Mv[E[i]], Pn.center, L, (double)T.Mv[E[i]]);
+        }
+        if (Pn.leaf) {
+            for (int64_t i = Pn.first; i < Pn.first + Pn.count; ++i) {
+                const uint32_t q = T.idx[i];
+                for (int64_t j = m; j < e; ++j) p2p(E[j], q);
+                apply_local(L, q, Pn.center);
+            }
+        } else {
+            for (int k = 0; k < 8; ++k) {
+                const int32_t c = Pn.child[k];
+                passive_field(E, m, e, c, local_translated(L, v_sub(nd(c).center, Pn.center)));
+            }
+        }
+    }
+
+    // Descent::lockstepDualTreeImplicitField (Descent/lockstepDualTreeImplicitField.h:18-170)
+    // over the active nodes A[b, e) (reordered in place, as the reference's span is).
+    void lockstep(std::vector<int32_t> &A, int64_t b, int64_t e, int32_t p, Local L) {
+        const ONode &Pn = nd(p);
+        if (Pn.count == 0) return;
+        if (b == e) {
+            for (int64_t i = Pn.first; i < Pn.first + Pn.count; ++i) apply_local(L, T.idx[i], Pn.center);
+            return;
+        }
+        // partitionByDescentCriterionLockstep (:22-58): far | leaf | non-leaf
+        int64_t end_far = b, begin_nonleaf = e;
+        for (int64_t i = b; i < begin_nonleaf; ++i) {
+            const int r = recommend(A[i], p);
+            if (r == 0) {
+                std::swap(A[i], A[end_far]);
+                ++end_far;
+            } else if (!nd(A[i]).leaf) {
+                --begin_nonleaf;
+                std::swap(A[i], A[begin_nonleaf]);
+                --i;
+            }
+        }
+        for (int64_t i = b; i < end_far; ++i) {
+            const ONode &a = nd(A[i]);
+            if (a.count > 0) cnt.nn++;
+            to_local(a.com, a.mass, Pn.center, L, account ? nmass[A[i]] : 0.0);
+        }
+        if (Pn.leaf) {
+            for (int64_t i = end_far; i < begin_nonleaf; ++i)
+                for (int64_t j = Pn.first; j < Pn.first + Pn.count; ++j) walk_into(A[i], T.idx[j]);
+            for (int64_t j = Pn.first; j < Pn.first + Pn.count; ++j) {
+                const uint32_t q = T.idx[j];
+                for (int64_t i = begin_nonleaf; i < e; ++i) walk_into(A[i], q);
+                apply_local(L, q, Pn.center);
+            }
+            return;
+        }
+        std::vector<uint32_t> ents;
+        for (int64_t i = end_far; i < begin_nonleaf; ++i) {
+            const ONode &a = nd(A[i]);
+            ents.insert(ents.end(), T.idx.begin() + a.first, T.idx.begin() + a.first + a.count);
+        }
+        passive_field(ents, 0, (int64_t)ents.size(), p, Local{});
+        std::vector<int32_t> deeper;
+        for (int64_t i = begin_nonleaf; i < e; ++i)
+            for (int k = 0; k < 8; ++k) deeper.push_back(nd(A[i]).child[k]);
+        for (int k = 0; k < 8; ++k) {
+            const int32_t c = Pn.child[k];
+            lockstep(deeper, 0, (int64_t)deeper.size(), c, local_translated(L, v_sub(nd(c).center, Pn.center)));
+        }
+    }
+
+    // Descent::balancedLockstepDualTreeImplicitField (:14-45): breadth-first
+    // descent of the active tree while the criterion asks to descend the
+    // active node only.
+    void balanced(int32_t s) {
+        std::deque<int32_t> q{0};
+        std::vector<int32_t> eq;
+        while (!q.empty()) {
+            const int32_t a = q.front();
+            const int r = recommend(a, s);
+            q.pop_front();
+            if (r != 1 || nd(a).leaf) eq.push_back(a);
+            else for (int k = 0; k < 8; ++k) q.push_back(nd(a).child[k]);
+        }
+        lockstep(eq, 0, (int64_t)eq.size(), s, Local{});
+    }
+
+    // Tree::loadBalancedBreak (Trees/Tree.h:82-106): a max-heap by body count;
+    // the largest node is replaced by its 8 children until n nodes are
+    // queued or the largest is a leaf.
+    std::vector<int32_t> starting_nodes(size_t n) const {
+        std::vector<int32_t> q{0};
+        auto less = [&](int32_t a, int32_t b) { return nd(a).count < nd(b).count; };
+        while (q.size() < n) {
+            std::pop_heap(q.begin(), q.end(), less);
+            const int32_t t = q.back();
+            if (nd(t).leaf) break;
+            q.pop_back();
+            for (int k = 0; k < 8; ++k) {
+                q.push_back(nd(t).child[k]);
+                std::push_heap(q.begin(), q.end(), less);
+            }
+        }
+        return q;
+    }
+
+    // ImplicitDualTraversalSolver::updateAccelerations (:46-68).
+    void run() {
+        if (T.n == 0) return;
+        for (int32_t s : starting_nodes(256)) balanced(s);
+    }
+};
+
 struct Handle {
     int kind;  // 0 octree, 1 lbvh
+    bool fmm = false;
     LBVH *l = nullptr;
     Octree *o = nullptr;
     ~Handle() { delete l; delete o; }
@@ -576,7 +842,8 @@ Params to_params(const float *fp, const int *ip) {
     Params p;
     p.solver = ip[0]; p.leaf_size = ip[1]; p.max_depth = ip[2]; p.morton_bits = ip[3];
     // 3 / 4: the quadrupole variants of the octree / LBVH solvers
-    // (QuadrupoleBarnesHutSolver, QuadrupoleLinearBVHSolver)
+    // (QuadrupoleBarnesHutSolver, QuadrupoleLinearBVHSolver); 5: the implicit
+    // FMM (an octree)
     if (p.solver == 3 || p.solver == 4) { p.solver -= 3; p.order = 2; }
     p.theta = fp[0]; p.G = fp[1]; p.eps2 = fp[2]; p.dt = fp[3];
     return p;
@@ -634,6 +901,7 @@ void *orc_tree_new(const int *ip, const float *fp, int64_t n, const float *x, co
                    const float *z, const float *m) {
     Params p = to_params(fp, ip);
     Handle *h = new Handle;
+    if (p.solver == 5) { p.solver = 0; h->fmm = true; }
     h->kind = p.solver;
     if (p.solver != 0 && p.solver != 1) { delete h; return nullptr; }
     if (p.solver == 1) {
@@ -768,6 +1036,50 @@ int orc_tree_accel(void *hp, int64_t nt, const float *tx, const float *ty, const
         if (np_or_null) np_or_null[i] = c.np;
     }
     return 0;
+}
+
+// The implicit FMM's accelerations for every body of an FMM tree (solver 5),
+// original order, and its four interaction counts (pp, np, pn, nn).  With
+// mass_or_null the same traversal instead adds each source's mass to every
+// target it reaches (FP64; the local fields carry mass too).
+int orc_fmm_accel(void *hp, float *ax, float *ay, float *az, uint64_t *counts4, double *mass_or_null) {
+    Handle *h = (Handle *)hp;
+    if (!h->fmm) return -1;
+    FMM f(*h->o, mass_or_null != nullptr);
+    f.run();
+    for (int64_t i = 0; i < h->o->n; ++i) {
+        if (mass_or_null) mass_or_null[i] = f.recv[i];
+        else { ax[i] = f.acc[i].x; ay[i] = f.acc[i].y; az[i] = f.acc[i].z; }
+    }
+    if (counts4) { counts4[0] = f.cnt.pp; counts4[1] = f.cnt.np; counts4[2] = f.cnt.pn; counts4[3] = f.cnt.nn; }
+    return 0;
+}
+
+// One source's local field at c (Gravity::particleNode into a
+// MultipoleAccelerationSummary<2>) evaluated at p (MultipoleAcceleration::at):
+// test pin for the expansion.
+int orc_fmm_p2l(const float *s3, float m, const float *c3, const float *p3, float G, float eps2, float *out3) {
+    Octree t;
+    t.prm.G = G;
+    t.prm.eps2 = eps2;
+    t.prm.theta = 0.4f;
+    t.n = 0;
+    FMM f(t, false);
+    Local L;
+    f.to_local(V3{s3[0], s3[1], s3[2]}, m, V3{c3[0], c3[1], c3[2]}, L, 0.0);
+    const V3 a = local_at(L, V3{p3[0] - c3[0], p3[1] - c3[1], p3[2] - c3[2]});
+    out3[0] = a.x; out3[1] = a.y; out3[2] = a.z;
+    return 0;
+}
+
+// The starting passive nodes of the implicit FMM (pre-order node ids).
+int64_t orc_fmm_starts(void *hp, int32_t *out, int64_t cap) {
+    Handle *h = (Handle *)hp;
+    if (!h->fmm) return -1;
+    FMM f(*h->o, false);
+    const std::vector<int32_t> s = f.starting_nodes(256);
+    for (size_t i = 0; i < s.size() && (int64_t)i < cap; ++i) out[i] = s[i];
+    return (int64_t)s.size();
 }
 
 // Solver::step (Simulation/Solver.h:103-139): accelerations from the tree,

@@ -501,6 +501,88 @@ def test_quadrupole_walk_same_decisions_smaller_error(orc, nbg, mono, quad, thet
     assert out[quad][0] < 0.5 * out[mono][0]
 
 
+# ------------------------------------------------------------------ implicit FMM (solver 5)
+
+@pytest.mark.parametrize("n,theta,leaf", [(1, 0.4, 16), (3, 0.4, 1), (2000, 0.4, 16), (20000, 0.4, 16),
+                                          (20000, 0.8, 8), (5000, 0.3, 64)])
+def test_fmm_every_source_reaches_every_target_once(orc, nbg, n, theta, leaf):
+    """Interaction-set invariant of the dual traversal: run with each
+    interaction adding its source mass (FP64; a node its bodies' mass, a local
+    field the mass it gathered), every body receives exactly the total mass,
+    i.e. every source body reaches it once, by one route (P2P, node-particle,
+    particle-to-local or node-to-local).  A missed or doubled list entry fails."""
+    b = nbg.plummer(n, seed=3 + n)
+    b.m[:] = np.random.default_rng(n).uniform(0.5, 1.5, n).astype(np.float32)
+    t = orc.Tree(5, b.x, b.y, b.z, b.m, theta=theta, leaf=leaf, eps2=1e-4)
+    got = t.fmm_mass_received()
+    total = float(b.m.astype(np.float64).sum())
+    assert np.allclose(got, total, rtol=1e-12, atol=0)
+
+
+def test_fmm_theta_zero_is_direct_sum(orc, nbg):
+    """theta = 0 approximates nothing: every pair goes through a leaf P2P,
+    so pp = n^2, the other counts are 0 and the result is the direct sum."""
+    n = 3000
+    b = nbg.plummer(n, seed=14)
+    t = orc.Tree(5, b.x, b.y, b.z, b.m, theta=0.0, leaf=16, eps2=1e-4)
+    acc, c = t.fmm_accel(counts=True)
+    assert c == (n * n, 0, 0, 0)
+    ref = orc.direct_f64(b.x, b.y, b.z, b.m, b.x, b.y, b.z, 1.0, 1e-4)
+    rel = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert np.median(rel) < 1e-6 and rel.max() < 1e-4
+
+
+def test_fmm_force_error_and_counts_fall_with_theta(orc, nbg):
+    """Against the FP64 direct sum the implicit FMM is as accurate as
+    Barnes-Hut at the same theta (its near field is the same walk); a larger
+    theta costs accuracy and saves interactions, P2P first."""
+    n = 20000
+    b = nbg.plummer(n, seed=5)
+    sel = np.arange(0, n, 10)
+    ref = orc.direct_f64(b.x, b.y, b.z, b.m, b.x[sel], b.y[sel], b.z[sel], 1.0, 1e-4)
+    res = {}
+    for th in (0.4, 0.7):
+        t = orc.Tree(5, b.x, b.y, b.z, b.m, theta=th, leaf=16, eps2=1e-4)
+        acc, c = t.fmm_accel(counts=True)
+        rel = np.linalg.norm(acc[sel] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        res[th] = (float(np.median(rel)), c)
+    assert res[0.4][0] < 1.5e-3 and res[0.4][0] < res[0.7][0] < 1e-2
+    assert res[0.7][1][0] < res[0.4][1][0] and sum(res[0.7][1]) < sum(res[0.4][1])
+
+
+def test_fmm_local_expansion_is_first_order_taylor(orc):
+    """A source's local field at c evaluated at c + h (vector plus gradient
+    tensor) is the first-order expansion of its force: the error falls ~4x
+    when h halves.  A wrong sign or factor on the gradient leaves an O(h)
+    error (~2x)."""
+    s = np.array([3.0, -2.0, 5.0], np.float32)
+    c = np.array([0.2, 0.1, -0.3], np.float32)
+    u = np.array([0.6, -0.3, 0.74], np.float64)
+    errs = []
+    for h in (0.2, 0.1, 0.05):
+        p = (c + h * u).astype(np.float32)
+        exact = orc.direct_f64([s[0]], [s[1]], [s[2]], [2.0], [p[0]], [p[1]], [p[2]], 1.5, 0.0)[0]
+        got = orc.fmm_p2l(s, 2.0, c, p, 1.5, 0.0)
+        errs.append(np.linalg.norm(got - exact))
+    for k in range(2):
+        assert 3.0 < errs[k] / errs[k + 1] < 5.5
+
+
+def test_fmm_starting_nodes_partition_the_bodies(orc, nbg):
+    """Tree::loadBalancedBreak(256): between 256 and 262 queued nodes whose
+    ranges are disjoint and cover every body."""
+    b = nbg.plummer(30000, seed=8)
+    t = orc.Tree(5, b.x, b.y, b.z, b.m, theta=0.4, leaf=16, eps2=1e-4)
+    st = t.fmm_starts()
+    nd = t.nodes()
+    assert 256 <= len(st) <= 262
+    cover = np.zeros(30000, np.int32)
+    for s in st:
+        f, k = int(nd["first"][s]), int(nd["count"][s])
+        cover[f:f + k] += 1
+    assert np.all(cover == 1)
+
+
 # ------------------------------------------------------------------ integrator, direct sum, energy
 
 def test_integrator_is_kick_then_drift(orc, nbg):
