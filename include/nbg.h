@@ -52,7 +52,13 @@ enum nbg_solver_kind {
     NBG_LINEAR_BVH = 1,
     NBG_NAIVE = 2,
     NBG_BARNES_HUT_QUADRUPOLE = 3,
-    NBG_LINEAR_BVH_QUADRUPOLE = 4
+    NBG_LINEAR_BVH_QUADRUPOLE = 4,
+    /* ImplicitFMMSolver = MultipoleImplicitFMMSolver<1> (Solvers/ImplicitFMMSolver.h:14-47,
+     * id "implicit-fmm-2p"): the lockstep dual traversal of ImplicitDualTraversalSolver
+     * (Solvers/ImplicitDualTraversalSolver.h:17-70) over the Barnes-Hut octree, monopole
+     * sources and local fields with their gradient; theta is its SideLengthOverDistance
+     * (reference default 0.4).  FAST sum mode only; its passive points are its bodies. */
+    NBG_IMPLICIT_FMM = 5
 };
 
 /* Per-body sum order.  ORDERED reproduces the reference's arithmetic exactly:
@@ -152,6 +158,12 @@ int nbg_sys_synchronize(nbg_sys *sys);
  * walk (on by default; FAST passes skip it while it is off, ORDERED passes
  * always count). */
 int nbg_sys_counts(nbg_sys *sys, uint64_t *pp, uint64_t *np);
+/* All four SimpleTrackingRule counters (Physics/Rules/SimpleTrackingRule.h:29-61):
+ * out4 = {pp, np, pn, nn}; pn (particle-node: a body into a local field) and
+ * nn (node-node: a node into a local field) are nonzero for the implicit FMM
+ * only.  Empty octants, which the reference keeps as body-less nodes, are not
+ * counted. */
+int nbg_sys_counts4(nbg_sys *sys, uint64_t *out4);
 int nbg_sys_enable_counts(nbg_sys *sys, int32_t on);
 
 /* Schedule statistics of the last FAST force pass (no reference counterpart;

@@ -27,6 +27,7 @@ LINEAR_BVH = 1
 NAIVE = 2
 BARNES_HUT_QUADRUPOLE = 3
 LINEAR_BVH_QUADRUPOLE = 4
+IMPLICIT_FMM = 5
 SUM_ORDERED = 0
 SUM_FAST = 1
 
@@ -38,7 +39,7 @@ EXPORTS = (
     "nbg_ctx_create", "nbg_ctx_destroy", "nbg_sys_create", "nbg_sys_destroy",
     "nbg_sys_set_params", "nbg_sys_get_params", "nbg_sys_set_state", "nbg_sys_get_state",
     "nbg_sys_update_accelerations", "nbg_sys_get_accelerations", "nbg_sys_accelerations_at", "nbg_sys_step",
-    "nbg_sys_synchronize", "nbg_sys_counts", "nbg_sys_enable_counts", "nbg_sys_enable_walk_stats",
+    "nbg_sys_synchronize", "nbg_sys_counts", "nbg_sys_counts4", "nbg_sys_enable_counts", "nbg_sys_enable_walk_stats",
     "nbg_sys_walk_stats",
     "nbg_sys_enable_timing", "nbg_sys_phase_times",
     "nbg_sys_debug_bbox", "nbg_sys_debug_order", "nbg_sys_debug_tree", "nbg_sys_debug_quads", "nbg_direct_f64",
@@ -141,6 +142,7 @@ def lib() -> ctypes.CDLL:
             "nbg_sys_step": (ctypes.c_int, [_vp, ctypes.c_int32]),
             "nbg_sys_synchronize": (ctypes.c_int, [_vp]),
             "nbg_sys_counts": (ctypes.c_int, [_vp, _u64p, _u64p]),
+            "nbg_sys_counts4": (ctypes.c_int, [_vp, _u64p]),
             "nbg_sys_walk_stats": (ctypes.c_int, [_vp, _u64p]),
             "nbg_sys_enable_walk_stats": (ctypes.c_int, [_vp, ctypes.c_int32]),
             "nbg_sys_enable_counts": (ctypes.c_int, [_vp, ctypes.c_int32]),
@@ -331,6 +333,12 @@ class System:
         pp, np_ = ctypes.c_uint64(), ctypes.c_uint64()
         _check(lib().nbg_sys_counts(self.handle, ctypes.byref(pp), ctypes.byref(np_)))
         return int(pp.value), int(np_.value)
+
+    def counts4(self):
+        """(pp, np, pn, nn) as SimpleTrackingRule counts them."""
+        out = (ctypes.c_uint64 * 4)()
+        _check(lib().nbg_sys_counts4(self.handle, out))
+        return tuple(int(v) for v in out)
 
     def enable_counts(self, on: bool = True):
         _check(lib().nbg_sys_enable_counts(self.handle, int(on)))
@@ -703,8 +711,38 @@ class NaiveSolver(_TreeSolver):
         return "Naive (MI355X)"
 
 
+class ImplicitFMMSolver(_TreeSolver):
+    """ImplicitFMMSolver = MultipoleImplicitFMMSolver<1>
+    (src/NBody/Simulation/Solvers/ImplicitFMMSolver.h:14-36): the implicit
+    dual traversal (ImplicitDualTraversalSolver.h:17-70) over the Barnes-Hut
+    octree with SideLengthOverDistance at theta 0.4.  FAST sum mode only."""
+
+    _kind = IMPLICIT_FMM
+
+    def __init__(self, simulation: Simulation, rule: Gravity, context: Context | None = None):
+        super().__init__(simulation, rule, context)
+        self._params.theta = 0.4  # ImplicitDualTraversalSolver's criterion (ImplicitDualTraversalSolver.h:22)
+        self._params.sum_mode = SUM_FAST
+
+    def id(self) -> str:
+        return "gpu-implicit-fmm-2p"
+
+    def name(self) -> str:
+        return "Implicit Fast Multipole Method (1-pole, MI355X)"
+
+    def maxDepth(self) -> int:
+        return self._params.max_depth
+
+    def setMaxDepth(self, d: int):
+        self._params.max_depth = d
+
+    def counts4(self):
+        return self._system().counts4() if self._sys is not None else (0, 0, 0, 0)
+
+
 SOLVERS = {"gpu-linear-bvh": LinearBVHSolver, "gpu-barnes-hut": BarnesHutSolver, "gpu-naive": NaiveSolver,
-           "gpu-linear-bvh-4p": QuadrupoleLinearBVHSolver, "gpu-barnes-hut-4p": QuadrupoleBarnesHutSolver}
+           "gpu-linear-bvh-4p": QuadrupoleLinearBVHSolver, "gpu-barnes-hut-4p": QuadrupoleBarnesHutSolver,
+           "gpu-implicit-fmm-2p": ImplicitFMMSolver}
 
 
 def select(solver_id: str):

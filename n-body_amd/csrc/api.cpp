@@ -2,6 +2,7 @@
 // device buffers of each system, enqueues the build / walk / update kernels
 // on the system's stream and reports errors as status codes.
 
+#include <algorithm>
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -19,7 +20,12 @@ namespace {
 
 constexpr int kPhases = 8;
 
-bool is_octree(int solver) { return solver == NBG_BARNES_HUT || solver == NBG_BARNES_HUT_QUADRUPOLE; }
+bool is_octree(int solver) {
+    return solver == NBG_BARNES_HUT || solver == NBG_BARNES_HUT_QUADRUPOLE || solver == NBG_IMPLICIT_FMM;
+}
+// Implicit FMM scratch: waves, node-list and body-list capacities per wave.
+constexpr int kFmmWaves = 2048, kFmmCapA = 8192, kFmmCapE = 32768;
+constexpr int64_t kFmmWaveWords = 4 * (int64_t)kFmmCapA + 2 * (int64_t)kFmmCapE;
 bool is_quad(int solver) { return solver == NBG_BARNES_HUT_QUADRUPOLE || solver == NBG_LINEAR_BVH_QUADRUPOLE; }
 constexpr int kEventSets = 256;
 
@@ -67,6 +73,12 @@ struct nbg_sys {
     // second moments, 2 float4 per node slot (allocated for the quadrupole solvers)
     float4 *quad = nullptr;
     int64_t quad_slots = 0;
+    // implicit FMM work (allocated for that solver): start flags by slot,
+    // starting nodes, leaf list, counters, per-wave list scratch
+    uint32_t *fmm_flag = nullptr, *fmm_leaves = nullptr, *fmm_misc = nullptr, *fmm_scratch = nullptr;
+    int32_t *fmm_starts = nullptr;
+    int64_t fmm_slots = 0;
+    int fmm_waves = 0;  // persistent waves of the FMM kernel (each owns kFmmWaveWords of scratch)
     // small
     float *bbox_partials = nullptr, *bbox = nullptr;
     // [1] err, [2] node count, [4..16) counts: pp, np, then the FAST walk's
@@ -83,14 +95,15 @@ namespace {
 
 int check_params(const nbg_params *p) {
     if (!p) return fail(NBG_ERR_ARG, "params is NULL");
-    if (p->solver < NBG_BARNES_HUT || p->solver > NBG_LINEAR_BVH_QUADRUPOLE)
-        return fail(NBG_ERR_ARG, "unknown solver kind");
+    if (p->solver < NBG_BARNES_HUT || p->solver > NBG_IMPLICIT_FMM) return fail(NBG_ERR_ARG, "unknown solver kind");
     if (!(p->theta >= 0.0f)) return fail(NBG_ERR_ARG, "theta must be >= 0");
     if (p->leaf_size < 1) return fail(NBG_ERR_ARG, "leaf_size must be >= 1");
     if (p->max_depth < 0 || p->max_depth > 32) return fail(NBG_ERR_ARG, "max_depth must be in [0, 32]");
     if (p->morton_bits != 30 && p->morton_bits != 63) return fail(NBG_ERR_ARG, "morton_bits must be 30 or 63");
     if (!(p->eps2 >= 0.0f)) return fail(NBG_ERR_ARG, "eps2 must be >= 0");
     if (p->sum_mode != NBG_SUM_ORDERED && p->sum_mode != NBG_SUM_FAST) return fail(NBG_ERR_ARG, "unknown sum mode");
+    if (p->solver == NBG_IMPLICIT_FMM && p->sum_mode != NBG_SUM_FAST)
+        return fail(NBG_ERR_ARG, "the implicit FMM runs in the FAST sum mode only");
     return NBG_OK;
 }
 
@@ -149,13 +162,49 @@ int ensure_quad(nbg_sys *s) {
     return NBG_OK;
 }
 
+void free_fmm(nbg_sys *s) {
+    void *ptrs[] = {s->fmm_flag, s->fmm_leaves, s->fmm_misc, s->fmm_scratch, s->fmm_starts};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    s->fmm_flag = s->fmm_leaves = s->fmm_misc = s->fmm_scratch = nullptr;
+    s->fmm_starts = nullptr;
+    s->fmm_slots = 0;
+    s->fmm_waves = 0;
+}
+
+// Allocates the implicit FMM's work buffers (no-op for other solvers or when
+// the current ones are large enough).
+int ensure_fmm(nbg_sys *s) {
+    if (s->prm.solver != NBG_IMPLICIT_FMM) return NBG_OK;
+    // one wave per leaf at most: small systems get a small scratch
+    const int waves = (int)std::min<int64_t>(kFmmWaves, s->cap > 0 ? s->cap : 1);
+    if (s->fmm_scratch && s->fmm_slots >= s->ob.cap && s->fmm_waves >= waves) return NBG_OK;
+    if (s->stream) NBG_HIP(hipStreamSynchronize(s->stream));
+    free_fmm(s);
+    int r = NBG_OK;
+    auto A = [&](auto **ptr, int64_t count) {
+        if (r == NBG_OK) r = dalloc(ptr, count);
+    };
+    A(&s->fmm_flag, s->ob.cap); A(&s->fmm_leaves, s->cap > 0 ? s->cap : 1); A(&s->fmm_misc, 64);
+    A(&s->fmm_starts, 512); A(&s->fmm_scratch, (int64_t)waves * kFmmWaveWords);
+    if (r != NBG_OK) {
+        free_fmm(s);
+        return r;
+    }
+    s->fmm_slots = s->ob.cap;
+    s->fmm_waves = waves;
+    return NBG_OK;
+}
+
 int ensure_buffers(nbg_sys *s) {
     if (int r = ensure_octree(s)) return r;
+    if (int r = ensure_fmm(s)) return r;
     return ensure_quad(s);
 }
 
 void free_sys(nbg_sys *s) {
     free_oct(s->ob);
+    free_fmm(s);
     if (s->quad) (void)hipFree(s->quad);
     void *ptrs[] = {s->x, s->y, s->z, s->m, s->vx, s->vy, s->vz, s->ax, s->ay, s->az, s->keys, s->keys_alt,
                     s->idx, s->idx_alt, s->bodies, s->kfirst, s->klast, s->kgamma, s->kparent, s->ksplit,
@@ -253,9 +302,32 @@ int enqueue_force_pass(nbg_sys *s) {
                               p.theta, s->err_word(), st, nullptr, nullptr, 0, quad);
     }
     mark(s, 6);
-    WalkArgs wa = walk_args(s);
-    const int stats = s->walk_stats ? 2 : (s->counting ? 1 : 0);
-    launch_walk(s, wa, stats);
+    if (p.solver == NBG_IMPLICIT_FMM) {
+        FmmArgs fa;
+        fa.n = n;
+        fa.bodies = s->bodies;
+        fa.nodes = s->ob.nodes;
+        fa.centre = s->ob.centre;
+        fa.misc = s->ob.misc;
+        fa.max_nodes = s->ob.cap;
+        fa.perm = s->idx;
+        fa.ax = s->ax; fa.ay = s->ay; fa.az = s->az;
+        fa.theta = p.theta; fa.G = p.G; fa.eps2 = p.eps2;
+        fa.start_flag = s->fmm_flag;
+        fa.starts = s->fmm_starts;
+        fa.nstarts = s->fmm_misc; fa.nleaves = s->fmm_misc + 1; fa.next = s->fmm_misc + 2;
+        fa.leaves = s->fmm_leaves;
+        fa.scratch = s->fmm_scratch;
+        fa.wave_words = kFmmWaveWords;
+        fa.cap_a = kFmmCapA; fa.cap_e = kFmmCapE; fa.waves = s->fmm_waves;
+        fa.counts = s->counts();
+        fa.err = s->err_word();
+        launch_fmm(fa, s->leaf_start, st);
+    } else {
+        WalkArgs wa = walk_args(s);
+        const int stats = s->walk_stats ? 2 : (s->counting ? 1 : 0);
+        launch_walk(s, wa, stats);
+    }
     mark(s, 7);
     NBG_HIP(hipGetLastError());
     s->tree_valid = true;
@@ -318,6 +390,7 @@ int sync_and_check(nbg_sys *s) {
         if (err & 2u) return fail(NBG_ERR_DEVICE, "tree traversal exceeded its stack, depth or index limits");
         if (err & 4u) return fail(NBG_ERR_DEVICE, "tree summaries found an invalid node index");
         if (err & 8u) return fail(NBG_ERR_DEVICE, "octree build found an inconsistent node index");
+        if (err & 16u) return fail(NBG_ERR_DEVICE, "implicit FMM interaction lists exceeded their capacity");
         return fail(NBG_ERR_DEVICE, "device error flag set");
     }
     fold_timing(s);
@@ -516,6 +589,8 @@ int nbg_sys_accelerations_at(nbg_sys *s, int64_t nt, const float *tx, const floa
     if (nt < 0 || nt > kMaxBodies) return fail(NBG_ERR_ARG, "target count out of range");
     if (nt > 0 && (!tx || !ty || !tz || !ax || !ay || !az)) return fail(NBG_ERR_ARG, "NULL array");
     if (!s->have_state) return fail(NBG_ERR_STATE, "set_state has not been called");
+    if (s->prm.solver == NBG_IMPLICIT_FMM)
+        return fail(NBG_ERR_STATE, "the implicit FMM's passive points are its own bodies");
     if (nt == 0) return NBG_OK;
     DeviceGuard g(s->ctx->device);
     if (s->n == 0) {  // no actives: every passive acceleration is zero
@@ -609,6 +684,21 @@ int nbg_sys_counts(nbg_sys *s, uint64_t *pp, uint64_t *np) {
     NBG_HIP(hipMemcpy(c, s->counts(), sizeof c, hipMemcpyDeviceToHost));
     if (pp) *pp = c[0];
     if (np) *np = c[1];
+    return NBG_OK;
+}
+
+int nbg_sys_counts4(nbg_sys *s, uint64_t *out4) {
+    if (!s || !out4) return fail(NBG_ERR_ARG, "sys or out is NULL");
+    if (!s->have_forces) return fail(NBG_ERR_STATE, "no force pass has run");
+    DeviceGuard g(s->ctx->device);
+    NBG_HIP(hipStreamSynchronize(s->stream));
+    unsigned long long c[4];
+    NBG_HIP(hipMemcpy(c, s->counts(), sizeof c, hipMemcpyDeviceToHost));
+    const bool fmm = s->prm.solver == NBG_IMPLICIT_FMM;
+    out4[0] = c[0];
+    out4[1] = c[1];
+    out4[2] = fmm ? c[2] : 0;
+    out4[3] = fmm ? c[3] : 0;
     return NBG_OK;
 }
 

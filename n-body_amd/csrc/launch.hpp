@@ -242,6 +242,32 @@ void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, int diag, h
 // Barnes-Hut traversal over the octree (same modes and statistics).
 void launch_walk_oct(const WalkArgs &a, int ordered, int diag, hipStream_t s);
 
+// Implicit FMM (fmm.hip) over a built octree with summaries: one wavefront
+// per passive leaf, `waves` persistent wavefronts, each with wave_words words
+// of scratch (four node lists of cap_a entries, two body lists of cap_e).
+// counts: [pp, np, pn, nn]; err bit 16: a list outgrew its capacity.
+struct FmmArgs {
+    int64_t n;
+    const float4 *bodies;     // tree order (x, y, z, m)
+    const OctRec *nodes;
+    const float4 *centre;     // cell centre xyz, side, by slot
+    const uint32_t *misc;     // octree misc ([0] node count)
+    int64_t max_nodes;
+    const uint32_t *perm;     // tree position -> original index
+    float *ax, *ay, *az;
+    float theta, G, eps2;
+    uint32_t *start_flag;     // max_nodes words
+    int32_t *starts;          // 512
+    uint32_t *nstarts, *nleaves, *next;
+    uint32_t *leaves;         // n
+    uint32_t *scratch;
+    int64_t wave_words;
+    int cap_a, cap_e, waves;
+    unsigned long long *counts;
+    uint32_t *err;
+};
+void launch_fmm(const FmmArgs &a, const int32_t *leaf_start, hipStream_t s);
+
 // v = v + a*dt; p = p + v*dt (original order).
 void launch_update(float *x, float *y, float *z, float *vx, float *vy, float *vz, const float *ax,
                    const float *ay, const float *az, float dt, int64_t n, hipStream_t s);

@@ -27,6 +27,7 @@
 
 #include "device.hpp"
 #include "launch.hpp"
+#include "walk_common.hpp"
 
 namespace nbg {
 
@@ -183,80 +184,6 @@ __global__ __launch_bounds__(64) void k_walk_lbvh_ordered(WalkArgs a) {
     }
 }
 
-// ---------------------------------------------------------------- FAST
-struct FastWave {
-    float px, py, pz;          // this lane's target
-    float rx, ry, rz;          // this lane's acceleration
-    float G;                   // per lane, so G * (scalar mass) is one VALU op
-    uint64_t pp, np;           // wave totals (interaction counts)
-    uint64_t visits, lanes, leaf_visits, leaf_lanes;  // wave-uniform diagnostics
-};
-
-// Monopole (or body) interaction with rsqrt + FMA: s = m * inv^3; the walk
-// multiplies the sum by G once at the end.  eps2 > 0 (SOFT) makes r2 > 0 for
-// every pair, coincident ones included (their d = 0 contributes exactly
-// zero); otherwise r2 = 0 is guarded.
-template <bool SOFT>
-__device__ __forceinline__ void interact(float bx, float by, float bz, float bm, float eps2, FastWave &W) {
-    const float dx = bx - W.px, dy = by - W.py, dz = bz - W.pz;
-    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
-    float inv = __builtin_amdgcn_rsqf(r2);
-    if (!SOFT) inv = r2 > 0.0f ? inv : 0.0f;
-    const float s = bm * (inv * (inv * inv));
-    W.rx = __builtin_fmaf(dx, s, W.rx);
-    W.ry = __builtin_fmaf(dy, s, W.ry);
-    W.rz = __builtin_fmaf(dz, s, W.rz);
-}
-
-// Quadrupole node interaction (Gravity::nodeParticle<2>, Gravity.h:238-259)
-// with rsqrt + FMA.  With d = c - p (the reference's R = -d), the reference's
-// R (-M) / r^3 + D3(R) . Q / 2 expands to
-//   d (M r^-3 + 7.5 r^-7 d.Q.d - 1.5 r^-5 tr Q) - 3 r^-5 Q d,
-// r^2 = d.d + eps2; G is applied once at the end of the walk.  d = 0 (the
-// target at the node's COM) contributes exactly zero.
-template <bool SOFT>
-__device__ __forceinline__ void interact_quad(float bx, float by, float bz, float bm, float4 q0, float4 q1,
-                                              float eps2, FastWave &W) {
-    const float dx = bx - W.px, dy = by - W.py, dz = bz - W.pz;
-    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
-    float inv = __builtin_amdgcn_rsqf(r2);
-    if (!SOFT) inv = r2 > 0.0f ? inv : 0.0f;
-    const float inv2 = inv * inv, inv3 = inv * inv2, inv5 = inv3 * inv2;
-    // Q d with Q = [[xx xy xz] [xy yy yz] [xz yz zz]] = [[q0.x q0.y q0.z] [q0.y q0.w q1.x] [q0.z q1.x q1.y]]
-    const float qx = __builtin_fmaf(q0.x, dx, __builtin_fmaf(q0.y, dy, q0.z * dz));
-    const float qy = __builtin_fmaf(q0.y, dx, __builtin_fmaf(q0.w, dy, q1.x * dz));
-    const float qz = __builtin_fmaf(q0.z, dx, __builtin_fmaf(q1.x, dy, q1.y * dz));
-    const float dqd = __builtin_fmaf(dx, qx, __builtin_fmaf(dy, qy, dz * qz));
-    const float s = __builtin_fmaf(bm, inv3, inv5 * __builtin_fmaf(7.5f * inv2, dqd, -1.5f * q1.z));
-    const float t = -3.0f * inv5;
-    W.rx = __builtin_fmaf(dx, s, __builtin_fmaf(t, qx, W.rx));
-    W.ry = __builtin_fmaf(dy, s, __builtin_fmaf(t, qy, W.ry));
-    W.rz = __builtin_fmaf(dz, s, __builtin_fmaf(t, qz, W.rz));
-}
-
-// The lanes in `take` interact with the `cnt` bodies starting at `first`.
-template <bool SOFT>
-__device__ __forceinline__ void leaf_bodies(const float4 *__restrict__ bodies, int first, int cnt, uint64_t take,
-                                            float eps2, FastWave &W) {
-    if (__builtin_amdgcn_inverse_ballot_w64(take)) {
-        int j = 0;
-        for (; j + 4 <= cnt; j += 4) {
-            const float4 b0 = bodies[first + j], b1 = bodies[first + j + 1];
-            const float4 b2 = bodies[first + j + 2], b3 = bodies[first + j + 3];
-            interact<SOFT>(b0.x, b0.y, b0.z, b0.w, eps2, W);
-            interact<SOFT>(b1.x, b1.y, b1.z, b1.w, eps2, W);
-            interact<SOFT>(b2.x, b2.y, b2.z, b2.w, eps2, W);
-            interact<SOFT>(b3.x, b3.y, b3.z, b3.w, eps2, W);
-        }
-        for (; j < cnt; ++j) {
-            const float4 b = bodies[first + j];
-            interact<SOFT>(b.x, b.y, b.z, b.w, eps2, W);
-        }
-    }
-}
-
-// Visits one node for the lanes in `mask` (wave-uniform); returns the lanes
-// that open it when it is internal, 0 when it is a leaf or all accept.
 // qn: the node's two moment records (QUAD only; uniform).
 template <bool SOFT, int STATS, bool QUAD>
 __device__ __forceinline__ uint64_t visit(const NodeRec &nd, const float4 *__restrict__ qn, uint64_t mask,
@@ -382,18 +309,6 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_forest(WalkArgs a) {
 }
 
 // ================================================================ octree
-// SideLengthOverDistance with the octree exclusion region
-// (Descent/DescentCriterion/SideLengthOverDistance.h:17-22,
-// Descent/DescentCriterionType.h:81-85): the distance to the centre of mass
-// in glm's order against the level's threshold (side / sqrt(d2) < theta is
-// monotone in d2), and the target outside [c - side, c + side).
-__device__ __forceinline__ bool oct_accepts(const OctRec &r, float px, float py, float pz) {
-    const float dx = px - r.com.x, dy = py - r.com.y, dz = pz - r.com.z;
-    const float d2 = dx * dx + dy * dy + dz * dz;
-    const bool inside = r.lo.x <= px && r.lo.y <= py && r.lo.z <= pz && px < r.hi.x && py < r.hi.y && pz < r.hi.z;
-    return d2 > r.lo.w && !inside;
-}
-
 // ORDERED: libstdc++'s std::transform_reduce over the 8 children of an
 // octree node (activeTree.h:66-76) computes init + ((r0 + r1) + (r2 + r3)),
 // then + ((r4 + r5) + (r6 + r7)), with init = 0 and r = 0 for an empty
@@ -541,39 +456,6 @@ __global__ __launch_bounds__(64) void k_walk_oct_ordered(WalkArgs a) {
     }
 }
 
-// FAST over the octree: the children of an opened node are contiguous and
-// visited in one group; opened internal children are pushed as groups.
-// FAST over the octree: the children of an opened node are contiguous and
-// visited in one group; opened internal children are pushed as groups.
-template <bool SOFT, int STATS, bool QUAD>
-__device__ __forceinline__ uint64_t visit_oct(const OctRec &nd, const float4 *__restrict__ qn, uint64_t mask,
-                                              const float4 *__restrict__ bodies, int64_t n, float eps2, FastWave &W,
-                                              bool &fail) {
-    const uint64_t acc = __ballot(oct_accepts(nd, W.px, W.py, W.pz)) & mask;
-    if (STATS >= 2) { W.visits += 1; W.lanes += __popcll(mask); }
-    if (STATS >= 1) W.np += __popcll(acc);
-    if (QUAD) {
-        if (acc) {
-            const float4 q0 = qn[0], q1 = qn[1];
-            if (__builtin_amdgcn_inverse_ballot_w64(acc))
-                interact_quad<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, q0, q1, eps2, W);
-        }
-    } else if (__builtin_amdgcn_inverse_ballot_w64(acc)) {
-        interact<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, eps2, W);
-    }
-    const uint64_t open = mask & ~acc;
-    if (nd.topo.z >= 0) return open;
-    if (open == 0) return 0;
-    const int first = nd.topo.x, cnt = nd.topo.y;
-    if (__builtin_expect((uint64_t)(uint32_t)first >= (uint64_t)n, 0)) { fail = true; return 0; }
-    if (__builtin_expect((uint64_t)(uint32_t)cnt > (uint64_t)n - (uint32_t)first, 0)) { fail = true; return 0; }
-    if (STATS >= 2) { W.leaf_visits += 1; W.leaf_lanes += __popcll(open); }
-    if (STATS >= 1) W.pp += (uint64_t)__popcll(open) * (uint64_t)cnt;
-    leaf_bodies<SOFT>(bodies, first, cnt, open, eps2, W);
-    return 0;
-}
-
-constexpr int kStackOct = 256;
 
 template <int WAVES, bool SOFT, int STATS, bool QUAD>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {

@@ -32,12 +32,16 @@ int main(int argc, char **argv) {
         nbody_gpu::Gravity rule(1.0f, 1e-4f);
         nbody_gpu::Solver::timeStep() = 1e-3f;
         auto solver = nbody_gpu::select(id, b, rule);
-        solver->theta() = 0.5f;
-        solver->sumMode() = NBG_SUM_ORDERED;
+        auto *fmm = dynamic_cast<nbody_gpu::GpuImplicitFMMSolver *>(solver.get());
+        if (!fmm) {
+            solver->theta() = 0.5f;
+            solver->sumMode() = NBG_SUM_ORDERED;
+        }
         if (auto *bh = dynamic_cast<nbody_gpu::GpuBarnesHutSolver *>(solver.get())) bh->maxLeafSize() = 16;
         if (auto *lb = dynamic_cast<nbody_gpu::GpuLinearBVHSolver *>(solver.get())) lb->maxLeafSize() = 16;
         if (auto *qb = dynamic_cast<nbody_gpu::GpuQuadrupoleBarnesHutSolver *>(solver.get())) qb->maxLeafSize() = 16;
         if (auto *ql = dynamic_cast<nbody_gpu::GpuQuadrupoleLinearBVHSolver *>(solver.get())) ql->maxLeafSize() = 16;
+        if (fmm) fmm->maxLeafSize() = 16;
         std::FILE *f = std::fopen(argv[3], "wb");
         if (!f) return 2;
         std::fwrite(&n, sizeof n, 1, f);

@@ -20,6 +20,7 @@
 // signals) are thrown as nbody_gpu::Error with nbg_last_error()'s message.
 #pragma once
 
+#include <array>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -128,15 +129,28 @@ public:
         return {pp, np};
     }
 
+    // All four counters {pp, np, pn, nn} (pn and nn: the implicit FMM's
+    // particle-node and node-node calls).
+    std::array<uint64_t, 4> interactionCounts4() {
+        std::array<uint64_t, 4> c{};
+        check(nbg_sys_counts4(sys_, c.data()), "counts4");
+        return c;
+    }
+
     Bodies &simulation() { return sim_; }
     Gravity &rule() { return rule_; }
     nbg_sys *handle() { return sys_; }
 
 protected:
-    Solver(Bodies &sim, Gravity &rule, int solver_kind, const Device &dev = default_device())
+    // theta / sum_mode: the solver's own defaults where they differ from
+    // nbg_params_default's (negative: keep those).
+    Solver(Bodies &sim, Gravity &rule, int solver_kind, const Device &dev = default_device(), float theta = -1.0f,
+           int sum_mode = -1)
         : sim_(sim), rule_(rule), dev_(dev) {
         nbg_params_default(&prm_);
         prm_.solver = solver_kind;
+        if (theta >= 0.0f) prm_.theta = theta;
+        if (sum_mode >= 0) prm_.sum_mode = sum_mode;
         if (sim_.ax.size() != sim_.size()) {
             sim_.ax.assign(sim_.size(), 0.0f);
             sim_.ay.assign(sim_.size(), 0.0f);
@@ -240,6 +254,19 @@ public:
     int &mortonBits() { return prm_.morton_bits; }
 };
 
+// ImplicitFMMSolver = MultipoleImplicitFMMSolver<1> (Solvers/ImplicitFMMSolver.h:14-36):
+// the implicit dual traversal over the octree, theta 0.4
+// (ImplicitDualTraversalSolver.h:22); FAST sum mode only.
+class GpuImplicitFMMSolver : public ActiveTreeSolver {
+public:
+    GpuImplicitFMMSolver(Bodies &sim, Gravity &rule, const Device &dev = default_device())
+        : ActiveTreeSolver(sim, rule, NBG_IMPLICIT_FMM, dev, 0.4f, NBG_SUM_FAST) {}
+    std::string id() override { return "gpu-implicit-fmm-2p"; }
+    std::string name() override { return "Implicit Fast Multipole Method (1-pole, MI355X)"; }
+    int &maxDepth() { return prm_.max_depth; }
+    int &maxLeafSize() { return prm_.leaf_size; }
+};
+
 // NaiveSolver (Solvers/NaiveSolver.h:14-52): direct summation, no tree.
 class GpuNaiveSolver : public Solver {
 public:
@@ -256,6 +283,7 @@ inline std::unique_ptr<ActiveTreeSolver> select(const std::string &id, Bodies &s
     if (id == "gpu-linear-bvh") return std::make_unique<GpuLinearBVHSolver>(sim, rule);
     if (id == "gpu-barnes-hut-4p") return std::make_unique<GpuQuadrupoleBarnesHutSolver>(sim, rule);
     if (id == "gpu-linear-bvh-4p") return std::make_unique<GpuQuadrupoleLinearBVHSolver>(sim, rule);
+    if (id == "gpu-implicit-fmm-2p") return std::make_unique<GpuImplicitFMMSolver>(sim, rule);
     throw Error(NBG_ERR_ARG, "unknown solver id " + id);
 }
 

@@ -1090,7 +1090,8 @@ int orc_step(const int *ip, const float *fp, int64_t n, float *x, float *y, floa
     for (int s = 0; s < nsteps; ++s) {
         void *h = orc_tree_new(ip, fp, n, x, y, z, m);
         if (!h) return -1;
-        orc_tree_accel(h, n, x, y, z, ax, ay, az, nullptr, nullptr, threads);
+        if (((Handle *)h)->fmm) orc_fmm_accel(h, ax, ay, az, nullptr, nullptr);
+        else orc_tree_accel(h, n, x, y, z, ax, ay, az, nullptr, nullptr, threads);
         orc_tree_free(h);
         for (int64_t i = 0; i < n; ++i) {
             vx[i] = vx[i] + ax[i] * p.dt; vy[i] = vy[i] + ay[i] * p.dt; vz[i] = vz[i] + az[i] * p.dt;
