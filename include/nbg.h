@@ -170,7 +170,10 @@ int nbg_sys_enable_counts(nbg_sys *sys, int32_t on);
  * diagnostics of the wavefront walk): out4 = {wave-level node visits, active
  * lanes summed over those visits, wave-level leaf visits, lanes interacting
  * with a leaf summed over leaf visits}.  Counted only while enabled (it
- * costs scalar instructions in the walk); zero after an ORDERED pass. */
+ * costs scalar instructions in the walk); zero after an ORDERED pass.
+ * For the implicit FMM (always available) out4 = {largest list-pool use of
+ * one traversal step in words, largest frontier of a step in nodes,
+ * traversal steps launched, list-pool capacity per step in words}. */
 int nbg_sys_enable_walk_stats(nbg_sys *sys, int32_t on);
 int nbg_sys_walk_stats(nbg_sys *sys, uint64_t *out4);
 

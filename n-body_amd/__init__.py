@@ -350,6 +350,8 @@ class System:
         out = np.zeros(4, np.uint64)
         _check(lib().nbg_sys_walk_stats(self.handle, _ptr(out, _u64p)))
         v = [int(x) for x in out]
+        if self.params.solver == IMPLICIT_FMM:
+            return {"pool_peak": v[0], "frontier_peak": v[1], "steps": v[2], "pool_capacity": v[3]}
         return {"node_visits": v[0], "active_lanes": v[1], "leaf_visits": v[2], "leaf_lanes": v[3]}
 
     def enable_timing(self, on: bool = True):

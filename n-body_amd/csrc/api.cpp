@@ -23,9 +23,10 @@ constexpr int kPhases = 8;
 bool is_octree(int solver) {
     return solver == NBG_BARNES_HUT || solver == NBG_BARNES_HUT_QUADRUPOLE || solver == NBG_IMPLICIT_FMM;
 }
-// Implicit FMM scratch: waves, node-list and body-list capacities per wave.
-constexpr int kFmmWaves = 2048, kFmmCapA = 8192, kFmmCapE = 32768;
-constexpr int64_t kFmmWaveWords = 4 * (int64_t)kFmmCapA + 2 * (int64_t)kFmmCapE;
+// Implicit FMM: waves of a traversal step, the seed queues' capacity, and
+// the list pool per step parity (words per body plus a floor).
+constexpr int kFmmWaves = 2048, kFmmSeedCap = 4096;
+constexpr int64_t kFmmPoolPerBody = 80, kFmmPoolFloor = (int64_t)1 << 22, kFmmPoolMax = (int64_t)1 << 31;
 bool is_quad(int solver) { return solver == NBG_BARNES_HUT_QUADRUPOLE || solver == NBG_LINEAR_BVH_QUADRUPOLE; }
 constexpr int kEventSets = 256;
 
@@ -75,10 +76,13 @@ struct nbg_sys {
     int64_t quad_slots = 0;
     // implicit FMM work (allocated for that solver): start flags by slot,
     // starting nodes, leaf list, counters, per-wave list scratch
-    uint32_t *fmm_flag = nullptr, *fmm_leaves = nullptr, *fmm_misc = nullptr, *fmm_scratch = nullptr;
+    // implicit FMM work (allocated for that solver): frontier items and list
+    // pool (two halves each), seed queues, counters, starting nodes
+    uint32_t *fmm_items = nullptr, *fmm_pool = nullptr, *fmm_seed = nullptr, *fmm_misc = nullptr;
     int32_t *fmm_starts = nullptr;
-    int64_t fmm_slots = 0;
-    int fmm_waves = 0;  // persistent waves of the FMM kernel (each owns kFmmWaveWords of scratch)
+    int64_t fmm_item_cap = 0, fmm_pool_cap = 0;
+    int fmm_seed_cap = 0;
+    int fmm_shift = 0;  // capacity doublings after passes that outgrew the pool
     // small
     float *bbox_partials = nullptr, *bbox = nullptr;
     // [1] err, [2] node count, [4..16) counts: pp, np, then the FAST walk's
@@ -163,36 +167,41 @@ int ensure_quad(nbg_sys *s) {
 }
 
 void free_fmm(nbg_sys *s) {
-    void *ptrs[] = {s->fmm_flag, s->fmm_leaves, s->fmm_misc, s->fmm_scratch, s->fmm_starts};
+    void *ptrs[] = {s->fmm_items, s->fmm_pool, s->fmm_seed, s->fmm_misc, s->fmm_starts};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    s->fmm_flag = s->fmm_leaves = s->fmm_misc = s->fmm_scratch = nullptr;
+    s->fmm_items = s->fmm_pool = s->fmm_seed = s->fmm_misc = nullptr;
     s->fmm_starts = nullptr;
-    s->fmm_slots = 0;
-    s->fmm_waves = 0;
+    s->fmm_item_cap = s->fmm_pool_cap = 0;
 }
 
 // Allocates the implicit FMM's work buffers (no-op for other solvers or when
 // the current ones are large enough).
 int ensure_fmm(nbg_sys *s) {
     if (s->prm.solver != NBG_IMPLICIT_FMM) return NBG_OK;
-    // one wave per leaf at most: small systems get a small scratch
-    const int waves = (int)std::min<int64_t>(kFmmWaves, s->cap > 0 ? s->cap : 1);
-    if (s->fmm_scratch && s->fmm_slots >= s->ob.cap && s->fmm_waves >= waves) return NBG_OK;
+    // a step's frontier holds disjoint non-empty subtrees: at most one item per body
+    const int64_t n = s->cap > 0 ? s->cap : 1;
+    const int64_t item_cap = n + 64;
+    const int64_t pool_cap = std::min(kFmmPoolMax, (kFmmPoolPerBody * n + kFmmPoolFloor) << s->fmm_shift);
+    const int seed_cap = kFmmSeedCap << std::min(s->fmm_shift, 4);
+    if (s->fmm_items && s->fmm_item_cap >= item_cap && s->fmm_pool_cap >= pool_cap && s->fmm_seed_cap >= seed_cap)
+        return NBG_OK;
     if (s->stream) NBG_HIP(hipStreamSynchronize(s->stream));
     free_fmm(s);
     int r = NBG_OK;
     auto A = [&](auto **ptr, int64_t count) {
         if (r == NBG_OK) r = dalloc(ptr, count);
     };
-    A(&s->fmm_flag, s->ob.cap); A(&s->fmm_leaves, s->cap > 0 ? s->cap : 1); A(&s->fmm_misc, 64);
-    A(&s->fmm_starts, 512); A(&s->fmm_scratch, (int64_t)waves * kFmmWaveWords);
+    A(&s->fmm_items, 2 * item_cap * kFmmItemWords); A(&s->fmm_pool, 2 * pool_cap);
+    A(&s->fmm_seed, (int64_t)kFmmStartCap * 3 * seed_cap); A(&s->fmm_misc, 64);
+    A(&s->fmm_starts, kFmmStartCap);
     if (r != NBG_OK) {
         free_fmm(s);
         return r;
     }
-    s->fmm_slots = s->ob.cap;
-    s->fmm_waves = waves;
+    s->fmm_item_cap = item_cap;
+    s->fmm_pool_cap = pool_cap;
+    s->fmm_seed_cap = seed_cap;
     return NBG_OK;
 }
 
@@ -313,16 +322,23 @@ int enqueue_force_pass(nbg_sys *s) {
         fa.perm = s->idx;
         fa.ax = s->ax; fa.ay = s->ay; fa.az = s->az;
         fa.theta = p.theta; fa.G = p.G; fa.eps2 = p.eps2;
-        fa.start_flag = s->fmm_flag;
         fa.starts = s->fmm_starts;
-        fa.nstarts = s->fmm_misc; fa.nleaves = s->fmm_misc + 1; fa.next = s->fmm_misc + 2;
-        fa.leaves = s->fmm_leaves;
-        fa.scratch = s->fmm_scratch;
-        fa.wave_words = kFmmWaveWords;
-        fa.cap_a = kFmmCapA; fa.cap_e = kFmmCapE; fa.waves = s->fmm_waves;
+        fa.ctr = s->fmm_misc;
+        fa.peak = s->fmm_misc + 16;
+        fa.nstarts = s->fmm_misc + 20;
+        fa.items[0] = s->fmm_items;
+        fa.items[1] = s->fmm_items + s->fmm_item_cap * kFmmItemWords;
+        fa.pool[0] = s->fmm_pool;
+        fa.pool[1] = s->fmm_pool + s->fmm_pool_cap;
+        fa.item_cap = s->fmm_item_cap;
+        fa.pool_cap = s->fmm_pool_cap;
+        fa.seed_scratch = s->fmm_seed;
+        fa.cap_a = s->fmm_seed_cap;
+        fa.waves = kFmmWaves;
+        fa.steps = p.max_depth + 4;  // a step per tree level below the shallowest starting node
         fa.counts = s->counts();
         fa.err = s->err_word();
-        launch_fmm(fa, s->leaf_start, st);
+        launch_fmm(fa, st);
     } else {
         WalkArgs wa = walk_args(s);
         const int stats = s->walk_stats ? 2 : (s->counting ? 1 : 0);
@@ -395,6 +411,24 @@ int sync_and_check(nbg_sys *s) {
     }
     fold_timing(s);
     return NBG_OK;
+}
+
+// A force pass.  The implicit FMM's list pool is sized by an estimate; a pass
+// that outgrows it (error bit 16 alone) is repeated with twice the capacity
+// (bodies and velocities are untouched until the update), at the cost of one
+// synchronisation per pass.
+int force_pass(nbg_sys *s) {
+    if (s->prm.solver != NBG_IMPLICIT_FMM) return enqueue_force_pass(s);
+    for (;;) {
+        if (int r = enqueue_force_pass(s)) return r;
+        NBG_HIP(hipStreamSynchronize(s->stream));
+        uint32_t err = 0;
+        NBG_HIP(hipMemcpy(&err, s->err_word(), sizeof err, hipMemcpyDeviceToHost));
+        if (err != 16u || s->fmm_pool_cap >= kFmmPoolMax) return NBG_OK;  // reported by sync_and_check
+        const uint32_t zero = 0;
+        NBG_HIP(hipMemcpy(s->err_word(), &zero, sizeof zero, hipMemcpyHostToDevice));
+        ++s->fmm_shift;
+    }
 }
 
 }  // namespace
@@ -561,7 +595,7 @@ int nbg_sys_update_accelerations(nbg_sys *s, float *ax, float *ay, float *az) {
     if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
     if (!s->have_state) return fail(NBG_ERR_STATE, "set_state has not been called");
     DeviceGuard g(s->ctx->device);
-    if (int r = enqueue_force_pass(s)) return r;
+    if (int r = force_pass(s)) return r;
     mark(s, kPhases);  // a force pass alone has an empty update phase
     if (int r = sync_and_check(s)) return r;
     s->have_forces = true;
@@ -659,7 +693,7 @@ int nbg_sys_step(nbg_sys *s, int32_t nsteps) {
     if (!s->have_state) return fail(NBG_ERR_STATE, "set_state has not been called");
     DeviceGuard g(s->ctx->device);
     for (int i = 0; i < nsteps; ++i) {
-        if (int r = enqueue_force_pass(s)) return r;
+        if (int r = force_pass(s)) return r;
         launch_update(s->x, s->y, s->z, s->vx, s->vy, s->vz, s->ax, s->ay, s->az, s->prm.dt, s->n, s->stream);
         mark(s, kPhases);
         s->tree_valid = false;  // positions moved after the tree was built
@@ -707,6 +741,15 @@ int nbg_sys_walk_stats(nbg_sys *s, uint64_t *out4) {
     if (!s->have_forces) return fail(NBG_ERR_STATE, "no force pass has run");
     DeviceGuard g(s->ctx->device);
     NBG_HIP(hipStreamSynchronize(s->stream));
+    if (s->prm.solver == NBG_IMPLICIT_FMM) {
+        uint32_t pk[4] = {0, 0, 0, 0};
+        if (s->n > 0 && s->fmm_misc) NBG_HIP(hipMemcpy(pk, s->fmm_misc + 16, sizeof pk, hipMemcpyDeviceToHost));
+        out4[0] = pk[0];
+        out4[1] = pk[1];
+        out4[2] = pk[2];
+        out4[3] = (uint64_t)s->fmm_pool_cap;
+        return NBG_OK;
+    }
     unsigned long long c[6];
     NBG_HIP(hipMemcpy(c, s->counts(), sizeof c, hipMemcpyDeviceToHost));
     for (int i = 0; i < 4; ++i) out4[i] = c[2 + i];

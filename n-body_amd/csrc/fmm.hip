@@ -1,5 +1,5 @@
 // Implicit fast multipole method (MultipoleImplicitFMMSolver<1>,
-// Solvers/ImplicitFMMSolver.h:14-33) on the octree of the Barnes-Hut build.
+// Solvers/ImplicitFMMSolver.h:14-36) on the octree of the Barnes-Hut build.
 //
 // The reference (ImplicitDualTraversalSolver::updateAccelerations,
 // Solvers/ImplicitDualTraversalSolver.h:46-68) breaks the tree into about 256
@@ -17,18 +17,19 @@
 // Descent/lockstepDualTreeImplicitField.h:18-170,
 // Descent/passiveTreeImplicitField.h:13-78).
 //
-// Here one wavefront serves one passive leaf L: it replays the traversal
-// along the path from L's starting node down to L with the lists held in
-// per-wave scratch (lanes classify 64 list entries at a time), so the
-// interaction set of every body in L is the reference's: the same criteria
-// (SideLengthOverDistance, node-node and particle-node forms, in FP32 with
-// the reference's operation order) take the same decisions.  Far-field
-// contributions are kept as per-lane partial local fields (translation is
-// linear, so partials are translated separately and summed once at L).
-// Work above L is repeated by every leaf below it; interaction counts count
-// each passive node's far-field events once (by the wave of the leaf that
-// holds the node's first body).  Empty octants, which the reference keeps as
-// body-less nodes, contribute nothing and have no record here.
+// Here the traversal runs breadth-first, one launch per traversal step: a
+// step takes the frontier of passive nodes (each with its local field, its
+// active-node list and its list of close bodies) and produces the next
+// frontier, the children of its internal nodes, which share their parent's
+// two lists.  One wavefront serves one passive node; lanes classify 64 list
+// entries at a time with the reference's criteria (SideLengthOverDistance,
+// node-node and particle-node forms, FP32 in the reference's operation
+// order), so every decision, and with it every interaction count, is the
+// reference's.  Lists live in a pool per step parity (a step reads the one
+// its predecessor wrote); a wave reserves its output lists with one atomic
+// add.  Far-field contributions are per-lane partial local fields summed once
+// per node.  Empty octants, which the reference keeps as body-less nodes,
+// contribute nothing and have no record here.
 
 #include "device.hpp"
 #include "launch.hpp"
@@ -83,9 +84,10 @@ __device__ void heap_pop(HeapEnt *h, int len) {
 
 constexpr int kFmmStarts = 256;
 constexpr int kHeapCap = kFmmStarts + 8;
+static_assert(kHeapCap <= kFmmStartCap, "starting-node buffer too small");
 
 __global__ __launch_bounds__(64) void k_fmm_starts(const OctRec *__restrict__ nodes, const uint32_t *__restrict__ misc,
-                                                   uint32_t *flag, int32_t *starts, uint32_t *nstarts, uint32_t *err) {
+                                                   int32_t *starts, uint32_t *nstarts, uint32_t *err) {
     __shared__ HeapEnt q[kHeapCap];
     if (threadIdx.x != 0) return;
     const int64_t T = misc[0];
@@ -112,20 +114,8 @@ __global__ __launch_bounds__(64) void k_fmm_starts(const OctRec *__restrict__ no
     }
     int k = 0;
     for (int i = 0; i < len; ++i)
-        if (q[i].slot >= 0) {
-            flag[q[i].slot] = 1u;
-            starts[k++] = q[i].slot;
-        }
+        if (q[i].slot >= 0) starts[k++] = q[i].slot;
     *nstarts = (uint32_t)k;
-}
-
-// Leaf slots, one per leaf (order immaterial).
-__global__ __launch_bounds__(256) void k_fmm_leaves(const int32_t *__restrict__ leaf_start, int64_t n, uint32_t *list,
-                                                    uint32_t *count) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int s = leaf_start[i];
-        if (s >= 0) list[atomicAdd(count, 1u)] = (uint32_t)s;
-    }
 }
 
 // ---------------------------------------------------------------- criteria
@@ -157,8 +147,8 @@ __device__ __forceinline__ bool fmm_particle_far(float x, float y, float z, floa
     return !in;
 }
 
-// A per-lane partial local field (MultipoleAcceleration<2>: vector and
-// symmetric gradient xx xy xz yy yz zz), without the factor G.
+// A local field (MultipoleAcceleration<2>: vector and symmetric gradient
+// xx xy xz yy yz zz), without the factor G.
 struct LocalF {
     float ax, ay, az, t0, t1, t2, t3, t4, t5;
 };
@@ -183,11 +173,12 @@ __device__ __forceinline__ void fmm_to_local(float sx, float sy, float sz, float
     F.t5 = __builtin_fmaf(Rz * Rz, m5, F.t5 - m3);
 }
 
-// MultipoleAcceleration::translated (MultipoleAcceleration.h:49-70): a += T d.
-__device__ __forceinline__ void fmm_translate(LocalF &F, float dx, float dy, float dz) {
-    F.ax = __builtin_fmaf(F.t0, dx, __builtin_fmaf(F.t1, dy, __builtin_fmaf(F.t2, dz, F.ax)));
-    F.ay = __builtin_fmaf(F.t1, dx, __builtin_fmaf(F.t3, dy, __builtin_fmaf(F.t4, dz, F.ay)));
-    F.az = __builtin_fmaf(F.t2, dx, __builtin_fmaf(F.t4, dy, __builtin_fmaf(F.t5, dz, F.az)));
+// MultipoleAcceleration::at (MultipoleAcceleration.h:33-47) and ::translated
+// (:49-70): a + T d.
+__device__ __forceinline__ void fmm_at(const LocalF &F, float dx, float dy, float dz, float &x, float &y, float &z) {
+    x = __builtin_fmaf(F.t0, dx, __builtin_fmaf(F.t1, dy, __builtin_fmaf(F.t2, dz, F.ax)));
+    y = __builtin_fmaf(F.t1, dx, __builtin_fmaf(F.t3, dy, __builtin_fmaf(F.t4, dz, F.ay)));
+    z = __builtin_fmaf(F.t2, dx, __builtin_fmaf(F.t4, dy, __builtin_fmaf(F.t5, dz, F.az)));
 }
 
 // Makes the wave's earlier global-memory list writes visible to its later
@@ -203,7 +194,8 @@ __device__ __forceinline__ float wave_fsum(float v) {
     return v;
 }
 
-// Appends v for every lane with `want`; returns the new (uniform) length.
+// Appends v for every lane with `want`; writes stay below `cap` (a longer
+// result sets `over`); returns the new (uniform) length, at most cap.
 __device__ __forceinline__ int wave_append(uint32_t *list, int len, int cap, bool want, uint32_t v, bool &over) {
     const uint64_t b = __ballot(want);
     if (want) {
@@ -215,9 +207,9 @@ __device__ __forceinline__ int wave_append(uint32_t *list, int len, int cap, boo
     return nl < cap ? nl : cap;
 }
 
-// Appends the non-empty children (contiguous slots c0 .. c0 + k - 1) of every
-// lane's node; returns the new length.  A child range outside the node array
-// (an inconsistent tree) sets `bad` and appends nothing for that lane.
+// Appends the children (contiguous slots c0 .. c0 + k - 1) of every lane's
+// node; returns the new length.  A child range outside the node array (an
+// inconsistent tree) sets `bad` and appends nothing for that lane.
 __device__ __forceinline__ int wave_append_children(uint32_t *list, int len, int cap, int c0, int k, int64_t max_nodes,
                                                     bool &over, bool &bad) {
     if (k > 0 && (c0 < 1 || k > 8 || (int64_t)c0 + k > max_nodes)) {
@@ -231,6 +223,17 @@ __device__ __forceinline__ int wave_append_children(uint32_t *list, int len, int
     const int nl = len + (int)__shfl(inc, kWave - 1, kWave);
     if (nl > cap) over = true;
     return nl < cap ? nl : cap;
+}
+
+// Reserves `words` of a pool (wave-uniform); -1 when it does not fit.
+__device__ __forceinline__ int64_t pool_reserve(uint32_t *counter, int64_t words, int64_t cap, uint32_t *peak) {
+    uint32_t off = 0;
+    if (lane_id() == 0 && words > 0) {
+        off = atomicAdd(counter, (uint32_t)words);
+        atomicMax(peak, off + (uint32_t)words);
+    }
+    const int64_t o = (int64_t)__shfl(off, 0, kWave);
+    return (words == 0 || o + words <= cap) ? o : -1;
 }
 
 // Descent::activeTree from `start` for the lanes in `lanes` (the octree's
@@ -283,11 +286,96 @@ __device__ void fmm_walk_from(const FmmArgs &a, int start, uint64_t lanes, FastW
     }
 }
 
-constexpr int kMaxPath = 40;
+// A frontier item: the passive node, its active list and close-body list
+// (offsets into the step's pool), its local field at the node's centre and
+// whether the active list holds child groups (first child | count << 28: the
+// children of the parent's close internal active nodes, expanded on reading)
+// or node slots.
+enum { kItP = 0, kItAoff = 1, kItAlen = 2, kItEoff = 3, kItElen = 4, kItF = 5, kItGroups = 14 };
 
+__device__ __forceinline__ void write_item(uint32_t *it, int P, uint32_t aoff, uint32_t alen, uint32_t eoff,
+                                           uint32_t elen, const LocalF &F, bool groups) {
+    uint4 *q = (uint4 *)it;
+    q[0] = make_uint4((uint32_t)P, aoff, alen, eoff);
+    q[1] = make_uint4(elen, __float_as_uint(F.ax), __float_as_uint(F.ay), __float_as_uint(F.az));
+    q[2] = make_uint4(__float_as_uint(F.t0), __float_as_uint(F.t1), __float_as_uint(F.t2), __float_as_uint(F.t3));
+    q[3] = make_uint4(__float_as_uint(F.t4), __float_as_uint(F.t5), groups ? 1u : 0u, 0u);
+}
+
+// Entry idx of an active list: a node slot, or (groups) child idx % 8 of
+// group idx / 8; -1 for an empty group position.
+__device__ __forceinline__ int active_entry(const uint32_t *list, uint32_t idx, bool groups) {
+    if (!groups) return (int)list[idx];
+    const uint32_t g = list[idx >> 3];
+    const uint32_t k = idx & 7u;
+    return k < (g >> 28) ? (int)(g & 0x0FFFFFFFu) + (int)k : -1;
+}
+
+// ---------------------------------------------------------------- seeds
+// Descent::balancedLockstepDualTreeImplicitField: for each starting node S,
+// the breadth-first descent of the active tree while the criterion asks to
+// descend the active node only; the nodes where it stops are S's active list.
+__global__ __launch_bounds__(64) void k_fmm_seed(FmmArgs a) {
+    const int w = blockIdx.x;
+    if (w >= (int)*a.nstarts) return;
+    const int lane = lane_id();
+    const OctRec *__restrict__ nodes = a.nodes;
+    const float4 *__restrict__ cen = a.centre;
+    const float theta2 = a.theta * a.theta;
+    uint32_t *base = a.seed_scratch + (int64_t)w * 3 * a.cap_a;
+    uint32_t *Q[2] = {base, base + a.cap_a};
+    uint32_t *EQ = base + 2 * a.cap_a;
+    const int S = uniform_i32(a.starts[w]);
+    bool over = false, bad = false;
+    if (S < 0 || S >= a.max_nodes) bad = true;
+    const float4 cS = cen[bad ? 0 : S];
+    int cur = bad ? 0 : 1, eq = 0, qi = 0, levels = 0;
+    if (lane == 0) Q[0][0] = 0u;
+    wave_mem_sync();
+    while (cur > 0) {
+        if (++levels > 128) {  // deeper than any octree: an inconsistent tree
+            bad = true;
+            break;
+        }
+        int nxt = 0;
+        for (int b = 0; b < cur; b += kWave) {
+            const bool has = b + lane < cur;
+            const int an = has ? (int)Q[qi][b + lane] : 0;
+            const OctRec ar = nodes[an];
+            const int r = has ? fmm_recommend(ar.com, cen[an], cS, theta2) : 0;
+            const bool down = has && r == 1 && ar.topo.z >= 0;
+            eq = wave_append(EQ, eq, a.cap_a, has && !down, (uint32_t)an, over);
+            nxt = wave_append_children(Q[qi ^ 1], nxt, a.cap_a, ar.topo.z, down ? oct_nchild(ar) : 0, a.max_nodes,
+                                       over, bad);
+        }
+        qi ^= 1;
+        cur = nxt;
+        wave_mem_sync();
+    }
+    bad = __ballot(bad) != 0;
+    uint32_t *ctr = a.ctr;  // step 0's input set
+    const int64_t off = bad || over ? -1 : pool_reserve(ctr + 2, eq, a.pool_cap, a.peak);
+    if (off < 0) {
+        if (lane == 0) atomicOr(a.err, bad ? 2u : 16u);
+        return;
+    }
+    for (int b = 0; b < eq; b += kWave)
+        if (b + lane < eq) a.pool[0][off + b + lane] = EQ[b + lane];
+    uint32_t slot = 0;
+    if (lane == 0) slot = atomicAdd(ctr + 0, 1u);
+    slot = __shfl(slot, 0, kWave);
+    if (lane == 0) {
+        const LocalF Z = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        write_item(a.items[0] + (int64_t)slot * kFmmItemWords, S, (uint32_t)off, (uint32_t)eq, 0u, 0u, Z, false);
+    }
+}
+
+// ---------------------------------------------------------------- one traversal step
+// Counter sets rotate over three (input, output, next output): step k reads
+// set k % 3 ({items, next item, pool words, -}), fills set (k + 1) % 3 and
+// clears set (k + 2) % 3, which step k - 1 read.
 template <bool SOFT>
-__global__ __launch_bounds__(64) void k_fmm_leaf(FmmArgs a) {
-    __shared__ int s_path[kMaxPath];
+__global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
     __shared__ int s_grp[kStackOct];
     __shared__ uint64_t s_msk[kStackOct];
 
@@ -295,170 +383,185 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(FmmArgs a) {
     const OctRec *__restrict__ nodes = a.nodes;
     const float4 *__restrict__ cen = a.centre;
     const float theta = a.theta, theta2 = a.theta * a.theta, eps2 = a.eps2;
-    uint32_t *base = a.scratch + (int64_t)blockIdx.x * a.wave_words;
-    uint32_t *A[2] = {base, base + a.cap_a};
-    uint32_t *LA = base + 2 * a.cap_a, *NA = base + 3 * a.cap_a;
-    uint32_t *E[2] = {base + 4 * a.cap_a, base + 4 * a.cap_a + a.cap_e};
+    uint32_t *ci = a.ctr + 4 * (k % 3), *co = a.ctr + 4 * ((k + 1) % 3), *cr = a.ctr + 4 * ((k + 2) % 3);
+    if (blockIdx.x == 0 && lane < 4) cr[lane] = 0u;
+    const uint32_t *__restrict__ itin = a.items[k & 1];
+    uint32_t *itout = a.items[(k + 1) & 1];
+    const uint32_t *pin = a.pool[k & 1];
+    uint32_t *pout = a.pool[(k + 1) & 1];
+    // a step that overflowed the frontier counted past the capacity (and flagged it)
+    const int nin = (int)(ci[0] < (uint64_t)a.item_cap ? ci[0] : (uint32_t)a.item_cap);
     uint64_t c_pp = 0, c_np = 0, c_pn = 0, c_nn = 0;
     bool fail = false, over = false;
 
     for (;;) {
         int job = 0;
-        if (lane == 0) job = (int)atomicAdd(a.next, 1u);
+        if (lane == 0) job = (int)atomicAdd(ci + 1, 1u);
         job = uniform_i32(__shfl(job, 0, kWave));
-        if (job >= (int)*a.nleaves) break;
-        const int L = uniform_i32((int)a.leaves[job]);
-        const OctRec lr = nodes[L];
-        // path from the starting node down to L
-        int d = 0;
-        {
-            int v = L;
-            while (v >= 0 && v < a.max_nodes && d < kMaxPath) {
-                if (lane == 0) s_path[d] = v;
-                ++d;
-                if (a.start_flag[v]) break;
-                v = nodes[v].topo.w;
-            }
-            if (v < 0 || v >= a.max_nodes || d >= kMaxPath) { fail = true; break; }
-            if (lr.topo.x < 0 || lr.topo.y < 0 || (int64_t)lr.topo.x + lr.topo.y > a.n) { fail = true; break; }
-        }
-        __builtin_amdgcn_wave_barrier();
-        const int S = uniform_i32(s_path[d - 1]);
-        // breadth-first active list for S (balancedLockstep), kept in A[0]
-        int cur = 1, eq = 0;
-        uint32_t *Q[2] = {LA, NA};  // breadth-first queues
-        if (lane == 0) Q[0][0] = 0u;
-        wave_mem_sync();
-        const float4 cS = cen[S];
-        int qi = 0, levels = 0;
-        while (cur > 0) {
-            if (++levels > kMaxPath + 32) {  // deeper than any octree: an inconsistent tree
-                fail = true;
-                break;
-            }
-            int nxt = 0;
-            for (int b = 0; b < cur; b += kWave) {
-                const bool has = b + lane < cur;
-                const int an = has ? (int)Q[qi][b + lane] : 0;
-                const OctRec ar = nodes[an];
-                const int r = has ? fmm_recommend(ar.com, cen[an], cS, theta2) : 0;
-                const bool down = has && r == 1 && ar.topo.z >= 0;
-                eq = wave_append(A[0], eq, a.cap_a, has && !down, (uint32_t)an, over);
-                nxt = wave_append_children(Q[qi ^ 1], nxt, a.cap_a, ar.topo.z, down ? oct_nchild(ar) : 0, a.max_nodes,
-                                           over, fail);
-            }
-            qi ^= 1;
-            cur = nxt;
-            wave_mem_sync();
-        }
-        if (__ballot(fail)) {
+        if (job >= nin) break;
+        const uint32_t *it = itin + (int64_t)job * kFmmItemWords;
+        const int P = uniform_i32((int)it[kItP]);
+        const uint32_t aoff = (uint32_t)uniform_i32((int)it[kItAoff]), alen = (uint32_t)uniform_i32((int)it[kItAlen]);
+        const uint32_t eoff = (uint32_t)uniform_i32((int)it[kItEoff]), elen = (uint32_t)uniform_i32((int)it[kItElen]);
+        const bool groups = uniform_i32((int)it[kItGroups]) != 0;
+        if (P < 0 || P >= a.max_nodes || (int64_t)aoff + alen > a.pool_cap || (int64_t)eoff + elen > a.pool_cap) {
             fail = true;
             break;
         }
-        int ai = 0, na = eq, ne = 0, ei = 0;
-        LocalF F = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int k = d - 1; k >= 0; --k) {
-            const int P = uniform_i32(s_path[k]);
-            const float4 cP = cen[P];
-            const bool leaf = k == 0;
-            const bool count_far = nodes[P].topo.x == lr.topo.x;  // one wave counts P's far-field events
-            // lockstep partition of the active list against P
-            int nla = 0, nna = 0;
-            for (int b = 0; b < na; b += kWave) {
-                const bool has = b + lane < na;
-                const int an = has ? (int)A[ai][b + lane] : 0;
-                const OctRec ar = nodes[an];
-                const int r = has ? fmm_recommend(ar.com, cen[an], cP, theta2) : 1;
-                const bool far = has && r == 0;
-                if (far) fmm_to_local(ar.com.x, ar.com.y, ar.com.z, ar.com.w, cP, eps2, F);
-                if (count_far) c_nn += __popcll(__ballot(far));
-                const bool isleaf = ar.topo.z < 0;
-                nla = wave_append(LA, nla, a.cap_a, has && !far && isleaf, (uint32_t)an, over);
-                nna = wave_append(NA, nna, a.cap_a, has && !far && !isleaf, (uint32_t)an, over);
+        const OctRec pr = nodes[P];
+        const float4 cP = cen[P];
+        const bool leaf = pr.topo.z < 0;
+        if (pr.topo.x < 0 || pr.topo.y < 0 || (int64_t)pr.topo.x + pr.topo.y > a.n) {
+            fail = true;
+            break;
+        }
+        const uint32_t *alist = pin + aoff;
+        const uint32_t vlen = groups ? 8u * alen : alen;
+        LocalF Fl = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        // pass 1 over the active list: far nodes into the local field; sizes of what stays
+        int nleafc = 0, ninner = 0;
+        uint32_t lbod = 0;
+        for (uint32_t b = 0; b < vlen; b += kWave) {
+            int an = b + lane < vlen ? active_entry(alist, b + lane, groups) : -1;
+            if (an >= a.max_nodes) { fail = true; an = -1; }
+            const bool has = an >= 0;
+            const OctRec ar = nodes[has ? an : 0];
+            const int r = has ? fmm_recommend(ar.com, cen[an], cP, theta2) : 1;
+            const bool far = has && r == 0;
+            if (far) fmm_to_local(ar.com.x, ar.com.y, ar.com.z, ar.com.w, cP, eps2, Fl);
+            c_nn += __popcll(__ballot(far));
+            const bool lf = has && !far && ar.topo.z < 0;
+            nleafc += __popcll(__ballot(lf));
+            ninner += __popcll(__ballot(has && !far && ar.topo.z >= 0));
+            if (lf) lbod += (uint32_t)ar.topo.y;
+        }
+        // pool layout: a leaf P keeps its close active nodes (walk starts);
+        // an internal P its close internal ones as child groups, then its
+        // close leaves; then the close bodies
+        const int nclose = nleafc + ninner;
+        const int64_t ecap = (int64_t)elen + (leaf ? 0 : (int64_t)wave_sum(lbod));
+        const int64_t off = pool_reserve(co + 2, nclose + ecap, a.pool_cap, a.peak);
+        if (off < 0) {
+            over = true;
+            break;
+        }
+        uint32_t *C = pout + off, *LA = C + ninner, *E2 = C + nclose;
+        // pass 2: write what stays
+        int nc = 0, ng = 0, nl = 0;
+        for (uint32_t b = 0; b < vlen; b += kWave) {
+            int an = b + lane < vlen ? active_entry(alist, b + lane, groups) : -1;
+            if (an >= a.max_nodes) an = -1;
+            const bool has = an >= 0;
+            const OctRec ar = nodes[has ? an : 0];
+            const int r = has ? fmm_recommend(ar.com, cen[an], cP, theta2) : 1;
+            const bool close = has && r != 0;
+            if (leaf) {
+                nc = wave_append(C, nc, nclose, close, (uint32_t)an, over);
+            } else {
+                const bool inner = close && ar.topo.z >= 0;
+                const int nch = inner ? oct_nchild(ar) : 0;
+                if (inner && (ar.topo.z < 1 || nch < 1 || (int64_t)ar.topo.z + nch > a.max_nodes)) fail = true;
+                const uint32_t g = (uint32_t)ar.topo.z | ((uint32_t)nch << 28);
+                ng = wave_append(C, ng, ninner, inner, g, over);
+                nl = wave_append(LA, nl, nleafc, close && !inner, (uint32_t)an, over);
             }
-            // close bodies of the passive-tree traversals against P
-            int nne = 0;
-            for (int b = 0; b < ne; b += kWave) {
-                const bool has = b + lane < ne;
-                const int e = has ? (int)E[ei][b + lane] : 0;
-                const float4 bd = a.bodies[e];
-                const bool far = has && fmm_particle_far(bd.x, bd.y, bd.z, cP, theta);
-                if (far) fmm_to_local(bd.x, bd.y, bd.z, bd.w, cP, eps2, F);
-                if (count_far) c_pn += __popcll(__ballot(far));
-                nne = wave_append(E[ei ^ 1], nne, a.cap_e, has && !far, (uint32_t)e, over);
-            }
-            ei ^= 1;
-            ne = nne;
-            wave_mem_sync();
-            if (!leaf) {
-                // bodies of the active leaves start a passive-tree traversal at P
-                for (int j = 0; j < nla; ++j) {
-                    const OctRec lf = nodes[uniform_i32((int)LA[j])];
-                    if (lf.topo.x < 0 || lf.topo.y < 0 || (int64_t)lf.topo.x + lf.topo.y > a.n) {
-                        fail = true;
-                        break;
-                    }
-                    for (int b = 0; b < lf.topo.y; b += kWave) {
-                        const bool has = b + lane < lf.topo.y;
-                        const int e = lf.topo.x + b + lane;
-                        const float4 bd = a.bodies[has ? e : lf.topo.x];
-                        const bool far = has && fmm_particle_far(bd.x, bd.y, bd.z, cP, theta);
-                        if (far) fmm_to_local(bd.x, bd.y, bd.z, bd.w, cP, eps2, F);
-                        if (count_far) c_pn += __popcll(__ballot(far));
-                        ne = wave_append(E[ei], ne, a.cap_e, has && !far, (uint32_t)e, over);
-                    }
+        }
+        // the close bodies handed down by the passive-tree traversals
+        int ne = 0;
+        for (uint32_t b = 0; b < elen; b += kWave) {
+            bool has = b + lane < elen;
+            const int e = has ? (int)pin[eoff + b + lane] : 0;
+            if (has && (e < 0 || (int64_t)e >= a.n)) { fail = true; has = false; }
+            const float4 bd = a.bodies[has ? e : 0];
+            const bool far = has && fmm_particle_far(bd.x, bd.y, bd.z, cP, theta);
+            if (far) fmm_to_local(bd.x, bd.y, bd.z, bd.w, cP, eps2, Fl);
+            c_pn += __popcll(__ballot(far));
+            ne = wave_append(E2, ne, (int)ecap, has && !far, (uint32_t)e, over);
+        }
+        wave_mem_sync();
+        if (!leaf) {
+            // bodies of the close active leaves start a passive-tree traversal at P
+            for (int j = 0; j < nl; ++j) {
+                const OctRec lf = nodes[uniform_i32((int)LA[j])];
+                if (lf.topo.x < 0 || lf.topo.y < 0 || (int64_t)lf.topo.x + lf.topo.y > a.n) {
+                    fail = true;
+                    break;
                 }
-                // children of the internal active nodes form the next list
-                int nxt = 0;
-                for (int b = 0; b < nna; b += kWave) {
-                    const bool has = b + lane < nna;
-                    const OctRec ar = nodes[has ? (int)NA[b + lane] : 0];
-                    nxt = wave_append_children(A[ai ^ 1], nxt, a.cap_a, ar.topo.z, has ? oct_nchild(ar) : 0, a.max_nodes,
-                                               over, fail);
+                for (int b = 0; b < lf.topo.y; b += kWave) {
+                    const bool has = b + lane < lf.topo.y;
+                    const int e = lf.topo.x + b + lane;
+                    const float4 bd = a.bodies[has ? e : lf.topo.x];
+                    const bool far = has && fmm_particle_far(bd.x, bd.y, bd.z, cP, theta);
+                    if (far) fmm_to_local(bd.x, bd.y, bd.z, bd.w, cP, eps2, Fl);
+                    c_pn += __popcll(__ballot(far));
+                    ne = wave_append(E2, ne, (int)ecap, has && !far, (uint32_t)e, over);
                 }
-                ai ^= 1;
-                na = nxt;
-                const float4 cN = cen[uniform_i32(s_path[k - 1])];
-                fmm_translate(F, cN.x - cP.x, cN.y - cP.y, cN.z - cP.z);
-                wave_mem_sync();
-                continue;
             }
-            // passive leaf: near field and the local field for its bodies
-            LocalF T;
-            T.ax = wave_fsum(F.ax); T.ay = wave_fsum(F.ay); T.az = wave_fsum(F.az);
-            T.t0 = wave_fsum(F.t0); T.t1 = wave_fsum(F.t1); T.t2 = wave_fsum(F.t2);
-            T.t3 = wave_fsum(F.t3); T.t4 = wave_fsum(F.t4); T.t5 = wave_fsum(F.t5);
-            wave_mem_sync();
-            for (int b = 0; b < lr.topo.y; b += kWave) {
-                const bool has = b + lane < lr.topo.y;
-                const int me = lr.topo.x + b + lane;
-                const float4 p = a.bodies[has ? me : lr.topo.x];
-                const uint64_t lanes = __ballot(has);
-                FastWave W;
-                W.px = p.x; W.py = p.y; W.pz = p.z;
-                W.rx = 0.0f; W.ry = 0.0f; W.rz = 0.0f;
-                W.pp = 0; W.np = 0;
-                W.visits = 0; W.lanes = 0; W.leaf_visits = 0; W.leaf_lanes = 0;
-                W.G = a.G;
-                for (int j = 0; j < nla; ++j) fmm_walk_from<SOFT>(a, uniform_i32((int)LA[j]), lanes, W, s_grp, s_msk, fail);
-                for (int j = 0; j < nna; ++j) fmm_walk_from<SOFT>(a, uniform_i32((int)NA[j]), lanes, W, s_grp, s_msk, fail);
+        }
+        LocalF F;
+        F.ax = __uint_as_float(it[kItF + 0]) + wave_fsum(Fl.ax);
+        F.ay = __uint_as_float(it[kItF + 1]) + wave_fsum(Fl.ay);
+        F.az = __uint_as_float(it[kItF + 2]) + wave_fsum(Fl.az);
+        F.t0 = __uint_as_float(it[kItF + 3]) + wave_fsum(Fl.t0);
+        F.t1 = __uint_as_float(it[kItF + 4]) + wave_fsum(Fl.t1);
+        F.t2 = __uint_as_float(it[kItF + 5]) + wave_fsum(Fl.t2);
+        F.t3 = __uint_as_float(it[kItF + 6]) + wave_fsum(Fl.t3);
+        F.t4 = __uint_as_float(it[kItF + 7]) + wave_fsum(Fl.t4);
+        F.t5 = __uint_as_float(it[kItF + 8]) + wave_fsum(Fl.t5);
+        if (!leaf && (ng > 0 || ne > 0)) {
+            // the children of P share its two lists; each gets the field translated to its centre
+            const int nch = oct_nchild(pr);
+            uint32_t base = 0;
+            if (lane == 0) {
+                base = atomicAdd(co + 0, (uint32_t)nch);
+                atomicMax(a.peak + 1, base + (uint32_t)nch);
+            }
+            base = __shfl(base, 0, kWave);
+            if ((int64_t)base + nch > a.item_cap || nch < 1 || nch > 8 || (int64_t)pr.topo.z + nch > a.max_nodes) {
+                over = true;
+                break;
+            }
+            if (lane < nch) {
+                const int c = pr.topo.z + lane;
+                const float4 cc = cen[c];
+                LocalF Fc = F;
+                fmm_at(F, cc.x - cP.x, cc.y - cP.y, cc.z - cP.z, Fc.ax, Fc.ay, Fc.az);
+                write_item(itout + (int64_t)(base + lane) * kFmmItemWords, c, (uint32_t)off, (uint32_t)ng,
+                           (uint32_t)(off + nclose), (uint32_t)ne, Fc, true);
+            }
+            continue;
+        }
+        // P is a leaf, or nothing below P interacts any more: finish P's bodies here
+        wave_mem_sync();
+        for (int b = 0; b < pr.topo.y; b += kWave) {
+            const bool has = b + lane < pr.topo.y;
+            const int me = pr.topo.x + b + lane;
+            const float4 p = a.bodies[has ? me : pr.topo.x];
+            const uint64_t lanes = __ballot(has);
+            FastWave W;
+            W.px = p.x; W.py = p.y; W.pz = p.z;
+            W.rx = 0.0f; W.ry = 0.0f; W.rz = 0.0f;
+            W.pp = 0; W.np = 0;
+            W.visits = 0; W.lanes = 0; W.leaf_visits = 0; W.leaf_lanes = 0;
+            W.G = a.G;
+            if (leaf) {
+                for (int j = 0; j < nc; ++j)
+                    fmm_walk_from<SOFT>(a, uniform_i32((int)C[j]), lanes, W, s_grp, s_msk, fail);
                 for (int j = 0; j < ne; ++j) {
-                    const float4 s = a.bodies[uniform_i32((int)E[ei][j])];
+                    const float4 s = a.bodies[uniform_i32((int)E2[j])];
                     if (__builtin_amdgcn_inverse_ballot_w64(lanes)) interact<SOFT>(s.x, s.y, s.z, s.w, eps2, W);
                 }
                 c_pp += W.pp + (uint64_t)ne * __popcll(lanes);
                 c_np += W.np;
-                if (has) {
-                    const float ox = p.x - cP.x, oy = p.y - cP.y, oz = p.z - cP.z;
-                    const float lx = __builtin_fmaf(T.t0, ox, __builtin_fmaf(T.t1, oy, __builtin_fmaf(T.t2, oz, T.ax)));
-                    const float ly = __builtin_fmaf(T.t1, ox, __builtin_fmaf(T.t3, oy, __builtin_fmaf(T.t4, oz, T.ay)));
-                    const float lz = __builtin_fmaf(T.t2, ox, __builtin_fmaf(T.t4, oy, __builtin_fmaf(T.t5, oz, T.az)));
-                    const uint32_t o = a.perm[me];
-                    if ((int64_t)o < a.n) {
-                        a.ax[o] = (W.rx + lx) * a.G;
-                        a.ay[o] = (W.ry + ly) * a.G;
-                        a.az[o] = (W.rz + lz) * a.G;
-                    }
+            }
+            if (has) {
+                float lx, ly, lz;
+                fmm_at(F, p.x - cP.x, p.y - cP.y, p.z - cP.z, lx, ly, lz);
+                const uint32_t o = a.perm[me];
+                if ((int64_t)o < a.n) {
+                    a.ax[o] = (W.rx + lx) * a.G;
+                    a.ay[o] = (W.ry + ly) * a.G;
+                    a.az[o] = (W.rz + lz) * a.G;
                 }
             }
         }
@@ -466,27 +569,40 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(FmmArgs a) {
         over = __ballot(over) != 0;
         if (fail || over) break;
     }
+    fail = __ballot(fail) != 0;
+    over = __ballot(over) != 0;
     if (lane == 0) {
         if (fail) atomicOr(a.err, 2u);
         if (over) atomicOr(a.err, 16u);
-        atomicAdd(&a.counts[0], (unsigned long long)c_pp);
-        atomicAdd(&a.counts[1], (unsigned long long)c_np);
-        atomicAdd(&a.counts[2], (unsigned long long)c_pn);
-        atomicAdd(&a.counts[3], (unsigned long long)c_nn);
+        if (c_pp) atomicAdd(&a.counts[0], (unsigned long long)c_pp);
+        if (c_np) atomicAdd(&a.counts[1], (unsigned long long)c_np);
+        if (c_pn) atomicAdd(&a.counts[2], (unsigned long long)c_pn);
+        if (c_nn) atomicAdd(&a.counts[3], (unsigned long long)c_nn);
+    }
+}
+
+// After the last step: a non-empty next frontier means the tree was deeper
+// than the steps launched.
+__global__ void k_fmm_tail(const uint32_t *ctr_next, uint32_t *peak, int steps, uint32_t *err) {
+    if (threadIdx.x == 0) {
+        if (ctr_next[0] != 0u) atomicOr(err, 2u);
+        peak[2] = (uint32_t)steps;
     }
 }
 
 }  // namespace
 
-void launch_fmm(const FmmArgs &a, const int32_t *leaf_start, hipStream_t s) {
+void launch_fmm(const FmmArgs &a, hipStream_t s) {
     if (a.n <= 0) return;
-    launch_fill_u32(a.start_flag, 0u, a.max_nodes, s);
-    launch_fill_u32(a.nleaves, 0u, 1, s);
-    launch_fill_u32(a.next, 0u, 1, s);
-    hipLaunchKernelGGL(k_fmm_starts, dim3(1), dim3(64), 0, s, a.nodes, a.misc, a.start_flag, a.starts, a.nstarts, a.err);
-    hipLaunchKernelGGL(k_fmm_leaves, dim3(256), dim3(256), 0, s, leaf_start, a.n, a.leaves, a.nleaves);
-    if (a.eps2 > 0.0f) hipLaunchKernelGGL(k_fmm_leaf<true>, dim3(a.waves), dim3(64), 0, s, a);
-    else hipLaunchKernelGGL(k_fmm_leaf<false>, dim3(a.waves), dim3(64), 0, s, a);
+    launch_fill_u32(a.ctr, 0u, 12, s);
+    launch_fill_u32(a.peak, 0u, 4, s);
+    hipLaunchKernelGGL(k_fmm_starts, dim3(1), dim3(64), 0, s, a.nodes, a.misc, a.starts, a.nstarts, a.err);
+    hipLaunchKernelGGL(k_fmm_seed, dim3(kHeapCap), dim3(64), 0, s, a);
+    for (int k = 0; k < a.steps; ++k) {
+        if (a.eps2 > 0.0f) hipLaunchKernelGGL(k_fmm_step<true>, dim3(a.waves), dim3(64), 0, s, a, k);
+        else hipLaunchKernelGGL(k_fmm_step<false>, dim3(a.waves), dim3(64), 0, s, a, k);
+    }
+    hipLaunchKernelGGL(k_fmm_tail, dim3(1), dim3(64), 0, s, a.ctr + 4 * (a.steps % 3), a.peak, a.steps, a.err);
 }
 
 }  // namespace nbg

@@ -242,10 +242,13 @@ void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, int diag, h
 // Barnes-Hut traversal over the octree (same modes and statistics).
 void launch_walk_oct(const WalkArgs &a, int ordered, int diag, hipStream_t s);
 
-// Implicit FMM (fmm.hip) over a built octree with summaries: one wavefront
-// per passive leaf, `waves` persistent wavefronts, each with wave_words words
-// of scratch (four node lists of cap_a entries, two body lists of cap_e).
-// counts: [pp, np, pn, nn]; err bit 16: a list outgrew its capacity.
+// Implicit FMM (fmm.hip): the lockstep dual traversal breadth-first, one
+// launch per traversal step.  Frontier items (kFmmItemWords each) and the
+// list pool alternate between two halves by step parity; ctr holds three
+// rotating sets of {items, next item, pool words, -}.  counts: [pp, np, pn,
+// nn]; err bit 16: a list or the frontier outgrew its capacity.
+constexpr int kFmmItemWords = 16;
+constexpr int kFmmStartCap = 512;
 struct FmmArgs {
     int64_t n;
     const float4 *bodies;     // tree order (x, y, z, m)
@@ -256,17 +259,19 @@ struct FmmArgs {
     const uint32_t *perm;     // tree position -> original index
     float *ax, *ay, *az;
     float theta, G, eps2;
-    uint32_t *start_flag;     // max_nodes words
-    int32_t *starts;          // 512
-    uint32_t *nstarts, *nleaves, *next;
-    uint32_t *leaves;         // n
-    uint32_t *scratch;
-    int64_t wave_words;
-    int cap_a, cap_e, waves;
+    int32_t *starts;          // kFmmStartCap
+    uint32_t *nstarts;
+    uint32_t *ctr;            // 12 words
+    uint32_t *peak;           // 4 words: pool words, items (largest step), steps run
+    uint32_t *items[2];       // item_cap items each
+    uint32_t *pool[2];        // pool_cap words each
+    int64_t item_cap, pool_cap;
+    uint32_t *seed_scratch;   // kFmmStartCap x 3 x cap_a words
+    int cap_a, waves, steps;
     unsigned long long *counts;
     uint32_t *err;
 };
-void launch_fmm(const FmmArgs &a, const int32_t *leaf_start, hipStream_t s);
+void launch_fmm(const FmmArgs &a, hipStream_t s);
 
 // v = v + a*dt; p = p + v*dt (original order).
 void launch_update(float *x, float *y, float *z, float *vx, float *vy, float *vz, const float *ax,

@@ -2,9 +2,10 @@
 Solvers/ImplicitFMMSolver.h:14-36) against the oracle's restatement of the
 reference's dual traversal.
 
-The GPU replays the traversal per passive leaf and takes every decision the
-reference takes (the same starting nodes, lockstep partitions, particle and
-node criteria in FP32), so the four SimpleTrackingRule counters
+The GPU runs the traversal breadth-first (one launch per traversal step) and
+takes every decision the reference takes (the same starting nodes, lockstep
+partitions, particle and node criteria in FP32), so the four SimpleTrackingRule
+counters
 (pp, np, pn, nn) must be equal.  It sums in FAST mode (rsqrt + FMA, partial
 local fields per lane), so accelerations agree within
 
@@ -120,6 +121,9 @@ def test_fmm_accuracy_at_one_million(nbg, gpu_ctx):
     assert np.median(rel) < 2e-3 and np.quantile(rel, 0.99) < 2e-2
     pp, np_, pn, nn = s.counts4()
     assert pp > 0 and np_ > 0 and pn > 0 and nn > 0
+    ws = s.walk_stats()
+    print(f"  counts {(pp, np_, pn, nn)}, traversal {ws}", flush=True)
+    assert 0 < ws["pool_peak"] <= ws["pool_capacity"] and 0 < ws["frontier_peak"] <= n
     again = s.update_accelerations()
     assert np.array_equal(again, acc)
     assert s.counts4() == (pp, np_, pn, nn)
