@@ -25,8 +25,8 @@ bool is_octree(int solver) {
 }
 // Implicit FMM: waves of a traversal step, the seed queues' capacity, and
 // the list pool per step parity (words per body plus a floor).
-constexpr int kFmmWaves = 2048, kFmmSeedCap = 4096;
-constexpr int64_t kFmmPoolPerBody = 80, kFmmPoolFloor = (int64_t)1 << 22, kFmmPoolMax = (int64_t)1 << 31;
+constexpr int kFmmWaves = 5120, kFmmSeedCap = 16384;
+constexpr int64_t kFmmPoolPerBody = 32, kFmmPoolFloor = (int64_t)1 << 22, kFmmPoolMax = (int64_t)1 << 31;
 bool is_quad(int solver) { return solver == NBG_BARNES_HUT_QUADRUPOLE || solver == NBG_LINEAR_BVH_QUADRUPOLE; }
 constexpr int kEventSets = 256;
 
@@ -193,7 +193,7 @@ int ensure_fmm(nbg_sys *s) {
         if (r == NBG_OK) r = dalloc(ptr, count);
     };
     A(&s->fmm_items, 2 * item_cap * kFmmItemWords); A(&s->fmm_pool, 2 * pool_cap);
-    A(&s->fmm_seed, (int64_t)kFmmStartCap * 3 * seed_cap); A(&s->fmm_misc, 64);
+    A(&s->fmm_seed, (int64_t)kFmmSeedWaves * 3 * seed_cap); A(&s->fmm_misc, 64);
     A(&s->fmm_starts, kFmmStartCap);
     if (r != NBG_OK) {
         free_fmm(s);

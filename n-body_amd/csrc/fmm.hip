@@ -84,7 +84,7 @@ __device__ void heap_pop(HeapEnt *h, int len) {
 
 constexpr int kFmmStarts = 256;
 constexpr int kHeapCap = kFmmStarts + 8;
-static_assert(kHeapCap <= kFmmStartCap, "starting-node buffer too small");
+static_assert(kHeapCap <= kFmmStartCap && kHeapCap <= kFmmSeedWaves, "starting-node buffers too small");
 
 __global__ __launch_bounds__(64) void k_fmm_starts(const OctRec *__restrict__ nodes, const uint32_t *__restrict__ misc,
                                                    int32_t *starts, uint32_t *nstarts, uint32_t *err) {
@@ -286,20 +286,23 @@ __device__ void fmm_walk_from(const FmmArgs &a, int start, uint64_t lanes, FastW
     }
 }
 
-// A frontier item: the passive node, its active list and close-body list
-// (offsets into the step's pool), its local field at the node's centre and
-// whether the active list holds child groups (first child | count << 28: the
+// A frontier item: a passive node, its active list and close-body list
+// (offsets into the step's pool) and its local field at the node's centre.
+// Flags: the active list holds child groups (first child | count << 28: the
 // children of the parent's close internal active nodes, expanded on reading)
-// or node slots.
-enum { kItP = 0, kItAoff = 1, kItAlen = 2, kItEoff = 3, kItElen = 4, kItF = 5, kItGroups = 14 };
+// rather than node slots; the item is a leaf group (some leaf children of the
+// node, at most 64 bodies together, given by a bit mask over its children;
+// the field is then the parent's).
+enum { kItP = 0, kItAoff = 1, kItAlen = 2, kItEoff = 3, kItElen = 4, kItF = 5, kItFlags = 14, kItLeaves = 15 };
+constexpr uint32_t kItemGroups = 1u, kItemLeafGroup = 2u;
 
 __device__ __forceinline__ void write_item(uint32_t *it, int P, uint32_t aoff, uint32_t alen, uint32_t eoff,
-                                           uint32_t elen, const LocalF &F, bool groups) {
+                                           uint32_t elen, const LocalF &F, uint32_t flags, uint32_t leaves) {
     uint4 *q = (uint4 *)it;
     q[0] = make_uint4((uint32_t)P, aoff, alen, eoff);
     q[1] = make_uint4(elen, __float_as_uint(F.ax), __float_as_uint(F.ay), __float_as_uint(F.az));
     q[2] = make_uint4(__float_as_uint(F.t0), __float_as_uint(F.t1), __float_as_uint(F.t2), __float_as_uint(F.t3));
-    q[3] = make_uint4(__float_as_uint(F.t4), __float_as_uint(F.t5), groups ? 1u : 0u, 0u);
+    q[3] = make_uint4(__float_as_uint(F.t4), __float_as_uint(F.t5), flags, leaves);
 }
 
 // Entry idx of an active list: a node slot, or (groups) child idx % 8 of
@@ -309,6 +312,186 @@ __device__ __forceinline__ int active_entry(const uint32_t *list, uint32_t idx, 
     const uint32_t g = list[idx >> 3];
     const uint32_t k = idx & 7u;
     return k < (g >> 28) ? (int)(g & 0x0FFFFFFFu) + (int)k : -1;
+}
+
+// ---------------------------------------------------------------- leaf groups
+// Up to eight leaf children of one passive node, at most 64 bodies together,
+// served by one wave: lane t holds body t of the group.  Each leaf takes its
+// own decisions against the shared lists (the reference's lockstep step at a
+// passive leaf: far active nodes and far bodies into the leaf's local field,
+// the other active nodes walked, the other bodies added directly), so the
+// interaction set of every body is the reference's; the wave walks each
+// active node once for all the lanes whose leaf found it close.
+struct GroupLds {
+    float4 cen[8];   // leaf centre, side
+    int first[8], count[8], pre[9];
+    float F[8][9];   // leaf local fields
+};
+
+template <bool SOFT>
+__device__ void fmm_leaf_group(const FmmArgs &a, int P, const uint32_t *alist, uint32_t alen, const uint32_t *elist,
+                               uint32_t elen, const uint32_t *it, uint32_t lmask, GroupLds &g, int *s_grp,
+                               uint64_t *s_msk, uint64_t &c_pp, uint64_t &c_np, uint64_t &c_pn, uint64_t &c_nn,
+                               bool &fail) {
+    const int lane = lane_id();
+    const OctRec *__restrict__ nodes = a.nodes;
+    const float4 *__restrict__ cen = a.centre;
+    const float theta = a.theta, theta2 = a.theta * a.theta, eps2 = a.eps2;
+    const OctRec pr = nodes[P];
+    const float4 cP = cen[P];
+    const int c0 = pr.topo.z, nch = oct_nchild(pr);
+    lmask &= 0xFFu;
+    if (c0 < 1 || nch < 1 || nch > 8 || (int64_t)c0 + nch > a.max_nodes || (lmask >> nch) != 0u || lmask == 0u) {
+        fail = true;
+        return;
+    }
+    const int k = __popc(lmask);
+    // lane j < k: the j-th leaf of the group, its field translated from P's centre
+    if (lane < k) {
+        uint32_t m = lmask;
+        for (int j = 0; j < lane; ++j) m &= m - 1u;
+        const int slot = c0 + (__ffs(m) - 1);
+        const OctRec lr = nodes[slot];
+        const float4 cl = cen[slot];
+        g.cen[lane] = cl;
+        g.first[lane] = lr.topo.x;
+        g.count[lane] = lr.topo.y;
+        LocalF F;
+        F.ax = __uint_as_float(it[kItF + 0]); F.ay = __uint_as_float(it[kItF + 1]); F.az = __uint_as_float(it[kItF + 2]);
+        F.t0 = __uint_as_float(it[kItF + 3]); F.t1 = __uint_as_float(it[kItF + 4]); F.t2 = __uint_as_float(it[kItF + 5]);
+        F.t3 = __uint_as_float(it[kItF + 6]); F.t4 = __uint_as_float(it[kItF + 7]); F.t5 = __uint_as_float(it[kItF + 8]);
+        float x, y, z;
+        fmm_at(F, cl.x - cP.x, cl.y - cP.y, cl.z - cP.z, x, y, z);
+        g.F[lane][0] = x; g.F[lane][1] = y; g.F[lane][2] = z;
+        g.F[lane][3] = F.t0; g.F[lane][4] = F.t1; g.F[lane][5] = F.t2;
+        g.F[lane][6] = F.t3; g.F[lane][7] = F.t4; g.F[lane][8] = F.t5;
+    }
+    wave_mem_sync();
+    // lane -> (leaf, body)
+    int total = 0, myleaf = 0;
+    for (int j = 0; j < k; ++j) {
+        const int f = g.first[j], c = g.count[j];
+        if (f < 0 || c < 1 || (int64_t)f + c > a.n) fail = true;
+        if (lane >= total) myleaf = j;
+        total += c;
+    }
+    if (fail || total > kWave) {
+        fail = true;
+        return;
+    }
+    const bool has = lane < total;
+    int pre = 0;
+    for (int j = 0; j < myleaf; ++j) pre += g.count[j];
+    const int me = g.first[myleaf] + (lane - pre);
+    const float4 p = a.bodies[has ? me : g.first[0]];
+
+    // pass A, leaf by leaf: far active nodes and far bodies into the leaf's
+    // field (per-lane partials, summed once per leaf)
+    const uint32_t vlen = 8u * alen;
+    for (int j = 0; j < k; ++j) {
+        const float4 cl = g.cen[j];
+        LocalF Fp = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (uint32_t b = 0; b < vlen; b += kWave) {
+            int an = b + lane < vlen ? active_entry(alist, b + lane, true) : -1;
+            if (an >= a.max_nodes) { fail = true; an = -1; }
+            const bool he = an >= 0;
+            const OctRec ar = nodes[he ? an : 0];
+            const bool far = he && fmm_recommend(ar.com, cen[he ? an : 0], cl, theta2) == 0;
+            if (far) fmm_to_local(ar.com.x, ar.com.y, ar.com.z, ar.com.w, cl, eps2, Fp);
+            c_nn += __popcll(__ballot(far));
+        }
+        for (uint32_t b = 0; b < elen; b += kWave) {
+            bool he = b + lane < elen;
+            const int e = he ? (int)elist[b + lane] : 0;
+            if (he && (e < 0 || (int64_t)e >= a.n)) { fail = true; he = false; }
+            const float4 bd = a.bodies[he ? e : 0];
+            const bool far = he && fmm_particle_far(bd.x, bd.y, bd.z, cl, theta);
+            if (far) fmm_to_local(bd.x, bd.y, bd.z, bd.w, cl, eps2, Fp);
+            c_pn += __popcll(__ballot(far));
+        }
+        const float s0 = wave_fsum(Fp.ax), s1 = wave_fsum(Fp.ay), s2 = wave_fsum(Fp.az);
+        const float s3 = wave_fsum(Fp.t0), s4 = wave_fsum(Fp.t1), s5 = wave_fsum(Fp.t2);
+        const float s6 = wave_fsum(Fp.t3), s7 = wave_fsum(Fp.t4), s8 = wave_fsum(Fp.t5);
+        if (lane == 0) {
+            g.F[j][0] += s0; g.F[j][1] += s1; g.F[j][2] += s2;
+            g.F[j][3] += s3; g.F[j][4] += s4; g.F[j][5] += s5;
+            g.F[j][6] += s6; g.F[j][7] += s7; g.F[j][8] += s8;
+        }
+    }
+    wave_mem_sync();
+
+    // pass B: walk each active node once for the lanes whose leaf found it close
+    FastWave W;
+    W.px = p.x; W.py = p.y; W.pz = p.z;
+    W.rx = 0.0f; W.ry = 0.0f; W.rz = 0.0f;
+    W.pp = 0; W.np = 0;
+    W.visits = 0; W.lanes = 0; W.leaf_visits = 0; W.leaf_lanes = 0;
+    W.G = a.G;
+    const uint32_t mybit = has ? 1u << myleaf : 0u;
+    for (uint32_t b = 0; b < vlen; b += kWave) {
+        int an = b + lane < vlen ? active_entry(alist, b + lane, true) : -1;
+        if (an >= a.max_nodes) an = -1;
+        const bool he = an >= 0;
+        const OctRec ar = nodes[he ? an : 0];
+        const float4 ca = cen[he ? an : 0];
+        uint32_t m = 0;
+        if (he) {
+#pragma unroll 1
+            for (int j = 0; j < k; ++j)
+                if (fmm_recommend(ar.com, ca, g.cen[j], theta2) != 0) m |= 1u << j;
+        }
+        uint64_t todo = __ballot(m != 0u);
+        while (todo) {
+            const int i = __ffsll((unsigned long long)todo) - 1;
+            todo &= todo - 1;
+            const int X = __builtin_amdgcn_readlane(an, i);
+            const uint32_t mm = (uint32_t)__builtin_amdgcn_readlane((int)m, i);
+            const uint64_t lanes = __ballot((mm & mybit) != 0u);
+            if (lanes) fmm_walk_from<SOFT>(a, X, lanes, W, s_grp, s_msk, fail);
+        }
+    }
+    uint64_t epp = 0;
+    for (uint32_t b = 0; b < elen; b += kWave) {
+        bool he = b + lane < elen;
+        const int e = he ? (int)elist[b + lane] : 0;
+        if (he && (e < 0 || (int64_t)e >= a.n)) he = false;
+        const float4 bd = a.bodies[he ? e : 0];
+        uint32_t m = 0;
+        if (he) {
+#pragma unroll 1
+            for (int j = 0; j < k; ++j)
+                if (!fmm_particle_far(bd.x, bd.y, bd.z, g.cen[j], theta)) m |= 1u << j;
+        }
+        uint64_t todo = __ballot(m != 0u);
+        while (todo) {
+            const int i = __ffsll((unsigned long long)todo) - 1;
+            todo &= todo - 1;
+            const uint32_t mm = (uint32_t)__builtin_amdgcn_readlane((int)m, i);
+            const uint64_t lanes = __ballot((mm & mybit) != 0u);
+            const float sx = readlane_f32(bd.x, i), sy = readlane_f32(bd.y, i);
+            const float sz = readlane_f32(bd.z, i), sm = readlane_f32(bd.w, i);
+            if ((mm & mybit) != 0u) interact<SOFT>(sx, sy, sz, sm, eps2, W);
+            epp += __popcll(lanes);
+        }
+    }
+    c_pp += W.pp + epp;
+    c_np += W.np;
+    if (has) {
+        const float4 cl = g.cen[myleaf];
+        LocalF F;
+        F.ax = g.F[myleaf][0]; F.ay = g.F[myleaf][1]; F.az = g.F[myleaf][2];
+        F.t0 = g.F[myleaf][3]; F.t1 = g.F[myleaf][4]; F.t2 = g.F[myleaf][5];
+        F.t3 = g.F[myleaf][6]; F.t4 = g.F[myleaf][7]; F.t5 = g.F[myleaf][8];
+        float lx, ly, lz;
+        fmm_at(F, p.x - cl.x, p.y - cl.y, p.z - cl.z, lx, ly, lz);
+        const uint32_t o = a.perm[me];
+        if ((int64_t)o < a.n) {
+            a.ax[o] = (W.rx + lx) * a.G;
+            a.ay[o] = (W.ry + ly) * a.G;
+            a.az[o] = (W.rz + lz) * a.G;
+        }
+    }
+    wave_mem_sync();  // the next item rewrites the group's LDS
 }
 
 // ---------------------------------------------------------------- seeds
@@ -366,7 +549,7 @@ __global__ __launch_bounds__(64) void k_fmm_seed(FmmArgs a) {
     slot = __shfl(slot, 0, kWave);
     if (lane == 0) {
         const LocalF Z = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        write_item(a.items[0] + (int64_t)slot * kFmmItemWords, S, (uint32_t)off, (uint32_t)eq, 0u, 0u, Z, false);
+        write_item(a.items[0] + (int64_t)slot * kFmmItemWords, S, (uint32_t)off, (uint32_t)eq, 0u, 0u, Z, 0u, 0u);
     }
 }
 
@@ -378,6 +561,8 @@ template <bool SOFT>
 __global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
     __shared__ int s_grp[kStackOct];
     __shared__ uint64_t s_msk[kStackOct];
+    __shared__ uint32_t s_desc[8];
+    __shared__ GroupLds s_g;
 
     const int lane = lane_id();
     const OctRec *__restrict__ nodes = a.nodes;
@@ -391,6 +576,7 @@ __global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
     uint32_t *pout = a.pool[(k + 1) & 1];
     // a step that overflowed the frontier counted past the capacity (and flagged it)
     const int nin = (int)(ci[0] < (uint64_t)a.item_cap ? ci[0] : (uint32_t)a.item_cap);
+    if ((int)blockIdx.x >= nin) return;  // (steps past the deepest leaf launch with nin = 0)
     uint64_t c_pp = 0, c_np = 0, c_pn = 0, c_nn = 0;
     bool fail = false, over = false;
 
@@ -403,10 +589,18 @@ __global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
         const int P = uniform_i32((int)it[kItP]);
         const uint32_t aoff = (uint32_t)uniform_i32((int)it[kItAoff]), alen = (uint32_t)uniform_i32((int)it[kItAlen]);
         const uint32_t eoff = (uint32_t)uniform_i32((int)it[kItEoff]), elen = (uint32_t)uniform_i32((int)it[kItElen]);
-        const bool groups = uniform_i32((int)it[kItGroups]) != 0;
+        const uint32_t flags = (uint32_t)uniform_i32((int)it[kItFlags]);
+        const bool groups = (flags & kItemGroups) != 0;
         if (P < 0 || P >= a.max_nodes || (int64_t)aoff + alen > a.pool_cap || (int64_t)eoff + elen > a.pool_cap) {
             fail = true;
             break;
+        }
+        if (flags & kItemLeafGroup) {
+            fmm_leaf_group<SOFT>(a, P, pin + aoff, alen, pin + eoff, elen, it, (uint32_t)uniform_i32((int)it[kItLeaves]),
+                                 s_g, s_grp, s_msk, c_pp, c_np, c_pn, c_nn, fail);
+            fail = __ballot(fail) != 0;
+            if (fail) break;
+            continue;
         }
         const OctRec pr = nodes[P];
         const float4 cP = cen[P];
@@ -509,26 +703,68 @@ __global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
         F.t4 = __uint_as_float(it[kItF + 7]) + wave_fsum(Fl.t4);
         F.t5 = __uint_as_float(it[kItF + 8]) + wave_fsum(Fl.t5);
         if (!leaf && (ng > 0 || ne > 0)) {
-            // the children of P share its two lists; each gets the field translated to its centre
+            // The children of P share its two lists.  An internal child gets an
+            // item with the field translated to its centre; leaf children of at
+            // most 64 bodies share items of at most 64 bodies (one wave serves
+            // their bodies together), carrying P's field.
             const int nch = oct_nchild(pr);
+            if (nch < 1 || nch > 8 || pr.topo.z < 1 || (int64_t)pr.topo.z + nch > a.max_nodes) {
+                fail = true;
+                break;
+            }
+            const OctRec chr = nodes[pr.topo.z + (lane < nch ? lane : 0)];
+            const int ccount = chr.topo.y;
+            const int cleaf = chr.topo.z < 0 ? 1 : 0;
+            int nit = 0, gbod = 0;
+            uint32_t gmask = 0;
+            for (int j = 0; j < nch; ++j) {
+                const int cnt = __shfl(ccount, j, kWave);
+                if (__shfl(cleaf, j, kWave) && cnt <= kWave) {
+                    if (gbod + cnt > kWave) {
+                        if (lane == 0) s_desc[nit] = 0x80000000u | gmask;
+                        ++nit;
+                        gmask = 0;
+                        gbod = 0;
+                    }
+                    gmask |= 1u << j;
+                    gbod += cnt;
+                } else {
+                    if (lane == 0) s_desc[nit] = (uint32_t)(pr.topo.z + j);
+                    ++nit;
+                }
+            }
+            if (gmask) {
+                if (lane == 0) s_desc[nit] = 0x80000000u | gmask;
+                ++nit;
+            }
             uint32_t base = 0;
             if (lane == 0) {
-                base = atomicAdd(co + 0, (uint32_t)nch);
-                atomicMax(a.peak + 1, base + (uint32_t)nch);
+                base = atomicAdd(co + 0, (uint32_t)nit);
+                atomicMax(a.peak + 1, base + (uint32_t)nit);
             }
             base = __shfl(base, 0, kWave);
-            if ((int64_t)base + nch > a.item_cap || nch < 1 || nch > 8 || (int64_t)pr.topo.z + nch > a.max_nodes) {
+            if ((int64_t)base + nit > a.item_cap) {
                 over = true;
                 break;
             }
-            if (lane < nch) {
-                const int c = pr.topo.z + lane;
-                const float4 cc = cen[c];
-                LocalF Fc = F;
-                fmm_at(F, cc.x - cP.x, cc.y - cP.y, cc.z - cP.z, Fc.ax, Fc.ay, Fc.az);
-                write_item(itout + (int64_t)(base + lane) * kFmmItemWords, c, (uint32_t)off, (uint32_t)ng,
-                           (uint32_t)(off + nclose), (uint32_t)ne, Fc, true);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (lane < nit) {
+                const uint32_t d = s_desc[lane];
+                uint32_t *dst = itout + (int64_t)(base + lane) * kFmmItemWords;
+                if (d & 0x80000000u) {
+                    write_item(dst, P, (uint32_t)off, (uint32_t)ng, (uint32_t)(off + nclose), (uint32_t)ne, F,
+                               kItemGroups | kItemLeafGroup, d & 0xFFu);
+                } else {
+                    const int c = (int)d;
+                    const float4 cc = cen[c];
+                    LocalF Fc = F;
+                    fmm_at(F, cc.x - cP.x, cc.y - cP.y, cc.z - cP.z, Fc.ax, Fc.ay, Fc.az);
+                    write_item(dst, c, (uint32_t)off, (uint32_t)ng, (uint32_t)(off + nclose), (uint32_t)ne, Fc,
+                               kItemGroups, 0u);
+                }
             }
+            __builtin_amdgcn_wave_barrier();
             continue;
         }
         // P is a leaf, or nothing below P interacts any more: finish P's bodies here

@@ -249,6 +249,7 @@ void launch_walk_oct(const WalkArgs &a, int ordered, int diag, hipStream_t s);
 // nn]; err bit 16: a list or the frontier outgrew its capacity.
 constexpr int kFmmItemWords = 16;
 constexpr int kFmmStartCap = 512;
+constexpr int kFmmSeedWaves = 272;  // one per starting node (at most 256 + 7)
 struct FmmArgs {
     int64_t n;
     const float4 *bodies;     // tree order (x, y, z, m)
@@ -266,7 +267,7 @@ struct FmmArgs {
     uint32_t *items[2];       // item_cap items each
     uint32_t *pool[2];        // pool_cap words each
     int64_t item_cap, pool_cap;
-    uint32_t *seed_scratch;   // kFmmStartCap x 3 x cap_a words
+    uint32_t *seed_scratch;   // kFmmSeedWaves x 3 x cap_a words
     int cap_a, waves, steps;
     unsigned long long *counts;
     uint32_t *err;
