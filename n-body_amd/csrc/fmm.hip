@@ -239,10 +239,9 @@ __device__ __forceinline__ int64_t pool_reserve(uint32_t *counter, int64_t words
 // Descent::activeTree from `start` for the lanes in `lanes` (the octree's
 // FAST group walk, counting interactions).
 template <bool SOFT>
-__device__ void fmm_walk_from(const FmmArgs &a, int start, uint64_t lanes, FastWave &W, int *s_grp, uint64_t *s_msk,
-                              bool &fail) {
+__device__ void fmm_walk_from(const FmmArgs &a, const OctRec &st, uint64_t lanes, FastWave &W, int *s_grp,
+                              uint64_t *s_msk, bool &fail) {
     const OctRec *__restrict__ nodes = a.nodes;
-    const OctRec st = nodes[start];
     uint64_t mask = visit_oct<SOFT, 1, false>(st, nullptr, lanes, a.bodies, a.n, a.eps2, W, fail);
     int c = st.topo.z, nch = oct_nchild(st);
     int sp = 0;
@@ -444,10 +443,21 @@ __device__ void fmm_leaf_group(const FmmArgs &a, int P, const uint32_t *alist, u
         while (todo) {
             const int i = __ffsll((unsigned long long)todo) - 1;
             todo &= todo - 1;
-            const int X = __builtin_amdgcn_readlane(an, i);
             const uint32_t mm = (uint32_t)__builtin_amdgcn_readlane((int)m, i);
             const uint64_t lanes = __ballot((mm & mybit) != 0u);
-            if (lanes) fmm_walk_from<SOFT>(a, X, lanes, W, s_grp, s_msk, fail);
+            if (lanes) {
+                // the node's record is already in lane i
+                OctRec st;
+                st.com = make_float4(readlane_f32(ar.com.x, i), readlane_f32(ar.com.y, i), readlane_f32(ar.com.z, i),
+                                     readlane_f32(ar.com.w, i));
+                st.lo = make_float4(readlane_f32(ar.lo.x, i), readlane_f32(ar.lo.y, i), readlane_f32(ar.lo.z, i),
+                                    readlane_f32(ar.lo.w, i));
+                st.hi = make_float4(readlane_f32(ar.hi.x, i), readlane_f32(ar.hi.y, i), readlane_f32(ar.hi.z, i),
+                                    readlane_f32(ar.hi.w, i));
+                st.topo = make_int4(__builtin_amdgcn_readlane(ar.topo.x, i), __builtin_amdgcn_readlane(ar.topo.y, i),
+                                    __builtin_amdgcn_readlane(ar.topo.z, i), __builtin_amdgcn_readlane(ar.topo.w, i));
+                fmm_walk_from<SOFT>(a, st, lanes, W, s_grp, s_msk, fail);
+            }
         }
     }
     uint64_t epp = 0;
@@ -782,7 +792,7 @@ __global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
             W.G = a.G;
             if (leaf) {
                 for (int j = 0; j < nc; ++j)
-                    fmm_walk_from<SOFT>(a, uniform_i32((int)C[j]), lanes, W, s_grp, s_msk, fail);
+                    fmm_walk_from<SOFT>(a, nodes[uniform_i32((int)C[j])], lanes, W, s_grp, s_msk, fail);
                 for (int j = 0; j < ne; ++j) {
                     const float4 s = a.bodies[uniform_i32((int)E2[j])];
                     if (__builtin_amdgcn_inverse_ballot_w64(lanes)) interact<SOFT>(s.x, s.y, s.z, s.w, eps2, W);
