@@ -11,7 +11,9 @@ bounding cube; the same run measures the octree Barnes-Hut solver on the
 same bodies and reports it in the "barnes_hut" block, and the quadrupole
 variants of both trees ("linear_bvh_4p", "barnes_hut_4p") at the opening
 angles the reference's benchmark tunes them to (src/Benchmark/profile.cpp:
-47-53: 0.72578126 and 0.41289064).  Each solver carries:
+47-53: 0.72578126 and 0.41289064), and the implicit FMM ("implicit_fmm") at
+its own criterion's angle (ImplicitDualTraversalSolver.h:22: 0.4).  Each
+solver carries:
 bodies*steps/s over the timed steps, interactions/s, the walk kernel's FP32
 roofline fraction (HIP events on the system stream), the force error against
 an FP64 direct sum on a 16,384-body sample, the energy drift over the run and
@@ -33,8 +35,12 @@ sys.path.insert(0, ROOT)
 METRIC = "bodies·steps/s at N=1M θ=0.5; batched-serving p50/p99 latency at fixed force err"
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (packed) peak
 FLOPS_PER_INTERACTION = 20
-SOLVER_NAMES = {0: "barnes-hut", 1: "linear-bvh", 3: "barnes-hut-4p", 4: "linear-bvh-4p"}
-WALK_KERNELS = {0: "k_walk_oct_fast", 1: "k_walk_lbvh_fast", 3: "k_walk_oct_fast", 4: "k_walk_lbvh_fast"}
+SOLVER_NAMES = {0: "barnes-hut", 1: "linear-bvh", 3: "barnes-hut-4p", 4: "linear-bvh-4p", 5: "implicit-fmm"}
+WALK_KERNELS = {0: "k_walk_oct_fast", 1: "k_walk_lbvh_fast", 3: "k_walk_oct_fast", 4: "k_walk_lbvh_fast",
+                5: "k_fmm_step"}
+# FP32 operations of one body or node into a local field (difference, r^2,
+# rsqrt and powers, the vector's 3 and the gradient's 6 accumulating FMAs).
+FLOPS_PER_LOCAL = 40
 # FP32 operations of one quadrupole node interaction in the FAST walk
 # (difference, r^2, rsqrt and powers, Q d, d.Q.d, the two coefficients and
 # the 6 accumulating FMAs); monopole node and body interactions count 20.
@@ -65,7 +71,7 @@ def load_traffic():
 
 
 def solver_theta(args, solver):
-    return {3: args.theta_4p_bh, 4: args.theta_4p_lbvh}.get(solver, args.theta)
+    return {3: args.theta_4p_bh, 4: args.theta_4p_lbvh, 5: args.theta_fmm}.get(solver, args.theta)
 
 
 def cpu_baseline(b, args, solver, threads):
@@ -79,6 +85,23 @@ def cpu_baseline(b, args, solver, threads):
     t = oracle.Tree(solver, b.x, b.y, b.z, b.m, theta=solver_theta(args, solver), leaf=args.leaf, bits=args.bits,
                     eps2=args.eps2)
     t_build = time.perf_counter() - t0
+    if solver == 5:
+        # the implicit FMM has no per-target walk: time the traversals from a
+        # sample of its load-balanced starting nodes, scaled by the bodies held
+        t0 = time.perf_counter()
+        covered = t.fmm_sample(args.cpu_fmm_starts)
+        t_walk = (time.perf_counter() - t0) * (n / max(1, covered))
+        step_s = t_build + t_walk
+        del t
+        return {
+            "value": n / step_s,
+            "unit": "bodies·steps/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"oracle implicit FMM (single thread): one build over all {n} bodies + the dual traversals "
+                      f"from {args.cpu_fmm_starts} of its starting nodes ({covered} bodies), scaled by N/{covered} "
+                      f"(build {t_build:.2f} s, traversal {t_walk:.2f} s extrapolated)",
+        }
     sample = min(n, args.cpu_sample)
     sel = np.linspace(0, n - 1, sample).astype(np.int64)
     t0 = time.perf_counter()
@@ -129,15 +152,17 @@ def measure(nbg, ctx, b, args, solver):
     s.enable_counts(True)
     s.enable_walk_stats(True)
     s.update_accelerations()
-    pp, npc = s.counts()
+    pp, npc, pn, nn = s.counts4()
     ws = s.walk_stats()
     s.enable_walk_stats(False)
 
     ms_per_step = 1e3 * elapsed / args.steps
-    inter = pp + npc
+    inter = pp + npc + pn + nn
     walk_ms = phases["walk"]
     quad = solver in (3, 4)
-    flops = FLOPS_PER_INTERACTION * pp + (FLOPS_PER_QUAD_NODE if quad else FLOPS_PER_INTERACTION) * npc
+    fmm = solver == 5
+    flops = (FLOPS_PER_INTERACTION * pp + (FLOPS_PER_QUAD_NODE if quad else FLOPS_PER_INTERACTION) * npc
+             + FLOPS_PER_LOCAL * (pn + nn))
     achieved = flops / (walk_ms * 1e-3) / 1e12
     waves = max(1, (args.n + 63) // 64)
     traffic = load_traffic().get(SOLVER_NAMES[solver])
@@ -148,12 +173,13 @@ def measure(nbg, ctx, b, args, solver):
         "interactions_per_step": inter,
         "interactions_per_s": inter / (ms_per_step * 1e-3),
         "phases_ms": phases,
+        "interaction_counts": {"pp": pp, "np": npc, "pn": pn, "nn": nn},
         "walk_schedule": {
             "node_visits_per_wave": ws["node_visits"] / waves,
             "lane_utilisation_nodes": ws["active_lanes"] / max(1, 64 * ws["node_visits"]),
             "leaf_visits_per_wave": ws["leaf_visits"] / waves,
             "lane_utilisation_leaves": ws["leaf_lanes"] / max(1, 64 * ws["leaf_visits"]),
-        } if ws["node_visits"] else None,
+        } if not fmm and ws["node_visits"] else (ws if fmm else None),
         "roofline": {
             "bound": "valu",
             "kernel": WALK_KERNELS[solver],
@@ -163,10 +189,13 @@ def measure(nbg, ctx, b, args, solver):
             "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": traffic,
             "note": (f"{FLOPS_PER_INTERACTION} flops per body interaction and "
-                     f"{FLOPS_PER_QUAD_NODE if quad else FLOPS_PER_INTERACTION} per node interaction "
-                     f"(SimpleTrackingRule counting) / mean walk-kernel time from HIP events on the system "
-                     f"stream; traffic = HBM bytes per walk launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, "
-                     f"copy-kernel calibrated"),
+                     f"{FLOPS_PER_QUAD_NODE if quad else FLOPS_PER_INTERACTION} per node interaction"
+                     + (f", {FLOPS_PER_LOCAL} per body or node into a local field" if fmm else "")
+                     + " (SimpleTrackingRule counting) / "
+                     + ("the traversal's device time (all its launches) " if fmm else "mean walk-kernel time ")
+                     + "from HIP events on the system stream; traffic = HBM bytes per "
+                     + ("traversal" if fmm else "walk launch")
+                     + " from rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated"),
         },
     }
     if not args.no_accuracy:
@@ -422,13 +451,17 @@ def main():
     ap.add_argument("--dt", type=float, default=5e-4)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--mode", choices=["fast", "ordered"], default="fast")
-    ap.add_argument("--solvers", default="linear-bvh,barnes-hut,linear-bvh-4p,barnes-hut-4p",
+    ap.add_argument("--solvers", default="linear-bvh,barnes-hut,linear-bvh-4p,barnes-hut-4p,implicit-fmm",
                     help="comma list; the first is the headline line")
     ap.add_argument("--theta-4p-lbvh", type=float, default=0.72578126,
                     help="quadrupole LBVH opening angle (the reference benchmark's tuned value)")
     ap.add_argument("--theta-4p-bh", type=float, default=0.41289064,
                     help="quadrupole octree opening angle (the reference benchmark's tuned value)")
+    ap.add_argument("--theta-fmm", type=float, default=0.4,
+                    help="implicit FMM opening angle (its descent criterion's default)")
     ap.add_argument("--cpu-sample", type=int, default=131072)
+    ap.add_argument("--cpu-fmm-starts", type=int, default=12,
+                    help="starting nodes whose traversals time the oracle's implicit FMM")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-forest", action="store_true", help="skip the batched-ensemble measurement")

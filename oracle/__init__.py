@@ -53,6 +53,7 @@ def lib() -> ctypes.CDLL:
             "orc_tree_quads": (ctypes.c_int, [_vp, fp]),
             "orc_fmm_accel": (ctypes.c_int, [_vp, fp, fp, fp, _u64p, _f64p]),
             "orc_fmm_starts": (ctypes.c_int64, [_vp, _i32p, ctypes.c_int64]),
+            "orc_fmm_sample": (ctypes.c_int64, [_vp, ctypes.c_int32]),
             "orc_fmm_p2l": (ctypes.c_int, [fp, ctypes.c_float, fp, fp, ctypes.c_float, ctypes.c_float, fp]),
             "orc_node_particle_quad": (ctypes.c_int, [fp, ctypes.c_float, fp, fp, ctypes.c_float, ctypes.c_float,
                                                       fp]),
@@ -176,6 +177,14 @@ class Tree:
         if lib().orc_fmm_accel(self.h, None, None, None, _p(c, _u64p), _p(out, _f64p)) != 0:
             raise ValueError("not an FMM tree")
         return out
+
+    def fmm_sample(self, k):
+        """Runs the implicit FMM's traversals from its first k starting nodes
+        (a timing sample); returns the number of bodies they hold."""
+        r = lib().orc_fmm_sample(self.h, int(k))
+        if r < 0:
+            raise ValueError("not an FMM tree")
+        return int(r)
 
     def fmm_starts(self):
         out = np.zeros(512, np.int32)

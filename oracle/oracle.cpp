@@ -1072,6 +1072,23 @@ int orc_fmm_p2l(const float *s3, float m, const float *c3, const float *p3, floa
     return 0;
 }
 
+// The implicit FMM's traversals from its first k starting nodes only (a CPU
+// timing sample of the whole pass); returns the number of bodies those
+// starting nodes hold.
+int64_t orc_fmm_sample(void *hp, int32_t k) {
+    Handle *h = (Handle *)hp;
+    if (!h->fmm) return -1;
+    FMM f(*h->o, false);
+    if (h->o->n == 0) return 0;
+    const std::vector<int32_t> st = f.starting_nodes(256);
+    int64_t covered = 0;
+    for (int32_t i = 0; i < k && i < (int32_t)st.size(); ++i) {
+        f.balanced(st[i]);
+        covered += f.nd(st[i]).count;
+    }
+    return covered;
+}
+
 // The starting passive nodes of the implicit FMM (pre-order node ids).
 int64_t orc_fmm_starts(void *hp, int32_t *out, int64_t cap) {
     Handle *h = (Handle *)hp;
