@@ -57,7 +57,11 @@ enum nbg_solver_kind {
      * id "implicit-fmm-2p"): the lockstep dual traversal of ImplicitDualTraversalSolver
      * (Solvers/ImplicitDualTraversalSolver.h:17-70) over the Barnes-Hut octree, monopole
      * sources and local fields with their gradient; theta is its SideLengthOverDistance
-     * (reference default 0.4).  FAST sum mode only; its passive points are its bodies. */
+     * (reference default 0.4).  FAST sum mode only.  Its passive points are its bodies:
+ * massless bodies are not in the reference's dual tree (CenterOfMassSummary::context,
+ * Physics/Summaries/CenterOfMassSummary.h:22-24) and keep the zero acceleration the
+ * solver resets them to (ImplicitDualTraversalSolver.h:50-54), so
+ * nbg_sys_accelerations_at refuses this kind. */
     NBG_IMPLICIT_FMM = 5
 };
 
