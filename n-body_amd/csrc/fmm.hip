@@ -384,14 +384,18 @@ __device__ void fmm_leaf_group(const FmmArgs &a, int P, const uint32_t *alist, u
     const int me = g.first[myleaf] + (lane - pre);
     const float4 p = a.bodies[has ? me : g.first[0]];
 
-    // pass A, leaf by leaf: far active nodes and far bodies into the leaf's
-    // field (per-lane partials, summed once per leaf)
+    // pass A: far active nodes and far bodies into the leaves' fields.  Lane
+    // t < kk evaluates entry t / k of a batch against leaf t % k, so a lane
+    // always serves one leaf (its partial field is that leaf's) and the k
+    // lanes of an entry read the same record.
     const uint32_t vlen = 8u * alen;
-    for (int j = 0; j < k; ++j) {
-        const float4 cl = g.cen[j];
+    {
+        const int epb = kWave / k, kk = epb * k, lj = lane % k, li = lane / k;
+        const float4 cl = g.cen[lj];
         LocalF Fp = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (uint32_t b = 0; b < vlen; b += kWave) {
-            int an = b + lane < vlen ? active_entry(alist, b + lane, true) : -1;
+        for (uint32_t b = 0; b < vlen; b += (uint32_t)epb) {
+            const uint32_t idx = b + (uint32_t)li;
+            int an = lane < kk && idx < vlen ? active_entry(alist, idx, true) : -1;
             if (an >= a.max_nodes) { fail = true; an = -1; }
             const bool he = an >= 0;
             const OctRec ar = nodes[he ? an : 0];
@@ -399,22 +403,28 @@ __device__ void fmm_leaf_group(const FmmArgs &a, int P, const uint32_t *alist, u
             if (far) fmm_to_local(ar.com.x, ar.com.y, ar.com.z, ar.com.w, cl, eps2, Fp);
             c_nn += __popcll(__ballot(far));
         }
-        for (uint32_t b = 0; b < elen; b += kWave) {
-            bool he = b + lane < elen;
-            const int e = he ? (int)elist[b + lane] : 0;
+        for (uint32_t b = 0; b < elen; b += (uint32_t)epb) {
+            const uint32_t idx = b + (uint32_t)li;
+            bool he = lane < kk && idx < elen;
+            const int e = he ? (int)elist[idx] : 0;
             if (he && (e < 0 || (int64_t)e >= a.n)) { fail = true; he = false; }
             const float4 bd = a.bodies[he ? e : 0];
             const bool far = he && fmm_particle_far(bd.x, bd.y, bd.z, cl, theta);
             if (far) fmm_to_local(bd.x, bd.y, bd.z, bd.w, cl, eps2, Fp);
             c_pn += __popcll(__ballot(far));
         }
-        const float s0 = wave_fsum(Fp.ax), s1 = wave_fsum(Fp.ay), s2 = wave_fsum(Fp.az);
-        const float s3 = wave_fsum(Fp.t0), s4 = wave_fsum(Fp.t1), s5 = wave_fsum(Fp.t2);
-        const float s6 = wave_fsum(Fp.t3), s7 = wave_fsum(Fp.t4), s8 = wave_fsum(Fp.t5);
-        if (lane == 0) {
-            g.F[j][0] += s0; g.F[j][1] += s1; g.F[j][2] += s2;
-            g.F[j][3] += s3; g.F[j][4] += s4; g.F[j][5] += s5;
-            g.F[j][6] += s6; g.F[j][7] += s7; g.F[j][8] += s8;
+        for (int j = 0; j < k; ++j) {
+            const bool mine = lane < kk && lj == j;
+            const float s0 = wave_fsum(mine ? Fp.ax : 0.0f), s1 = wave_fsum(mine ? Fp.ay : 0.0f);
+            const float s2 = wave_fsum(mine ? Fp.az : 0.0f), s3 = wave_fsum(mine ? Fp.t0 : 0.0f);
+            const float s4 = wave_fsum(mine ? Fp.t1 : 0.0f), s5 = wave_fsum(mine ? Fp.t2 : 0.0f);
+            const float s6 = wave_fsum(mine ? Fp.t3 : 0.0f), s7 = wave_fsum(mine ? Fp.t4 : 0.0f);
+            const float s8 = wave_fsum(mine ? Fp.t5 : 0.0f);
+            if (lane == 0) {
+                g.F[j][0] += s0; g.F[j][1] += s1; g.F[j][2] += s2;
+                g.F[j][3] += s3; g.F[j][4] += s4; g.F[j][5] += s5;
+                g.F[j][6] += s6; g.F[j][7] += s7; g.F[j][8] += s8;
+            }
         }
     }
     wave_mem_sync();
