@@ -68,6 +68,53 @@ def test_fmm_matches_oracle(nbg, orc, gpu_ctx, n, leaf, theta, max_depth):
     assert w <= FAST_TOL
 
 
+def test_fmm_matches_oracle_200k(nbg, orc, gpu_ctx):
+    """A larger case against the oracle (about 7 s of CPU for the oracle)."""
+    n = 200000
+    b = nbg.plummer(n, seed=21)
+    s, acc = _gpu(nbg, gpu_ctx, b, 16)
+    ref, c = _oracle(orc, b, 16)
+    assert s.counts4() == c
+    w = _worst(acc, ref)
+    print(f"  counts {c}, worst {w:.3g}", flush=True)
+    assert w <= FAST_TOL
+
+
+def test_fmm_two_clusters(nbg, orc, gpu_ctx):
+    """Two Plummer spheres far apart (the colliding-spheres configuration at a
+    small size): deep subtrees under a shallow root, far fields across the gap."""
+    n = 20000
+    a1, a2 = nbg.plummer(n, seed=31), nbg.plummer(n, seed=32)
+    b = nbg.Bodies.empty(2 * n)
+    for f in ("x", "y", "z", "vx", "vy", "vz"):
+        getattr(b, f)[:n] = getattr(a1, f)
+        getattr(b, f)[n:] = getattr(a2, f)
+    b.x[:n] -= 10.0; b.x[n:] += 10.0
+    b.y[:n] += 1.0; b.y[n:] -= 1.0
+    b.m[:] = 0.5 / n
+    s, acc = _gpu(nbg, gpu_ctx, b, 16, eps2=4e-6)
+    ref, c = _oracle(orc, b, 16, eps2=4e-6)
+    assert s.counts4() == c
+    assert _worst(acc, ref) <= FAST_TOL
+
+
+def test_fmm_small_theta_grows_the_pool(nbg, orc, gpu_ctx):
+    """theta = 0.1: long close lists outgrow the list pool's first size (32
+    words per body plus 64K), so the pass is repeated with a larger pool; the
+    result is the oracle's, and the next pass starts with the larger pool."""
+    n = 30000
+    b = nbg.plummer(n, seed=23)
+    s, acc = _gpu(nbg, gpu_ctx, b, 16, theta=0.1)
+    ref, c = _oracle(orc, b, 16, theta=0.1)
+    ws = s.walk_stats()
+    print(f"  counts {c}, traversal {ws}", flush=True)
+    assert ws["pool_capacity"] > 32 * n + (1 << 16) and ws["pool_peak"] <= ws["pool_capacity"]
+    assert s.counts4() == c
+    assert _worst(acc, ref) <= FAST_TOL
+    again = s.update_accelerations()
+    assert np.array_equal(again, acc) and s.walk_stats()["pool_capacity"] == ws["pool_capacity"]
+
+
 def test_fmm_uniform_lattice(nbg, orc, gpu_ctx):
     """A cubic lattice of equal masses: many nodes hold equal body counts, so
     the load-balanced break resolves heap ties (std::push_heap/pop_heap) and
