@@ -87,20 +87,21 @@ def cpu_baseline(b, args, solver, threads):
     t_build = time.perf_counter() - t0
     if solver == 5:
         # the implicit FMM has no per-target walk: time the traversals from a
-        # sample of its load-balanced starting nodes, scaled by the bodies held
+        # sample of its load-balanced starting nodes (one per thread at a time,
+        # as the reference's tbb::parallel_for_each), scaled by the bodies held
         t0 = time.perf_counter()
-        covered = t.fmm_sample(args.cpu_fmm_starts)
+        covered = t.fmm_sample(args.cpu_fmm_starts, threads=threads)
         t_walk = (time.perf_counter() - t0) * (n / max(1, covered))
         step_s = t_build + t_walk
         del t
         return {
             "value": n / step_s,
             "unit": "bodies·steps/s",
-            "cores": 1,
+            "cores": threads,
             "kind": "port",
-            "sample": f"oracle implicit FMM (single thread): one build over all {n} bodies + the dual traversals "
-                      f"from {args.cpu_fmm_starts} of its starting nodes ({covered} bodies), scaled by N/{covered} "
-                      f"(build {t_build:.2f} s, traversal {t_walk:.2f} s extrapolated)",
+            "sample": f"oracle implicit FMM on {threads} threads: one build over all {n} bodies + the dual "
+                      f"traversals from {args.cpu_fmm_starts} of its starting nodes ({covered} bodies), scaled by "
+                      f"N/{covered} (build {t_build:.2f} s, traversal {t_walk:.2f} s extrapolated)",
         }
     sample = min(n, args.cpu_sample)
     sel = np.linspace(0, n - 1, sample).astype(np.int64)
@@ -460,7 +461,7 @@ def main():
     ap.add_argument("--theta-fmm", type=float, default=0.4,
                     help="implicit FMM opening angle (its descent criterion's default)")
     ap.add_argument("--cpu-sample", type=int, default=131072)
-    ap.add_argument("--cpu-fmm-starts", type=int, default=12,
+    ap.add_argument("--cpu-fmm-starts", type=int, default=64,
                     help="starting nodes whose traversals time the oracle's implicit FMM")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")

@@ -51,9 +51,9 @@ def lib() -> ctypes.CDLL:
             "orc_tree_nodes": (ctypes.c_int, [_vp, _i64p, _i64p, _i32p, _i32p, fp, fp]),
             "orc_tree_parents": (ctypes.c_int, [_vp, _i32p]),
             "orc_tree_quads": (ctypes.c_int, [_vp, fp]),
-            "orc_fmm_accel": (ctypes.c_int, [_vp, fp, fp, fp, _u64p, _f64p]),
+            "orc_fmm_accel": (ctypes.c_int, [_vp, fp, fp, fp, _u64p, _f64p, ctypes.c_int32]),
             "orc_fmm_starts": (ctypes.c_int64, [_vp, _i32p, ctypes.c_int64]),
-            "orc_fmm_sample": (ctypes.c_int64, [_vp, ctypes.c_int32]),
+            "orc_fmm_sample": (ctypes.c_int64, [_vp, ctypes.c_int32, ctypes.c_int32]),
             "orc_fmm_p2l": (ctypes.c_int, [fp, ctypes.c_float, fp, fp, ctypes.c_float, ctypes.c_float, fp]),
             "orc_node_particle_quad": (ctypes.c_int, [fp, ctypes.c_float, fp, fp, ctypes.c_float, ctypes.c_float,
                                                       fp]),
@@ -158,12 +158,12 @@ class Tree:
         lib().orc_tree_quads(self.h, _p(q))
         return q
 
-    def fmm_accel(self, counts=False):
+    def fmm_accel(self, counts=False, threads=0):
         """Implicit FMM accelerations of every body (original order) and the
         counts (pp, np, pn, nn)."""
         a = [np.zeros(self.n, np.float32) for _ in range(3)]
         c = np.zeros(4, np.uint64)
-        rc = lib().orc_fmm_accel(self.h, *[_p(v) for v in a], _p(c, _u64p), None)
+        rc = lib().orc_fmm_accel(self.h, *[_p(v) for v in a], _p(c, _u64p), None, int(threads))
         if rc != 0:
             raise ValueError("not an FMM tree")
         acc = np.stack(a, axis=1)
@@ -174,14 +174,15 @@ class Tree:
         FMM's traversal reaches it with (each source once: the total mass)."""
         out = np.zeros(self.n, np.float64)
         c = np.zeros(4, np.uint64)
-        if lib().orc_fmm_accel(self.h, None, None, None, _p(c, _u64p), _p(out, _f64p)) != 0:
+        if lib().orc_fmm_accel(self.h, None, None, None, _p(c, _u64p), _p(out, _f64p), 0) != 0:
             raise ValueError("not an FMM tree")
         return out
 
-    def fmm_sample(self, k):
+    def fmm_sample(self, k, threads=0):
         """Runs the implicit FMM's traversals from its first k starting nodes
-        (a timing sample); returns the number of bodies they hold."""
-        r = lib().orc_fmm_sample(self.h, int(k))
+        (a timing sample) on `threads` threads (0: all); returns the number of
+        bodies they hold."""
+        r = lib().orc_fmm_sample(self.h, int(k), int(threads))
         if r < 0:
             raise ValueError("not an FMM tree")
         return int(r)

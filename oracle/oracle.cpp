@@ -600,16 +600,14 @@ struct FMMCounts {
     uint64_t pp = 0, np = 0, pn = 0, nn = 0;  // SimpleTrackingRule's four counters (empty sources excluded)
 };
 
-struct FMM {
-    const Octree &T;
-    float theta;
-    bool account;
+// Results shared by the traversals of all starting nodes: each body lies under
+// exactly one starting node, so the traversals (one per thread at a time, as the
+// reference's tbb::parallel_for_each runs them) write disjoint entries.
+struct FMMShared {
     std::vector<V3> acc;       // by original body index
-    std::vector<double> recv;   // accounting mode: mass received by each body
-    std::vector<double> nmass;  // accounting mode: FP64 mass of every node's bodies
-    FMMCounts cnt;
-
-    FMM(const Octree &t, bool acc_mode) : T(t), theta(t.prm.theta), account(acc_mode) {
+    std::vector<double> recv;  // accounting mode: mass received by each body
+    std::vector<double> nmass; // accounting mode: FP64 mass of every node's bodies
+    FMMShared(const Octree &T, bool account) {
         acc.assign(T.n, kZero);
         recv.assign(T.n, 0.0);
         if (account) {
@@ -619,6 +617,20 @@ struct FMM {
                     nmass[i] += (double)T.Mv[T.idx[j]];
         }
     }
+};
+
+struct FMM {
+    const Octree &T;
+    float theta;
+    bool account;
+    FMMShared &sh;
+    std::vector<V3> &acc;
+    std::vector<double> &recv, &nmass;
+    FMMCounts cnt;
+
+    FMM(const Octree &t, bool acc_mode, FMMShared &shared)
+        : T(t), theta(t.prm.theta), account(acc_mode), sh(shared), acc(shared.acc), recv(shared.recv),
+          nmass(shared.nmass) {}
 
     const ONode &nd(int32_t i) const { return T.nodes[i]; }
     V3 pos(uint32_t e) const { return T.P[e]; }
@@ -823,12 +835,34 @@ struct FMM {
         return q;
     }
 
-    // ImplicitDualTraversalSolver::updateAccelerations (:46-68).
-    void run() {
-        if (T.n == 0) return;
-        for (int32_t s : starting_nodes(256)) balanced(s);
-    }
 };
+
+// ImplicitDualTraversalSolver::updateAccelerations (:46-68): the traversals of
+// the first `k` starting nodes (all of them for k < 0), on `threads` threads
+// (0: OpenMP's default); returns the bodies those starting nodes hold.
+int64_t fmm_run(const Octree &T, FMMShared &sh, bool account, int k, int threads, FMMCounts &total) {
+    if (T.n == 0) return 0;
+    FMM f0(T, account, sh);
+    std::vector<int32_t> st = f0.starting_nodes(256);
+    if (k >= 0 && (size_t)k < st.size()) st.resize((size_t)k);
+    int64_t covered = 0;
+    for (int32_t s : st) covered += T.nodes[s].count;
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel
+    {
+        FMM f(T, account, sh);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t i = 0; i < (int64_t)st.size(); ++i) f.balanced(st[(size_t)i]);
+#pragma omp critical
+        {
+            total.pp += f.cnt.pp;
+            total.np += f.cnt.np;
+            total.pn += f.cnt.pn;
+            total.nn += f.cnt.nn;
+        }
+    }
+    return covered;
+}
 
 struct Handle {
     int kind;  // 0 octree, 1 lbvh
@@ -1042,16 +1076,19 @@ int orc_tree_accel(void *hp, int64_t nt, const float *tx, const float *ty, const
 // original order, and its four interaction counts (pp, np, pn, nn).  With
 // mass_or_null the same traversal instead adds each source's mass to every
 // target it reaches (FP64; the local fields carry mass too).
-int orc_fmm_accel(void *hp, float *ax, float *ay, float *az, uint64_t *counts4, double *mass_or_null) {
+int orc_fmm_accel(void *hp, float *ax, float *ay, float *az, uint64_t *counts4, double *mass_or_null,
+                  int32_t threads) {
     Handle *h = (Handle *)hp;
     if (!h->fmm) return -1;
-    FMM f(*h->o, mass_or_null != nullptr);
-    f.run();
+    const bool account = mass_or_null != nullptr;
+    FMMShared sh(*h->o, account);
+    FMMCounts c;
+    fmm_run(*h->o, sh, account, -1, threads, c);
     for (int64_t i = 0; i < h->o->n; ++i) {
-        if (mass_or_null) mass_or_null[i] = f.recv[i];
-        else { ax[i] = f.acc[i].x; ay[i] = f.acc[i].y; az[i] = f.acc[i].z; }
+        if (mass_or_null) mass_or_null[i] = sh.recv[i];
+        else { ax[i] = sh.acc[i].x; ay[i] = sh.acc[i].y; az[i] = sh.acc[i].z; }
     }
-    if (counts4) { counts4[0] = f.cnt.pp; counts4[1] = f.cnt.np; counts4[2] = f.cnt.pn; counts4[3] = f.cnt.nn; }
+    if (counts4) { counts4[0] = c.pp; counts4[1] = c.np; counts4[2] = c.pn; counts4[3] = c.nn; }
     return 0;
 }
 
@@ -1064,7 +1101,8 @@ int orc_fmm_p2l(const float *s3, float m, const float *c3, const float *p3, floa
     t.prm.eps2 = eps2;
     t.prm.theta = 0.4f;
     t.n = 0;
-    FMM f(t, false);
+    FMMShared sh(t, false);
+    FMM f(t, false, sh);
     Local L;
     f.to_local(V3{s3[0], s3[1], s3[2]}, m, V3{c3[0], c3[1], c3[2]}, L, 0.0);
     const V3 a = local_at(L, V3{p3[0] - c3[0], p3[1] - c3[1], p3[2] - c3[2]});
@@ -1075,25 +1113,20 @@ int orc_fmm_p2l(const float *s3, float m, const float *c3, const float *p3, floa
 // The implicit FMM's traversals from its first k starting nodes only (a CPU
 // timing sample of the whole pass); returns the number of bodies those
 // starting nodes hold.
-int64_t orc_fmm_sample(void *hp, int32_t k) {
+int64_t orc_fmm_sample(void *hp, int32_t k, int32_t threads) {
     Handle *h = (Handle *)hp;
     if (!h->fmm) return -1;
-    FMM f(*h->o, false);
-    if (h->o->n == 0) return 0;
-    const std::vector<int32_t> st = f.starting_nodes(256);
-    int64_t covered = 0;
-    for (int32_t i = 0; i < k && i < (int32_t)st.size(); ++i) {
-        f.balanced(st[i]);
-        covered += f.nd(st[i]).count;
-    }
-    return covered;
+    FMMShared sh(*h->o, false);
+    FMMCounts c;
+    return fmm_run(*h->o, sh, false, k < 0 ? 0 : k, threads, c);
 }
 
 // The starting passive nodes of the implicit FMM (pre-order node ids).
 int64_t orc_fmm_starts(void *hp, int32_t *out, int64_t cap) {
     Handle *h = (Handle *)hp;
     if (!h->fmm) return -1;
-    FMM f(*h->o, false);
+    FMMShared sh(*h->o, false);
+    FMM f(*h->o, false, sh);
     const std::vector<int32_t> s = f.starting_nodes(256);
     for (size_t i = 0; i < s.size() && (int64_t)i < cap; ++i) out[i] = s[i];
     return (int64_t)s.size();
@@ -1107,7 +1140,7 @@ int orc_step(const int *ip, const float *fp, int64_t n, float *x, float *y, floa
     for (int s = 0; s < nsteps; ++s) {
         void *h = orc_tree_new(ip, fp, n, x, y, z, m);
         if (!h) return -1;
-        if (((Handle *)h)->fmm) orc_fmm_accel(h, ax, ay, az, nullptr, nullptr);
+        if (((Handle *)h)->fmm) orc_fmm_accel(h, ax, ay, az, nullptr, nullptr, threads);
         else orc_tree_accel(h, n, x, y, z, ax, ay, az, nullptr, nullptr, threads);
         orc_tree_free(h);
         for (int64_t i = 0; i < n; ++i) {
