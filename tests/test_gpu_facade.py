@@ -43,3 +43,21 @@ def test_facade_matches_oracle(nbg, orc, gpu_ctx, tmp_path, sid, solver):
                                               leaf=16, bits=63, eps2=1e-4, dt=1e-3)
     for g, ref in zip(after, (px, py, pz, pvx, pvy, pvz)):
         assert np.array_equal(g, ref)
+
+
+def test_facade_implicit_fmm(nbg, orc, gpu_ctx, tmp_path):
+    """select("gpu-implicit-fmm-2p"): the C++ facade's implicit FMM (theta 0.4,
+    FAST) agrees with the oracle's within the FAST tolerance, steps included."""
+    n = 2000
+    out = tmp_path / "fc_fmm.bin"
+    r = subprocess.run([PROG, "gpu-implicit-fmm-2p", str(n), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    (x, y, z, m), acc, after = _read(out, n)
+    ref = orc.Tree(5, x, y, z, m, theta=0.4, leaf=16, eps2=1e-4).fmm_accel()
+    mag = np.linalg.norm(ref, axis=1)
+    err = np.linalg.norm(acc - ref, axis=1)
+    assert np.max(err / np.maximum(mag, np.sqrt(np.mean(mag ** 2)))) <= 1e-5
+    b = nbg.plummer(n, seed=7)
+    (px, py, pz, pvx, pvy, pvz), _ = orc.step(5, (b.x, b.y, b.z, b.m, b.vx, b.vy, b.vz), 2, theta=0.4, leaf=16,
+                                              bits=63, eps2=1e-4, dt=1e-3)
+    assert np.max(np.abs(after[0] - px)) < 1e-6 and np.max(np.abs(after[3] - pvx)) < 1e-4
