@@ -24,9 +24,9 @@ bool is_octree(int solver) {
     return solver == NBG_BARNES_HUT || solver == NBG_BARNES_HUT_QUADRUPOLE || solver == NBG_IMPLICIT_FMM;
 }
 // Implicit FMM: waves of a traversal step, the seed queues' capacity, and
-// the list pool per step parity (words per body plus a floor).
+// the list pool of a pass (words per body plus a floor).
 constexpr int kFmmWaves = 5120, kFmmSeedCap = 16384;
-constexpr int64_t kFmmPoolPerBody = 32, kFmmPoolFloor = (int64_t)1 << 16, kFmmPoolMax = (int64_t)1 << 31;
+constexpr int64_t kFmmPoolPerBody = 64, kFmmPoolFloor = (int64_t)1 << 17, kFmmPoolMax = (int64_t)1 << 31;
 bool is_quad(int solver) { return solver == NBG_BARNES_HUT_QUADRUPOLE || solver == NBG_LINEAR_BVH_QUADRUPOLE; }
 constexpr int kEventSets = 256;
 
@@ -192,7 +192,7 @@ int ensure_fmm(nbg_sys *s) {
     auto A = [&](auto **ptr, int64_t count) {
         if (r == NBG_OK) r = dalloc(ptr, count);
     };
-    A(&s->fmm_items, 2 * item_cap * kFmmItemWords); A(&s->fmm_pool, 2 * pool_cap);
+    A(&s->fmm_items, 3 * item_cap * kFmmItemWords); A(&s->fmm_pool, pool_cap);
     A(&s->fmm_seed, (int64_t)kFmmSeedWaves * 3 * seed_cap); A(&s->fmm_misc, 64);
     A(&s->fmm_starts, kFmmStartCap);
     if (r != NBG_OK) {
@@ -324,12 +324,14 @@ int enqueue_force_pass(nbg_sys *s) {
         fa.theta = p.theta; fa.G = p.G; fa.eps2 = p.eps2;
         fa.starts = s->fmm_starts;
         fa.ctr = s->fmm_misc;
+        fa.pool_ctr = s->fmm_misc + 12;
+        fa.group_ctr = s->fmm_misc + 13;
         fa.peak = s->fmm_misc + 16;
         fa.nstarts = s->fmm_misc + 20;
         fa.items[0] = s->fmm_items;
         fa.items[1] = s->fmm_items + s->fmm_item_cap * kFmmItemWords;
-        fa.pool[0] = s->fmm_pool;
-        fa.pool[1] = s->fmm_pool + s->fmm_pool_cap;
+        fa.groups = s->fmm_items + 2 * s->fmm_item_cap * kFmmItemWords;
+        fa.pool = s->fmm_pool;
         fa.item_cap = s->fmm_item_cap;
         fa.pool_cap = s->fmm_pool_cap;
         fa.seed_scratch = s->fmm_seed;

@@ -25,11 +25,13 @@
 // entries at a time with the reference's criteria (SideLengthOverDistance,
 // node-node and particle-node forms, FP32 in the reference's operation
 // order), so every decision, and with it every interaction count, is the
-// reference's.  Lists live in a pool per step parity (a step reads the one
-// its predecessor wrote); a wave reserves its output lists with one atomic
-// add.  Far-field contributions are per-lane partial local fields summed once
-// per node.  Empty octants, which the reference keeps as body-less nodes,
-// contribute nothing and have no record here.
+// reference's.  Lists live in one pool for the pass; a wave reserves its
+// output lists with one atomic add.  Leaf children are collected as leaf
+// groups (up to 64 bodies) and served by a last launch over all of them, so
+// the heavy near-field work is one load-balanced launch rather than the tail
+// of every step.  Far-field contributions are per-lane partial local fields
+// summed once per node.  Empty octants, which the reference keeps as
+// body-less nodes, contribute nothing and have no record here.
 
 #include "device.hpp"
 #include "launch.hpp"
@@ -557,13 +559,13 @@ __global__ __launch_bounds__(64) void k_fmm_seed(FmmArgs a) {
     }
     bad = __ballot(bad) != 0;
     uint32_t *ctr = a.ctr;  // step 0's input set
-    const int64_t off = bad || over ? -1 : pool_reserve(ctr + 2, eq, a.pool_cap, a.peak);
+    const int64_t off = bad || over ? -1 : pool_reserve(a.pool_ctr, eq, a.pool_cap, a.peak);
     if (off < 0) {
         if (lane == 0) atomicOr(a.err, bad ? 2u : 16u);
         return;
     }
     for (int b = 0; b < eq; b += kWave)
-        if (b + lane < eq) a.pool[0][off + b + lane] = EQ[b + lane];
+        if (b + lane < eq) a.pool[off + b + lane] = EQ[b + lane];
     uint32_t slot = 0;
     if (lane == 0) slot = atomicAdd(ctr + 0, 1u);
     slot = __shfl(slot, 0, kWave);
@@ -582,7 +584,6 @@ __global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
     __shared__ int s_grp[kStackOct];
     __shared__ uint64_t s_msk[kStackOct];
     __shared__ uint32_t s_desc[8];
-    __shared__ GroupLds s_g;
 
     const int lane = lane_id();
     const OctRec *__restrict__ nodes = a.nodes;
@@ -592,8 +593,8 @@ __global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
     if (blockIdx.x == 0 && lane < 4) cr[lane] = 0u;
     const uint32_t *__restrict__ itin = a.items[k & 1];
     uint32_t *itout = a.items[(k + 1) & 1];
-    const uint32_t *pin = a.pool[k & 1];
-    uint32_t *pout = a.pool[(k + 1) & 1];
+    const uint32_t *pin = a.pool;
+    uint32_t *pout = a.pool;
     // a step that overflowed the frontier counted past the capacity (and flagged it)
     const int nin = (int)(ci[0] < (uint64_t)a.item_cap ? ci[0] : (uint32_t)a.item_cap);
     if ((int)blockIdx.x >= nin) return;  // (steps past the deepest leaf launch with nin = 0)
@@ -615,12 +616,9 @@ __global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
             fail = true;
             break;
         }
-        if (flags & kItemLeafGroup) {
-            fmm_leaf_group<SOFT>(a, P, pin + aoff, alen, pin + eoff, elen, it, (uint32_t)uniform_i32((int)it[kItLeaves]),
-                                 s_g, s_grp, s_msk, c_pp, c_np, c_pn, c_nn, fail);
-            fail = __ballot(fail) != 0;
-            if (fail) break;
-            continue;
+        if (flags & kItemLeafGroup) {  // leaf groups never enter the frontier
+            fail = true;
+            break;
         }
         const OctRec pr = nodes[P];
         const float4 cP = cen[P];
@@ -654,7 +652,7 @@ __global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
         // close leaves; then the close bodies
         const int nclose = nleafc + ninner;
         const int64_t ecap = (int64_t)elen + (leaf ? 0 : (int64_t)wave_sum(lbod));
-        const int64_t off = pool_reserve(co + 2, nclose + ecap, a.pool_cap, a.peak);
+        const int64_t off = pool_reserve(a.pool_ctr, nclose + ecap, a.pool_cap, a.peak);
         if (off < 0) {
             over = true;
             break;
@@ -757,25 +755,33 @@ __global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
                 if (lane == 0) s_desc[nit] = 0x80000000u | gmask;
                 ++nit;
             }
-            uint32_t base = 0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t d = lane < nit ? s_desc[lane] : 0u;
+            const uint64_t gb = __ballot(lane < nit && (d & 0x80000000u) != 0u);
+            const uint64_t nb = __ballot(lane < nit && (d & 0x80000000u) == 0u);
+            const int ngrp = __popcll(gb), nnode = __popcll(nb);
+            uint32_t base = 0, gbase = 0;
             if (lane == 0) {
-                base = atomicAdd(co + 0, (uint32_t)nit);
-                atomicMax(a.peak + 1, base + (uint32_t)nit);
+                if (nnode) {
+                    base = atomicAdd(co + 0, (uint32_t)nnode);
+                    atomicMax(a.peak + 1, base + (uint32_t)nnode);
+                }
+                if (ngrp) gbase = atomicAdd(a.group_ctr, (uint32_t)ngrp);
             }
             base = __shfl(base, 0, kWave);
-            if ((int64_t)base + nit > a.item_cap) {
+            gbase = __shfl(gbase, 0, kWave);
+            if ((int64_t)base + nnode > a.item_cap || (int64_t)gbase + ngrp > a.item_cap) {
                 over = true;
                 break;
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
             if (lane < nit) {
-                const uint32_t d = s_desc[lane];
-                uint32_t *dst = itout + (int64_t)(base + lane) * kFmmItemWords;
                 if (d & 0x80000000u) {
+                    uint32_t *dst = a.groups + (int64_t)(gbase + __popcll(gb & lanemask_lt())) * kFmmItemWords;
                     write_item(dst, P, (uint32_t)off, (uint32_t)ng, (uint32_t)(off + nclose), (uint32_t)ne, F,
                                kItemGroups | kItemLeafGroup, d & 0xFFu);
                 } else {
+                    uint32_t *dst = itout + (int64_t)(base + __popcll(nb & lanemask_lt())) * kFmmItemWords;
                     const int c = (int)d;
                     const float4 cc = cen[c];
                     LocalF Fc = F;
@@ -837,6 +843,46 @@ __global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
     }
 }
 
+// The passive leaves, in groups collected by all the steps: one launch, one
+// group per wave at a time.
+template <bool SOFT>
+__global__ __launch_bounds__(64) void k_fmm_groups(FmmArgs a) {
+    __shared__ int s_grp[kStackOct];
+    __shared__ uint64_t s_msk[kStackOct];
+    __shared__ GroupLds s_g;
+
+    const int lane = lane_id();
+    const int64_t ng = a.group_ctr[0] < (uint64_t)a.item_cap ? (int64_t)a.group_ctr[0] : a.item_cap;
+    if ((int64_t)blockIdx.x >= ng) return;
+    uint64_t c_pp = 0, c_np = 0, c_pn = 0, c_nn = 0;
+    bool fail = false;
+    for (;;) {
+        int job = 0;
+        if (lane == 0) job = (int)atomicAdd(a.group_ctr + 1, 1u);
+        job = uniform_i32(__shfl(job, 0, kWave));
+        if (job >= ng) break;
+        const uint32_t *it = a.groups + (int64_t)job * kFmmItemWords;
+        const int P = uniform_i32((int)it[kItP]);
+        const uint32_t aoff = (uint32_t)uniform_i32((int)it[kItAoff]), alen = (uint32_t)uniform_i32((int)it[kItAlen]);
+        const uint32_t eoff = (uint32_t)uniform_i32((int)it[kItEoff]), elen = (uint32_t)uniform_i32((int)it[kItElen]);
+        if (P < 0 || P >= a.max_nodes || (int64_t)aoff + alen > a.pool_cap || (int64_t)eoff + elen > a.pool_cap) {
+            fail = true;
+            break;
+        }
+        fmm_leaf_group<SOFT>(a, P, a.pool + aoff, alen, a.pool + eoff, elen, it, (uint32_t)uniform_i32((int)it[kItLeaves]),
+                             s_g, s_grp, s_msk, c_pp, c_np, c_pn, c_nn, fail);
+        fail = __ballot(fail) != 0;
+        if (fail) break;
+    }
+    if (lane == 0) {
+        if (fail) atomicOr(a.err, 2u);
+        if (c_pp) atomicAdd(&a.counts[0], (unsigned long long)c_pp);
+        if (c_np) atomicAdd(&a.counts[1], (unsigned long long)c_np);
+        if (c_pn) atomicAdd(&a.counts[2], (unsigned long long)c_pn);
+        if (c_nn) atomicAdd(&a.counts[3], (unsigned long long)c_nn);
+    }
+}
+
 // After the last step: a non-empty next frontier means the tree was deeper
 // than the steps launched.
 __global__ void k_fmm_tail(const uint32_t *ctr_next, uint32_t *peak, int steps, uint32_t *err) {
@@ -851,6 +897,8 @@ __global__ void k_fmm_tail(const uint32_t *ctr_next, uint32_t *peak, int steps, 
 void launch_fmm(const FmmArgs &a, hipStream_t s) {
     if (a.n <= 0) return;
     launch_fill_u32(a.ctr, 0u, 12, s);
+    launch_fill_u32(a.pool_ctr, 0u, 1, s);
+    launch_fill_u32(a.group_ctr, 0u, 2, s);
     launch_fill_u32(a.peak, 0u, 4, s);
     hipLaunchKernelGGL(k_fmm_starts, dim3(1), dim3(64), 0, s, a.nodes, a.misc, a.starts, a.nstarts, a.err);
     hipLaunchKernelGGL(k_fmm_seed, dim3(kHeapCap), dim3(64), 0, s, a);
@@ -858,6 +906,8 @@ void launch_fmm(const FmmArgs &a, hipStream_t s) {
         if (a.eps2 > 0.0f) hipLaunchKernelGGL(k_fmm_step<true>, dim3(a.waves), dim3(64), 0, s, a, k);
         else hipLaunchKernelGGL(k_fmm_step<false>, dim3(a.waves), dim3(64), 0, s, a, k);
     }
+    if (a.eps2 > 0.0f) hipLaunchKernelGGL(k_fmm_groups<true>, dim3(kFmmGroupWaves), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL(k_fmm_groups<false>, dim3(kFmmGroupWaves), dim3(64), 0, s, a);
     hipLaunchKernelGGL(k_fmm_tail, dim3(1), dim3(64), 0, s, a.ctr + 4 * (a.steps % 3), a.peak, a.steps, a.err);
 }
 

@@ -243,11 +243,14 @@ void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, int diag, h
 void launch_walk_oct(const WalkArgs &a, int ordered, int diag, hipStream_t s);
 
 // Implicit FMM (fmm.hip): the lockstep dual traversal breadth-first, one
-// launch per traversal step.  Frontier items (kFmmItemWords each) and the
-// list pool alternate between two halves by step parity; ctr holds three
-// rotating sets of {items, next item, pool words, -}.  counts: [pp, np, pn,
-// nn]; err bit 16: a list or the frontier outgrew its capacity.
+// launch per traversal step, then one launch for the passive leaves.
+// Frontier items (kFmmItemWords each) alternate between two halves by step
+// parity; ctr holds three rotating sets of {items, next item, -, -}; leaf
+// groups collect in `groups` for the last launch; lists live in one pool for
+// the whole pass.  counts: [pp, np, pn, nn]; err bit 16: a list, the frontier
+// or the leaf groups outgrew their capacity.
 constexpr int kFmmItemWords = 16;
+constexpr int kFmmGroupWaves = 7168;  // the leaf-group launch (7 waves per SIMD)
 constexpr int kFmmStartCap = 512;
 constexpr int kFmmSeedWaves = 272;  // one per starting node (at most 256 + 7)
 struct FmmArgs {
@@ -263,9 +266,12 @@ struct FmmArgs {
     int32_t *starts;          // kFmmStartCap
     uint32_t *nstarts;
     uint32_t *ctr;            // 12 words
+    uint32_t *pool_ctr;       // pool words used
+    uint32_t *group_ctr;      // [0] leaf groups, [1] next leaf group
     uint32_t *peak;           // 4 words: pool words, items (largest step), steps run
     uint32_t *items[2];       // item_cap items each
-    uint32_t *pool[2];        // pool_cap words each
+    uint32_t *groups;         // item_cap items
+    uint32_t *pool;           // pool_cap words
     int64_t item_cap, pool_cap;
     uint32_t *seed_scratch;   // kFmmSeedWaves x 3 x cap_a words
     int cap_a, waves, steps;
