@@ -250,7 +250,7 @@ void launch_walk_oct(const WalkArgs &a, int ordered, int diag, hipStream_t s);
 // the whole pass.  counts: [pp, np, pn, nn]; err bit 16: a list, the frontier
 // or the leaf groups outgrew their capacity.
 constexpr int kFmmItemWords = 16;
-constexpr int kFmmGroupWaves = 7168;  // the leaf-group launch (7 waves per SIMD)
+constexpr int kFmmGroupWaves = 8192;  // the leaf-group launch (8 waves per SIMD)
 constexpr int kFmmStartCap = 512;
 constexpr int kFmmSeedWaves = 272;  // one per starting node (at most 256 + 7)
 struct FmmArgs {
