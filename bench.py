@@ -37,7 +37,7 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (packed) peak
 FLOPS_PER_INTERACTION = 20
 SOLVER_NAMES = {0: "barnes-hut", 1: "linear-bvh", 3: "barnes-hut-4p", 4: "linear-bvh-4p", 5: "implicit-fmm"}
 WALK_KERNELS = {0: "k_walk_oct_fast", 1: "k_walk_lbvh_fast", 3: "k_walk_oct_fast", 4: "k_walk_lbvh_fast",
-                5: "k_fmm_step"}
+                5: "k_fmm_groups + k_fmm_step (the whole traversal)"}
 # FP32 operations of one body or node into a local field (difference, r^2,
 # rsqrt and powers, the vector's 3 and the gradient's 6 accumulating FMAs).
 FLOPS_PER_LOCAL = 40
