@@ -641,3 +641,14 @@ def test_direct_f32_is_the_sequential_reference_sum(orc, nbg):
                 r = ((((-R) * b.m[j]).astype(f32) * f32(1.0)).astype(f32) / f32(rr * f32(rr * rr))).astype(f32)
             acc = (acc + r).astype(f32)
         assert np.array_equal(got[i], acc), i
+
+
+def test_fmm_threads_do_not_change_results(orc, nbg):
+    """The oracle runs the starting nodes' traversals on OpenMP threads (as the
+    reference's parallel_for_each); every body belongs to one starting node,
+    so results and counts are independent of the thread count."""
+    b = nbg.plummer(20000, seed=44)
+    t = orc.Tree(5, b.x, b.y, b.z, b.m, theta=0.4, leaf=16, eps2=1e-4)
+    a1, c1 = t.fmm_accel(counts=True, threads=1)
+    a4, c4 = t.fmm_accel(counts=True, threads=4)
+    assert c1 == c4 and np.array_equal(a1, a4)
