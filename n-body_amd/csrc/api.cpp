@@ -79,6 +79,7 @@ struct nbg_sys {
     // implicit FMM work (allocated for that solver): frontier items and list
     // pool (two halves each), seed queues, counters, starting nodes
     uint32_t *fmm_items = nullptr, *fmm_pool = nullptr, *fmm_seed = nullptr, *fmm_misc = nullptr;
+    uint32_t *fmm_ulist = nullptr;
     int32_t *fmm_starts = nullptr;
     int64_t fmm_item_cap = 0, fmm_pool_cap = 0;
     int fmm_seed_cap = 0;
@@ -167,10 +168,10 @@ int ensure_quad(nbg_sys *s) {
 }
 
 void free_fmm(nbg_sys *s) {
-    void *ptrs[] = {s->fmm_items, s->fmm_pool, s->fmm_seed, s->fmm_misc, s->fmm_starts};
+    void *ptrs[] = {s->fmm_items, s->fmm_pool, s->fmm_seed, s->fmm_misc, s->fmm_starts, s->fmm_ulist};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    s->fmm_items = s->fmm_pool = s->fmm_seed = s->fmm_misc = nullptr;
+    s->fmm_items = s->fmm_pool = s->fmm_seed = s->fmm_misc = s->fmm_ulist = nullptr;
     s->fmm_starts = nullptr;
     s->fmm_item_cap = s->fmm_pool_cap = 0;
 }
@@ -194,7 +195,7 @@ int ensure_fmm(nbg_sys *s) {
     };
     A(&s->fmm_items, 3 * item_cap * kFmmItemWords); A(&s->fmm_pool, pool_cap);
     A(&s->fmm_seed, (int64_t)kFmmSeedWaves * 3 * seed_cap); A(&s->fmm_misc, 64);
-    A(&s->fmm_starts, kFmmStartCap);
+    A(&s->fmm_starts, kFmmStartCap); A(&s->fmm_ulist, (int64_t)kFmmGroupWaves * kFmmUlistCap);
     if (r != NBG_OK) {
         free_fmm(s);
         return r;
@@ -335,6 +336,8 @@ int enqueue_force_pass(nbg_sys *s) {
         fa.item_cap = s->fmm_item_cap;
         fa.pool_cap = s->fmm_pool_cap;
         fa.seed_scratch = s->fmm_seed;
+        fa.ulist = s->fmm_ulist;
+        fa.ulist_cap = kFmmUlistCap;
         fa.cap_a = s->fmm_seed_cap;
         fa.waves = kFmmWaves;
         fa.steps = p.max_depth + 4;  // a step per tree level below the shallowest starting node

@@ -332,8 +332,8 @@ struct GroupLds {
 template <bool SOFT>
 __device__ void fmm_leaf_group(const FmmArgs &a, int P, const uint32_t *alist, uint32_t alen, const uint32_t *elist,
                                uint32_t elen, const uint32_t *it, uint32_t lmask, GroupLds &g, int *s_grp,
-                               uint64_t *s_msk, uint64_t &c_pp, uint64_t &c_np, uint64_t &c_pn, uint64_t &c_nn,
-                               bool &fail) {
+                               uint64_t *s_msk, uint32_t *ulist, uint64_t &c_pp, uint64_t &c_np, uint64_t &c_pn,
+                               uint64_t &c_nn, bool &fail) {
     const int lane = lane_id();
     const OctRec *__restrict__ nodes = a.nodes;
     const float4 *__restrict__ cen = a.centre;
@@ -390,9 +390,14 @@ __device__ void fmm_leaf_group(const FmmArgs &a, int P, const uint32_t *alist, u
     // t < kk evaluates entry t / k of a batch against leaf t % k, so a lane
     // always serves one leaf (its partial field is that leaf's) and the k
     // lanes of an entry read the same record.
+    // The active nodes some leaf found close are collected (while they fit
+    // the wave's buffer) for the walk pass.
     const uint32_t vlen = 8u * alen;
+    int nu = 0;
+    bool ufull = false;
     {
         const int epb = kWave / k, kk = epb * k, lj = lane % k, li = lane / k;
+        const uint64_t kmask = (1ull << k) - 1;
         const float4 cl = g.cen[lj];
         LocalF Fp = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (uint32_t b = 0; b < vlen; b += (uint32_t)epb) {
@@ -401,9 +406,14 @@ __device__ void fmm_leaf_group(const FmmArgs &a, int P, const uint32_t *alist, u
             if (an >= a.max_nodes) { fail = true; an = -1; }
             const bool he = an >= 0;
             const OctRec ar = nodes[he ? an : 0];
-            const bool far = he && fmm_recommend(ar.com, cen[he ? an : 0], cl, theta2) == 0;
+            const int r = he ? fmm_recommend(ar.com, cen[he ? an : 0], cl, theta2) : 1;
+            const bool far = he && r == 0;
             if (far) fmm_to_local(ar.com.x, ar.com.y, ar.com.z, ar.com.w, cl, eps2, Fp);
             c_nn += __popcll(__ballot(far));
+            const uint64_t cb = __ballot(he && r != 0);
+            // the first lane of an entry appends it when any of its leaves found it close
+            const bool first = lane < kk && lj == 0 && ((cb >> lane) & kmask) != 0ull;
+            nu = wave_append(ulist, nu, a.ulist_cap, first, (uint32_t)an, ufull);
         }
         for (uint32_t b = 0; b < elen; b += (uint32_t)epb) {
             const uint32_t idx = b + (uint32_t)li;
@@ -439,8 +449,10 @@ __device__ void fmm_leaf_group(const FmmArgs &a, int P, const uint32_t *alist, u
     W.visits = 0; W.lanes = 0; W.leaf_visits = 0; W.leaf_lanes = 0;
     W.G = a.G;
     const uint32_t mybit = has ? 1u << myleaf : 0u;
-    for (uint32_t b = 0; b < vlen; b += kWave) {
-        int an = b + lane < vlen ? active_entry(alist, b + lane, true) : -1;
+    // the collected entries (all entries again when the buffer overflowed)
+    const uint32_t wlen = ufull ? vlen : (uint32_t)nu;
+    for (uint32_t b = 0; b < wlen; b += kWave) {
+        int an = b + lane < wlen ? (ufull ? active_entry(alist, b + lane, true) : (int)ulist[b + lane]) : -1;
         if (an >= a.max_nodes) an = -1;
         const bool he = an >= 0;
         const OctRec ar = nodes[he ? an : 0];
@@ -854,6 +866,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_
     const int lane = lane_id();
     const int64_t ng = a.group_ctr[0] < (uint64_t)a.item_cap ? (int64_t)a.group_ctr[0] : a.item_cap;
     if ((int64_t)blockIdx.x >= ng) return;
+    uint32_t *ulist = a.ulist + (int64_t)blockIdx.x * a.ulist_cap;
     uint64_t c_pp = 0, c_np = 0, c_pn = 0, c_nn = 0;
     bool fail = false;
     for (;;) {
@@ -870,7 +883,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_
             break;
         }
         fmm_leaf_group<SOFT>(a, P, a.pool + aoff, alen, a.pool + eoff, elen, it, (uint32_t)uniform_i32((int)it[kItLeaves]),
-                             s_g, s_grp, s_msk, c_pp, c_np, c_pn, c_nn, fail);
+                             s_g, s_grp, s_msk, ulist, c_pp, c_np, c_pn, c_nn, fail);
         fail = __ballot(fail) != 0;
         if (fail) break;
     }

@@ -251,6 +251,7 @@ void launch_walk_oct(const WalkArgs &a, int ordered, int diag, hipStream_t s);
 // or the leaf groups outgrew their capacity.
 constexpr int kFmmItemWords = 16;
 constexpr int kFmmGroupWaves = 8192;  // the leaf-group launch (8 waves per SIMD)
+constexpr int kFmmUlistCap = 4096;    // close active nodes kept per leaf-group wave
 constexpr int kFmmStartCap = 512;
 constexpr int kFmmSeedWaves = 272;  // one per starting node (at most 256 + 7)
 struct FmmArgs {
@@ -274,6 +275,8 @@ struct FmmArgs {
     uint32_t *pool;           // pool_cap words
     int64_t item_cap, pool_cap;
     uint32_t *seed_scratch;   // kFmmSeedWaves x 3 x cap_a words
+    uint32_t *ulist;          // kFmmGroupWaves x ulist_cap node slots
+    int ulist_cap;
     int cap_a, waves, steps;
     unsigned long long *counts;
     uint32_t *err;
