@@ -26,6 +26,7 @@ bool is_octree(int solver) {
 // Implicit FMM: waves of a traversal step, the seed queues' capacity, and
 // the list pool of a pass (words per body plus a floor).
 constexpr int kFmmWaves = 5120, kFmmSeedCap = 16384;
+static_assert(kFmmWaves <= kFmmGroupWaves, "the step waves share the leaf-group waves' close-node lists");
 constexpr int64_t kFmmPoolPerBody = 32, kFmmPoolFloor = (int64_t)1 << 17, kFmmPoolMax = (int64_t)1 << 31;
 bool is_quad(int solver) { return solver == NBG_BARNES_HUT_QUADRUPOLE || solver == NBG_LINEAR_BVH_QUADRUPOLE; }
 constexpr int kEventSets = 256;

@@ -607,6 +607,7 @@ __global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
     uint32_t *itout = a.items[(k + 1) & 1];
     const uint32_t *pin = a.pool;
     uint32_t *pout = a.pool;
+    uint32_t *ulist = a.ulist + (int64_t)blockIdx.x * a.ulist_cap;
     // a step that overflowed the frontier counted past the capacity (and flagged it)
     const int nin = (int)(ci[0] < (uint64_t)a.item_cap ? ci[0] : (uint32_t)a.item_cap);
     if ((int)blockIdx.x >= nin) return;  // (steps past the deepest leaf launch with nin = 0)
@@ -642,8 +643,10 @@ __global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
         const uint32_t *alist = pin + aoff;
         const uint32_t vlen = groups ? 8u * alen : alen;
         LocalF Fl = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        // pass 1 over the active list: far nodes into the local field; sizes of what stays
-        int nleafc = 0, ninner = 0;
+        // pass 1 over the active list: far nodes into the local field; sizes of
+        // what stays, whose slots are collected (while they fit) for pass 2
+        int nleafc = 0, ninner = 0, nu = 0;
+        bool ufull = false;
         uint32_t lbod = 0;
         for (uint32_t b = 0; b < vlen; b += kWave) {
             int an = b + lane < vlen ? active_entry(alist, b + lane, groups) : -1;
@@ -658,6 +661,7 @@ __global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
             nleafc += __popcll(__ballot(lf));
             ninner += __popcll(__ballot(has && !far && ar.topo.z >= 0));
             if (lf) lbod += (uint32_t)ar.topo.y;
+            nu = wave_append(ulist, nu, a.ulist_cap, has && !far, (uint32_t)an, ufull);
         }
         // pool layout: a leaf P keeps its close active nodes (walk starts);
         // an internal P its close internal ones as child groups, then its
@@ -670,15 +674,16 @@ __global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
             break;
         }
         uint32_t *C = pout + off, *LA = C + ninner, *E2 = C + nclose;
-        // pass 2: write what stays
+        // pass 2: write what stays (the collected slots are all close)
+        wave_mem_sync();
         int nc = 0, ng = 0, nl = 0;
-        for (uint32_t b = 0; b < vlen; b += kWave) {
-            int an = b + lane < vlen ? active_entry(alist, b + lane, groups) : -1;
+        const uint32_t wlen = ufull ? vlen : (uint32_t)nu;
+        for (uint32_t b = 0; b < wlen; b += kWave) {
+            int an = b + lane < wlen ? (ufull ? active_entry(alist, b + lane, groups) : (int)ulist[b + lane]) : -1;
             if (an >= a.max_nodes) an = -1;
             const bool has = an >= 0;
             const OctRec ar = nodes[has ? an : 0];
-            const int r = has ? fmm_recommend(ar.com, cen[an], cP, theta2) : 1;
-            const bool close = has && r != 0;
+            const bool close = has && (!ufull || fmm_recommend(ar.com, cen[an], cP, theta2) != 0);
             if (leaf) {
                 nc = wave_append(C, nc, nclose, close, (uint32_t)an, over);
             } else {
