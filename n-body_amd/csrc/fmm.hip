@@ -90,34 +90,57 @@ static_assert(kHeapCap <= kFmmStartCap && kHeapCap <= kFmmSeedWaves, "starting-n
 
 __global__ __launch_bounds__(64) void k_fmm_starts(const OctRec *__restrict__ nodes, const uint32_t *__restrict__ misc,
                                                    int32_t *starts, uint32_t *nstarts, uint32_t *err) {
+    // lane 0 keeps the heap in LDS; the popped node's children are read by
+    // one lane each, so a round costs two dependent loads, not nine
     __shared__ HeapEnt q[kHeapCap];
-    if (threadIdx.x != 0) return;
+    const int lane = lane_id();
     const int64_t T = misc[0];
     int len = 1;
-    q[0] = HeapEnt{0, nodes[0].topo.y};
+    bool bad = false;
+    if (lane == 0) q[0] = HeapEnt{0, nodes[0].topo.y};
+    __builtin_amdgcn_wave_barrier();
     while (len < kFmmStarts) {
-        heap_pop(q, len);
-        const HeapEnt t = q[len - 1];
-        if (t.slot < 0 || nodes[t.slot].topo.z < 0) break;
+        if (lane == 0) heap_pop(q, len);
+        __builtin_amdgcn_wave_barrier();
+        const int slot = uniform_i32(q[len - 1].slot);
+        if (slot < 0) break;
+        const OctRec r = nodes[slot];
+        if (r.topo.z < 0) break;
         --len;
-        const OctRec &r = nodes[t.slot];
-        const int c0 = r.topo.z, nch = oct_nchild(r);
-        if (c0 <= t.slot || (int64_t)c0 + nch > T) { atomicOr(err, 8u); break; }
+        const int c0 = uniform_i32(r.topo.z), nch = uniform_i32(oct_nchild(r));
+        if (c0 <= slot || nch < 1 || nch > 8 || (int64_t)c0 + nch > T) {
+            bad = true;
+            break;
+        }
+        int oc = 8, cnt = 0;
+        if (lane < nch) {
+            const OctRec cr = nodes[c0 + lane];
+            oc = oct_octant(cr);
+            cnt = cr.topo.y;
+        }
         int j = 0;
         for (int oct = 0; oct < 8; ++oct) {
+            const int ocj = __shfl(oc, j & 7, kWave), cj = __shfl(cnt, j & 7, kWave);
             HeapEnt e{-1, 0};
-            if (j < nch && oct_octant(nodes[c0 + j]) == oct) {
-                e = HeapEnt{c0 + j, nodes[c0 + j].topo.y};
+            if (j < nch && ocj == oct) {
+                e = HeapEnt{c0 + j, cj};
                 ++j;
             }
-            q[len++] = e;
-            heap_push(q, len);
+            if (lane == 0) {
+                q[len] = e;
+                heap_push(q, len + 1);
+            }
+            ++len;
+            __builtin_amdgcn_wave_barrier();
         }
     }
-    int k = 0;
-    for (int i = 0; i < len; ++i)
-        if (q[i].slot >= 0) starts[k++] = q[i].slot;
-    *nstarts = (uint32_t)k;
+    if (lane == 0) {
+        if (bad) atomicOr(err, 8u);
+        int k = 0;
+        for (int i = 0; i < len; ++i)
+            if (q[i].slot >= 0) starts[k++] = q[i].slot;
+        *nstarts = (uint32_t)k;
+    }
 }
 
 // ---------------------------------------------------------------- criteria
