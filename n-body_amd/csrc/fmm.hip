@@ -348,7 +348,7 @@ __device__ __forceinline__ int active_entry(const uint32_t *list, uint32_t idx, 
 // active node once for all the lanes whose leaf found it close.
 struct GroupLds {
     float4 cen[8];   // leaf centre, side
-    int first[8], count[8], pre[9];
+    int first[8], count[8];
     float F[8][9];   // leaf local fields
 };
 
