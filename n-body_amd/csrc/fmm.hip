@@ -33,6 +33,9 @@
 // summed once per node.  Empty octants, which the reference keeps as
 // body-less nodes, contribute nothing and have no record here.
 
+#include <cmath>
+#include <limits>
+
 #include "device.hpp"
 #include "launch.hpp"
 #include "walk_common.hpp"
@@ -146,12 +149,27 @@ __global__ __launch_bounds__(64) void k_fmm_starts(const OctRec *__restrict__ no
 // ---------------------------------------------------------------- criteria
 // SideLengthOverDistance, node-node (Descent/DescentCriterion/SideLengthOverDistance.h:33-56):
 // 0 = Approximate, else DescendActiveNode (1) | DescendPassiveNode (2).
-__device__ __forceinline__ int fmm_recommend(float4 acom, float4 ac, float4 pc, float theta2) {
+// fl(x / y) < t for FP32 x, y >= 0 and an FP32 t >= 0, without the
+// division: with p the float below t and m = (p + t) / 2, fl(x / y) <= p
+// exactly when x / y < m.  A quotient of floats never equals m: m's mantissa
+// is an odd 25-bit integer, so x = m y would need a 25-bit mantissa (x = 0
+// aside, where 0 / y is below t for y > 0 and NaN for y = 0, as 0 < m y
+// says).  m y has at most 49 significant bits, so the FP64 product and
+// comparison are exact; infinities compare as the quotient would.
+struct QuotientBound {
+    double mid;
+};
+
+__device__ __forceinline__ bool quotient_below(float x, float y, QuotientBound t) {
+    return (double)x < t.mid * (double)y;
+}
+
+__device__ __forceinline__ int fmm_recommend(float4 acom, float4 ac, float4 pc, QuotientBound theta2) {
     const float sa2 = ac.w * ac.w, sp2 = pc.w * pc.w;
     const float dx = pc.x - acom.x, dy = pc.y - acom.y, dz = pc.z - acom.z;
     const float d2 = dx * dx + dy * dy + dz * dz;
     const float big = sa2 < sp2 ? sp2 : sa2;
-    if (ieee_div(4.0f * big, d2) < theta2) {
+    if (quotient_below(4.0f * big, d2, theta2)) {
         // exclusion regions [c - side, c + side] overlap (DescentCriterionType.h:81-85, BoundingBox.h:77-82)
         const bool meet = fminf(pc.x + pc.w, ac.x + ac.w) >= fmaxf(pc.x - pc.w, ac.x - ac.w) &&
                           fminf(pc.y + pc.w, ac.y + ac.w) >= fmaxf(pc.y - pc.w, ac.y - ac.w) &&
@@ -163,10 +181,10 @@ __device__ __forceinline__ int fmm_recommend(float4 acom, float4 ac, float4 pc, 
 
 // SideLengthOverDistance, particle-node (SideLengthOverDistance.h:25-29):
 // 2 side / |x - c| < theta and x outside the cell [c - side/2, c + side/2).
-__device__ __forceinline__ bool fmm_particle_far(float x, float y, float z, float4 pc, float theta) {
+__device__ __forceinline__ bool fmm_particle_far(float x, float y, float z, float4 pc, QuotientBound theta) {
     const float dx = x - pc.x, dy = y - pc.y, dz = z - pc.z;
     const float dist = ieee_sqrt(dx * dx + dy * dy + dz * dz);
-    if (!(ieee_div(2.0f * pc.w, dist) < theta)) return false;
+    if (!quotient_below(2.0f * pc.w, dist, theta)) return false;
     const float h = pc.w / 2.0f;
     const bool in = pc.x - h <= x && pc.y - h <= y && pc.z - h <= z && x < pc.x + h && y < pc.y + h && z < pc.z + h;
     return !in;
@@ -360,7 +378,8 @@ __device__ void fmm_leaf_group(const FmmArgs &a, int P, const uint32_t *alist, u
     const int lane = lane_id();
     const OctRec *__restrict__ nodes = a.nodes;
     const float4 *__restrict__ cen = a.centre;
-    const float theta = a.theta, theta2 = a.theta * a.theta, eps2 = a.eps2;
+    const QuotientBound theta = {a.theta_mid}, theta2 = {a.theta2_mid};
+    const float eps2 = a.eps2;
     const OctRec pr = nodes[P];
     const float4 cP = cen[P];
     const int c0 = pr.topo.z, nch = oct_nchild(pr);
@@ -561,7 +580,7 @@ __global__ __launch_bounds__(64) void k_fmm_seed(FmmArgs a) {
     const int lane = lane_id();
     const OctRec *__restrict__ nodes = a.nodes;
     const float4 *__restrict__ cen = a.centre;
-    const float theta2 = a.theta * a.theta;
+    const QuotientBound theta2 = {a.theta2_mid};
     uint32_t *base = a.seed_scratch + (int64_t)w * 3 * a.cap_a;
     uint32_t *Q[2] = {base, base + a.cap_a};
     uint32_t *EQ = base + 2 * a.cap_a;
@@ -623,7 +642,8 @@ __global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
     const int lane = lane_id();
     const OctRec *__restrict__ nodes = a.nodes;
     const float4 *__restrict__ cen = a.centre;
-    const float theta = a.theta, theta2 = a.theta * a.theta, eps2 = a.eps2;
+    const QuotientBound theta = {a.theta_mid}, theta2 = {a.theta2_mid};
+    const float eps2 = a.eps2;
     uint32_t *ci = a.ctr + 4 * (k % 3), *co = a.ctr + 4 * ((k + 1) % 3), *cr = a.ctr + 4 * ((k + 2) % 3);
     if (blockIdx.x == 0 && lane < 4) cr[lane] = 0u;
     const uint32_t *__restrict__ itin = a.items[k & 1];
@@ -935,8 +955,20 @@ __global__ void k_fmm_tail(const uint32_t *ctr_next, uint32_t *peak, int steps, 
 
 }  // namespace
 
-void launch_fmm(const FmmArgs &a, hipStream_t s) {
-    if (a.n <= 0) return;
+namespace {
+// The midpoint between t and the float below it (exact in FP64).
+double quotient_bound(float t) {
+    const float p = std::nextafter(t, -std::numeric_limits<float>::infinity());
+    return ((double)p + (double)t) / 2.0;
+}
+}  // namespace
+
+void launch_fmm(const FmmArgs &args, hipStream_t s) {
+    if (args.n <= 0) return;
+    FmmArgs a = args;
+    const float theta2 = args.theta * args.theta;
+    a.theta2_mid = quotient_bound(theta2);
+    a.theta_mid = quotient_bound(args.theta);
     launch_fill_u32(a.ctr, 0u, 12, s);
     launch_fill_u32(a.pool_ctr, 0u, 1, s);
     launch_fill_u32(a.group_ctr, 0u, 2, s);

@@ -264,6 +264,9 @@ struct FmmArgs {
     const uint32_t *perm;     // tree position -> original index
     float *ax, *ay, *az;
     float theta, G, eps2;
+    // the criteria's bounds on FP32 quotients (fmm.hip's quotient_below): the
+    // midpoints below theta * theta and below theta (set by launch_fmm)
+    double theta2_mid, theta_mid;
     int32_t *starts;          // kFmmStartCap
     uint32_t *nstarts;
     uint32_t *ctr;            // 12 words

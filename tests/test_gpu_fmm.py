@@ -228,13 +228,14 @@ def test_fmm_degenerate_sets(nbg, orc, gpu_ctx):
     s = nbg.System(gpu_ctx, _params(nbg, 16), 0)
     s.set_state(nbg.Bodies.empty(0))
     assert s.update_accelerations().shape == (0, 3)
-    # every body at one point
+    # every body at one point: zero-size cells, 0/0 in both criteria
     b = nbg.Bodies.empty(300)
     b.m[:] = 1.0
-    s, acc = _gpu(nbg, gpu_ctx, b, 16, max_depth=4)
-    ref, c = _oracle(orc, b, 16, max_depth=4)
-    assert s.counts4() == c
-    assert np.array_equal(acc, np.zeros_like(acc)) and np.array_equal(ref, np.zeros_like(ref))
+    for theta in (0.4, 0.5, 0.7, 0.3):
+        s, acc = _gpu(nbg, gpu_ctx, b, 16, max_depth=4, theta=theta)
+        ref, c = _oracle(orc, b, 16, max_depth=4, theta=theta)
+        assert s.counts4() == c, theta
+        assert np.array_equal(acc, np.zeros_like(acc)) and np.array_equal(ref, np.zeros_like(ref))
 
 
 def test_fmm_facade(nbg, orc, gpu_ctx):
