@@ -256,7 +256,9 @@ def bench_forest(nbg, ctx, args):
     systems = ensemble(nbg, args.forest_systems, 256, 65536, args.forest_seed)
     total = sum(b.n for b, _ in systems)
     p = nbg.default_params()
-    p.solver, p.morton_bits, p.theta, p.leaf_size = nbg.LINEAR_BVH, 30, 0.5, 16
+    quad = args.forest_solver == "linear-bvh-quadrupole"
+    p.solver = nbg.LINEAR_BVH_QUADRUPOLE if quad else nbg.LINEAR_BVH
+    p.morton_bits, p.theta, p.leaf_size = 30, 0.5, 16
     p.eps2, p.dt, p.sum_mode = 1e-4, 1e-3, nbg.SUM_FAST
     B = nbg.Batch(ctx, total, len(systems), sum_mode=nbg.SUM_FAST)
     for b, _ in systems:
@@ -297,7 +299,8 @@ def bench_forest(nbg, ctx, args):
     alone = bodies_alone * args.forest_steps / t_alone
     return {
         "workload": f"{len(systems)} Plummer systems, N log-uniform in [256, 65536] (total {total}), a ~ U[0.5, 2], "
-                    f"M = 1, eps2 = 1e-4, theta = 0.5, dt = 1e-3, {args.forest_steps} steps, linear BVH 30-bit keys, "
+                    f"M = 1, eps2 = 1e-4, theta = 0.5, dt = 1e-3, {args.forest_steps} steps, "
+                    f"{'quadrupole ' if quad else ''}linear BVH 30-bit keys, "
                     f"FAST sums, seed {args.forest_seed}",
         "value": value,
         "unit": "bodies·steps/s",
@@ -308,8 +311,9 @@ def bench_forest(nbg, ctx, args):
         "autotune_ms_per_pass": tune_ms,
         "interactions_per_step": pp + npc,
         "interactions_per_s": (pp + npc) * args.forest_steps / el,
-        "roofline_frac_of_fp32_peak": FLOPS_PER_INTERACTION * (pp + npc) * args.forest_steps / el / 1e12
-                                      / FP32_PEAK_TFLOPS,
+        # the per-interaction flop count is the monopole kernel's
+        "roofline_frac_of_fp32_peak": None if quad else FLOPS_PER_INTERACTION * (pp + npc) * args.forest_steps
+                                      / el / 1e12 / FP32_PEAK_TFLOPS,
         "one_at_a_time": {"value": alone, "unit": "bodies·steps/s",
                           "sample": f"{len(sample)} of the systems stepped alone through nbg_sys "
                                     f"(same parameters and leaf size), bodies-weighted"},
@@ -469,6 +473,8 @@ def main():
     ap.add_argument("--forest-systems", type=int, default=4096)
     ap.add_argument("--forest-steps", type=int, default=20)
     ap.add_argument("--forest-seed", type=int, default=7)
+    ap.add_argument("--forest-solver", choices=("linear-bvh", "linear-bvh-quadrupole"), default="linear-bvh",
+                    help="solver of every system in the batched ensemble")
     ap.add_argument("--forest-alone-sample", type=int, default=64)
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
     ap.add_argument("--serve-jobs", type=int, default=60)

@@ -229,11 +229,12 @@ int nbg_sys_potential_f64(nbg_sys *sys, double *W);
 /* ---- batched forest: many independent systems stepped together ----
  *
  * No reference counterpart (the reference steps one Simulation).  Each system
- * is a linear-BVH Solver (Solvers/LinearBVHSolver.h:18-28) with its own
- * theta, leaf size, G, eps2 and dt; every pass builds all systems' trees in
- * one segmented build and walks them in one launch.  A system's results
- * are the same bits it gets alone in an nbg_sys with the same parameters
- * (30-bit Morton keys, the batch's sum mode).  Systems can join and leave
+ * is a linear-BVH Solver (Solvers/LinearBVHSolver.h:18-28), or the quadrupole
+ * form of it (NBG_LINEAR_BVH_QUADRUPOLE; all members of a batch run the same
+ * one), with its own theta, leaf size, G, eps2 and dt; every pass builds all
+ * systems' trees in one segmented build and walks them in one launch.  A
+ * system's results are the same bits it gets alone in an nbg_sys with the
+ * same parameters (30-bit Morton keys, the batch's sum mode).  Systems can join and leave
  * between steps (continuous batching); state stays in HBM. */
 typedef struct nbg_batch nbg_batch;
 
@@ -245,8 +246,10 @@ int nbg_batch_create(nbg_ctx *ctx, int64_t max_bodies, int32_t max_systems, int3
 int nbg_batch_destroy(nbg_batch *b);
 
 /* Adds a system of n >= 1 bodies (host SoA copied in; vx/vy/vz may be NULL).
- * p->solver must be NBG_LINEAR_BVH and p->morton_bits 30.  *id receives the
- * system's handle.  NBG_ERR_BUDGET when the batch is full. */
+ * p->solver must be NBG_LINEAR_BVH or NBG_LINEAR_BVH_QUADRUPOLE, the same as
+ * the other members', and p->morton_bits 30.  *id receives the system's
+ * handle.  NBG_ERR_BUDGET when the batch is full (the first quadrupole member
+ * also allocates 64 bytes per body of capacity for the node moments). */
 int nbg_batch_add(nbg_batch *b, const nbg_params *p, int64_t n, const float *x, const float *y, const float *z,
                   const float *m, const float *vx, const float *vy, const float *vz, int32_t *id);
 

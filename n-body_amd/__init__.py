@@ -404,8 +404,9 @@ class System:
 # ------------------------------------------------------------------ batched forest
 
 class Batch:
-    """nbg_batch: independent linear-BVH systems stepped together in one
-    segmented build and one walk launch per step."""
+    """nbg_batch: independent linear-BVH systems (all monopole or all
+    quadrupole) stepped together in one segmented build and one walk launch
+    per step."""
 
     def __init__(self, ctx: Context, max_bodies: int, max_systems: int, sum_mode: int = SUM_FAST):
         h = _vp()

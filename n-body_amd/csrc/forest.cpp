@@ -3,6 +3,8 @@
 // order; one pass builds every system's linear BVH (segmented boxes, one
 // sort of system-prefixed keys, one radix-tree build, one summary pass) and
 // walks every system in one launch, with wavefronts aligned to systems.
+// The members share one solver: the monopole linear BVH or its quadrupole
+// form (second moments summarized with the tree, quadrupole walk).
 
 #include <hip/hip_runtime.h>
 
@@ -47,6 +49,9 @@ struct nbg_batch {
     bool counting = true;
     bool use_graph = false;
     bool all_soft = true;
+    // every member runs the same solver: NBG_LINEAR_BVH or
+    // NBG_LINEAR_BVH_QUADRUPOLE (set by the first member of an empty batch)
+    int32_t solver = NBG_LINEAR_BVH;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool graph_valid = false;
@@ -71,6 +76,9 @@ struct nbg_batch {
     MacParam *mp_tab = nullptr;
     float *bbox6 = nullptr;
     uint32_t *misc = nullptr;  // [1] err, [2] node count, [4..16) counters
+    // second moments of every node (2 float4 per node), allocated when the
+    // first quadrupole member joins
+    float4 *quad = nullptr;
 
     float *x() { return st[0]; }
     float *y() { return st[1]; }
@@ -109,7 +117,8 @@ void free_batch(nbg_batch *b) {
     std::vector<void *> ptrs = {b->bsys, b->keys, b->keys_alt, b->idx, b->idx_alt, b->bodies, b->kfirst, b->klast,
                                 b->kgamma, b->kparent, b->ksplit, b->krank, b->leaf_start, b->glist, b->scan_ws,
                                 b->sort_status, b->nodes, b->bmin, b->bmax, b->flags, b->sys_first, b->wave_first,
-                                b->old_first, b->sys_root, b->leaf_tab, b->sys_prm, b->mp_tab, b->bbox6, b->misc};
+                                b->old_first, b->sys_root, b->leaf_tab, b->sys_prm, b->mp_tab, b->bbox6, b->misc,
+                                b->quad};
     for (float *p : b->st) ptrs.push_back(p);
     for (float *p : b->stage) ptrs.push_back(p);
     for (void *p : ptrs)
@@ -119,11 +128,29 @@ void free_batch(nbg_batch *b) {
 
 int check_member_params(const nbg_params *p) {
     if (!p) return fail(NBG_ERR_ARG, "params is NULL");
-    if (p->solver != NBG_LINEAR_BVH) return fail(NBG_ERR_ARG, "the batched forest runs the linear-BVH solver");
+    if (p->solver != NBG_LINEAR_BVH && p->solver != NBG_LINEAR_BVH_QUADRUPOLE)
+        return fail(NBG_ERR_ARG, "the batched forest runs the linear-BVH solvers (monopole or quadrupole)");
     if (p->morton_bits != 30) return fail(NBG_ERR_ARG, "the batched forest uses 30-bit Morton keys");
     if (!(p->theta >= 0.0f)) return fail(NBG_ERR_ARG, "theta must be >= 0");
     if (p->leaf_size < 1) return fail(NBG_ERR_ARG, "leaf_size must be >= 1");
     if (!(p->eps2 >= 0.0f)) return fail(NBG_ERR_ARG, "eps2 must be >= 0");
+    return NBG_OK;
+}
+
+// Members share one solver; `self` (or null) is the member being changed.
+// A quadrupole batch allocates its node moments here, outside any pass.
+int check_member_solver(nbg_batch *b, const nbg_params *p, const Member *self) {
+    for (const Member &m : b->sys)
+        if (&m != self && m.p.solver != p->solver)
+            return fail(NBG_ERR_ARG, "the systems of a batch share one solver (monopole or quadrupole linear BVH)");
+    if (p->solver == NBG_LINEAR_BVH_QUADRUPOLE && !b->quad) {
+        NBG_HIP(hipStreamSynchronize(b->stream));
+        if (dalloc(&b->quad, 2 * 2 * b->cap_n) != NBG_OK) {
+            b->quad = nullptr;
+            (void)hipGetLastError();  // the failed allocation must not fail the next launch check
+            return fail(NBG_ERR_BUDGET, "no device memory for the quadrupole moments");
+        }
+    }
     return NBG_OK;
 }
 
@@ -137,6 +164,7 @@ int upload_tables(nbg_batch *b) {
     std::vector<MacParam> mp(S);
     int64_t off = 0, w = 0;
     b->all_soft = true;
+    if (S) b->solver = b->sys[0].p.solver;
     for (size_t i = 0; i < S; ++i) {
         Member &m = b->sys[i];
         m.first = off;
@@ -181,8 +209,9 @@ void enqueue_pass(nbg_batch *b) {
     launch_lbvh_topology(b->keys, 63, n, 0, b->kfirst, b->klast, b->kgamma, b->ksplit, b->krank, b->kparent,
                          b->scan_ws, b->nodes, b->leaf_start, b->node_count(), s, 30, b->leaf_tab);
     launch_gather(b->x(), b->y(), b->z(), b->m(), b->idx, n, b->bodies, s);
+    float4 *quad = b->solver == NBG_LINEAR_BVH_QUADRUPOLE ? b->quad : nullptr;
     launch_lbvh_summaries(b->bodies, n, b->nodes, b->leaf_start, 2 * n, b->flags, b->glist, b->bmin, b->bmax, 0.5f,
-                          b->err_word(), s, b->mp_tab, b->keys, 30);
+                          b->err_word(), s, b->mp_tab, b->keys, 30, quad);
     launch_forest_roots(b->nodes, b->node_count(), 2 * n, b->keys, n, S, b->sys_root, b->err_word(), s);
     WalkArgs wa;
     wa.n = n;
@@ -213,6 +242,7 @@ void enqueue_pass(nbg_batch *b) {
     wa.sys_first = b->sys_first;
     wa.sys_root = b->sys_root;
     wa.sys_prm = b->sys_prm;
+    wa.quad = quad;
     const bool ordered = b->sum_mode == NBG_SUM_ORDERED;
     launch_walk_lbvh(wa, ordered, 32, (b->counting || ordered) ? 1 : 0, s);
 }
@@ -323,6 +353,7 @@ int nbg_batch_add(nbg_batch *b, const nbg_params *p, int64_t n, const float *x, 
     if ((int64_t)b->sys.size() >= b->cap_s) return fail(NBG_ERR_BUDGET, "batch system capacity reached");
     if (used + n > b->cap_n) return fail(NBG_ERR_BUDGET, "batch body capacity reached");
     DeviceGuard g(b->ctx->device);
+    if (int r = check_member_solver(b, p, nullptr)) return r;
     hipStream_t s = b->stream;
     NBG_HIP(hipStreamSynchronize(s));
     const size_t bytes = (size_t)n * 4;
@@ -392,6 +423,8 @@ int nbg_batch_set_params(nbg_batch *b, int32_t id, const nbg_params *p) {
     if (int r = check_member_params(p)) return r;
     Member *mm = find(b, id);
     if (!mm) return fail(NBG_ERR_ARG, "unknown system id");
+    DeviceGuard g(b->ctx->device);
+    if (int r = check_member_solver(b, p, mm)) return r;
     mm->p = *p;
     b->tables_dirty = true;
     return NBG_OK;

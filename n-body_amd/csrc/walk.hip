@@ -215,7 +215,7 @@ __device__ __forceinline__ uint64_t visit(const NodeRec &nd, const float4 *__res
 }
 
 // Forest: the same traversal for the wave's system (see wave_task).
-template <int WAVES, bool SOFT, int STATS>
+template <int WAVES, bool SOFT, int STATS, bool QUAD>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_forest(WalkArgs a) {
     __shared__ int s_node[WAVES][kStackFast];
     __shared__ uint64_t s_mask[WAVES][kStackFast];
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_forest(WalkArgs a) {
         return;
     }
     const NodeRec root = nodes[T.root];
-    uint64_t mask = visit<SOFT, STATS, false>(root, nullptr, vmask, bodies, n, eps2, W, fail);
+    uint64_t mask = visit<SOFT, STATS, QUAD>(root, a.quad + 2 * T.root, vmask, bodies, n, eps2, W, fail);
     int c = root.topo.z;
     int sp = 0;
     uint32_t iters = 0;
@@ -271,8 +271,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_forest(WalkArgs a) {
         if (__builtin_expect((uint32_t)c >= last_pair, 0)) { fail = true; break; }
         if (__builtin_expect(++iters > kMaxWalkIters, 0)) { fail = true; break; }
         const NodeRec r0 = nodes[c], r1 = nodes[c + 1];
-        const uint64_t o0 = visit<SOFT, STATS, false>(r0, nullptr, mask, bodies, n, eps2, W, fail);
-        const uint64_t o1 = visit<SOFT, STATS, false>(r1, nullptr, mask, bodies, n, eps2, W, fail);
+        const uint64_t o0 = visit<SOFT, STATS, QUAD>(r0, a.quad + 2 * c, mask, bodies, n, eps2, W, fail);
+        const uint64_t o1 = visit<SOFT, STATS, QUAD>(r1, a.quad + 2 * c + 2, mask, bodies, n, eps2, W, fail);
         // a failed visit opens nothing, so the walk still ends; fail is
         // reported after the loop
         if (o0 != 0) {
@@ -674,9 +674,12 @@ static void launch_fast(const WalkArgs &a, hipStream_t s) {
     constexpr int W = 4;
     const unsigned blocks = (unsigned)((walk_waves(a) + W - 1) / W);
     const bool soft = a.eps2 > 0.0f;
-    if (a.nsys > 0) {
-        if (soft) hipLaunchKernelGGL((k_walk_lbvh_forest<W, true, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
-        else hipLaunchKernelGGL((k_walk_lbvh_forest<W, false, STATS>), dim3(blocks), dim3(64 * W), 0, s, a);
+    if (a.nsys > 0 && a.quad) {
+        if (soft) hipLaunchKernelGGL((k_walk_lbvh_forest<W, true, STATS, true>), dim3(blocks), dim3(64 * W), 0, s, a);
+        else hipLaunchKernelGGL((k_walk_lbvh_forest<W, false, STATS, true>), dim3(blocks), dim3(64 * W), 0, s, a);
+    } else if (a.nsys > 0) {
+        if (soft) hipLaunchKernelGGL((k_walk_lbvh_forest<W, true, STATS, false>), dim3(blocks), dim3(64 * W), 0, s, a);
+        else hipLaunchKernelGGL((k_walk_lbvh_forest<W, false, STATS, false>), dim3(blocks), dim3(64 * W), 0, s, a);
     } else if (a.quad) {
         launch_fast_single<STATS, true>(a, blocks, s);
     } else {
@@ -692,7 +695,9 @@ void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, int stats, 
         else launch_fast<0>(a, s);
     } else {
         const unsigned blocks = (unsigned)walk_waves(a);
-        if (a.nsys > 0)
+        if (a.nsys > 0 && a.quad)
+            hipLaunchKernelGGL((k_walk_lbvh_ordered<32, true, true>), dim3(blocks), dim3(64), 0, s, a);
+        else if (a.nsys > 0)
             hipLaunchKernelGGL((k_walk_lbvh_ordered<32, true, false>), dim3(blocks), dim3(64), 0, s, a);
         else if (a.quad && max_depth <= 32)
             hipLaunchKernelGGL((k_walk_lbvh_ordered<32, false, true>), dim3(blocks), dim3(64), 0, s, a);

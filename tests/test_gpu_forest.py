@@ -3,7 +3,9 @@
 Every system's result must be the bits it gets alone (an nbg_sys with the
 same parameters, 30-bit keys, same sum mode) — in both sum modes, because
 each wavefront serves one system and each system's subtree, summaries and
-traversal order are its own — and, in ORDERED mode, the oracle's bits."""
+traversal order are its own — and, in ORDERED mode, the oracle's bits.
+The same holds for a batch of quadrupole linear-BVH systems (solver 4),
+whose node second moments are summarized with the forest."""
 
 import numpy as np
 import pytest
@@ -11,9 +13,9 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _params(nbg, theta=0.5, leaf=16, eps2=1e-4, G=1.0, dt=1e-3, mode=1):
+def _params(nbg, theta=0.5, leaf=16, eps2=1e-4, G=1.0, dt=1e-3, mode=1, solver=1):
     p = nbg.default_params()
-    p.solver, p.morton_bits = nbg.LINEAR_BVH, 30
+    p.solver, p.morton_bits = solver, 30
     p.theta, p.leaf_size, p.eps2, p.G, p.dt, p.sum_mode = theta, leaf, eps2, G, dt, mode
     return p
 
@@ -29,8 +31,8 @@ def _systems(nbg):
     return out
 
 
-def _alone(nbg, ctx, b, kw, mode, steps=0):
-    s = nbg.System(ctx, _params(nbg, mode=mode, **kw), b.n)
+def _alone(nbg, ctx, b, kw, mode, steps=0, solver=1):
+    s = nbg.System(ctx, _params(nbg, mode=mode, solver=solver, **kw), b.n)
     s.set_state(b)
     if steps:
         s.step(steps)
@@ -45,35 +47,36 @@ def _alone(nbg, ctx, b, kw, mode, steps=0):
     return a, st, c
 
 
+@pytest.mark.parametrize("solver", [1, 4])
 @pytest.mark.parametrize("mode", [0, 1])
-def test_forest_pass_equals_alone_and_oracle(nbg, orc, gpu_ctx, mode):
+def test_forest_pass_equals_alone_and_oracle(nbg, orc, gpu_ctx, mode, solver):
     systems = _systems(nbg)
     B = nbg.Batch(gpu_ctx, 40000, 16, sum_mode=mode)
-    ids = [B.add(_params(nbg, mode=mode, **kw), b) for b, kw in systems]
+    ids = [B.add(_params(nbg, mode=mode, solver=solver, **kw), b) for b, kw in systems]
     B.update_accelerations()
     tot = [0, 0]
     for sid, (b, kw) in zip(ids, systems):
-        print(f"[forest pass] mode={mode} n={b.n} {kw}", flush=True)
+        print(f"[forest pass] solver={solver} mode={mode} n={b.n} {kw}", flush=True)
         _, acc = B.get(sid, accelerations=True)
-        a1, _, c = _alone(nbg, gpu_ctx, b, kw, mode)
+        a1, _, c = _alone(nbg, gpu_ctx, b, kw, mode, solver=solver)
         tot[0] += c[0]
         tot[1] += c[1]
         assert np.array_equal(acc.view(np.uint32), a1.view(np.uint32)), f"n={b.n}: batch differs from alone"
         if mode == 0:
-            t = orc.Tree(1, b.x, b.y, b.z, b.m, theta=kw["theta"], leaf=kw["leaf"], bits=30, G=kw["G"],
+            t = orc.Tree(solver, b.x, b.y, b.z, b.m, theta=kw["theta"], leaf=kw["leaf"], bits=30, G=kw["G"],
                          eps2=kw["eps2"])
             assert np.array_equal(acc, t.accel(b.x, b.y, b.z)), f"n={b.n}: batch differs from the oracle"
     assert B.counts() == tuple(tot)
     B.close()
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_forest_membership_changes(nbg, gpu_ctx, mode):
+@pytest.mark.parametrize("mode,solver", [(0, 1), (1, 1), (0, 4), (1, 4)])
+def test_forest_membership_changes(nbg, gpu_ctx, mode, solver):
     """Systems join and leave between steps; the others' trajectories are
     unaffected (same bits as alone with the same number of steps)."""
     systems = _systems(nbg)
     B = nbg.Batch(gpu_ctx, 60000, 16, sum_mode=mode)
-    ids = [B.add(_params(nbg, mode=mode, **kw), b) for b, kw in systems[:6]]
+    ids = [B.add(_params(nbg, mode=mode, solver=solver, **kw), b) for b, kw in systems[:6]]
     steps = {sid: 0 for sid in ids}
     data = {sid: systems[k] for k, sid in enumerate(ids)}
     B.step(3)
@@ -83,7 +86,7 @@ def test_forest_membership_changes(nbg, gpu_ctx, mode):
     del steps[ids[2]]
     B.remove(ids[0])           # and the first
     del steps[ids[0]]
-    late = B.add(_params(nbg, mode=mode, **systems[6][1]), systems[6][0])  # a new one joins
+    late = B.add(_params(nbg, mode=mode, solver=solver, **systems[6][1]), systems[6][0])  # a new one joins
     steps[late] = 0
     data[late] = systems[6]
     B.step(2)
@@ -94,18 +97,19 @@ def test_forest_membership_changes(nbg, gpu_ctx, mode):
     for sid, k in steps.items():
         b, kw = data[sid]
         st = B.get(sid)
-        _, ref, _ = _alone(nbg, gpu_ctx, b, kw, mode, steps=k)
+        _, ref, _ = _alone(nbg, gpu_ctx, b, kw, mode, steps=k, solver=solver)
         for g, r in zip((st.x, st.y, st.z, st.vx, st.vy, st.vz), (ref.x, ref.y, ref.z, ref.vx, ref.vy, ref.vz)):
             assert np.array_equal(g, r), f"system {sid} (n={b.n}) after {k} steps"
     B.close()
 
 
-def test_forest_graph_replay_equals_eager(nbg, gpu_ctx):
+@pytest.mark.parametrize("solver", [1, 4])
+def test_forest_graph_replay_equals_eager(nbg, gpu_ctx, solver):
     systems = _systems(nbg)[:6]
     runs = []
     for graph in (False, True):
         B = nbg.Batch(gpu_ctx, 40000, 8)
-        ids = [B.add(_params(nbg, **kw), b) for b, kw in systems]
+        ids = [B.add(_params(nbg, solver=solver, **kw), b) for b, kw in systems]
         B.use_graph(graph)
         snaps = []
         for _ in range(4):     # replays with host synchronisation in between
@@ -148,14 +152,48 @@ def test_forest_capacity_and_arguments(nbg, gpu_ctx):
     p.solver = nbg.BARNES_HUT
     with pytest.raises(nbg.NbgError):
         B.add(p, nbg.plummer(10, seed=2))  # octree systems are not batched
+    with pytest.raises(nbg.NbgError):
+        B.add(_params(nbg, solver=4), nbg.plummer(10, seed=3))  # members share one solver
     B.add(_params(nbg), nbg.plummer(10, seed=3))
     with pytest.raises(nbg.NbgError):
         B.add(_params(nbg), nbg.plummer(10, seed=4))  # 3 > 2 systems
     B.close()
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_forest_large_equals_alone(nbg, gpu_ctx, mode):
+def test_forest_solver_kind_changes(nbg, orc, gpu_ctx):
+    """A batch's solver follows its members: a sole member switched between
+    the monopole and quadrupole kinds, and a batch emptied and refilled with
+    the other kind, give each kind's bits alone and the oracle's."""
+    b1, b2 = nbg.plummer(3000, seed=71), nbg.plummer(2000, seed=72, a=1.5)
+    kw = dict(theta=0.5, leaf=16, eps2=1e-4, G=1.0, dt=1e-3)
+    B = nbg.Batch(gpu_ctx, 8000, 4, sum_mode=0)
+
+    def check(sid, b, solver):
+        B.update_accelerations()
+        _, acc = B.get(sid, accelerations=True)
+        a1, _, _ = _alone(nbg, gpu_ctx, b, kw, 0, solver=solver)
+        assert np.array_equal(acc.view(np.uint32), a1.view(np.uint32)), f"solver {solver}"
+        t = orc.Tree(solver, b.x, b.y, b.z, b.m, theta=0.5, leaf=16, bits=30, eps2=1e-4)
+        assert np.array_equal(acc, t.accel(b.x, b.y, b.z)), f"solver {solver}: oracle"
+
+    sid = B.add(_params(nbg, mode=0, **kw), b1)
+    check(sid, b1, 1)
+    for solver in (4, 1, 4):
+        B.set_params(sid, _params(nbg, mode=0, solver=solver, **kw))
+        check(sid, b1, solver)
+    other = B.add(_params(nbg, mode=0, solver=4, **kw), b2)
+    with pytest.raises(nbg.NbgError):
+        B.set_params(sid, _params(nbg, mode=0, solver=1, **kw))  # would mix the kinds
+    check(other, b2, 4)
+    B.remove(sid)
+    B.remove(other)
+    sid = B.add(_params(nbg, mode=0, solver=1, **kw), b2)
+    check(sid, b2, 1)
+    B.close()
+
+
+@pytest.mark.parametrize("mode,solver", [(0, 1), (1, 1), (0, 4)])
+def test_forest_large_equals_alone(nbg, gpu_ctx, mode, solver):
     """A forest above 2M bodies (the summaries then run their middle,
     many-workgroup pass, whose ranges cut through systems): every system
     still gets its bits alone."""
@@ -165,11 +203,11 @@ def test_forest_large_equals_alone(nbg, gpu_ctx, mode):
     total = sum(b.n for b, _ in systems)
     assert total > 8 * 4096 * 64
     B = nbg.Batch(gpu_ctx, total, len(systems), sum_mode=mode)
-    ids = [B.add(_params(nbg, mode=mode, **kw), b) for b, kw in systems]
+    ids = [B.add(_params(nbg, mode=mode, solver=solver, **kw), b) for b, kw in systems]
     B.update_accelerations()
     for sid, (b, kw) in zip(ids, systems):
-        print(f"[forest large] mode={mode} n={b.n}", flush=True)
+        print(f"[forest large] solver={solver} mode={mode} n={b.n}", flush=True)
         _, acc = B.get(sid, accelerations=True)
-        a1, _, _ = _alone(nbg, gpu_ctx, b, kw, mode)
+        a1, _, _ = _alone(nbg, gpu_ctx, b, kw, mode, solver=solver)
         assert np.array_equal(acc.view(np.uint32), a1.view(np.uint32))
     B.close()
