@@ -21,6 +21,7 @@ the oracle's CPU time for the same solver on this host.
 """
 
 import argparse
+import copy
 import importlib
 import json
 import os
@@ -464,6 +465,8 @@ def main():
                     help="quadrupole octree opening angle (the reference benchmark's tuned value)")
     ap.add_argument("--theta-fmm", type=float, default=0.4,
                     help="implicit FMM opening angle (its descent criterion's default)")
+    ap.add_argument("--theta-secondary", type=float, default=0.3,
+                    help="second opening angle measured for the monopole trees (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--cpu-fmm-starts", type=int, default=64,
                     help="starting nodes whose traversals time the oracle's implicit FMM")
@@ -522,6 +525,17 @@ def main():
         s0.close()
 
     res = {sv: measure(nbg, ctx, b, args, sv) for sv in solvers}
+    # the 1M configuration's second opening angle, for the monopole trees
+    extra = {}
+    if args.theta_secondary > 0:
+        for sv in (1, 0):
+            if sv in solvers:
+                a2 = copy.copy(args)
+                a2.theta = args.theta_secondary
+                log(f"[bench] {SOLVER_NAMES[sv]} at theta {a2.theta:g}")
+                m2 = measure(nbg, ctx, b, a2, sv)
+                m2["roofline"]["traffic"] = None  # the profiled walk traffic is theta 0.5's
+                extra[f"{SOLVER_NAMES[sv].replace('-', '_')}_theta{a2.theta:g}"] = m2
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     if not args.no_cpu:
         for sv in solvers:
@@ -572,6 +586,7 @@ def main():
             out[k] = v
     for sv in solvers[1:]:
         out[SOLVER_NAMES[sv].replace("-", "_")] = res[sv]
+    out.update(extra)
     if forest is not None:
         out["batched_ensemble"] = forest
     if serving is not None:
