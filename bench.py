@@ -322,6 +322,54 @@ def bench_forest(nbg, ctx, args):
     }
 
 
+def two_spheres(nbg, n_each):
+    """Two Plummer spheres of n_each bodies (a = 1, M = 0.5 each, seeds 31 and
+    32), centres +-(10, 1, 0), bulk velocities -+(0.3, 0, 0)."""
+    a = nbg.plummer(n_each, seed=31, M=0.5)
+    c = nbg.plummer(n_each, seed=32, M=0.5)
+    f32 = np.float32
+    return nbg.Bodies(np.concatenate([a.x + f32(10.0), c.x - f32(10.0)]),
+                      np.concatenate([a.y + f32(1.0), c.y - f32(1.0)]), np.concatenate([a.z, c.z]),
+                      np.concatenate([a.m, c.m]), np.concatenate([a.vx - f32(0.3), c.vx + f32(0.3)]),
+                      np.concatenate([a.vy, c.vy]), np.concatenate([a.vz, c.vz]))
+
+
+def bench_two_spheres(nbg, ctx, args):
+    """The colliding-spheres configuration (2 x 8,388,608 bodies, eps2 4e-6,
+    theta 0.5, leaf 16, 63-bit keys, dt 1e-3, 50 steps) for the linear BVH
+    and the octree, FAST sums: throughput and the force error against the FP64
+    direct sum on 16,384 bodies.  The energy drift is not computed here: the
+    O(N^2) FP64 potential of 16.7M bodies takes minutes per evaluation."""
+    log("[bench] two spheres: generating 2 x 8388608 bodies")
+    b = two_spheres(nbg, 8388608)
+    a2 = copy.copy(args)
+    a2.n, a2.eps2, a2.dt, a2.bits, a2.theta, a2.leaf = b.n, 4e-6, 1e-3, 63, 0.5, 16
+    a2.steps, a2.warmup, a2.no_accuracy = args.two_spheres_steps, 1, True
+    sub = np.random.default_rng(0).choice(b.n, 16384, replace=False)
+    direct = ctx.direct_f64(b, b.x[sub], b.y[sub], b.z[sub], 1.0, a2.eps2)
+    out = {"workload": f"two Plummer spheres of 8388608 bodies (a = 1, M = 0.5 each, seeds 31 and 32), centres "
+                       f"+-(10, 1, 0), bulk velocities -+(0.3, 0, 0); eps2 4e-6, theta 0.5, leaf 16, 63-bit keys, "
+                       f"dt 1e-3, {a2.steps} steps, FAST sums",
+           "n_bodies": b.n}
+    for sv in (1, 0):
+        m = measure(nbg, ctx, b, a2, sv)
+        m["roofline"]["traffic"] = None  # the profiled walk traffic is the 1M workload's
+        p = nbg.default_params()
+        p.solver, p.theta, p.leaf_size, p.morton_bits, p.eps2, p.dt = sv, 0.5, 16, 63, a2.eps2, a2.dt
+        p.sum_mode = nbg.SUM_FAST
+        s = nbg.System(ctx, p, b.n)
+        s.set_state(b)
+        acc = s.update_accelerations()[sub]
+        s.close()
+        rel = np.linalg.norm(acc - direct, axis=1) / np.linalg.norm(direct, axis=1)
+        m["accuracy"] = {"force_err_vs_fp64_median": float(np.median(rel)),
+                         "force_err_vs_fp64_p99": float(np.quantile(rel, 0.99)),
+                         "force_err_vs_fp64_max": float(rel.max()), "sample": int(len(sub)),
+                         "energy_drift": None}
+        out[SOLVER_NAMES[sv].replace("-", "_")] = m
+    return out
+
+
 def bench_serving(nbg, ctx, args):
     """Serving: a Poisson stream of jobs (Plummer sphere or uniform cube, N
     log-uniform in [1024, 2097152], steps uniform in [10, 500], theta in
@@ -480,6 +528,9 @@ def main():
                     help="solver of every system in the batched ensemble")
     ap.add_argument("--forest-alone-sample", type=int, default=64)
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--two-spheres", action="store_true",
+                    help="only measure the 16.7M-body colliding-spheres configuration (one JSON line)")
+    ap.add_argument("--two-spheres-steps", type=int, default=50)
     ap.add_argument("--serve-jobs", type=int, default=60)
     ap.add_argument("--serve-load", type=float, default=0.8)
     ap.add_argument("--serve-seed", type=int, default=11)
@@ -511,6 +562,10 @@ def main():
     if nbg.device_count() == 0:
         raise SystemExit("no GPU visible")
     ctx = nbg.Context(0)
+    if args.two_spheres:
+        print(json.dumps(bench_two_spheres(nbg, ctx, args)), flush=True)
+        ctx.close()
+        return
     log(f"[bench] generating Plummer N={args.n} seed={args.seed}")
     b = nbg.plummer(args.n, seed=args.seed)
     if not args.no_accuracy:
