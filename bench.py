@@ -339,7 +339,8 @@ def bench_two_spheres(nbg, ctx, args):
     theta 0.5, leaf 16, 63-bit keys, dt 1e-3, 50 steps) for the linear BVH
     and the octree, FAST sums: throughput and the force error against the FP64
     direct sum on 16,384 bodies.  The energy drift is not computed here: the
-    O(N^2) FP64 potential of 16.7M bodies takes minutes per evaluation."""
+    O(N^2) FP64 potential of 16.7M bodies takes about six minutes per
+    evaluation."""
     log("[bench] two spheres: generating 2 x 8388608 bodies")
     b = two_spheres(nbg, 8388608)
     a2 = copy.copy(args)
