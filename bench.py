@@ -71,6 +71,76 @@ def load_traffic():
     return out
 
 
+def host_info():
+    """CPU model and core counts of this host (the CPU baseline's machine)."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def error_stats(acc, ref, m):
+    """Force error of `acc` against the FP64 reference `ref` (N x 3): median,
+    p99 and max of |da| / |a_ref|, and the reference benchmark's
+    constitutional error (src/Benchmark/Graders/ConstitutionalGrader.h:50-64):
+    100 * max_i |F_i - F_ref,i| / min(|F_ref,i|, F_rms), F = m a, F_rms the
+    RMS reference force over the bodies compared."""
+    acc = np.asarray(acc, np.float64)
+    ref = np.asarray(ref, np.float64)
+    m = np.asarray(m, np.float64)
+    d = np.linalg.norm(acc - ref, axis=1)
+    mag = np.linalg.norm(ref, axis=1)
+    rel = d / mag
+    F, dF = mag * m, d * m
+    f_rms = np.sqrt(np.mean(F ** 2))
+    return {
+        "force_err_vs_fp64_median": float(np.median(rel)),
+        "force_err_vs_fp64_p99": float(np.quantile(rel, 0.99)),
+        "force_err_vs_fp64_max": float(rel.max()),
+        "constitutional_error_pct": float(100.0 * np.max(dF / np.minimum(F, f_rms))),
+        "sample": int(len(rel)),
+    }
+
+
+def tolerance_stats(acc, ref):
+    """FAST accelerations against the reference's own FP32 arithmetic (the
+    ORDERED pass, bit-exact with the oracle) over every body: the largest
+    |da| / max(|a_ref|, a_rms), the largest strict |da| / |a_ref|, and the
+    fraction of bodies with |da| > 1e-5 |a_ref|."""
+    acc = np.asarray(acc, np.float64)
+    ref = np.asarray(ref, np.float64)
+    d = np.linalg.norm(acc - ref, axis=1)
+    mag = np.linalg.norm(ref, axis=1)
+    floor = np.sqrt(np.mean(mag ** 2))
+    nz = mag > 0
+    return {
+        "max_rel_floored": float(np.max(d / np.maximum(mag, floor))),
+        "max_rel_strict": float(np.max(d[nz] / mag[nz])) if nz.any() else 0.0,
+        "frac_above_1e-5_strict": float(np.mean(d > 1e-5 * mag)),
+        "bodies": int(len(d)),
+    }
+
+
+def energy(m, vx, vy, vz, a, dt, W):
+    """Total energy with velocities synchronised to the positions: the
+    reference's step (Solver.h:103-139: v += a dt, then x += v dt) carries
+    v_{n-1/2}, so v(t_n) = v + a(x_n) dt / 2 at both ends of a run."""
+    m = np.asarray(m, np.float64)
+    v = np.stack([vx, vy, vz], 1).astype(np.float64) + np.asarray(a, np.float64) * (dt / 2)
+    return 0.5 * np.sum(m * (v ** 2).sum(1)) + W
+
+
 def solver_theta(args, solver):
     return {3: args.theta_4p_bh, 4: args.theta_4p_lbvh, 5: args.theta_fmm}.get(solver, args.theta)
 
@@ -125,12 +195,13 @@ def cpu_baseline(b, args, solver, threads):
     }
 
 
-def measure(nbg, ctx, b, args, solver):
+def measure(nbg, ctx, b, args, solver, mode=None):
+    mode = mode or args.mode
     p = nbg.default_params()
     p.solver = solver
     p.theta, p.leaf_size, p.morton_bits = solver_theta(args, solver), args.leaf, args.bits
     p.eps2, p.dt = args.eps2, args.dt
-    p.sum_mode = nbg.SUM_FAST if args.mode == "fast" else nbg.SUM_ORDERED
+    p.sum_mode = nbg.SUM_FAST if mode == "fast" else nbg.SUM_ORDERED
     s = nbg.System(ctx, p, args.n)
     s.set_state(b)
     a0 = s.update_accelerations()  # force pass at t = 0
@@ -158,6 +229,10 @@ def measure(nbg, ctx, b, args, solver):
     ws = s.walk_stats()
     s.enable_walk_stats(False)
 
+    # ORDERED walks always count (counts come with the bit-exact sums); the
+    # FAST walk's schedule statistics do not apply
+    if mode != "fast":
+        ws = {"node_visits": 0, "active_lanes": 0, "leaf_visits": 0, "leaf_lanes": 0}
     ms_per_step = 1e3 * elapsed / args.steps
     inter = pp + npc + pn + nn
     walk_ms = phases["walk"]
@@ -171,6 +246,7 @@ def measure(nbg, ctx, b, args, solver):
     out = {
         "value": args.n * args.steps / elapsed,
         "ms_per_step": ms_per_step,
+        "sum_mode": mode,
         "theta": p.theta,
         "interactions_per_step": inter,
         "interactions_per_s": inter / (ms_per_step * 1e-3),
@@ -184,7 +260,7 @@ def measure(nbg, ctx, b, args, solver):
         } if not fmm and ws["node_visits"] else (ws if fmm else None),
         "roofline": {
             "bound": "valu",
-            "kernel": WALK_KERNELS[solver],
+            "kernel": WALK_KERNELS[solver] if mode == "fast" else WALK_KERNELS[solver].replace("fast", "ordered"),
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
@@ -201,30 +277,28 @@ def measure(nbg, ctx, b, args, solver):
         },
     }
     if not args.no_accuracy:
-        log(f"[bench] {SOLVER_NAMES[solver]}: accuracy (FP64 direct sum on 16384 bodies, energy drift)")
+        log(f"[bench] {SOLVER_NAMES[solver]}: accuracy (FP64 direct sum on {len(args.sub)} bodies, energy drift)")
         st = s.get_state()
-        # synchronised end state: v(t_N) = v_{N-1/2} + a(x_N) dt / 2
         end = nbg.Bodies(st.x, st.y, st.z, b.m, st.vx, st.vy, st.vz)
         s2 = nbg.System(ctx, p, args.n)
         s2.set_state(end)
         aN = s2.update_accelerations()
         WN = s2.potential_f64()
-        m = b.m.astype(np.float64)
-        K0 = 0.5 * np.sum(m * (b.vx.astype(np.float64) ** 2 + b.vy.astype(np.float64) ** 2
-                               + b.vz.astype(np.float64) ** 2))
-        vs = np.stack([end.vx, end.vy, end.vz], 1).astype(np.float64) + aN.astype(np.float64) * (args.dt / 2)
-        KN = 0.5 * np.sum(m * (vs ** 2).sum(1))
-        E0, EN = K0 + args.W0, KN + WN
-        rel = np.linalg.norm(a0[args.sub] - args.direct, axis=1) / np.linalg.norm(args.direct, axis=1)
-        out["accuracy"] = {
-            "force_err_vs_fp64_median": float(np.median(rel)),
-            "force_err_vs_fp64_p99": float(np.quantile(rel, 0.99)),
-            "force_err_vs_fp64_max": float(rel.max()),
-            "sample": int(len(args.sub)),
-            "energy_drift": float(abs(EN - E0) / abs(E0)),
-            "E0": float(E0), "E_end": float(EN),
-        }
+        E0 = energy(b.m, b.vx, b.vy, b.vz, a0, args.dt, args.W0)
+        EN = energy(b.m, end.vx, end.vy, end.vz, aN, args.dt, WN)
+        out["accuracy"] = error_stats(a0[args.sub], args.direct, b.m[args.sub])
+        out["accuracy"].update({"energy_drift": float(abs(EN - E0) / abs(E0)), "E0": float(E0), "E_end": float(EN),
+                                "drift_steps": args.steps})
         s2.close()
+        if mode == "fast" and solver != 5:
+            # the same pass in the reference's FP32 arithmetic (ORDERED, the
+            # oracle's bits) over every body
+            po = type(p).from_buffer_copy(p)
+            po.sum_mode = nbg.SUM_ORDERED
+            so = nbg.System(ctx, po, args.n)
+            so.set_state(b)
+            out["accuracy"]["vs_reference_fp32"] = tolerance_stats(a0, so.update_accelerations())
+            so.close()
     s.close()
     return out
 
@@ -262,12 +336,23 @@ def bench_forest(nbg, ctx, args):
     p.morton_bits, p.theta, p.leaf_size = 30, 0.5, 16
     p.eps2, p.dt, p.sum_mode = 1e-4, 1e-3, nbg.SUM_FAST
     B = nbg.Batch(ctx, total, len(systems), sum_mode=nbg.SUM_FAST)
-    for b, _ in systems:
-        B.add(p, b)
+    ids = [B.add(p, b) for b, _ in systems]
     B.enable_counts(False)
     log(f"[bench] forest: {len(systems)} systems, {total} bodies; autotuning leaf size")
     best, tune_ms = B.autotune((8, 16, 32, 64), repeats=2)
     p.leaf_size = best
+    # accuracy sample: each sampled system's first pass and energy, alone
+    # (the forest gives each system the bits it gets alone)
+    acc_sample = np.random.default_rng(3).choice(len(systems), min(args.forest_accuracy_sample, len(systems)),
+                                                 replace=False)
+    start = {}
+    for k in acc_sample:
+        b, _ = systems[int(k)]
+        s = nbg.System(ctx, p, b.n)
+        s.set_state(b)
+        a0 = s.update_accelerations()
+        start[int(k)] = (a0, s.potential_f64())
+        s.close()
     B.use_graph(True)
     B.step(1)            # capture + warm-up
     B.synchronize()
@@ -278,6 +363,20 @@ def bench_forest(nbg, ctx, args):
     B.enable_counts(True)
     B.update_accelerations()
     pp, npc = B.counts()
+    errs, drifts = [], []
+    for k in acc_sample:
+        b, _ = systems[int(k)]
+        a0, W0 = start[int(k)]
+        d = ctx.direct_f64(b, b.x, b.y, b.z, 1.0, p.eps2)
+        errs.append(error_stats(a0, d, b.m))
+        end, aN = B.get(ids[int(k)], accelerations=True)
+        s = nbg.System(ctx, p, b.n)
+        s.set_state(nbg.Bodies(end.x, end.y, end.z, b.m, end.vx, end.vy, end.vz))
+        WN = s.potential_f64()
+        s.close()
+        E0 = energy(b.m, b.vx, b.vy, b.vz, a0, p.dt, W0)
+        EN = energy(b.m, end.vx, end.vy, end.vz, aN, p.dt, WN)
+        drifts.append(abs(EN - E0) / abs(E0))
     B.close()
     value = total * args.forest_steps / el
     # one system at a time through nbg_sys, on a sample of the same systems
@@ -319,6 +418,16 @@ def bench_forest(nbg, ctx, args):
                           "sample": f"{len(sample)} of the systems stepped alone through nbg_sys "
                                     f"(same parameters and leaf size), bodies-weighted"},
         "batched_over_one_at_a_time": value / alone,
+        "accuracy": {
+            "sample": f"{len(acc_sample)} systems: first force pass against the FP64 direct sum over all their bodies, "
+                      f"energy drift over the warm-up step and the {args.forest_steps} timed steps",
+            "force_err_vs_fp64_median": float(np.median([e["force_err_vs_fp64_median"] for e in errs])),
+            "force_err_vs_fp64_p99": float(np.max([e["force_err_vs_fp64_p99"] for e in errs])),
+            "force_err_vs_fp64_max": float(np.max([e["force_err_vs_fp64_max"] for e in errs])),
+            "constitutional_error_pct_max": float(np.max([e["constitutional_error_pct"] for e in errs])),
+            "energy_drift_median": float(np.median(drifts)),
+            "energy_drift_max": float(np.max(drifts)),
+        },
     }
 
 
@@ -334,40 +443,121 @@ def two_spheres(nbg, n_each):
                       np.concatenate([a.vy, c.vy]), np.concatenate([a.vz, c.vz]))
 
 
-def bench_two_spheres(nbg, ctx, args):
+def bench_two_spheres(nbg, ctx, args, threads):
     """The colliding-spheres configuration (2 x 8,388,608 bodies, eps2 4e-6,
     theta 0.5, leaf 16, 63-bit keys, dt 1e-3, 50 steps) for the linear BVH
-    and the octree, FAST sums: throughput and the force error against the FP64
-    direct sum on 16,384 bodies.  The energy drift is not computed here: the
-    O(N^2) FP64 potential of 16.7M bodies takes about six minutes per
-    evaluation."""
+    and the octree, FAST sums: throughput, the force error against the FP64
+    direct sum on 16,384 bodies, the error against the reference's FP32
+    arithmetic over every body, the energy drift over the 50 steps (FP64
+    potential, each pair once, in launches of about 1e12 pairs), and the
+    oracle's CPU time for one step (build + a walk sample)."""
     log("[bench] two spheres: generating 2 x 8388608 bodies")
     b = two_spheres(nbg, 8388608)
     a2 = copy.copy(args)
     a2.n, a2.eps2, a2.dt, a2.bits, a2.theta, a2.leaf = b.n, 4e-6, 1e-3, 63, 0.5, 16
-    a2.steps, a2.warmup, a2.no_accuracy = args.two_spheres_steps, 1, True
-    sub = np.random.default_rng(0).choice(b.n, 16384, replace=False)
-    direct = ctx.direct_f64(b, b.x[sub], b.y[sub], b.z[sub], 1.0, a2.eps2)
+    a2.steps, a2.warmup, a2.no_accuracy = args.two_spheres_steps, 1, False
+    a2.sub = np.random.default_rng(0).choice(b.n, 16384, replace=False)
+    log("[bench] two spheres: FP64 direct sum on 16384 bodies, initial potential")
+    a2.direct = ctx.direct_f64(b, b.x[a2.sub], b.y[a2.sub], b.z[a2.sub], 1.0, a2.eps2)
+    p0 = nbg.default_params()
+    p0.eps2 = a2.eps2
+    s0 = nbg.System(ctx, p0, b.n)
+    s0.set_state(b)
+    t0 = time.perf_counter()
+    a2.W0 = s0.potential_f64()
+    t_pot = time.perf_counter() - t0
+    s0.close()
+    log(f"[bench] two spheres: potential took {t_pot:.1f} s")
     out = {"workload": f"two Plummer spheres of 8388608 bodies (a = 1, M = 0.5 each, seeds 31 and 32), centres "
                        f"+-(10, 1, 0), bulk velocities -+(0.3, 0, 0); eps2 4e-6, theta 0.5, leaf 16, 63-bit keys, "
                        f"dt 1e-3, {a2.steps} steps, FAST sums",
-           "n_bodies": b.n}
+           "n_bodies": b.n, "potential_eval_s": t_pot}
     for sv in (1, 0):
         m = measure(nbg, ctx, b, a2, sv)
         m["roofline"]["traffic"] = None  # the profiled walk traffic is the 1M workload's
-        p = nbg.default_params()
-        p.solver, p.theta, p.leaf_size, p.morton_bits, p.eps2, p.dt = sv, 0.5, 16, 63, a2.eps2, a2.dt
-        p.sum_mode = nbg.SUM_FAST
-        s = nbg.System(ctx, p, b.n)
-        s.set_state(b)
-        acc = s.update_accelerations()[sub]
-        s.close()
-        rel = np.linalg.norm(acc - direct, axis=1) / np.linalg.norm(direct, axis=1)
-        m["accuracy"] = {"force_err_vs_fp64_median": float(np.median(rel)),
-                         "force_err_vs_fp64_p99": float(np.quantile(rel, 0.99)),
-                         "force_err_vs_fp64_max": float(rel.max()), "sample": int(len(sub)),
-                         "energy_drift": None}
+        if not args.no_cpu:
+            a3 = copy.copy(a2)
+            a3.cpu_sample = args.two_spheres_cpu_sample
+            log(f"[bench] two spheres: CPU baseline ({SOLVER_NAMES[sv]}) with {threads} threads")
+            m["cpu_baseline"] = cpu_baseline(b, a3, sv, threads)
+            m["gpu_over_cpu"] = m["value"] / m["cpu_baseline"]["value"]
         out[SOLVER_NAMES[sv].replace("-", "_")] = m
+    return out
+
+
+def bench_c1(nbg, ctx, args, threads):
+    """The 10,000-body configuration: Plummer N = 10,000 (seed 0), eps2 1e-4,
+    theta 0.5, leaf 16, 63-bit keys (the reference's default key width,
+    MortonSort.h:25), dt 1e-3, 100 steps, both trees and both sum modes,
+    timed over the whole run and reported per step as the reference's
+    timedRun does (src/Benchmark/benchmark.h:29-39), beside the oracle's
+    100 steps on this host's cores (the reference's CPU path), with the force
+    error of the first pass against the FP64 direct sum over every body and
+    the energy drift of each run."""
+    import oracle
+
+    n, steps, eps2, dt, theta, leaf, bits = 10000, 100, 1e-4, 1e-3, 0.5, 16, 63
+    b = nbg.plummer(n, seed=0)
+    direct = ctx.direct_f64(b, b.x, b.y, b.z, 1.0, eps2)
+
+    def potential(x, y, z):
+        p0 = nbg.default_params()
+        p0.eps2 = eps2
+        s0 = nbg.System(ctx, p0, n)
+        s0.set_state(nbg.Bodies(x, y, z, b.m, b.vx, b.vy, b.vz))
+        w = s0.potential_f64()
+        s0.close()
+        return w
+
+    W0 = potential(b.x, b.y, b.z)
+    out = {"workload": f"Plummer N = {n} (a = 1, M = 1, G = 1, r < 10, seed 0), eps2 {eps2}, theta {theta}, "
+                       f"leaf {leaf}, {bits}-bit keys, dt {dt}, {steps} steps"}
+    for sv in (1, 0):
+        name = SOLVER_NAMES[sv]
+        log(f"[bench] 10k bodies: {name}")
+        blk = {}
+        for mode in ("fast", "ordered"):
+            p = nbg.default_params()
+            p.solver, p.theta, p.leaf_size, p.morton_bits, p.eps2, p.dt = sv, theta, leaf, bits, eps2, dt
+            p.sum_mode = nbg.SUM_FAST if mode == "fast" else nbg.SUM_ORDERED
+            s = nbg.System(ctx, p, n)
+            s.set_state(b)
+            a0 = s.update_accelerations()
+            s.enable_counts(False)
+            s.step(5)
+            s.synchronize()
+            s.set_state(b)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.step(steps)
+            s.synchronize()
+            el = time.perf_counter() - t0
+            aN = s.update_accelerations()
+            st = s.get_state()
+            s.close()
+            E0 = energy(b.m, b.vx, b.vy, b.vz, a0, dt, W0)
+            EN = energy(b.m, st.vx, st.vy, st.vz, aN, dt, potential(st.x, st.y, st.z))
+            acc = error_stats(a0, direct, b.m)
+            acc["energy_drift"] = float(abs(EN - E0) / abs(E0))
+            blk[mode] = {"value": n * steps / el, "unit": "bodies·steps/s", "ms_per_step": 1e3 * el / steps,
+                         "accuracy": acc}
+        # the oracle: all 100 steps, timed, with its own force error and drift
+        bodies = (b.x, b.y, b.z, b.m, b.vx, b.vy, b.vz)
+        t0 = time.perf_counter()
+        (x, y, z, vx, vy, vz), _ = oracle.step(sv, bodies, steps, theta=theta, leaf=leaf, bits=bits, eps2=eps2,
+                                              dt=dt, threads=threads)
+        el = time.perf_counter() - t0
+        a0c = oracle.Tree(sv, b.x, b.y, b.z, b.m, theta=theta, leaf=leaf, bits=bits, eps2=eps2).accel(b.x, b.y, b.z)
+        aNc = oracle.Tree(sv, x, y, z, b.m, theta=theta, leaf=leaf, bits=bits, eps2=eps2).accel(x, y, z)
+        E0 = energy(b.m, b.vx, b.vy, b.vz, a0c, dt, W0)
+        EN = energy(b.m, vx, vy, vz, aNc, dt, potential(x, y, z))
+        cacc = error_stats(a0c, direct, b.m)
+        cacc["energy_drift"] = float(abs(EN - E0) / abs(E0))
+        blk["cpu_baseline"] = {"value": n * steps / el, "unit": "bodies·steps/s", "cores": threads, "kind": "port",
+                               "sample": f"the oracle's {name} run, all {steps} steps", "accuracy": cacc}
+        blk["gpu_over_cpu"] = blk["fast"]["value"] / blk["cpu_baseline"]["value"]
+        blk["ordered_gpu_over_cpu"] = blk["ordered"]["value"] / blk["cpu_baseline"]["value"]
+        out[name.replace("-", "_")] = blk
     return out
 
 
@@ -531,7 +721,14 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
     ap.add_argument("--two-spheres", action="store_true",
                     help="only measure the 16.7M-body colliding-spheres configuration (one JSON line)")
+    ap.add_argument("--no-two-spheres", action="store_true",
+                    help="skip the colliding-spheres block of the default run")
     ap.add_argument("--two-spheres-steps", type=int, default=50)
+    ap.add_argument("--two-spheres-cpu-sample", type=int, default=32768)
+    ap.add_argument("--no-c1", action="store_true", help="skip the 10,000-body block")
+    ap.add_argument("--no-ordered", action="store_true", help="skip the ORDERED (bit-exact) blocks")
+    ap.add_argument("--ordered-steps", type=int, default=5)
+    ap.add_argument("--forest-accuracy-sample", type=int, default=32)
     ap.add_argument("--serve-jobs", type=int, default=60)
     ap.add_argument("--serve-load", type=float, default=0.8)
     ap.add_argument("--serve-seed", type=int, default=11)
@@ -563,8 +760,10 @@ def main():
     if nbg.device_count() == 0:
         raise SystemExit("no GPU visible")
     ctx = nbg.Context(0)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    host = host_info()
     if args.two_spheres:
-        print(json.dumps(bench_two_spheres(nbg, ctx, args)), flush=True)
+        print(json.dumps(bench_two_spheres(nbg, ctx, args, threads)), flush=True)
         ctx.close()
         return
     log(f"[bench] generating Plummer N={args.n} seed={args.seed}")
@@ -592,13 +791,42 @@ def main():
                 m2 = measure(nbg, ctx, b, a2, sv)
                 m2["roofline"]["traffic"] = None  # the profiled walk traffic is theta 0.5's
                 extra[f"{SOLVER_NAMES[sv].replace('-', '_')}_theta{a2.theta:g}"] = m2
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    # the bit-exact path (the reference's FP32 arithmetic) at the same configuration
+    ordered = {}
+    if not args.no_ordered:
+        for sv in (1, 0):
+            if sv in solvers:
+                a2 = copy.copy(args)
+                a2.steps, a2.warmup = args.ordered_steps, 1
+                log(f"[bench] {SOLVER_NAMES[sv]} ORDERED")
+                m2 = measure(nbg, ctx, b, a2, sv, mode="ordered")
+                m2["roofline"]["traffic"] = None  # the profiled walk traffic is the FAST walk's
+                ordered[sv] = m2
+                extra[f"{SOLVER_NAMES[sv].replace('-', '_')}_ordered"] = m2
     if not args.no_cpu:
         for sv in solvers:
             log(f"[bench] CPU baseline ({SOLVER_NAMES[sv]}) with {threads} threads")
             cb = cpu_baseline(b, args, sv, threads)
+            cb["host"] = host
             res[sv]["cpu_baseline"] = cb
             res[sv]["gpu_over_cpu"] = res[sv]["value"] / cb["value"]
+            if sv in ordered:
+                ordered[sv]["cpu_baseline"] = cb
+                ordered[sv]["gpu_over_cpu"] = ordered[sv]["value"] / cb["value"]
+
+    c1 = None
+    if not args.no_c1:
+        try:
+            c1 = bench_c1(nbg, ctx, args, threads)
+        except Exception as e:  # reported, not fatal for the headline line
+            c1 = {"error": f"{type(e).__name__}: {e}"}
+
+    c3 = None
+    if not args.no_two_spheres:
+        try:
+            c3 = bench_two_spheres(nbg, ctx, args, threads)
+        except Exception as e:
+            c3 = {"error": f"{type(e).__name__}: {e}"}
 
     forest = None
     if not args.no_forest:
@@ -643,6 +871,10 @@ def main():
     for sv in solvers[1:]:
         out[SOLVER_NAMES[sv].replace("-", "_")] = res[sv]
     out.update(extra)
+    if c1 is not None:
+        out["plummer_10k"] = c1
+    if c3 is not None:
+        out["two_spheres"] = c3
     if forest is not None:
         out["batched_ensemble"] = forest
     if serving is not None:

@@ -267,6 +267,13 @@ int nbg_batch_info(nbg_batch *b, int32_t *nsys, int64_t *nbodies);
 int nbg_batch_get(nbg_batch *b, int32_t id, float *x, float *y, float *z, float *vx, float *vy, float *vz,
                   float *ax, float *ay, float *az);
 
+/* Non-finite guard (no reference counterpart: the reference lets NaN
+ * propagate): finds the systems whose positions or velocities hold a NaN or
+ * an infinity.  Writes up to cap of their ids to ids and their number to
+ * *count.  Removing them before the next step keeps their values out of the
+ * shared build. */
+int nbg_batch_nonfinite(nbg_batch *b, int32_t *ids, int32_t cap, int32_t *count);
+
 /* Force pass for every system (ActiveTreeSolver::updateAccelerations each). */
 int nbg_batch_update_accelerations(nbg_batch *b);
 
@@ -303,7 +310,10 @@ typedef struct nbg_server nbg_server;
 
 int nbg_server_create(nbg_ctx *ctx, int64_t hbm_budget_bytes, int32_t max_systems, int32_t sum_mode,
                       nbg_server **out);
-/* Stops the worker (jobs still queued or running are abandoned). */
+/* Stops the worker.  Jobs still queued or running end with NBG_ERR_STATE;
+ * clients blocked in nbg_server_wait return that status, and destroy returns
+ * once they have left.  A finished job's result is held until it is waited
+ * for (or until destroy). */
 int nbg_server_destroy(nbg_server *s);
 int nbg_server_capacity(nbg_server *s, int64_t *max_bodies, int32_t *max_systems);
 

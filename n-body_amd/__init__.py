@@ -47,7 +47,7 @@ EXPORTS = (
     "nbg_batch_bytes", "nbg_batch_create", "nbg_batch_destroy", "nbg_batch_add", "nbg_batch_remove",
     "nbg_batch_set_params", "nbg_batch_info", "nbg_batch_get", "nbg_batch_update_accelerations", "nbg_batch_step",
     "nbg_batch_synchronize", "nbg_batch_counts", "nbg_batch_enable_counts", "nbg_batch_use_graph",
-    "nbg_batch_autotune",
+    "nbg_batch_autotune", "nbg_batch_nonfinite",
     "nbg_server_create", "nbg_server_destroy", "nbg_server_capacity", "nbg_server_submit", "nbg_server_wait",
     "nbg_server_stats",
 )
@@ -122,6 +122,8 @@ def lib() -> ctypes.CDLL:
             "nbg_batch_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)]),
             "nbg_batch_get": (ctypes.c_int, [_vp, ctypes.c_int32] + [_f32p] * 9),
             "nbg_batch_update_accelerations": (ctypes.c_int, [_vp]),
+            "nbg_batch_nonfinite": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
+                                                   ctypes.POINTER(ctypes.c_int32)]),
             "nbg_batch_step": (ctypes.c_int, [_vp, ctypes.c_int32]),
             "nbg_batch_synchronize": (ctypes.c_int, [_vp]),
             "nbg_batch_counts": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
@@ -456,6 +458,15 @@ class Batch:
         _check(lib().nbg_batch_get(self.handle, sid, *[_ptr(v) for v in (out.x, out.y, out.z, out.vx, out.vy, out.vz)],
                                    *[_ptr(v) if v is not None else None for v in acc]))
         return (out, np.stack(acc, axis=1)) if accelerations else out
+
+    def nonfinite(self):
+        """Ids of the systems whose positions or velocities hold NaN or inf."""
+        cap = max(1, len(self._n))
+        ids = np.zeros(cap, np.int32)
+        k = ctypes.c_int32()
+        _check(lib().nbg_batch_nonfinite(self.handle, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cap,
+                                         ctypes.byref(k)))
+        return [int(v) for v in ids[:min(k.value, cap)]]
 
     def update_accelerations(self):
         _check(lib().nbg_batch_update_accelerations(self.handle))

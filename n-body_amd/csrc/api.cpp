@@ -85,6 +85,10 @@ struct nbg_sys {
     int64_t fmm_item_cap = 0, fmm_pool_cap = 0;
     int fmm_seed_cap = 0;
     int fmm_shift = 0;  // capacity doublings after passes that outgrew the pool
+    // passive-point walks (nbg_sys_accelerations_at): one buffer that grows
+    // to the largest target count seen and is kept for later calls
+    void *pts = nullptr;
+    size_t pts_bytes = 0;
     // small
     float *bbox_partials = nullptr, *bbox = nullptr;
     // [1] err, [2] node count, [4..16) counts: pp, np, then the FAST walk's
@@ -217,6 +221,7 @@ void free_sys(nbg_sys *s) {
     free_oct(s->ob);
     free_fmm(s);
     if (s->quad) (void)hipFree(s->quad);
+    if (s->pts) (void)hipFree(s->pts);
     void *ptrs[] = {s->x, s->y, s->z, s->m, s->vx, s->vy, s->vz, s->ax, s->ay, s->az, s->keys, s->keys_alt,
                     s->idx, s->idx_alt, s->bodies, s->kfirst, s->klast, s->kgamma, s->kparent, s->ksplit,
                     s->krank, s->leaf_start, s->glist, s->scan_ws, s->sort_status, s->nodes, s->bmin,
@@ -401,20 +406,24 @@ void launch_walk(nbg_sys *s, const WalkArgs &wa, int stats) {
         launch_walk_lbvh(wa, p.sum_mode == NBG_SUM_ORDERED, p.morton_bits == 30 ? 32 : 64, stats, s->stream);
 }
 
+// Clears the device error word (stream idle) and reports the failure its
+// bits name.
+int report_device_error(nbg_sys *s, uint32_t err) {
+    uint32_t zero = 0;
+    (void)hipMemcpy(s->err_word(), &zero, sizeof zero, hipMemcpyHostToDevice);
+    if (err & 1u) return fail(NBG_ERR_DEVICE, "radix sort look-back exceeded its spin limit");
+    if (err & 2u) return fail(NBG_ERR_DEVICE, "tree traversal exceeded its stack, depth or index limits");
+    if (err & 4u) return fail(NBG_ERR_DEVICE, "tree summaries found an invalid node index");
+    if (err & 8u) return fail(NBG_ERR_DEVICE, "octree build found an inconsistent node index");
+    if (err & 16u) return fail(NBG_ERR_DEVICE, "implicit FMM interaction lists exceeded their capacity");
+    return fail(NBG_ERR_DEVICE, "device error flag set");
+}
+
 int sync_and_check(nbg_sys *s) {
     NBG_HIP(hipStreamSynchronize(s->stream));
     uint32_t err = 0;
     NBG_HIP(hipMemcpy(&err, s->err_word(), sizeof err, hipMemcpyDeviceToHost));
-    if (err) {
-        uint32_t zero = 0;
-        (void)hipMemcpy(s->err_word(), &zero, sizeof zero, hipMemcpyHostToDevice);
-        if (err & 1u) return fail(NBG_ERR_DEVICE, "radix sort look-back exceeded its spin limit");
-        if (err & 2u) return fail(NBG_ERR_DEVICE, "tree traversal exceeded its stack, depth or index limits");
-        if (err & 4u) return fail(NBG_ERR_DEVICE, "tree summaries found an invalid node index");
-        if (err & 8u) return fail(NBG_ERR_DEVICE, "octree build found an inconsistent node index");
-        if (err & 16u) return fail(NBG_ERR_DEVICE, "implicit FMM interaction lists exceeded their capacity");
-        return fail(NBG_ERR_DEVICE, "device error flag set");
-    }
+    if (err) return report_device_error(s, err);
     fold_timing(s);
     return NBG_OK;
 }
@@ -430,11 +439,111 @@ int force_pass(nbg_sys *s) {
         NBG_HIP(hipStreamSynchronize(s->stream));
         uint32_t err = 0;
         NBG_HIP(hipMemcpy(&err, s->err_word(), sizeof err, hipMemcpyDeviceToHost));
-        if (err != 16u || s->fmm_pool_cap >= kFmmPoolMax) return NBG_OK;  // reported by sync_and_check
+        if (err == 0) return NBG_OK;
+        // a failed pass is reported here, before the caller enqueues an update
+        // that would move the bodies with partial accelerations
+        if (err != 16u || s->fmm_pool_cap >= kFmmPoolMax) return report_device_error(s, err);
         const uint32_t zero = 0;
         NBG_HIP(hipMemcpy(s->err_word(), &zero, sizeof zero, hipMemcpyHostToDevice));
         ++s->fmm_shift;
     }
+}
+
+// Device scratch of a passive-point walk of nt targets, carved from one
+// buffer: positions (3 nt floats), results (3 nt floats), Morton keys and
+// their sort's alternate buffer (2 nt u64), the permutation and its
+// alternate (2 nt u32), the targets in walk order (nt float4) and the sort's
+// status words.  Offsets are 256-byte aligned.
+struct PointsLayout {
+    size_t t3, a3, k2, i2, t4, status, bytes;
+};
+constexpr size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+constexpr PointsLayout points_layout(int64_t nt) {
+    PointsLayout L{};
+    size_t o = 0;
+    L.t3 = o; o = align256(o + (size_t)nt * 3 * sizeof(float));
+    L.a3 = o; o = align256(o + (size_t)nt * 3 * sizeof(float));
+    L.k2 = o; o = align256(o + (size_t)nt * 2 * sizeof(uint64_t));
+    L.i2 = o; o = align256(o + (size_t)nt * 2 * sizeof(uint32_t));
+    L.t4 = o; o = align256(o + (size_t)nt * sizeof(float4));
+    L.status = o; o = align256(o + (size_t)sort_status_words(nt, 64) * sizeof(uint32_t));
+    L.bytes = o;
+    return L;
+}
+
+struct PointsBuf {
+    float *t3, *a3;
+    uint64_t *k2;
+    uint32_t *i2, *status;
+    float4 *t4;
+};
+
+PointsBuf points_carve(void *base, int64_t nt) {
+    const PointsLayout L = points_layout(nt);
+    char *b = (char *)base;
+    PointsBuf p;
+    p.t3 = (float *)(b + L.t3);
+    p.a3 = (float *)(b + L.a3);
+    p.k2 = (uint64_t *)(b + L.k2);
+    p.i2 = (uint32_t *)(b + L.i2);
+    p.t4 = (float4 *)(b + L.t4);
+    p.status = (uint32_t *)(b + L.status);
+    return p;
+}
+
+int check_points(nbg_sys *s, int64_t nt, const float *tx, const float *ty, const float *tz, float *ax, float *ay,
+                 float *az) {
+    if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
+    if (nt < 0 || nt > kMaxBodies) return fail(NBG_ERR_ARG, "target count out of range");
+    if (nt > 0 && (!tx || !ty || !tz || !ax || !ay || !az)) return fail(NBG_ERR_ARG, "NULL array");
+    if (!s->have_state) return fail(NBG_ERR_STATE, "set_state has not been called");
+    if (s->prm.solver == NBG_IMPLICIT_FMM)
+        return fail(NBG_ERR_STATE, "the implicit FMM's passive points are its own bodies");
+    return NBG_OK;
+}
+
+// The walk of nt > 0 passive points through the device scratch `pb` (laid
+// out by points_layout(nt)).  Uses the tree of the system's current state,
+// building it first when the state or parameters changed since the last
+// force pass.  Targets are walked in Morton order over the system's bounding
+// box (63-bit keys, clamped outside it) so that a wavefront's targets are
+// spatially adjacent; results return in the caller's order.  Blocking.
+int walk_points(nbg_sys *s, int64_t nt, const float *tx, const float *ty, const float *tz, float *ax, float *ay,
+                float *az, const PointsBuf &pb) {
+    if (s->n == 0) {  // no actives: every passive acceleration is zero
+        for (int64_t i = 0; i < nt; ++i) ax[i] = ay[i] = az[i] = 0.0f;
+        return NBG_OK;
+    }
+    if (!s->tree_valid) {
+        // builds the tree (and, as a side effect, the bodies' own accelerations)
+        if (int r = enqueue_force_pass(s)) return r;
+        mark(s, kPhases);
+        if (int r = sync_and_check(s)) return r;
+        s->have_forces = true;
+    }
+    hipStream_t st = s->stream;
+    const size_t b = (size_t)nt * sizeof(float);
+    NBG_HIP(hipMemcpyAsync(pb.t3, tx, b, hipMemcpyHostToDevice, st));
+    NBG_HIP(hipMemcpyAsync(pb.t3 + nt, ty, b, hipMemcpyHostToDevice, st));
+    NBG_HIP(hipMemcpyAsync(pb.t3 + 2 * nt, tz, b, hipMemcpyHostToDevice, st));
+    launch_morton(pb.t3, pb.t3 + nt, pb.t3 + 2 * nt, nt, s->bbox, 63, pb.k2, pb.i2, st);
+    launch_radix_sort(pb.k2, pb.i2, pb.k2 + nt, pb.i2 + nt, nt, 63, pb.status, s->err_word(), st);
+    launch_gather(pb.t3, pb.t3 + nt, pb.t3 + 2 * nt, pb.t3, pb.i2, nt, pb.t4, st);
+    launch_fill_u32(s->misc + 4, 0u, 12, st);
+    WalkArgs wa = walk_args(s);
+    wa.nt = nt;
+    wa.targets = pb.t4;
+    wa.perm = pb.i2;
+    wa.ax = pb.a3;
+    wa.ay = pb.a3 + nt;
+    wa.az = pb.a3 + 2 * nt;
+    launch_walk(s, wa, s->counting || s->prm.sum_mode == NBG_SUM_ORDERED ? 1 : 0);
+    NBG_HIP(hipGetLastError());
+    if (int r = sync_and_check(s)) return r;
+    NBG_HIP(hipMemcpy(ax, pb.a3, b, hipMemcpyDeviceToHost));
+    NBG_HIP(hipMemcpy(ay, pb.a3 + nt, b, hipMemcpyDeviceToHost));
+    NBG_HIP(hipMemcpy(az, pb.a3 + 2 * nt, b, hipMemcpyDeviceToHost));
+    return NBG_OK;
 }
 
 }  // namespace
@@ -625,72 +734,19 @@ int nbg_sys_get_accelerations(nbg_sys *s, float *ax, float *ay, float *az) {
 
 int nbg_sys_accelerations_at(nbg_sys *s, int64_t nt, const float *tx, const float *ty, const float *tz, float *ax,
                              float *ay, float *az) {
-    if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
-    if (nt < 0 || nt > kMaxBodies) return fail(NBG_ERR_ARG, "target count out of range");
-    if (nt > 0 && (!tx || !ty || !tz || !ax || !ay || !az)) return fail(NBG_ERR_ARG, "NULL array");
-    if (!s->have_state) return fail(NBG_ERR_STATE, "set_state has not been called");
-    if (s->prm.solver == NBG_IMPLICIT_FMM)
-        return fail(NBG_ERR_STATE, "the implicit FMM's passive points are its own bodies");
+    if (int r = check_points(s, nt, tx, ty, tz, ax, ay, az)) return r;
     if (nt == 0) return NBG_OK;
     DeviceGuard g(s->ctx->device);
-    if (s->n == 0) {  // no actives: every passive acceleration is zero
-        for (int64_t i = 0; i < nt; ++i) ax[i] = ay[i] = az[i] = 0.0f;
-        return NBG_OK;
+    const size_t need = points_layout(nt).bytes;
+    if (need > s->pts_bytes) {
+        NBG_HIP(hipStreamSynchronize(s->stream));
+        if (s->pts) (void)hipFree(s->pts);
+        s->pts = nullptr;
+        s->pts_bytes = 0;
+        NBG_HIP(hipMalloc(&s->pts, need));
+        s->pts_bytes = need;
     }
-    if (!s->tree_valid) {
-        // builds the tree (and, as a side effect, the bodies' own accelerations)
-        if (int r = enqueue_force_pass(s)) return r;
-        mark(s, kPhases);
-        if (int r = sync_and_check(s)) return r;
-        s->have_forces = true;
-    }
-    // Targets are walked in Morton order over the system's bounding box
-    // (63-bit keys, clamped outside it) so that a wavefront's targets are
-    // spatially adjacent; results return in the caller's order.
-    hipStream_t st = s->stream;
-    float *t3 = nullptr, *a3 = nullptr;
-    uint64_t *k2 = nullptr;
-    uint32_t *i2 = nullptr, *status = nullptr;
-    float4 *t4 = nullptr;
-    int r = NBG_OK;
-    auto A = [&](auto **ptr, int64_t count) {
-        if (r == NBG_OK) r = dalloc(ptr, count);
-    };
-    A(&t3, 3 * nt); A(&a3, 3 * nt); A(&k2, 2 * nt); A(&i2, 2 * nt); A(&t4, nt);
-    A(&status, sort_status_words(nt, 64));
-    if (r == NBG_OK) {
-        auto up = [&](float *d, const float *h) {
-            hipError_t e = hipMemcpyAsync(d, h, (size_t)nt * 4, hipMemcpyHostToDevice, st);
-            if (e != hipSuccess && r == NBG_OK) r = fail(NBG_ERR_HIP, std::string("H2D: ") + hipGetErrorString(e));
-        };
-        up(t3, tx); up(t3 + nt, ty); up(t3 + 2 * nt, tz);
-    }
-    if (r == NBG_OK) {
-        launch_morton(t3, t3 + nt, t3 + 2 * nt, nt, s->bbox, 63, k2, i2, st);
-        launch_radix_sort(k2, i2, k2 + nt, i2 + nt, nt, 63, status, s->err_word(), st);
-        launch_gather(t3, t3 + nt, t3 + 2 * nt, t3, i2, nt, t4, st);
-        launch_fill_u32(s->misc + 4, 0u, 12, st);
-        WalkArgs wa = walk_args(s);
-        wa.nt = nt;
-        wa.targets = t4;
-        wa.perm = i2;
-        wa.ax = a3;
-        wa.ay = a3 + nt;
-        wa.az = a3 + 2 * nt;
-        launch_walk(s, wa, s->counting || s->prm.sum_mode == NBG_SUM_ORDERED ? 1 : 0);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) r = fail(NBG_ERR_HIP, std::string("walk launch: ") + hipGetErrorString(e));
-    }
-    if (r == NBG_OK) r = sync_and_check(s);
-    if (r == NBG_OK) {
-        hipError_t e = hipMemcpy(ax, a3, (size_t)nt * 4, hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemcpy(ay, a3 + nt, (size_t)nt * 4, hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemcpy(az, a3 + 2 * nt, (size_t)nt * 4, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) r = fail(NBG_ERR_HIP, std::string("D2H: ") + hipGetErrorString(e));
-    }
-    for (void *p : {(void *)t3, (void *)a3, (void *)k2, (void *)i2, (void *)t4, (void *)status})
-        if (p) (void)hipFree(p);
-    return r;
+    return walk_points(s, nt, tx, ty, tz, ax, ay, az, points_carve(s->pts, nt));
 }
 
 int nbg_sys_step(nbg_sys *s, int32_t nsteps) {
@@ -952,24 +1008,39 @@ int nbg_sys_potential_f64(nbg_sys *s, double *W) {
     const int64_t n = s->n;
     *W = 0.0;
     if (n < 2) return NBG_OK;
-    const int64_t nb = potential_blocks(n);
+    const int64_t nw = potential_waves(n);
     double *part = nullptr;
-    float4 *packed = nullptr;
-    NBG_HIP(hipMalloc((void **)&part, (size_t)nb * sizeof(double)));
-    hipError_t e = hipMalloc((void **)&packed, (size_t)n * sizeof(float4));
+    double4 *packed = nullptr;
+    NBG_HIP(hipMalloc((void **)&part, (size_t)nw * sizeof(double)));
+    hipError_t e = hipMalloc((void **)&packed, (size_t)n * sizeof(double4));
     if (e == hipSuccess) {
-        launch_pack(s->x, s->y, s->z, s->m, n, packed, s->stream);
-        launch_potential_f64(packed, n, s->prm.G, s->prm.eps2, part, s->stream);
-        e = hipStreamSynchronize(s->stream);
+        launch_pack_f64(s->x, s->y, s->z, s->m, n, packed, s->stream);
+        // Launches of about kPairsPerLaunch pairs each (wave w holds about
+        // 64 (n - 64 w) of them), each waited for, so no single kernel runs
+        // long even at tens of millions of bodies.
+        constexpr double kPairsPerLaunch = 1e12;
+        int64_t w0 = 0;
+        while (w0 < nw && e == hipSuccess) {
+            int64_t w1 = w0;
+            double pairs = 0.0;
+            while (w1 < nw && (w1 == w0 || pairs < kPairsPerLaunch)) {
+                pairs += 64.0 * (double)(n - 64 * w1);
+                ++w1;
+            }
+            launch_potential_f64(packed, n, (double)s->prm.eps2, w0, w1, part, s->stream);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+            w0 = w1;
+        }
     }
-    std::vector<double> h(nb);
-    if (e == hipSuccess) e = hipMemcpy(h.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost);
+    std::vector<double> h(nw);
+    if (e == hipSuccess) e = hipMemcpy(h.data(), part, nw * sizeof(double), hipMemcpyDeviceToHost);
     (void)hipFree(part);
     if (packed) (void)hipFree(packed);
     if (e != hipSuccess) return fail(NBG_ERR_HIP, std::string("potential: ") + hipGetErrorString(e));
     double w = 0.0;
     for (double v : h) w += v;
-    *W = w;
+    *W = -(double)s->prm.G * w;
     return NBG_OK;
 }
 

@@ -76,6 +76,7 @@ struct nbg_batch {
     MacParam *mp_tab = nullptr;
     float *bbox6 = nullptr;
     uint32_t *misc = nullptr;  // [1] err, [2] node count, [4..16) counters
+    uint32_t *bad = nullptr;   // one word per system slot: the non-finite guard's flags
     // second moments of every node (2 float4 per node), allocated when the
     // first quadrupole member joins
     float4 *quad = nullptr;
@@ -107,7 +108,7 @@ int64_t bytes_for(int64_t cap_n, int64_t cap_s) {
     b += 4 * 2 * (2 * n + 64) + 4 * (128 + n / kScanTile);    // glist (two queues), scan_ws
     b += 4 * sort_status_words(n, 64);                        // sort status
     b += (int64_t)sizeof(NodeRec) * 2 * n + 2 * 16 * 2 * n + 4 * 2 * n;  // nodes, bmin/bmax, flags
-    b += 3 * 8 * (s + 1) + 2 * 4 * s + 16 * s + (int64_t)sizeof(MacParam) * s + 24 * s + 4 * 32;
+    b += 3 * 8 * (s + 1) + 2 * 4 * s + 16 * s + (int64_t)sizeof(MacParam) * s + 24 * s + 4 * 32 + 4 * s;
     return b;
 }
 
@@ -118,7 +119,7 @@ void free_batch(nbg_batch *b) {
                                 b->kgamma, b->kparent, b->ksplit, b->krank, b->leaf_start, b->glist, b->scan_ws,
                                 b->sort_status, b->nodes, b->bmin, b->bmax, b->flags, b->sys_first, b->wave_first,
                                 b->old_first, b->sys_root, b->leaf_tab, b->sys_prm, b->mp_tab, b->bbox6, b->misc,
-                                b->quad};
+                                b->quad, b->bad};
     for (float *p : b->st) ptrs.push_back(p);
     for (float *p : b->stage) ptrs.push_back(p);
     for (void *p : ptrs)
@@ -318,6 +319,7 @@ int nbg_batch_create(nbg_ctx *ctx, int64_t max_bodies, int32_t max_systems, int3
     A(&b->sys_first, S + 1); A(&b->wave_first, S + 1); A(&b->old_first, S + 1);
     A(&b->sys_root, S); A(&b->leaf_tab, S); A(&b->sys_prm, S); A(&b->mp_tab, S); A(&b->bbox6, 6 * S);
     A(&b->misc, 32);
+    A(&b->bad, S);
     if (r == NBG_OK) {
         hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipMemsetAsync(b->misc, 0, 32 * 4, b->stream);
@@ -503,6 +505,30 @@ int nbg_batch_counts(nbg_batch *b, uint64_t *pp, uint64_t *np) {
     NBG_HIP(hipMemcpy(c, b->counts(), sizeof c, hipMemcpyDeviceToHost));
     if (pp) *pp = c[0];
     if (np) *np = c[1];
+    return NBG_OK;
+}
+
+int nbg_batch_nonfinite(nbg_batch *b, int32_t *ids, int32_t cap, int32_t *count) {
+    if (!b || !count || (cap > 0 && !ids)) return fail(NBG_ERR_ARG, "bad arguments");
+    *count = 0;
+    if (b->sys.empty()) return NBG_OK;
+    DeviceGuard g(b->ctx->device);
+    if (int r = upload_tables(b)) return r;  // bsys must describe the current membership
+    const int64_t S = (int64_t)b->sys.size();
+    hipStream_t s = b->stream;
+    launch_fill_u32(b->bad, 0u, S, s);
+    launch_forest_nonfinite(b->x(), b->y(), b->z(), b->vx(), b->vy(), b->vz(), b->bsys, b->n, b->bad, s);
+    NBG_HIP(hipGetLastError());
+    if (int r = sync_check(b)) return r;
+    std::vector<uint32_t> h((size_t)S);
+    NBG_HIP(hipMemcpy(h.data(), b->bad, (size_t)S * 4, hipMemcpyDeviceToHost));
+    int32_t k = 0;
+    for (int64_t i = 0; i < S; ++i)
+        if (h[(size_t)i]) {
+            if (k < cap) ids[k] = b->sys[(size_t)i].id;
+            ++k;
+        }
+    *count = k;
     return NBG_OK;
 }
 

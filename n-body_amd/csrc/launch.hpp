@@ -13,10 +13,10 @@ constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortBlock * kSortItems;
 constexpr int kRadix = 256;
 
-inline int64_t sort_tiles(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
-inline int sort_passes(int key_bits) { return (key_bits + 7) / 8; }
+constexpr int64_t sort_tiles(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
+constexpr int sort_passes(int key_bits) { return (key_bits + 7) / 8; }
 // Words of status memory one sort needs (per pass: tiles x 256 + counters).
-inline int64_t sort_status_words(int64_t n, int key_bits) {
+constexpr int64_t sort_status_words(int64_t n, int key_bits) {
     return 8 * kRadix + 64 + (int64_t)sort_passes(key_bits) * (sort_tiles(n) * kRadix);
 }
 
@@ -187,6 +187,11 @@ void launch_update_forest(float *x, float *y, float *z, float *vx, float *vy, fl
                           const float *ay, const float *az, const uint32_t *bsys, const float4 *sys_prm, int64_t n,
                           hipStream_t s);
 
+// Forest non-finite guard: bad[bsys[i]] = 1 for every body i whose position
+// or velocity is NaN or infinite (bad is cleared by the caller).
+void launch_forest_nonfinite(const float *x, const float *y, const float *z, const float *vx, const float *vy,
+                             const float *vz, const uint32_t *bsys, int64_t n, uint32_t *bad, hipStream_t s);
+
 // Host side of mac_lbvh_exact: the FP64 midpoint below T = RN(theta*theta)
 // and whether a tie rounds down (T has an odd significand).
 inline void mac_threshold(float theta, double *mid, int *tie) {
@@ -304,9 +309,14 @@ void launch_direct_f64(const float4 *src, int64_t ns, const float4 *tgt, int64_t
 void launch_pack(const float *x, const float *y, const float *z, const float *m, int64_t n, float4 *out,
                  hipStream_t s);
 
-// FP64 softened potential: per-block partial sums into partial[blocks].
-int64_t potential_blocks(int64_t n);
-void launch_potential_f64(const float4 *bodies, int64_t n, double G, double eps2, double *partial,
+// FP64 softened potential, each pair i < j once: wave w (64 targets) of
+// [w0, w1) writes sum_{i in wave} m_i sum_{j > i} m_j / sqrt(r^2 + eps2)
+// into partial[w]; -G times the sum of all potential_waves(n) partials is
+// the potential.  bodies: (x, y, z, m) in FP64 (launch_pack_f64).
+int64_t potential_waves(int64_t n);
+void launch_pack_f64(const float *x, const float *y, const float *z, const float *m, int64_t n, double4 *out,
+                     hipStream_t s);
+void launch_potential_f64(const double4 *bodies, int64_t n, double eps2, int64_t w0, int64_t w1, double *partial,
                           hipStream_t s);
 
 }  // namespace nbg

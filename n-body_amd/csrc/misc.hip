@@ -58,6 +58,30 @@ void launch_update_forest(float *x, float *y, float *z, float *vx, float *vy, fl
                        sys_prm, n);
 }
 
+// Non-finite guard of a forest: bad[s] becomes nonzero when a body of
+// system s holds a NaN or infinite position or velocity.  bad holds one word
+// per system and is cleared by the caller.
+__global__ __launch_bounds__(256) void k_forest_nonfinite(const float *__restrict__ x, const float *__restrict__ y,
+                                                          const float *__restrict__ z, const float *__restrict__ vx,
+                                                          const float *__restrict__ vy, const float *__restrict__ vz,
+                                                          const uint32_t *__restrict__ bsys, int64_t n,
+                                                          uint32_t *__restrict__ bad) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        // x - x is 0 for a finite x and NaN for NaN or +-inf
+        const float t = (x[i] - x[i]) + (y[i] - y[i]) + (z[i] - z[i]) + (vx[i] - vx[i]) + (vy[i] - vy[i]) +
+                        (vz[i] - vz[i]);
+        if (t != 0.0f) bad[bsys[i]] = 1u;
+    }
+}
+
+void launch_forest_nonfinite(const float *x, const float *y, const float *z, const float *vx, const float *vy,
+                             const float *vz, const uint32_t *bsys, int64_t n, uint32_t *bad, hipStream_t s) {
+    if (n <= 0) return;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_forest_nonfinite, dim3((unsigned)blocks), dim3(256), 0, s, x, y, z, vx, vy, vz, bsys, n, bad);
+}
+
 // Forest compaction after a system leaves: new position i of system s
 // takes old position old_first[s] + (i - new_first[s]), for up to 8 arrays.
 struct ArrayPtrs {
@@ -190,33 +214,65 @@ void launch_direct_f64(const float4 *src, int64_t ns, const float4 *tgt, int64_t
     hipLaunchKernelGGL(k_direct_f64, dim3(blocks), dim3(256), 0, s, src, ns, tgt, nt, G, eps2, ax, ay, az);
 }
 
-// FP64 softened potential W = -1/2 sum_i sum_{j != i} G m_i m_j / sqrt(r^2 + eps2).
-__global__ __launch_bounds__(256) void k_potential_f64(const float4 *__restrict__ b, int64_t n, double G,
-                                                       double eps2, double *__restrict__ partial) {
-    __shared__ float4 s[256];
-    __shared__ double s_red[256];
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const float4 p = i < n ? b[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    double w = 0.0;
-    for (int64_t base = 0; base < n; base += 256) {
-        __syncthreads();
-        if (base + threadIdx.x < n) s[threadIdx.x] = b[base + threadIdx.x];
-        __syncthreads();
-        const int lim = (n - base) < 256 ? (int)(n - base) : 256;
-        for (int j = 0; j < lim; ++j) {
-            if (base + j == i) continue;
-            const float4 q = s[j];
-            const double rx = (double)p.x - q.x, ry = (double)p.y - q.y, rz = (double)p.z - q.z;
-            w += (double)q.w / sqrt(rx * rx + ry * ry + rz * rz + eps2);
+// FP64 softened potential W = -sum_{i<j} G m_i m_j / sqrt(r^2 + eps2),
+// each pair once.  Wave w takes targets i = 64 w + lane and the sources
+// j > i in ascending order (wave-uniform, read with scalar loads from an
+// FP64 copy of the bodies); 1 / sqrt is the hardware estimate refined by two
+// Newton steps in FP64 (relative error near FP64 rounding).  Waves
+// [w0, w1) of a chunk write one partial each (the lanes' sums, fixed order),
+// so the host's in-order sum is deterministic.
+__device__ __forceinline__ double rsqrt_f64(double r2) {
+    double y = __builtin_amdgcn_rsq(r2);
+    const double h = 0.5 * r2;
+    y = y * __builtin_fma(-h, y * y, 1.5);
+    y = y * __builtin_fma(-h, y * y, 1.5);
+    return y;
+}
+
+__global__ __launch_bounds__(256) void k_potential_f64(const double4 *__restrict__ b, int64_t n, double eps2,
+                                                       int64_t w0, int64_t w1, double *__restrict__ partial) {
+    const int lane = lane_id();
+    const int64_t w = w0 + (int64_t)blockIdx.x * 4 + uniform_i32(threadIdx.x / kWave);
+    if (w >= w1) return;
+    const int64_t base = w * kWave, i = base + lane;
+    const bool valid = i < n;
+    const double4 p = valid ? b[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    double acc = 0.0;
+    // the wave's own tile: j in (i, min(base + 64, n))
+    const int64_t tile_end = base + kWave < n ? base + kWave : n;
+    for (int64_t j = base; j < tile_end; ++j) {
+        const double4 q = b[j];
+        const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+        const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, eps2)));
+        const double t = q.w * rsqrt_f64(r2);
+        acc += j > i ? t : 0.0;
+    }
+    int64_t j = tile_end;
+    for (; j + 4 <= n; j += 4) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double4 q = b[j + k];
+            const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+            const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, eps2)));
+            acc = __builtin_fma(q.w, rsqrt_f64(r2), acc);
         }
     }
-    s_red[threadIdx.x] = (i < n) ? -0.5 * G * (double)p.w * w : 0.0;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) s_red[threadIdx.x] += s_red[threadIdx.x + o];
-        __syncthreads();
+    for (; j < n; ++j) {
+        const double4 q = b[j];
+        const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+        const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, eps2)));
+        acc = __builtin_fma(q.w, rsqrt_f64(r2), acc);
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = s_red[0];
+    double v = valid ? p.w * acc : 0.0;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) partial[w] = v;
+}
+
+__global__ __launch_bounds__(256) void k_pack_f64(const float *__restrict__ x, const float *__restrict__ y,
+                                                  const float *__restrict__ z, const float *__restrict__ m, int64_t n,
+                                                  double4 *__restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = make_double4(x[i], y[i], z[i], m[i]);
 }
 
 __global__ __launch_bounds__(256) void k_pack(const float *__restrict__ x, const float *__restrict__ y,
@@ -234,12 +290,21 @@ void launch_pack(const float *x, const float *y, const float *z, const float *m,
     hipLaunchKernelGGL(k_pack, dim3((unsigned)blocks), dim3(256), 0, s, x, y, z, m, n, out);
 }
 
-int64_t potential_blocks(int64_t n) { return (n + 255) / 256; }
+int64_t potential_waves(int64_t n) { return (n + kWave - 1) / kWave; }
 
-void launch_potential_f64(const float4 *bodies, int64_t n, double G, double eps2, double *partial, hipStream_t s) {
+void launch_pack_f64(const float *x, const float *y, const float *z, const float *m, int64_t n, double4 *out,
+                     hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_potential_f64, dim3((unsigned)potential_blocks(n)), dim3(256), 0, s, bodies, n, G, eps2,
-                       partial);
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_pack_f64, dim3((unsigned)blocks), dim3(256), 0, s, x, y, z, m, n, out);
+}
+
+void launch_potential_f64(const double4 *bodies, int64_t n, double eps2, int64_t w0, int64_t w1, double *partial,
+                          hipStream_t s) {
+    if (n <= 0 || w1 <= w0) return;
+    const unsigned blocks = (unsigned)((w1 - w0 + 3) / 4);
+    hipLaunchKernelGGL(k_potential_f64, dim3(blocks), dim3(256), 0, s, bodies, n, eps2, w0, w1, partial);
 }
 
 }  // namespace nbg

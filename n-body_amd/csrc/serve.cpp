@@ -1,17 +1,20 @@
 // Continuous-batching server over one batched forest (host code only).
 //
 // Clients submit jobs (a system plus a step count) from any thread.  A
-// worker thread owns the forest: before every step it admits waiting jobs in
-// arrival order while the forest's body and system capacity (sized from an
-// HBM byte budget) allow, steps every admitted system once, and retires the
-// systems that have reached their step count (final state copied to host
-// memory, slot freed).  A job therefore runs alongside whatever else is on
-// the device, and its result is the same bits it gets alone.
+// worker thread owns the forest: before every step it admits waiting jobs
+// (plan_admission: arrival order with backfill, each job bypassed a bounded
+// number of times) while the forest's body and system capacity (sized from
+// an HBM byte budget) allow, steps every admitted system once, retires the
+// systems whose state became non-finite (that job alone fails) and those
+// that have reached their step count (final state copied to host memory,
+// slot freed).  A job therefore runs alongside whatever else is on the
+// device, and its result is the same bits it gets alone.
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -22,6 +25,7 @@
 #include <vector>
 
 #include "../../include/nbg.h"
+#include "admission.hpp"
 #include "host_util.hpp"
 #include "launch.hpp"
 
@@ -38,6 +42,7 @@ struct Job {
     int32_t steps = 0, remaining = 0;
     std::vector<float> st[7];   // x y z m vx vy vz in; x y z vx vy vz out
     int32_t sys = -1;           // forest handle while admitted
+    int32_t bypassed = 0;       // later jobs admitted while this one waited
     Clock::time_point t_submit, t_admit, t_done;
     bool done = false;
     int status = NBG_OK;
@@ -45,6 +50,9 @@ struct Job {
 };
 
 double secs(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+// Admissions of later jobs a waiting job lets pass before it holds the queue.
+constexpr int32_t kMaxBypass = 32;
 
 }  // namespace
 
@@ -61,6 +69,7 @@ struct nbg_server {
     int64_t next_id = 1;
     int64_t used_n = 0;
     bool stop = false;
+    int waiters = 0;  // clients inside nbg_server_wait (destroy waits for them to leave)
     std::thread worker;
     // statistics
     Clock::time_point t_start;
@@ -81,15 +90,24 @@ void worker_loop(nbg_server *S) {
             std::unique_lock<std::mutex> lk(S->mu);
             S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty() || !S->active.empty(); });
             if (S->stop) return;
-            // first come, first served: the queue head waits until it fits
-            while (!S->pending.empty()) {
-                auto &j = S->pending.front();
-                if (S->used_n + j->n > S->cap_n || (int64_t)S->active.size() + (int64_t)admit.size() >= S->cap_s)
-                    break;
-                S->used_n += j->n;
-                admit.push_back(j);
-                S->pending.pop_front();
+            std::vector<Waiting> q(S->pending.size());
+            for (size_t i = 0; i < q.size(); ++i) q[i] = Waiting{S->pending[i]->n, S->pending[i]->bypassed};
+            const std::vector<size_t> pick =
+                plan_admission(q, S->cap_n - S->used_n, (int64_t)S->cap_s - (int64_t)S->active.size(), kMaxBypass);
+            std::deque<std::shared_ptr<Job>> rest;
+            size_t k = 0;
+            for (size_t i = 0; i < S->pending.size(); ++i) {
+                auto &j = S->pending[i];
+                j->bypassed = q[i].bypassed;
+                if (k < pick.size() && pick[k] == i) {
+                    S->used_n += j->n;
+                    admit.push_back(j);
+                    ++k;
+                } else {
+                    rest.push_back(j);
+                }
             }
+            S->pending.swap(rest);
         }
         for (auto &j : admit) {
             int32_t sid = -1;
@@ -120,6 +138,11 @@ void worker_loop(nbg_server *S) {
         const auto t0 = Clock::now();
         int r = nbg_batch_step(S->batch, 1);
         if (r == NBG_OK) r = nbg_batch_synchronize(S->batch);
+        // systems whose state became NaN or infinite fail alone and leave
+        // before the next build
+        std::vector<int32_t> bad(S->active.size() + 1);
+        int32_t nbad = 0;
+        if (r == NBG_OK) r = nbg_batch_nonfinite(S->batch, bad.data(), (int32_t)bad.size(), &nbad);
         const auto t1 = Clock::now();
         std::vector<std::shared_ptr<Job>> finished, keep;
         {
@@ -135,8 +158,18 @@ void worker_loop(nbg_server *S) {
                     finished.push_back(j);
                     continue;
                 }
-                if (--j->remaining <= 0) finished.push_back(j);
-                else keep.push_back(j);
+                --j->remaining;
+                if (std::find(bad.begin(), bad.begin() + std::min<int32_t>(nbad, (int32_t)bad.size()), j->sys) !=
+                    bad.begin() + std::min<int32_t>(nbad, (int32_t)bad.size())) {
+                    j->status = NBG_ERR_DEVICE;
+                    j->error = "the system's state became non-finite after step " +
+                               std::to_string(j->steps - j->remaining);
+                    finished.push_back(j);
+                } else if (j->remaining <= 0) {
+                    finished.push_back(j);
+                } else {
+                    keep.push_back(j);
+                }
             }
             S->active = keep;
         }
@@ -198,6 +231,25 @@ int nbg_server_destroy(nbg_server *S) {
     }
     S->cv_work.notify_all();
     if (S->worker.joinable()) S->worker.join();
+    {
+        // jobs still queued or running end with an error, so that clients
+        // blocked in nbg_server_wait return before the server is freed
+        std::unique_lock<std::mutex> lk(S->mu);
+        const auto now = Clock::now();
+        auto abandon = [&](const std::shared_ptr<Job> &j) {
+            if (j->done) return;
+            j->status = NBG_ERR_STATE;
+            j->error = "the server was destroyed before the job finished";
+            j->t_done = now;
+            j->done = true;
+        };
+        for (auto &j : S->pending) abandon(j);
+        for (auto &j : S->active) abandon(j);
+        S->pending.clear();
+        S->active.clear();
+        S->cv_done.notify_all();
+        S->cv_done.wait(lk, [&] { return S->waiters == 0; });
+    }
     nbg_batch_destroy(S->batch);
     delete S;
     return NBG_OK;
@@ -219,6 +271,12 @@ int nbg_server_submit(nbg_server *S, const nbg_params *p, int64_t n, const float
     if (p->solver != NBG_LINEAR_BVH || p->morton_bits != 30)
         return fail(NBG_ERR_ARG, "served jobs run the linear-BVH solver with 30-bit keys");
     if (n > S->cap_n) return fail(NBG_ERR_BUDGET, "the job is larger than the server's HBM budget holds");
+    // a non-finite input would enter the shared build: refuse it here
+    const float *chk[7] = {x, y, z, m, vx, vy, vz};
+    for (const float *a : chk)
+        if (a)
+            for (int64_t i = 0; i < n; ++i)
+                if (!std::isfinite(a[i])) return fail(NBG_ERR_ARG, "a job's positions, masses and velocities must be finite");
     auto j = std::make_shared<Job>();
     j->p = *p;
     j->n = n;
@@ -251,9 +309,15 @@ int nbg_server_wait(nbg_server *S, int64_t job, double timeout_s, float *x, floa
         if (it == S->jobs.end()) return fail(NBG_ERR_ARG, "unknown job id");
         j = it->second;
         auto ready = [&] { return j->done; };
+        ++S->waiters;
+        bool ok = true;
         if (timeout_s < 0) S->cv_done.wait(lk, ready);
-        else if (!S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready))
-            return fail(NBG_ERR_STATE, "job not finished yet");
+        else ok = S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready);
+        --S->waiters;
+        // the last waiter to leave lets a pending nbg_server_destroy go on; S is
+        // not touched after this block
+        if (S->waiters == 0) S->cv_done.notify_all();
+        if (!ok) return fail(NBG_ERR_STATE, "job not finished yet");
         S->jobs.erase(job);
     }
     if (j->status != NBG_OK) return fail(j->status, "job failed: " + j->error);

@@ -15,8 +15,8 @@
 // (Physics/Rules/Gravity.h:199-209), leaf sums sequential from zero
 // (activeTree.h:44-52) and child results bracketed as libstdc++'s
 // std::transform_reduce does for two children, (0 + r0) + r1
-// (activeTree.h:66-76).  Each lane's pending partial sum lives in LDS, one
-// slot per tree depth (the depth is wave-uniform).
+// (activeTree.h:66-76).  Each lane's pending partial sums live in a private
+// array, one slot per tree depth (the depth is wave-uniform).
 //
 // FAST: the same opening decisions (the MAC is still exact) and so the same
 // interaction set, evaluated two siblings per step.  All control flow is on
@@ -82,17 +82,25 @@ __device__ __forceinline__ bool lbvh_accepts(float4 mc, float px, float py, floa
 }
 
 // ---------------------------------------------------------------- ORDERED
+constexpr int kOrderedWaves = 4;  // waves per workgroup of the ORDERED walks
+
 template <int MAXD, bool FOREST, bool QUAD>
-__global__ __launch_bounds__(64) void k_walk_lbvh_ordered(WalkArgs a) {
-    __shared__ int s_node[kStackLBVH];
-    __shared__ int s_meta[kStackLBVH];
-    __shared__ uint64_t s_mask[kStackLBVH];
-    __shared__ float s_save[MAXD * 3 * kWave];
+__global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_lbvh_ordered(WalkArgs a) {
+    __shared__ int s_node_all[kOrderedWaves][kStackLBVH];
+    __shared__ int s_meta_all[kOrderedWaves][kStackLBVH];
+    __shared__ uint64_t s_mask_all[kOrderedWaves][kStackLBVH];
+    // Per lane, the pending partial sum of every open depth: a private array
+    // indexed by the wave-uniform depth (scratch memory, coalesced across
+    // lanes), so the walk's occupancy is not bounded by LDS.
+    float s_save[MAXD * 3];
 
     const float4 *__restrict__ bodies = a.bodies;
     const NodeRec *__restrict__ nodes = a.nodes;
-    const int lane = lane_id();
-    const WaveTask T = wave_task<FOREST>(a, xcd_block(blockIdx.x, gridDim.x));
+    const int lane = lane_id(), wv = uniform_i32(threadIdx.x / kWave);
+    int *s_node = s_node_all[wv];
+    int *s_meta = s_meta_all[wv];
+    uint64_t *s_mask = s_mask_all[wv];
+    const WaveTask T = wave_task<FOREST>(a, (int64_t)xcd_block(blockIdx.x, gridDim.x) * kOrderedWaves + wv);
     const int64_t tgt = T.base + lane;
     const bool valid = tgt < T.end;
     const uint64_t vmask = __ballot(valid);
@@ -120,9 +128,9 @@ __global__ __launch_bounds__(64) void k_walk_lbvh_ordered(WalkArgs a) {
         if (meta & kMetaComplete) {
             if (act) {
                 const float vx = rx, vy = ry, vz = rz;
-                rx = s_save[(depth * 3 + 0) * kWave + lane] + vx;
-                ry = s_save[(depth * 3 + 1) * kWave + lane] + vy;
-                rz = s_save[(depth * 3 + 2) * kWave + lane] + vz;
+                rx = s_save[depth * 3 + 0] + vx;
+                ry = s_save[depth * 3 + 1] + vy;
+                rz = s_save[depth * 3 + 2] + vz;
             }
             continue;
         }
@@ -159,9 +167,9 @@ __global__ __launch_bounds__(64) void k_walk_lbvh_ordered(WalkArgs a) {
         } else {
             if (sp + 3 > kStackLBVH || depth >= MAXD) { fail = true; break; }
             if (op) {
-                s_save[(depth * 3 + 0) * kWave + lane] = rx;
-                s_save[(depth * 3 + 1) * kWave + lane] = ry;
-                s_save[(depth * 3 + 2) * kWave + lane] = rz;
+                s_save[depth * 3 + 0] = rx;
+                s_save[depth * 3 + 1] = ry;
+                s_save[depth * 3 + 2] = rz;
                 rx = 0.0f; ry = 0.0f; rz = 0.0f;
             }
             if (lane == 0) {
@@ -313,16 +321,16 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_forest(WalkArgs a) {
 // octree node (activeTree.h:66-76) computes init + ((r0 + r1) + (r2 + r3)),
 // then + ((r4 + r5) + (r6 + r7)), with init = 0 and r = 0 for an empty
 // child.  Per lane and per tree level the walk keeps that bracket's state
-// in LDS (R: the running init, A: the open pair sum, B: the pending third
-// term) and applies octant q's result with oct_step.
+// in a private array (R: the running init, A: the open pair sum, B: the
+// pending third term) and applies octant q's result with oct_step.
 template <int MAXL>
 __device__ __forceinline__ void oct_step(float *st, int q, bool act, float vx, float vy, float vz) {
     if (!act) return;
-    float *R = st, *A = st + 3 * kWave, *B = st + 6 * kWave;
+    float *R = st, *A = st + 3, *B = st + 6;
     const float v[3] = {vx, vy, vz};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const int o = c * kWave;
+        const int o = c;
         switch (q & 3) {
         case 0: A[o] = v[c]; break;
         case 1: A[o] = A[o] + v[c]; break;
@@ -374,16 +382,21 @@ __device__ __forceinline__ uint64_t oct_eval_ordered(const OctRec &nd, const flo
 }
 
 template <int MAXL, bool QUAD>
-__global__ __launch_bounds__(64) void k_walk_oct_ordered(WalkArgs a) {
-    __shared__ int s_node[MAXL];
-    __shared__ int s_pos[MAXL];          // next child index | next octant << 8
-    __shared__ uint64_t s_mask[MAXL];
-    __shared__ float s_st[MAXL * 9 * kWave];
+__global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_oct_ordered(WalkArgs a) {
+    __shared__ int s_node_all[kOrderedWaves][MAXL];
+    __shared__ int s_pos_all[kOrderedWaves][MAXL];  // next child index | next octant << 8
+    __shared__ uint64_t s_mask_all[kOrderedWaves][MAXL];
+    // Per lane and level, the bracket state (R, A, B; 3 floats each): a
+    // private array indexed by the wave-uniform level (scratch memory).
+    float s_st[MAXL * 9];
 
     const float4 *__restrict__ bodies = a.bodies;
     const OctRec *__restrict__ nodes = a.onodes;
-    const int lane = lane_id();
-    const int64_t tgt = xcd_block(blockIdx.x, gridDim.x) * kWave + lane;
+    const int lane = lane_id(), wv = uniform_i32(threadIdx.x / kWave);
+    int *s_node = s_node_all[wv];
+    int *s_pos = s_pos_all[wv];
+    uint64_t *s_mask = s_mask_all[wv];
+    const int64_t tgt = ((int64_t)xcd_block(blockIdx.x, gridDim.x) * kOrderedWaves + wv) * kWave + lane;
     const bool valid = tgt < a.nt;
     const uint64_t vmask = __ballot(valid);
     if (vmask == 0) return;
@@ -420,7 +433,7 @@ __global__ __launch_bounds__(64) void k_walk_oct_ordered(WalkArgs a) {
         const OctRec xr = nodes[X];
         const int cb = xr.topo.z, nch = oct_nchild(xr);
         if (cb < 0 || nch < 1 || nch > 8 || (int64_t)cb + nch > a.max_nodes) { fail = true; break; }
-        float *st = s_st + top * 9 * kWave + lane;
+        float *st = s_st + top * 9;
         int j = pos & 0xFF, q = pos >> 8;
         if (j > 0) {
             const int k = oct_octant(nodes[cb + j - 1]);
@@ -430,7 +443,7 @@ __global__ __launch_bounds__(64) void k_walk_oct_ordered(WalkArgs a) {
         }
         if (j == nch) {
             for (; q < 8; ++q) oct_step<MAXL>(st, q, act, 0.0f, 0.0f, 0.0f);
-            if (act) { rx = st[0]; ry = st[kWave]; rz = st[2 * kWave]; }
+            if (act) { rx = st[0]; ry = st[1]; rz = st[2]; }
             sp = top;
             continue;
         }
@@ -564,13 +577,14 @@ void launch_walk_oct(const WalkArgs &a, int ordered, int stats, hipStream_t s) {
         else if (stats == 1) launch_oct_fast<1>(a, s);
         else launch_oct_fast<0>(a, s);
     } else {
-        const unsigned blocks = (unsigned)((a.nt + 63) / 64);
+        constexpr int W = kOrderedWaves;
+        const unsigned blocks = (unsigned)((a.nt + 64 * W - 1) / (64 * W));
         if (a.quad) {
-            if (a.levels <= 18) hipLaunchKernelGGL((k_walk_oct_ordered<18, true>), dim3(blocks), dim3(64), 0, s, a);
-            else hipLaunchKernelGGL((k_walk_oct_ordered<kOctLevels, true>), dim3(blocks), dim3(64), 0, s, a);
+            if (a.levels <= 18) hipLaunchKernelGGL((k_walk_oct_ordered<18, true>), dim3(blocks), dim3(64 * W), 0, s, a);
+            else hipLaunchKernelGGL((k_walk_oct_ordered<kOctLevels, true>), dim3(blocks), dim3(64 * W), 0, s, a);
         } else {
-            if (a.levels <= 18) hipLaunchKernelGGL((k_walk_oct_ordered<18, false>), dim3(blocks), dim3(64), 0, s, a);
-            else hipLaunchKernelGGL((k_walk_oct_ordered<kOctLevels, false>), dim3(blocks), dim3(64), 0, s, a);
+            if (a.levels <= 18) hipLaunchKernelGGL((k_walk_oct_ordered<18, false>), dim3(blocks), dim3(64 * W), 0, s, a);
+            else hipLaunchKernelGGL((k_walk_oct_ordered<kOctLevels, false>), dim3(blocks), dim3(64 * W), 0, s, a);
         }
     }
 }
@@ -694,19 +708,21 @@ void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, int stats, 
         else if (stats == 1) launch_fast<1>(a, s);
         else launch_fast<0>(a, s);
     } else {
-        const unsigned blocks = (unsigned)walk_waves(a);
+        constexpr int W = kOrderedWaves;
+        const unsigned blocks = (unsigned)((walk_waves(a) + W - 1) / W);
+        const dim3 g(blocks), t(64 * W);
         if (a.nsys > 0 && a.quad)
-            hipLaunchKernelGGL((k_walk_lbvh_ordered<32, true, true>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((k_walk_lbvh_ordered<32, true, true>), g, t, 0, s, a);
         else if (a.nsys > 0)
-            hipLaunchKernelGGL((k_walk_lbvh_ordered<32, true, false>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((k_walk_lbvh_ordered<32, true, false>), g, t, 0, s, a);
         else if (a.quad && max_depth <= 32)
-            hipLaunchKernelGGL((k_walk_lbvh_ordered<32, false, true>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((k_walk_lbvh_ordered<32, false, true>), g, t, 0, s, a);
         else if (a.quad)
-            hipLaunchKernelGGL((k_walk_lbvh_ordered<64, false, true>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((k_walk_lbvh_ordered<64, false, true>), g, t, 0, s, a);
         else if (max_depth <= 32)
-            hipLaunchKernelGGL((k_walk_lbvh_ordered<32, false, false>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((k_walk_lbvh_ordered<32, false, false>), g, t, 0, s, a);
         else
-            hipLaunchKernelGGL((k_walk_lbvh_ordered<64, false, false>), dim3(blocks), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((k_walk_lbvh_ordered<64, false, false>), g, t, 0, s, a);
     }
 }
 

@@ -1,0 +1,71 @@
+"""Parity at the configuration bench.py measures, at its own size.
+
+Workload: Plummer sphere N = 1,048,576 (seed 1), eps2 2.5e-5, leaf 16,
+theta 0.5 (the headline) and 0.3 (the configuration's second opening
+angle); the linear BVH with 30-bit Morton keys and the octree (max_depth 32).
+
+For each, the oracle (the reference's CPU path restated, OpenMP over
+bodies) walks all 1,048,576 bodies once, and the device passes are compared
+with it over every body:
+  * ORDERED: accelerations bit-exact, interaction counts equal;
+  * FAST (the timed path): interaction counts equal (the same opening
+    decisions), and per body |da| <= 1e-5 * max(|a_ref|, a_rms).
+The strict relative error max |da| / |a_ref| and the fraction of bodies with
+|da| > 1e-5 |a_ref| are printed, with the reference benchmark's
+constitutional error (ConstitutionalGrader.h:50-64) of FAST against the
+oracle's FP32 result.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FAST_TOL = 1e-5
+N = 1 << 20
+
+
+@pytest.fixture(scope="module")
+def plummer_1m(nbg):
+    return nbg.plummer(N, seed=1)
+
+
+def _params(nbg, solver, theta, mode):
+    p = nbg.default_params()
+    p.solver = solver
+    p.theta, p.leaf_size, p.max_depth, p.morton_bits = theta, 16, 32, 30
+    p.eps2, p.dt, p.sum_mode = 2.5e-5, 5e-4, mode
+    return p
+
+
+@pytest.mark.parametrize("solver,theta", [(1, 0.5), (1, 0.3), (0, 0.5), (0, 0.3)])
+def test_benchmark_configuration(nbg, orc, gpu_ctx, plummer_1m, solver, theta):
+    b = plummer_1m
+    name = {0: "octree", 1: "linear BVH"}[solver]
+    print(f"[bench config] {name} n={N} theta={theta}: oracle walk over every body", flush=True)
+    t = orc.Tree(solver, b.x, b.y, b.z, b.m, theta=theta, leaf=16, max_depth=32, bits=30, eps2=2.5e-5)
+    ref, pp, npc = t.accel(b.x, b.y, b.z, counts=True)
+    del t
+    counts = (int(pp.sum()), int(npc.sum()))
+    mag = np.linalg.norm(ref.astype(np.float64), axis=1)
+    floor = np.sqrt(np.mean(mag ** 2))
+    for mode in (1, 0):
+        print(f"  device pass, {'FAST' if mode else 'ORDERED'}", flush=True)
+        s = nbg.System(gpu_ctx, _params(nbg, solver, theta, mode), N)
+        s.set_state(b)
+        acc = s.update_accelerations()
+        got = s.counts()
+        s.close()
+        assert got == counts, f"interaction counts {got} != oracle {counts}"
+        if mode == 0:
+            assert np.array_equal(acc, ref), "ORDERED accelerations must be bit-exact"
+            continue
+        d = np.linalg.norm(acc.astype(np.float64) - ref, axis=1)
+        floored = float(np.max(d / np.maximum(mag, floor)))
+        strict = float(np.max(d[mag > 0] / mag[mag > 0]))
+        above = float(np.mean(d > 1e-5 * mag))
+        F, dF = mag * b.m, d * b.m
+        const = float(100.0 * np.max(dF / np.minimum(F, np.sqrt(np.mean(F ** 2)))))
+        print(f"  FAST vs oracle: max floored {floored:.3g}, max strict {strict:.3g}, "
+              f"fraction above 1e-5 strict {above:.3g}, constitutional {const:.3g} %, counts {counts}", flush=True)
+        assert floored <= FAST_TOL

@@ -183,13 +183,28 @@ def test_direct_f64_gpu_matches_oracle(nbg, orc, gpu_ctx):
     assert np.allclose(d, r, rtol=1e-11, atol=0)
 
 
-def test_potential_gpu_matches_oracle(nbg, orc, gpu_ctx):
-    b = nbg.plummer(4000, seed=13)
+@pytest.mark.parametrize("n", [2, 3, 63, 65, 130, 4000])
+def test_potential_gpu_matches_oracle(nbg, orc, gpu_ctx, n):
+    """The device potential (each pair once, refined FP64 rsqrt, several
+    launches for large n) against the oracle's FP64 double sum."""
+    b = nbg.plummer(n, seed=13 + n)
     s = nbg.System(gpu_ctx, _params(nbg, 16, 63), b.n)
     s.set_state(b)
     w = s.potential_f64()
     r = orc.potential_f64(b.x, b.y, b.z, b.m, 1.0, 1e-4)
-    assert abs(w - r) <= 1e-10 * abs(r)
+    assert abs(w - r) <= 1e-12 * abs(r)
+
+
+def test_potential_gpu_closed_form(nbg, gpu_ctx):
+    """Two bodies at distance 3 with eps2 = 16: W = -G m1 m2 / 5."""
+    b = nbg.Bodies.empty(2)
+    b.x[1] = 3.0
+    b.m[:] = (2.0, 0.5)
+    p = _params(nbg, 16, 63, eps2=16.0)
+    p.G = 4.0
+    s = nbg.System(gpu_ctx, p, 2)
+    s.set_state(b)
+    assert abs(s.potential_f64() - (-4.0 * 2.0 * 0.5 / 5.0)) <= 1e-15
 
 
 def test_phase_timing(nbg, gpu_ctx):

@@ -220,13 +220,17 @@ def test_octree_solver_switch(nbg, orc, gpu_ctx):
 
 def test_octree_large_sample(nbg, orc, gpu_ctx):
     """1M-body Plummer sphere: device FAST and ORDERED passes against the
-    oracle's walk for a sample of targets in the same tree."""
+    oracle's walk for a sample of targets in the same tree; the device's
+    interaction counts over all bodies equal the oracle's (its passive-point
+    walk at every body's position meets the same nodes and bodies)."""
     n = 1 << 20
     print(f"[octree large] n={n}", flush=True)
     b = nbg.plummer(n, seed=1)
     t = orc.Tree(0, b.x, b.y, b.z, b.m, theta=0.5, leaf=16, eps2=2.5e-5)
     sel = np.random.default_rng(3).choice(n, 8192, replace=False)
     ref = t.accel(b.x[sel], b.y[sel], b.z[sel])
+    _, opp, onp = t.accel(b.x, b.y, b.z, counts=True)
+    ocounts = (int(opp.sum()), int(onp.sum()))
     mag = np.linalg.norm(ref, axis=1)
     floor = np.sqrt(np.mean(mag ** 2))
     for mode in (1, 0):
@@ -238,4 +242,5 @@ def test_octree_large_sample(nbg, orc, gpu_ctx):
             assert np.max(err / np.maximum(mag, floor)) <= FAST_TOL
         pp, npc = s.counts()
         print(f"  mode {mode}: pp={pp} np={npc}", flush=True)
+        assert (pp, npc) == ocounts
         s.close()

@@ -60,3 +60,22 @@ def test_passive_without_actives(nbg, gpu_ctx):
     s.set_state(nbg.Bodies.empty(0))
     a = s.accelerations_at(np.zeros(3), np.ones(3), np.zeros(3))
     assert a.shape == (3, 3) and not a.any()
+
+
+def test_passive_scratch_reused_across_sizes(nbg, orc, gpu_ctx):
+    """The passive-point walk keeps its device scratch between calls: target
+    counts that grow, shrink and grow again give the oracle's bits each time."""
+    b = nbg.plummer(5000, seed=23)
+    p = nbg.default_params()
+    p.solver, p.theta, p.leaf_size, p.morton_bits, p.eps2, p.sum_mode = 1, 0.5, 16, 63, 1e-4, 0
+    s = nbg.System(gpu_ctx, p, b.n)
+    s.set_state(b)
+    t = orc.Tree(1, b.x, b.y, b.z, b.m, theta=0.5, leaf=16, bits=63, eps2=1e-4)
+    for k, nt in enumerate((3000, 17, 9001, 64, 1)):
+        pts = _points(b, nt, 40 + k) if nt >= 20 else np.random.default_rng(k).standard_normal((nt, 3)).astype(
+            np.float32)
+        acc = s.accelerations_at(pts[:, 0], pts[:, 1], pts[:, 2])
+        ref, pp, npc = t.accel(pts[:, 0], pts[:, 1], pts[:, 2], counts=True)
+        assert np.array_equal(acc.view(np.uint32), ref.view(np.uint32)), f"nt={nt}"
+        assert s.counts() == (int(pp.sum()), int(npc.sum()))
+    s.close()

@@ -80,3 +80,79 @@ def test_serving_budget_admission(nbg, gpu_ctx):
     assert st["jobs_done"] == 5 and st["peak_bodies"] <= cap and st["peak_systems"] == 1
     assert max(t["queued"] for t in times) > 0.0
     srv.close()
+
+
+def test_serving_backfill(nbg, gpu_ctx):
+    """A job that does not fit waits while smaller jobs behind it are admitted
+    into the room left (backfill), and it still runs once room frees."""
+    budget = nbg.Batch.bytes_for(10000, 8)
+    srv = nbg.Server(gpu_ctx, budget, max_systems=8)
+    cap, _ = srv.capacity()
+    first = srv.submit(_params(nbg, 0.5), nbg.plummer(cap // 2 + 100, seed=40), 300)
+    time.sleep(0.2)  # the first job is running
+    big = srv.submit(_params(nbg, 0.5), nbg.plummer(cap // 2 + 100, seed=41), 2)   # does not fit beside it
+    small = srv.submit(_params(nbg, 0.5), nbg.plummer(500, seed=42), 2)            # fits beside it
+    _, t_small = srv.wait(small, timeout=120.0)
+    _, t_big = srv.wait(big, timeout=120.0)
+    srv.wait(first, timeout=120.0)
+    print(f"[serve backfill] small {t_small} big {t_big}", flush=True)
+    # the small job was admitted while the big one still waited
+    assert t_small["latency"] < t_big["queued"]
+    srv.close()
+
+
+def test_serving_nonfinite_job_fails_alone(nbg, gpu_ctx):
+    """A job whose state becomes infinite (eps2 = 0 and a pair 1e-18 apart:
+    the acceleration overflows) fails with its own error; the jobs beside it
+    finish with the bits they get alone.  A non-finite input is refused at
+    submission."""
+    budget = nbg.Batch.bytes_for(40000, 16)
+    srv = nbg.Server(gpu_ctx, budget, max_systems=16)
+    bad_in = nbg.plummer(100, seed=50)
+    bad_in.x[7] = np.nan
+    with pytest.raises(nbg.NbgError):
+        srv.submit(_params(nbg, 0.5), bad_in, 3)
+    good = [(nbg.plummer(3000 + 500 * k, seed=60 + k), 0.5, 6) for k in range(3)]
+    blow = nbg.Bodies.empty(2)
+    blow.x[1] = np.float32(1e-18)
+    blow.m[:] = 1.0
+    ids = [srv.submit(_params(nbg, th), b, st) for b, th, st in good[:2]]
+    bad = srv.submit(_params(nbg, 0.5, eps2=0.0), blow, 6)
+    ids.append(srv.submit(_params(nbg, good[2][1]), good[2][0], good[2][2]))
+    with pytest.raises(nbg.NbgError, match="non-finite"):
+        srv.wait(bad, timeout=120.0)
+    for (b, theta, steps), j in zip(good, ids):
+        out, _ = srv.wait(j, timeout=120.0)
+        ref = _alone(nbg, gpu_ctx, b, theta, steps, 1)
+        for g, r in zip((out.x, out.y, out.z, out.vx, out.vy, out.vz), (ref.x, ref.y, ref.z, ref.vx, ref.vy, ref.vz)):
+            assert np.array_equal(g, r)
+    srv.close()
+
+
+def test_serving_destroy_releases_waiters(nbg, gpu_ctx):
+    """Destroying a server with jobs queued and running ends them with an
+    error; a client blocked in wait() returns instead of hanging."""
+    import threading
+
+    budget = nbg.Batch.bytes_for(20000, 4)
+    srv = nbg.Server(gpu_ctx, budget, max_systems=4)
+    cap, _ = srv.capacity()
+    running = srv.submit(_params(nbg, 0.5), nbg.plummer(cap - 10, seed=70), 100000)
+    queued = srv.submit(_params(nbg, 0.5), nbg.plummer(cap - 10, seed=71), 5)
+    result = {}
+
+    def waiter():
+        try:
+            srv.wait(queued)
+            result["status"] = "finished"
+        except nbg.NbgError as e:
+            result["status"] = str(e)
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    time.sleep(0.5)
+    srv.close()
+    th.join(timeout=60.0)
+    assert not th.is_alive()
+    assert "destroyed" in result["status"]
+    del running
