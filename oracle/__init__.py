@@ -15,7 +15,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# ORACLE_LIB names another build of oracle.cpp (the mutation tests load
+# builds with a planted mistake through it).
+LIB_PATH = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 
 _f32p = ctypes.POINTER(ctypes.c_float)
 _f64p = ctypes.POINTER(ctypes.c_double)
@@ -57,6 +59,8 @@ def lib() -> ctypes.CDLL:
             "orc_fmm_p2l": (ctypes.c_int, [fp, ctypes.c_float, fp, fp, ctypes.c_float, ctypes.c_float, fp]),
             "orc_node_particle_quad": (ctypes.c_int, [fp, ctypes.c_float, fp, fp, ctypes.c_float, ctypes.c_float,
                                                       fp]),
+            "orc_fmm_recommend": (ctypes.c_int, [fp, fp, fp, ctypes.c_float]),
+            "orc_fmm_particle_far": (ctypes.c_int, [fp, fp, ctypes.c_float]),
             "orc_tree_order": (ctypes.c_int, [_vp, _u32p, _u64p]),
             "orc_tree_bbox": (ctypes.c_int, [_vp, fp]),
             "orc_tree_accel": (ctypes.c_int, [_vp, ctypes.c_int64, fp, fp, fp, fp, fp, fp, _u64p, _u64p,
@@ -267,6 +271,18 @@ def fmm_p2l(s3, m, c3, p3, G, eps2):
     out = np.zeros(3, np.float32)
     lib().orc_fmm_p2l(_p(_f32(s3)), m, _p(_f32(c3)), _p(_f32(p3)), G, eps2, _p(out))
     return out
+
+
+def fmm_recommend(a_cell4, a_com3, p_cell4, theta):
+    """The implicit FMM's node-node criterion: 0 approximate, else
+    1 (descend the active node) | 2 (descend the passive node).  Cells are
+    (centre x, y, z, side)."""
+    return int(lib().orc_fmm_recommend(_p(_f32(a_cell4)), _p(_f32(a_com3)), _p(_f32(p_cell4)), theta))
+
+
+def fmm_particle_far(x3, p_cell4, theta):
+    """The implicit FMM's particle-node criterion (body far from a passive cell)."""
+    return bool(lib().orc_fmm_particle_far(_p(_f32(x3)), _p(_f32(p_cell4)), theta))
 
 
 def boxes_intersect(a6, b6):

@@ -20,7 +20,12 @@
 // Parity pins (see tests/test_oracle.py): reference KATs gravity.cpp:13-26,
 // centerOfMassSummary.cpp:12-47, boundingBox.cpp:12-32; closed forms for
 // Morton interleave, bbox, COM, integrator; brute force for sort/topology;
-// theta->0 walk == direct sum over leaf order; FP64 direct-sum force error.
+// the octree child-centre recurrence and cell containment; an independent
+// float32 restatement of the COM summation order (both trees); truth tables
+// of the implicit FMM's two criteria; theta->0 walk == direct sum over leaf
+// order; FP64 direct-sum force error.  tests/test_oracle_mutations.py plants
+// a mistake in the octree, COM and criterion functions and checks that the
+// pins catch it.
 
 #include <algorithm>
 #include <cfloat>
@@ -33,6 +38,14 @@
 
 #ifdef _OPENMP
 #include <omp.h>
+#endif
+
+// Test switch: a build with -DORC_MUTATE=k (k = 1..7) plants one deliberate
+// mistake, beside the line it changes, so that the tests can show that
+// their pins catch it (tests/test_oracle_mutations.py); the default build
+// plants none.
+#ifndef ORC_MUTATE
+#define ORC_MUTATE 0
 #endif
 
 namespace {
@@ -145,22 +158,15 @@ inline V3 node_particle_quad(V3 c, float M, const Quad &Q, V3 p, float G, float 
         else B = (a == b ? Rv[d] : Rv[a]) + 0.0f * 3.0f;
         D[u] = cp * g3 + B * g2;
     }
-    auto u3 = [](int i, int j, int k) {
-        int t[3] = {i, j, k};
-        std::sort(t, t + 3);
-        for (int u = 0; u < 10; ++u)
-            if (kDims[u][0] == t[0] && kDims[u][1] == t[1] && kDims[u][2] == t[2]) return u;
-        return -1;
-    };
-    auto u2 = [](int j, int k) {
-        if (j > k) std::swap(j, k);
-        static const int kIdx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-        return kIdx[j][k];
-    };
+    // storage index of D_ijk (the sorted triple's place in kDims) and of Q_jk
+    static constexpr int kU3[3][3][3] = {{{0, 1, 2}, {1, 3, 4}, {2, 4, 5}},
+                                         {{1, 3, 4}, {3, 6, 7}, {4, 7, 8}},
+                                         {{2, 4, 5}, {4, 7, 8}, {5, 8, 9}}};
+    static constexpr int kU2[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
     float h[3] = {0.0f, 0.0f, 0.0f};
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
-            for (int k = 0; k < 3; ++k) h[i] += D[u3(i, j, k)] * Q.q[u2(j, k)];
+            for (int k = 0; k < 3; ++k) h[i] += D[kU3[i][j][k]] * Q.q[kU2[j][k]];
     dphi = v_add(dphi, V3{h[0] / 2.0f, h[1] / 2.0f, h[2] / 2.0f});
     return V3{G * dphi.x, G * dphi.y, G * dphi.z};
 }
@@ -242,7 +248,11 @@ struct LBVH {
         float M = 0.0f;
         V3 C = kZero;
         V3 mn{FLT_MAX, FLT_MAX, FLT_MAX}, mx{-FLT_MAX, -FLT_MAX, -FLT_MAX};
+#if ORC_MUTATE == 2
+        for (int64_t i = nd.first + nd.count - 1; i >= nd.first; --i) {
+#else
         for (int64_t i = nd.first; i < nd.first + nd.count; ++i) {
+#endif
             float m = mass[i];
             M += m;
             C = v_add(C, v_mul(pos[i], m));
@@ -424,7 +434,11 @@ struct Octree {
         Sp s_zyx[8] = {{beg, mx00}, {mx00, my0}, {my0, mx01}, {mx01, mz},
                        {mz, mx10}, {mx10, my1}, {my1, mx11}, {mx11, end}};
         float cs = nd.side / 2.0f;
+#if ORC_MUTATE == 1
+        float off = cs;
+#else
         float off = cs / 2.0f;
+#endif
         int32_t base = (int32_t)nodes.size();
         for (int k = 0; k < 8; ++k) {
             int xb = (k >> 2) & 1, yb = (k >> 1) & 1, zb = k & 1;   // child index 4x+2y+z
@@ -454,7 +468,11 @@ struct Octree {
                 C = v_add(C, v_mul(P[e], Mv[e]));
             }
         } else {
+#if ORC_MUTATE == 7
+            for (int k = 0; k < 7; ++k) {
+#else
             for (int k = 0; k < 8; ++k) {
+#endif
                 const ONode &ch = nodes[nd.child[k]];
                 M += ch.mass;
                 C = v_add(C, v_mul(ch.com, ch.mass));
@@ -619,6 +637,62 @@ struct FMMShared {
     }
 };
 
+// SideLengthOverDistance, node-node (SideLengthOverDistance.h:33-56):
+// 0 = Approximate, else DescendActiveNode (1) | DescendPassiveNode (2)
+// (DescentCriterionType.h:21-27).  The distance runs from the active node's
+// centre of mass to the passive node's centre; exclusion regions are
+// [c - side, c + side] (DescentCriterionType.h:81-85) with closed-interval
+// overlap (BoundingBox.h:77-82).
+int fmm_recommend(const ONode &A, const ONode &Pn, float theta) {
+    const float sa2 = A.side * A.side, sp2 = Pn.side * Pn.side;
+#if ORC_MUTATE == 6
+    const float d2 = v_len2(v_sub(Pn.center, A.center));
+#else
+    const float d2 = v_len2(v_sub(Pn.center, A.com));
+#endif
+    const float big = sa2 < sp2 ? sp2 : sa2;
+    if (((4.0f * big) / d2) < (theta * theta)) {
+#if ORC_MUTATE == 3
+        const float ps = Pn.side / 2.0f, as = A.side / 2.0f;
+#else
+        const float ps = Pn.side, as = A.side;
+#endif
+        const V3 pl{Pn.center.x - ps, Pn.center.y - ps, Pn.center.z - ps};
+        const V3 ph{Pn.center.x + ps, Pn.center.y + ps, Pn.center.z + ps};
+        const V3 al{A.center.x - as, A.center.y - as, A.center.z - as};
+        const V3 ah{A.center.x + as, A.center.y + as, A.center.z + as};
+        const bool meet = std::min(ph.x, ah.x) >= std::max(pl.x, al.x) && std::min(ph.y, ah.y) >= std::max(pl.y, al.y) &&
+                          std::min(ph.z, ah.z) >= std::max(pl.z, al.z);
+        if (!meet) return 0;
+    }
+#if ORC_MUTATE == 4
+    int r = (sp2 <= sa2 * 4.0f) ? 2 : 0;
+    r |= (sa2 <= sp2 * 4.0f) ? 1 : 0;
+#else
+    int r = (sa2 <= sp2 * 4.0f) ? 2 : 0;
+    r |= (sp2 <= sa2 * 4.0f) ? 1 : 0;
+#endif
+    return r;
+}
+
+// SideLengthOverDistance, particle-node (SideLengthOverDistance.h:25-29):
+// 2 side / |x - c| < theta and x outside the cell [c - side/2, c + side/2)
+// (OctreeNode::boundingBox, Trees/Octree.h:92-98; half-open contains,
+// Physics/BoundingBox.h:61-64).
+bool fmm_particle_far(V3 x, const ONode &Pn, float theta) {
+    const float dist = std::sqrt(v_len2(v_sub(x, Pn.center)));
+    if (!(((2.0f * Pn.side) / dist) < theta)) return false;
+    const float h = Pn.side / 2.0f;
+#if ORC_MUTATE == 5
+    const bool in = Pn.center.x - h <= x.x && Pn.center.y - h <= x.y && Pn.center.z - h <= x.z &&
+                    x.x <= Pn.center.x + h && x.y <= Pn.center.y + h && x.z <= Pn.center.z + h;
+#else
+    const bool in = Pn.center.x - h <= x.x && Pn.center.y - h <= x.y && Pn.center.z - h <= x.z &&
+                    x.x < Pn.center.x + h && x.y < Pn.center.y + h && x.z < Pn.center.z + h;
+#endif
+    return !in;
+}
+
 struct FMM {
     const Octree &T;
     float theta;
@@ -635,40 +709,8 @@ struct FMM {
     const ONode &nd(int32_t i) const { return T.nodes[i]; }
     V3 pos(uint32_t e) const { return T.P[e]; }
 
-    // SideLengthOverDistance, node-node (SideLengthOverDistance.h:33-56):
-    // 0 = Approximate, else DescendActiveNode (1) | DescendPassiveNode (2).
-    // Exclusion regions are [c - side, c + side] (DescentCriterionType.h:81-85),
-    // closed-interval overlap (BoundingBox.h:77-82).
-    int recommend(int32_t a, int32_t p) const {
-        const ONode &A = nd(a), &Pn = nd(p);
-        const float sa2 = A.side * A.side, sp2 = Pn.side * Pn.side;
-        const float d2 = v_len2(v_sub(Pn.center, A.com));
-        const float big = sa2 < sp2 ? sp2 : sa2;
-        if (((4.0f * big) / d2) < (theta * theta)) {
-            const V3 pl{Pn.center.x - Pn.side, Pn.center.y - Pn.side, Pn.center.z - Pn.side};
-            const V3 ph{Pn.center.x + Pn.side, Pn.center.y + Pn.side, Pn.center.z + Pn.side};
-            const V3 al{A.center.x - A.side, A.center.y - A.side, A.center.z - A.side};
-            const V3 ah{A.center.x + A.side, A.center.y + A.side, A.center.z + A.side};
-            const bool meet = std::min(ph.x, ah.x) >= std::max(pl.x, al.x) && std::min(ph.y, ah.y) >= std::max(pl.y, al.y) &&
-                              std::min(ph.z, ah.z) >= std::max(pl.z, al.z);
-            if (!meet) return 0;
-        }
-        int r = (sa2 <= sp2 * 4.0f) ? 2 : 0;
-        r |= (sp2 <= sa2 * 4.0f) ? 1 : 0;
-        return r;
-    }
-
-    // SideLengthOverDistance, particle-node (SideLengthOverDistance.h:25-29):
-    // 2 side / |x - c| < theta and x outside the cell [c - side/2, c + side/2).
-    bool particle_far(V3 x, int32_t p) const {
-        const ONode &Pn = nd(p);
-        const float dist = std::sqrt(v_len2(v_sub(x, Pn.center)));
-        if (!(((2.0f * Pn.side) / dist) < theta)) return false;
-        const float h = Pn.side / 2.0f;
-        const bool in = Pn.center.x - h <= x.x && Pn.center.y - h <= x.y && Pn.center.z - h <= x.z &&
-                        x.x < Pn.center.x + h && x.y < Pn.center.y + h && x.z < Pn.center.z + h;
-        return !in;
-    }
+    int recommend(int32_t a, int32_t p) const { return fmm_recommend(nd(a), nd(p), theta); }
+    bool particle_far(V3 x, int32_t p) const { return fmm_particle_far(x, nd(p), theta); }
 
     // Gravity::particleNode into a MultipoleAccelerationSummary<2>
     // (Physics/Rules/Gravity.h:281-304): R = c - s, r = sqrt(R.R + eps2),
@@ -921,6 +963,26 @@ uint64_t orc_expand_bits_magic(uint64_t i, int bits) {
     i = (i | i << 4) & 0x10c30c30c30c30c3ull;
     i = (i | i << 2) & 0x1249249249249249ull;
     return i;
+}
+
+// The implicit FMM's two criterion forms on explicit cells, for the truth
+// tables of tests/test_oracle.py: cell = centre xyz, side; com = the active
+// node's centre of mass.
+int orc_fmm_recommend(const float *a_cell4, const float *a_com3, const float *p_cell4, float theta) {
+    ONode A, P;
+    A.center = V3{a_cell4[0], a_cell4[1], a_cell4[2]};
+    A.side = a_cell4[3];
+    A.com = V3{a_com3[0], a_com3[1], a_com3[2]};
+    P.center = V3{p_cell4[0], p_cell4[1], p_cell4[2]};
+    P.side = p_cell4[3];
+    return fmm_recommend(A, P, theta);
+}
+
+int orc_fmm_particle_far(const float *x3, const float *p_cell4, float theta) {
+    ONode P;
+    P.center = V3{p_cell4[0], p_cell4[1], p_cell4[2]};
+    P.side = p_cell4[3];
+    return fmm_particle_far(V3{x3[0], x3[1], x3[2]}, P, theta) ? 1 : 0;
 }
 
 uint64_t orc_expand_bits(uint64_t v, int bits) { return spread_bits(v, bits == 30 ? 10 : 21); }

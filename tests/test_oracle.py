@@ -10,7 +10,7 @@ import math
 import numpy as np
 import pytest
 
-from treecheck import lbvh_check, octree_check, summaries_check
+from treecheck import com_restatement, lbvh_check, octree_check, same_bits, summaries_check
 
 
 def _rand_bodies(n, seed, spread=1.0):
@@ -652,3 +652,85 @@ def test_fmm_threads_do_not_change_results(orc, nbg):
     a1, c1 = t.fmm_accel(counts=True, threads=1)
     a4, c4 = t.fmm_accel(counts=True, threads=4)
     assert c1 == c4 and np.array_equal(a1, a4)
+
+
+# ------------------------------------------------------------------ pins added for
+# the octree cells, the COM summation order and the implicit FMM's criteria
+
+@pytest.mark.parametrize("solver,bits", [(1, 63), (1, 30), (0, 63)])
+@pytest.mark.parametrize("n,leaf", [(1, 16), (9, 1), (1000, 16), (5000, 8), (20000, 64)])
+def test_com_summation_order_restated(orc, nbg, solver, bits, n, leaf):
+    """Every node's COM and mass equal, bit for bit, an independent numpy
+    float32 restatement of CenterOfMassSummary's leaf and child loops
+    (CenterOfMassSummary.h:30-55) in the tree's own order."""
+    b = nbg.plummer(n, seed=100 + n)
+    if n > 1:
+        b.m[::5] = np.float32(0.0)  # massless bodies inside leaves
+        b.m[1::5] *= np.float32(3.0)
+    t = orc.Tree(solver, b.x, b.y, b.z, b.m, leaf=leaf, bits=bits)
+    nd = t.nodes()
+    _, perm = t.order()
+    pos = np.stack([b.x, b.y, b.z], 1)[perm]
+    want = com_restatement(nd["first"], nd["count"], nd["leaf"], nd["parent"], nd["depth"], pos, b.m[perm],
+                           solver == 0)
+    assert same_bits(nd["com4"], want)
+
+
+def _cell(c, side):
+    return np.array([c[0], c[1], c[2], side], np.float32)
+
+
+def test_fmm_recommend_truth_table(orc):
+    """SideLengthOverDistance's node-node form (SideLengthOverDistance.h:33-56):
+    approximate (0) iff 4 max(s_a^2, s_p^2) / |com_a - c_p|^2 < theta^2 and the
+    exclusion regions [c - s, c + s] do not meet (closed intervals); else
+    descend the passive node (2) when s_a^2 <= 4 s_p^2 and the active node (1)
+    when s_p^2 <= 4 s_a^2."""
+    f32 = np.float32
+    z3 = np.zeros(3, f32)
+    up = np.nextafter(f32(4.0), f32(5.0))
+    # far apart, equal sides: approximate
+    assert orc.fmm_recommend(_cell(z3, 1.0), z3, _cell((10, 0, 0), 1.0), 0.5) == 0
+    # quotient exactly theta^2 (4 * 1 / 16 = 0.25): the test is strict, so descend both
+    assert orc.fmm_recommend(_cell(z3, 1.0), z3, _cell((4, 0, 0), 1.0), 0.5) == 3
+    assert orc.fmm_recommend(_cell(z3, 1.0), z3, _cell((up, 0, 0), 1.0), 0.5) == 0
+    # touching exclusion regions ([-1, 1] and [1, 3]) meet; a gap of one ulp does not
+    assert orc.fmm_recommend(_cell(z3, 1.0), z3, _cell((2, 0, 0), 1.0), 1.5) == 3
+    two_up = np.nextafter(f32(2.0), f32(3.0))
+    assert orc.fmm_recommend(_cell(z3, 1.0), z3, _cell((two_up, 0, 0), 1.0), 1.5) == 0
+    # the regions are [c - s, c + s], not the cells [c - s/2, c + s/2]: cells 3 apart
+    # with sides 2 touch as regions ([-2, 2] and [1, 5]) though the cells do not
+    assert orc.fmm_recommend(_cell(z3, 2.0), z3, _cell((3, 0, 0), 2.0), 4.0) == 3
+    # side ratios (near pair, no approximation): exactly 2 descends both,
+    # beyond 2 only the larger node, 4 only the larger node
+    near = _cell((1, 0, 0), 1.0)
+    assert orc.fmm_recommend(_cell(z3, 2.0), z3, near, 0.5) == 3
+    assert orc.fmm_recommend(_cell(z3, np.nextafter(f32(2.0), f32(3.0))), z3, near, 0.5) == 1
+    assert orc.fmm_recommend(_cell(z3, 4.0), z3, near, 0.5) == 1
+    assert orc.fmm_recommend(_cell(z3, 0.5), z3, near, 0.5) == 3
+    assert orc.fmm_recommend(_cell(z3, 0.25), z3, near, 0.5) == 2
+    # the distance runs from the active node's centre of mass, not its centre
+    com = np.array([-8.0, 0, 0], np.float32)
+    assert orc.fmm_recommend(_cell(z3, 1.0), z3, _cell((4, 0, 0), 1.0), 0.5) == 3
+    assert orc.fmm_recommend(_cell(z3, 1.0), com, _cell((4, 0, 0), 1.0), 0.5) == 0
+    # and to the passive node's centre (the other node's COM is not an input)
+    assert orc.fmm_recommend(_cell(z3, 1.0), z3, _cell((0, 0, 12), 1.0), 0.5) == 0
+
+
+def test_fmm_particle_far_truth_table(orc):
+    """SideLengthOverDistance's particle-node form (SideLengthOverDistance.h:25-29):
+    2 s / |x - c| < theta and x outside the half-open cell [c - s/2, c + s/2)
+    (OctreeNode::boundingBox, Octree.h:92-98; BoundingBox::contains :61-64)."""
+    f32 = np.float32
+    cell = _cell((0, 0, 0), 1.0)
+    assert orc.fmm_particle_far((10, 0, 0), cell, 0.5)
+    # quotient exactly theta (2 / 4 = 0.5) is not below it
+    assert not orc.fmm_particle_far((4, 0, 0), cell, 0.5)
+    assert orc.fmm_particle_far((np.nextafter(f32(4.0), f32(5.0)), 0, 0), cell, 0.5)
+    # with a large theta only the cell decides: closed below, open above
+    assert not orc.fmm_particle_far((-0.5, 0, 0), cell, 100.0)
+    assert orc.fmm_particle_far((0.5, 0, 0), cell, 100.0)
+    assert not orc.fmm_particle_far((np.nextafter(f32(0.5), f32(0.0)), 0, 0), cell, 100.0)
+    assert orc.fmm_particle_far((0, 0, np.nextafter(f32(-0.5), f32(-1.0))), cell, 100.0)
+    # the cell is [c - s/2, c + s/2): a point at 0.75 of the side is outside
+    assert orc.fmm_particle_far((0.75, 0, 0), cell, 100.0)

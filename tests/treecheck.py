@@ -9,7 +9,9 @@ define the reference's trees:
   * Octree (Trees/Octree.h:33-61,100-147,185-205): levels 0 and 1 always
     split; deeper nodes split iff > L bodies and depth < 2 + maxDepth; each
     body of a child lies on the child's side of every parent centre plane
-    (p[d] < c[d] for the lower half).
+    (p[d] < c[d] for the lower half); child k's centre is the parent's plus
+    (+-(side/2)/2) per axis in FP32 with k = 4x + 2y + z; every body lies in
+    its cell's box [c - side/2, c + side/2).
 """
 
 from __future__ import annotations
@@ -108,6 +110,34 @@ def octree_check(pos: np.ndarray, perm: np.ndarray, nodes: dict, L: int, max_dep
                 lower = cc[dim] < pc[dim]
                 side = body[:, dim] < pc[dim]
                 assert np.all(side == lower), f"node {i}: body on the wrong side of its parent's plane"
+    # Child centres: OctreeNode::split (Trees/Octree.h:113-146) places child
+    # k (= 4x + 2y + z, the children array order) at the parent centre plus
+    # (+-o, +-o, +-o), o = (side / 2) / 2 in FP32, + on the axes whose bit is
+    # set.  The children of a node appear in k order (pre-order node ids).
+    f32 = np.float32
+    kids = {}
+    for i in range(1, len(depth)):
+        kids.setdefault(int(parent[i]), []).append(i)
+    for pnode, ch in kids.items():
+        o = f32(f32(geo[pnode, 3] / f32(2)) / f32(2))
+        pc = geo[pnode, :3].astype(f32)
+        for k, c in enumerate(ch):
+            bits = ((k >> 2) & 1, (k >> 1) & 1, k & 1)
+            want = np.array([f32(pc[d] + (o if bits[d] else -o)) for d in range(3)], f32)
+            assert np.array_equal(geo[c, :3], want), f"centre of child {k} of node {pnode}"
+    # Containment: every body of a cell lies in the cell [c - s/2, c + s/2)
+    # (OctreeNode::boundingBox, Octree.h:92-98), up to the rounding of the
+    # FP32 centres (a body on the parent's plane side can sit half an ulp of
+    # the centre outside its child's nominal box).
+    for i in range(len(depth)):
+        c = int(count[i])
+        if c == 0:
+            continue
+        body = pos[perm[first[i]:first[i] + c]].astype(np.float64)
+        ctr = geo[i, :3].astype(np.float64)
+        h = float(geo[i, 3]) / 2.0
+        slack = np.spacing(np.abs(geo[i, :3]).astype(f32) + f32(geo[i, 3])).astype(np.float64)
+        assert np.all(body >= ctr - h - slack) and np.all(body < ctr + h + slack), f"body outside cell {i}"
     # a parent's count is the sum of its children's
     sums = np.bincount(parent[1:], weights=count[1:], minlength=len(depth))
     inner = ~leaf.astype(bool)
@@ -116,3 +146,56 @@ def octree_check(pos: np.ndarray, perm: np.ndarray, nodes: dict, L: int, max_dep
     lc = count[leaf.astype(bool)].sum()
     assert lc == n, "leaves must hold every body once"
     assert sorted(perm.tolist()) == list(range(n))
+
+
+def com_restatement(first, count, leaf, parent, depth, pos_sorted, mass_sorted, octree):
+    """CenterOfMassSummary::summarize (Physics/Summaries/CenterOfMassSummary.h:
+    30-55) restated in numpy float32, independently of the oracle: a leaf
+    sums M += m and C = C + m p over its bodies in order, an internal node
+    sums M += M_c and C = C + com_c M_c over its children in order (pre-order
+    ids), then com = C / M.  Octree nodes that are never summarized keep
+    the default zero COM and mass: empty nodes below level 1 (NodeBase::refine
+    returns on them, Trees/NodeBase.h:57-75); levels 0 and 1 are always split
+    and summarized (Trees/Tree.h:172-216), so an empty one gets 0 / 0.
+    Returns com4 (x, y, z, M) per node."""
+    f32 = np.float32
+    k = len(first)
+    M = np.zeros(k, f32)
+    C = np.zeros((k, 3), f32)
+    summarized = np.ones(k, bool)
+    if octree:
+        summarized = (count > 0) | (depth <= 1)
+    lf = np.flatnonzero(leaf.astype(bool) & summarized & (count > 0))
+    cnt = count[lf]
+    for j in range(int(cnt.max()) if len(lf) else 0):
+        a = lf[cnt > j]
+        idx = first[a] + j
+        m = mass_sorted[idx].astype(f32)
+        M[a] = M[a] + m
+        C[a] = C[a] + pos_sorted[idx].astype(f32) * m[:, None]
+    kids = {}
+    for i in range(1, k):
+        kids.setdefault(int(parent[i]), []).append(i)
+    inner = [i for i in range(k) if not leaf[i] and summarized[i]]
+    com = np.zeros((k, 3), f32)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        done = np.zeros(k, bool)
+        for i in lf:
+            com[i] = C[i] / M[i]
+            done[i] = True
+        for i in sorted(inner, key=lambda v: -int(depth[v])):  # children before parents
+            for c in kids.get(i, []):
+                M[i] = M[i] + M[c]
+                C[i] = C[i] + com[c] * M[c]
+            com[i] = C[i] / M[i]
+            done[i] = True
+    out = np.zeros((k, 4), f32)
+    out[:, :3] = com
+    out[:, 3] = M
+    return out
+
+
+def same_bits(a, b):
+    """Element-wise equality of float arrays with NaN equal to NaN."""
+    a, b = np.asarray(a), np.asarray(b)
+    return bool(np.all((a == b) | (np.isnan(a) & np.isnan(b))))
