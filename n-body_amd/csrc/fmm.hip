@@ -485,10 +485,7 @@ __device__ void fmm_leaf_group(const FmmArgs &a, int P, const uint32_t *alist, u
 
     // pass B: walk each active node once for the lanes whose leaf found it close
     FastWave W;
-    W.px = p.x; W.py = p.y; W.pz = p.z;
-    W.rx = 0.0f; W.ry = 0.0f; W.rz = 0.0f;
-    W.pp = 0; W.np = 0;
-    W.visits = 0; W.lanes = 0; W.leaf_visits = 0; W.leaf_lanes = 0;
+    fast_init(W, p.x, p.y, p.z);
     W.G = a.G;
     const uint32_t mybit = has ? 1u << myleaf : 0u;
     // the collected entries (all entries again when the buffer overflowed)
@@ -562,9 +559,8 @@ __device__ void fmm_leaf_group(const FmmArgs &a, int P, const uint32_t *alist, u
         fmm_at(F, p.x - cl.x, p.y - cl.y, p.z - cl.z, lx, ly, lz);
         const uint32_t o = a.perm[me];
         if ((int64_t)o < a.n) {
-            a.ax[o] = (W.rx + lx) * a.G;
-            a.ay[o] = (W.ry + ly) * a.G;
-            a.az[o] = (W.rz + lz) * a.G;
+            const float3 r = fast_result(W, a.G, lx, ly, lz);
+            a.ax[o] = r.x; a.ay[o] = r.y; a.az[o] = r.z;
         }
     }
     wave_mem_sync();  // the next item rewrites the group's LDS
@@ -861,10 +857,7 @@ __global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
             const float4 p = a.bodies[has ? me : pr.topo.x];
             const uint64_t lanes = __ballot(has);
             FastWave W;
-            W.px = p.x; W.py = p.y; W.pz = p.z;
-            W.rx = 0.0f; W.ry = 0.0f; W.rz = 0.0f;
-            W.pp = 0; W.np = 0;
-            W.visits = 0; W.lanes = 0; W.leaf_visits = 0; W.leaf_lanes = 0;
+            fast_init(W, p.x, p.y, p.z);
             W.G = a.G;
             if (leaf) {
                 for (int j = 0; j < nc; ++j)
@@ -881,9 +874,8 @@ __global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
                 fmm_at(F, p.x - cP.x, p.y - cP.y, p.z - cP.z, lx, ly, lz);
                 const uint32_t o = a.perm[me];
                 if ((int64_t)o < a.n) {
-                    a.ax[o] = (W.rx + lx) * a.G;
-                    a.ay[o] = (W.ry + ly) * a.G;
-                    a.az[o] = (W.rz + lz) * a.G;
+                    const float3 r = fast_result(W, a.G, lx, ly, lz);
+                    a.ax[o] = r.x; a.ay[o] = r.y; a.az[o] = r.z;
                 }
             }
         }

@@ -240,10 +240,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_forest(WalkArgs a) {
 
     FastWave W;
     const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    W.px = me.x; W.py = me.y; W.pz = me.z;
-    W.rx = 0.0f; W.ry = 0.0f; W.rz = 0.0f;
-    W.pp = 0; W.np = 0;
-    W.visits = 0; W.lanes = 0; W.leaf_visits = 0; W.leaf_lanes = 0;
+    fast_init(W, me.x, me.y, me.z);
     // A VGPR copy of G (non-volatile asm: no memory effects, so the node and
     // body loads below stay scalar).
     W.G = T.G;
@@ -302,7 +299,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_forest(WalkArgs a) {
     if (fail && lane == 0) atomicOr(a.err, 2u);
     if (valid) {
         const uint32_t o = a.perm[tgt];
-        if ((int64_t)o < a.nt) { a.ax[o] = W.rx * W.G; a.ay[o] = W.ry * W.G; a.az[o] = W.rz * W.G; }
+        if ((int64_t)o < a.nt) {
+            const float3 r = fast_result(W, W.G);
+            a.ax[o] = r.x; a.ay[o] = r.y; a.az[o] = r.z;
+        }
     }
     if (lane == 0) {
         atomicAdd(&a.counts[0], (unsigned long long)W.pp);
@@ -485,10 +485,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
 
     FastWave W;
     const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    W.px = me.x; W.py = me.y; W.pz = me.z;
-    W.rx = 0.0f; W.ry = 0.0f; W.rz = 0.0f;
-    W.pp = 0; W.np = 0;
-    W.visits = 0; W.lanes = 0; W.leaf_visits = 0; W.leaf_lanes = 0;
+    fast_init(W, me.x, me.y, me.z);
     W.G = a.G;
     const float eps2 = a.eps2;
     const int64_t n = a.n;
@@ -540,7 +537,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
     if (fail && lane == 0) atomicOr(a.err, 2u);
     if (valid) {
         const uint32_t o = a.perm[tgt];
-        if ((int64_t)o < a.nt) { a.ax[o] = W.rx * W.G; a.ay[o] = W.ry * W.G; a.az[o] = W.rz * W.G; }
+        if ((int64_t)o < a.nt) {
+            const float3 r = fast_result(W, W.G);
+            a.ax[o] = r.x; a.ay[o] = r.y; a.az[o] = r.z;
+        }
     }
     if (lane == 0) {
         atomicAdd(&a.counts[0], (unsigned long long)W.pp);
@@ -605,10 +605,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
 
     FastWave W;
     const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    W.px = me.x; W.py = me.y; W.pz = me.z;
-    W.rx = 0.0f; W.ry = 0.0f; W.rz = 0.0f;
-    W.pp = 0; W.np = 0;
-    W.visits = 0; W.lanes = 0; W.leaf_visits = 0; W.leaf_lanes = 0;
+    fast_init(W, me.x, me.y, me.z);
     // A VGPR copy of G (non-volatile asm: no memory effects, so the node and
     // body loads below stay scalar).
     asm("v_mov_b32 %0, %1" : "=v"(W.G) : "s"(a.G));
@@ -659,7 +656,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     if (fail && lane == 0) atomicOr(a.err, 2u);
     if (valid) {
         const uint32_t o = a.perm[tgt];
-        if ((int64_t)o < a.nt) { a.ax[o] = W.rx * W.G; a.ay[o] = W.ry * W.G; a.az[o] = W.rz * W.G; }
+        if ((int64_t)o < a.nt) {
+            const float3 r = fast_result(W, W.G);
+            a.ax[o] = r.x; a.ay[o] = r.y; a.az[o] = r.z;
+        }
     }
     if (lane == 0) {
         atomicAdd(&a.counts[0], (unsigned long long)W.pp);

@@ -7,13 +7,44 @@
 
 namespace nbg {
 
+// A lane's acceleration is summed in two stages: interactions accumulate
+// into an FP32 partial (rx, ry, rz) that is added into an FP64 running sum
+// (sx, sy, sz) after every leaf, so no FP32 sum runs over more than one
+// leaf and the node interactions met since the previous one.  A single FP32
+// accumulator over the whole walk (thousands of terms in traversal order)
+// drifts from the reference's pairwise tree-ordered sum by more than 1e-5
+// of |a| at theta 0.3; the two-stage sum stays within FP32 rounding of the
+// exact value of the interaction set.
 struct FastWave {
     float px, py, pz;          // this lane's target
-    float rx, ry, rz;          // this lane's acceleration
+    float rx, ry, rz;          // FP32 partial since the last flush
+    double sx, sy, sz;         // FP64 running sum
     float G;                   // per lane, so G * (scalar mass) is one VALU op
     uint64_t pp, np;           // wave totals (interaction counts)
     uint64_t visits, lanes, leaf_visits, leaf_lanes;  // wave-uniform diagnostics
 };
+
+__device__ __forceinline__ void fast_init(FastWave &W, float px, float py, float pz) {
+    W.px = px; W.py = py; W.pz = pz;
+    W.rx = 0.0f; W.ry = 0.0f; W.rz = 0.0f;
+    W.sx = 0.0; W.sy = 0.0; W.sz = 0.0;
+    W.pp = 0; W.np = 0;
+    W.visits = 0; W.lanes = 0; W.leaf_visits = 0; W.leaf_lanes = 0;
+}
+
+__device__ __forceinline__ void fast_flush(FastWave &W) {
+    W.sx += (double)W.rx; W.sy += (double)W.ry; W.sz += (double)W.rz;
+    W.rx = 0.0f; W.ry = 0.0f; W.rz = 0.0f;
+}
+
+// The lane's acceleration, (running sum + partial + extra) * G, rounded once.
+__device__ __forceinline__ float3 fast_result(const FastWave &W, float G, float ex = 0.0f, float ey = 0.0f,
+                                              float ez = 0.0f) {
+    const double g = (double)G;
+    return make_float3((float)((W.sx + ((double)W.rx + (double)ex)) * g),
+                       (float)((W.sy + ((double)W.ry + (double)ey)) * g),
+                       (float)((W.sz + ((double)W.rz + (double)ez)) * g));
+}
 
 // Monopole (or body) interaction with rsqrt + FMA: s = m * inv^3; the walk
 // multiplies the sum by G once at the end.  eps2 > 0 (SOFT) makes r2 > 0 for
@@ -76,6 +107,7 @@ __device__ __forceinline__ void leaf_bodies(const float4 *__restrict__ bodies, i
             interact<SOFT>(b.x, b.y, b.z, b.w, eps2, W);
         }
     }
+    fast_flush(W);
 }
 
 // Visits one node for the lanes in `mask` (wave-uniform); returns the lanes

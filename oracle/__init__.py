@@ -65,6 +65,8 @@ def lib() -> ctypes.CDLL:
             "orc_tree_bbox": (ctypes.c_int, [_vp, fp]),
             "orc_tree_accel": (ctypes.c_int, [_vp, ctypes.c_int64, fp, fp, fp, fp, fp, fp, _u64p, _u64p,
                                               ctypes.c_int]),
+            "orc_tree_accel_f64": (ctypes.c_int, [_vp, ctypes.c_int64, fp, fp, fp, _f64p, _f64p, _f64p, _u64p,
+                                                  _u64p, ctypes.c_int]),
             "orc_step": (ctypes.c_int, [ip, fp, ctypes.c_int64, fp, fp, fp, fp, fp, fp, fp, fp, fp, fp,
                                         ctypes.c_int, ctypes.c_int]),
             "orc_direct_f32": (ctypes.c_int, [ctypes.c_int64, fp, fp, fp, fp, ctypes.c_int64, fp, fp, fp,
@@ -218,6 +220,23 @@ class Tree:
         npc = np.zeros(nt, np.uint64)
         lib().orc_tree_accel(self.h, nt, _p(tx), _p(ty), _p(tz), *[_p(v) for v in a], _p(pp, _u64p),
                              _p(npc, _u64p), threads)
+        acc = np.stack(a, axis=1)
+        return (acc, pp, npc) if counts else acc
+
+
+    def accel_f64(self, tx, ty, tz, threads=0, counts=False):
+        """The monopole walk's interaction set (the same FP32 opening
+        decisions as accel) evaluated and summed in FP64: the exact value of
+        the reference's FP32 result for each target, to measure rounding."""
+        tx, ty, tz = _f32(tx), _f32(ty), _f32(tz)
+        nt = len(tx)
+        a = [np.zeros(nt, np.float64) for _ in range(3)]
+        pp = np.zeros(nt, np.uint64)
+        npc = np.zeros(nt, np.uint64)
+        rc = lib().orc_tree_accel_f64(self.h, nt, _p(tx), _p(ty), _p(tz), *[_p(v, _f64p) for v in a],
+                                      _p(pp, _u64p), _p(npc, _u64p), threads)
+        if rc != 0:
+            raise ValueError("accel_f64 needs a monopole Barnes-Hut or linear-BVH tree")
         acc = np.stack(a, axis=1)
         return (acc, pp, npc) if counts else acc
 

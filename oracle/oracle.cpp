@@ -23,7 +23,8 @@
 // the octree child-centre recurrence and cell containment; an independent
 // float32 restatement of the COM summation order (both trees); truth tables
 // of the implicit FMM's two criteria; theta->0 walk == direct sum over leaf
-// order; FP64 direct-sum force error.  tests/test_oracle_mutations.py plants
+// order; FP64 direct-sum force error; the FP64 same-set walk (walk64) equals
+// the FP64 direct sum at theta 0 and has the FP32 walk's counts.  tests/test_oracle_mutations.py plants
 // a mistake in the octree, COM and criterion functions and checks that the
 // pins catch it.
 
@@ -88,6 +89,22 @@ inline V3 particle_particle(V3 a, float m, V3 p, float G, float eps2) {
     float r3 = r * (r * r);
     V3 nR{-R.x, -R.y, -R.z};
     return v_div(v_mul(v_mul(nR, m), G), r3);
+}
+
+// FP64 evaluation of one monopole interaction (the softened Newtonian
+// acceleration of Gravity.h:199-209 without FP32 rounding): the exact value
+// of a term of the reference's interaction set, for measuring the FP32
+// rounding of a walk (walk64 below).
+struct D3 {
+    double x, y, z;
+};
+
+inline void pp64(V3 a, float m, V3 p, double G, double eps2, D3 &acc) {
+    if (v_eq(a, p)) return;
+    const double Rx = (double)p.x - a.x, Ry = (double)p.y - a.y, Rz = (double)p.z - a.z;
+    const double r = std::sqrt(Rx * Rx + Ry * Ry + Rz * Rz + eps2);
+    const double s = -(double)m * G / (r * r * r);
+    acc.x += Rx * s; acc.y += Ry * s; acc.z += Rz * s;
 }
 
 // ---------------------------------------------------------------- quadrupole
@@ -382,6 +399,27 @@ struct LBVH {
         V3 r1 = walk(nd.child1, p, theta2, cnt);
         return v_add(v_add(kZero, r0), r1);
     }
+
+    // The same opening decisions as walk() (FP32 MAC), each monopole
+    // interaction evaluated and summed in FP64.
+    void walk64(int32_t id, V3 p, float theta2, D3 &acc, Counts &cnt) const {
+        const LNode &nd = nodes[id];
+        if (nd.count == 0) return;
+        if (mac(nd, p, theta2)) {
+            cnt.np++;
+            pp64(nd.com, nd.mass, p, prm.G, prm.eps2, acc);
+            return;
+        }
+        if (nd.leaf) {
+            for (int64_t i = nd.first; i < nd.first + nd.count; ++i) {
+                cnt.pp++;
+                pp64(pos[i], mass[i], p, prm.G, prm.eps2, acc);
+            }
+            return;
+        }
+        walk64(nd.child0, p, theta2, acc, cnt);
+        walk64(nd.child1, p, theta2, acc, cnt);
+    }
 };
 
 // ---------------------------------------------------------------- Octree
@@ -581,6 +619,27 @@ struct Octree {
         init = v_add(init, v_add(v_add(r[0], r[1]), v_add(r[2], r[3])));
         init = v_add(init, v_add(v_add(r[4], r[5]), v_add(r[6], r[7])));
         return init;
+    }
+
+    // The same opening decisions as walk() (FP32 MAC), each monopole
+    // interaction evaluated and summed in FP64.
+    void walk64(int32_t id, V3 p, D3 &acc, Counts &cnt) const {
+        const ONode &nd = nodes[id];
+        if (nd.count == 0) return;
+        if (mac(nd, p)) {
+            cnt.np++;
+            pp64(nd.com, nd.mass, p, prm.G, prm.eps2, acc);
+            return;
+        }
+        if (nd.leaf) {
+            for (int64_t i = nd.first; i < nd.first + nd.count; ++i) {
+                uint32_t e = idx[i];
+                cnt.pp++;
+                pp64(P[e], Mv[e], p, prm.G, prm.eps2, acc);
+            }
+            return;
+        }
+        for (int k = 0; k < 8; ++k) walk64(nd.child[k], p, acc, cnt);
     }
 };
 
@@ -1126,6 +1185,36 @@ int orc_tree_accel(void *hp, int64_t nt, const float *tx, const float *ty, const
             a = h->l->nodes.empty() ? kZero : h->l->walk(0, p, t2, c);
         } else {
             a = h->o->nodes.empty() ? kZero : h->o->walk(0, p, c);
+        }
+        ax[i] = a.x; ay[i] = a.y; az[i] = a.z;
+        if (pp_or_null) pp_or_null[i] = c.pp;
+        if (np_or_null) np_or_null[i] = c.np;
+    }
+    return 0;
+}
+
+// The monopole walk of orc_tree_accel with every interaction evaluated and
+// summed in FP64 (the FP32 opening decisions are unchanged): the exact
+// value of the reference's interaction set for each target, against which
+// the FP32 rounding of the reference's walk and of a device walk are
+// measured.  Returns -1 for quadrupole and FMM trees.
+int orc_tree_accel_f64(void *hp, int64_t nt, const float *tx, const float *ty, const float *tz, double *ax,
+                       double *ay, double *az, uint64_t *pp_or_null, uint64_t *np_or_null, int threads) {
+    Handle *h = (Handle *)hp;
+    if (h->fmm || (h->kind == 1 ? h->l->prm.order : h->o->prm.order) != 1) return -1;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < nt; ++i) {
+        Counts c;
+        D3 a{0.0, 0.0, 0.0};
+        V3 p{tx[i], ty[i], tz[i]};
+        if (h->kind == 1) {
+            float t2 = h->l->prm.theta * h->l->prm.theta;
+            if (!h->l->nodes.empty()) h->l->walk64(0, p, t2, a, c);
+        } else if (!h->o->nodes.empty()) {
+            h->o->walk64(0, p, a, c);
         }
         ax[i] = a.x; ay[i] = a.y; az[i] = a.z;
         if (pp_or_null) pp_or_null[i] = c.pp;

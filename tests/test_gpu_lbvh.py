@@ -191,7 +191,8 @@ def test_potential_gpu_matches_oracle(nbg, orc, gpu_ctx, n):
     s = nbg.System(gpu_ctx, _params(nbg, 16, 63), b.n)
     s.set_state(b)
     w = s.potential_f64()
-    r = orc.potential_f64(b.x, b.y, b.z, b.m, 1.0, 1e-4)
+    # the system's eps2 is an FP32 parameter: the oracle gets the same value
+    r = orc.potential_f64(b.x, b.y, b.z, b.m, 1.0, float(np.float32(1e-4)))
     assert abs(w - r) <= 1e-12 * abs(r)
 
 

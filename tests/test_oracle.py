@@ -332,6 +332,45 @@ def test_walk_theta_zero_is_direct_sum(orc, nbg, solver):
     assert np.all(err <= 1e-4 * scale + 1e-5 * np.sqrt(np.mean(scale ** 2)))
 
 
+@pytest.mark.parametrize("solver", [0, 1])
+def test_walk_f64_is_the_exact_interaction_set(orc, nbg, solver):
+    """accel_f64 takes the FP32 walk's opening decisions and sums each
+    interaction in FP64.  At theta = 0 it is the FP64 direct sum (every body
+    meets every body); at theta 0.5 / 0.3 its counts equal the FP32 walk's,
+    and the FP32 walk (pairwise tree-ordered sums) stays within FP32
+    rounding of it."""
+    n = 2000
+    eps2 = float(np.float32(1e-4))  # the tree's eps2 is FP32
+    b = nbg.plummer(n, seed=11)
+    t = orc.Tree(solver, b.x, b.y, b.z, b.m, theta=0.0, leaf=16, eps2=1e-4)
+    a64, pp, npc = t.accel_f64(b.x, b.y, b.z, counts=True)
+    assert np.all(pp == n) and np.all(npc == 0)
+    ref = orc.direct_f64(b.x, b.y, b.z, b.m, b.x, b.y, b.z, 1.0, eps2)
+    assert np.all(np.linalg.norm(a64 - ref, axis=1) <= 1e-12 * np.linalg.norm(ref, axis=1))
+    b = nbg.plummer(20000, seed=5)
+    sel = np.arange(0, 20000, 7)
+    for theta in (0.5, 0.3):
+        t = orc.Tree(solver, b.x, b.y, b.z, b.m, theta=theta, leaf=16, eps2=1e-4)
+        a32, pp32, np32 = t.accel(b.x[sel], b.y[sel], b.z[sel], counts=True)
+        a64, pp64, np64 = t.accel_f64(b.x[sel], b.y[sel], b.z[sel], counts=True)
+        assert np.array_equal(pp32, pp64) and np.array_equal(np32, np64)
+        assert np64.sum() > 0
+        mag = np.linalg.norm(a64, axis=1)
+        err = np.linalg.norm(a32 - a64, axis=1) / np.maximum(mag, np.sqrt(np.mean(mag ** 2)))
+        assert err.max() < 5e-6
+        # the FP64 set is not the FP64 direct sum: the approximation error
+        # is orders of magnitude above the rounding
+        d = orc.direct_f64(b.x, b.y, b.z, b.m, b.x[sel], b.y[sel], b.z[sel], 1.0, eps2)
+        assert np.median(np.linalg.norm(a64 - d, axis=1) / np.linalg.norm(d, axis=1)) > 1e-5
+
+
+def test_walk_f64_refuses_quadrupole_trees(orc, nbg):
+    b = nbg.plummer(100, seed=1)
+    t = orc.Tree(4, b.x, b.y, b.z, b.m, theta=0.5, leaf=16, eps2=1e-4)
+    with pytest.raises(ValueError):
+        t.accel_f64(b.x, b.y, b.z)
+
+
 @pytest.mark.parametrize("solver,theta,tol", [(0, 0.5, 3e-3), (1, 0.5, 5e-3), (1, 0.3, 1e-3)])
 def test_walk_force_error_vs_fp64(orc, nbg, solver, theta, tol):
     """Monopole BH error shrinks with theta; median error at the stated
