@@ -153,14 +153,18 @@ __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_lbvh_ordered(WalkAr
         if (nd.topo.z < 0) {
             const int first = nd.topo.x, cnt = nd.topo.y;
             if (first < 0 || cnt < 0 || (int64_t)first + cnt > a.n) { fail = true; break; }
-            if (op) {
-                float lx = 0.0f, ly = 0.0f, lz = 0.0f;
-                for (int j = 0; j < cnt; ++j) {
-                    const float4 b = bodies[first + j];
+            // the body loop runs in wave-uniform flow so its loads are scalar
+            // (one per body for the wave); the opened lanes sum
+            float lx = 0.0f, ly = 0.0f, lz = 0.0f;
+            for (int j = 0; j < cnt; ++j) {
+                const float4 b = bodies[first + j];
+                if (op) {
                     float vx, vy, vz;
                     pp_exact(b.x, b.y, b.z, b.w, px, py, pz, G, eps2, vx, vy, vz);
                     lx = lx + vx; ly = ly + vy; lz = lz + vz;
                 }
+            }
+            if (op) {
                 cpp += (uint32_t)cnt;
                 rx = rx + lx; ry = ry + ly; rz = rz + lz;
             }
@@ -196,7 +200,7 @@ __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_lbvh_ordered(WalkAr
 template <bool SOFT, int STATS, bool QUAD>
 __device__ __forceinline__ uint64_t visit(const NodeRec &nd, const float4 *__restrict__ qn, uint64_t mask,
                                           const float4 *__restrict__ bodies, int64_t n, float eps2, FastWave &W,
-                                          bool &fail) {
+                                          bool &fail, float4 *sh) {
     const uint64_t acc = __ballot(lbvh_accepts(nd.mac, W.px, W.py, W.pz)) & mask;
     if (STATS >= 2) { W.visits += 1; W.lanes += __popcll(mask); }
     if (STATS >= 1) W.np += __popcll(acc);
@@ -218,7 +222,7 @@ __device__ __forceinline__ uint64_t visit(const NodeRec &nd, const float4 *__res
     if (__builtin_expect((uint32_t)cnt > (uint32_t)n - (uint32_t)first, 0)) { fail = true; return 0; }
     if (STATS >= 2) { W.leaf_visits += 1; W.leaf_lanes += __popcll(open); }
     if (STATS >= 1) W.pp += (uint64_t)__popcll(open) * (uint64_t)cnt;
-    leaf_bodies<SOFT>(bodies, first, cnt, open, eps2, W);
+    leaf_eval<SOFT>(bodies, first, cnt, open, eps2, W, sh);
     return 0;
 }
 
@@ -227,6 +231,7 @@ template <int WAVES, bool SOFT, int STATS, bool QUAD>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_forest(WalkArgs a) {
     __shared__ int s_node[WAVES][kStackFast];
     __shared__ uint64_t s_mask[WAVES][kStackFast];
+    __shared__ float4 s_leaf[WAVES][kLeafScratch];
 
     const float4 *__restrict__ bodies = a.bodies;
     const NodeRec *__restrict__ nodes = a.nodes;
@@ -256,7 +261,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_forest(WalkArgs a) {
         return;
     }
     const NodeRec root = nodes[T.root];
-    uint64_t mask = visit<SOFT, STATS, QUAD>(root, a.quad + 2 * T.root, vmask, bodies, n, eps2, W, fail);
+    uint64_t mask = visit<SOFT, STATS, QUAD>(root, a.quad + 2 * T.root, vmask, bodies, n, eps2, W, fail, s_leaf[wv]);
     int c = root.topo.z;
     int sp = 0;
     uint32_t iters = 0;
@@ -276,8 +281,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_forest(WalkArgs a) {
         if (__builtin_expect((uint32_t)c >= last_pair, 0)) { fail = true; break; }
         if (__builtin_expect(++iters > kMaxWalkIters, 0)) { fail = true; break; }
         const NodeRec r0 = nodes[c], r1 = nodes[c + 1];
-        const uint64_t o0 = visit<SOFT, STATS, QUAD>(r0, a.quad + 2 * c, mask, bodies, n, eps2, W, fail);
-        const uint64_t o1 = visit<SOFT, STATS, QUAD>(r1, a.quad + 2 * c + 2, mask, bodies, n, eps2, W, fail);
+        const uint64_t o0 = visit<SOFT, STATS, QUAD>(r0, a.quad + 2 * c, mask, bodies, n, eps2, W, fail, s_leaf[wv]);
+        const uint64_t o1 = visit<SOFT, STATS, QUAD>(r1, a.quad + 2 * c + 2, mask, bodies, n, eps2, W, fail, s_leaf[wv]);
         // a failed visit opens nothing, so the walk still ends; fail is
         // reported after the loop
         if (o0 != 0) {
@@ -367,14 +372,18 @@ __device__ __forceinline__ uint64_t oct_eval_ordered(const OctRec &nd, const flo
     if (nd.topo.z >= 0) return open;
     const int first = nd.topo.x, cnt = nd.topo.y;
     if (first < 0 || cnt < 0 || (int64_t)first + cnt > n) { fail = true; return 0; }
-    if ((open >> lane) & 1ull) {
-        float lx = 0.0f, ly = 0.0f, lz = 0.0f;
-        for (int j = 0; j < cnt; ++j) {
-            const float4 b = bodies[first + j];
+    // the body loop runs in wave-uniform flow so its loads are scalar
+    const bool op = (open >> lane) & 1ull;
+    float lx = 0.0f, ly = 0.0f, lz = 0.0f;
+    for (int j = 0; j < cnt; ++j) {
+        const float4 b = bodies[first + j];
+        if (op) {
             float vx, vy, vz;
             pp_exact(b.x, b.y, b.z, b.w, px, py, pz, G, eps2, vx, vy, vz);
             lx = lx + vx; ly = ly + vy; lz = lz + vz;
         }
+    }
+    if (op) {
         cpp += (uint32_t)cnt;
         rx = lx; ry = ly; rz = lz;
     }
@@ -474,6 +483,7 @@ template <int WAVES, bool SOFT, int STATS, bool QUAD>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
     __shared__ int s_grp[WAVES][kStackOct];       // first child | nchild << 28
     __shared__ uint64_t s_mask[WAVES][kStackOct];
+    __shared__ float4 s_leaf[WAVES][kLeafScratch];
 
     const float4 *__restrict__ bodies = a.bodies;
     const OctRec *__restrict__ nodes = a.onodes;
@@ -492,7 +502,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
     bool fail = false;
 
     const OctRec root = nodes[0];
-    uint64_t mask = visit_oct<SOFT, STATS, QUAD>(root, a.quad, vmask, bodies, n, eps2, W, fail);
+    uint64_t mask = visit_oct<SOFT, STATS, QUAD>(root, a.quad, vmask, bodies, n, eps2, W, fail, s_leaf[wv]);
     int c = root.topo.z, nch = oct_nchild(root);
     int sp = 0;
     const int64_t max_iters = a.max_nodes + 64;  // each node's group is visited at most once
@@ -517,7 +527,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
         int nc = 0, nn = 0;
         for (int k = 0; k < nch; ++k) {
             const OctRec r = nodes[c + k];
-            const uint64_t o = visit_oct<SOFT, STATS, QUAD>(r, a.quad + 2 * (c + k), mask, bodies, n, eps2, W, fail);
+            const uint64_t o = visit_oct<SOFT, STATS, QUAD>(r, a.quad + 2 * (c + k), mask, bodies, n, eps2, W, fail, s_leaf[wv]);
             if (o) {
                 if (nm) {
                     if (sp >= kStackOct) { fail = true; break; }
@@ -594,6 +604,7 @@ template <int WAVES, bool SOFT, int STATS, bool QUAD>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     __shared__ int s_node[WAVES][kStackFast];
     __shared__ uint64_t s_mask[WAVES][kStackFast];
+    __shared__ float4 s_leaf[WAVES][kLeafScratch];
 
     const float4 *__restrict__ bodies = a.bodies;
     const NodeRec *__restrict__ nodes = a.nodes;
@@ -617,7 +628,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     // The root is visited on its own; afterwards every step visits the two
     // children (c, c + 1) of a node that the lanes in `mask` opened.
     const NodeRec root = nodes[0];
-    uint64_t mask = visit<SOFT, STATS, QUAD>(root, a.quad, vmask, bodies, n, eps2, W, fail);
+    uint64_t mask = visit<SOFT, STATS, QUAD>(root, a.quad, vmask, bodies, n, eps2, W, fail, s_leaf[wv]);
     int c = root.topo.z;
     int sp = 0;
     uint32_t iters = 0;
@@ -633,8 +644,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
         if (__builtin_expect((uint32_t)c >= last_pair, 0)) { fail = true; break; }
         if (__builtin_expect(++iters > kMaxWalkIters, 0)) { fail = true; break; }
         const NodeRec r0 = nodes[c], r1 = nodes[c + 1];
-        const uint64_t o0 = visit<SOFT, STATS, QUAD>(r0, a.quad + 2 * c, mask, bodies, n, eps2, W, fail);
-        const uint64_t o1 = visit<SOFT, STATS, QUAD>(r1, a.quad + 2 * c + 2, mask, bodies, n, eps2, W, fail);
+        const uint64_t o0 = visit<SOFT, STATS, QUAD>(r0, a.quad + 2 * c, mask, bodies, n, eps2, W, fail, s_leaf[wv]);
+        const uint64_t o1 = visit<SOFT, STATS, QUAD>(r1, a.quad + 2 * c + 2, mask, bodies, n, eps2, W, fail, s_leaf[wv]);
         // a failed visit opens nothing, so the walk still ends; fail is
         // reported after the loop
         if (o0 != 0) {

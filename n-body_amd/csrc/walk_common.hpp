@@ -110,6 +110,85 @@ __device__ __forceinline__ void leaf_bodies(const float4 *__restrict__ bodies, i
     fast_flush(W);
 }
 
+// Sum of v over each 16-lane row of the wave (four DPP row shifts); the
+// row's last lane holds the row's total.  Every lane of the wave must be
+// active.
+__device__ __forceinline__ float row_sum16(float v) {
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, true));  // row_shr:1
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x112, 0xf, 0xf, true));  // row_shr:2
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x114, 0xf, 0xf, true));  // row_shr:4
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x118, 0xf, 0xf, true));  // row_shr:8
+    return v;
+}
+
+// A leaf opened by few lanes (k = popcount(open)).  The per-lane loop costs
+// cnt wave instructions-sets whatever k is; here the k x cnt (target, body)
+// pairs are spread over the wave instead: each 16-lane row takes one target
+// (up to four per round) and each of its lanes bodies j, j + 16, ... of the
+// leaf; rows are summed by DPP and the row's last lane returns the sum to
+// the target's lane through LDS.  `sh` is the wave's 32-entry LDS scratch
+// (targets in slots 0..15 by rank, sums in 16..31).  Requires k <= 16 and
+// every lane of the wave active (the walks call it in wave-uniform flow).
+template <bool SOFT>
+__device__ __forceinline__ void leaf_rows(const float4 *__restrict__ bodies, int first, int cnt, uint64_t open,
+                                          float eps2, FastWave &W, float4 *sh) {
+    const int lane = lane_id();
+    const int k = __popcll(open);
+    const bool mine = (open >> lane) & 1ull;
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(open >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)open, 0u));
+    if (mine) sh[rank] = make_float4(W.px, W.py, W.pz, 0.0f);
+    __builtin_amdgcn_wave_barrier();
+    const int row = lane >> 4, j = lane & 15;
+    for (int q = 0; q < k; q += 4) {
+        const int tr = q + row;
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        if (tr < k) {
+            const float4 t = sh[tr];
+            for (int c = j; c < cnt; c += 16) {
+                const float4 b = bodies[first + c];
+                const float dx = b.x - t.x, dy = b.y - t.y, dz = b.z - t.z;
+                const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+                float inv = __builtin_amdgcn_rsqf(r2);
+                if (!SOFT) inv = r2 > 0.0f ? inv : 0.0f;
+                const float sc = b.w * (inv * (inv * inv));
+                fx = __builtin_fmaf(dx, sc, fx);
+                fy = __builtin_fmaf(dy, sc, fy);
+                fz = __builtin_fmaf(dz, sc, fz);
+            }
+        }
+        fx = row_sum16(fx);
+        fy = row_sum16(fy);
+        fz = row_sum16(fz);
+        if (j == 15 && tr < k) sh[16 + tr] = make_float4(fx, fy, fz, 0.0f);
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (mine) {
+        const float4 r = sh[16 + rank];
+        W.rx += r.x;
+        W.ry += r.y;
+        W.rz += r.z;
+    }
+    __builtin_amdgcn_wave_barrier();  // the next leaf rewrites sh
+    fast_flush(W);
+}
+
+// The opened lanes of a leaf interact with its bodies: spread over rows
+// (leaf_rows) when that takes fewer wave instructions than the per-lane loop
+// (leaf_bodies), judged per leaf from k and cnt (wave-uniform); `sh` null
+// always takes the per-lane loop.
+template <bool SOFT>
+__device__ __forceinline__ void leaf_eval(const float4 *__restrict__ bodies, int first, int cnt, uint64_t open,
+                                          float eps2, FastWave &W, float4 *sh) {
+    const int k = __popcll(open);
+    // rows: ~16 instructions of set-up, per round ~13 per 16 bodies + ~16 for
+    // the row sums; per-lane loop: ~12 per body
+    if (sh && k <= 16 && 16 + ((k + 3) >> 2) * (13 * ((cnt + 15) >> 4) + 16) < 12 * cnt)
+        leaf_rows<SOFT>(bodies, first, cnt, open, eps2, W, sh);
+    else
+        leaf_bodies<SOFT>(bodies, first, cnt, open, eps2, W);
+}
+
 // Visits one node for the lanes in `mask` (wave-uniform); returns the lanes
 // that open it when it is internal, 0 when it is a leaf or all accept.
 
@@ -130,7 +209,7 @@ __device__ __forceinline__ bool oct_accepts(const OctRec &r, float px, float py,
 template <bool SOFT, int STATS, bool QUAD>
 __device__ __forceinline__ uint64_t visit_oct(const OctRec &nd, const float4 *__restrict__ qn, uint64_t mask,
                                               const float4 *__restrict__ bodies, int64_t n, float eps2, FastWave &W,
-                                              bool &fail) {
+                                              bool &fail, float4 *sh = nullptr) {
     const uint64_t acc = __ballot(oct_accepts(nd, W.px, W.py, W.pz)) & mask;
     if (STATS >= 2) { W.visits += 1; W.lanes += __popcll(mask); }
     if (STATS >= 1) W.np += __popcll(acc);
@@ -151,10 +230,11 @@ __device__ __forceinline__ uint64_t visit_oct(const OctRec &nd, const float4 *__
     if (__builtin_expect((uint64_t)(uint32_t)cnt > (uint64_t)n - (uint32_t)first, 0)) { fail = true; return 0; }
     if (STATS >= 2) { W.leaf_visits += 1; W.leaf_lanes += __popcll(open); }
     if (STATS >= 1) W.pp += (uint64_t)__popcll(open) * (uint64_t)cnt;
-    leaf_bodies<SOFT>(bodies, first, cnt, open, eps2, W);
+    leaf_eval<SOFT>(bodies, first, cnt, open, eps2, W, sh);
     return 0;
 }
 
 constexpr int kStackOct = 256;
+constexpr int kLeafScratch = 32;  // float4 per wave for leaf_rows
 
 }  // namespace nbg
