@@ -896,9 +896,11 @@ __global__ __launch_bounds__(64) void k_fmm_step(FmmArgs a, int k) {
 }
 
 // The passive leaves, in groups collected by all the steps: one launch, one
-// group per wave at a time.
+// group per wave at a time.  No waves-per-EU cap: held to 64 VGPRs the
+// kernel spilled wave-wide values inside divergent branches (see
+// tests/test_code_object.py); it needs ~77.
 template <bool SOFT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_fmm_groups(FmmArgs a) {
+__global__ __launch_bounds__(64) void k_fmm_groups(FmmArgs a) {
     __shared__ int s_grp[kStackOct];
     __shared__ uint64_t s_msk[kStackOct];
     __shared__ GroupLds s_g;

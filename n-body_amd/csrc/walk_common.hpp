@@ -121,45 +121,39 @@ __device__ __forceinline__ float row_sum16(float v) {
     return v;
 }
 
-// A leaf opened by few lanes (k = popcount(open)).  The per-lane loop costs
-// cnt wave instructions-sets whatever k is; here the k x cnt (target, body)
-// pairs are spread over the wave instead: each 16-lane row takes one target
-// (up to four per round) and each of its lanes bodies j, j + 16, ... of the
-// leaf; rows are summed by DPP and the row's last lane returns the sum to
-// the target's lane through LDS.  `sh` is the wave's 32-entry LDS scratch
-// (targets in slots 0..15 by rank, sums in 16..31).  Requires k <= 16 and
-// every lane of the wave active (the walks call it in wave-uniform flow).
+// A leaf opened by few lanes (k = popcount(open)) and holding at most 16
+// bodies.  The per-lane loop costs cnt wave instruction-sets whatever k is;
+// here the k x cnt (target, body) pairs are spread over the wave instead:
+// lane j of each 16-lane row holds body j of the leaf (one vector load,
+// issued before any round) and each row takes one target per round, four
+// per round; rows are summed by DPP and the row's last lane returns the sum
+// to the target's lane through LDS.  `sh` is the wave's 32-entry LDS
+// scratch (targets in slots 0..15 by rank, sums in 16..31).  Rows past the
+// last target repeat it and are not written back; lanes past the last body
+// hold a massless body at the origin (an exact zero term).  Requires
+// 1 <= k <= 16, cnt <= 16 and every lane of the wave active (the walks call
+// it in wave-uniform flow).
 template <bool SOFT>
 __device__ __forceinline__ void leaf_rows(const float4 *__restrict__ bodies, int first, int cnt, uint64_t open,
                                           float eps2, FastWave &W, float4 *sh) {
     const int lane = lane_id();
+    const int row = lane >> 4, j = lane & 15;
+    const float4 b = j < cnt ? bodies[first + j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const int k = __popcll(open);
     const bool mine = (open >> lane) & 1ull;
     const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(open >> 32),
                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)open, 0u));
     if (mine) sh[rank] = make_float4(W.px, W.py, W.pz, 0.0f);
     __builtin_amdgcn_wave_barrier();
-    const int row = lane >> 4, j = lane & 15;
     for (int q = 0; q < k; q += 4) {
         const int tr = q + row;
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-        if (tr < k) {
-            const float4 t = sh[tr];
-            for (int c = j; c < cnt; c += 16) {
-                const float4 b = bodies[first + c];
-                const float dx = b.x - t.x, dy = b.y - t.y, dz = b.z - t.z;
-                const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
-                float inv = __builtin_amdgcn_rsqf(r2);
-                if (!SOFT) inv = r2 > 0.0f ? inv : 0.0f;
-                const float sc = b.w * (inv * (inv * inv));
-                fx = __builtin_fmaf(dx, sc, fx);
-                fy = __builtin_fmaf(dy, sc, fy);
-                fz = __builtin_fmaf(dz, sc, fz);
-            }
-        }
-        fx = row_sum16(fx);
-        fy = row_sum16(fy);
-        fz = row_sum16(fz);
+        const float4 t = sh[tr < k ? tr : k - 1];
+        const float dx = b.x - t.x, dy = b.y - t.y, dz = b.z - t.z;
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+        float inv = __builtin_amdgcn_rsqf(r2);
+        if (!SOFT) inv = r2 > 0.0f ? inv : 0.0f;
+        const float sc = b.w * (inv * (inv * inv));
+        const float fx = row_sum16(dx * sc), fy = row_sum16(dy * sc), fz = row_sum16(dz * sc);
         if (j == 15 && tr < k) sh[16 + tr] = make_float4(fx, fy, fz, 0.0f);
     }
     __builtin_amdgcn_wave_barrier();
@@ -181,9 +175,9 @@ template <bool SOFT>
 __device__ __forceinline__ void leaf_eval(const float4 *__restrict__ bodies, int first, int cnt, uint64_t open,
                                           float eps2, FastWave &W, float4 *sh) {
     const int k = __popcll(open);
-    // rows: ~16 instructions of set-up, per round ~13 per 16 bodies + ~16 for
-    // the row sums; per-lane loop: ~12 per body
-    if (sh && k <= 16 && 16 + ((k + 3) >> 2) * (13 * ((cnt + 15) >> 4) + 16) < 12 * cnt)
+    // rows: ~16 instructions of set-up and ~30 per round of four targets;
+    // per-lane loop: ~12 per body
+    if (sh && k <= 16 && cnt <= 16 && 16 + 30 * ((k + 3) >> 2) < 12 * cnt)
         leaf_rows<SOFT>(bodies, first, cnt, open, eps2, W, sh);
     else
         leaf_bodies<SOFT>(bodies, first, cnt, open, eps2, W);
