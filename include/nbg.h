@@ -147,6 +147,18 @@ int nbg_sys_get_accelerations(nbg_sys *sys, float *ax, float *ay, float *az);
 int nbg_sys_accelerations_at(nbg_sys *sys, int64_t nt, const float *tx, const float *ty, const float *tz,
                              float *ax, float *ay, float *az);
 
+/* The same walk as nbg_sys_accelerations_at, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc); it has no counterpart in the reference.  It calls
+ * no hipMalloc, hipFree or allocating library routine: its scratch is one
+ * static device array per GPU (about 17 MB, held until the process ends),
+ * so at most NBG_NOALLOC_MAX_POINTS targets per call (more: NBG_ERR_ARG).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (from several threads, on any systems of that device) race. */
+#define NBG_NOALLOC_MAX_POINTS 262144
+int nbg_sys_accelerations_at_noalloc(nbg_sys *sys, int64_t nt, const float *tx, const float *ty, const float *tz,
+                                     float *ax, float *ay, float *az);
+
 /* Solver::step() repeated nsteps times (Simulation/Solver.h:103-139):
  * updateAccelerations, then v = v + a*dt, then p = p + v*dt, on the device,
  * with no host copies.  Asynchronous: returns after enqueueing. */

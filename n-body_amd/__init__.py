@@ -38,7 +38,7 @@ EXPORTS = (
     "nbg_params_default", "nbg_last_error", "nbg_version", "nbg_device_count",
     "nbg_ctx_create", "nbg_ctx_destroy", "nbg_sys_create", "nbg_sys_destroy",
     "nbg_sys_set_params", "nbg_sys_get_params", "nbg_sys_set_state", "nbg_sys_get_state",
-    "nbg_sys_update_accelerations", "nbg_sys_get_accelerations", "nbg_sys_accelerations_at", "nbg_sys_step",
+    "nbg_sys_update_accelerations", "nbg_sys_get_accelerations", "nbg_sys_accelerations_at", "nbg_sys_accelerations_at_noalloc", "nbg_sys_step",
     "nbg_sys_synchronize", "nbg_sys_counts", "nbg_sys_counts4", "nbg_sys_enable_counts", "nbg_sys_enable_walk_stats",
     "nbg_sys_walk_stats",
     "nbg_sys_enable_timing", "nbg_sys_phase_times",
@@ -111,6 +111,7 @@ def lib() -> ctypes.CDLL:
             "nbg_sys_update_accelerations": (ctypes.c_int, [_vp] + [_f32p] * 3),
             "nbg_sys_get_accelerations": (ctypes.c_int, [_vp] + [_f32p] * 3),
             "nbg_sys_accelerations_at": (ctypes.c_int, [_vp, ctypes.c_int64] + [_f32p] * 6),
+            "nbg_sys_accelerations_at_noalloc": (ctypes.c_int, [_vp, ctypes.c_int64] + [_f32p] * 6),
             "nbg_batch_bytes": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_int32]),
             "nbg_batch_create": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                                 ctypes.POINTER(_vp)]),
@@ -323,6 +324,16 @@ class System:
         nt = len(t[0])
         a = [np.zeros(nt, np.float32) for _ in range(3)]
         _check(lib().nbg_sys_accelerations_at(self.handle, nt, *[_ptr(v) for v in t], *[_ptr(v) for v in a]))
+        return np.stack(a, axis=1)
+
+    def accelerations_at_noalloc(self, tx, ty, tz) -> np.ndarray:
+        """accelerations_at through the allocation-free entry point (static
+        device scratch, at most NOALLOC_MAX_POINTS targets)."""
+        t = [_f32(v) for v in (tx, ty, tz)]
+        nt = len(t[0])
+        a = [np.zeros(nt, np.float32) for _ in range(3)]
+        _check(lib().nbg_sys_accelerations_at_noalloc(self.handle, nt, *[_ptr(v) for v in t],
+                                                      *[_ptr(v) for v in a]))
         return np.stack(a, axis=1)
 
     def step(self, nsteps: int = 1):

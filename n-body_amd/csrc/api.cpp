@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "../../include/nbg.h"
@@ -491,6 +492,36 @@ PointsBuf points_carve(void *base, int64_t nt) {
     return p;
 }
 
+// Scratch of nbg_sys_accelerations_at_noalloc: a file-scope device array
+// (points_layout of NBG_NOALLOC_MAX_POINTS targets, about 17 MB) that every
+// GPU the library runs on holds until the process ends.  In host code its
+// name is not a device address: nbg_ctx_create looks up each device's own
+// copy with hipGetSymbolAddress, with that device current, and host code
+// uses only that pointer (kernel arguments, copies, fills).
+static __device__ __align__(256) unsigned char scratch[points_layout(NBG_NOALLOC_MAX_POINTS).bytes];
+
+constexpr int kMaxNoAllocDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kMaxNoAllocDevices] = {};
+
+// This device's scratch pointer, looked up once per device (the caller has
+// made the device current).
+int noalloc_scratch_of(int device, void **out) {
+    *out = nullptr;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (device < kMaxNoAllocDevices && g_noalloc_ptr[device]) {
+        *out = g_noalloc_ptr[device];
+        return NBG_OK;
+    }
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(NBG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    if (device < kMaxNoAllocDevices) g_noalloc_ptr[device] = p;
+    *out = p;
+    return NBG_OK;
+}
+
 int check_points(nbg_sys *s, int64_t nt, const float *tx, const float *ty, const float *tz, float *ax, float *ay,
                  float *az) {
     if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
@@ -580,8 +611,11 @@ int nbg_ctx_create(int device, nbg_ctx **out) {
     if (hipGetDeviceCount(&c) != hipSuccess || c <= 0) return fail(NBG_ERR_NO_DEVICE, "no HIP device available");
     if (device < 0 || device >= c) return fail(NBG_ERR_ARG, "device index out of range");
     NBG_HIP(hipSetDevice(device));
+    void *scr = nullptr;
+    if (int r = noalloc_scratch_of(device, &scr)) return r;
     nbg_ctx *ctx = new nbg_ctx;
     ctx->device = device;
+    ctx->noalloc_scratch = scr;
     *out = ctx;
     return NBG_OK;
 }
@@ -747,6 +781,18 @@ int nbg_sys_accelerations_at(nbg_sys *s, int64_t nt, const float *tx, const floa
         s->pts_bytes = need;
     }
     return walk_points(s, nt, tx, ty, tz, ax, ay, az, points_carve(s->pts, nt));
+}
+
+int nbg_sys_accelerations_at_noalloc(nbg_sys *s, int64_t nt, const float *tx, const float *ty, const float *tz,
+                                     float *ax, float *ay, float *az) {
+    if (int r = check_points(s, nt, tx, ty, tz, ax, ay, az)) return r;
+    if (nt > NBG_NOALLOC_MAX_POINTS)
+        return fail(NBG_ERR_ARG, "nbg_sys_accelerations_at_noalloc: more than NBG_NOALLOC_MAX_POINTS (262144) "
+                                 "targets; use nbg_sys_accelerations_at");
+    if (nt == 0) return NBG_OK;
+    if (!s->ctx->noalloc_scratch) return fail(NBG_ERR_STATE, "the context has no device scratch");
+    DeviceGuard g(s->ctx->device);
+    return walk_points(s, nt, tx, ty, tz, ax, ay, az, points_carve(s->ctx->noalloc_scratch, nt));
 }
 
 int nbg_sys_step(nbg_sys *s, int32_t nsteps) {

@@ -47,4 +47,7 @@ struct DeviceGuard {
 // A context binds one GPU.
 struct nbg_ctx {
     int device = 0;
+    // this device's copy of the allocation-free passive-point scratch
+    // (looked up with hipGetSymbolAddress at nbg_ctx_create)
+    void *noalloc_scratch = nullptr;
 };
