@@ -891,6 +891,12 @@ void launch_forest_build(const float *x, const float *y, const float *z, int64_t
     launch_radix_sort_u64(keys, idx, keys_alt, idx_alt, n, passes, sort_status, err, s);
 }
 
+void launch_seg_bbox(const float *x, const float *y, const float *z, int nsys, const int64_t *sys_first,
+                     float *bbox6, hipStream_t s) {
+    if (nsys <= 0) return;
+    hipLaunchKernelGGL(k_seg_bbox, dim3((unsigned)nsys), dim3(256), 0, s, x, y, z, sys_first, bbox6);
+}
+
 void launch_forest_roots(const NodeRec *nodes, const uint32_t *node_count, int64_t max_nodes, const uint64_t *keys,
                          int64_t n, int nsys, int32_t *sys_root, uint32_t *err, hipStream_t s) {
     if (n <= 0 || nsys <= 0) return;

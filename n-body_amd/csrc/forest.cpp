@@ -1,10 +1,15 @@
 // C ABI of the batched forest: many independent systems stepped together.
 // Host code only.  The systems' states are concatenated in HBM in membership
-// order; one pass builds every system's linear BVH (segmented boxes, one
-// sort of system-prefixed keys, one radix-tree build, one summary pass) and
-// walks every system in one launch, with wavefronts aligned to systems.
-// The members share one solver: the monopole linear BVH or its quadrupole
-// form (second moments summarized with the tree, quadrupole walk).
+// order; one pass builds every system's tree and walks every system in one
+// launch, with wavefronts aligned to systems.  The members share one solver:
+//   * the linear BVH (LinearBVHSolver): segmented boxes, one sort of
+//     system-prefixed 30-bit keys, one radix-tree build, one summary pass;
+//   * the Barnes-Hut octree (BarnesHutSolver): segmented boxes, each body's
+//     octant path over its own system's cube and depth, one sort by (system,
+//     path), one octree build in which systems are a digit above the root,
+//     one summary pass;
+// each monopole or quadrupole (second moments summarized with the tree,
+// quadrupole walk).
 
 #include <hip/hip_runtime.h>
 
@@ -77,9 +82,15 @@ struct nbg_batch {
     float *bbox6 = nullptr;
     uint32_t *misc = nullptr;  // [1] err, [2] node count, [4..16) counters
     uint32_t *bad = nullptr;   // one word per system slot: the non-finite guard's flags
-    // second moments of every node (2 float4 per node), allocated when the
-    // first quadrupole member joins
+    // second moments of every node (2 float4 per node slot), allocated when
+    // the first quadrupole member joins (linear BVH) or with the octree
     float4 *quad = nullptr;
+    int64_t quad_slots = 0;
+    // octree members: work and nodes sized for the members' node bound
+    // (allocated when the tables are uploaded), levels per system
+    OctBuffers ob;
+    int32_t *lv_tab = nullptr;
+    int max_levels = 0;
 
     float *x() { return st[0]; }
     float *y() { return st[1]; }
@@ -112,7 +123,22 @@ int64_t bytes_for(int64_t cap_n, int64_t cap_s) {
     return b;
 }
 
+bool is_octree_solver(int32_t solver) { return solver == NBG_BARNES_HUT || solver == NBG_BARNES_HUT_QUADRUPOLE; }
+bool is_quad_solver(int32_t solver) {
+    return solver == NBG_LINEAR_BVH_QUADRUPOLE || solver == NBG_BARNES_HUT_QUADRUPOLE;
+}
+
+void free_octree(OctBuffers &o) {
+    void *ptrs[] = {o.key_hi, o.key_lo, o.lo_sorted, o.nown, o.nbase, o.nfirst, o.ncount, o.nparent, o.nmeta,
+                    o.occ, o.cb, o.flags, o.glist, o.scan_ws, o.nodes, o.centre, o.thr, o.misc};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    o = OctBuffers{};
+}
+
 void free_batch(nbg_batch *b) {
+    free_octree(b->ob);
+    if (b->lv_tab) (void)hipFree(b->lv_tab);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     std::vector<void *> ptrs = {b->bsys, b->keys, b->keys_alt, b->idx, b->idx_alt, b->bodies, b->kfirst, b->klast,
@@ -129,9 +155,13 @@ void free_batch(nbg_batch *b) {
 
 int check_member_params(const nbg_params *p) {
     if (!p) return fail(NBG_ERR_ARG, "params is NULL");
-    if (p->solver != NBG_LINEAR_BVH && p->solver != NBG_LINEAR_BVH_QUADRUPOLE)
-        return fail(NBG_ERR_ARG, "the batched forest runs the linear-BVH solvers (monopole or quadrupole)");
-    if (p->morton_bits != 30) return fail(NBG_ERR_ARG, "the batched forest uses 30-bit Morton keys");
+    const bool octree = is_octree_solver(p->solver);
+    if (!octree && p->solver != NBG_LINEAR_BVH && p->solver != NBG_LINEAR_BVH_QUADRUPOLE)
+        return fail(NBG_ERR_ARG, "the batched forest runs the Barnes-Hut octree and linear-BVH solvers "
+                                 "(monopole or quadrupole)");
+    if (!octree && p->morton_bits != 30)
+        return fail(NBG_ERR_ARG, "the batched forest's linear BVH uses 30-bit Morton keys");
+    if (octree && (p->max_depth < 0 || p->max_depth > 32)) return fail(NBG_ERR_ARG, "max_depth must be in [0, 32]");
     if (!(p->theta >= 0.0f)) return fail(NBG_ERR_ARG, "theta must be >= 0");
     if (p->leaf_size < 1) return fail(NBG_ERR_ARG, "leaf_size must be >= 1");
     if (!(p->eps2 >= 0.0f)) return fail(NBG_ERR_ARG, "eps2 must be >= 0");
@@ -140,18 +170,61 @@ int check_member_params(const nbg_params *p) {
 
 // Members share one solver; `self` (or null) is the member being changed.
 // A quadrupole batch allocates its node moments here, outside any pass.
+int ensure_quad(nbg_batch *b, int64_t slots) {
+    if (b->quad && b->quad_slots >= slots) return NBG_OK;
+    NBG_HIP(hipStreamSynchronize(b->stream));
+    if (b->quad) (void)hipFree(b->quad);
+    b->quad = nullptr;
+    b->quad_slots = 0;
+    if (dalloc(&b->quad, 2 * slots) != NBG_OK) {
+        b->quad = nullptr;
+        (void)hipGetLastError();  // the failed allocation must not fail the next launch check
+        return fail(NBG_ERR_BUDGET, "no device memory for the quadrupole moments");
+    }
+    b->quad_slots = slots;
+    return NBG_OK;
+}
+
 int check_member_solver(nbg_batch *b, const nbg_params *p, const Member *self) {
     for (const Member &m : b->sys)
         if (&m != self && m.p.solver != p->solver)
-            return fail(NBG_ERR_ARG, "the systems of a batch share one solver (monopole or quadrupole linear BVH)");
-    if (p->solver == NBG_LINEAR_BVH_QUADRUPOLE && !b->quad) {
+            return fail(NBG_ERR_ARG, "the systems of a batch share one solver");
+    // the octree's moments are sized with its nodes (upload_tables)
+    if (p->solver == NBG_LINEAR_BVH_QUADRUPOLE) return ensure_quad(b, 2 * b->cap_n);
+    return NBG_OK;
+}
+
+// Octree work and node arrays for the current members (their node bound is
+// the sum of each system's), and the quadrupole moments with them.
+int ensure_octree(nbg_batch *b) {
+    int64_t cap = 0;
+    for (const Member &m : b->sys) cap += octree_capacity(m.n, m.p.leaf_size, m.p.max_depth);
+    if (cap >= ((int64_t)1 << 28))
+        return fail(NBG_ERR_BUDGET, "the forest's octree node bound exceeds 2^28 slots; raise leaf sizes");
+    OctBuffers &o = b->ob;
+    if (o.cap < cap) {
         NBG_HIP(hipStreamSynchronize(b->stream));
-        if (dalloc(&b->quad, 2 * 2 * b->cap_n) != NBG_OK) {
-            b->quad = nullptr;
-            (void)hipGetLastError();  // the failed allocation must not fail the next launch check
-            return fail(NBG_ERR_BUDGET, "no device memory for the quadrupole moments");
+        free_octree(o);
+        const int64_t n = b->cap_n;
+        int r = NBG_OK;
+        auto A = [&](auto **ptr, int64_t count) {
+            if (r == NBG_OK) r = dalloc(ptr, count);
+        };
+        A(&o.key_hi, n); A(&o.key_lo, n); A(&o.lo_sorted, n); A(&o.nown, n); A(&o.nbase, n);
+        A(&o.nfirst, cap); A(&o.ncount, cap); A(&o.nparent, cap); A(&o.nmeta, cap);
+        A(&o.occ, cap); A(&o.cb, cap); A(&o.flags, cap); A(&o.glist, 2 * (cap + 64));
+        A(&o.scan_ws, 128 + (cap + kScanTile - 1) / kScanTile);
+        A(&o.nodes, cap); A(&o.centre, cap); A(&o.thr, (int64_t)kOctThrStride * b->cap_s); A(&o.misc, 64);
+        if (r != NBG_OK) {
+            free_octree(o);
+            (void)hipGetLastError();
+            return fail(NBG_ERR_BUDGET, "no device memory for the forest's octree");
         }
+        o.cap = cap;
+        o.n = n;
+        NBG_HIP(hipMemset(o.misc, 0, 64 * sizeof(uint32_t)));
     }
+    if (b->solver == NBG_BARNES_HUT_QUADRUPOLE) return ensure_quad(b, o.cap);
     return NBG_OK;
 }
 
@@ -160,11 +233,12 @@ int upload_tables(nbg_batch *b) {
     if (!b->tables_dirty) return NBG_OK;
     const size_t S = b->sys.size();
     std::vector<int64_t> first(S + 1), waves(S + 1);
-    std::vector<int32_t> leaf(S);
+    std::vector<int32_t> leaf(S), levels(S), roots(S);
     std::vector<float4> prm(S);
     std::vector<MacParam> mp(S);
     int64_t off = 0, w = 0;
     b->all_soft = true;
+    b->max_levels = 0;
     if (S) b->solver = b->sys[0].p.solver;
     for (size_t i = 0; i < S; ++i) {
         Member &m = b->sys[i];
@@ -174,10 +248,15 @@ int upload_tables(nbg_batch *b) {
         off += m.n;
         w += (m.n + 63) / 64;
         leaf[i] = m.p.leaf_size;
+        levels[i] = m.p.max_depth + 2;
+        if (levels[i] > b->max_levels) b->max_levels = levels[i];
+        roots[i] = (int32_t)i;  // an octree forest puts system i's root in slot i
         prm[i] = make_float4(m.p.G, m.p.eps2, m.p.theta, m.p.dt);
         mac_threshold(m.p.theta, &mp[i].mid, &mp[i].tie);
         if (!(m.p.eps2 > 0.0f)) b->all_soft = false;
     }
+    if (S && is_octree_solver(b->solver))
+        if (int r = ensure_octree(b)) return r;
     first[S] = off;
     waves[S] = w;
     b->n = off;
@@ -189,6 +268,9 @@ int upload_tables(nbg_batch *b) {
         NBG_HIP(hipMemcpyAsync(b->leaf_tab, leaf.data(), S * 4, hipMemcpyHostToDevice, s));
         NBG_HIP(hipMemcpyAsync(b->sys_prm, prm.data(), S * sizeof(float4), hipMemcpyHostToDevice, s));
         NBG_HIP(hipMemcpyAsync(b->mp_tab, mp.data(), S * sizeof(MacParam), hipMemcpyHostToDevice, s));
+        NBG_HIP(hipMemcpyAsync(b->lv_tab, levels.data(), S * 4, hipMemcpyHostToDevice, s));
+        if (is_octree_solver(b->solver))
+            NBG_HIP(hipMemcpyAsync(b->sys_root, roots.data(), S * 4, hipMemcpyHostToDevice, s));
     }
     for (size_t i = 0; i < S; ++i)
         launch_fill_u32(b->bsys + b->sys[i].first, (uint32_t)i, b->sys[i].n, s);
@@ -198,6 +280,54 @@ int upload_tables(nbg_batch *b) {
     return NBG_OK;
 }
 
+// The octree form of enqueue_pass: every system's Barnes-Hut octree
+// (BarnesHutSolver, Solvers/BarnesHutSolver.h:14-32) in one build, walked
+// in one launch.
+void enqueue_pass_octree(nbg_batch *b) {
+    const int64_t n = b->n;
+    const int S = (int)b->sys.size();
+    hipStream_t s = b->stream;
+    OctBuffers &ob = b->ob;
+    launch_seg_bbox(b->x(), b->y(), b->z(), S, b->sys_first, b->bbox6, s);
+    launch_octree_forest_build(b->x(), b->y(), b->z(), n, S, b->bsys, b->bbox6, b->lv_tab, b->max_levels, ob,
+                               b->keys, b->idx, b->keys_alt, b->idx_alt, b->sort_status, b->err_word(), s);
+    launch_octree_forest_topology(n, S, b->bbox6, b->sys_prm, b->lv_tab, b->leaf_tab, b->sys_first, b->max_levels,
+                                  ob, b->keys, b->keys_alt, b->idx_alt, b->idx, b->leaf_start, b->err_word(), s);
+    launch_gather(b->x(), b->y(), b->z(), b->m(), b->idx, n, b->bodies, s);
+    float4 *quad = b->solver == NBG_BARNES_HUT_QUADRUPOLE ? b->quad : nullptr;
+    launch_octree_summaries(b->bodies, n, ob, b->leaf_start, b->err_word(), s, quad);
+    WalkArgs wa;
+    wa.n = n;
+    wa.bodies = b->bodies;
+    wa.nt = n;
+    wa.targets = b->bodies;
+    wa.perm = b->idx;
+    wa.nodes = nullptr;
+    wa.onodes = ob.nodes;
+    wa.levels = b->max_levels;
+    wa.theta = 0.0f;
+    wa.theta2 = 0.0f;
+    wa.mac_mid = 0.0;
+    wa.mac_tie = 0;
+    wa.G = 1.0f;
+    wa.eps2 = b->all_soft ? 1.0f : 0.0f;  // see enqueue_pass
+    wa.max_nodes = ob.cap;
+    wa.ax = b->ax();
+    wa.ay = b->ay();
+    wa.az = b->az();
+    wa.counts = b->counts();
+    wa.err = b->err_word();
+    wa.nsys = S;
+    wa.nwaves = b->nwaves;
+    wa.wave_first = b->wave_first;
+    wa.sys_first = b->sys_first;
+    wa.sys_root = b->sys_root;
+    wa.sys_prm = b->sys_prm;
+    wa.quad = quad;
+    const bool ordered = b->sum_mode == NBG_SUM_ORDERED;
+    launch_walk_oct(wa, ordered, (b->counting || ordered) ? 1 : 0, s);
+}
+
 // One force pass over every system (enqueue only).
 void enqueue_pass(nbg_batch *b) {
     const int64_t n = b->n;
@@ -205,6 +335,10 @@ void enqueue_pass(nbg_batch *b) {
     hipStream_t s = b->stream;
     launch_fill_u32(b->misc + 4, 0u, 12, s);
     if (n == 0) return;
+    if (is_octree_solver(b->solver)) {
+        enqueue_pass_octree(b);
+        return;
+    }
     launch_forest_build(b->x(), b->y(), b->z(), n, S, b->sys_first, b->bsys, b->bbox6, b->keys, b->idx, b->keys_alt,
                         b->idx_alt, b->sort_status, b->err_word(), s);
     launch_lbvh_topology(b->keys, 63, n, 0, b->kfirst, b->klast, b->kgamma, b->ksplit, b->krank, b->kparent,
@@ -318,6 +452,7 @@ int nbg_batch_create(nbg_ctx *ctx, int64_t max_bodies, int32_t max_systems, int3
     A(&b->nodes, 2 * n); A(&b->bmin, 2 * n); A(&b->bmax, 2 * n); A(&b->flags, 2 * n);
     A(&b->sys_first, S + 1); A(&b->wave_first, S + 1); A(&b->old_first, S + 1);
     A(&b->sys_root, S); A(&b->leaf_tab, S); A(&b->sys_prm, S); A(&b->mp_tab, S); A(&b->bbox6, 6 * S);
+    A(&b->lv_tab, S);
     A(&b->misc, 32);
     A(&b->bad, S);
     if (r == NBG_OK) {

@@ -68,6 +68,7 @@ __host__ __device__ inline int oct_level(const OctRec &r) { return (int)(__built
 // 0..20 live in key_hi bits 62..0, levels 21..33 in key_lo bits 38..0.
 constexpr int kOctHiLevels = 21;
 constexpr int kOctLevels = 34;   // max_depth 32 below the two forced levels
+constexpr int kOctThrStride = 64;  // opening thresholds per system (one per level)
 
 // Upper bound on the number of non-empty octree nodes: every body is in one
 // leaf, and on each level 2..levels-1 an internal node holds > leaf bodies.
@@ -110,6 +111,29 @@ void launch_octree_sort(int64_t n, int max_depth, OctBuffers &ob, uint64_t *keys
 void launch_octree_topology(int64_t n, const float *bbox6, int leaf_size, int max_depth, float theta,
                             OctBuffers &ob, const uint64_t *keys, const uint32_t *idx_alt, uint32_t *idx,
                             int32_t *leaf_start, uint32_t *err, hipStream_t s);
+// Forest of octrees: bodies of system s (bsys, original order) get paths
+// over their own system's cube (bbox6 + 6 s) and depth (lv_tab[s] levels);
+// the sort orders positions by system, then path.  Leaves the sorted high
+// words in keys (low words in ob.lo_sorted when max_levels > 21), the
+// sorted system index in keys_alt and the permutation in idx_alt.
+void launch_octree_forest_build(const float *x, const float *y, const float *z, int64_t n, int nsys,
+                                const uint32_t *bsys, const float *bbox6, const int32_t *lv_tab, int max_levels,
+                                OctBuffers &ob, uint64_t *keys, uint32_t *idx, uint64_t *keys_alt, uint32_t *idx_alt,
+                                uint32_t *sort_status, uint32_t *err, hipStream_t s);
+
+// Forest topology: every system gets its own octree (root slot s, its own
+// leaf size, depth, and opening thresholds from its cube and theta =
+// sys_prm[s].z; ob.thr holds kOctThrStride floats per system).
+void launch_octree_forest_topology(int64_t n, int nsys, const float *bbox6, const float4 *sys_prm,
+                                   const int32_t *lv_tab, const int32_t *leaf_tab, const int64_t *sys_first,
+                                   int max_levels, OctBuffers &ob, const uint64_t *keys, const uint64_t *sys_sorted,
+                                   const uint32_t *idx_alt, uint32_t *idx, int32_t *leaf_start, uint32_t *err,
+                                   hipStream_t s);
+
+// Per-system bounding boxes (one workgroup per system).
+void launch_seg_bbox(const float *x, const float *y, const float *z, int nsys, const int64_t *sys_first,
+                     float *bbox6, hipStream_t s);
+
 void launch_octree_summaries(const float4 *bodies, int64_t n, OctBuffers &ob, const int32_t *leaf_start,
                              uint32_t *err, hipStream_t s, float4 *quad = nullptr);
 

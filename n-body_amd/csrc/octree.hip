@@ -78,50 +78,63 @@ struct OctKeys {
     const uint64_t *lo;  // sorted; null when levels <= kOctHiLevels
     int64_t n;
     int levels;
+    // Forest (many systems, sorted by system, then path): the system of each
+    // position, each system's first position (nsys + 1 entries) and its
+    // levels and leaf size; null for one system.  The system acts as a digit
+    // above level 0: positions of different systems share no digit (-1), so
+    // every cell, the root included, ends at its system's boundary.
+    const uint64_t *sys = nullptr;
+    const int64_t *sys_first = nullptr;
+    const int32_t *lv_tab = nullptr;
+    const int32_t *leaf_tab = nullptr;
     __device__ __forceinline__ uint64_t H(int64_t i) const { return hi[i]; }
     __device__ __forceinline__ uint64_t L(int64_t i) const { return lo ? lo[i] : 0ull; }
-    __device__ __forceinline__ int cpd_to(int64_t k, uint64_t h, uint64_t l) const {
-        return oct_cpd(H(k), L(k), h, l, levels);
+    __device__ __forceinline__ int S(int64_t i) const { return sys ? (int)sys[i] : 0; }
+    __device__ __forceinline__ int lev(int s) const { return lv_tab ? lv_tab[s] : levels; }
+    __device__ __forceinline__ int64_t end_of(int s) const { return sys ? sys_first[s + 1] : n; }
+    __device__ __forceinline__ int cpd_to(int64_t k, uint64_t h, uint64_t l, int s) const {
+        if (sys && (int)sys[k] != s) return -1;
+        return oct_cpd(H(k), L(k), h, l, lev(s));
     }
 };
 
 // First position after i outside the level-L cell that contains i, searched
 // in (i, lim]: exponential probe, then bisection.
-__device__ int64_t cell_end(const OctKeys &K, int64_t i, uint64_t h, uint64_t l, int L, int64_t lim) {
+__device__ int64_t cell_end(const OctKeys &K, int64_t i, uint64_t h, uint64_t l, int sy, int L, int64_t lim) {
     int64_t in = i, out = lim;
     for (int64_t step = 1;; step *= 2) {
         const int64_t k = i + step;
         if (k >= lim) break;
-        if (K.cpd_to(k, h, l) < L) { out = k; break; }
+        if (K.cpd_to(k, h, l, sy) < L) { out = k; break; }
         in = k;
     }
     while (out - in > 1) {
         const int64_t mid = in + (out - in) / 2;
-        if (K.cpd_to(mid, h, l) < L) out = mid;
+        if (K.cpd_to(mid, h, l, sy) < L) out = mid;
         else in = mid;
     }
     return out;
 }
 
 // First position of the level-L cell that contains i.
-__device__ int64_t cell_start(const OctKeys &K, int64_t i, uint64_t h, uint64_t l, int L) {
+__device__ int64_t cell_start(const OctKeys &K, int64_t i, uint64_t h, uint64_t l, int sy, int L) {
     int64_t in = i, out = -1;
     for (int64_t step = 1;; step *= 2) {
         const int64_t k = i - step;
         if (k < 0) break;
-        if (K.cpd_to(k, h, l) < L) { out = k; break; }
+        if (K.cpd_to(k, h, l, sy) < L) { out = k; break; }
         in = k;
     }
     while (in - out > 1) {
         const int64_t mid = out + (in - out) / 2;
-        if (K.cpd_to(mid, h, l) < L) out = mid;
+        if (K.cpd_to(mid, h, l, sy) < L) out = mid;
         else in = mid;
     }
     return in;
 }
 
 __device__ __forceinline__ int first_level(const OctKeys &K, int64_t i) {
-    return i == 0 ? 0 : K.cpd_to(i - 1, K.H(i), K.L(i)) + 1;
+    return i == 0 ? 0 : K.cpd_to(i - 1, K.H(i), K.L(i), K.S(i)) + 1;
 }
 
 // The chain of nodes whose ranges start at position i, shallowest first:
@@ -130,21 +143,23 @@ __device__ __forceinline__ int first_level(const OctKeys &K, int64_t i) {
 // levels 0-2 always exist (Tree::depthSplit(2), Trees/Tree.h:172-194).
 template <typename F>
 __device__ void oct_chain(const OctKeys &K, int64_t i, int leaf, F &&emit) {
-    const int64_t n = K.n;
     const uint64_t h = K.H(i), l = K.L(i);
+    const int sy = K.S(i), levels = K.lev(sy);
+    if (K.leaf_tab) leaf = K.leaf_tab[sy];
+    const int64_t send = K.end_of(sy);
     const int lv0 = first_level(K, i);
-    if (lv0 > K.levels) return;
-    int64_t lim = n;
+    if (lv0 > levels) return;
+    int64_t lim = send;
     if (lv0 > 2) {
-        const int64_t s0 = cell_start(K, i, h, l, lv0 - 1);
-        const int64_t e0 = cell_end(K, i, h, l, lv0 - 1, n);
+        const int64_t s0 = cell_start(K, i, h, l, sy, lv0 - 1);
+        const int64_t e0 = cell_end(K, i, h, l, sy, lv0 - 1, send);
         if (e0 - s0 <= leaf) return;
         lim = e0;
     }
     for (int L = lv0;; ++L) {
-        const int64_t end = L == 0 ? n : cell_end(K, i, h, l, L, lim);
+        const int64_t end = L == 0 ? send : cell_end(K, i, h, l, sy, L, lim);
         const int64_t cnt = end - i;
-        const bool is_leaf = L >= 2 && (cnt <= leaf || L >= K.levels);
+        const bool is_leaf = L >= 2 && (cnt <= leaf || L >= levels);
         emit(L, cnt, is_leaf);
         if (is_leaf) break;
         lim = end;
@@ -172,17 +187,13 @@ __device__ float oct_threshold(float side, float theta) {
 
 // ------------------------------------------------------------------ kernels
 
-template <bool LO>
-__global__ __launch_bounds__(256) void k_oct_keys(const float *__restrict__ x, const float *__restrict__ y,
-                                                  const float *__restrict__ z, int64_t n,
-                                                  const float *__restrict__ bb, int levels, uint64_t *khi,
-                                                  uint64_t *klo, uint64_t *klo_copy, uint32_t *idx) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// The octant path of a point over `levels` levels of the cube of bbox bb.
+__device__ __forceinline__ void oct_path(const float *bb, int levels, float px, float py, float pz, uint64_t &hi,
+                                         uint64_t &lo) {
     float cx, cy, cz, side;
     oct_root(bb, cx, cy, cz, side);
-    const float px = x[i], py = y[i], pz = z[i];
-    uint64_t hi = 0, lo = 0;
+    hi = 0;
+    lo = 0;
     for (int l = 0; l < levels; ++l) {
         // partition<D> (Trees/Octree.h:33-61): p[d] < c[d] goes to the lower half
         const int xb = !(px < cx), yb = !(py < cy), zb = !(pz < cz);
@@ -195,6 +206,17 @@ __global__ __launch_bounds__(256) void k_oct_keys(const float *__restrict__ x, c
         cz = cz + (zb ? off : -off);
         side = cs;
     }
+}
+
+template <bool LO>
+__global__ __launch_bounds__(256) void k_oct_keys(const float *__restrict__ x, const float *__restrict__ y,
+                                                  const float *__restrict__ z, int64_t n,
+                                                  const float *__restrict__ bb, int levels, uint64_t *khi,
+                                                  uint64_t *klo, uint64_t *klo_copy, uint32_t *idx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t hi, lo;
+    oct_path(bb, levels, x[i], y[i], z[i], hi, lo);
     khi[i] = hi;
     if (LO) {
         klo[i] = lo;
@@ -210,6 +232,37 @@ __global__ __launch_bounds__(256) void k_gather_u64(const uint64_t *__restrict__
     if (i >= n) return;
     const uint32_t j = idx[i];
     dst[i] = j < n ? src[j] : 0ull;
+}
+
+// Forest: each body's path over its own system's cube and depth, kept in
+// original order (khi, klo) and copied into the sort buffer (the low word
+// when some system has more than kOctHiLevels levels, else the high word).
+template <bool LO>
+__global__ __launch_bounds__(256) void k_oct_keys_forest(const float *__restrict__ x, const float *__restrict__ y,
+                                                         const float *__restrict__ z, int64_t n,
+                                                         const uint32_t *__restrict__ bsys,
+                                                         const float *__restrict__ bbox6,
+                                                         const int32_t *__restrict__ lv_tab, uint64_t *khi,
+                                                         uint64_t *klo, uint64_t *ksort, uint32_t *idx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t sy = bsys[i];
+    uint64_t hi, lo;
+    oct_path(bbox6 + 6 * (int64_t)sy, lv_tab[sy], x[i], y[i], z[i], hi, lo);
+    khi[i] = hi;
+    if (LO) klo[i] = lo;
+    ksort[i] = LO ? lo : hi;
+    idx[i] = (uint32_t)i;
+}
+
+// dst[i] = system of the body at sorted position i (a sort key)
+__global__ __launch_bounds__(256) void k_gather_sys(const uint32_t *__restrict__ bsys,
+                                                    const uint32_t *__restrict__ idx, int64_t n,
+                                                    uint64_t *__restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = idx[i];
+    dst[i] = j < n ? (uint64_t)bsys[j] : ~0ull;
 }
 
 __global__ __launch_bounds__(256) void k_oct_count(OctKeys K, int leaf, uint32_t *nown) {
@@ -239,7 +292,7 @@ __global__ __launch_bounds__(256) void k_oct_nodes(OctKeys K, int leaf, const ui
             parent = me - 1;
         } else {
             // first node of the chain: its parent's range starts further left
-            const int64_t j = cell_start(K, i, h, l, L - 1);
+            const int64_t j = cell_start(K, i, h, l, K.S(i), L - 1);
             parent = (int64_t)nbase[j] + (L - 1 - first_level(K, j));
         }
         if (me >= cap || parent >= me || (L > 0 && parent < 0)) {
@@ -273,21 +326,34 @@ __global__ void k_oct_levels(const float *__restrict__ bb, int levels, float the
     thr[L] = oct_threshold(side, theta);
 }
 
-// Slot of node id: its parent's first-child slot plus the number of
-// non-empty siblings in lower octants.  -1 if the parent link is invalid
-// (parents precede their children in pre-order).
-__device__ __forceinline__ int64_t oct_slot(int64_t id, const int32_t *__restrict__ nparent,
-                                            const uint32_t *__restrict__ nmeta, const uint32_t *__restrict__ occ,
-                                            const uint32_t *__restrict__ cb) {
-    if (id == 0) return 0;
-    const int32_t p = nparent[id];
-    if (p < 0 || p >= id) return -1;
-    const uint32_t oct = (nmeta[id] >> 8) & 7u;
-    return 1 + (int64_t)cb[p] + __popc(occ[p] & ((1u << oct) - 1u));
+// Forest: block s computes system s's table (thr + kOctThrStride s) from
+// its cube, depth and theta (sys_prm[s].z).
+__global__ void k_oct_levels_forest(const float *__restrict__ bbox6, const int32_t *__restrict__ lv_tab,
+                                    const float4 *__restrict__ sys_prm, float *thr) {
+    const int sy = blockIdx.x, L = threadIdx.x;
+    if (L > lv_tab[sy]) return;
+    float cx, cy, cz, side;
+    oct_root(bbox6 + 6 * sy, cx, cy, cz, side);
+    for (int l = 0; l < L; ++l) side = side / 2.0f;
+    thr[kOctThrStride * sy + L] = oct_threshold(side, sys_prm[sy].z);
 }
 
-__global__ __launch_bounds__(256) void k_oct_emit(const uint32_t *__restrict__ misc, int64_t cap,
-                                                  const float *__restrict__ bb, OctKeys K,
+// Slot of node id of system sy: a root takes slot sy (the roots of the
+// nroots systems come first), any other node its parent's first-child slot
+// plus the number of non-empty siblings in lower octants.  -1 if the parent
+// link is invalid (parents precede their children in pre-order).
+__device__ __forceinline__ int64_t oct_slot(int64_t id, int sy, int nroots, const int32_t *__restrict__ nparent,
+                                            const uint32_t *__restrict__ nmeta, const uint32_t *__restrict__ occ,
+                                            const uint32_t *__restrict__ cb) {
+    const int32_t p = nparent[id];
+    if (p < 0) return ((nmeta[id] & 0xFFu) == 0u && sy < nroots) ? sy : -1;
+    if (p >= id) return -1;
+    const uint32_t oct = (nmeta[id] >> 8) & 7u;
+    return nroots + (int64_t)cb[p] + __popc(occ[p] & ((1u << oct) - 1u));
+}
+
+__global__ __launch_bounds__(256) void k_oct_emit(const uint32_t *__restrict__ misc, int64_t cap, int nroots,
+                                                  const float *__restrict__ bbox6, OctKeys K,
                                                   const float *__restrict__ thr,
                                                   const int32_t *__restrict__ nfirst,
                                                   const int32_t *__restrict__ ncount,
@@ -305,24 +371,29 @@ __global__ __launch_bounds__(256) void k_oct_emit(const uint32_t *__restrict__ m
     const bool is_leaf = (meta >> 12) & 1u;
     const int32_t p = nparent[id];
     const int32_t first = nfirst[id];
-    if ((id == 0) != (p < 0) || p >= id || L > K.levels || first < 0 || first >= K.n) {
+    if ((L == 0) != (p < 0) || p >= id || first < 0 || first >= K.n) {
         atomicOr(err, 8u);
         return;
     }
-    const int64_t slot = oct_slot(id, nparent, nmeta, occ, cb);
-    const int64_t pslot = p < 0 ? -1 : oct_slot(p, nparent, nmeta, occ, cb);
+    const int sy = K.S(first);
+    if (L > K.lev(sy)) {
+        atomicOr(err, 8u);
+        return;
+    }
+    const int64_t slot = oct_slot(id, sy, nroots, nparent, nmeta, occ, cb);
+    const int64_t pslot = p < 0 ? -1 : oct_slot(p, sy, nroots, nparent, nmeta, occ, cb);
     const uint32_t nch = is_leaf ? 0u : (uint32_t)__popc(occ[id]);
-    const int64_t child = is_leaf ? -1 : 1 + (int64_t)cb[id];
+    const int64_t child = is_leaf ? -1 : nroots + (int64_t)cb[id];
     if (slot < 0 || slot >= T || (p >= 0 && pslot < 0) || child + (int64_t)nch > T) {
         atomicOr(err, 8u);
         return;
     }
     float cx, cy, cz, side;
-    oct_cell(bb, K.H(first), K.L(first), L, cx, cy, cz, side);
+    oct_cell(bbox6 + 6 * (int64_t)sy, K.H(first), K.L(first), L, cx, cy, cz, side);
     OctRec r;
     r.com = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     // exclusionRegion(OctreeNode) = [c - side, c + side) (Descent/DescentCriterionType.h:81-85)
-    r.lo = make_float4(cx - side, cy - side, cz - side, thr[L]);
+    r.lo = make_float4(cx - side, cy - side, cz - side, thr[kOctThrStride * (int64_t)sy + L]);
     r.hi = make_float4(cx + side, cy + side, cz + side, __uint_as_float(oct | (nch << 8) | ((uint32_t)L << 16)));
     r.topo = make_int4(first, ncount[id], (int)child, (int)pslot);
     nodes[slot] = r;
@@ -602,7 +673,70 @@ void launch_octree_topology(int64_t n, const float *bbox6, int leaf_size, int ma
     launch_scan_u32(ob.cb, ob.cb, ob.cap, ob.scan_ws, ob.misc + 1, s);
     hipLaunchKernelGGL(k_oct_levels, dim3(1), dim3(64), 0, s, bbox6, levels, theta, ob.thr);
     launch_fill_u32((uint32_t *)leaf_start, 0xFFFFFFFFu, n, s);
-    hipLaunchKernelGGL(k_oct_emit, dim3(gc), dim3(256), 0, s, ob.misc, ob.cap, bbox6, K, ob.thr, ob.nfirst,
+    hipLaunchKernelGGL(k_oct_emit, dim3(gc), dim3(256), 0, s, ob.misc, ob.cap, 1, bbox6, K, ob.thr, ob.nfirst,
+                       ob.ncount, ob.nparent, ob.nmeta, ob.occ, ob.cb, ob.nodes, ob.centre, leaf_start, err);
+    hipLaunchKernelGGL(k_oct_leafsort, dim3(g), dim3(256), 0, s, n, leaf_start, ob.nodes, ob.cap, idx_alt, idx,
+                       err);
+}
+
+void launch_octree_forest_build(const float *x, const float *y, const float *z, int64_t n, int nsys,
+                                const uint32_t *bsys, const float *bbox6, const int32_t *lv_tab, int max_levels,
+                                OctBuffers &ob, uint64_t *keys, uint32_t *idx, uint64_t *keys_alt, uint32_t *idx_alt,
+                                uint32_t *sort_status, uint32_t *err, hipStream_t s) {
+    if (n <= 0 || nsys <= 0) return;
+    const unsigned g = blocks_for(n, 256);
+    const bool lo = max_levels > kOctHiLevels;
+    // paths in original order (ob.key_hi / ob.key_lo) and the first sort key
+    if (lo)
+        hipLaunchKernelGGL(k_oct_keys_forest<true>, dim3(g), dim3(256), 0, s, x, y, z, n, bsys, bbox6, lv_tab,
+                           ob.key_hi, ob.key_lo, keys, idx);
+    else
+        hipLaunchKernelGGL(k_oct_keys_forest<false>, dim3(g), dim3(256), 0, s, x, y, z, n, bsys, bbox6, lv_tab,
+                           ob.key_hi, (uint64_t *)nullptr, keys, idx_alt);
+    if (lo) {
+        // low word (odd pass count): (keys, idx) -> (keys_alt, idx_alt)
+        int passes = (3 * (max_levels - kOctHiLevels) + 7) / 8;
+        if (!(passes & 1)) ++passes;
+        launch_radix_sort_u64(keys, idx, keys_alt, idx_alt, n, passes, sort_status, err, s);
+        if (n == 1) launch_fill_u32(idx_alt, 0u, 1, s);
+        hipLaunchKernelGGL(k_gather_u64, dim3(g), dim3(256), 0, s, ob.key_hi, idx_alt, n, keys);
+    }
+    // high word, 8 passes: (keys, idx_alt) -> (keys, idx_alt)
+    launch_radix_sort_u64(keys, idx_alt, keys_alt, idx, n, 8, sort_status, err, s);
+    // the system above every digit (an even pass count): (keys_alt, idx_alt) -> (keys_alt, idx_alt)
+    hipLaunchKernelGGL(k_gather_sys, dim3(g), dim3(256), 0, s, bsys, idx_alt, n, keys_alt);
+    launch_radix_sort_u64(keys_alt, idx_alt, keys, idx, n, nsys <= 65536 ? 2 : 4, sort_status, err, s);
+    // sorted paths
+    hipLaunchKernelGGL(k_gather_u64, dim3(g), dim3(256), 0, s, ob.key_hi, idx_alt, n, keys);
+    if (lo) hipLaunchKernelGGL(k_gather_u64, dim3(g), dim3(256), 0, s, ob.key_lo, idx_alt, n, ob.lo_sorted);
+}
+
+void launch_octree_forest_topology(int64_t n, int nsys, const float *bbox6, const float4 *sys_prm,
+                                   const int32_t *lv_tab, const int32_t *leaf_tab, const int64_t *sys_first,
+                                   int max_levels, OctBuffers &ob, const uint64_t *keys, const uint64_t *sys_sorted,
+                                   const uint32_t *idx_alt, uint32_t *idx, int32_t *leaf_start, uint32_t *err,
+                                   hipStream_t s) {
+    if (n <= 0 || nsys <= 0) return;
+    OctKeys K;
+    K.hi = keys;
+    K.levels = max_levels;
+    K.lo = max_levels > kOctHiLevels ? ob.lo_sorted : nullptr;
+    K.n = n;
+    K.sys = sys_sorted;
+    K.sys_first = sys_first;
+    K.lv_tab = lv_tab;
+    K.leaf_tab = leaf_tab;
+    const unsigned g = blocks_for(n, 256), gc = blocks_for(ob.cap, 256);
+    hipLaunchKernelGGL(k_oct_count, dim3(g), dim3(256), 0, s, K, 0, ob.nown);
+    launch_scan_u32(ob.nown, ob.nbase, n, ob.scan_ws, ob.misc, s);
+    launch_fill_u32(ob.occ, 0u, ob.cap, s);
+    hipLaunchKernelGGL(k_oct_nodes, dim3(g), dim3(256), 0, s, K, 0, ob.nbase, ob.cap, ob.nfirst, ob.ncount,
+                       ob.nparent, ob.nmeta, ob.occ, err);
+    hipLaunchKernelGGL(k_oct_nchild, dim3(gc), dim3(256), 0, s, ob.misc, ob.cap, ob.nmeta, ob.occ, ob.cb);
+    launch_scan_u32(ob.cb, ob.cb, ob.cap, ob.scan_ws, ob.misc + 1, s);
+    hipLaunchKernelGGL(k_oct_levels_forest, dim3((unsigned)nsys), dim3(64), 0, s, bbox6, lv_tab, sys_prm, ob.thr);
+    launch_fill_u32((uint32_t *)leaf_start, 0xFFFFFFFFu, n, s);
+    hipLaunchKernelGGL(k_oct_emit, dim3(gc), dim3(256), 0, s, ob.misc, ob.cap, nsys, bbox6, K, ob.thr, ob.nfirst,
                        ob.ncount, ob.nparent, ob.nmeta, ob.occ, ob.cb, ob.nodes, ob.centre, leaf_start, err);
     hipLaunchKernelGGL(k_oct_leafsort, dim3(g), dim3(256), 0, s, n, leaf_start, ob.nodes, ob.cap, idx_alt, idx,
                        err);

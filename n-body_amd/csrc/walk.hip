@@ -390,7 +390,7 @@ __device__ __forceinline__ uint64_t oct_eval_ordered(const OctRec &nd, const flo
     return 0;
 }
 
-template <int MAXL, bool QUAD>
+template <int MAXL, bool QUAD, bool FOREST>
 __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_oct_ordered(WalkArgs a) {
     __shared__ int s_node_all[kOrderedWaves][MAXL];
     __shared__ int s_pos_all[kOrderedWaves][MAXL];  // next child index | next octant << 8
@@ -405,25 +405,30 @@ __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_oct_ordered(WalkArg
     int *s_node = s_node_all[wv];
     int *s_pos = s_pos_all[wv];
     uint64_t *s_mask = s_mask_all[wv];
-    const int64_t tgt = ((int64_t)xcd_block(blockIdx.x, gridDim.x) * kOrderedWaves + wv) * kWave + lane;
-    const bool valid = tgt < a.nt;
+    const WaveTask T = wave_task<FOREST>(a, (int64_t)xcd_block(blockIdx.x, gridDim.x) * kOrderedWaves + wv);
+    const int64_t tgt = T.base + lane;
+    const bool valid = tgt < T.end;
     const uint64_t vmask = __ballot(valid);
     if (vmask == 0) return;
+    if (FOREST && (T.root < 0 || T.root >= a.max_nodes)) {
+        if (lane == 0) atomicOr(a.err, 2u);
+        return;
+    }
 
     const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const float px = me.x, py = me.y, pz = me.z;
-    const float G = a.G, eps2 = a.eps2;
+    const float G = T.G, eps2 = T.eps2;
     float rx = 0.0f, ry = 0.0f, rz = 0.0f;
     uint32_t cpp = 0, cnp = 0;
     bool fail = false;
 
     int sp = 0;
     {
-        const OctRec root = nodes[0];
-        const uint64_t O = oct_eval_ordered<QUAD>(root, a.quad, vmask, lane, px, py, pz, G, eps2, bodies, a.n, rx,
-                                                  ry, rz, cpp, cnp, fail);
+        const OctRec root = nodes[T.root];
+        const uint64_t O = oct_eval_ordered<QUAD>(root, a.quad + 2 * T.root, vmask, lane, px, py, pz, G, eps2,
+                                                  bodies, a.n, rx, ry, rz, cpp, cnp, fail);
         if (O) {
-            if (lane == 0) { s_node[0] = 0; s_pos[0] = 0; s_mask[0] = O; }
+            if (lane == 0) { s_node[0] = T.root; s_pos[0] = 0; s_mask[0] = O; }
             sp = 1;
         }
     }
@@ -479,7 +484,9 @@ __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_oct_ordered(WalkArg
 }
 
 
-template <int WAVES, bool SOFT, int STATS, bool QUAD>
+// Single system: wave w walks targets 64 w .. 64 w + 63 from the root;
+// forest: the wave's system's targets from its root (wave_task).
+template <int WAVES, bool SOFT, int STATS, bool QUAD, bool FOREST>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
     __shared__ int s_grp[WAVES][kStackOct];       // first child | nchild << 28
     __shared__ uint64_t s_mask[WAVES][kStackOct];
@@ -487,22 +494,28 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
 
     const float4 *__restrict__ bodies = a.bodies;
     const OctRec *__restrict__ nodes = a.onodes;
-    const int lane = lane_id(), wv = threadIdx.x / kWave;
-    const int64_t tgt = (xcd_block(blockIdx.x, gridDim.x) * WAVES + wv) * kWave + lane;
-    const bool valid = tgt < a.nt;
+    const int lane = lane_id(), wv = uniform_i32(threadIdx.x / kWave);
+    const WaveTask T = wave_task<FOREST>(a, xcd_block(blockIdx.x, gridDim.x) * WAVES + wv);
+    const int64_t tgt = T.base + lane;
+    const bool valid = tgt < T.end;
     const uint64_t vmask = __ballot(valid);
     if (vmask == 0) return;
+    if (FOREST && (T.root < 0 || T.root >= a.max_nodes)) {
+        if (lane == 0) atomicOr(a.err, 2u);
+        return;
+    }
 
     FastWave W;
     const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     fast_init(W, me.x, me.y, me.z);
-    W.G = a.G;
-    const float eps2 = a.eps2;
+    W.G = T.G;
+    const float eps2 = T.eps2;
     const int64_t n = a.n;
     bool fail = false;
 
-    const OctRec root = nodes[0];
-    uint64_t mask = visit_oct<SOFT, STATS, QUAD>(root, a.quad, vmask, bodies, n, eps2, W, fail, s_leaf[wv]);
+    const OctRec root = nodes[T.root];
+    uint64_t mask = visit_oct<SOFT, STATS, QUAD>(root, a.quad + 2 * T.root, vmask, bodies, n, eps2, W, fail,
+                                                 s_leaf[wv]);
     int c = root.topo.z, nch = oct_nchild(root);
     int sp = 0;
     const int64_t max_iters = a.max_nodes + 64;  // each node's group is visited at most once
@@ -564,14 +577,21 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
     }
 }
 
+// Waves the walk launches: one per 64 targets, or the forest's aligned waves.
+static int64_t walk_waves(const WalkArgs &a) { return a.nsys > 0 ? a.nwaves : (a.nt + kWave - 1) / kWave; }
+
 template <int STATS, bool QUAD>
 static void launch_oct_fast_q(const WalkArgs &a, hipStream_t s) {
     constexpr int W = 4;
-    const unsigned blocks = (unsigned)((a.nt + 64 * W - 1) / (64 * W));
-    if (a.eps2 > 0.0f)
-        hipLaunchKernelGGL((k_walk_oct_fast<W, true, STATS, QUAD>), dim3(blocks), dim3(64 * W), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_walk_oct_fast<W, false, STATS, QUAD>), dim3(blocks), dim3(64 * W), 0, s, a);
+    const unsigned blocks = (unsigned)((walk_waves(a) + W - 1) / W);
+    const dim3 g(blocks), t(64 * W);
+    if (a.nsys > 0) {
+        if (a.eps2 > 0.0f) hipLaunchKernelGGL((k_walk_oct_fast<W, true, STATS, QUAD, true>), g, t, 0, s, a);
+        else hipLaunchKernelGGL((k_walk_oct_fast<W, false, STATS, QUAD, true>), g, t, 0, s, a);
+    } else {
+        if (a.eps2 > 0.0f) hipLaunchKernelGGL((k_walk_oct_fast<W, true, STATS, QUAD, false>), g, t, 0, s, a);
+        else hipLaunchKernelGGL((k_walk_oct_fast<W, false, STATS, QUAD, false>), g, t, 0, s, a);
+    }
 }
 
 template <int STATS>
@@ -588,13 +608,23 @@ void launch_walk_oct(const WalkArgs &a, int ordered, int stats, hipStream_t s) {
         else launch_oct_fast<0>(a, s);
     } else {
         constexpr int W = kOrderedWaves;
-        const unsigned blocks = (unsigned)((a.nt + 64 * W - 1) / (64 * W));
-        if (a.quad) {
-            if (a.levels <= 18) hipLaunchKernelGGL((k_walk_oct_ordered<18, true>), dim3(blocks), dim3(64 * W), 0, s, a);
-            else hipLaunchKernelGGL((k_walk_oct_ordered<kOctLevels, true>), dim3(blocks), dim3(64 * W), 0, s, a);
+        const unsigned blocks = (unsigned)((walk_waves(a) + W - 1) / W);
+        const dim3 g(blocks), t(64 * W);
+        const bool shallow = a.levels <= 18;  // a.levels: the deepest member's levels in a forest
+        if (a.nsys > 0) {
+            if (a.quad) {
+                if (shallow) hipLaunchKernelGGL((k_walk_oct_ordered<18, true, true>), g, t, 0, s, a);
+                else hipLaunchKernelGGL((k_walk_oct_ordered<kOctLevels, true, true>), g, t, 0, s, a);
+            } else {
+                if (shallow) hipLaunchKernelGGL((k_walk_oct_ordered<18, false, true>), g, t, 0, s, a);
+                else hipLaunchKernelGGL((k_walk_oct_ordered<kOctLevels, false, true>), g, t, 0, s, a);
+            }
+        } else if (a.quad) {
+            if (shallow) hipLaunchKernelGGL((k_walk_oct_ordered<18, true, false>), g, t, 0, s, a);
+            else hipLaunchKernelGGL((k_walk_oct_ordered<kOctLevels, true, false>), g, t, 0, s, a);
         } else {
-            if (a.levels <= 18) hipLaunchKernelGGL((k_walk_oct_ordered<18, false>), dim3(blocks), dim3(64 * W), 0, s, a);
-            else hipLaunchKernelGGL((k_walk_oct_ordered<kOctLevels, false>), dim3(blocks), dim3(64 * W), 0, s, a);
+            if (shallow) hipLaunchKernelGGL((k_walk_oct_ordered<18, false, false>), g, t, 0, s, a);
+            else hipLaunchKernelGGL((k_walk_oct_ordered<kOctLevels, false, false>), g, t, 0, s, a);
         }
     }
 }
@@ -684,8 +714,6 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     }
 }
 
-// Waves the walk launches: one per 64 targets, or the forest's aligned waves.
-static int64_t walk_waves(const WalkArgs &a) { return a.nsys > 0 ? a.nwaves : (a.nt + kWave - 1) / kWave; }
 
 template <int STATS, bool QUAD>
 static void launch_fast_single(const WalkArgs &a, unsigned blocks, hipStream_t s) {

@@ -1,10 +1,12 @@
-"""Batched forest: independent linear-BVH systems built and walked together.
+"""Batched forest: independent systems built and walked together.
 
 Every system's result must be the bits it gets alone (an nbg_sys with the
-same parameters, 30-bit keys, same sum mode) — in both sum modes, because
-each wavefront serves one system and each system's subtree, summaries and
+same parameters, same sum mode) — in both sum modes, because each
+wavefront serves one system and each system's tree, summaries and
 traversal order are its own — and, in ORDERED mode, the oracle's bits.
-The same holds for a batch of quadrupole linear-BVH systems (solver 4),
+This holds for all four tree solvers a batch runs: the linear BVH with
+30-bit keys (solver 1) and the Barnes-Hut octree (solver 0, BarnesHutSolver,
+Solvers/BarnesHutSolver.h:14-32), each also in its quadrupole form (4, 3),
 whose node second moments are summarized with the forest."""
 
 import numpy as np
@@ -47,7 +49,7 @@ def _alone(nbg, ctx, b, kw, mode, steps=0, solver=1):
     return a, st, c
 
 
-@pytest.mark.parametrize("solver", [1, 4])
+@pytest.mark.parametrize("solver", [1, 4, 0, 3])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_forest_pass_equals_alone_and_oracle(nbg, orc, gpu_ctx, mode, solver):
     systems = _systems(nbg)
@@ -70,7 +72,7 @@ def test_forest_pass_equals_alone_and_oracle(nbg, orc, gpu_ctx, mode, solver):
     B.close()
 
 
-@pytest.mark.parametrize("mode,solver", [(0, 1), (1, 1), (0, 4), (1, 4)])
+@pytest.mark.parametrize("mode,solver", [(0, 1), (1, 1), (0, 4), (1, 4), (0, 0), (1, 0), (1, 3)])
 def test_forest_membership_changes(nbg, gpu_ctx, mode, solver):
     """Systems join and leave between steps; the others' trajectories are
     unaffected (same bits as alone with the same number of steps)."""
@@ -103,7 +105,7 @@ def test_forest_membership_changes(nbg, gpu_ctx, mode, solver):
     B.close()
 
 
-@pytest.mark.parametrize("solver", [1, 4])
+@pytest.mark.parametrize("solver", [1, 4, 0, 3])
 def test_forest_graph_replay_equals_eager(nbg, gpu_ctx, solver):
     systems = _systems(nbg)[:6]
     runs = []
@@ -123,10 +125,11 @@ def test_forest_graph_replay_equals_eager(nbg, gpu_ctx, solver):
             assert np.array_equal(a, b)
 
 
-def test_forest_autotune(nbg, gpu_ctx):
+@pytest.mark.parametrize("solver", [1, 0])
+def test_forest_autotune(nbg, gpu_ctx, solver):
     systems = _systems(nbg)
     B = nbg.Batch(gpu_ctx, 40000, 16)
-    ids = [B.add(_params(nbg, **kw), b) for b, kw in systems]
+    ids = [B.add(_params(nbg, solver=solver, **kw), b) for b, kw in systems]
     before = [B.get(sid).x.copy() for sid in ids]
     best, ms = B.autotune((8, 16, 32, 64), repeats=2)
     print(f"[forest autotune] best leaf {best}, ms {ms}", flush=True)
@@ -137,7 +140,7 @@ def test_forest_autotune(nbg, gpu_ctx):
     b, kw = systems[4]
     _, acc = B.get(ids[4], accelerations=True)
     kw = dict(kw, leaf=best)
-    a1, _, _ = _alone(nbg, gpu_ctx, b, kw, 1)
+    a1, _, _ = _alone(nbg, gpu_ctx, b, kw, 1, solver=solver)
     assert np.array_equal(acc, a1)
     B.close()
 
@@ -151,7 +154,10 @@ def test_forest_capacity_and_arguments(nbg, gpu_ctx):
     p = _params(nbg)
     p.solver = nbg.BARNES_HUT
     with pytest.raises(nbg.NbgError):
-        B.add(p, nbg.plummer(10, seed=2))  # octree systems are not batched
+        B.add(p, nbg.plummer(10, seed=2))  # members share one solver (octree and linear BVH)
+    p.solver = nbg.IMPLICIT_FMM
+    with pytest.raises(nbg.NbgError):
+        B.add(p, nbg.plummer(10, seed=2))  # the implicit FMM is not batched
     with pytest.raises(nbg.NbgError):
         B.add(_params(nbg, solver=4), nbg.plummer(10, seed=3))  # members share one solver
     B.add(_params(nbg), nbg.plummer(10, seed=3))
@@ -162,8 +168,9 @@ def test_forest_capacity_and_arguments(nbg, gpu_ctx):
 
 def test_forest_solver_kind_changes(nbg, orc, gpu_ctx):
     """A batch's solver follows its members: a sole member switched between
-    the monopole and quadrupole kinds, and a batch emptied and refilled with
-    the other kind, give each kind's bits alone and the oracle's."""
+    the linear-BVH and octree, monopole and quadrupole kinds, and a batch
+    emptied and refilled with another kind, give each kind's bits alone and
+    the oracle's."""
     b1, b2 = nbg.plummer(3000, seed=71), nbg.plummer(2000, seed=72, a=1.5)
     kw = dict(theta=0.5, leaf=16, eps2=1e-4, G=1.0, dt=1e-3)
     B = nbg.Batch(gpu_ctx, 8000, 4, sum_mode=0)
@@ -178,7 +185,7 @@ def test_forest_solver_kind_changes(nbg, orc, gpu_ctx):
 
     sid = B.add(_params(nbg, mode=0, **kw), b1)
     check(sid, b1, 1)
-    for solver in (4, 1, 4):
+    for solver in (4, 0, 3, 1, 4):
         B.set_params(sid, _params(nbg, mode=0, solver=solver, **kw))
         check(sid, b1, solver)
     other = B.add(_params(nbg, mode=0, solver=4, **kw), b2)
@@ -192,7 +199,7 @@ def test_forest_solver_kind_changes(nbg, orc, gpu_ctx):
     B.close()
 
 
-@pytest.mark.parametrize("mode,solver", [(0, 1), (1, 1), (0, 4)])
+@pytest.mark.parametrize("mode,solver", [(0, 1), (1, 1), (0, 4), (0, 0), (1, 0)])
 def test_forest_large_equals_alone(nbg, gpu_ctx, mode, solver):
     """A forest above 2M bodies (the summaries then run their middle,
     many-workgroup pass, whose ranges cut through systems): every system
