@@ -205,6 +205,7 @@ def measure(nbg, ctx, b, args, solver, mode=None):
     p.theta, p.leaf_size, p.morton_bits = solver_theta(args, solver), args.leaf, args.bits
     p.eps2, p.dt = args.eps2, args.dt
     p.sum_mode = nbg.SUM_FAST if mode == "fast" else nbg.SUM_ORDERED
+    p.targets_per_lane = getattr(args, "tpl", 1)
     s = nbg.System(ctx, p, args.n)
     s.set_state(b)
     a0 = s.update_accelerations()  # force pass at t = 0
@@ -338,16 +339,22 @@ def bench_forest(nbg, ctx, args):
     systems = ensemble(nbg, args.forest_systems, 256, 65536, args.forest_seed)
     total = sum(b.n for b, _ in systems)
     p = nbg.default_params()
-    quad = args.forest_solver == "linear-bvh-quadrupole"
-    p.solver = nbg.LINEAR_BVH_QUADRUPOLE if quad else nbg.LINEAR_BVH
-    p.morton_bits, p.theta, p.leaf_size = 30, 0.5, 16
+    solver = args.forest_solver
+    quad = solver.endswith("quadrupole")
+    octree = solver.startswith("barnes-hut")
+    p.solver = {"linear-bvh": nbg.LINEAR_BVH, "linear-bvh-quadrupole": nbg.LINEAR_BVH_QUADRUPOLE,
+                "barnes-hut": nbg.BARNES_HUT, "barnes-hut-quadrupole": nbg.BARNES_HUT_QUADRUPOLE}[solver]
+    p.morton_bits, p.theta, p.leaf_size, p.max_depth = 30, 0.5, 16, 32
     p.eps2, p.dt, p.sum_mode = 1e-4, 1e-3, nbg.SUM_FAST
     B = nbg.Batch(ctx, total, len(systems), sum_mode=nbg.SUM_FAST)
     ids = [B.add(p, b) for b, _ in systems]
     B.enable_counts(False)
-    log(f"[bench] forest: {len(systems)} systems, {total} bodies; autotuning leaf size")
-    best, tune_ms = B.autotune((8, 16, 32, 64), repeats=2)
+    log(f"[bench] forest ({solver}): {len(systems)} systems, {total} bodies; autotuning leaf size x bodies per "
+        f"wavefront")
+    best, best_tpl, tune = B.autotune_grid((8, 16, 32, 64), (1, 2, 4), repeats=2)
+    tune_ms = {f"leaf {l}, {64 * t} bodies/wavefront": v for (l, t), v in tune.items()}
     p.leaf_size = best
+    p.targets_per_lane = best_tpl
     # accuracy sample: each sampled system's first pass and energy, alone
     # (the forest gives each system the bits it gets alone)
     acc_sample = np.random.default_rng(3).choice(len(systems), min(args.forest_accuracy_sample, len(systems)),
@@ -407,7 +414,7 @@ def bench_forest(nbg, ctx, args):
     return {
         "workload": f"{len(systems)} Plummer systems, N log-uniform in [256, 65536] (total {total}), a ~ U[0.5, 2], "
                     f"M = 1, eps2 = 1e-4, theta = 0.5, dt = 1e-3, {args.forest_steps} steps, "
-                    f"{'quadrupole ' if quad else ''}linear BVH 30-bit keys, "
+                    f"{'quadrupole ' if quad else ''}{'Barnes-Hut octree' if octree else 'linear BVH 30-bit keys'}, "
                     f"FAST sums, seed {args.forest_seed}",
         "value": value,
         "unit": "bodies·steps/s",
@@ -415,6 +422,7 @@ def bench_forest(nbg, ctx, args):
         "systems": len(systems),
         "bodies": total,
         "leaf_size_autotuned": best,
+        "bodies_per_wavefront_autotuned": 64 * best_tpl,
         "autotune_ms_per_pass": tune_ms,
         "interactions_per_step": pp + npc,
         "interactions_per_s": (pp + npc) * args.forest_steps / el,
@@ -711,6 +719,8 @@ def main():
     ap.add_argument("--dt", type=float, default=5e-4)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--mode", choices=["fast", "ordered"], default="fast")
+    ap.add_argument("--tpl", type=int, default=1, choices=(1, 2, 4),
+                    help="targets per lane of the FAST linear-BVH walk (64 x this many bodies per wavefront)")
     ap.add_argument("--solvers", default="linear-bvh,barnes-hut,linear-bvh-4p,barnes-hut-4p,implicit-fmm",
                     help="comma list; the first is the headline line")
     ap.add_argument("--theta-4p-lbvh", type=float, default=0.72578126,
@@ -730,7 +740,8 @@ def main():
     ap.add_argument("--forest-systems", type=int, default=4096)
     ap.add_argument("--forest-steps", type=int, default=20)
     ap.add_argument("--forest-seed", type=int, default=7)
-    ap.add_argument("--forest-solver", choices=("linear-bvh", "linear-bvh-quadrupole"), default="linear-bvh",
+    ap.add_argument("--forest-solver", choices=("linear-bvh", "linear-bvh-quadrupole", "barnes-hut",
+                                                "barnes-hut-quadrupole"), default="linear-bvh",
                     help="solver of every system in the batched ensemble")
     ap.add_argument("--forest-alone-sample", type=int, default=64)
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
@@ -845,12 +856,20 @@ def main():
         except Exception as e:
             c3 = {"error": f"{type(e).__name__}: {e}"}
 
-    forest = None
+    forest = forest_oct = None
     if not args.no_forest:
         try:
             forest = bench_forest(nbg, ctx, args)
         except Exception as e:  # reported, not fatal for the headline line
             forest = {"error": f"{type(e).__name__}: {e}"}
+        if args.forest_solver == "linear-bvh":
+            # the same ensemble as Barnes-Hut octree members
+            a2 = copy.copy(args)
+            a2.forest_solver = "barnes-hut"
+            try:
+                forest_oct = bench_forest(nbg, ctx, a2)
+            except Exception as e:
+                forest_oct = {"error": f"{type(e).__name__}: {e}"}
 
     serving = None
     if not args.no_serving:
@@ -894,6 +913,8 @@ def main():
         out["two_spheres"] = c3
     if forest is not None:
         out["batched_ensemble"] = forest
+    if forest_oct is not None:
+        out["batched_ensemble_barnes_hut"] = forest_oct
     if serving is not None:
         out["serving"] = serving
     ctx.close()

@@ -83,6 +83,9 @@ typedef struct nbg_params {
     float eps2;          /* Gravity "epsilon", added to r^2; default 1e-5 (Gravity.h:195,207) */
     float dt;            /* Solver::timeStep(), default 1e-3 (Simulation/Solver.h:34,101) */
     int32_t sum_mode;    /* nbg_sum_mode */
+    int32_t targets_per_lane; /* FAST walks: bodies each lane walks the tree for, 1, 2 or 4 (0 reads as 1);
+                                 a wavefront serves 64 x this many bodies (no reference counterpart:
+                                 a schedule choice that leaves every result's interaction set unchanged) */
 } nbg_params;
 
 typedef struct nbg_ctx nbg_ctx;
@@ -225,6 +228,30 @@ int nbg_sys_debug_tree(nbg_sys *sys, int64_t *num_nodes, int64_t *first, int64_t
  * in nbg_sys_debug_tree's node order; q6 holds cap nodes. */
 int nbg_sys_debug_quads(nbg_sys *sys, int64_t cap, float *q6);
 
+/* ---- scenario files (the reference's formats) ----
+ *
+ * Simulation::from_json (src/NBody/Simulation/Simulation.cpp:154-228) and
+ * Simulation::from_tipsy (:273-395), read into host arrays in file order.
+ * Per particle, flags: NBG_SCENARIO_POSITION / _VELOCITY (given in the file),
+ * _ACTIVE (has a Mass: a source, goes to nbg_sys_set_state) and _PASSIVE (has
+ * an Acceleration: receives forces; a passive particle without a mass goes
+ * to nbg_sys_accelerations_at).  JSON: "passive": false drops the
+ * Acceleration unless the file gives one (Simulation.cpp:208-211).  Tipsy:
+ * the stars, or the gas when there are no stars, never the dark matter; all
+ * active and passive; either byte order. */
+#define NBG_SCENARIO_POSITION 1
+#define NBG_SCENARIO_VELOCITY 2
+#define NBG_SCENARIO_ACTIVE 4
+#define NBG_SCENARIO_PASSIVE 8
+typedef struct nbg_scenario nbg_scenario;
+int nbg_scenario_load_json(const char *path, nbg_scenario **out);
+int nbg_scenario_load_tipsy(const char *path, nbg_scenario **out);
+int64_t nbg_scenario_count(const nbg_scenario *s);
+/* Copies the particles out; any array may be NULL (n entries each). */
+int nbg_scenario_get(const nbg_scenario *s, float *x, float *y, float *z, float *vx, float *vy, float *vz, float *m,
+                     uint8_t *flags);
+int nbg_scenario_free(nbg_scenario *s);
+
 /* ---- direct summation (NaiveSolver, Solvers/NaiveSolver.h:22-52) ---- */
 
 /* FP64 direct sum on the device: accelerations at nt target points from ns
@@ -308,6 +335,20 @@ int nbg_batch_use_graph(nbg_batch *b, int32_t on);
  * pass time of candidate k. */
 int nbg_batch_autotune(nbg_batch *b, const int32_t *leaf_sizes, int32_t k, int32_t repeats, int32_t *best,
                        float *ms_out);
+
+/* The autotuner over leaf size x bodies per wavefront: times every pair
+ * (leaf_sizes[i], targets per lane tpls[j] in {1, 2, 4}; a wavefront serves
+ * 64 x tpl bodies) as above and keeps the fastest pair.  ms_out[i * kt + j]
+ * (optional) receives each mean pass time, -1 for pairs the batch's walk
+ * does not run (more than one target per lane exists for the FAST
+ * linear-BVH walk only).  The traversal stacks need no tuning: each walk's
+ * stack holds the deepest path its trees can have. */
+int nbg_batch_autotune_grid(nbg_batch *b, const int32_t *leaf_sizes, int32_t kl, const int32_t *tpls, int32_t kt,
+                            int32_t repeats, int32_t *best_leaf, int32_t *best_tpl, float *ms_out);
+
+/* Targets per lane of the batch's FAST linear-BVH walk (1, 2 or 4); the
+ * autotuner sets it too.  Results do not depend on it. */
+int nbg_batch_set_targets_per_lane(nbg_batch *b, int32_t tpl);
 
 /* ---- serving: continuous batching over a forest ----
  *

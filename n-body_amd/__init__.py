@@ -47,7 +47,8 @@ EXPORTS = (
     "nbg_batch_bytes", "nbg_batch_create", "nbg_batch_destroy", "nbg_batch_add", "nbg_batch_remove",
     "nbg_batch_set_params", "nbg_batch_info", "nbg_batch_get", "nbg_batch_update_accelerations", "nbg_batch_step",
     "nbg_batch_synchronize", "nbg_batch_counts", "nbg_batch_enable_counts", "nbg_batch_use_graph",
-    "nbg_batch_autotune", "nbg_batch_nonfinite",
+    "nbg_batch_autotune", "nbg_batch_autotune_grid", "nbg_batch_set_targets_per_lane", "nbg_batch_nonfinite",
+    "nbg_scenario_load_json", "nbg_scenario_load_tipsy", "nbg_scenario_count", "nbg_scenario_get", "nbg_scenario_free",
     "nbg_server_create", "nbg_server_destroy", "nbg_server_capacity", "nbg_server_submit", "nbg_server_wait",
     "nbg_server_stats",
 )
@@ -74,6 +75,7 @@ class Params(ctypes.Structure):
         ("eps2", ctypes.c_float),
         ("dt", ctypes.c_float),
         ("sum_mode", ctypes.c_int32),
+        ("targets_per_lane", ctypes.c_int32),
     ]
 
 
@@ -132,6 +134,14 @@ def lib() -> ctypes.CDLL:
             "nbg_batch_use_graph": (ctypes.c_int, [_vp, ctypes.c_int32]),
             "nbg_batch_autotune": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, ctypes.c_int32,
                                                   ctypes.POINTER(ctypes.c_int32), _f32p]),
+            "nbg_batch_autotune_grid": (ctypes.c_int, [_vp, _i32p, ctypes.c_int32, _i32p, ctypes.c_int32,
+                                                       ctypes.c_int32, _i32p, _i32p, _f32p]),
+            "nbg_batch_set_targets_per_lane": (ctypes.c_int, [_vp, ctypes.c_int32]),
+            "nbg_scenario_load_json": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
+            "nbg_scenario_load_tipsy": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
+            "nbg_scenario_count": (ctypes.c_int64, [_vp]),
+            "nbg_scenario_get": (ctypes.c_int, [_vp] + [_f32p] * 7 + [ctypes.POINTER(ctypes.c_uint8)]),
+            "nbg_scenario_free": (ctypes.c_int, [_vp]),
             "nbg_server_create": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                                  ctypes.POINTER(_vp)]),
             "nbg_server_destroy": (ctypes.c_int, [_vp]),
@@ -499,6 +509,23 @@ class Batch:
     def use_graph(self, on: bool = True):
         _check(lib().nbg_batch_use_graph(self.handle, int(on)))
 
+    def autotune_grid(self, leaf_sizes=(8, 16, 32, 64), tpls=(1, 2, 4), repeats: int = 2):
+        """Leaf size x targets per lane autotuner; returns (best leaf, best
+        targets per lane, {(leaf, tpl): ms}) with untimed pairs left out."""
+        ls = np.asarray(leaf_sizes, np.int32)
+        ts = np.asarray(tpls, np.int32)
+        ms = np.zeros(len(ls) * len(ts), np.float32)
+        bl, bt = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().nbg_batch_autotune_grid(self.handle, ls.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(ls),
+                                             ts.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(ts), repeats,
+                                             ctypes.byref(bl), ctypes.byref(bt), _ptr(ms)))
+        table = {(int(l), int(t)): float(ms[i * len(ts) + j]) for i, l in enumerate(ls) for j, t in enumerate(ts)
+                 if ms[i * len(ts) + j] >= 0}
+        return bl.value, bt.value, table
+
+    def set_targets_per_lane(self, tpl: int):
+        _check(lib().nbg_batch_set_targets_per_lane(self.handle, int(tpl)))
+
     def autotune(self, leaf_sizes=(8, 16, 32, 64), repeats: int = 2):
         ls = np.asarray(leaf_sizes, np.int32)
         ms = np.zeros(len(ls), np.float32)
@@ -506,6 +533,26 @@ class Batch:
         _check(lib().nbg_batch_autotune(self.handle, ls.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(ls),
                                         repeats, ctypes.byref(best), _ptr(ms)))
         return best.value, {int(a): float(t) for a, t in zip(ls, ms)}
+
+
+SCENARIO_POSITION, SCENARIO_VELOCITY, SCENARIO_ACTIVE, SCENARIO_PASSIVE = 1, 2, 4, 8
+
+
+def load_scenario(path: str, kind: str = "json"):
+    """A reference scenario file read by the library's C reader
+    (nbg_scenario_load_json / _tipsy): (x, y, z, vx, vy, vz, m, flags)
+    arrays in file order."""
+    h = _vp()
+    fn = lib().nbg_scenario_load_json if kind == "json" else lib().nbg_scenario_load_tipsy
+    _check(fn(os.fsencode(path), ctypes.byref(h)))
+    try:
+        n = lib().nbg_scenario_count(h)
+        arrs = [np.zeros(n, np.float32) for _ in range(7)]
+        flags = np.zeros(n, np.uint8)
+        _check(lib().nbg_scenario_get(h, *[_ptr(a) for a in arrs], flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+    finally:
+        lib().nbg_scenario_free(h)
+    return (*arrs, flags)
 
 
 class Server:

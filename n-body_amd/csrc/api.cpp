@@ -115,6 +115,8 @@ int check_params(const nbg_params *p) {
     if (p->sum_mode != NBG_SUM_ORDERED && p->sum_mode != NBG_SUM_FAST) return fail(NBG_ERR_ARG, "unknown sum mode");
     if (p->solver == NBG_IMPLICIT_FMM && p->sum_mode != NBG_SUM_FAST)
         return fail(NBG_ERR_ARG, "the implicit FMM runs in the FAST sum mode only");
+    if (p->targets_per_lane != 0 && p->targets_per_lane != 1 && p->targets_per_lane != 2 && p->targets_per_lane != 4)
+        return fail(NBG_ERR_ARG, "targets_per_lane must be 1, 2 or 4");
     return NBG_OK;
 }
 
@@ -388,6 +390,7 @@ WalkArgs walk_args(nbg_sys *s) {
     wa.counts = s->counts();
     wa.err = s->err_word();
     wa.quad = is_quad(p.solver) ? s->quad : nullptr;
+    wa.tpl = p.targets_per_lane > 1 ? p.targets_per_lane : 1;
     return wa;
 }
 
@@ -592,6 +595,7 @@ void nbg_params_default(nbg_params *p) {
     p->eps2 = 0.00001f;
     p->dt = 0.001f;
     p->sum_mode = NBG_SUM_ORDERED;
+    p->targets_per_lane = 1;
 }
 
 const char *nbg_last_error(void) { return g_err.c_str(); }

@@ -91,6 +91,9 @@ struct nbg_batch {
     OctBuffers ob;
     int32_t *lv_tab = nullptr;
     int max_levels = 0;
+    // targets per lane of the FAST linear-BVH walk (the autotuner's
+    // bodies-per-wavefront choice: 64 x tpl); the other walks use 1
+    int32_t tpl = 1;
 
     float *x() { return st[0]; }
     float *y() { return st[1]; }
@@ -124,9 +127,6 @@ int64_t bytes_for(int64_t cap_n, int64_t cap_s) {
 }
 
 bool is_octree_solver(int32_t solver) { return solver == NBG_BARNES_HUT || solver == NBG_BARNES_HUT_QUADRUPOLE; }
-bool is_quad_solver(int32_t solver) {
-    return solver == NBG_LINEAR_BVH_QUADRUPOLE || solver == NBG_BARNES_HUT_QUADRUPOLE;
-}
 
 void free_octree(OctBuffers &o) {
     void *ptrs[] = {o.key_hi, o.key_lo, o.lo_sorted, o.nown, o.nbase, o.nfirst, o.ncount, o.nparent, o.nmeta,
@@ -170,6 +170,12 @@ int check_member_params(const nbg_params *p) {
 
 // Members share one solver; `self` (or null) is the member being changed.
 // A quadrupole batch allocates its node moments here, outside any pass.
+// Targets per lane the walk of this batch uses.
+int walk_tpl(const nbg_batch *b) {
+    const bool lbvh = b->solver == NBG_LINEAR_BVH || b->solver == NBG_LINEAR_BVH_QUADRUPOLE;
+    return lbvh && b->sum_mode == NBG_SUM_FAST ? b->tpl : 1;
+}
+
 int ensure_quad(nbg_batch *b, int64_t slots) {
     if (b->quad && b->quad_slots >= slots) return NBG_OK;
     NBG_HIP(hipStreamSynchronize(b->stream));
@@ -240,13 +246,14 @@ int upload_tables(nbg_batch *b) {
     b->all_soft = true;
     b->max_levels = 0;
     if (S) b->solver = b->sys[0].p.solver;
+    const int64_t width = 64 * (int64_t)walk_tpl(b);  // targets per wave
     for (size_t i = 0; i < S; ++i) {
         Member &m = b->sys[i];
         m.first = off;
         first[i] = off;
         waves[i] = w;
         off += m.n;
-        w += (m.n + 63) / 64;
+        w += (m.n + width - 1) / width;
         leaf[i] = m.p.leaf_size;
         levels[i] = m.p.max_depth + 2;
         if (levels[i] > b->max_levels) b->max_levels = levels[i];
@@ -378,6 +385,7 @@ void enqueue_pass(nbg_batch *b) {
     wa.sys_root = b->sys_root;
     wa.sys_prm = b->sys_prm;
     wa.quad = quad;
+    wa.tpl = walk_tpl(b);
     const bool ordered = b->sum_mode == NBG_SUM_ORDERED;
     launch_walk_lbvh(wa, ordered, 32, (b->counting || ordered) ? 1 : 0, s);
 }
@@ -681,11 +689,14 @@ int nbg_batch_use_graph(nbg_batch *b, int32_t on) {
     return NBG_OK;
 }
 
-int nbg_batch_autotune(nbg_batch *b, const int32_t *leaf_sizes, int32_t k, int32_t repeats, int32_t *best,
-                       float *ms_out) {
-    if (!b || !leaf_sizes || k < 1 || repeats < 1) return fail(NBG_ERR_ARG, "bad autotune arguments");
-    for (int i = 0; i < k; ++i)
+int nbg_batch_autotune_grid(nbg_batch *b, const int32_t *leaf_sizes, int32_t kl, const int32_t *tpls, int32_t kt,
+                            int32_t repeats, int32_t *best_leaf, int32_t *best_tpl, float *ms_out) {
+    if (!b || !leaf_sizes || kl < 1 || !tpls || kt < 1 || repeats < 1)
+        return fail(NBG_ERR_ARG, "bad autotune arguments");
+    for (int i = 0; i < kl; ++i)
         if (leaf_sizes[i] < 1) return fail(NBG_ERR_ARG, "leaf sizes must be >= 1");
+    for (int j = 0; j < kt; ++j)
+        if (tpls[j] != 1 && tpls[j] != 2 && tpls[j] != 4) return fail(NBG_ERR_ARG, "targets per lane: 1, 2 or 4");
     if (b->sys.empty()) return fail(NBG_ERR_STATE, "the batch is empty");
     DeviceGuard g(b->ctx->device);
     hipEvent_t e0, e1;
@@ -694,31 +705,56 @@ int nbg_batch_autotune(nbg_batch *b, const int32_t *leaf_sizes, int32_t k, int32
     const bool counting = b->counting;
     b->counting = false;
     float best_ms = 0.0f;
-    int32_t best_leaf = leaf_sizes[0];
+    bool have = false;
+    int32_t bl = leaf_sizes[0], bt = b->tpl;
     int r = NBG_OK;
-    for (int i = 0; i < k && r == NBG_OK; ++i) {
-        for (Member &mm : b->sys) mm.p.leaf_size = leaf_sizes[i];
-        b->tables_dirty = true;
-        if ((r = upload_tables(b))) break;
-        enqueue_pass(b);  // warm-up
-        (void)hipEventRecord(e0, b->stream);
-        for (int t = 0; t < repeats; ++t) enqueue_pass(b);
-        (void)hipEventRecord(e1, b->stream);
-        if ((r = sync_check(b))) break;
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        ms /= (float)repeats;
-        if (ms_out) ms_out[i] = ms;
-        if (i == 0 || ms < best_ms) { best_ms = ms; best_leaf = leaf_sizes[i]; }
+    for (int i = 0; i < kl && r == NBG_OK; ++i) {
+        for (int j = 0; j < kt && r == NBG_OK; ++j) {
+            if (ms_out) ms_out[i * kt + j] = -1.0f;
+            b->tpl = tpls[j];
+            // a walk without a multi-target form runs one target per lane:
+            // its other candidates would time the same schedule again
+            if (walk_tpl(b) != tpls[j]) continue;
+            for (Member &mm : b->sys) mm.p.leaf_size = leaf_sizes[i];
+            b->tables_dirty = true;
+            if ((r = upload_tables(b))) break;
+            enqueue_pass(b);  // warm-up
+            (void)hipEventRecord(e0, b->stream);
+            for (int t = 0; t < repeats; ++t) enqueue_pass(b);
+            (void)hipEventRecord(e1, b->stream);
+            if ((r = sync_check(b))) break;
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            ms /= (float)repeats;
+            if (ms_out) ms_out[i * kt + j] = ms;
+            if (!have || ms < best_ms) { best_ms = ms; bl = leaf_sizes[i]; bt = tpls[j]; have = true; }
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     b->counting = counting;
-    for (Member &mm : b->sys) mm.p.leaf_size = best_leaf;
+    for (Member &mm : b->sys) mm.p.leaf_size = bl;
+    b->tpl = bt;
     b->tables_dirty = true;
     b->have_forces = false;
-    if (best) *best = best_leaf;
+    if (best_leaf) *best_leaf = bl;
+    if (best_tpl) *best_tpl = bt;
     return r;
+}
+
+int nbg_batch_autotune(nbg_batch *b, const int32_t *leaf_sizes, int32_t k, int32_t repeats, int32_t *best,
+                       float *ms_out) {
+    if (!b) return fail(NBG_ERR_ARG, "batch is NULL");
+    const int32_t tpl = b->tpl;
+    return nbg_batch_autotune_grid(b, leaf_sizes, k, &tpl, 1, repeats, best, nullptr, ms_out);
+}
+
+int nbg_batch_set_targets_per_lane(nbg_batch *b, int32_t tpl) {
+    if (!b) return fail(NBG_ERR_ARG, "batch is NULL");
+    if (tpl != 1 && tpl != 2 && tpl != 4) return fail(NBG_ERR_ARG, "targets per lane: 1, 2 or 4");
+    b->tpl = tpl;
+    b->tables_dirty = true;
+    return NBG_OK;
 }
 
 }  // extern "C"

@@ -261,6 +261,9 @@ struct WalkArgs {
     // Quadrupole trees: second moments per node slot (2 float4, layout in
     // device.hpp); null for monopole trees.
     const float4 *quad = nullptr;
+    // FAST single-system walks: targets per lane (1, 2 or 4); a wavefront
+    // serves 64 tpl consecutive targets, lane l of slot t target 64 t + l.
+    int tpl = 1;
 };
 
 // Barnes-Hut traversal over the LBVH.  ordered != 0 selects the

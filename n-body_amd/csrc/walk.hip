@@ -44,11 +44,13 @@ struct WaveTask {
     float G, eps2;
 };
 
-template <bool FOREST>
+// Waves are 64 TPL targets wide (TPL targets per lane); a forest's wave
+// tables count waves of that width.
+template <bool FOREST, int TPL = 1>
 __device__ __forceinline__ WaveTask wave_task(const WalkArgs &a, int64_t w) {
     WaveTask t;
     if (!FOREST) {
-        t.base = w * kWave; t.end = a.nt; t.root = 0; t.G = a.G; t.eps2 = a.eps2;
+        t.base = w * (kWave * TPL); t.end = a.nt; t.root = 0; t.G = a.G; t.eps2 = a.eps2;
         return t;
     }
     if (w >= a.nwaves) {
@@ -61,7 +63,7 @@ __device__ __forceinline__ WaveTask wave_task(const WalkArgs &a, int64_t w) {
         if (a.wave_first[mid] <= w) lo = mid;
         else hi = mid;
     }
-    t.base = a.sys_first[lo] + (w - a.wave_first[lo]) * kWave;
+    t.base = a.sys_first[lo] + (w - a.wave_first[lo]) * (kWave * TPL);
     t.end = a.sys_first[lo + 1];
     t.root = a.sys_root[lo];
     const float4 q = a.sys_prm[lo];
@@ -629,6 +631,135 @@ void launch_walk_oct(const WalkArgs &a, int ordered, int stats, hipStream_t s) {
     }
 }
 
+// TPL > 1 targets per lane: wave w serves the 64 TPL targets from its
+// task's base (wave_task), slot t of lane l being target base + 64 t + l.
+// Each step visits the sibling pair (c, c + 1) for the union of the slots'
+// masks; every slot keeps its own mask, opening decisions and sums (a slot
+// whose mask is empty skips the node), so each target meets exactly the
+// nodes and bodies it meets alone, in the same order, while the node
+// loads, the traversal and its stack are shared by TPL targets.
+template <int WAVES, bool SOFT, int STATS, bool QUAD, int TPL, bool FOREST>
+__global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_multi(WalkArgs a) {
+    __shared__ int s_node[WAVES][kStackFast];
+    __shared__ uint64_t s_mask[WAVES][kStackFast][TPL];
+    __shared__ float4 s_leaf[WAVES][kLeafScratch];
+
+    const float4 *__restrict__ bodies = a.bodies;
+    const NodeRec *__restrict__ nodes = a.nodes;
+    const int lane = lane_id(), wv = uniform_i32(threadIdx.x / kWave);
+    const WaveTask T = wave_task<FOREST, TPL>(a, xcd_block(blockIdx.x, gridDim.x) * WAVES + wv);
+    const int64_t base = T.base;
+    FastWave W[TPL];
+    uint64_t mask[TPL];
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+        const int64_t tgt = base + t * kWave + lane;
+        const bool valid = tgt < T.end;
+        mask[t] = __ballot(valid);
+        const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        fast_init(W[t], me.x, me.y, me.z);
+        W[t].G = T.G;
+    }
+    if (mask[0] == 0) return;  // slot 0 holds the wave's first targets
+    if (FOREST && (T.root < 0 || T.root >= a.max_nodes)) {
+        if (lane == 0) atomicOr(a.err, 2u);
+        return;
+    }
+    const float eps2 = T.eps2;
+    const int64_t n = a.n;
+    const uint32_t last_pair = (uint32_t)(a.max_nodes - 1);
+    bool fail = false;
+
+    const NodeRec root = nodes[T.root];
+#pragma unroll
+    for (int t = 0; t < TPL; ++t)
+        mask[t] = mask[t] ? visit<SOFT, STATS, QUAD>(root, a.quad + 2 * T.root, mask[t], bodies, n, eps2, W[t], fail,
+                                                     s_leaf[wv])
+                          : 0ull;
+    int c = root.topo.z;
+    int sp = 0;
+    uint32_t iters = 0;
+    while (true) {
+        uint64_t any = 0;
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) any |= mask[t];
+        if (any == 0) {
+            if (sp == 0) break;
+            --sp;
+            __builtin_amdgcn_wave_barrier();
+            c = uniform_i32(s_node[wv][sp]);
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) mask[t] = uniform_u64(s_mask[wv][sp][t]);
+        }
+        c = uniform_i32(c);  // keeps the node loads scalar
+        if (__builtin_expect((uint32_t)c >= last_pair, 0)) { fail = true; break; }
+        if (__builtin_expect(++iters > kMaxWalkIters, 0)) { fail = true; break; }
+        const NodeRec r0 = nodes[c], r1 = nodes[c + 1];
+        uint64_t o0[TPL], o1[TPL], any0 = 0, any1 = 0;
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) {
+            o0[t] = mask[t] ? visit<SOFT, STATS, QUAD>(r0, a.quad + 2 * c, mask[t], bodies, n, eps2, W[t], fail,
+                                                       s_leaf[wv])
+                            : 0ull;
+            any0 |= o0[t];
+        }
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) {
+            o1[t] = mask[t] ? visit<SOFT, STATS, QUAD>(r1, a.quad + 2 * c + 2, mask[t], bodies, n, eps2, W[t], fail,
+                                                       s_leaf[wv])
+                            : 0ull;
+            any1 |= o1[t];
+        }
+        if (any0 != 0) {
+            if (any1 != 0) {
+                if (sp >= kStackFast) { fail = true; break; }
+                if (lane == 0) {
+                    s_node[wv][sp] = r1.topo.z;
+#pragma unroll
+                    for (int t = 0; t < TPL; ++t) s_mask[wv][sp][t] = o1[t];
+                }
+                __builtin_amdgcn_wave_barrier();
+                ++sp;
+            }
+            c = r0.topo.z;
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) mask[t] = o0[t];
+        } else if (any1 != 0) {
+            c = r1.topo.z;
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) mask[t] = o1[t];
+        } else {
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) mask[t] = 0;
+        }
+    }
+    if (fail && lane == 0) atomicOr(a.err, 2u);
+    uint64_t pp = 0, np = 0, visits = 0, lanes = 0, lvis = 0, llanes = 0;
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+        const int64_t tgt = base + t * kWave + lane;
+        if (tgt < T.end) {
+            const uint32_t o = a.perm[tgt];
+            if ((int64_t)o < a.nt) {
+                const float3 r = fast_result(W[t], W[t].G);
+                a.ax[o] = r.x; a.ay[o] = r.y; a.az[o] = r.z;
+            }
+        }
+        pp += W[t].pp; np += W[t].np;
+        visits += W[t].visits; lanes += W[t].lanes; lvis += W[t].leaf_visits; llanes += W[t].leaf_lanes;
+    }
+    if (lane == 0) {
+        atomicAdd(&a.counts[0], (unsigned long long)pp);
+        atomicAdd(&a.counts[1], (unsigned long long)np);
+        if (STATS >= 2) {
+            atomicAdd(&a.counts[2], (unsigned long long)visits);
+            atomicAdd(&a.counts[3], (unsigned long long)lanes);
+            atomicAdd(&a.counts[4], (unsigned long long)lvis);
+            atomicAdd(&a.counts[5], (unsigned long long)llanes);
+        }
+    }
+}
+
 // Single system: wave w walks targets 64 w .. 64 w + 63 from the root.
 template <int WAVES, bool SOFT, int STATS, bool QUAD>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
@@ -715,9 +846,26 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
 }
 
 
+template <int STATS, bool QUAD, int TPL>
+static void launch_multi(const WalkArgs &a, hipStream_t s) {
+    constexpr int W = 4;
+    const int64_t waves = a.nsys > 0 ? a.nwaves : (a.nt + kWave * TPL - 1) / (kWave * TPL);
+    const dim3 g((unsigned)((waves + W - 1) / W)), t(64 * W);
+    const bool soft = a.eps2 > 0.0f;
+    if (a.nsys > 0) {
+        if (soft) hipLaunchKernelGGL((k_walk_lbvh_multi<W, true, STATS, QUAD, TPL, true>), g, t, 0, s, a);
+        else hipLaunchKernelGGL((k_walk_lbvh_multi<W, false, STATS, QUAD, TPL, true>), g, t, 0, s, a);
+    } else {
+        if (soft) hipLaunchKernelGGL((k_walk_lbvh_multi<W, true, STATS, QUAD, TPL, false>), g, t, 0, s, a);
+        else hipLaunchKernelGGL((k_walk_lbvh_multi<W, false, STATS, QUAD, TPL, false>), g, t, 0, s, a);
+    }
+}
+
 template <int STATS, bool QUAD>
 static void launch_fast_single(const WalkArgs &a, unsigned blocks, hipStream_t s) {
     constexpr int W = 4;
+    if (a.tpl == 2) return launch_multi<STATS, QUAD, 2>(a, s);
+    if (a.tpl == 4) return launch_multi<STATS, QUAD, 4>(a, s);
     if (a.eps2 > 0.0f) hipLaunchKernelGGL((k_walk_lbvh_fast<W, true, STATS, QUAD>), dim3(blocks), dim3(64 * W), 0, s, a);
     else hipLaunchKernelGGL((k_walk_lbvh_fast<W, false, STATS, QUAD>), dim3(blocks), dim3(64 * W), 0, s, a);
 }
@@ -727,7 +875,15 @@ static void launch_fast(const WalkArgs &a, hipStream_t s) {
     constexpr int W = 4;
     const unsigned blocks = (unsigned)((walk_waves(a) + W - 1) / W);
     const bool soft = a.eps2 > 0.0f;
-    if (a.nsys > 0 && a.quad) {
+    if (a.nsys > 0 && a.tpl > 1) {
+        if (a.quad) {
+            if (a.tpl == 2) launch_multi<STATS, true, 2>(a, s);
+            else launch_multi<STATS, true, 4>(a, s);
+        } else {
+            if (a.tpl == 2) launch_multi<STATS, false, 2>(a, s);
+            else launch_multi<STATS, false, 4>(a, s);
+        }
+    } else if (a.nsys > 0 && a.quad) {
         if (soft) hipLaunchKernelGGL((k_walk_lbvh_forest<W, true, STATS, true>), dim3(blocks), dim3(64 * W), 0, s, a);
         else hipLaunchKernelGGL((k_walk_lbvh_forest<W, false, STATS, true>), dim3(blocks), dim3(64 * W), 0, s, a);
     } else if (a.nsys > 0) {

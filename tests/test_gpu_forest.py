@@ -145,6 +145,57 @@ def test_forest_autotune(nbg, gpu_ctx, solver):
     B.close()
 
 
+@pytest.mark.parametrize("solver", [1, 4])
+def test_forest_targets_per_lane_same_bits(nbg, gpu_ctx, solver):
+    """The FAST linear-BVH forest walk with 2 or 4 targets per lane (128 or
+    256 bodies per wavefront, waves aligned to systems at that width): each
+    system keeps the bits of one target per lane, and of itself alone."""
+    systems = _systems(nbg)
+    runs = []
+    for tpl in (1, 2, 4):
+        B = nbg.Batch(gpu_ctx, 40000, 16, sum_mode=1)
+        ids = [B.add(_params(nbg, mode=1, solver=solver, **kw), b) for b, kw in systems]
+        B.set_targets_per_lane(tpl)
+        B.update_accelerations()
+        runs.append(([B.get(sid, accelerations=True)[1] for sid in ids], B.counts()))
+        B.close()
+    for tpl, (accs, c) in zip((2, 4), runs[1:]):
+        assert c == runs[0][1]
+        for (b, _), a, a1 in zip(systems, accs, runs[0][0]):
+            assert np.array_equal(a.view(np.uint32), a1.view(np.uint32)), f"{tpl} targets per lane, n={b.n}"
+    b, kw = systems[6]
+    a1, _, _ = _alone(nbg, gpu_ctx, b, kw, 1, solver=solver)
+    assert np.array_equal(runs[2][0][6], a1)
+
+
+def test_forest_autotune_grid(nbg, gpu_ctx):
+    """Leaf size x bodies per wavefront: every pair is timed, the fastest is
+    kept, no body moves, and the batch then gives each system the bits it
+    gets alone with the chosen leaf size; an octree batch times one target
+    per lane only."""
+    systems = _systems(nbg)
+    B = nbg.Batch(gpu_ctx, 40000, 16)
+    ids = [B.add(_params(nbg, **kw), b) for b, kw in systems]
+    before = [B.get(sid).x.copy() for sid in ids]
+    leaf, tpl, table = B.autotune_grid((8, 16), (1, 2, 4), repeats=1)
+    print(f"[forest autotune grid] leaf {leaf}, targets per lane {tpl}: {table}", flush=True)
+    assert set(table) == {(l, t) for l in (8, 16) for t in (1, 2, 4)} and all(v > 0 for v in table.values())
+    assert (leaf, tpl) == min(table, key=table.get)
+    for sid, x0 in zip(ids, before):
+        assert np.array_equal(B.get(sid).x, x0)
+    B.update_accelerations()
+    b, kw = systems[6]
+    a1, _, _ = _alone(nbg, gpu_ctx, b, dict(kw, leaf=leaf), 1)
+    assert np.array_equal(B.get(ids[6], accelerations=True)[1], a1)
+    B.close()
+    B = nbg.Batch(gpu_ctx, 40000, 16)
+    for b, kw in systems:
+        B.add(_params(nbg, solver=0, **kw), b)
+    leaf, tpl, table = B.autotune_grid((8, 16), (1, 2, 4), repeats=1)
+    assert set(table) == {(8, 1), (16, 1)} and tpl == 1
+    B.close()
+
+
 def test_forest_capacity_and_arguments(nbg, gpu_ctx):
     B = nbg.Batch(gpu_ctx, 100, 2)
     b = nbg.plummer(60, seed=1)

@@ -93,6 +93,37 @@ def test_lbvh_fast_within_tolerance(nbg, orc, gpu_ctx, n, leaf, bits, theta):
     assert worst <= FAST_TOL
 
 
+@pytest.mark.parametrize("n,bits,quad", [(20037, 30, False), (50000, 63, False), (20037, 30, True), (130, 30, False),
+                                          (1, 30, False)])
+def test_lbvh_targets_per_lane_same_bits(nbg, gpu_ctx, n, bits, quad):
+    """FAST with 2 or 4 targets per lane (128 or 256 bodies per wavefront):
+    every 64-target slot keeps its own masks and visits its nodes in the
+    same depth-first order as a wavefront of its own, so accelerations and
+    counts are the bits of one target per lane (sizes not a multiple of the
+    wavefront's targets included)."""
+    b = nbg.plummer(n, seed=41)
+    res = []
+    for tpl in (1, 2, 4):
+        p = _params(nbg, 16, bits, mode=1)
+        if quad:
+            p.solver = nbg.LINEAR_BVH_QUADRUPOLE
+        p.targets_per_lane = tpl
+        s = nbg.System(gpu_ctx, p, n)
+        s.set_state(b)
+        res.append((s.update_accelerations(), s.counts()))
+        s.close()
+    for tpl, (acc, c) in zip((2, 4), res[1:]):
+        assert c == res[0][1], f"{tpl} targets per lane: counts"
+        assert np.array_equal(acc.view(np.uint32), res[0][0].view(np.uint32)), f"{tpl} targets per lane: bits"
+
+
+def test_targets_per_lane_is_checked(nbg, gpu_ctx):
+    p = _params(nbg, 16, 30, mode=1)
+    p.targets_per_lane = 3
+    with pytest.raises(nbg.NbgError):
+        nbg.System(gpu_ctx, p, 100)
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 def test_lbvh_steps_match_oracle(nbg, orc, gpu_ctx, mode):
     """Solver::step x5 on the device vs the oracle's step (bit-exact ORDERED)."""
