@@ -246,7 +246,8 @@ def measure(nbg, ctx, b, args, solver, mode=None):
              + FLOPS_PER_LOCAL * (pn + nn))
     achieved = flops / (walk_ms * 1e-3) / 1e12
     waves = max(1, (args.n + 63) // 64)
-    traffic = load_traffic().get(SOLVER_NAMES[solver])
+    # the traffic file keys the ORDERED walks' rows "<solver>-ordered"
+    traffic = load_traffic().get(SOLVER_NAMES[solver] + ("" if mode == "fast" else "-ordered"))
     out = {
         "value": args.n * args.steps / elapsed,
         "ms_per_step": ms_per_step,
@@ -647,7 +648,11 @@ def bench_serving(nbg, ctx, args):
     log(f"[bench] serving: {len(specs)} jobs, {work.sum():.3g} body-steps, batched capacity {capacity:.3g}/s, "
         f"arrival rate {rate:.3g}/s (~{arrivals[-1]:.0f} s of arrivals)")
 
-    srv = nbg.Server(ctx, 200 * 10 ** 9, max_systems=4096, sum_mode=nbg.SUM_FAST)
+    # jobs of at most serve_express bodies run in an express forest (8 GB of
+    # the budget) that steps concurrently with the bulk forest
+    xmax = args.serve_express
+    srv = nbg.Server(ctx, 200 * 10 ** 9, max_systems=4096, sum_mode=nbg.SUM_FAST, express_max_bodies=xmax,
+                     express_bytes=8 * 10 ** 9 if xmax > 0 else 0, express_systems=4096 if xmax > 0 else 0)
     cap_bodies, _ = srv.capacity()
     ids = []
     t0 = time.perf_counter()
@@ -679,11 +684,39 @@ def bench_serving(nbg, ctx, args):
     srv.close()
     served = work.sum() / wall
     lat = np.array(lat)
+    # latency by job size: small jobs should not wait behind large ones
+    sizes = np.array([n for _, n, _, _, _ in specs])
+    classes = {}
+    for name, lo, hi in (("N < 16k", 0, 16384), ("16k <= N < 256k", 16384, 262144), ("N >= 256k", 262144, 1 << 40)):
+        sel = (sizes >= lo) & (sizes < hi)
+        if sel.any():
+            classes[name] = {"jobs": int(sel.sum()), "latency_p50_s": float(np.quantile(lat[sel], 0.5)),
+                             "latency_max_s": float(lat[sel].max()),
+                             "latency_per_Mbody_step_p50_s": float(np.quantile(lat[sel] / (work[sel] / 1e6), 0.5))}
+    # force error of the served jobs: a served job's bits are its bits alone
+    # (tests/test_gpu_serve.py), so the first force pass of a sample of the
+    # jobs, alone, against the FP64 direct sum on up to 4,096 of its bodies
+    err_sample = [int(k) for k in np.argsort(sizes)[::max(1, len(sizes) // 8)][:8]]
+    errs = []
+    for k in err_sample:
+        kind, n, steps, theta, _ = specs[k]
+        b = bodies[k]
+        s = nbg.System(ctx, params(theta), n)
+        s.set_state(b)
+        a0 = s.update_accelerations()
+        s.close()
+        sub = np.random.default_rng(k).choice(n, min(n, 4096), replace=False)
+        d = ctx.direct_f64(b, b.x[sub], b.y[sub], b.z[sub], 1.0, 1e-4)
+        e = error_stats(a0[sub], d, b.m[sub])
+        errs.append({"job": k, "kind": kind, "n": n, "theta": theta, "force_err_vs_fp64_median": e[
+            "force_err_vs_fp64_median"], "force_err_vs_fp64_p99": e["force_err_vs_fp64_p99"]})
     return {
         "workload": f"{len(specs)} jobs (Plummer or cube side 2, N log-uniform [1024, 2097152], steps U[10, 500], "
                     f"theta in {{0.3, 0.5, 0.7}}, eps2 1e-4, dt 1e-3, linear BVH 30-bit, FAST), Poisson arrivals at "
                     f"{rate:.3g}/s = {args.serve_load} x the batched capacity {capacity:.3g} bodies*steps/s measured "
-                    f"on {len(sample)} of the jobs, 200 GB HBM budget ({cap_bodies} bodies), seed {args.serve_seed}",
+                    f"on {len(sample)} of the jobs, 200 GB HBM budget ({cap_bodies} bodies), "
+                    + (f"express lane for jobs of <= {xmax} bodies (8 GB of the budget), " if xmax > 0 else "")
+                    + f"seed {args.serve_seed}",
         "value": served,
         "unit": "bodies·steps/s",
         "latency_p50_s": float(np.quantile(lat, 0.5)),
@@ -703,6 +736,8 @@ def bench_serving(nbg, ctx, args):
                                 "sample": f"the same {len(sample)} jobs stepped together in one forest (5 steps)"},
         "batched_over_one_at_a_time": capacity / alone,
         "served_over_one_at_a_time": served / alone,
+        "latency_by_size": classes,
+        "force_error_by_job": errs,
     }
 
 
@@ -759,6 +794,8 @@ def main():
     ap.add_argument("--forest-accuracy-sample", type=int, default=32)
     ap.add_argument("--serve-jobs", type=int, default=60)
     ap.add_argument("--serve-load", type=float, default=0.8)
+    ap.add_argument("--serve-express", type=int, default=131072,
+                    help="largest job (bodies) served by the express forest; 0: one forest for every job")
     ap.add_argument("--serve-seed", type=int, default=11)
     ap.add_argument("--serve-alone-sample", type=int, default=16)
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -363,6 +363,16 @@ typedef struct nbg_server nbg_server;
 
 int nbg_server_create(nbg_ctx *ctx, int64_t hbm_budget_bytes, int32_t max_systems, int32_t sum_mode,
                       nbg_server **out);
+/* The same server with an express lane: jobs of at most express_max_bodies
+ * bodies go to a second forest (express_budget_bytes of the budget,
+ * express_max_systems slots) with its own worker thread and stream, which
+ * steps concurrently with the bulk forest.  A small job then advances at the
+ * pace of a forest of small systems rather than one step per step of the
+ * largest resident job.  nbg_server_capacity reports the largest job either
+ * lane accepts and the slots of both.  No reference counterpart. */
+int nbg_server_create_express(nbg_ctx *ctx, int64_t hbm_budget_bytes, int32_t max_systems, int32_t sum_mode,
+                              int64_t express_max_bodies, int64_t express_budget_bytes, int32_t express_max_systems,
+                              nbg_server **out);
 /* Stops the worker.  Jobs still queued or running end with NBG_ERR_STATE;
  * clients blocked in nbg_server_wait return that status, and destroy returns
  * once they have left.  A finished job's result is held until it is waited

@@ -49,7 +49,7 @@ EXPORTS = (
     "nbg_batch_synchronize", "nbg_batch_counts", "nbg_batch_enable_counts", "nbg_batch_use_graph",
     "nbg_batch_autotune", "nbg_batch_autotune_grid", "nbg_batch_set_targets_per_lane", "nbg_batch_nonfinite",
     "nbg_scenario_load_json", "nbg_scenario_load_tipsy", "nbg_scenario_count", "nbg_scenario_get", "nbg_scenario_free",
-    "nbg_server_create", "nbg_server_destroy", "nbg_server_capacity", "nbg_server_submit", "nbg_server_wait",
+    "nbg_server_create", "nbg_server_create_express", "nbg_server_destroy", "nbg_server_capacity", "nbg_server_submit", "nbg_server_wait",
     "nbg_server_stats",
 )
 
@@ -144,6 +144,9 @@ def lib() -> ctypes.CDLL:
             "nbg_scenario_free": (ctypes.c_int, [_vp]),
             "nbg_server_create": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                                  ctypes.POINTER(_vp)]),
+            "nbg_server_create_express": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                                         ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
+                                                         ctypes.POINTER(_vp)]),
             "nbg_server_destroy": (ctypes.c_int, [_vp]),
             "nbg_server_capacity": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64),
                                                    ctypes.POINTER(ctypes.c_int32)]),
@@ -557,11 +560,21 @@ def load_scenario(path: str, kind: str = "json"):
 
 class Server:
     """nbg_server: continuous batching of submitted jobs over one forest,
-    within an HBM byte budget."""
+    within an HBM byte budget; with express_max_bodies > 0, jobs up to that
+    size run in a second forest (express_bytes of the budget,
+    express_systems slots) that steps concurrently with the first
+    (nbg_server_create_express)."""
 
-    def __init__(self, ctx: Context, hbm_budget_bytes: int, max_systems: int = 4096, sum_mode: int = SUM_FAST):
+    def __init__(self, ctx: Context, hbm_budget_bytes: int, max_systems: int = 4096, sum_mode: int = SUM_FAST,
+                 express_max_bodies: int = 0, express_bytes: int = 0, express_systems: int = 0):
         h = _vp()
-        _check(lib().nbg_server_create(ctx.handle, int(hbm_budget_bytes), max_systems, sum_mode, ctypes.byref(h)))
+        if express_max_bodies > 0:
+            _check(lib().nbg_server_create_express(ctx.handle, int(hbm_budget_bytes), max_systems, sum_mode,
+                                                   int(express_max_bodies), int(express_bytes), express_systems,
+                                                   ctypes.byref(h)))
+        else:
+            _check(lib().nbg_server_create(ctx.handle, int(hbm_budget_bytes), max_systems, sum_mode,
+                                           ctypes.byref(h)))
         self.handle = h
         self._n = {}
 

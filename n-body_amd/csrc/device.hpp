@@ -212,6 +212,18 @@ __device__ __forceinline__ int64_t xcd_block(unsigned b, unsigned grid) {
 
 __device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// A read-only device array seen through the constant address space.  A
+// uniform load through it is always a scalar load: the compiler need not
+// prove that no store in the kernel reaches it (its proof gives up in large
+// unrolled loops that write LDS, and then emits per-lane vector loads).
+// Only for data the kernel never writes.
+template <typename T>
+using const_ptr = const __attribute__((address_space(4))) T *;
+template <typename T>
+__device__ __forceinline__ const_ptr<T> as_const(const T *p) {
+    return (const_ptr<T>)p;
+}
+
 __device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
     uint32_t lo = __builtin_amdgcn_readfirstlane((int)(uint32_t)v);
     uint32_t hi = __builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));

@@ -1,7 +1,7 @@
-// Continuous-batching server over one batched forest (host code only).
+// Continuous-batching server over batched forests (host code only).
 //
 // Clients submit jobs (a system plus a step count) from any thread.  A
-// worker thread owns the forest: before every step it admits waiting jobs
+// worker thread owns each forest (lane): before every step it admits waiting jobs
 // (plan_admission: arrival order with backfill, each job bypassed a bounded
 // number of times) while the forest's body and system capacity (sized from
 // an HBM byte budget) allow, steps every admitted system once, retires the
@@ -56,62 +56,72 @@ constexpr int32_t kMaxBypass = 32;
 
 }  // namespace
 
-struct nbg_server {
-    nbg_ctx *ctx = nullptr;
+// One forest with its own worker thread, HIP stream and queue.  A server has
+// a bulk lane and optionally an express lane for small jobs: the lanes step
+// concurrently, so a small job advances at the pace of a forest of small
+// systems instead of one step per step of the largest resident job.
+struct ServeLane {
     nbg_batch *batch = nullptr;
-    int64_t cap_n = 0;
-    int32_t cap_s = 0;
-    std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    int64_t cap_n = 0;    // forest body capacity
+    int32_t cap_s = 0;    // forest system slots
+    int64_t max_job = 0;  // largest job routed to this lane
     std::deque<std::shared_ptr<Job>> pending;
     std::vector<std::shared_ptr<Job>> active;
+    int64_t used_n = 0;
+    std::thread worker;
+};
+
+struct nbg_server {
+    nbg_ctx *ctx = nullptr;
+    std::vector<std::unique_ptr<ServeLane>> lanes;  // [0] bulk, [1] express (optional)
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
     std::map<int64_t, std::shared_ptr<Job>> jobs;
     int64_t next_id = 1;
-    int64_t used_n = 0;
     bool stop = false;
     int waiters = 0;  // clients inside nbg_server_wait (destroy waits for them to leave)
-    std::thread worker;
     // statistics
-    Clock::time_point t_start;
+    Clock::time_point t_start, t_busy_from;
+    int busy_lanes = 0;    // lanes inside a step; device-busy time is their union
     double busy_s = 0.0;
     int64_t steps_run = 0, jobs_done = 0;
     double body_steps = 0.0;
-    int64_t peak_bodies = 0, peak_systems = 0;
+    int64_t resident_n = 0, resident_s = 0, peak_bodies = 0, peak_systems = 0;
     std::string worker_error;
 };
 
 namespace {
 
-void worker_loop(nbg_server *S) {
+void worker_loop(nbg_server *S, ServeLane *L) {
     DeviceGuard g(S->ctx->device);
     for (;;) {
         std::vector<std::shared_ptr<Job>> admit;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty() || !S->active.empty(); });
+            S->cv_work.wait(lk, [&] { return S->stop || !L->pending.empty() || !L->active.empty(); });
             if (S->stop) return;
-            std::vector<Waiting> q(S->pending.size());
-            for (size_t i = 0; i < q.size(); ++i) q[i] = Waiting{S->pending[i]->n, S->pending[i]->bypassed};
+            std::vector<Waiting> q(L->pending.size());
+            for (size_t i = 0; i < q.size(); ++i) q[i] = Waiting{L->pending[i]->n, L->pending[i]->bypassed};
             const std::vector<size_t> pick =
-                plan_admission(q, S->cap_n - S->used_n, (int64_t)S->cap_s - (int64_t)S->active.size(), kMaxBypass);
+                plan_admission(q, L->cap_n - L->used_n, (int64_t)L->cap_s - (int64_t)L->active.size(), kMaxBypass);
             std::deque<std::shared_ptr<Job>> rest;
             size_t k = 0;
-            for (size_t i = 0; i < S->pending.size(); ++i) {
-                auto &j = S->pending[i];
+            for (size_t i = 0; i < L->pending.size(); ++i) {
+                auto &j = L->pending[i];
                 j->bypassed = q[i].bypassed;
                 if (k < pick.size() && pick[k] == i) {
-                    S->used_n += j->n;
+                    L->used_n += j->n;
                     admit.push_back(j);
                     ++k;
                 } else {
                     rest.push_back(j);
                 }
             }
-            S->pending.swap(rest);
+            L->pending.swap(rest);
         }
         for (auto &j : admit) {
             int32_t sid = -1;
-            const int r = nbg_batch_add(S->batch, &j->p, j->n, j->st[0].data(), j->st[1].data(), j->st[2].data(),
+            const int r = nbg_batch_add(L->batch, &j->p, j->n, j->st[0].data(), j->st[1].data(), j->st[2].data(),
                                         j->st[3].data(), j->st[4].data(), j->st[5].data(), j->st[6].data(), &sid);
             std::lock_guard<std::mutex> lk(S->mu);
             j->t_admit = Clock::now();
@@ -120,38 +130,39 @@ void worker_loop(nbg_server *S) {
                 j->error = nbg_last_error();
                 j->done = true;
                 j->t_done = j->t_admit;
-                S->used_n -= j->n;
+                L->used_n -= j->n;
                 S->cv_done.notify_all();
                 continue;
             }
             j->sys = sid;
-            S->active.push_back(j);
+            L->active.push_back(j);
+            S->resident_n += j->n;
+            S->resident_s += 1;
         }
         int64_t nb = 0;
         {
             std::lock_guard<std::mutex> lk(S->mu);
-            if (S->active.empty()) continue;
-            for (auto &j : S->active) nb += j->n;
-            if (nb > S->peak_bodies) S->peak_bodies = nb;
-            if ((int64_t)S->active.size() > S->peak_systems) S->peak_systems = (int64_t)S->active.size();
+            if (L->active.empty()) continue;
+            for (auto &j : L->active) nb += j->n;
+            S->peak_bodies = std::max(S->peak_bodies, S->resident_n);
+            S->peak_systems = std::max(S->peak_systems, S->resident_s);
+            if (S->busy_lanes++ == 0) S->t_busy_from = Clock::now();
         }
-        const auto t0 = Clock::now();
-        int r = nbg_batch_step(S->batch, 1);
-        if (r == NBG_OK) r = nbg_batch_synchronize(S->batch);
+        int r = nbg_batch_step(L->batch, 1);
+        if (r == NBG_OK) r = nbg_batch_synchronize(L->batch);
         // systems whose state became NaN or infinite fail alone and leave
         // before the next build
-        std::vector<int32_t> bad(S->active.size() + 1);
+        std::vector<int32_t> bad(L->active.size() + 1);
         int32_t nbad = 0;
-        if (r == NBG_OK) r = nbg_batch_nonfinite(S->batch, bad.data(), (int32_t)bad.size(), &nbad);
-        const auto t1 = Clock::now();
+        if (r == NBG_OK) r = nbg_batch_nonfinite(L->batch, bad.data(), (int32_t)bad.size(), &nbad);
         std::vector<std::shared_ptr<Job>> finished, keep;
         {
             std::lock_guard<std::mutex> lk(S->mu);
-            S->busy_s += secs(t0, t1);
+            if (--S->busy_lanes == 0) S->busy_s += secs(S->t_busy_from, Clock::now());
             S->steps_run += 1;
             S->body_steps += (double)nb;
             if (r != NBG_OK) S->worker_error = nbg_last_error();
-            for (auto &j : S->active) {
+            for (auto &j : L->active) {
                 if (r != NBG_OK) {
                     j->status = r;
                     j->error = S->worker_error;
@@ -171,56 +182,94 @@ void worker_loop(nbg_server *S) {
                     keep.push_back(j);
                 }
             }
-            S->active = keep;
+            L->active = keep;
         }
         for (auto &j : finished) {
             if (j->status == NBG_OK) {
                 // out: x y z into st[0..2], vx vy vz into st[4..6]
-                const int rr = nbg_batch_get(S->batch, j->sys, j->st[0].data(), j->st[1].data(), j->st[2].data(),
+                const int rr = nbg_batch_get(L->batch, j->sys, j->st[0].data(), j->st[1].data(), j->st[2].data(),
                                              j->st[4].data(), j->st[5].data(), j->st[6].data(), nullptr, nullptr,
                                              nullptr);
                 if (rr != NBG_OK) { j->status = rr; j->error = nbg_last_error(); }
             }
-            (void)nbg_batch_remove(S->batch, j->sys);
+            (void)nbg_batch_remove(L->batch, j->sys);
             std::lock_guard<std::mutex> lk(S->mu);
             j->t_done = Clock::now();
             j->done = true;
-            S->used_n -= j->n;
+            L->used_n -= j->n;
+            S->resident_n -= j->n;
+            S->resident_s -= 1;
             S->jobs_done += 1;
             S->cv_done.notify_all();
         }
     }
 }
 
+// Largest body capacity whose forest of max_systems slots fits the budget.
+int64_t forest_capacity(int64_t budget, int32_t max_systems) {
+    const int64_t fixed = nbg_batch_bytes(1, max_systems);
+    const int64_t per = (nbg_batch_bytes(1 << 20, max_systems) - fixed) / ((1 << 20) - 1) + 1;
+    int64_t cap = budget > fixed ? (budget - fixed) / per : 0;
+    if (cap > kMaxBodies) cap = kMaxBodies;
+    while (cap > 0 && nbg_batch_bytes(cap, max_systems) > budget) cap -= 1 + cap / 1000;
+    return cap;
+}
+
+void free_server(nbg_server *S) {
+    for (auto &L : S->lanes)
+        if (L->batch) nbg_batch_destroy(L->batch);
+    delete S;
+}
+
 }  // namespace
 
 extern "C" {
 
-int nbg_server_create(nbg_ctx *ctx, int64_t hbm_budget_bytes, int32_t max_systems, int32_t sum_mode,
-                      nbg_server **out) {
+int nbg_server_create_express(nbg_ctx *ctx, int64_t hbm_budget_bytes, int32_t max_systems, int32_t sum_mode,
+                              int64_t express_max_bodies, int64_t express_budget_bytes, int32_t express_max_systems,
+                              nbg_server **out) {
     if (!ctx || !out) return fail(NBG_ERR_ARG, "ctx or out is NULL");
     *out = nullptr;
     if (max_systems < 1) return fail(NBG_ERR_ARG, "max_systems must be >= 1");
-    // largest body capacity whose forest fits the budget
-    const int64_t fixed = nbg_batch_bytes(1, max_systems);
-    const int64_t per = (nbg_batch_bytes(1 << 20, max_systems) - fixed) / ((1 << 20) - 1) + 1;
-    int64_t cap = hbm_budget_bytes > fixed ? (hbm_budget_bytes - fixed) / per : 0;
-    if (cap > kMaxBodies) cap = kMaxBodies;
-    while (cap > 0 && nbg_batch_bytes(cap, max_systems) > hbm_budget_bytes) cap -= 1 + cap / 1000;
+    const bool express = express_max_bodies > 0;
+    if (express && (express_max_systems < 1 || express_budget_bytes <= 0 || express_budget_bytes >= hbm_budget_bytes))
+        return fail(NBG_ERR_ARG, "the express lane needs system slots and a budget inside the server's");
+    const int64_t bulk_budget = express ? hbm_budget_bytes - express_budget_bytes : hbm_budget_bytes;
+    const int64_t cap = forest_capacity(bulk_budget, max_systems);
     if (cap < 1) return fail(NBG_ERR_BUDGET, "the HBM budget does not hold a single body");
+    int64_t xcap = 0;
+    if (express) {
+        xcap = forest_capacity(express_budget_bytes, express_max_systems);
+        if (xcap < express_max_bodies)
+            return fail(NBG_ERR_BUDGET, "the express budget does not hold one job of express_max_bodies");
+    }
     nbg_server *S = new nbg_server;
     S->ctx = ctx;
-    S->cap_n = cap;
-    S->cap_s = max_systems;
-    if (int r = nbg_batch_create(ctx, cap, max_systems, sum_mode, &S->batch)) {
-        delete S;
+    auto add_lane = [&](int64_t c, int32_t slots, int64_t max_job) {
+        auto L = std::make_unique<ServeLane>();
+        L->cap_n = c;
+        L->cap_s = slots;
+        L->max_job = max_job;
+        if (int r = nbg_batch_create(ctx, c, slots, sum_mode, &L->batch)) return r;
+        nbg_batch_enable_counts(L->batch, 0);
+        S->lanes.push_back(std::move(L));
+        return (int)NBG_OK;
+    };
+    int r = add_lane(cap, max_systems, cap);
+    if (r == NBG_OK && express) r = add_lane(xcap, express_max_systems, express_max_bodies);
+    if (r != NBG_OK) {
+        free_server(S);
         return r;
     }
-    nbg_batch_enable_counts(S->batch, 0);
     S->t_start = Clock::now();
-    S->worker = std::thread(worker_loop, S);
+    for (auto &L : S->lanes) L->worker = std::thread(worker_loop, S, L.get());
     *out = S;
     return NBG_OK;
+}
+
+int nbg_server_create(nbg_ctx *ctx, int64_t hbm_budget_bytes, int32_t max_systems, int32_t sum_mode,
+                      nbg_server **out) {
+    return nbg_server_create_express(ctx, hbm_budget_bytes, max_systems, sum_mode, 0, 0, 0, out);
 }
 
 int nbg_server_destroy(nbg_server *S) {
@@ -230,7 +279,8 @@ int nbg_server_destroy(nbg_server *S) {
         S->stop = true;
     }
     S->cv_work.notify_all();
-    if (S->worker.joinable()) S->worker.join();
+    for (auto &L : S->lanes)
+        if (L->worker.joinable()) L->worker.join();
     {
         // jobs still queued or running end with an error, so that clients
         // blocked in nbg_server_wait return before the server is freed
@@ -243,22 +293,29 @@ int nbg_server_destroy(nbg_server *S) {
             j->t_done = now;
             j->done = true;
         };
-        for (auto &j : S->pending) abandon(j);
-        for (auto &j : S->active) abandon(j);
-        S->pending.clear();
-        S->active.clear();
+        for (auto &L : S->lanes) {
+            for (auto &j : L->pending) abandon(j);
+            for (auto &j : L->active) abandon(j);
+            L->pending.clear();
+            L->active.clear();
+        }
         S->cv_done.notify_all();
         S->cv_done.wait(lk, [&] { return S->waiters == 0; });
     }
-    nbg_batch_destroy(S->batch);
-    delete S;
+    free_server(S);
     return NBG_OK;
 }
 
 int nbg_server_capacity(nbg_server *S, int64_t *max_bodies, int32_t *max_systems) {
     if (!S) return fail(NBG_ERR_ARG, "server is NULL");
-    if (max_bodies) *max_bodies = S->cap_n;
-    if (max_systems) *max_systems = S->cap_s;
+    int64_t nb = 0;
+    int32_t ns = 0;
+    for (auto &L : S->lanes) {
+        nb = std::max(nb, L->max_job);
+        ns += L->cap_s;
+    }
+    if (max_bodies) *max_bodies = nb;
+    if (max_systems) *max_systems = ns;
     return NBG_OK;
 }
 
@@ -270,7 +327,11 @@ int nbg_server_submit(nbg_server *S, const nbg_params *p, int64_t n, const float
     if (nsteps < 1) return fail(NBG_ERR_ARG, "a job needs at least one step");
     if (p->solver != NBG_LINEAR_BVH || p->morton_bits != 30)
         return fail(NBG_ERR_ARG, "served jobs run the linear-BVH solver with 30-bit keys");
-    if (n > S->cap_n) return fail(NBG_ERR_BUDGET, "the job is larger than the server's HBM budget holds");
+    // the express lane (if any) takes the jobs it is sized for; the bulk lane
+    // takes the rest
+    ServeLane *L = S->lanes[0].get();
+    if (S->lanes.size() > 1 && n <= S->lanes[1]->max_job) L = S->lanes[1].get();
+    if (n > L->cap_n) return fail(NBG_ERR_BUDGET, "the job is larger than the server's HBM budget holds");
     // a non-finite input would enter the shared build: refuse it here
     const float *chk[7] = {x, y, z, m, vx, vy, vz};
     for (const float *a : chk)
@@ -292,7 +353,7 @@ int nbg_server_submit(nbg_server *S, const nbg_params *p, int64_t n, const float
         j->id = S->next_id++;
         j->t_submit = Clock::now();
         S->jobs[j->id] = j;
-        S->pending.push_back(j);
+        L->pending.push_back(j);
         *job = j->id;
     }
     S->cv_work.notify_all();
@@ -345,7 +406,9 @@ int nbg_server_stats(nbg_server *S, double *out8) {
     out8[4] = secs(S->t_start, Clock::now());
     out8[5] = (double)S->peak_bodies;
     out8[6] = (double)S->peak_systems;
-    out8[7] = (double)S->pending.size();
+    size_t waiting = 0;
+    for (auto &L : S->lanes) waiting += L->pending.size();
+    out8[7] = (double)waiting;
     return NBG_OK;
 }
 

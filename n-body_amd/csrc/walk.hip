@@ -493,6 +493,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
     __shared__ int s_grp[WAVES][kStackOct];       // first child | nchild << 28
     __shared__ uint64_t s_mask[WAVES][kStackOct];
     __shared__ float4 s_leaf[WAVES][kLeafScratch];
+    __shared__ int4 s_lq[WAVES][8];                // a group's opened leaves: first, cnt, open mask
 
     const float4 *__restrict__ bodies = a.bodies;
     const OctRec *__restrict__ nodes = a.onodes;
@@ -538,12 +539,21 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
             fail = true;
             break;
         }
+        // The group's children, unrolled (constant record offsets, one
+        // uniform guard per child): node interactions now, opened internal
+        // children pushed (the last one is walked next), opened leaves queued
+        // in LDS and evaluated after the group.
         uint64_t nm = 0;
-        int nc = 0, nn = 0;
-        for (int k = 0; k < nch; ++k) {
-            const OctRec r = nodes[c + k];
-            const uint64_t o = visit_oct<SOFT, STATS, QUAD>(r, a.quad + 2 * (c + k), mask, bodies, n, eps2, W, fail, s_leaf[wv]);
-            if (o) {
+        int nc = 0, nn = 0, nl = 0;
+        const OctRec *grp = nodes + c;
+        const float4 *qg = a.quad + 2 * c;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (k >= nch) break;
+            const OctRec r = load_uniform(grp + k);
+            const uint64_t o = oct_node<SOFT, STATS, QUAD>(r, qg + 2 * k, mask, eps2, W);
+            if (o == 0) continue;
+            if (r.topo.z >= 0) {
                 if (nm) {
                     if (sp >= kStackOct) { fail = true; break; }
                     if (lane == 0) { s_grp[wv][sp] = nc | (nn << 28); s_mask[wv][sp] = nm; }
@@ -553,8 +563,20 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
                 nm = o;
                 nc = r.topo.z;
                 nn = oct_nchild(r);
+            } else {
+                if (!oct_leaf_ok<STATS>(r, o, n, W)) { fail = true; break; }
+                if (lane == 0) s_lq[wv][nl] = make_int4(r.topo.x, r.topo.y, (int)(uint32_t)o, (int)(uint32_t)(o >> 32));
+                ++nl;
             }
         }
+        __builtin_amdgcn_wave_barrier();
+        for (int i = 0; i < nl; ++i) {
+            const int4 q = s_lq[wv][i];
+            const int first = uniform_i32(q.x), cnt = uniform_i32(q.y);
+            const uint64_t o = ((uint64_t)(uint32_t)uniform_i32(q.w) << 32) | (uint32_t)uniform_i32(q.z);
+            leaf_eval<SOFT>(bodies, first, cnt, o, eps2, W, s_leaf[wv]);
+        }
+        __builtin_amdgcn_wave_barrier();  // the next group rewrites s_lq
         mask = nm;
         c = nc;
         nch = nn;

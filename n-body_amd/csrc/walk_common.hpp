@@ -88,22 +88,52 @@ __device__ __forceinline__ void interact_quad(float bx, float by, float bz, floa
     W.rz = __builtin_fmaf(dz, s, __builtin_fmaf(t, qz, W.rz));
 }
 
+// Plain four-word records, read field by field out of the constant address
+// space (HIP's vector types do not copy out of it).
+struct alignas(16) Body4 {
+    float x, y, z, w;
+};
+struct alignas(16) Int4Pod {
+    int x, y, z, w;
+};
+__device__ __forceinline__ Body4 load_body(const_ptr<Body4> p) { return Body4{p->x, p->y, p->z, p->w}; }
+
+// Scalar load of a wave-uniform float4 or octree record that the kernel
+// never writes (see as_const).
+__device__ __forceinline__ float4 load_uniform(const float4 *p) {
+    const const_ptr<Body4> q = as_const(reinterpret_cast<const Body4 *>(p));
+    return make_float4(q->x, q->y, q->z, q->w);
+}
+__device__ __forceinline__ OctRec load_uniform(const OctRec *p) {
+    const const_ptr<Body4> f = as_const(reinterpret_cast<const Body4 *>(p));
+    const const_ptr<Int4Pod> t = as_const(reinterpret_cast<const Int4Pod *>(p));
+    OctRec r;
+    r.com = make_float4(f[0].x, f[0].y, f[0].z, f[0].w);
+    r.lo = make_float4(f[1].x, f[1].y, f[1].z, f[1].w);
+    r.hi = make_float4(f[2].x, f[2].y, f[2].z, f[2].w);
+    r.topo = make_int4(t[3].x, t[3].y, t[3].z, t[3].w);
+    return r;
+}
+
 // The lanes in `take` interact with the `cnt` bodies starting at `first`.
 template <bool SOFT>
 __device__ __forceinline__ void leaf_bodies(const float4 *__restrict__ bodies, int first, int cnt, uint64_t take,
                                             float eps2, FastWave &W) {
     if (__builtin_amdgcn_inverse_ballot_w64(take)) {
+        // uniform: scalar loads (a plain four-float record, which copies out
+        // of the constant address space)
+        const const_ptr<Body4> lb = as_const(reinterpret_cast<const Body4 *>(bodies + first));
         int j = 0;
         for (; j + 4 <= cnt; j += 4) {
-            const float4 b0 = bodies[first + j], b1 = bodies[first + j + 1];
-            const float4 b2 = bodies[first + j + 2], b3 = bodies[first + j + 3];
+            const Body4 b0 = load_body(lb + j), b1 = load_body(lb + j + 1);
+            const Body4 b2 = load_body(lb + j + 2), b3 = load_body(lb + j + 3);
             interact<SOFT>(b0.x, b0.y, b0.z, b0.w, eps2, W);
             interact<SOFT>(b1.x, b1.y, b1.z, b1.w, eps2, W);
             interact<SOFT>(b2.x, b2.y, b2.z, b2.w, eps2, W);
             interact<SOFT>(b3.x, b3.y, b3.z, b3.w, eps2, W);
         }
         for (; j < cnt; ++j) {
-            const float4 b = bodies[first + j];
+            const Body4 b = load_body(lb + j);
             interact<SOFT>(b.x, b.y, b.z, b.w, eps2, W);
         }
     }
@@ -198,33 +228,93 @@ __device__ __forceinline__ bool oct_accepts(const OctRec &r, float px, float py,
     return d2 > r.lo.w && !inside;
 }
 
+// Monopole node interaction from the criterion's offset e = p - com and its
+// squared length d2 (computed in glm's order for the criterion):
+// r^2 = d2 + eps2, a = -e m r^-3 (interact() with d = -e; r^2 rounds once
+// more than interact's FMA chain, well inside FP32 accuracy).
+template <bool SOFT>
+__device__ __forceinline__ void interact_e(float ex, float ey, float ez, float d2, float bm, float eps2,
+                                           FastWave &W) {
+    const float r2 = d2 + eps2;
+    float inv = __builtin_amdgcn_rsqf(r2);
+    if (!SOFT) inv = r2 > 0.0f ? inv : 0.0f;
+    const float s = bm * (inv * (inv * inv));
+    W.rx = __builtin_fmaf(-ex, s, W.rx);
+    W.ry = __builtin_fmaf(-ey, s, W.ry);
+    W.rz = __builtin_fmaf(-ez, s, W.rz);
+}
+
+// Quadrupole node interaction (interact_quad) from e = p - com and d2.
+// With d = -e: Q d = -(Q e), d.Q.d = e.Q.e, so
+//   a = -e (M r^-3 + 7.5 r^-7 e.Q.e - 1.5 r^-5 tr Q) + 3 r^-5 Q e.
+template <bool SOFT>
+__device__ __forceinline__ void interact_quad_e(float ex, float ey, float ez, float d2, float bm, float4 q0,
+                                                float4 q1, float eps2, FastWave &W) {
+    const float r2 = d2 + eps2;
+    float inv = __builtin_amdgcn_rsqf(r2);
+    if (!SOFT) inv = r2 > 0.0f ? inv : 0.0f;
+    const float inv2 = inv * inv, inv3 = inv * inv2, inv5 = inv3 * inv2;
+    const float qx = __builtin_fmaf(q0.x, ex, __builtin_fmaf(q0.y, ey, q0.z * ez));
+    const float qy = __builtin_fmaf(q0.y, ex, __builtin_fmaf(q0.w, ey, q1.x * ez));
+    const float qz = __builtin_fmaf(q0.z, ex, __builtin_fmaf(q1.x, ey, q1.y * ez));
+    const float eqe = __builtin_fmaf(ex, qx, __builtin_fmaf(ey, qy, ez * qz));
+    const float s = __builtin_fmaf(bm, inv3, inv5 * __builtin_fmaf(7.5f * inv2, eqe, -1.5f * q1.z));
+    const float t = 3.0f * inv5;
+    W.rx = __builtin_fmaf(-ex, s, __builtin_fmaf(t, qx, W.rx));
+    W.ry = __builtin_fmaf(-ey, s, __builtin_fmaf(t, qy, W.ry));
+    W.rz = __builtin_fmaf(-ez, s, __builtin_fmaf(t, qz, W.rz));
+}
+
+// The node half of an octree visit: the criterion for the lanes in `mask`
+// (wave-uniform), the node interaction for those that accept; returns the
+// lanes that open the node.
+template <bool SOFT, int STATS, bool QUAD>
+__device__ __forceinline__ uint64_t oct_node(const OctRec &nd, const float4 *__restrict__ qn, uint64_t mask,
+                                             float eps2, FastWave &W) {
+    // oct_accepts_e as one ballot per comparison: the lane masks combine in
+    // scalar registers without turning the predicate into a vector value
+    const float ex = W.px - nd.com.x, ey = W.py - nd.com.y, ez = W.pz - nd.com.z;
+    const float d2 = ex * ex + ey * ey + ez * ez;
+    const uint64_t inside = __ballot(nd.lo.x <= W.px) & __ballot(nd.lo.y <= W.py) & __ballot(nd.lo.z <= W.pz) &
+                            __ballot(W.px < nd.hi.x) & __ballot(W.py < nd.hi.y) & __ballot(W.pz < nd.hi.z);
+    const uint64_t acc = __ballot(d2 > nd.lo.w) & ~inside & mask;
+    if (STATS >= 2) { W.visits += 1; W.lanes += __popcll(mask); }
+    if (STATS >= 1) W.np += __popcll(acc);
+    if (QUAD) {
+        if (acc) {
+            const float4 q0 = load_uniform(qn), q1 = load_uniform(qn + 1);
+            if (__builtin_amdgcn_inverse_ballot_w64(acc))
+                interact_quad_e<SOFT>(ex, ey, ez, d2, nd.com.w, q0, q1, eps2, W);
+        }
+    } else if (__builtin_amdgcn_inverse_ballot_w64(acc)) {
+        interact_e<SOFT>(ex, ey, ez, d2, nd.com.w, eps2, W);
+    }
+    return mask & ~acc;
+}
+
+// A leaf's body range checked against the n bodies: first in [0, n) and cnt
+// in [0, n - first] (n < 2^31); counts its interactions.
+template <int STATS>
+__device__ __forceinline__ bool oct_leaf_ok(const OctRec &nd, uint64_t open, int64_t n, FastWave &W) {
+    const int first = nd.topo.x, cnt = nd.topo.y;
+    if (__builtin_expect((uint64_t)(uint32_t)first >= (uint64_t)n, 0)) return false;
+    if (__builtin_expect((uint64_t)(uint32_t)cnt > (uint64_t)n - (uint32_t)first, 0)) return false;
+    if (STATS >= 2) { W.leaf_visits += 1; W.leaf_lanes += __popcll(open); }
+    if (STATS >= 1) W.pp += (uint64_t)__popcll(open) * (uint64_t)cnt;
+    return true;
+}
+
 // FAST over the octree: the children of an opened node are contiguous and
 // visited in one group; opened internal children are pushed as groups.
 template <bool SOFT, int STATS, bool QUAD>
 __device__ __forceinline__ uint64_t visit_oct(const OctRec &nd, const float4 *__restrict__ qn, uint64_t mask,
                                               const float4 *__restrict__ bodies, int64_t n, float eps2, FastWave &W,
                                               bool &fail, float4 *sh = nullptr) {
-    const uint64_t acc = __ballot(oct_accepts(nd, W.px, W.py, W.pz)) & mask;
-    if (STATS >= 2) { W.visits += 1; W.lanes += __popcll(mask); }
-    if (STATS >= 1) W.np += __popcll(acc);
-    if (QUAD) {
-        if (acc) {
-            const float4 q0 = qn[0], q1 = qn[1];
-            if (__builtin_amdgcn_inverse_ballot_w64(acc))
-                interact_quad<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, q0, q1, eps2, W);
-        }
-    } else if (__builtin_amdgcn_inverse_ballot_w64(acc)) {
-        interact<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, eps2, W);
-    }
-    const uint64_t open = mask & ~acc;
+    const uint64_t open = oct_node<SOFT, STATS, QUAD>(nd, qn, mask, eps2, W);
     if (nd.topo.z >= 0) return open;
     if (open == 0) return 0;
-    const int first = nd.topo.x, cnt = nd.topo.y;
-    if (__builtin_expect((uint64_t)(uint32_t)first >= (uint64_t)n, 0)) { fail = true; return 0; }
-    if (__builtin_expect((uint64_t)(uint32_t)cnt > (uint64_t)n - (uint32_t)first, 0)) { fail = true; return 0; }
-    if (STATS >= 2) { W.leaf_visits += 1; W.leaf_lanes += __popcll(open); }
-    if (STATS >= 1) W.pp += (uint64_t)__popcll(open) * (uint64_t)cnt;
-    leaf_eval<SOFT>(bodies, first, cnt, open, eps2, W, sh);
+    if (!oct_leaf_ok<STATS>(nd, open, n, W)) { fail = true; return 0; }
+    leaf_eval<SOFT>(bodies, nd.topo.x, nd.topo.y, open, eps2, W, sh);
     return 0;
 }
 

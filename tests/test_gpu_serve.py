@@ -156,3 +156,48 @@ def test_serving_destroy_releases_waiters(nbg, gpu_ctx):
     assert not th.is_alive()
     assert "destroyed" in result["status"]
     del running
+
+
+@pytest.mark.parametrize("mode", [1, 0])
+def test_serving_express_lane_results_equal_alone(nbg, gpu_ctx, mode):
+    """With an express lane, jobs up to its size run in a second forest that
+    steps concurrently with the bulk forest; every job, in either lane, ends
+    with the bits it gets alone.  A long large job in the bulk lane does not
+    hold the small jobs back: they finish while it runs."""
+    budget = nbg.Batch.bytes_for(300000, 16)
+    xbytes = nbg.Batch.bytes_for(40000, 32)
+    srv = nbg.Server(gpu_ctx, budget + xbytes, max_systems=16, sum_mode=mode, express_max_bodies=8192,
+                     express_bytes=xbytes, express_systems=32)
+    cap, slots = srv.capacity()
+    assert 299000 <= cap <= 300000 and slots == 48
+    big_b = nbg.plummer(200000, seed=80)
+    big = srv.submit(_params(nbg, 0.5, mode=mode), big_b, 150)
+    time.sleep(0.05)
+    small = _jobs(nbg, 10, 300, 8192, 5)          # express lane
+    mid = _jobs(nbg, 3, 9000, 30000, 6)           # bulk lane, beside the big job
+    ids = [srv.submit(_params(nbg, th, mode=mode), b, st) for b, th, st in small + mid]
+    results = [srv.wait(j, timeout=300.0) for j in ids]
+    big_out, t_big = srv.wait(big, timeout=300.0)
+    st = srv.stats()
+    print(f"[serve express] mode={mode} stats {st} big {t_big}", flush=True)
+    assert st["jobs_done"] == len(ids) + 1
+    for (b, theta, steps), (out, t) in zip(small + mid, results):
+        ref = _alone(nbg, gpu_ctx, b, theta, steps, mode)
+        for g, r in zip((out.x, out.y, out.z, out.vx, out.vy, out.vz), (ref.x, ref.y, ref.z, ref.vx, ref.vy, ref.vz)):
+            assert np.array_equal(g, r), f"job n={b.n} theta={theta} steps={steps}"
+    ref = _alone(nbg, gpu_ctx, big_b, 0.5, 150, mode)
+    for g, r in zip((big_out.x, big_out.vx), (ref.x, ref.vx)):
+        assert np.array_equal(g, r)
+    # the small jobs (at most 12 steps each) finished while the 150-step
+    # 200k-body job was still on the device
+    assert max(t["latency"] for _, t in results[:len(small)]) < t_big["latency"]
+    srv.close()
+
+
+def test_serving_express_lane_arguments(nbg, gpu_ctx):
+    budget = nbg.Batch.bytes_for(20000, 4)
+    with pytest.raises(nbg.NbgError):   # express budget not inside the server's
+        nbg.Server(gpu_ctx, budget, max_systems=4, express_max_bodies=1000, express_bytes=budget, express_systems=4)
+    with pytest.raises(nbg.NbgError):   # express lane cannot hold one job of its size
+        nbg.Server(gpu_ctx, budget, max_systems=4, express_max_bodies=10 ** 6,
+                   express_bytes=nbg.Batch.bytes_for(1000, 4), express_systems=4)
