@@ -26,6 +26,37 @@ def test_exports_every_declared_symbol(nbg):
     assert sorted(nbg.EXPORTS) == declared
 
 
+def test_params_layout_matches_header(nbg, tmp_path):
+    """nbg_params as the C compiler lays it out (size and every field's
+    offset) equals the ctypes mirror the bindings pass by pointer."""
+    import shutil
+    import subprocess
+
+    import pytest
+
+    if shutil.which("gcc") is None:
+        pytest.skip("needs gcc")
+    fields = [f for f, _ in nbg.Params._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stddef.h>\n#include <stdio.h>\n#include \"nbg.h\"\nint main(void) {\n"
+                   + '  printf("size %zu\\n", sizeof(nbg_params));\n'
+                   + "".join(f'  printf("{f} %zu\\n", offsetof(nbg_params, {f}));\n' for f in fields)
+                   + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                        check=True).stdout.splitlines())
+    assert int(out["size"]) == ctypes_sizeof(nbg.Params)
+    for f in fields:
+        assert int(out[f]) == getattr(nbg.Params, f).offset, f
+
+
+def ctypes_sizeof(t):
+    import ctypes
+
+    return ctypes.sizeof(t)
+
+
 def test_reference_defaults(nbg):
     p = nbg.default_params()
     assert p.solver == nbg.LINEAR_BVH
