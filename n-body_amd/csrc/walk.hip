@@ -397,9 +397,13 @@ __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_oct_ordered(WalkArg
     __shared__ int s_node_all[kOrderedWaves][MAXL];
     __shared__ int s_pos_all[kOrderedWaves][MAXL];  // next child index | next octant << 8
     __shared__ uint64_t s_mask_all[kOrderedWaves][MAXL];
-    // Per lane and level, the bracket state (R, A, B; 3 floats each): a
-    // private array indexed by the wave-uniform level (scratch memory).
+    // Per lane and level, the bracket state (R, A, B; 3 floats each).  The
+    // top frame's state lives in registers (cur); the frames below it are
+    // parked in a private array indexed by the wave-uniform level (scratch
+    // memory), written when a child is opened and read back when it
+    // completes, so the child visits in between touch registers only.
     float s_st[MAXL * 9];
+    float cur[9];
 
     const float4 *__restrict__ bodies = a.bodies;
     const OctRec *__restrict__ nodes = a.onodes;
@@ -449,18 +453,21 @@ __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_oct_ordered(WalkArg
         const OctRec xr = nodes[X];
         const int cb = xr.topo.z, nch = oct_nchild(xr);
         if (cb < 0 || nch < 1 || nch > 8 || (int64_t)cb + nch > a.max_nodes) { fail = true; break; }
-        float *st = s_st + top * 9;
         int j = pos & 0xFF, q = pos >> 8;
         if (j > 0) {
             const int k = oct_octant(nodes[cb + j - 1]);
-            for (; q < k; ++q) oct_step<MAXL>(st, q, act, 0.0f, 0.0f, 0.0f);
-            oct_step<MAXL>(st, k, act, rx, ry, rz);
+            for (; q < k; ++q) oct_step<MAXL>(cur, q, act, 0.0f, 0.0f, 0.0f);
+            oct_step<MAXL>(cur, k, act, rx, ry, rz);
             q = k + 1;
         }
         if (j == nch) {
-            for (; q < 8; ++q) oct_step<MAXL>(st, q, act, 0.0f, 0.0f, 0.0f);
-            if (act) { rx = st[0]; ry = st[1]; rz = st[2]; }
+            for (; q < 8; ++q) oct_step<MAXL>(cur, q, act, 0.0f, 0.0f, 0.0f);
+            if (act) { rx = cur[0]; ry = cur[1]; rz = cur[2]; }
             sp = top;
+            if (top > 0) {  // the parent frame is the top again
+#pragma unroll
+                for (int i = 0; i < 9; ++i) cur[i] = s_st[(top - 1) * 9 + i];
+            }
             continue;
         }
         if (lane == 0) s_pos[top] = (j + 1) | (q << 8);
@@ -470,6 +477,10 @@ __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_oct_ordered(WalkArg
         if (open) {
             if (sp >= MAXL) { fail = true; break; }
             if (lane == 0) { s_node[sp] = cb + j; s_pos[sp] = 0; s_mask[sp] = open; }
+            // park this frame's state; the child's frame starts in registers
+            // (its first octant step writes before anything reads)
+#pragma unroll
+            for (int i = 0; i < 9; ++i) s_st[top * 9 + i] = cur[i];
             ++sp;
         }
     }
