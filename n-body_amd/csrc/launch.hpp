@@ -28,8 +28,6 @@ constexpr int kScanTile = kScanBlock * kScanItems;
 // Body count above which the tree arrays' 32-bit indices would overflow.
 constexpr int64_t kMaxBodies = (int64_t)1 << 29;
 
-// Per-wave traversal stack entries.
-constexpr int kStackLBVH = 192;
 
 
 

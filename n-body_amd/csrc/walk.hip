@@ -86,14 +86,24 @@ __device__ __forceinline__ bool lbvh_accepts(float4 mc, float px, float py, floa
 // ---------------------------------------------------------------- ORDERED
 constexpr int kOrderedWaves = 4;  // waves per workgroup of the ORDERED walks
 
+// Levels of pending partial sums an ORDERED linear-BVH wave keeps in LDS.
+constexpr int kSaveRing = 4;
+
 template <int MAXD, bool FOREST, bool QUAD>
 __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_lbvh_ordered(WalkArgs a) {
-    __shared__ int s_node_all[kOrderedWaves][kStackLBVH];
-    __shared__ int s_meta_all[kOrderedWaves][kStackLBVH];
-    __shared__ uint64_t s_mask_all[kOrderedWaves][kStackLBVH];
-    // Per lane, the pending partial sum of every open depth: a private array
-    // indexed by the wave-uniform depth (scratch memory, coalesced across
-    // lanes), so the walk's occupancy is not bounded by LDS.
+    // The traversal stack holds, per level of the current path, at most the
+    // completion marker and the second child, plus the three entries of a push.
+    constexpr int kStack = 2 * MAXD + 4;
+    __shared__ int s_node_all[kOrderedWaves][kStack];
+    __shared__ int s_meta_all[kOrderedWaves][kStack];
+    __shared__ uint64_t s_mask_all[kOrderedWaves][kStack];
+    // Per lane, the pending partial sum of every open depth.  The pending
+    // depths are those of the current path, saved in increasing depth and
+    // taken back in decreasing depth, so the deepest kSaveRing of them live
+    // in an LDS ring (slot d mod kSaveRing, lane-major: no bank conflicts)
+    // and only older ones go to a private array indexed by the wave-uniform
+    // depth (scratch memory): depths >= ring_base are in the ring.
+    __shared__ float s_ring_all[kOrderedWaves][kSaveRing * 3 * kWave];
     float s_save[MAXD * 3];
 
     const float4 *__restrict__ bodies = a.bodies;
@@ -102,6 +112,7 @@ __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_lbvh_ordered(WalkAr
     int *s_node = s_node_all[wv];
     int *s_meta = s_meta_all[wv];
     uint64_t *s_mask = s_mask_all[wv];
+    float *ring = s_ring_all[wv];
     const WaveTask T = wave_task<FOREST>(a, (int64_t)xcd_block(blockIdx.x, gridDim.x) * kOrderedWaves + wv);
     const int64_t tgt = T.base + lane;
     const bool valid = tgt < T.end;
@@ -116,6 +127,7 @@ __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_lbvh_ordered(WalkAr
 
     if (lane == 0) { s_node[0] = T.root; s_meta[0] = 0; s_mask[0] = vmask; }
     int sp = 1;
+    int ring_base = 0;
     bool fail = false;
     uint32_t iters = 0;
     while (sp > 0) {
@@ -128,11 +140,24 @@ __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_lbvh_ordered(WalkAr
         const bool act = (mask >> lane) & 1ull;
         const int depth = meta & 0xFFFF;
         if (meta & kMetaComplete) {
-            if (act) {
-                const float vx = rx, vy = ry, vz = rz;
-                rx = s_save[depth * 3 + 0] + vx;
-                ry = s_save[depth * 3 + 1] + vy;
-                rz = s_save[depth * 3 + 2] + vz;
+            // the deepest pending depth comes back: from the ring, or (the
+            // ring then holds nothing) from scratch
+            if (depth >= ring_base) {
+                const float *slot = ring + (depth % kSaveRing) * 3 * kWave + lane;
+                if (act) {
+                    const float vx = rx, vy = ry, vz = rz;
+                    rx = slot[0] + vx;
+                    ry = slot[kWave] + vy;
+                    rz = slot[2 * kWave] + vz;
+                }
+            } else {
+                ring_base = depth;
+                if (act) {
+                    const float vx = rx, vy = ry, vz = rz;
+                    rx = s_save[depth * 3 + 0] + vx;
+                    ry = s_save[depth * 3 + 1] + vy;
+                    rz = s_save[depth * 3 + 2] + vz;
+                }
             }
             continue;
         }
@@ -171,11 +196,21 @@ __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_lbvh_ordered(WalkAr
                 rx = rx + lx; ry = ry + ly; rz = rz + lz;
             }
         } else {
-            if (sp + 3 > kStackLBVH || depth >= MAXD) { fail = true; break; }
+            if (sp + 3 > kStack || depth >= MAXD) { fail = true; break; }
+            // pending depths ring_base .. depth - 1 are in the ring; a full
+            // ring moves its shallowest level (all lanes) to scratch
+            if (depth - ring_base >= kSaveRing) {
+                const float *old = ring + (ring_base % kSaveRing) * 3 * kWave + lane;
+                s_save[ring_base * 3 + 0] = old[0];
+                s_save[ring_base * 3 + 1] = old[kWave];
+                s_save[ring_base * 3 + 2] = old[2 * kWave];
+                ++ring_base;
+            }
             if (op) {
-                s_save[depth * 3 + 0] = rx;
-                s_save[depth * 3 + 1] = ry;
-                s_save[depth * 3 + 2] = rz;
+                float *slot = ring + (depth % kSaveRing) * 3 * kWave + lane;
+                slot[0] = rx;
+                slot[kWave] = ry;
+                slot[2 * kWave] = rz;
                 rx = 0.0f; ry = 0.0f; rz = 0.0f;
             }
             if (lane == 0) {
