@@ -263,6 +263,55 @@ __device__ __forceinline__ uint64_t visit(const NodeRec &nd, const float4 *__res
     return 0;
 }
 
+// Visits the sibling pair (r0, r1) for the lanes in `mask`: both opening
+// tests first (independent loads and ballots the scheduler can overlap),
+// then both node interactions, then the leaves; returns in o0 / o1 the
+// lanes that open each internal child.  The same interactions as visit() on
+// r0 then r1, summed in another order (FAST).
+template <bool SOFT, int STATS, bool QUAD>
+__device__ __forceinline__ void visit_pair(const NodeRec &r0, const NodeRec &r1, const float4 *__restrict__ qn,
+                                           uint64_t mask, const float4 *__restrict__ bodies, int64_t n, float eps2,
+                                           FastWave &W, bool &fail, float4 *sh, uint64_t &o0, uint64_t &o1) {
+    const uint64_t a0 = __ballot(lbvh_accepts(r0.mac, W.px, W.py, W.pz)) & mask;
+    const uint64_t a1 = __ballot(lbvh_accepts(r1.mac, W.px, W.py, W.pz)) & mask;
+    if (STATS >= 2) { W.visits += 2; W.lanes += 2 * __popcll(mask); }
+    if (STATS >= 1) W.np += __popcll(a0) + __popcll(a1);
+    if (QUAD) {
+        if (a0) {
+            const float4 q0 = load_uniform(qn), q1 = load_uniform(qn + 1);
+            if (__builtin_amdgcn_inverse_ballot_w64(a0))
+                interact_quad<SOFT>(r0.com.x, r0.com.y, r0.com.z, r0.com.w, q0, q1, eps2, W);
+        }
+        if (a1) {
+            const float4 q0 = load_uniform(qn + 2), q1 = load_uniform(qn + 3);
+            if (__builtin_amdgcn_inverse_ballot_w64(a1))
+                interact_quad<SOFT>(r1.com.x, r1.com.y, r1.com.z, r1.com.w, q0, q1, eps2, W);
+        }
+    } else {
+        if (__builtin_amdgcn_inverse_ballot_w64(a0)) interact<SOFT>(r0.com.x, r0.com.y, r0.com.z, r0.com.w, eps2, W);
+        if (__builtin_amdgcn_inverse_ballot_w64(a1)) interact<SOFT>(r1.com.x, r1.com.y, r1.com.z, r1.com.w, eps2, W);
+    }
+    o0 = mask & ~a0;
+    o1 = mask & ~a1;
+    const NodeRec *rs[2] = {&r0, &r1};
+    uint64_t *os[2] = {&o0, &o1};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const NodeRec &nd = *rs[k];
+        uint64_t &open = *os[k];
+        if (nd.topo.z >= 0 || open == 0) continue;
+        const int first = nd.topo.x, cnt = nd.topo.y;
+        const uint64_t take = open;
+        open = 0;
+        // first in [0, n) and cnt in [0, n - first] (n < 2^29: 32-bit unsigned, two scalar tests)
+        if (__builtin_expect((uint32_t)first >= (uint32_t)n, 0)) { fail = true; continue; }
+        if (__builtin_expect((uint32_t)cnt > (uint32_t)n - (uint32_t)first, 0)) { fail = true; continue; }
+        if (STATS >= 2) { W.leaf_visits += 1; W.leaf_lanes += __popcll(take); }
+        if (STATS >= 1) W.pp += (uint64_t)__popcll(take) * (uint64_t)cnt;
+        leaf_eval<SOFT>(bodies, first, cnt, take, eps2, W, sh);
+    }
+}
+
 // Forest: the same traversal for the wave's system (see wave_task).
 template <int WAVES, bool SOFT, int STATS, bool QUAD>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_forest(WalkArgs a) {
@@ -317,9 +366,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_forest(WalkArgs a) {
         c = uniform_i32(c);  // keeps the node loads scalar
         if (__builtin_expect((uint32_t)c >= last_pair, 0)) { fail = true; break; }
         if (__builtin_expect(++iters > kMaxWalkIters, 0)) { fail = true; break; }
-        const NodeRec r0 = nodes[c], r1 = nodes[c + 1];
-        const uint64_t o0 = visit<SOFT, STATS, QUAD>(r0, a.quad + 2 * c, mask, bodies, n, eps2, W, fail, s_leaf[wv]);
-        const uint64_t o1 = visit<SOFT, STATS, QUAD>(r1, a.quad + 2 * c + 2, mask, bodies, n, eps2, W, fail, s_leaf[wv]);
+        const NodeRec r0 = load_uniform(nodes + c), r1 = load_uniform(nodes + c + 1);
+        uint64_t o0, o1;
+        visit_pair<SOFT, STATS, QUAD>(r0, r1, a.quad + 2 * c, mask, bodies, n, eps2, W, fail, s_leaf[wv], o0, o1);
         // a failed visit opens nothing, so the walk still ends; fail is
         // reported after the loop
         if (o0 != 0) {
@@ -762,20 +811,17 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_multi(WalkArgs a) {
         c = uniform_i32(c);  // keeps the node loads scalar
         if (__builtin_expect((uint32_t)c >= last_pair, 0)) { fail = true; break; }
         if (__builtin_expect(++iters > kMaxWalkIters, 0)) { fail = true; break; }
-        const NodeRec r0 = nodes[c], r1 = nodes[c + 1];
+        const NodeRec r0 = load_uniform(nodes + c), r1 = load_uniform(nodes + c + 1);
         uint64_t o0[TPL], o1[TPL], any0 = 0, any1 = 0;
+        // each slot visits the pair as the one-target walk does (same order of sums)
 #pragma unroll
         for (int t = 0; t < TPL; ++t) {
-            o0[t] = mask[t] ? visit<SOFT, STATS, QUAD>(r0, a.quad + 2 * c, mask[t], bodies, n, eps2, W[t], fail,
-                                                       s_leaf[wv])
-                            : 0ull;
+            o0[t] = 0;
+            o1[t] = 0;
+            if (mask[t])
+                visit_pair<SOFT, STATS, QUAD>(r0, r1, a.quad + 2 * c, mask[t], bodies, n, eps2, W[t], fail,
+                                              s_leaf[wv], o0[t], o1[t]);
             any0 |= o0[t];
-        }
-#pragma unroll
-        for (int t = 0; t < TPL; ++t) {
-            o1[t] = mask[t] ? visit<SOFT, STATS, QUAD>(r1, a.quad + 2 * c + 2, mask[t], bodies, n, eps2, W[t], fail,
-                                                       s_leaf[wv])
-                            : 0ull;
             any1 |= o1[t];
         }
         if (any0 != 0) {
@@ -872,9 +918,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
         c = uniform_i32(c);  // keeps the node loads scalar
         if (__builtin_expect((uint32_t)c >= last_pair, 0)) { fail = true; break; }
         if (__builtin_expect(++iters > kMaxWalkIters, 0)) { fail = true; break; }
-        const NodeRec r0 = nodes[c], r1 = nodes[c + 1];
-        const uint64_t o0 = visit<SOFT, STATS, QUAD>(r0, a.quad + 2 * c, mask, bodies, n, eps2, W, fail, s_leaf[wv]);
-        const uint64_t o1 = visit<SOFT, STATS, QUAD>(r1, a.quad + 2 * c + 2, mask, bodies, n, eps2, W, fail, s_leaf[wv]);
+        const NodeRec r0 = load_uniform(nodes + c), r1 = load_uniform(nodes + c + 1);
+        uint64_t o0, o1;
+        visit_pair<SOFT, STATS, QUAD>(r0, r1, a.quad + 2 * c, mask, bodies, n, eps2, W, fail, s_leaf[wv], o0, o1);
         // a failed visit opens nothing, so the walk still ends; fail is
         // reported after the loop
         if (o0 != 0) {

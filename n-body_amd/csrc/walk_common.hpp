@@ -115,6 +115,16 @@ __device__ __forceinline__ OctRec load_uniform(const OctRec *p) {
     return r;
 }
 
+__device__ __forceinline__ NodeRec load_uniform(const NodeRec *p) {
+    const const_ptr<Body4> f = as_const(reinterpret_cast<const Body4 *>(p));
+    const const_ptr<Int4Pod> t = as_const(reinterpret_cast<const Int4Pod *>(p));
+    NodeRec r;
+    r.mac = make_float4(f[0].x, f[0].y, f[0].z, f[0].w);
+    r.com = make_float4(f[1].x, f[1].y, f[1].z, f[1].w);
+    r.topo = make_int4(t[2].x, t[2].y, t[2].z, t[2].w);
+    return r;
+}
+
 // The lanes in `take` interact with the `cnt` bodies starting at `first`.
 template <bool SOFT>
 __device__ __forceinline__ void leaf_bodies(const float4 *__restrict__ bodies, int first, int cnt, uint64_t take,
