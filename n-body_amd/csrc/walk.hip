@@ -15,15 +15,18 @@
 // (Physics/Rules/Gravity.h:199-209), leaf sums sequential from zero
 // (activeTree.h:44-52) and child results bracketed as libstdc++'s
 // std::transform_reduce does for two children, (0 + r0) + r1
-// (activeTree.h:66-76).  Each lane's pending partial sums live in a private
-// array, one slot per tree depth (the depth is wave-uniform).
+// (activeTree.h:66-76).  Each lane's pending partial sums of the open depths
+// (the depth is wave-uniform) sit in an LDS ring for the deepest levels and a
+// private array for the rest (linear BVH), or in registers for the top frame
+// and a private array below it (octree).
 //
 // FAST: the same opening decisions (the MAC is still exact) and so the same
-// interaction set, evaluated two siblings per step.  All control flow is on
-// wave-uniform ballots; the lanes that take an interaction are selected by
-// turning the ballot back into an execution mask (inverse ballot), so no
-// per-lane predicate is computed in vector registers.  Interactions are
-// accumulated with rsqrt and FMA into one register triple in traversal order.
+// interaction set, evaluated two siblings (linear BVH) or one child group
+// (octree) per step.  All control flow is on wave-uniform ballots; the lanes
+// that take an interaction are selected by turning the ballot back into an
+// execution mask (inverse ballot), so no per-lane predicate is computed in
+// vector registers.  Interactions use rsqrt and FMA, summed in FP32 per leaf
+// and in FP64 across leaves (walk_common.hpp).
 
 #include "device.hpp"
 #include "launch.hpp"
