@@ -864,8 +864,7 @@ def main():
                 a2 = copy.copy(args)
                 a2.steps, a2.warmup = args.ordered_steps, 1
                 log(f"[bench] {SOLVER_NAMES[sv]} ORDERED")
-                m2 = measure(nbg, ctx, b, a2, sv, mode="ordered")
-                m2["roofline"]["traffic"] = None  # the profiled walk traffic is the FAST walk's
+                m2 = measure(nbg, ctx, b, a2, sv, mode="ordered")  # traffic: the "-ordered" rows
                 ordered[sv] = m2
                 extra[f"{SOLVER_NAMES[sv].replace('-', '_')}_ordered"] = m2
     if not args.no_cpu:
