@@ -398,6 +398,37 @@ int nbg_server_wait(nbg_server *s, int64_t job, double timeout_s, float *x, floa
  * waiting}. */
 int nbg_server_stats(nbg_server *s, double *out8);
 
+/* ---- job snapshots ----
+ *
+ * No reference counterpart: the reference saves whole scenarios as JSON
+ * (src/NBody/Simulation/Simulation.cpp:53-63,154-228) with no step counter.
+ * A snapshot is one contiguous little-endian buffer holding a job between
+ * two steps:
+ *   [0, 64)    "NBGSNAP1", u32 version 1, u32 sizeof(nbg_params), i64 n,
+ *              i32 steps_total, i32 steps_done, u64 checksum (FNV-1a 64 of
+ *              every byte from offset 64 on), 24 zero bytes
+ *   [64, 112)  the job's nbg_params, zero padded
+ *   [112, 112 + 28 n)  float32 x, y, z, m, vx, vy, vz, n values each
+ * A step reads only positions, velocities and masses, so a job resumed from
+ * a snapshot ends with the same bits as the job run uninterrupted, on a
+ * server of the same sum mode.  Buffers must be 4-byte aligned. */
+
+/* Copies a job's state into buf (cap bytes) and lets the job run on;
+ * *size receives the snapshot's size, and buf NULL only asks for it.  A
+ * resident job is copied at its forest's next step boundary (the call waits
+ * for at most one forest step); a queued job gives its submitted state, a
+ * finished one its final state (steps_done == steps_total), a failed one its
+ * error. */
+int nbg_server_checkpoint(nbg_server *s, int64_t job, void *buf, int64_t cap, int64_t *size);
+/* The same, and the job leaves the server: its forest slot frees, clients
+ * waiting on it return NBG_ERR_STATE and its id is forgotten.  A finished
+ * job is collected with nbg_server_wait instead (NBG_ERR_STATE here). */
+int nbg_server_suspend(nbg_server *s, int64_t job, void *buf, int64_t cap, int64_t *size);
+/* Queues the job a snapshot holds for its remaining steps, under a new id.
+ * NBG_ERR_ARG for a damaged snapshot (checksum), another format version or
+ * parameter layout, a size that does not match n, or no steps left. */
+int nbg_server_resume(nbg_server *s, const void *buf, int64_t size, int64_t *job);
+
 /* ---- synthetic inputs (host, CPU) ---- */
 
 /* Plummer sphere: scale radius a, total mass M, gravitational constant G,

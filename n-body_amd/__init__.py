@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import struct
 from dataclasses import dataclass
 
 import numpy as np
@@ -50,7 +51,7 @@ EXPORTS = (
     "nbg_batch_autotune", "nbg_batch_autotune_grid", "nbg_batch_set_targets_per_lane", "nbg_batch_nonfinite",
     "nbg_scenario_load_json", "nbg_scenario_load_tipsy", "nbg_scenario_count", "nbg_scenario_get", "nbg_scenario_free",
     "nbg_server_create", "nbg_server_create_express", "nbg_server_destroy", "nbg_server_capacity", "nbg_server_submit", "nbg_server_wait",
-    "nbg_server_stats",
+    "nbg_server_stats", "nbg_server_checkpoint", "nbg_server_suspend", "nbg_server_resume",
 )
 
 
@@ -155,6 +156,12 @@ def lib() -> ctypes.CDLL:
             "nbg_server_wait": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_double] + [_f32p] * 6
                                 + [ctypes.POINTER(ctypes.c_double)]),
             "nbg_server_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
+            "nbg_server_checkpoint": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
+                                                     ctypes.POINTER(ctypes.c_int64)]),
+            "nbg_server_suspend": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
+                                                  ctypes.POINTER(ctypes.c_int64)]),
+            "nbg_server_resume": (ctypes.c_int, [_vp, ctypes.c_void_p, ctypes.c_int64,
+                                                 ctypes.POINTER(ctypes.c_int64)]),
             "nbg_sys_step": (ctypes.c_int, [_vp, ctypes.c_int32]),
             "nbg_sys_synchronize": (ctypes.c_int, [_vp]),
             "nbg_sys_counts": (ctypes.c_int, [_vp, _u64p, _u64p]),
@@ -617,6 +624,43 @@ class Server:
         _check(lib().nbg_server_stats(self.handle, o))
         keys = ("jobs_done", "steps_run", "body_steps", "busy_s", "wall_s", "peak_bodies", "peak_systems", "waiting")
         return {k: float(v) for k, v in zip(keys, o)}
+
+    def _snapshot(self, fn, job: int) -> bytes:
+        size = ctypes.c_int64()
+        _check(fn(self.handle, job, None, 0, ctypes.byref(size)))
+        buf = np.empty(size.value, dtype=np.uint8)
+        _check(fn(self.handle, job, buf.ctypes.data_as(ctypes.c_void_p), size.value, ctypes.byref(size)))
+        return buf.tobytes()
+
+    def checkpoint(self, job: int) -> bytes:
+        """The job's state at its forest's next step boundary, as a snapshot
+        (nbg_server_checkpoint); the job runs on."""
+        return self._snapshot(lib().nbg_server_checkpoint, job)
+
+    def suspend(self, job: int) -> bytes:
+        """The job's snapshot, and the job leaves the server
+        (nbg_server_suspend)."""
+        snap = self._snapshot(lib().nbg_server_suspend, job)
+        self._n.pop(job, None)
+        return snap
+
+    def resume(self, snapshot: bytes) -> int:
+        """Queues the job a snapshot holds for its remaining steps; returns
+        its new id (nbg_server_resume)."""
+        buf = np.frombuffer(snapshot, dtype=np.uint8).copy()  # aligned
+        job = ctypes.c_int64()
+        _check(lib().nbg_server_resume(self.handle, buf.ctypes.data_as(ctypes.c_void_p), buf.size,
+                                       ctypes.byref(job)))
+        self._n[job.value] = snapshot_info(snapshot)["n"]
+        return job.value
+
+
+def snapshot_info(snapshot: bytes) -> dict:
+    """Header fields of a job snapshot (nbg_server_checkpoint's format)."""
+    if len(snapshot) < 64 or snapshot[:8] != b"NBGSNAP1":
+        raise ValueError("not a job snapshot")
+    version, params_bytes, n, total, done = struct.unpack_from("<IIqii", snapshot, 8)
+    return {"version": version, "params_bytes": params_bytes, "n": n, "steps_total": total, "steps_done": done}
 
 
 # ------------------------------------------------------------------ reference-shaped facade

@@ -9,6 +9,12 @@
 // that have reached their step count (final state copied to host memory,
 // slot freed).  A job therefore runs alongside whatever else is on the
 // device, and its result is the same bits it gets alone.
+//
+// A job's state can be copied out between two steps into a snapshot buffer
+// (checkpoint: the job runs on; suspend: it leaves the server) and queued
+// again from one (resume).  Only a lane's worker touches its forest, so a
+// snapshot of a resident job is taken by that worker at its next step
+// boundary while the client waits.
 
 #include <hip/hip_runtime.h>
 
@@ -16,6 +22,8 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdint>
+#include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
@@ -31,6 +39,8 @@
 
 using namespace nbg;
 
+struct ServeLane;
+
 namespace {
 
 using Clock = std::chrono::steady_clock;
@@ -40,6 +50,8 @@ struct Job {
     nbg_params p{};
     int64_t n = 0;
     int32_t steps = 0, remaining = 0;
+    int32_t steps_before = 0;   // steps run before the snapshot this job resumed from
+    ServeLane *lane = nullptr;  // the forest it is queued for
     std::vector<float> st[7];   // x y z m vx vy vz in; x y z vx vy vz out
     int32_t sys = -1;           // forest handle while admitted
     int32_t bypassed = 0;       // later jobs admitted while this one waited
@@ -48,6 +60,66 @@ struct Job {
     int status = NBG_OK;
     std::string error;
 };
+
+// A snapshot asked of a job that may be resident in a forest.  The lane's
+// worker fills the arrays section of the client's buffer between two steps.
+struct SnapRequest {
+    std::shared_ptr<Job> job;
+    bool leave = false;       // suspend: the job leaves the server
+    char *arrays = nullptr;   // the buffer's x y z m vx vy vz section
+    int32_t steps_done = 0;
+    bool done = false;
+    int status = NBG_OK;
+    std::string error;
+};
+
+// Job snapshot, one contiguous little-endian buffer:
+//   [0, 64)    header (SnapHeader)
+//   [64, 112)  nbg_params, zero padded
+//   [112, ...) float32 arrays x, y, z, m, vx, vy, vz of n values each
+// The checksum is FNV-1a 64 over every byte from offset 64 to the end.
+struct SnapHeader {
+    char magic[8];          // "NBGSNAP1"
+    uint32_t version;       // 1
+    uint32_t params_bytes;  // sizeof(nbg_params)
+    int64_t n;
+    int32_t steps_total;    // steps of the job as first submitted
+    int32_t steps_done;     // steps run when the snapshot was taken
+    uint64_t checksum;
+    char reserved[24];
+};
+static_assert(sizeof(SnapHeader) == 64, "snapshot header layout");
+constexpr char kSnapMagic[8] = {'N', 'B', 'G', 'S', 'N', 'A', 'P', '1'};
+constexpr int64_t kSnapArrays = 64 + (int64_t)((sizeof(nbg_params) + 15) / 16 * 16);
+
+int64_t snap_bytes(int64_t n) { return kSnapArrays + 7 * (int64_t)sizeof(float) * n; }
+
+uint64_t fnv1a64(const unsigned char *p, int64_t len) {
+    uint64_t h = 14695981039346656037ull;
+    for (int64_t i = 0; i < len; ++i) h = (h ^ p[i]) * 1099511628211ull;
+    return h;
+}
+
+// The host copy of a job's state (submitted state while queued, final state
+// once finished) into a snapshot's arrays section.
+void snap_copy_host(char *arrays, const Job &j) {
+    for (int k = 0; k < 7; ++k) std::memcpy(arrays + (size_t)k * j.n * sizeof(float), j.st[k].data(), j.n * sizeof(float));
+}
+
+// Header, parameters and checksum around arrays already in place.
+void snap_finish(char *buf, const Job &j, int32_t steps_done) {
+    std::memset(buf + 64, 0, (size_t)(kSnapArrays - 64));
+    std::memcpy(buf + 64, &j.p, sizeof(nbg_params));
+    SnapHeader h{};
+    std::memcpy(h.magic, kSnapMagic, 8);
+    h.version = 1;
+    h.params_bytes = (uint32_t)sizeof(nbg_params);
+    h.n = j.n;
+    h.steps_total = j.steps_before + j.steps;
+    h.steps_done = steps_done;
+    h.checksum = fnv1a64((const unsigned char *)buf + 64, snap_bytes(j.n) - 64);
+    std::memcpy(buf, &h, sizeof h);
+}
 
 double secs(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
 
@@ -67,6 +139,7 @@ struct ServeLane {
     int64_t max_job = 0;  // largest job routed to this lane
     std::deque<std::shared_ptr<Job>> pending;
     std::vector<std::shared_ptr<Job>> active;
+    std::vector<std::shared_ptr<SnapRequest>> requests;  // snapshots of jobs not queued here
     int64_t used_n = 0;
     std::thread worker;
 };
@@ -92,13 +165,77 @@ struct nbg_server {
 
 namespace {
 
+// A suspended job ends here: clients waiting on it get NBG_ERR_STATE and its
+// id is forgotten.  Called with S->mu held.
+void end_suspended(nbg_server *S, Job &j) {
+    j.status = NBG_ERR_STATE;
+    j.error = "the job was suspended into a snapshot";
+    j.t_done = Clock::now();
+    j.done = true;
+    S->jobs.erase(j.id);
+}
+
+// Fulfils snapshot requests at a step boundary (worker thread).  resident[i]:
+// request i's job is in L->active, so its state is in the forest.
+void serve_snapshots(nbg_server *S, ServeLane *L, const std::vector<std::shared_ptr<SnapRequest>> &reqs,
+                     const std::vector<char> &resident) {
+    for (size_t i = 0; i < reqs.size(); ++i) {
+        if (!resident[i]) continue;
+        SnapRequest &q = *reqs[i];
+        const Job &j = *q.job;
+        float *a = reinterpret_cast<float *>(q.arrays);
+        const int64_t n = j.n;
+        const int r = nbg_batch_get(L->batch, j.sys, a, a + n, a + 2 * n, a + 4 * n, a + 5 * n, a + 6 * n, nullptr,
+                                    nullptr, nullptr);
+        std::memcpy(a + 3 * n, j.st[3].data(), (size_t)n * sizeof(float));  // masses never change
+        if (r != NBG_OK) {
+            q.status = r;
+            q.error = nbg_last_error();
+        } else if (q.leave) {
+            (void)nbg_batch_remove(L->batch, j.sys);
+        }
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    for (size_t i = 0; i < reqs.size(); ++i) {
+        SnapRequest &q = *reqs[i];
+        Job &j = *q.job;
+        if (resident[i]) {
+            if (q.status == NBG_OK) {
+                q.steps_done = j.steps_before + j.steps - j.remaining;
+                if (q.leave) {
+                    L->active.erase(std::find(L->active.begin(), L->active.end(), q.job));
+                    L->used_n -= j.n;
+                    S->resident_n -= j.n;
+                    S->resident_s -= 1;
+                    end_suspended(S, j);
+                }
+            }
+        } else if (j.done && j.status == NBG_OK && !q.leave) {
+            // finished before this step boundary: its final state
+            snap_copy_host(q.arrays, j);
+            q.steps_done = j.steps_before + j.steps;
+        } else {
+            q.status = NBG_ERR_STATE;
+            q.error = !j.done ? "the job is not resident in its forest"
+                              : j.status == NBG_OK ? "the job has finished: collect it with nbg_server_wait"
+                                                   : "job failed: " + j.error;
+        }
+        q.done = true;
+    }
+    S->cv_done.notify_all();
+}
+
 void worker_loop(nbg_server *S, ServeLane *L) {
     DeviceGuard g(S->ctx->device);
     for (;;) {
         std::vector<std::shared_ptr<Job>> admit;
+        std::vector<std::shared_ptr<SnapRequest>> reqs;
+        std::vector<char> resident;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            S->cv_work.wait(lk, [&] { return S->stop || !L->pending.empty() || !L->active.empty(); });
+            S->cv_work.wait(lk, [&] {
+                return S->stop || !L->pending.empty() || !L->active.empty() || !L->requests.empty();
+            });
             if (S->stop) return;
             std::vector<Waiting> q(L->pending.size());
             for (size_t i = 0; i < q.size(); ++i) q[i] = Waiting{L->pending[i]->n, L->pending[i]->bypassed};
@@ -118,7 +255,11 @@ void worker_loop(nbg_server *S, ServeLane *L) {
                 }
             }
             L->pending.swap(rest);
+            reqs.swap(L->requests);
+            for (auto &q : reqs)
+                resident.push_back(std::find(L->active.begin(), L->active.end(), q->job) != L->active.end());
         }
+        if (!reqs.empty()) serve_snapshots(S, L, reqs, resident);
         for (auto &j : admit) {
             int32_t sid = -1;
             const int r = nbg_batch_add(L->batch, &j->p, j->n, j->st[0].data(), j->st[1].data(), j->st[2].data(),
@@ -296,8 +437,14 @@ int nbg_server_destroy(nbg_server *S) {
         for (auto &L : S->lanes) {
             for (auto &j : L->pending) abandon(j);
             for (auto &j : L->active) abandon(j);
+            for (auto &q : L->requests) {
+                q->status = NBG_ERR_STATE;
+                q->error = "the server was destroyed before the snapshot was taken";
+                q->done = true;
+            }
             L->pending.clear();
             L->active.clear();
+            L->requests.clear();
         }
         S->cv_done.notify_all();
         S->cv_done.wait(lk, [&] { return S->waiters == 0; });
@@ -319,9 +466,13 @@ int nbg_server_capacity(nbg_server *S, int64_t *max_bodies, int32_t *max_systems
     return NBG_OK;
 }
 
-int nbg_server_submit(nbg_server *S, const nbg_params *p, int64_t n, const float *x, const float *y, const float *z,
-                      const float *m, const float *vx, const float *vy, const float *vz, int32_t nsteps,
-                      int64_t *job) {
+}  // extern "C"
+
+namespace {
+
+int submit_job(nbg_server *S, const nbg_params *p, int64_t n, const float *x, const float *y, const float *z,
+               const float *m, const float *vx, const float *vy, const float *vz, int32_t nsteps,
+               int32_t steps_before, int64_t *job) {
     if (!S || !p || !job) return fail(NBG_ERR_ARG, "server, params or job is NULL");
     if (n < 1 || !x || !y || !z || !m) return fail(NBG_ERR_ARG, "a job needs n >= 1 positions and masses");
     if (nsteps < 1) return fail(NBG_ERR_ARG, "a job needs at least one step");
@@ -342,6 +493,8 @@ int nbg_server_submit(nbg_server *S, const nbg_params *p, int64_t n, const float
     j->p = *p;
     j->n = n;
     j->steps = j->remaining = nsteps;
+    j->steps_before = steps_before;
+    j->lane = L;
     const float *src[7] = {x, y, z, m, vx, vy, vz};
     for (int k = 0; k < 7; ++k) {
         if (src[k]) j->st[k].assign(src[k], src[k] + n);
@@ -358,6 +511,101 @@ int nbg_server_submit(nbg_server *S, const nbg_params *p, int64_t n, const float
     }
     S->cv_work.notify_all();
     return NBG_OK;
+}
+
+int snapshot_job(nbg_server *S, int64_t job, bool leave, void *buf, int64_t cap, int64_t *size) {
+    if (!S || !size) return fail(NBG_ERR_ARG, "server or size is NULL");
+    std::shared_ptr<Job> j;
+    int32_t steps_done = 0;
+    {
+        std::unique_lock<std::mutex> lk(S->mu);
+        auto it = S->jobs.find(job);
+        if (it == S->jobs.end()) return fail(NBG_ERR_ARG, "unknown job id");
+        j = it->second;
+        *size = snap_bytes(j->n);
+        if (!buf) return NBG_OK;
+        if (cap < *size) return fail(NBG_ERR_ARG, "the snapshot buffer is smaller than the job's snapshot");
+        if (reinterpret_cast<uintptr_t>(buf) % alignof(float))
+            return fail(NBG_ERR_ARG, "the snapshot buffer must be 4-byte aligned");
+        char *arrays = static_cast<char *>(buf) + kSnapArrays;
+        ServeLane *L = j->lane;
+        auto pit = std::find(L->pending.begin(), L->pending.end(), j);
+        if (j->done) {
+            if (j->status != NBG_OK) return fail(j->status, "job failed: " + j->error);
+            if (leave) return fail(NBG_ERR_STATE, "the job has finished: collect it with nbg_server_wait");
+            snap_copy_host(arrays, *j);  // its final state
+            steps_done = j->steps_before + j->steps;
+        } else if (pit != L->pending.end()) {
+            // still queued: the submitted state (not admitted while the lock is held)
+            snap_copy_host(arrays, *j);
+            steps_done = j->steps_before;
+            if (leave) {
+                L->pending.erase(pit);
+                end_suspended(S, *j);
+                S->cv_done.notify_all();
+            }
+        } else {
+            // resident (or being admitted): the lane's worker copies it at its
+            // next step boundary
+            auto q = std::make_shared<SnapRequest>();
+            q->job = j;
+            q->leave = leave;
+            q->arrays = arrays;
+            L->requests.push_back(q);
+            S->cv_work.notify_all();
+            ++S->waiters;
+            S->cv_done.wait(lk, [&] { return q->done; });
+            --S->waiters;
+            // as in nbg_server_wait: S is not touched after this block
+            if (S->waiters == 0) S->cv_done.notify_all();
+            if (q->status != NBG_OK) return fail(q->status, q->error);
+            steps_done = q->steps_done;
+        }
+    }
+    snap_finish(static_cast<char *>(buf), *j, steps_done);
+    return NBG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nbg_server_submit(nbg_server *S, const nbg_params *p, int64_t n, const float *x, const float *y, const float *z,
+                      const float *m, const float *vx, const float *vy, const float *vz, int32_t nsteps,
+                      int64_t *job) {
+    return submit_job(S, p, n, x, y, z, m, vx, vy, vz, nsteps, 0, job);
+}
+
+int nbg_server_checkpoint(nbg_server *S, int64_t job, void *buf, int64_t cap, int64_t *size) {
+    return snapshot_job(S, job, false, buf, cap, size);
+}
+
+int nbg_server_suspend(nbg_server *S, int64_t job, void *buf, int64_t cap, int64_t *size) {
+    return snapshot_job(S, job, true, buf, cap, size);
+}
+
+int nbg_server_resume(nbg_server *S, const void *buf, int64_t size, int64_t *job) {
+    if (!S || !buf || !job) return fail(NBG_ERR_ARG, "server, snapshot or job is NULL");
+    if (reinterpret_cast<uintptr_t>(buf) % alignof(float))
+        return fail(NBG_ERR_ARG, "the snapshot buffer must be 4-byte aligned");
+    if (size < kSnapArrays) return fail(NBG_ERR_ARG, "the buffer is too short to hold a job snapshot");
+    const char *b = static_cast<const char *>(buf);
+    SnapHeader h;
+    std::memcpy(&h, b, sizeof h);
+    if (std::memcmp(h.magic, kSnapMagic, 8) != 0 || h.version != 1 || h.params_bytes != sizeof(nbg_params))
+        return fail(NBG_ERR_ARG, "not a version-1 job snapshot with this library's parameter layout");
+    if (h.n < 1 || h.n > (size - kSnapArrays) / (7 * (int64_t)sizeof(float)) || size != snap_bytes(h.n))
+        return fail(NBG_ERR_ARG, "the snapshot's size does not match its body count");
+    if (h.steps_done < 0 || h.steps_total < 1 || h.steps_done >= h.steps_total)
+        return fail(NBG_ERR_ARG, "the snapshot's job has no steps left to run");
+    if (fnv1a64(reinterpret_cast<const unsigned char *>(b) + 64, size - 64) != h.checksum)
+        return fail(NBG_ERR_ARG, "the snapshot's checksum does not match its contents");
+    nbg_params p;
+    std::memcpy(&p, b + 64, sizeof p);
+    const float *a = reinterpret_cast<const float *>(b + kSnapArrays);
+    const int64_t n = h.n;
+    return submit_job(S, &p, n, a, a + n, a + 2 * n, a + 3 * n, a + 4 * n, a + 5 * n, a + 6 * n,
+                      h.steps_total - h.steps_done, h.steps_done, job);
 }
 
 int nbg_server_wait(nbg_server *S, int64_t job, double timeout_s, float *x, float *y, float *z, float *vx, float *vy,

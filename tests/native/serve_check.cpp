@@ -4,8 +4,9 @@
 // (nbg_batch_*) are served here by a host-memory forest with a simple
 // deterministic step, so the server's own code runs unchanged: lane worker
 // threads, admission, retirement, the non-finite guard, waiters with and
-// without time-outs, statistics read while jobs run, and destroy with a
-// client still waiting.  Exits nonzero on the first failed expectation (a
+// without time-outs, statistics read while jobs run, destroy with a client
+// still waiting, and job snapshots (checkpoint, suspend, resume) of queued,
+// resident and finished jobs.  Exits nonzero on the first failed expectation (a
 // data race aborts the run by itself).
 #include <hip/hip_runtime.h>
 
@@ -13,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <random>
@@ -242,8 +244,133 @@ static void client(nbg_server *S, int seed, int njobs) {
     }
 }
 
+static std::vector<char> take(nbg_server *S, int64_t id, bool leave, int *status = nullptr) {
+    int64_t size = 0;
+    int r = leave ? nbg_server_suspend(S, id, nullptr, 0, &size) : nbg_server_checkpoint(S, id, nullptr, 0, &size);
+    EXPECT(r == NBG_OK && size > 112, "size query of job %lld: %d", (long long)id, r);
+    std::vector<char> buf((size_t)size + 8);
+    char *b = buf.data() + (8 - reinterpret_cast<uintptr_t>(buf.data()) % 8) % 8;  // 8-byte aligned
+    r = leave ? nbg_server_suspend(S, id, b, size, &size) : nbg_server_checkpoint(S, id, b, size, &size);
+    if (status) *status = r;
+    else EXPECT(r == NBG_OK, "snapshot of job %lld: %d %s", (long long)id, r, nbg_last_error());
+    return r == NBG_OK ? std::vector<char>(b, b + size) : std::vector<char>();
+}
+
+static int64_t resume(nbg_server *S, const std::vector<char> &snap, int expect = NBG_OK) {
+    std::vector<uint64_t> al((snap.size() + 7) / 8);  // aligned copy
+    std::memcpy(al.data(), snap.data(), snap.size());
+    int64_t id = -1;
+    const int r = nbg_server_resume(S, al.data(), (int64_t)snap.size(), &id);
+    EXPECT(r == expect, "resume: %d (expected %d) %s", r, expect, nbg_last_error());
+    return id;
+}
+
+static void snap_steps(const std::vector<char> &snap, int32_t *total, int32_t *done) {
+    std::memcpy(total, snap.data() + 24, 4);
+    std::memcpy(done, snap.data() + 28, 4);
+}
+
+// The full state after the job's steps, checked against the host steps
+// applied alone without interruption.
+static void expect_result(nbg_server *S, int64_t id, const Job &j, const char *what) {
+    const int64_t n = (int64_t)j.h.s[3].size();
+    std::vector<float> out[6];
+    for (auto &v : out) v.resize(n);
+    const int r = nbg_server_wait(S, id, 120.0, out[0].data(), out[1].data(), out[2].data(), out[3].data(),
+                                  out[4].data(), out[5].data(), nullptr);
+    EXPECT(r == NBG_OK, "%s: %d %s", what, r, nbg_last_error());
+    if (r != NBG_OK) return;
+    HostSystem ref = j.h;
+    for (int s = 0; s < j.steps; ++s) host_step(ref);
+    const int map[6] = {0, 1, 2, 4, 5, 6};
+    for (int c = 0; c < 6; ++c)
+        EXPECT(std::memcmp(out[c].data(), ref.s[map[c]].data(), n * sizeof(float)) == 0, "%s: component %d", what, c);
+}
+
+static void snapshot_checks(nbg_ctx *ctx) {
+    nbg_server *S = nullptr;
+    // one lane of 20,000 bodies: a 15,000-body job keeps a 9,000-body one queued
+    EXPECT(nbg_server_create(ctx, nbg_batch_bytes(20000, 4), 4, NBG_SUM_FAST, &S) == NBG_OK, "create");
+    std::mt19937 g(11);
+    const Job big = make_job(g, 15000, 400, false), queued = make_job(g, 9000, 30, false);
+    const int64_t ib = submit(S, big), iq = submit(S, queued);
+    // a queued job: its submitted state, steps_done 0; suspended and resumed
+    std::vector<char> sq = take(S, iq, true);
+    int32_t tot = 0, done = -1;
+    snap_steps(sq, &tot, &done);
+    EXPECT(tot == 30 && done == 0, "queued snapshot steps %d/%d", done, tot);
+    std::vector<float> o(9000);
+    EXPECT(nbg_server_wait(S, iq, 0.0, o.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr) ==
+               NBG_ERR_ARG, "a suspended job's id is forgotten");
+    // a resident job: checkpoint (runs on) and later suspend; both resumed
+    while (true) {
+        double st[8];
+        nbg_server_stats(S, st);
+        if (st[1] >= 3) break;  // a few forest steps have run
+        std::this_thread::yield();
+    }
+    std::vector<char> c1 = take(S, ib, false);
+    snap_steps(c1, &tot, &done);
+    EXPECT(tot == 400 && done > 0 && done < 400, "resident checkpoint steps %d/%d", done, tot);
+    const int64_t ic = resume(S, c1);
+    std::vector<char> s1 = take(S, ib, true);  // the original leaves
+    int32_t done2 = 0;
+    snap_steps(s1, &tot, &done2);
+    EXPECT(done2 >= done && done2 < 400, "suspend after checkpoint steps %d then %d", done, done2);
+    const int64_t is = resume(S, s1);
+    const int64_t iq2 = resume(S, sq);
+    // a resumed job checkpointed again carries the steps before its snapshot
+    std::vector<char> c2 = take(S, ic, false);
+    int32_t done3 = 0;
+    snap_steps(c2, &tot, &done3);
+    EXPECT(tot == 400 && done3 >= done, "checkpoint of a resumed job: %d/%d", done3, tot);
+    const int64_t ic2 = resume(S, c2);
+    // a finished job: checkpoint gives its final state (steps_done ==
+    // steps_total), which has no steps left to resume; suspend is refused
+    const Job one = make_job(g, 10, 1, false);
+    const int64_t ifin = submit(S, one);
+    bool finished = false;
+    for (int k = 0; k < 20000 && !finished; ++k) {
+        int r = 0;
+        std::vector<char> f = take(S, ifin, false, &r);
+        EXPECT(r == NBG_OK, "checkpoint of a 1-step job: %d %s", r, nbg_last_error());
+        if (r != NBG_OK) break;
+        snap_steps(f, &tot, &done);
+        if (tot == done) {
+            finished = true;
+            HostSystem ref = one.h;
+            host_step(ref);
+            EXPECT(std::memcmp(f.data() + 112, ref.s[0].data(), 40) == 0, "final x in a finished job's snapshot");
+            resume(S, f, NBG_ERR_ARG);
+            take(S, ifin, true, &r);
+            EXPECT(r == NBG_ERR_STATE, "suspend of a finished job: %d", r);
+        } else {
+            std::this_thread::sleep_for(std::chrono::milliseconds(1));
+        }
+    }
+    EXPECT(finished, "the 1-step job finished");
+    expect_result(S, ifin, one, "a finished job after its checkpoint");
+    // damaged and foreign buffers are refused
+    std::vector<char> bad = sq;
+    bad[200] ^= 1;
+    resume(S, bad, NBG_ERR_ARG);
+    bad = sq;
+    bad.resize(bad.size() - 4);
+    resume(S, bad, NBG_ERR_ARG);
+    bad = sq;
+    bad[0] = 'X';
+    resume(S, bad, NBG_ERR_ARG);
+    // every resumed job ends with the uninterrupted bits
+    expect_result(S, ic, big, "resumed checkpoint");
+    expect_result(S, is, big, "resumed suspension");
+    expect_result(S, ic2, big, "resumed checkpoint of a resumed job");
+    expect_result(S, iq2, queued, "resumed queued job");
+    nbg_server_destroy(S);
+}
+
 int main() {
     nbg_ctx ctx;
+    snapshot_checks(&ctx);
     const int64_t budget = nbg_batch_bytes(20000, 16), xbytes = nbg_batch_bytes(6000, 32);
     for (int express = 0; express < 2; ++express) {
         nbg_server *S = nullptr;

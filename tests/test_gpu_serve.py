@@ -201,3 +201,54 @@ def test_serving_express_lane_arguments(nbg, gpu_ctx):
     with pytest.raises(nbg.NbgError):   # express lane cannot hold one job of its size
         nbg.Server(gpu_ctx, budget, max_systems=4, express_max_bodies=10 ** 6,
                    express_bytes=nbg.Batch.bytes_for(1000, 4), express_systems=4)
+
+
+def _same_bits(out, ref, what):
+    for g, r in zip((out.x, out.y, out.z, out.vx, out.vy, out.vz), (ref.x, ref.y, ref.z, ref.vx, ref.vy, ref.vz)):
+        assert np.array_equal(g, r), what
+
+
+@pytest.mark.parametrize("mode", [1, 0])
+def test_serving_snapshot_resume_equals_uninterrupted(nbg, gpu_ctx, mode):
+    """Job snapshots: a resident job checkpointed (it runs on), a resident job
+    suspended, and a queued job suspended, each resumed from its snapshot on
+    this server or on a second one, end with the bits of the same system
+    stepped alone without interruption.  A queued job's snapshot holds its
+    submitted state; a suspended job's id is forgotten."""
+    budget = nbg.Batch.bytes_for(60000, 16)
+    srv = nbg.Server(gpu_ctx, budget, max_systems=16, sum_mode=mode)
+    srv2 = nbg.Server(gpu_ctx, budget, max_systems=16, sum_mode=mode)
+    a = (nbg.plummer(40000, seed=51), 0.5, 200)
+    b = (nbg.cube(15000, seed=52, side=2.0), 0.3, 150)
+    c = (nbg.plummer(30000, seed=53), 0.7, 10)     # 85k bodies with a: queued
+    ia, ib, ic = [srv.submit(_params(nbg, th, mode=mode), body, st) for body, th, st in (a, b, c)]
+    t0 = time.time()
+    while srv.stats()["steps_run"] < 3 and time.time() - t0 < 60.0:
+        time.sleep(0.001)
+    snap_c = srv.suspend(ic)
+    info = nbg.snapshot_info(snap_c)
+    assert info["steps_done"] == 0 and info["steps_total"] == 10 and info["n"] == 30000
+    arrays = np.frombuffer(snap_c, dtype=np.float32, offset=112).reshape(7, 30000)
+    for k, v in enumerate((c[0].x, c[0].y, c[0].z, c[0].m, c[0].vx, c[0].vy, c[0].vz)):
+        assert np.array_equal(arrays[k], v)
+    srv._n[ic] = c[0].n
+    with pytest.raises(nbg.NbgError, match="unknown job"):
+        srv.wait(ic, timeout=0.0)
+    snap_a = srv.checkpoint(ia)
+    snap_b = srv.suspend(ib)
+    ia_info, ib_info = nbg.snapshot_info(snap_a), nbg.snapshot_info(snap_b)
+    print(f"[serve snapshot] mode={mode} a {ia_info} b {ib_info}", flush=True)
+    assert 0 < ia_info["steps_done"] < 200 and 0 < ib_info["steps_done"] < 150
+    ra = srv2.resume(snap_a)
+    rb = srv.resume(snap_b)
+    rc = srv2.resume(snap_c)
+    _same_bits(srv.wait(ia, timeout=300.0)[0], _alone(nbg, gpu_ctx, a[0], a[1], a[2], mode), "checkpointed job")
+    _same_bits(srv2.wait(ra, timeout=300.0)[0], _alone(nbg, gpu_ctx, a[0], a[1], a[2], mode), "resumed checkpoint")
+    _same_bits(srv.wait(rb, timeout=300.0)[0], _alone(nbg, gpu_ctx, b[0], b[1], b[2], mode), "resumed suspension")
+    _same_bits(srv2.wait(rc, timeout=300.0)[0], _alone(nbg, gpu_ctx, c[0], c[1], c[2], mode), "resumed queued job")
+    bad = bytearray(snap_b)
+    bad[1000] ^= 0x10
+    with pytest.raises(nbg.NbgError, match="checksum"):
+        srv.resume(bytes(bad))
+    srv.close()
+    srv2.close()
