@@ -41,7 +41,7 @@ EXPORTS = (
     "nbg_sys_set_params", "nbg_sys_get_params", "nbg_sys_set_state", "nbg_sys_get_state",
     "nbg_sys_update_accelerations", "nbg_sys_get_accelerations", "nbg_sys_accelerations_at", "nbg_sys_accelerations_at_noalloc", "nbg_sys_step",
     "nbg_sys_synchronize", "nbg_sys_counts", "nbg_sys_counts4", "nbg_sys_enable_counts", "nbg_sys_enable_walk_stats",
-    "nbg_sys_walk_stats",
+    "nbg_sys_walk_stats", "nbg_sys_set_walk_split",
     "nbg_sys_enable_timing", "nbg_sys_phase_times",
     "nbg_sys_debug_bbox", "nbg_sys_debug_order", "nbg_sys_debug_tree", "nbg_sys_debug_quads", "nbg_direct_f64",
     "nbg_sys_potential_f64", "nbg_generate_plummer", "nbg_generate_cube",
@@ -167,6 +167,7 @@ def lib() -> ctypes.CDLL:
             "nbg_sys_counts": (ctypes.c_int, [_vp, _u64p, _u64p]),
             "nbg_sys_counts4": (ctypes.c_int, [_vp, _u64p]),
             "nbg_sys_walk_stats": (ctypes.c_int, [_vp, _u64p]),
+            "nbg_sys_set_walk_split": (ctypes.c_int, [_vp, ctypes.c_int32]),
             "nbg_sys_enable_walk_stats": (ctypes.c_int, [_vp, ctypes.c_int32]),
             "nbg_sys_enable_counts": (ctypes.c_int, [_vp, ctypes.c_int32]),
             "nbg_sys_enable_timing": (ctypes.c_int, [_vp, ctypes.c_int32]),
@@ -378,6 +379,11 @@ class System:
 
     def enable_walk_stats(self, on: bool = True):
         _check(lib().nbg_sys_enable_walk_stats(self.handle, int(on)))
+
+    def set_walk_split(self, waves: int = 0):
+        """Waves per group of 64 targets in the FAST linear-BVH walk (0:
+        chosen by size; results are the same bits for every choice)."""
+        _check(lib().nbg_sys_set_walk_split(self.handle, int(waves)))
 
     def walk_stats(self) -> dict:
         out = np.zeros(4, np.uint64)

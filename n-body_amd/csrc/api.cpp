@@ -86,6 +86,12 @@ struct nbg_sys {
     int64_t fmm_item_cap = 0, fmm_pool_cap = 0;
     int fmm_seed_cap = 0;
     int fmm_shift = 0;  // capacity doublings after passes that outgrew the pool
+    // split FAST linear-BVH walks of small systems: per-target sums and
+    // per-group block counts (see WalkArgs::split); split_req 0 = automatic
+    double *split_part = nullptr;
+    int32_t *split_nb = nullptr;
+    int64_t split_cap = 0;
+    int split_req = 0;
     // passive-point walks (nbg_sys_accelerations_at): one buffer that grows
     // to the largest target count seen and is kept for later calls
     void *pts = nullptr;
@@ -214,15 +220,52 @@ int ensure_fmm(nbg_sys *s) {
     return NBG_OK;
 }
 
+bool is_lbvh(int solver) { return solver == NBG_LINEAR_BVH || solver == NBG_LINEAR_BVH_QUADRUPOLE; }
+
+// Split waves per group for the system's FAST linear-BVH walk (1 for other
+// solvers and the ORDERED mode).
+int split_of(const nbg_sys *s) {
+    if (!is_lbvh(s->prm.solver) || s->prm.sum_mode != NBG_SUM_FAST) return 1;
+    if (s->prm.targets_per_lane > 1) return 1;
+    return s->split_req > 0 ? s->split_req : split_waves(s->n);
+}
+
+void free_split(nbg_sys *s) {
+    if (s->split_part) (void)hipFree(s->split_part);
+    if (s->split_nb) (void)hipFree(s->split_nb);
+    s->split_part = nullptr;
+    s->split_nb = nullptr;
+    s->split_cap = 0;
+}
+
+// Allocates the split walk's sums when the system's walk splits (no-op
+// otherwise or when the current ones are large enough).
+int ensure_split(nbg_sys *s) {
+    if (split_of(s) <= 1 || s->split_cap >= s->n) return NBG_OK;
+    if (s->stream) NBG_HIP(hipStreamSynchronize(s->stream));
+    free_split(s);
+    const int64_t n = s->n > 0 ? s->n : 1;
+    int r = dalloc(&s->split_part, (int64_t)kSplitSlots * 3 * n);
+    if (r == NBG_OK) r = dalloc(&s->split_nb, (n + 63) / 64);
+    if (r != NBG_OK) {
+        free_split(s);
+        return r;
+    }
+    s->split_cap = n;
+    return NBG_OK;
+}
+
 int ensure_buffers(nbg_sys *s) {
     if (int r = ensure_octree(s)) return r;
     if (int r = ensure_fmm(s)) return r;
+    if (int r = ensure_split(s)) return r;
     return ensure_quad(s);
 }
 
 void free_sys(nbg_sys *s) {
     free_oct(s->ob);
     free_fmm(s);
+    free_split(s);
     if (s->quad) (void)hipFree(s->quad);
     if (s->pts) (void)hipFree(s->pts);
     void *ptrs[] = {s->x, s->y, s->z, s->m, s->vx, s->vy, s->vz, s->ax, s->ay, s->az, s->keys, s->keys_alt,
@@ -355,6 +398,13 @@ int enqueue_force_pass(nbg_sys *s) {
         launch_fmm(fa, st);
     } else {
         WalkArgs wa = walk_args(s);
+        const int k = split_of(s);
+        if (k > 1 && s->split_part && s->split_cap >= s->n) {
+            wa.split = k;
+            wa.split_part = s->split_part;
+            wa.split_nb = s->split_nb;
+            wa.split_stride = s->n;
+        }
         const int stats = s->walk_stats ? 2 : (s->counting ? 1 : 0);
         launch_walk(s, wa, stats);
     }
@@ -877,6 +927,20 @@ int nbg_sys_enable_counts(nbg_sys *s, int32_t on) {
 int nbg_sys_enable_walk_stats(nbg_sys *s, int32_t on) {
     if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
     s->walk_stats = on != 0;
+    return NBG_OK;
+}
+
+int nbg_sys_set_walk_split(nbg_sys *s, int32_t waves) {
+    if (!s) return fail(NBG_ERR_ARG, "sys is NULL");
+    if (waves != 0 && waves != 1 && waves != 2 && waves != 4 && waves != 8 && waves != 16)
+        return fail(NBG_ERR_ARG, "waves must be 0 (automatic), 1, 2, 4, 8 or 16");
+    DeviceGuard g(s->ctx->device);
+    const int old = s->split_req;
+    s->split_req = waves;
+    if (int r = ensure_split(s)) {
+        s->split_req = old;
+        return r;
+    }
     return NBG_OK;
 }
 

@@ -262,7 +262,30 @@ struct WalkArgs {
     // FAST single-system walks: targets per lane (1, 2 or 4); a wavefront
     // serves 64 tpl consecutive targets, lane l of slot t target 64 t + l.
     int tpl = 1;
+    // FAST single-system linear-BVH walks with one target per lane: split
+    // waves per group of 64 targets (1 = none).  With split > 1 the walk
+    // stores per-target sums in split_part (kSplitSlots x 3 doubles per
+    // target, component-major, split_stride doubles apart) and per-group
+    // block counts in split_nb, and a second kernel adds them up.
+    int split = 1;
+    double *split_part = nullptr;
+    int32_t *split_nb = nullptr;
+    int64_t split_stride = 0;
 };
+
+// Sums a split walk stores per target: the top nodes' and one per block of
+// frontier subtrees.
+constexpr int kSplitSlots = 17;
+// Largest system whose FAST linear-BVH walk splits its groups over waves.
+constexpr int64_t kSplitMaxBodies = 65536;
+// Split waves per group for a system of n bodies (1: the chip is full).
+inline int split_waves(int64_t n) {
+    if (n > kSplitMaxBodies || n < 1) return 1;
+    const int64_t groups = (n + 63) / 64;
+    int k = 1;
+    while (k < 16 && groups * k * 2 <= 8192) k *= 2;
+    return k;
+}
 
 // Barnes-Hut traversal over the LBVH.  ordered != 0 selects the
 // bit-exact reference arithmetic; diag != 0 makes the FAST walk also count

@@ -26,7 +26,9 @@
 // that take an interaction are selected by turning the ballot back into an
 // execution mask (inverse ballot), so no per-lane predicate is computed in
 // vector registers.  Interactions use rsqrt and FMA, summed in FP32 per leaf
-// and in FP64 across leaves (walk_common.hpp).
+// and in FP64 across leaves (walk_common.hpp); the linear-BVH walks add those
+// sums per fixed block of subtrees, which lets a small system spread each
+// group of 64 targets over several waves with unchanged bits.
 
 #include "device.hpp"
 #include "launch.hpp"
@@ -315,98 +317,454 @@ __device__ __forceinline__ void visit_pair(const NodeRec &r0, const NodeRec &r1,
     }
 }
 
-// Forest: the same traversal for the wave's system (see wave_task).
-template <int WAVES, bool SOFT, int STATS, bool QUAD>
-__global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_forest(WalkArgs a) {
-    __shared__ int s_node[WAVES][kStackFast];
-    __shared__ uint64_t s_mask[WAVES][kStackFast];
-    __shared__ float4 s_leaf[WAVES][kLeafScratch];
+// ---------------------------------------------------------------- FAST LBVH
+// Every FAST linear-BVH walk (one or several targets per lane, one system or
+// a forest, one wave or several per group of 64 targets) sums a target's
+// interactions with the same structure, so its bits do not depend on which
+// of them walks it:
+//   top nodes: internal nodes above depth kTopDepth holding more than
+//     ceil(root count / 32) bodies; the walk over them ("top walk") sums the
+//     nodes it accepts into s_top and lists, in the order it meets them, the
+//     children of top nodes that are not top nodes ("frontier": at most
+//     2^kTopDepth entries);
+//   blocks: each run of kFrontierBlock consecutive frontier entries that a
+//     target reaches is walked depth-first as before, its interactions summed
+//     from zero (FP32 per leaf, FP64 across leaves) into B_b;
+//   result: ((s_top + B_0) + B_1) + ... in FP64, times G, rounded once.
+// Only node records decide the structure, so a wave walking a group alone
+// and K waves splitting its blocks (the sums meeting in k_walk_split_reduce)
+// give the same bits.  The opening decisions, and so the interaction set,
+// are those of the plain depth-first walk.
+constexpr int kTopDepth = 7;
+constexpr int kMaxFrontier = 1 << kTopDepth;
+constexpr int kFrontierBlock = 8;
+static_assert(1 + kMaxFrontier / kFrontierBlock == kSplitSlots, "split slots: s_top and one per block");
 
-    const float4 *__restrict__ bodies = a.bodies;
-    const NodeRec *__restrict__ nodes = a.nodes;
-    // the wave index is uniform; readfirstlane makes the per-wave task scalar
-    const int lane = lane_id(), wv = uniform_i32(threadIdx.x / kWave);
-    const WaveTask T = wave_task<true>(a, xcd_block(blockIdx.x, gridDim.x) * WAVES + wv);
-    const int64_t tgt = T.base + lane;
-    const bool valid = tgt < T.end;
-    const uint64_t vmask = __ballot(valid);
-    if (vmask == 0) return;
+__device__ __forceinline__ bool lbvh_top(const NodeRec &r, int depth, int cut) {
+    return r.topo.z >= 0 && r.topo.y > cut && depth < kTopDepth;
+}
 
-    FastWave W;
-    const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    fast_init(W, me.x, me.y, me.z);
-    // A VGPR copy of G (non-volatile asm: no memory effects, so the node and
-    // body loads below stay scalar).
-    W.G = T.G;
-    const float eps2 = T.eps2;
-    const int64_t n = a.n;
-    const uint32_t last_pair = (uint32_t)(a.max_nodes - 1);
-    bool fail = false;
-
-    // The root is visited on its own; afterwards every step visits the two
-    // children (c, c + 1) of a node that the lanes in `mask` opened.
-    if (T.root < 0 || T.root >= a.max_nodes) {
-        if (lane == 0) atomicOr(a.err, 2u);
-        return;
+// A lane accepts node nd (slot sums W): its interaction (monopole or
+// quadrupole, qn the node's two moment records).
+template <bool SOFT, bool QUAD>
+__device__ __forceinline__ void take_node(const NodeRec &nd, const float4 *__restrict__ qn, uint64_t acc, float eps2,
+                                          FastWave &W) {
+    if (QUAD) {
+        if (acc) {
+            const float4 q0 = load_uniform(qn), q1 = load_uniform(qn + 1);
+            if (__builtin_amdgcn_inverse_ballot_w64(acc))
+                interact_quad<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, q0, q1, eps2, W);
+        }
+    } else if (__builtin_amdgcn_inverse_ballot_w64(acc)) {
+        interact<SOFT>(nd.com.x, nd.com.y, nd.com.z, nd.com.w, eps2, W);
     }
-    const NodeRec root = nodes[T.root];
-    uint64_t mask = visit<SOFT, STATS, QUAD>(root, a.quad + 2 * T.root, vmask, bodies, n, eps2, W, fail, s_leaf[wv]);
-    int c = root.topo.z;
-    int sp = 0;
-    uint32_t iters = 0;
+}
+
+// The top walk from node `root` for TPL slots of lanes (vmask[t]): the
+// accepted top nodes' interactions go into W[t] when INTERACT; the frontier
+// (node, per-slot masks) goes to f_node / f_mask[i * TPL + t].  Returns the
+// number of frontier entries.  st_* is a stack of kTopDepth entries.
+template <bool SOFT, int STATS, bool QUAD, int TPL, bool INTERACT>
+__device__ int top_walk(const NodeRec *__restrict__ nodes, const float4 *__restrict__ quad, int root,
+                        const uint64_t (&vmask)[TPL], float eps2, FastWave (&W)[TPL], int *f_node, uint64_t *f_mask,
+                        int *st_node, uint64_t *st_mask, int *st_dep, uint32_t last_pair, bool &fail) {
+    const NodeRec r = load_uniform(nodes + root);
+    const int cut = (r.topo.y + 31) / 32;
+    if (!lbvh_top(r, 0, cut)) {
+        if (lane_id() == 0) {
+            f_node[0] = root;
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) f_mask[t] = vmask[t];
+        }
+        __builtin_amdgcn_wave_barrier();
+        return 1;
+    }
+    uint64_t m[TPL];
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+        const uint64_t acc = __ballot(lbvh_accepts(r.mac, W[t].px, W[t].py, W[t].pz)) & vmask[t];
+        if (INTERACT) {
+            if (STATS >= 2) { W[t].visits += 1; W[t].lanes += __popcll(vmask[t]); }
+            if (STATS >= 1) W[t].np += __popcll(acc);
+            take_node<SOFT, QUAD>(r, quad + 2 * root, acc, eps2, W[t]);
+        }
+        m[t] = vmask[t] & ~acc;
+    }
+    int F = 0, sp = 0, c = r.topo.z, dep = 1;
     while (true) {
-        // wave-uniform by construction; readfirstlane tells the compiler so
-        // and keeps the traversal state and its mask arithmetic scalar
-        mask = uniform_u64(mask);
-        sp = uniform_i32(sp);
-        if (mask == 0) {
+        uint64_t any = 0;
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) any |= m[t];
+        if (any == 0) {
             if (sp == 0) break;
             --sp;
             __builtin_amdgcn_wave_barrier();
-            c = uniform_i32(s_node[wv][sp]);
-            mask = uniform_u64(s_mask[wv][sp]);
+            c = uniform_i32(st_node[sp]);
+            dep = uniform_i32(st_dep[sp]);
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) m[t] = uniform_u64(st_mask[sp * TPL + t]);
+        }
+        c = uniform_i32(c);
+        if (__builtin_expect((uint32_t)c >= last_pair, 0)) { fail = true; break; }
+        const NodeRec r0 = load_uniform(nodes + c), r1 = load_uniform(nodes + c + 1);
+        uint64_t o0[TPL], o1[TPL], any0 = 0, any1 = 0;
+        const NodeRec *rs[2] = {&r0, &r1};
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const NodeRec &nd = *rs[k];
+            uint64_t *o = k ? o1 : o0;
+            if (lbvh_top(nd, dep, cut)) {
+#pragma unroll
+                for (int t = 0; t < TPL; ++t) {
+                    const uint64_t acc = __ballot(lbvh_accepts(nd.mac, W[t].px, W[t].py, W[t].pz)) & m[t];
+                    if (INTERACT) {
+                        if (STATS >= 2 && m[t]) { W[t].visits += 1; W[t].lanes += __popcll(m[t]); }
+                        if (STATS >= 1) W[t].np += __popcll(acc);
+                        take_node<SOFT, QUAD>(nd, quad + 2 * (c + k), acc, eps2, W[t]);
+                    }
+                    o[t] = m[t] & ~acc;
+                }
+            } else {
+                if (F >= kMaxFrontier) { fail = true; break; }  // unreachable: depth < kTopDepth
+                if (lane_id() == 0) {
+                    f_node[F] = c + k;
+#pragma unroll
+                    for (int t = 0; t < TPL; ++t) f_mask[F * TPL + t] = m[t];
+                }
+                ++F;
+#pragma unroll
+                for (int t = 0; t < TPL; ++t) o[t] = 0;
+            }
+        }
+        if (fail) break;
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) { any0 |= o0[t]; any1 |= o1[t]; }
+        if (any0 != 0) {
+            if (any1 != 0) {
+                if (sp >= kTopDepth) { fail = true; break; }
+                if (lane_id() == 0) {
+                    st_node[sp] = r1.topo.z;
+                    st_dep[sp] = dep + 1;
+#pragma unroll
+                    for (int t = 0; t < TPL; ++t) st_mask[sp * TPL + t] = o1[t];
+                }
+                __builtin_amdgcn_wave_barrier();
+                ++sp;
+            }
+            c = r0.topo.z;
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) m[t] = o0[t];
+        } else if (any1 != 0) {
+            c = r1.topo.z;
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) m[t] = o1[t];
+        } else {
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) m[t] = 0;
+        }
+        ++dep;
+    }
+    __builtin_amdgcn_wave_barrier();
+    return F;
+}
+
+// Depth-first walk of the subtree of frontier node f (visited on its own,
+// then sibling pairs) for the slots' lanes m0[t]; node loads, traversal and
+// stack are shared by the TPL slots, each keeping its own masks and sums.
+template <bool SOFT, int STATS, bool QUAD, int TPL>
+__device__ __forceinline__ void walk_subtree(const NodeRec *__restrict__ nodes, const float4 *__restrict__ quad,
+                                             int f, const uint64_t (&m0)[TPL], const float4 *__restrict__ bodies,
+                                             int64_t n, float eps2, FastWave (&W)[TPL], bool &fail, int *s_node,
+                                             uint64_t *s_mask, float4 *sh, uint32_t last_pair, uint32_t &iters) {
+    const NodeRec r = load_uniform(nodes + f);
+    uint64_t mask[TPL];
+#pragma unroll
+    for (int t = 0; t < TPL; ++t)
+        mask[t] = m0[t] ? visit<SOFT, STATS, QUAD>(r, quad + 2 * f, m0[t], bodies, n, eps2, W[t], fail, sh) : 0ull;
+    int c = r.topo.z;
+    int sp = 0;
+    while (true) {
+        uint64_t any = 0;
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) any |= mask[t];
+        if (any == 0) {
+            if (sp == 0) break;
+            --sp;
+            __builtin_amdgcn_wave_barrier();
+            c = uniform_i32(s_node[sp]);
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) mask[t] = uniform_u64(s_mask[sp * TPL + t]);
         }
         c = uniform_i32(c);  // keeps the node loads scalar
         if (__builtin_expect((uint32_t)c >= last_pair, 0)) { fail = true; break; }
         if (__builtin_expect(++iters > kMaxWalkIters, 0)) { fail = true; break; }
         const NodeRec r0 = load_uniform(nodes + c), r1 = load_uniform(nodes + c + 1);
-        uint64_t o0, o1;
-        visit_pair<SOFT, STATS, QUAD>(r0, r1, a.quad + 2 * c, mask, bodies, n, eps2, W, fail, s_leaf[wv], o0, o1);
+        uint64_t o0[TPL], o1[TPL], any0 = 0, any1 = 0;
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) {
+            o0[t] = 0;
+            o1[t] = 0;
+            if (TPL == 1 || mask[t])
+                visit_pair<SOFT, STATS, QUAD>(r0, r1, quad + 2 * c, mask[t], bodies, n, eps2, W[t], fail, sh, o0[t],
+                                              o1[t]);
+            any0 |= o0[t];
+            any1 |= o1[t];
+        }
         // a failed visit opens nothing, so the walk still ends; fail is
-        // reported after the loop
-        if (o0 != 0) {
-            if (o1 != 0) {
+        // reported by the kernel
+        if (any0 != 0) {
+            if (any1 != 0) {
                 if (sp >= kStackFast) { fail = true; break; }
-                if (lane == 0) { s_node[wv][sp] = r1.topo.z; s_mask[wv][sp] = o1; }
+                if (lane_id() == 0) {
+                    s_node[sp] = r1.topo.z;
+#pragma unroll
+                    for (int t = 0; t < TPL; ++t) s_mask[sp * TPL + t] = o1[t];
+                }
                 __builtin_amdgcn_wave_barrier();
                 ++sp;
             }
             c = r0.topo.z;
-            mask = o0;
-        } else if (o1 != 0) {
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) mask[t] = o0[t];
+        } else if (any1 != 0) {
             c = r1.topo.z;
-            mask = o1;
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) mask[t] = o1[t];
         } else {
-            mask = 0;
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) mask[t] = 0;
+        }
+    }
+}
+
+// FAST linear-BVH walk.  TPL targets per lane: the wave serves 64 TPL
+// targets from its task's base, slot t of lane l being target base + 64 t
+// + l.  FOREST: the wave's system from the forest's wave table (wave_task).
+// SPLIT (one system, TPL 1): a.split waves serve each group of 64 targets;
+// wave k of a group walks the blocks b with b mod a.split == k (wave 0 also
+// the top nodes) and stores their sums for k_walk_split_reduce.
+template <int WAVES, bool SOFT, int STATS, bool QUAD, int TPL, bool FOREST, bool SPLIT>
+__global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
+    static_assert(!SPLIT || (TPL == 1 && !FOREST), "split groups are one-slot single-system walks");
+    __shared__ int s_node[WAVES][kStackFast];
+    __shared__ uint64_t s_mask[WAVES][kStackFast * TPL];
+    __shared__ float4 s_leaf[WAVES][kLeafScratch];
+    __shared__ int f_node[WAVES][kMaxFrontier];
+    __shared__ uint64_t f_mask[WAVES][kMaxFrontier * TPL];
+    __shared__ int st_dep[WAVES][kTopDepth];
+
+    const float4 *__restrict__ bodies = a.bodies;
+    const NodeRec *__restrict__ nodes = a.nodes;
+    // the wave index is uniform: readfirstlane makes the per-wave task
+    // scalar, while LDS addresses use the per-lane copy (one VALU op each)
+    const int lane = lane_id(), wv = threadIdx.x / kWave;
+    const int wvu = uniform_i32(wv);
+    const int64_t gw = (int64_t)xcd_block(blockIdx.x, gridDim.x) * WAVES + wvu;
+    const int K = SPLIT ? a.split : 1;
+    const int64_t grp = SPLIT ? gw / K : gw;
+    const int k = SPLIT ? (int)(gw - grp * K) : 0;
+    const WaveTask T = wave_task<FOREST, TPL>(a, grp);
+    const int64_t base = T.base;
+    FastWave W[TPL];
+    uint64_t vmask[TPL];
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+        const int64_t tgt = base + t * kWave + lane;
+        const bool valid = tgt < T.end;
+        vmask[t] = __ballot(valid);
+        const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        fast_init(W[t], me.x, me.y, me.z);
+        W[t].G = T.G;
+    }
+    if (vmask[0] == 0) return;  // slot 0 holds the wave's first targets
+    if (FOREST && (T.root < 0 || T.root >= a.max_nodes)) {
+        if (lane == 0) atomicOr(a.err, 2u);
+        return;
+    }
+    const float eps2 = T.eps2;
+    const int64_t n = a.n;
+    const uint32_t last_pair = (uint32_t)(a.max_nodes - 1);
+    bool fail = false;
+
+    int F;
+    if (!SPLIT || k == 0)
+        F = top_walk<SOFT, STATS, QUAD, TPL, true>(nodes, a.quad, T.root, vmask, eps2, W, f_node[wvu], f_mask[wvu],
+                                                   s_node[wvu], s_mask[wvu], st_dep[wvu], last_pair, fail);
+    else
+        F = top_walk<SOFT, STATS, QUAD, TPL, false>(nodes, a.quad, T.root, vmask, eps2, W, f_node[wvu], f_mask[wvu],
+                                                    s_node[wvu], s_mask[wvu], st_dep[wvu], last_pair, fail);
+    F = uniform_i32(F);
+    // s_top, then each block's sum from zero
+    double S[TPL][3];
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+        fast_flush(W[t]);
+        S[t][0] = W[t].sx; S[t][1] = W[t].sy; S[t][2] = W[t].sz;
+        W[t].sx = 0.0; W[t].sy = 0.0; W[t].sz = 0.0;
+    }
+    const int64_t tgt0 = base + lane;
+    const int64_t stride = a.split_stride;
+    if (SPLIT && k == 0 && tgt0 < T.end) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) a.split_part[d * stride + tgt0] = S[0][d];
+        if (lane == 0) a.split_nb[grp] = (F + kFrontierBlock - 1) / kFrontierBlock;
+    }
+    uint32_t iters = 0;
+    if constexpr (TPL == 1) {
+        // One loop, shaped as the plain depth-first walk's: when a frontier
+        // subtree is done (empty mask, empty stack) the exit path closes the
+        // block if it ends there and starts the next owned entry.
+        FastWave &V = W[0];
+        int i = -1, c = 0, sp = 0;
+        uint64_t mask = 0;
+        while (true) {
+            if (mask == 0) {
+              if (sp == 0) {
+                // next frontier entry whose visit opens an internal node
+                bool done = false;
+                do {
+                    if (i >= 0 && ((i + 1) % kFrontierBlock == 0 || i + 1 == F)) {
+                        fast_flush(V);
+                        if (SPLIT) {
+                            if (tgt0 < T.end) {
+                                const int64_t o = (int64_t)(1 + i / kFrontierBlock) * 3 * stride + tgt0;
+                                a.split_part[o] = V.sx;
+                                a.split_part[o + stride] = V.sy;
+                                a.split_part[o + 2 * stride] = V.sz;
+                            }
+                        } else {
+                            S[0][0] += V.sx; S[0][1] += V.sy; S[0][2] += V.sz;
+                        }
+                        V.sx = 0.0; V.sy = 0.0; V.sz = 0.0;
+                    }
+                    ++i;
+                    if (SPLIT && i % kFrontierBlock == 0) {
+                        // the first entry of the next block this wave owns
+                        const int blk = i / kFrontierBlock;
+                        i = (blk + (k - blk % K + K) % K) * kFrontierBlock;
+                    }
+                    if (i >= F) {
+                        done = true;
+                        break;
+                    }
+                    const int f = uniform_i32(f_node[wv][i]);
+                    const NodeRec r = load_uniform(nodes + f);
+                    mask = visit<SOFT, STATS, QUAD>(r, a.quad + 2 * f, uniform_u64(f_mask[wv][i]), bodies, n, eps2, V,
+                                                    fail, s_leaf[wv]);
+                    c = r.topo.z;
+                } while (mask == 0);
+                if (done) break;
+              } else {
+                --sp;
+                __builtin_amdgcn_wave_barrier();
+                c = uniform_i32(s_node[wv][sp]);
+                mask = uniform_u64(s_mask[wv][sp]);
+              }
+            }
+            c = uniform_i32(c);  // keeps the node loads scalar
+            if (__builtin_expect((uint32_t)c >= last_pair, 0)) { fail = true; break; }
+            if (__builtin_expect(++iters > kMaxWalkIters, 0)) { fail = true; break; }
+            const NodeRec r0 = load_uniform(nodes + c), r1 = load_uniform(nodes + c + 1);
+            uint64_t o0, o1;
+            visit_pair<SOFT, STATS, QUAD>(r0, r1, a.quad + 2 * c, mask, bodies, n, eps2, V, fail, s_leaf[wv], o0, o1);
+            // a failed visit opens nothing, so the walk still ends; fail is
+            // reported after the loop
+            if (o0 != 0) {
+                if (o1 != 0) {
+                    if (sp >= kStackFast) { fail = true; break; }
+                    if (lane == 0) { s_node[wv][sp] = r1.topo.z; s_mask[wv][sp] = o1; }
+                    __builtin_amdgcn_wave_barrier();
+                    ++sp;
+                }
+                c = r0.topo.z;
+                mask = o0;
+            } else if (o1 != 0) {
+                c = r1.topo.z;
+                mask = o1;
+            } else {
+                mask = 0;
+            }
+        }
+    } else {
+        int cnt[TPL];
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) cnt[t] = 0;
+        for (int i = 0; i < F; ++i) {
+            uint64_t m[TPL];
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) m[t] = uniform_u64(f_mask[wvu][i * TPL + t]);
+            walk_subtree<SOFT, STATS, QUAD, TPL>(nodes, a.quad, uniform_i32(f_node[wvu][i]), m, bodies, n, eps2, W,
+                                                 fail, s_node[wvu], s_mask[wvu], s_leaf[wvu], last_pair, iters);
+            // each slot's blocks are runs of kFrontierBlock of its own entries
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) {
+                if (m[t] == 0 || ++cnt[t] % kFrontierBlock != 0) continue;
+                fast_flush(W[t]);
+                S[t][0] += W[t].sx; S[t][1] += W[t].sy; S[t][2] += W[t].sz;
+                W[t].sx = 0.0; W[t].sy = 0.0; W[t].sz = 0.0;
+            }
+        }
+        // the last, partial block of each slot
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) {
+            if (cnt[t] % kFrontierBlock == 0) continue;
+            fast_flush(W[t]);
+            S[t][0] += W[t].sx; S[t][1] += W[t].sy; S[t][2] += W[t].sz;
         }
     }
     if (fail && lane == 0) atomicOr(a.err, 2u);
-    if (valid) {
-        const uint32_t o = a.perm[tgt];
-        if ((int64_t)o < a.nt) {
-            const float3 r = fast_result(W, W.G);
-            a.ax[o] = r.x; a.ay[o] = r.y; a.az[o] = r.z;
+    uint64_t pp = 0, np = 0, visits = 0, lanes = 0, lvis = 0, llanes = 0;
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+        const int64_t tgt = base + t * kWave + lane;
+        if (!SPLIT && tgt < T.end) {
+            const uint32_t o = a.perm[tgt];
+            if ((int64_t)o < a.nt) {
+                const double g = (double)W[t].G;
+                a.ax[o] = (float)(S[t][0] * g);
+                a.ay[o] = (float)(S[t][1] * g);
+                a.az[o] = (float)(S[t][2] * g);
+            }
         }
+        pp += W[t].pp; np += W[t].np;
+        visits += W[t].visits; lanes += W[t].lanes; lvis += W[t].leaf_visits; llanes += W[t].leaf_lanes;
     }
     if (lane == 0) {
-        atomicAdd(&a.counts[0], (unsigned long long)W.pp);
-        atomicAdd(&a.counts[1], (unsigned long long)W.np);
+        atomicAdd(&a.counts[0], (unsigned long long)pp);
+        atomicAdd(&a.counts[1], (unsigned long long)np);
         if (STATS >= 2) {
-            atomicAdd(&a.counts[2], (unsigned long long)W.visits);
-            atomicAdd(&a.counts[3], (unsigned long long)W.lanes);
-            atomicAdd(&a.counts[4], (unsigned long long)W.leaf_visits);
-            atomicAdd(&a.counts[5], (unsigned long long)W.leaf_lanes);
+            atomicAdd(&a.counts[2], (unsigned long long)visits);
+            atomicAdd(&a.counts[3], (unsigned long long)lanes);
+            atomicAdd(&a.counts[4], (unsigned long long)lvis);
+            atomicAdd(&a.counts[5], (unsigned long long)llanes);
         }
+    }
+}
+
+// The split walk's sums, ((s_top + B_0) + B_1) + ... for each target in
+// FP64, times G, rounded once and stored in output order.
+__global__ __launch_bounds__(256) void k_walk_split_reduce(WalkArgs a) {
+    const int64_t tgt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tgt >= a.nt) return;
+    const int64_t stride = a.split_stride;
+    int nb = a.split_nb[tgt / kWave];
+    if (nb < 0 || nb > kMaxFrontier / kFrontierBlock) {
+        atomicOr(a.err, 2u);
+        nb = 0;
+    }
+    double S[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) S[d] = a.split_part[d * stride + tgt];
+    for (int b = 0; b < nb; ++b) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) S[d] += a.split_part[((int64_t)(1 + b) * 3 + d) * stride + tgt];
+    }
+    const uint32_t o = a.perm[tgt];
+    if ((int64_t)o < a.nt) {
+        const double g = (double)a.G;
+        a.ax[o] = (float)(S[0] * g);
+        a.ay[o] = (float)(S[1] * g);
+        a.az[o] = (float)(S[2] * g);
     }
 }
 
@@ -751,266 +1109,39 @@ void launch_walk_oct(const WalkArgs &a, int ordered, int stats, hipStream_t s) {
     }
 }
 
-// TPL > 1 targets per lane: wave w serves the 64 TPL targets from its
-// task's base (wave_task), slot t of lane l being target base + 64 t + l.
-// Each step visits the sibling pair (c, c + 1) for the union of the slots'
-// masks; every slot keeps its own mask, opening decisions and sums (a slot
-// whose mask is empty skips the node), so each target meets exactly the
-// nodes and bodies it meets alone, in the same order, while the node
-// loads, the traversal and its stack are shared by TPL targets.
-template <int WAVES, bool SOFT, int STATS, bool QUAD, int TPL, bool FOREST>
-__global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_multi(WalkArgs a) {
-    __shared__ int s_node[WAVES][kStackFast];
-    __shared__ uint64_t s_mask[WAVES][kStackFast][TPL];
-    __shared__ float4 s_leaf[WAVES][kLeafScratch];
-
-    const float4 *__restrict__ bodies = a.bodies;
-    const NodeRec *__restrict__ nodes = a.nodes;
-    const int lane = lane_id(), wv = uniform_i32(threadIdx.x / kWave);
-    const WaveTask T = wave_task<FOREST, TPL>(a, xcd_block(blockIdx.x, gridDim.x) * WAVES + wv);
-    const int64_t base = T.base;
-    FastWave W[TPL];
-    uint64_t mask[TPL];
-#pragma unroll
-    for (int t = 0; t < TPL; ++t) {
-        const int64_t tgt = base + t * kWave + lane;
-        const bool valid = tgt < T.end;
-        mask[t] = __ballot(valid);
-        const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        fast_init(W[t], me.x, me.y, me.z);
-        W[t].G = T.G;
-    }
-    if (mask[0] == 0) return;  // slot 0 holds the wave's first targets
-    if (FOREST && (T.root < 0 || T.root >= a.max_nodes)) {
-        if (lane == 0) atomicOr(a.err, 2u);
-        return;
-    }
-    const float eps2 = T.eps2;
-    const int64_t n = a.n;
-    const uint32_t last_pair = (uint32_t)(a.max_nodes - 1);
-    bool fail = false;
-
-    const NodeRec root = nodes[T.root];
-#pragma unroll
-    for (int t = 0; t < TPL; ++t)
-        mask[t] = mask[t] ? visit<SOFT, STATS, QUAD>(root, a.quad + 2 * T.root, mask[t], bodies, n, eps2, W[t], fail,
-                                                     s_leaf[wv])
-                          : 0ull;
-    int c = root.topo.z;
-    int sp = 0;
-    uint32_t iters = 0;
-    while (true) {
-        uint64_t any = 0;
-#pragma unroll
-        for (int t = 0; t < TPL; ++t) any |= mask[t];
-        if (any == 0) {
-            if (sp == 0) break;
-            --sp;
-            __builtin_amdgcn_wave_barrier();
-            c = uniform_i32(s_node[wv][sp]);
-#pragma unroll
-            for (int t = 0; t < TPL; ++t) mask[t] = uniform_u64(s_mask[wv][sp][t]);
-        }
-        c = uniform_i32(c);  // keeps the node loads scalar
-        if (__builtin_expect((uint32_t)c >= last_pair, 0)) { fail = true; break; }
-        if (__builtin_expect(++iters > kMaxWalkIters, 0)) { fail = true; break; }
-        const NodeRec r0 = load_uniform(nodes + c), r1 = load_uniform(nodes + c + 1);
-        uint64_t o0[TPL], o1[TPL], any0 = 0, any1 = 0;
-        // each slot visits the pair as the one-target walk does (same order of sums)
-#pragma unroll
-        for (int t = 0; t < TPL; ++t) {
-            o0[t] = 0;
-            o1[t] = 0;
-            if (mask[t])
-                visit_pair<SOFT, STATS, QUAD>(r0, r1, a.quad + 2 * c, mask[t], bodies, n, eps2, W[t], fail,
-                                              s_leaf[wv], o0[t], o1[t]);
-            any0 |= o0[t];
-            any1 |= o1[t];
-        }
-        if (any0 != 0) {
-            if (any1 != 0) {
-                if (sp >= kStackFast) { fail = true; break; }
-                if (lane == 0) {
-                    s_node[wv][sp] = r1.topo.z;
-#pragma unroll
-                    for (int t = 0; t < TPL; ++t) s_mask[wv][sp][t] = o1[t];
-                }
-                __builtin_amdgcn_wave_barrier();
-                ++sp;
-            }
-            c = r0.topo.z;
-#pragma unroll
-            for (int t = 0; t < TPL; ++t) mask[t] = o0[t];
-        } else if (any1 != 0) {
-            c = r1.topo.z;
-#pragma unroll
-            for (int t = 0; t < TPL; ++t) mask[t] = o1[t];
-        } else {
-#pragma unroll
-            for (int t = 0; t < TPL; ++t) mask[t] = 0;
-        }
-    }
-    if (fail && lane == 0) atomicOr(a.err, 2u);
-    uint64_t pp = 0, np = 0, visits = 0, lanes = 0, lvis = 0, llanes = 0;
-#pragma unroll
-    for (int t = 0; t < TPL; ++t) {
-        const int64_t tgt = base + t * kWave + lane;
-        if (tgt < T.end) {
-            const uint32_t o = a.perm[tgt];
-            if ((int64_t)o < a.nt) {
-                const float3 r = fast_result(W[t], W[t].G);
-                a.ax[o] = r.x; a.ay[o] = r.y; a.az[o] = r.z;
-            }
-        }
-        pp += W[t].pp; np += W[t].np;
-        visits += W[t].visits; lanes += W[t].lanes; lvis += W[t].leaf_visits; llanes += W[t].leaf_lanes;
-    }
-    if (lane == 0) {
-        atomicAdd(&a.counts[0], (unsigned long long)pp);
-        atomicAdd(&a.counts[1], (unsigned long long)np);
-        if (STATS >= 2) {
-            atomicAdd(&a.counts[2], (unsigned long long)visits);
-            atomicAdd(&a.counts[3], (unsigned long long)lanes);
-            atomicAdd(&a.counts[4], (unsigned long long)lvis);
-            atomicAdd(&a.counts[5], (unsigned long long)llanes);
-        }
-    }
-}
-
-// Single system: wave w walks targets 64 w .. 64 w + 63 from the root.
-template <int WAVES, bool SOFT, int STATS, bool QUAD>
-__global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
-    __shared__ int s_node[WAVES][kStackFast];
-    __shared__ uint64_t s_mask[WAVES][kStackFast];
-    __shared__ float4 s_leaf[WAVES][kLeafScratch];
-
-    const float4 *__restrict__ bodies = a.bodies;
-    const NodeRec *__restrict__ nodes = a.nodes;
-    const int lane = lane_id(), wv = threadIdx.x / kWave;
-    const int64_t tgt = (xcd_block(blockIdx.x, gridDim.x) * WAVES + wv) * kWave + lane;
-    const bool valid = tgt < a.nt;
-    const uint64_t vmask = __ballot(valid);
-    if (vmask == 0) return;
-
-    FastWave W;
-    const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    fast_init(W, me.x, me.y, me.z);
-    // A VGPR copy of G (non-volatile asm: no memory effects, so the node and
-    // body loads below stay scalar).
-    asm("v_mov_b32 %0, %1" : "=v"(W.G) : "s"(a.G));
-    const float eps2 = a.eps2;
-    const int64_t n = a.n;
-    const uint32_t last_pair = (uint32_t)(a.max_nodes - 1);
-    bool fail = false;
-
-    // The root is visited on its own; afterwards every step visits the two
-    // children (c, c + 1) of a node that the lanes in `mask` opened.
-    const NodeRec root = nodes[0];
-    uint64_t mask = visit<SOFT, STATS, QUAD>(root, a.quad, vmask, bodies, n, eps2, W, fail, s_leaf[wv]);
-    int c = root.topo.z;
-    int sp = 0;
-    uint32_t iters = 0;
-    while (true) {
-        if (mask == 0) {
-            if (sp == 0) break;
-            --sp;
-            __builtin_amdgcn_wave_barrier();
-            c = uniform_i32(s_node[wv][sp]);
-            mask = uniform_u64(s_mask[wv][sp]);
-        }
-        c = uniform_i32(c);  // keeps the node loads scalar
-        if (__builtin_expect((uint32_t)c >= last_pair, 0)) { fail = true; break; }
-        if (__builtin_expect(++iters > kMaxWalkIters, 0)) { fail = true; break; }
-        const NodeRec r0 = load_uniform(nodes + c), r1 = load_uniform(nodes + c + 1);
-        uint64_t o0, o1;
-        visit_pair<SOFT, STATS, QUAD>(r0, r1, a.quad + 2 * c, mask, bodies, n, eps2, W, fail, s_leaf[wv], o0, o1);
-        // a failed visit opens nothing, so the walk still ends; fail is
-        // reported after the loop
-        if (o0 != 0) {
-            if (o1 != 0) {
-                if (sp >= kStackFast) { fail = true; break; }
-                if (lane == 0) { s_node[wv][sp] = r1.topo.z; s_mask[wv][sp] = o1; }
-                __builtin_amdgcn_wave_barrier();
-                ++sp;
-            }
-            c = r0.topo.z;
-            mask = o0;
-        } else if (o1 != 0) {
-            c = r1.topo.z;
-            mask = o1;
-        } else {
-            mask = 0;
-        }
-    }
-    if (fail && lane == 0) atomicOr(a.err, 2u);
-    if (valid) {
-        const uint32_t o = a.perm[tgt];
-        if ((int64_t)o < a.nt) {
-            const float3 r = fast_result(W, W.G);
-            a.ax[o] = r.x; a.ay[o] = r.y; a.az[o] = r.z;
-        }
-    }
-    if (lane == 0) {
-        atomicAdd(&a.counts[0], (unsigned long long)W.pp);
-        atomicAdd(&a.counts[1], (unsigned long long)W.np);
-        if (STATS >= 2) {
-            atomicAdd(&a.counts[2], (unsigned long long)W.visits);
-            atomicAdd(&a.counts[3], (unsigned long long)W.lanes);
-            atomicAdd(&a.counts[4], (unsigned long long)W.leaf_visits);
-            atomicAdd(&a.counts[5], (unsigned long long)W.leaf_lanes);
-        }
-    }
-}
-
-
-template <int STATS, bool QUAD, int TPL>
-static void launch_multi(const WalkArgs &a, hipStream_t s) {
+template <int STATS, bool QUAD, int TPL, bool FOREST, bool SPLIT>
+static void launch_lbvh_fast_k(const WalkArgs &a, int64_t waves, hipStream_t s) {
     constexpr int W = 4;
-    const int64_t waves = a.nsys > 0 ? a.nwaves : (a.nt + kWave * TPL - 1) / (kWave * TPL);
     const dim3 g((unsigned)((waves + W - 1) / W)), t(64 * W);
-    const bool soft = a.eps2 > 0.0f;
-    if (a.nsys > 0) {
-        if (soft) hipLaunchKernelGGL((k_walk_lbvh_multi<W, true, STATS, QUAD, TPL, true>), g, t, 0, s, a);
-        else hipLaunchKernelGGL((k_walk_lbvh_multi<W, false, STATS, QUAD, TPL, true>), g, t, 0, s, a);
-    } else {
-        if (soft) hipLaunchKernelGGL((k_walk_lbvh_multi<W, true, STATS, QUAD, TPL, false>), g, t, 0, s, a);
-        else hipLaunchKernelGGL((k_walk_lbvh_multi<W, false, STATS, QUAD, TPL, false>), g, t, 0, s, a);
-    }
+    if (a.eps2 > 0.0f) hipLaunchKernelGGL((k_walk_lbvh_fast<W, true, STATS, QUAD, TPL, FOREST, SPLIT>), g, t, 0, s, a);
+    else hipLaunchKernelGGL((k_walk_lbvh_fast<W, false, STATS, QUAD, TPL, FOREST, SPLIT>), g, t, 0, s, a);
 }
 
 template <int STATS, bool QUAD>
-static void launch_fast_single(const WalkArgs &a, unsigned blocks, hipStream_t s) {
-    constexpr int W = 4;
-    if (a.tpl == 2) return launch_multi<STATS, QUAD, 2>(a, s);
-    if (a.tpl == 4) return launch_multi<STATS, QUAD, 4>(a, s);
-    if (a.eps2 > 0.0f) hipLaunchKernelGGL((k_walk_lbvh_fast<W, true, STATS, QUAD>), dim3(blocks), dim3(64 * W), 0, s, a);
-    else hipLaunchKernelGGL((k_walk_lbvh_fast<W, false, STATS, QUAD>), dim3(blocks), dim3(64 * W), 0, s, a);
+static void launch_fast_q(const WalkArgs &a, hipStream_t s) {
+    const int tpl = a.tpl == 2 || a.tpl == 4 ? a.tpl : 1;
+    if (a.nsys > 0) {
+        // a forest's wave table counts waves of 64 tpl targets
+        if (tpl == 2) launch_lbvh_fast_k<STATS, QUAD, 2, true, false>(a, a.nwaves, s);
+        else if (tpl == 4) launch_lbvh_fast_k<STATS, QUAD, 4, true, false>(a, a.nwaves, s);
+        else launch_lbvh_fast_k<STATS, QUAD, 1, true, false>(a, a.nwaves, s);
+        return;
+    }
+    const int64_t groups = (a.nt + kWave - 1) / kWave;
+    if (tpl == 2) return launch_lbvh_fast_k<STATS, QUAD, 2, false, false>(a, (groups + 1) / 2, s);
+    if (tpl == 4) return launch_lbvh_fast_k<STATS, QUAD, 4, false, false>(a, (groups + 3) / 4, s);
+    if (a.split > 1 && a.split_part && a.split_nb) {
+        launch_lbvh_fast_k<STATS, QUAD, 1, false, true>(a, groups * a.split, s);
+        hipLaunchKernelGGL(k_walk_split_reduce, dim3((unsigned)((a.nt + 255) / 256)), dim3(256), 0, s, a);
+        return;
+    }
+    launch_lbvh_fast_k<STATS, QUAD, 1, false, false>(a, groups, s);
 }
 
 template <int STATS>
 static void launch_fast(const WalkArgs &a, hipStream_t s) {
-    constexpr int W = 4;
-    const unsigned blocks = (unsigned)((walk_waves(a) + W - 1) / W);
-    const bool soft = a.eps2 > 0.0f;
-    if (a.nsys > 0 && a.tpl > 1) {
-        if (a.quad) {
-            if (a.tpl == 2) launch_multi<STATS, true, 2>(a, s);
-            else launch_multi<STATS, true, 4>(a, s);
-        } else {
-            if (a.tpl == 2) launch_multi<STATS, false, 2>(a, s);
-            else launch_multi<STATS, false, 4>(a, s);
-        }
-    } else if (a.nsys > 0 && a.quad) {
-        if (soft) hipLaunchKernelGGL((k_walk_lbvh_forest<W, true, STATS, true>), dim3(blocks), dim3(64 * W), 0, s, a);
-        else hipLaunchKernelGGL((k_walk_lbvh_forest<W, false, STATS, true>), dim3(blocks), dim3(64 * W), 0, s, a);
-    } else if (a.nsys > 0) {
-        if (soft) hipLaunchKernelGGL((k_walk_lbvh_forest<W, true, STATS, false>), dim3(blocks), dim3(64 * W), 0, s, a);
-        else hipLaunchKernelGGL((k_walk_lbvh_forest<W, false, STATS, false>), dim3(blocks), dim3(64 * W), 0, s, a);
-    } else if (a.quad) {
-        launch_fast_single<STATS, true>(a, blocks, s);
-    } else {
-        launch_fast_single<STATS, false>(a, blocks, s);
-    }
+    if (a.quad) launch_fast_q<STATS, true>(a, s);
+    else launch_fast_q<STATS, false>(a, s);
 }
 
 void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, int stats, hipStream_t s) {

@@ -1,0 +1,60 @@
+"""The FAST linear-BVH walk split over several waves per group of 64
+targets: every split (1 to 16 waves, and the size-chosen default) gives the
+same bits and the same interaction counts, and those counts are the
+oracle's.  The same bits as the forest and the several-targets-per-lane
+walks are checked where those are tested against a system alone."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FAST_TOL = 1e-5
+
+
+def _params(nbg, solver, theta=0.5, leaf=16, bits=63, eps2=1e-4):
+    p = nbg.default_params()
+    p.solver, p.theta, p.leaf_size, p.morton_bits, p.eps2 = solver, theta, leaf, bits, eps2
+    p.sum_mode = nbg.SUM_FAST
+    return p
+
+
+CASES = [(1, "plummer"), (2, "cube"), (63, "plummer"), (65, "cube"), (1000, "plummer"), (4099, "cube"),
+         (10000, "plummer"), (50000, "plummer"), (65536, "cube")]
+
+
+@pytest.mark.parametrize("quad", [False, True])
+@pytest.mark.parametrize("n,kind", CASES)
+def test_split_walk_same_bits(nbg, orc, gpu_ctx, n, kind, quad):
+    print(f"[split] n={n} {kind} quad={quad}", flush=True)
+    b = nbg.plummer(n, seed=3000 + n) if kind == "plummer" else nbg.cube(n, seed=3000 + n, side=2.0)
+    solver = nbg.LINEAR_BVH_QUADRUPOLE if quad else nbg.LINEAR_BVH
+    s = nbg.System(gpu_ctx, _params(nbg, solver), n)
+    s.set_state(b)
+    ref = None
+    for k in (1, 2, 4, 8, 16, 0):
+        s.set_walk_split(k)
+        acc = s.update_accelerations()
+        counts = s.counts()
+        if ref is None:
+            ref = (acc, counts)
+            continue
+        assert np.array_equal(acc, ref[0]), f"split {k}"
+        assert counts == ref[1], f"split {k}"
+    s.close()
+    if quad:
+        return
+    t = orc.Tree(1, b.x, b.y, b.z, b.m, theta=0.5, leaf=16, bits=63, eps2=1e-4)
+    want, pp, npc = t.accel(b.x, b.y, b.z, counts=True)
+    assert ref[1] == (int(pp.sum()), int(npc.sum()))
+    mag = np.linalg.norm(want, axis=1)
+    floor = np.sqrt(np.mean(mag ** 2)) if n > 1 else 1.0
+    err = np.linalg.norm(ref[0] - want, axis=1) / np.maximum(mag, floor)
+    assert float(err.max()) <= FAST_TOL
+
+
+def test_split_argument(nbg, gpu_ctx):
+    s = nbg.System(gpu_ctx, _params(nbg, nbg.LINEAR_BVH), 100)
+    with pytest.raises(nbg.NbgError):
+        s.set_walk_split(3)
+    s.close()
