@@ -222,12 +222,16 @@ int ensure_fmm(nbg_sys *s) {
 
 bool is_lbvh(int solver) { return solver == NBG_LINEAR_BVH || solver == NBG_LINEAR_BVH_QUADRUPOLE; }
 
-// Split waves per group for the system's FAST linear-BVH walk (1 for other
-// solvers and the ORDERED mode).
+// Split waves per group for the system's FAST tree walk (1 for the other
+// solvers, the ORDERED mode and several targets per lane).  The octree's
+// frontier holds at most 8 blocks, so it uses at most 8 waves.
 int split_of(const nbg_sys *s) {
-    if (!is_lbvh(s->prm.solver) || s->prm.sum_mode != NBG_SUM_FAST) return 1;
-    if (s->prm.targets_per_lane > 1) return 1;
-    return s->split_req > 0 ? s->split_req : split_waves(s->n);
+    if (s->prm.sum_mode != NBG_SUM_FAST) return 1;
+    const bool lbvh = is_lbvh(s->prm.solver);
+    if (!lbvh && !(is_octree(s->prm.solver) && s->prm.solver != NBG_IMPLICIT_FMM)) return 1;
+    if (lbvh && s->prm.targets_per_lane > 1) return 1;
+    const int k = s->split_req > 0 ? s->split_req : split_waves(s->n);
+    return lbvh ? k : std::min(k, 8);
 }
 
 void free_split(nbg_sys *s) {

@@ -942,19 +942,86 @@ __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_oct_ordered(WalkArg
 }
 
 
-// Single system: wave w walks targets 64 w .. 64 w + 63 from the root;
-// forest: the wave's system's targets from its root (wave_task).
-template <int WAVES, bool SOFT, int STATS, bool QUAD, bool FOREST>
+// The octree's top walk: the root, then its child group when every child in
+// it is internal and holds more than ceil(root count / 32) bodies; the nodes
+// it accepts sum into s_top.  The frontier is the child group of each root
+// child some lane opens (or the root's own group when it is not expanded):
+// at most 8 entries, each one block.  Sums per block as in the FAST
+// linear-BVH walks; an entry (group, mask) is exactly the group loop's own
+// state.
+constexpr int kOctMaxFrontier = 8;
+
+__device__ __forceinline__ bool oct_top(const OctRec &r, int cut) { return r.topo.z >= 0 && r.topo.y > cut; }
+
+template <bool SOFT, int STATS, bool QUAD, bool INTERACT>
+__device__ int oct_top_walk(const OctRec *__restrict__ nodes, const float4 *__restrict__ quad,
+                            const float4 *__restrict__ bodies, int64_t n, int root, uint64_t vmask, float eps2,
+                            FastWave &W, int *f_grp, uint64_t *f_mask, int64_t max_nodes, float4 *sh, bool &fail) {
+    const OctRec r = load_uniform(nodes + root);
+    float ex, ey, ez, d2;
+    uint64_t open;
+    if (INTERACT) open = visit_oct<SOFT, STATS, QUAD>(r, quad + 2 * root, vmask, bodies, n, eps2, W, fail, sh);
+    else open = r.topo.z >= 0 ? vmask & ~oct_accept_mask(r, vmask, W, ex, ey, ez, d2) : 0ull;
+    if (open == 0) return 0;  // a leaf root, or every lane accepted it
+    const int c = r.topo.z, nch = oct_nchild(r);
+    if (c < 0 || nch < 1 || nch > 8 || (int64_t)c + nch > max_nodes) {
+        fail = true;
+        return 0;
+    }
+    const int cut = (r.topo.y + 31) / 32;
+    bool expand = true;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (k < nch) expand = expand && oct_top(load_uniform(nodes + c + k), cut);
+    if (!expand) {
+        if (lane_id() == 0) { f_grp[0] = c | (nch << 28); f_mask[0] = open; }
+        __builtin_amdgcn_wave_barrier();
+        return 1;
+    }
+    int F = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (k >= nch) continue;
+        const OctRec x = load_uniform(nodes + c + k);
+        const uint64_t o = INTERACT ? oct_node<SOFT, STATS, QUAD>(x, quad + 2 * (c + k), open, eps2, W)
+                                    : open & ~oct_accept_mask(x, open, W, ex, ey, ez, d2);
+        if (o == 0) continue;
+        const int cc = x.topo.z, nn = oct_nchild(x);
+        if (cc < 0 || nn < 1 || nn > 8 || (int64_t)cc + nn > max_nodes) {
+            fail = true;
+            break;
+        }
+        if (lane_id() == 0) { f_grp[F] = cc | (nn << 28); f_mask[F] = o; }
+        ++F;
+    }
+    __builtin_amdgcn_wave_barrier();
+    return F;
+}
+
+// FAST octree walk.  FOREST: the wave's system from the forest's wave table.
+// SPLIT (one system): a.split waves per group of 64 targets, wave k walking
+// the blocks b with b mod a.split == k (wave 0 also the top nodes), the sums
+// meeting in k_walk_split_reduce.
+template <int WAVES, bool SOFT, int STATS, bool QUAD, bool FOREST, bool SPLIT>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
+    static_assert(!(SPLIT && FOREST), "split groups are single-system walks");
     __shared__ int s_grp[WAVES][kStackOct];       // first child | nchild << 28
     __shared__ uint64_t s_mask[WAVES][kStackOct];
     __shared__ float4 s_leaf[WAVES][kLeafScratch];
     __shared__ int4 s_lq[WAVES][8];                // a group's opened leaves: first, cnt, open mask
+    __shared__ int f_grp[WAVES][kOctMaxFrontier];  // first child | nchild << 28
+    __shared__ uint64_t f_mask[WAVES][kOctMaxFrontier];
 
     const float4 *__restrict__ bodies = a.bodies;
     const OctRec *__restrict__ nodes = a.onodes;
-    const int lane = lane_id(), wv = uniform_i32(threadIdx.x / kWave);
-    const WaveTask T = wave_task<FOREST>(a, xcd_block(blockIdx.x, gridDim.x) * WAVES + wv);
+    // the wave index is uniform: readfirstlane makes the per-wave task
+    // scalar, while LDS addresses use the per-lane copy (fewer SGPRs)
+    const int lane = lane_id(), wv = threadIdx.x / kWave;
+    const int64_t gw = (int64_t)xcd_block(blockIdx.x, gridDim.x) * WAVES + uniform_i32(wv);
+    const int K = SPLIT ? a.split : 1;
+    const int64_t grp = SPLIT ? gw / K : gw;
+    const int k = SPLIT ? (int)(gw - grp * K) : 0;
+    const WaveTask T = wave_task<FOREST>(a, grp);
     const int64_t tgt = T.base + lane;
     const bool valid = tgt < T.end;
     const uint64_t vmask = __ballot(valid);
@@ -972,22 +1039,67 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
     const int64_t n = a.n;
     bool fail = false;
 
-    const OctRec root = nodes[T.root];
-    uint64_t mask = visit_oct<SOFT, STATS, QUAD>(root, a.quad + 2 * T.root, vmask, bodies, n, eps2, W, fail,
-                                                 s_leaf[wv]);
-    int c = root.topo.z, nch = oct_nchild(root);
-    int sp = 0;
+    int F;
+    if (!SPLIT || k == 0)
+        F = oct_top_walk<SOFT, STATS, QUAD, true>(nodes, a.quad, bodies, n, T.root, vmask, eps2, W, f_grp[wv],
+                                                  f_mask[wv], a.max_nodes, s_leaf[wv], fail);
+    else
+        F = oct_top_walk<SOFT, STATS, QUAD, false>(nodes, a.quad, bodies, n, T.root, vmask, eps2, W, f_grp[wv],
+                                                   f_mask[wv], a.max_nodes, s_leaf[wv], fail);
+    F = uniform_i32(F);
+    // s_top, then each block's sum from zero
+    fast_flush(W);
+    double S0 = W.sx, S1 = W.sy, S2 = W.sz;
+    W.sx = 0.0; W.sy = 0.0; W.sz = 0.0;
+    const int64_t stride = a.split_stride;
+    if (SPLIT && k == 0 && valid) {
+        a.split_part[tgt] = S0;
+        a.split_part[stride + tgt] = S1;
+        a.split_part[2 * stride + tgt] = S2;
+        if (lane == 0) a.split_nb[grp] = F;
+    }
+
+    // One loop, shaped as the plain walk's: when a frontier subtree is done
+    // (empty mask, empty stack) the refill path closes its block and starts
+    // the next owned entry.
+    int i = -1, c = 0, nch = 0, sp = 0;
+    uint64_t mask = 0;
     const int64_t max_iters = a.max_nodes + 64;  // each node's group is visited at most once
     int64_t iters = 0;
     while (!fail) {
         if (mask == 0) {
-            if (sp == 0) break;
-            --sp;
-            __builtin_amdgcn_wave_barrier();
-            const int g = uniform_i32(s_grp[wv][sp]);
-            c = g & 0x0FFFFFFF;
-            nch = (int)((uint32_t)g >> 28);
-            mask = uniform_u64(s_mask[wv][sp]);
+            if (sp == 0) {
+                // the next frontier group this wave owns, after closing the
+                // block of the entry just walked (each entry is a block)
+                if (i >= 0) {
+                    fast_flush(W);
+                    if (SPLIT) {
+                        if (valid) {
+                            const int64_t o = (int64_t)(1 + i) * 3 * stride + tgt;
+                            a.split_part[o] = W.sx;
+                            a.split_part[o + stride] = W.sy;
+                            a.split_part[o + 2 * stride] = W.sz;
+                        }
+                    } else {
+                        S0 += W.sx; S1 += W.sy; S2 += W.sz;
+                    }
+                    W.sx = 0.0; W.sy = 0.0; W.sz = 0.0;
+                }
+                ++i;
+                if (SPLIT) i += (k - i % K + K) % K;
+                if (i >= F) break;
+                const int g = uniform_i32(f_grp[wv][i]);
+                c = g & 0x0FFFFFFF;
+                nch = (int)((uint32_t)g >> 28);
+                mask = uniform_u64(f_mask[wv][i]);
+            } else {
+                --sp;
+                __builtin_amdgcn_wave_barrier();
+                const int g = uniform_i32(s_grp[wv][sp]);
+                c = g & 0x0FFFFFFF;
+                nch = (int)((uint32_t)g >> 28);
+                mask = uniform_u64(s_mask[wv][sp]);
+            }
         }
         c = uniform_i32(c);
         nch = uniform_i32(nch);
@@ -1001,13 +1113,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
         // in LDS and evaluated after the group.
         uint64_t nm = 0;
         int nc = 0, nn = 0, nl = 0;
-        const OctRec *grp = nodes + c;
+        const OctRec *grp_nodes = nodes + c;
         const float4 *qg = a.quad + 2 * c;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            if (k >= nch) break;
-            const OctRec r = load_uniform(grp + k);
-            const uint64_t o = oct_node<SOFT, STATS, QUAD>(r, qg + 2 * k, mask, eps2, W);
+        for (int q = 0; q < 8; ++q) {
+            if (q >= nch) break;
+            const OctRec r = load_uniform(grp_nodes + q);
+            const uint64_t o = oct_node<SOFT, STATS, QUAD>(r, qg + 2 * q, mask, eps2, W);
             if (o == 0) continue;
             if (r.topo.z >= 0) {
                 if (nm) {
@@ -1026,8 +1138,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
             }
         }
         __builtin_amdgcn_wave_barrier();
-        for (int i = 0; i < nl; ++i) {
-            const int4 q = s_lq[wv][i];
+        for (int j = 0; j < nl; ++j) {
+            const int4 q = s_lq[wv][j];
             const int first = uniform_i32(q.x), cnt = uniform_i32(q.y);
             const uint64_t o = ((uint64_t)(uint32_t)uniform_i32(q.w) << 32) | (uint32_t)uniform_i32(q.z);
             leaf_eval<SOFT>(bodies, first, cnt, o, eps2, W, s_leaf[wv]);
@@ -1038,11 +1150,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
         nch = nn;
     }
     if (fail && lane == 0) atomicOr(a.err, 2u);
-    if (valid) {
+    if (!SPLIT && valid) {
         const uint32_t o = a.perm[tgt];
         if ((int64_t)o < a.nt) {
-            const float3 r = fast_result(W, W.G);
-            a.ax[o] = r.x; a.ay[o] = r.y; a.az[o] = r.z;
+            const double g = (double)W.G;
+            a.ax[o] = (float)(S0 * g);
+            a.ay[o] = (float)(S1 * g);
+            a.az[o] = (float)(S2 * g);
         }
     }
     if (lane == 0) {
@@ -1066,11 +1180,17 @@ static void launch_oct_fast_q(const WalkArgs &a, hipStream_t s) {
     const unsigned blocks = (unsigned)((walk_waves(a) + W - 1) / W);
     const dim3 g(blocks), t(64 * W);
     if (a.nsys > 0) {
-        if (a.eps2 > 0.0f) hipLaunchKernelGGL((k_walk_oct_fast<W, true, STATS, QUAD, true>), g, t, 0, s, a);
-        else hipLaunchKernelGGL((k_walk_oct_fast<W, false, STATS, QUAD, true>), g, t, 0, s, a);
+        if (a.eps2 > 0.0f) hipLaunchKernelGGL((k_walk_oct_fast<W, true, STATS, QUAD, true, false>), g, t, 0, s, a);
+        else hipLaunchKernelGGL((k_walk_oct_fast<W, false, STATS, QUAD, true, false>), g, t, 0, s, a);
+    } else if (a.split > 1 && a.split_part && a.split_nb) {
+        const int64_t waves = (a.nt + kWave - 1) / kWave * a.split;
+        const dim3 gs((unsigned)((waves + W - 1) / W));
+        if (a.eps2 > 0.0f) hipLaunchKernelGGL((k_walk_oct_fast<W, true, STATS, QUAD, false, true>), gs, t, 0, s, a);
+        else hipLaunchKernelGGL((k_walk_oct_fast<W, false, STATS, QUAD, false, true>), gs, t, 0, s, a);
+        hipLaunchKernelGGL(k_walk_split_reduce, dim3((unsigned)((a.nt + 255) / 256)), dim3(256), 0, s, a);
     } else {
-        if (a.eps2 > 0.0f) hipLaunchKernelGGL((k_walk_oct_fast<W, true, STATS, QUAD, false>), g, t, 0, s, a);
-        else hipLaunchKernelGGL((k_walk_oct_fast<W, false, STATS, QUAD, false>), g, t, 0, s, a);
+        if (a.eps2 > 0.0f) hipLaunchKernelGGL((k_walk_oct_fast<W, true, STATS, QUAD, false, false>), g, t, 0, s, a);
+        else hipLaunchKernelGGL((k_walk_oct_fast<W, false, STATS, QUAD, false, false>), g, t, 0, s, a);
     }
 }
 

@@ -275,19 +275,28 @@ __device__ __forceinline__ void interact_quad_e(float ex, float ey, float ez, fl
     W.rz = __builtin_fmaf(-ez, s, __builtin_fmaf(t, qz, W.rz));
 }
 
+// The lanes of `mask` that accept octree node nd (oct_accepts_e as one
+// ballot per comparison: the lane masks combine in scalar registers without
+// turning the predicate into a vector value); e = p - com and d2 = |e|^2.
+__device__ __forceinline__ uint64_t oct_accept_mask(const OctRec &nd, uint64_t mask, const FastWave &W, float &ex,
+                                                    float &ey, float &ez, float &d2) {
+    ex = W.px - nd.com.x;
+    ey = W.py - nd.com.y;
+    ez = W.pz - nd.com.z;
+    d2 = ex * ex + ey * ey + ez * ez;
+    const uint64_t inside = __ballot(nd.lo.x <= W.px) & __ballot(nd.lo.y <= W.py) & __ballot(nd.lo.z <= W.pz) &
+                            __ballot(W.px < nd.hi.x) & __ballot(W.py < nd.hi.y) & __ballot(W.pz < nd.hi.z);
+    return __ballot(d2 > nd.lo.w) & ~inside & mask;
+}
+
 // The node half of an octree visit: the criterion for the lanes in `mask`
 // (wave-uniform), the node interaction for those that accept; returns the
 // lanes that open the node.
 template <bool SOFT, int STATS, bool QUAD>
 __device__ __forceinline__ uint64_t oct_node(const OctRec &nd, const float4 *__restrict__ qn, uint64_t mask,
                                              float eps2, FastWave &W) {
-    // oct_accepts_e as one ballot per comparison: the lane masks combine in
-    // scalar registers without turning the predicate into a vector value
-    const float ex = W.px - nd.com.x, ey = W.py - nd.com.y, ez = W.pz - nd.com.z;
-    const float d2 = ex * ex + ey * ey + ez * ez;
-    const uint64_t inside = __ballot(nd.lo.x <= W.px) & __ballot(nd.lo.y <= W.py) & __ballot(nd.lo.z <= W.pz) &
-                            __ballot(W.px < nd.hi.x) & __ballot(W.py < nd.hi.y) & __ballot(W.pz < nd.hi.z);
-    const uint64_t acc = __ballot(d2 > nd.lo.w) & ~inside & mask;
+    float ex, ey, ez, d2;
+    const uint64_t acc = oct_accept_mask(nd, mask, W, ex, ey, ez, d2);
     if (STATS >= 2) { W.visits += 1; W.lanes += __popcll(mask); }
     if (STATS >= 1) W.np += __popcll(acc);
     if (QUAD) {

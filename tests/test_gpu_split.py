@@ -1,8 +1,9 @@
-"""The FAST linear-BVH walk split over several waves per group of 64
-targets: every split (1 to 16 waves, and the size-chosen default) gives the
-same bits and the same interaction counts, and those counts are the
-oracle's.  The same bits as the forest and the several-targets-per-lane
-walks are checked where those are tested against a system alone."""
+"""The FAST tree walks (linear BVH and octree, monopole and quadrupole)
+split over several waves per group of 64 targets: every split (1 to 16
+waves, and the size-chosen default) gives the same bits and the same
+interaction counts, and those counts are the oracle's.  The same bits as
+the forest and the several-targets-per-lane walks are checked where those
+are tested against a system alone."""
 
 import numpy as np
 import pytest
@@ -23,12 +24,16 @@ CASES = [(1, "plummer"), (2, "cube"), (63, "plummer"), (65, "cube"), (1000, "plu
          (10000, "plummer"), (50000, "plummer"), (65536, "cube")]
 
 
+@pytest.mark.parametrize("tree", ["lbvh", "octree"])
 @pytest.mark.parametrize("quad", [False, True])
 @pytest.mark.parametrize("n,kind", CASES)
-def test_split_walk_same_bits(nbg, orc, gpu_ctx, n, kind, quad):
-    print(f"[split] n={n} {kind} quad={quad}", flush=True)
+def test_split_walk_same_bits(nbg, orc, gpu_ctx, n, kind, quad, tree):
+    print(f"[split] n={n} {kind} quad={quad} {tree}", flush=True)
     b = nbg.plummer(n, seed=3000 + n) if kind == "plummer" else nbg.cube(n, seed=3000 + n, side=2.0)
-    solver = nbg.LINEAR_BVH_QUADRUPOLE if quad else nbg.LINEAR_BVH
+    if tree == "lbvh":
+        solver = nbg.LINEAR_BVH_QUADRUPOLE if quad else nbg.LINEAR_BVH
+    else:
+        solver = nbg.BARNES_HUT_QUADRUPOLE if quad else nbg.BARNES_HUT
     s = nbg.System(gpu_ctx, _params(nbg, solver), n)
     s.set_state(b)
     ref = None
@@ -44,7 +49,7 @@ def test_split_walk_same_bits(nbg, orc, gpu_ctx, n, kind, quad):
     s.close()
     if quad:
         return
-    t = orc.Tree(1, b.x, b.y, b.z, b.m, theta=0.5, leaf=16, bits=63, eps2=1e-4)
+    t = orc.Tree(1 if tree == "lbvh" else 0, b.x, b.y, b.z, b.m, theta=0.5, leaf=16, bits=63, eps2=1e-4)
     want, pp, npc = t.accel(b.x, b.y, b.z, counts=True)
     assert ref[1] == (int(pp.sum()), int(npc.sum()))
     mag = np.linalg.norm(want, axis=1)
