@@ -331,10 +331,12 @@ __device__ __forceinline__ void visit_pair(const NodeRec &r0, const NodeRec &r1,
 //     target reaches is walked depth-first as before, its interactions summed
 //     from zero (FP32 per leaf, FP64 across leaves) into B_b;
 //   result: ((s_top + B_0) + B_1) + ... in FP64, times G, rounded once.
-// Only node records decide the structure, so a wave walking a group alone
-// and K waves splitting its blocks (the sums meeting in k_walk_split_reduce)
-// give the same bits.  The opening decisions, and so the interaction set,
-// are those of the plain depth-first walk.
+// A tree of more than kSplitMaxBodies bodies (one that never splits) has no
+// top nodes: its whole walk is one entry, the plain walk.  Only node records
+// decide the structure, so a wave walking a group alone and K waves
+// splitting its blocks (the sums meeting in k_walk_split_reduce) give the
+// same bits.  The opening decisions, and so the interaction set, are those
+// of the plain depth-first walk.
 constexpr int kTopDepth = 7;
 constexpr int kMaxFrontier = 1 << kTopDepth;
 constexpr int kFrontierBlock = 8;
@@ -370,7 +372,8 @@ __device__ int top_walk(const NodeRec *__restrict__ nodes, const float4 *__restr
                         int *st_node, uint64_t *st_mask, int *st_dep, uint32_t last_pair, bool &fail) {
     const NodeRec r = load_uniform(nodes + root);
     const int cut = (r.topo.y + 31) / 32;
-    if (!lbvh_top(r, 0, cut)) {
+    // a tree too large to split is one frontier entry: the plain walk
+    if (!lbvh_top(r, 0, cut) || r.topo.y > kSplitMaxBodies) {
         if (lane_id() == 0) {
             f_node[0] = root;
 #pragma unroll
@@ -741,6 +744,96 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
     }
 }
 
+// A single system above kSplitMaxBodies bodies, one target per lane: the
+// plain walk (wave w walks targets 64 w .. 64 w + 63 from the root, one FP32
+// partial flushed into the FP64 sum after every leaf).  Such a tree has no
+// top nodes, so the block form's result is the same bits: one entry, the
+// root, walked the same way, its FP64 sum added to s_top = +0.0 (a sum that
+// starts at +0.0 never becomes -0.0, so the addition is exact).
+template <int WAVES, bool SOFT, int STATS, bool QUAD>
+__global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_plain(WalkArgs a) {
+    __shared__ int s_node[WAVES][kStackFast];
+    __shared__ uint64_t s_mask[WAVES][kStackFast];
+    __shared__ float4 s_leaf[WAVES][kLeafScratch];
+
+    const float4 *__restrict__ bodies = a.bodies;
+    const NodeRec *__restrict__ nodes = a.nodes;
+    const int lane = lane_id(), wv = threadIdx.x / kWave;
+    const int64_t tgt = (xcd_block(blockIdx.x, gridDim.x) * WAVES + wv) * kWave + lane;
+    const bool valid = tgt < a.nt;
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0) return;
+
+    FastWave W;
+    const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    fast_init(W, me.x, me.y, me.z);
+    // A VGPR copy of G (non-volatile asm: no memory effects, so the node and
+    // body loads below stay scalar).
+    asm("v_mov_b32 %0, %1" : "=v"(W.G) : "s"(a.G));
+    const float eps2 = a.eps2;
+    const int64_t n = a.n;
+    const uint32_t last_pair = (uint32_t)(a.max_nodes - 1);
+    bool fail = false;
+
+    // The root is visited on its own; afterwards every step visits the two
+    // children (c, c + 1) of a node that the lanes in `mask` opened.
+    const NodeRec root = nodes[0];
+    uint64_t mask = visit<SOFT, STATS, QUAD>(root, a.quad, vmask, bodies, n, eps2, W, fail, s_leaf[wv]);
+    int c = root.topo.z;
+    int sp = 0;
+    uint32_t iters = 0;
+    while (true) {
+        if (mask == 0) {
+            if (sp == 0) break;
+            --sp;
+            __builtin_amdgcn_wave_barrier();
+            c = uniform_i32(s_node[wv][sp]);
+            mask = uniform_u64(s_mask[wv][sp]);
+        }
+        c = uniform_i32(c);  // keeps the node loads scalar
+        if (__builtin_expect((uint32_t)c >= last_pair, 0)) { fail = true; break; }
+        if (__builtin_expect(++iters > kMaxWalkIters, 0)) { fail = true; break; }
+        const NodeRec r0 = load_uniform(nodes + c), r1 = load_uniform(nodes + c + 1);
+        uint64_t o0, o1;
+        visit_pair<SOFT, STATS, QUAD>(r0, r1, a.quad + 2 * c, mask, bodies, n, eps2, W, fail, s_leaf[wv], o0, o1);
+        // a failed visit opens nothing, so the walk still ends; fail is
+        // reported after the loop
+        if (o0 != 0) {
+            if (o1 != 0) {
+                if (sp >= kStackFast) { fail = true; break; }
+                if (lane == 0) { s_node[wv][sp] = r1.topo.z; s_mask[wv][sp] = o1; }
+                __builtin_amdgcn_wave_barrier();
+                ++sp;
+            }
+            c = r0.topo.z;
+            mask = o0;
+        } else if (o1 != 0) {
+            c = r1.topo.z;
+            mask = o1;
+        } else {
+            mask = 0;
+        }
+    }
+    if (fail && lane == 0) atomicOr(a.err, 2u);
+    if (valid) {
+        const uint32_t o = a.perm[tgt];
+        if ((int64_t)o < a.nt) {
+            const float3 r = fast_result(W, W.G);
+            a.ax[o] = r.x; a.ay[o] = r.y; a.az[o] = r.z;
+        }
+    }
+    if (lane == 0) {
+        atomicAdd(&a.counts[0], (unsigned long long)W.pp);
+        atomicAdd(&a.counts[1], (unsigned long long)W.np);
+        if (STATS >= 2) {
+            atomicAdd(&a.counts[2], (unsigned long long)W.visits);
+            atomicAdd(&a.counts[3], (unsigned long long)W.lanes);
+            atomicAdd(&a.counts[4], (unsigned long long)W.leaf_visits);
+            atomicAdd(&a.counts[5], (unsigned long long)W.leaf_lanes);
+        }
+    }
+}
+
 // The split walk's sums, ((s_top + B_0) + B_1) + ... for each target in
 // FP64, times G, rounded once and stored in output order.
 __global__ __launch_bounds__(256) void k_walk_split_reduce(WalkArgs a) {
@@ -942,8 +1035,9 @@ __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_oct_ordered(WalkArg
 }
 
 
-// The octree's top walk: the root, then its child group when every child in
-// it is internal and holds more than ceil(root count / 32) bodies; the nodes
+// The octree's top walk: the root, then (for a tree of at most
+// kSplitMaxBodies bodies) its child group when every child in it is internal
+// and holds more than ceil(root count / 32) bodies; the nodes
 // it accepts sum into s_top.  The frontier is the child group of each root
 // child some lane opens (or the root's own group when it is not expanded):
 // at most 8 entries, each one block.  Sums per block as in the FAST
@@ -969,7 +1063,8 @@ __device__ int oct_top_walk(const OctRec *__restrict__ nodes, const float4 *__re
         return 0;
     }
     const int cut = (r.topo.y + 31) / 32;
-    bool expand = true;
+    // a tree too large to split keeps the root's group as its one entry
+    bool expand = r.topo.y <= kSplitMaxBodies;
 #pragma unroll
     for (int k = 0; k < 8; ++k)
         if (k < nch) expand = expand && oct_top(load_uniform(nodes + c + k), cut);
@@ -1253,6 +1348,13 @@ static void launch_fast_q(const WalkArgs &a, hipStream_t s) {
     if (a.split > 1 && a.split_part && a.split_nb) {
         launch_lbvh_fast_k<STATS, QUAD, 1, false, true>(a, groups * a.split, s);
         hipLaunchKernelGGL(k_walk_split_reduce, dim3((unsigned)((a.nt + 255) / 256)), dim3(256), 0, s, a);
+        return;
+    }
+    if (a.n > kSplitMaxBodies) {
+        constexpr int W = 4;
+        const dim3 g((unsigned)((groups + W - 1) / W)), t(64 * W);
+        if (a.eps2 > 0.0f) hipLaunchKernelGGL((k_walk_lbvh_plain<W, true, STATS, QUAD>), g, t, 0, s, a);
+        else hipLaunchKernelGGL((k_walk_lbvh_plain<W, false, STATS, QUAD>), g, t, 0, s, a);
         return;
     }
     launch_lbvh_fast_k<STATS, QUAD, 1, false, false>(a, groups, s);

@@ -94,7 +94,7 @@ def test_lbvh_fast_within_tolerance(nbg, orc, gpu_ctx, n, leaf, bits, theta):
 
 
 @pytest.mark.parametrize("n,bits,quad", [(20037, 30, False), (50000, 63, False), (20037, 30, True), (130, 30, False),
-                                          (1, 30, False)])
+                                          (1, 30, False), (100003, 30, False), (100003, 30, True)])
 def test_lbvh_targets_per_lane_same_bits(nbg, gpu_ctx, n, bits, quad):
     """FAST with 2 or 4 targets per lane (128 or 256 bodies per wavefront):
     every 64-target slot keeps its own masks and visits its nodes in the

@@ -63,3 +63,27 @@ def test_split_argument(nbg, gpu_ctx):
     with pytest.raises(nbg.NbgError):
         s.set_walk_split(3)
     s.close()
+
+
+@pytest.mark.parametrize("tree", ["lbvh", "octree"])
+@pytest.mark.parametrize("quad", [False, True])
+def test_forest_member_equals_alone_on_both_sides_of_the_split_limit(nbg, gpu_ctx, tree, quad):
+    """Systems of 65,536 bodies or fewer (block form, split alone) and larger
+    (one block: the plain walk alone) in one forest: each member's
+    accelerations are the bits of the same system alone."""
+    if tree == "lbvh":
+        solver = nbg.LINEAR_BVH_QUADRUPOLE if quad else nbg.LINEAR_BVH
+    else:
+        solver = nbg.BARNES_HUT_QUADRUPOLE if quad else nbg.BARNES_HUT
+    systems = [nbg.plummer(70001, seed=81), nbg.cube(9000, seed=82, side=2.0), nbg.plummer(65536, seed=83)]
+    p = _params(nbg, solver, bits=30)
+    B = nbg.Batch(gpu_ctx, 150000, 8, sum_mode=nbg.SUM_FAST)
+    ids = [B.add(p, b) for b in systems]
+    B.update_accelerations()
+    for sid, b in zip(ids, systems):
+        _, got = B.get(sid, accelerations=True)
+        s = nbg.System(gpu_ctx, p, b.n)
+        s.set_state(b)
+        want = s.update_accelerations()
+        s.close()
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), f"n={b.n}"
