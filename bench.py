@@ -37,7 +37,9 @@ METRIC = "bodies·steps/s at N=1M θ=0.5; batched-serving p50/p99 latency at fix
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (packed) peak
 FLOPS_PER_INTERACTION = 20
 SOLVER_NAMES = {0: "barnes-hut", 1: "linear-bvh", 3: "barnes-hut-4p", 4: "linear-bvh-4p", 5: "implicit-fmm"}
-WALK_KERNELS = {0: "k_walk_oct_fast", 1: "k_walk_lbvh_fast", 3: "k_walk_oct_fast", 4: "k_walk_lbvh_fast",
+ORDERED_WALK_KERNELS = {0: "k_walk_oct_ordered", 1: "k_walk_lbvh_ordered", 3: "k_walk_oct_ordered",
+                        4: "k_walk_lbvh_ordered"}
+WALK_KERNELS = {0: "k_walk_oct_fast", 1: "k_walk_lbvh_plain", 3: "k_walk_oct_fast", 4: "k_walk_lbvh_plain",
                 5: "k_fmm_groups + k_fmm_step (the whole traversal)"}
 # FP32 operations of one body or node into a local field (difference, r^2,
 # rsqrt and powers, the vector's 3 and the gradient's 6 accumulating FMAs).
@@ -265,7 +267,7 @@ def measure(nbg, ctx, b, args, solver, mode=None):
         } if not fmm and ws["node_visits"] else (ws if fmm else None),
         "roofline": {
             "bound": "valu",
-            "kernel": WALK_KERNELS[solver] if mode == "fast" else WALK_KERNELS[solver].replace("fast", "ordered"),
+            "kernel": WALK_KERNELS[solver] if mode == "fast" else ORDERED_WALK_KERNELS[solver],
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
