@@ -196,14 +196,17 @@ int nbg_sys_enable_counts(nbg_sys *sys, int32_t on);
 int nbg_sys_enable_walk_stats(nbg_sys *sys, int32_t on);
 int nbg_sys_walk_stats(nbg_sys *sys, uint64_t *out4);
 
-/* Waves per group of 64 targets in the FAST tree walks (linear BVH with one
- * target per lane, octree; no reference counterpart: a schedule choice).  0
- * (the default) chooses by size: systems of at most 65,536 bodies fill the
- * GPU with up to 16 waves per group (octree: 8), larger ones use 1.  Results
- * are the same bits for every choice: the walk sums each target's
- * interactions per fixed block of subtrees and adds the blocks in one order
- * (the forest and the several-targets-per-lane walks use the same blocks).
- * A split walk keeps 408 bytes per body of sums in HBM. */
+/* Waves per group of 64 targets in the tree walks (FAST: linear BVH with one
+ * target per lane, octree; ORDERED: linear BVH; no reference counterpart: a
+ * schedule choice).  0 (the default) chooses by size: systems of at most
+ * 65,536 bodies (ORDERED: 16,384) fill the GPU with up to 16 waves per group
+ * (FAST octree: 8), larger ones use 1.  Results are the same bits for every
+ * choice: FAST sums each target's interactions per fixed block of subtrees and
+ * adds the blocks in one order (the forest and the several-targets-per-lane
+ * walks use the same blocks); ORDERED computes each frontier subtree's result
+ * from zero, as the reference's recursion does, and adds it where the walk
+ * meets it.  A split walk keeps 408 (FAST) or 1,536 (ORDERED) bytes per body
+ * of sums in HBM. */
 int nbg_sys_set_walk_split(nbg_sys *sys, int32_t waves);
 
 /* Per-phase device time (ms) of the last force pass, measured with HIP events

@@ -91,6 +91,7 @@ struct nbg_sys {
     double *split_part = nullptr;
     int32_t *split_nb = nullptr;
     int64_t split_cap = 0;
+    int64_t split_words = 0;  // doubles per target in split_part
     int split_req = 0;
     // passive-point walks (nbg_sys_accelerations_at): one buffer that grows
     // to the largest target count seen and is kept for later calls
@@ -226,8 +227,11 @@ bool is_lbvh(int solver) { return solver == NBG_LINEAR_BVH || solver == NBG_LINE
 // solvers, the ORDERED mode and several targets per lane).  The octree's
 // frontier holds at most 8 blocks, so it uses at most 8 waves.
 int split_of(const nbg_sys *s) {
-    if (s->prm.sum_mode != NBG_SUM_FAST) return 1;
     const bool lbvh = is_lbvh(s->prm.solver);
+    if (s->prm.sum_mode != NBG_SUM_FAST) {
+        if (!lbvh || s->n > kOrderedSplitMaxBodies) return 1;
+        return s->split_req > 0 ? s->split_req : split_waves(s->n);
+    }
     if (!lbvh && !(is_octree(s->prm.solver) && s->prm.solver != NBG_IMPLICIT_FMM)) return 1;
     if (lbvh && s->prm.targets_per_lane > 1) return 1;
     const int k = s->split_req > 0 ? s->split_req : split_waves(s->n);
@@ -240,22 +244,30 @@ void free_split(nbg_sys *s) {
     s->split_part = nullptr;
     s->split_nb = nullptr;
     s->split_cap = 0;
+    s->split_words = 0;
 }
 
 // Allocates the split walk's sums when the system's walk splits (no-op
 // otherwise or when the current ones are large enough).
+// Doubles of split sums per target: the FAST blocks' FP64 sums, or the
+// ORDERED frontier results (FP32, two to a double).
+int64_t split_words(const nbg_sys *s) {
+    return s->prm.sum_mode == NBG_SUM_FAST ? (int64_t)kSplitSlots * 3 : (int64_t)kOrderedSplitSlots * 3 / 2;
+}
+
 int ensure_split(nbg_sys *s) {
-    if (split_of(s) <= 1 || s->split_cap >= s->n) return NBG_OK;
+    if (split_of(s) <= 1 || (s->split_cap >= s->n && s->split_words >= split_words(s))) return NBG_OK;
     if (s->stream) NBG_HIP(hipStreamSynchronize(s->stream));
     free_split(s);
     const int64_t n = s->n > 0 ? s->n : 1;
-    int r = dalloc(&s->split_part, (int64_t)kSplitSlots * 3 * n);
+    int r = dalloc(&s->split_part, split_words(s) * n);
     if (r == NBG_OK) r = dalloc(&s->split_nb, (n + 63) / 64);
     if (r != NBG_OK) {
         free_split(s);
         return r;
     }
     s->split_cap = n;
+    s->split_words = split_words(s);
     return NBG_OK;
 }
 
@@ -403,9 +415,10 @@ int enqueue_force_pass(nbg_sys *s) {
     } else {
         WalkArgs wa = walk_args(s);
         const int k = split_of(s);
-        if (k > 1 && s->split_part && s->split_cap >= s->n) {
+        if (k > 1 && s->split_part && s->split_cap >= s->n && s->split_words >= split_words(s)) {
             wa.split = k;
             wa.split_part = s->split_part;
+            wa.split_part_f = reinterpret_cast<float *>(s->split_part);
             wa.split_nb = s->split_nb;
             wa.split_stride = s->n;
         }

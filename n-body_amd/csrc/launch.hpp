@@ -269,6 +269,7 @@ struct WalkArgs {
     // block counts in split_nb, and a second kernel adds them up.
     int split = 1;
     double *split_part = nullptr;
+    float *split_part_f = nullptr;  // ORDERED split: kMaxFrontier x 3 floats per target
     int32_t *split_nb = nullptr;
     int64_t split_stride = 0;
 };
@@ -278,6 +279,10 @@ struct WalkArgs {
 constexpr int kSplitSlots = 17;
 // Largest system whose FAST linear-BVH walk splits its groups over waves.
 constexpr int64_t kSplitMaxBodies = 65536;
+// ORDERED split: one result per frontier subtree (at most kOrderedSplitSlots)
+// per target, for systems of at most kOrderedSplitMaxBodies bodies.
+constexpr int kOrderedSplitSlots = 128;
+constexpr int64_t kOrderedSplitMaxBodies = 16384;
 // Split waves per group for a system of n bodies (1: the chip is full).
 inline int split_waves(int64_t n) {
     if (n > kSplitMaxBodies || n < 1) return 1;

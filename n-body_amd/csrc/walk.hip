@@ -41,6 +41,19 @@ constexpr int kStackFast = 96;
 // Upper bound on loop iterations per wave; a valid tree needs at most ~3 per node.
 constexpr uint32_t kMaxWalkIters = 1u << 28;
 
+// The top of a linear BVH (see the FAST walks below): internal nodes above
+// depth kTopDepth holding more than `cut` bodies.  Their other children are
+// the frontier, at most 2^kTopDepth nodes.
+constexpr int kTopDepth = 7;
+constexpr int kMaxFrontier = 1 << kTopDepth;
+constexpr int kFrontierBlock = 8;
+static_assert(1 + kMaxFrontier / kFrontierBlock == kSplitSlots, "split slots: s_top and one per block");
+static_assert(kMaxFrontier == kOrderedSplitSlots, "ORDERED split: one slot per frontier entry");
+
+__device__ __forceinline__ bool lbvh_top(const NodeRec &r, int depth, int cut) {
+    return r.topo.z >= 0 && r.topo.y > cut && depth < kTopDepth;
+}
+
 // The targets, root and rule of wave w: the whole tree for one system, or
 // the system that owns wave w in a forest (all wave-uniform).
 struct WaveTask {
@@ -94,46 +107,27 @@ constexpr int kOrderedWaves = 4;  // waves per workgroup of the ORDERED walks
 // Levels of pending partial sums an ORDERED linear-BVH wave keeps in LDS.
 constexpr int kSaveRing = 4;
 
-template <int MAXD, bool FOREST, bool QUAD>
-__global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_lbvh_ordered(WalkArgs a) {
-    // The traversal stack holds, per level of the current path, at most the
-    // completion marker and the second child, plus the three entries of a push.
+// The ORDERED descent from node `start` for the lanes of `smask`, the
+// reference's recursion (activeTree.h:24-79) with its bracketing: each node's
+// result into r (from zero), counts into cpp / cnp.  COMBINE: a visited node
+// that is not a top node (lbvh_top with the root's cut) is a frontier node:
+// its result is not walked but taken from part (slot fi, counted in visit
+// order) and added as the walk would add it.
+template <int MAXD, bool QUAD, bool COMBINE>
+__device__ __forceinline__ void ordered_descent(const WalkArgs &a, int start, uint64_t smask, int cut, float px,
+                                                float py, float pz, float G, float eps2, int *s_node, int *s_meta,
+                                                uint64_t *s_mask, float *ring, float &rx, float &ry, float &rz,
+                                                uint32_t &cpp, uint32_t &cnp, bool &fail, int64_t tgt) {
     constexpr int kStack = 2 * MAXD + 4;
-    __shared__ int s_node_all[kOrderedWaves][kStack];
-    __shared__ int s_meta_all[kOrderedWaves][kStack];
-    __shared__ uint64_t s_mask_all[kOrderedWaves][kStack];
-    // Per lane, the pending partial sum of every open depth.  The pending
-    // depths are those of the current path, saved in increasing depth and
-    // taken back in decreasing depth, so the deepest kSaveRing of them live
-    // in an LDS ring (slot d mod kSaveRing, lane-major: no bank conflicts)
-    // and only older ones go to a private array indexed by the wave-uniform
-    // depth (scratch memory): depths >= ring_base are in the ring.
-    __shared__ float s_ring_all[kOrderedWaves][kSaveRing * 3 * kWave];
     float s_save[MAXD * 3];
-
     const float4 *__restrict__ bodies = a.bodies;
     const NodeRec *__restrict__ nodes = a.nodes;
-    const int lane = lane_id(), wv = uniform_i32(threadIdx.x / kWave);
-    int *s_node = s_node_all[wv];
-    int *s_meta = s_meta_all[wv];
-    uint64_t *s_mask = s_mask_all[wv];
-    float *ring = s_ring_all[wv];
-    const WaveTask T = wave_task<FOREST>(a, (int64_t)xcd_block(blockIdx.x, gridDim.x) * kOrderedWaves + wv);
-    const int64_t tgt = T.base + lane;
-    const bool valid = tgt < T.end;
-    const uint64_t vmask = __ballot(valid);
-    if (vmask == 0) return;
-
-    const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float px = me.x, py = me.y, pz = me.z;
-    float rx = 0.0f, ry = 0.0f, rz = 0.0f;
-    uint32_t cpp = 0, cnp = 0;
-    const float G = T.G, eps2 = T.eps2;
-
-    if (lane == 0) { s_node[0] = T.root; s_meta[0] = 0; s_mask[0] = vmask; }
+    const int lane = lane_id();
+    int fi = 0;
+    if (lane == 0) { s_node[0] = start; s_meta[0] = 0; s_mask[0] = smask; }
+    __builtin_amdgcn_wave_barrier();
     int sp = 1;
     int ring_base = 0;
-    bool fail = false;
     uint32_t iters = 0;
     while (sp > 0) {
         if (++iters > kMaxWalkIters) { fail = true; break; }
@@ -168,6 +162,18 @@ __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_lbvh_ordered(WalkAr
         }
         if (node < 0 || node >= a.max_nodes) { fail = true; break; }
         const NodeRec nd = nodes[node];
+        if (COMBINE && !lbvh_top(nd, depth, cut)) {
+            // a frontier node: its subtree's result, walked from zero elsewhere
+            if (fi >= kMaxFrontier) { fail = true; break; }
+            if (act) {
+                const int64_t o = (int64_t)fi * 3 * a.split_stride + tgt;
+                const float vx = a.split_part_f[o], vy = a.split_part_f[o + a.split_stride],
+                            vz = a.split_part_f[o + 2 * a.split_stride];
+                rx = rx + vx; ry = ry + vy; rz = rz + vz;
+            }
+            ++fi;
+            continue;
+        }
         const bool pass = act && lbvh_accepts(nd.mac, px, py, pz);
         if (pass) {
             ++cnp;
@@ -226,6 +232,161 @@ __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_lbvh_ordered(WalkAr
             sp += 3;
         }
     }
+}
+
+// ORDERED linear-BVH walk, one wave per 64 targets from the root.
+template <int MAXD, bool FOREST, bool QUAD>
+__global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_lbvh_ordered(WalkArgs a) {
+    constexpr int kStack = 2 * MAXD + 4;
+    __shared__ int s_node_all[kOrderedWaves][kStack];
+    __shared__ int s_meta_all[kOrderedWaves][kStack];
+    __shared__ uint64_t s_mask_all[kOrderedWaves][kStack];
+    // Per lane, the pending partial sum of every open depth.  The pending
+    // depths are those of the current path, saved in increasing depth and
+    // taken back in decreasing depth, so the deepest kSaveRing of them live
+    // in an LDS ring (slot d mod kSaveRing, lane-major: no bank conflicts)
+    // and only older ones go to a private array indexed by the wave-uniform
+    // depth (scratch memory): depths >= ring_base are in the ring.
+    __shared__ float s_ring_all[kOrderedWaves][kSaveRing * 3 * kWave];
+
+    const int lane = lane_id(), wv = uniform_i32(threadIdx.x / kWave);
+    const WaveTask T = wave_task<FOREST>(a, (int64_t)xcd_block(blockIdx.x, gridDim.x) * kOrderedWaves + wv);
+    const int64_t tgt = T.base + lane;
+    const bool valid = tgt < T.end;
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0) return;
+    const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+    uint32_t cpp = 0, cnp = 0;
+    bool fail = false;
+    ordered_descent<MAXD, QUAD, false>(a, T.root, vmask, 0, me.x, me.y, me.z, T.G, T.eps2, s_node_all[wv],
+                                       s_meta_all[wv], s_mask_all[wv], s_ring_all[wv], rx, ry, rz, cpp, cnp, fail,
+                                       tgt);
+    if (fail && lane == 0) atomicOr(a.err, 2u);
+    if (valid) {
+        const uint32_t o = a.perm[tgt];
+        if ((int64_t)o < a.nt) { a.ax[o] = rx; a.ay[o] = ry; a.az[o] = rz; }
+    }
+    const uint32_t wpp = wave_sum(valid ? cpp : 0u), wnp = wave_sum(valid ? cnp : 0u);
+    if (lane == 0) {
+        atomicAdd(&a.counts[0], (unsigned long long)wpp);
+        atomicAdd(&a.counts[1], (unsigned long long)wnp);
+    }
+}
+
+// ORDERED split (one system): a.split waves per group of 64 targets.  Each
+// lists the frontier (the non-top children of opened top nodes, in the
+// ORDERED walk's visit order, masks from the exact criterion), and wave k
+// walks entries i with i mod a.split == k from zero, storing each subtree's
+// result per target in split_part_f for k_walk_lbvh_ordered_combine.
+template <int MAXD, bool QUAD>
+__global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_lbvh_ordered_split(WalkArgs a) {
+    constexpr int kStack = 2 * MAXD + 4;
+    __shared__ int s_node_all[kOrderedWaves][kStack];
+    __shared__ int s_meta_all[kOrderedWaves][kStack];
+    __shared__ uint64_t s_mask_all[kOrderedWaves][kStack];
+    __shared__ float s_ring_all[kOrderedWaves][kSaveRing * 3 * kWave];
+    __shared__ int f_node_all[kOrderedWaves][kMaxFrontier];
+    __shared__ uint64_t f_mask_all[kOrderedWaves][kMaxFrontier];
+
+    const int lane = lane_id(), wv = uniform_i32(threadIdx.x / kWave);
+    const int K = a.split;
+    const int64_t gw = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kOrderedWaves + wv;
+    const int64_t grp = gw / K;
+    const int k = (int)(gw - grp * K);
+    const WaveTask T = wave_task<false>(a, grp);
+    const int64_t tgt = T.base + lane;
+    const bool valid = tgt < T.end;
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0) return;
+    const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const NodeRec *__restrict__ nodes = a.nodes;
+    int *s_node = s_node_all[wv];
+    int *s_meta = s_meta_all[wv];
+    uint64_t *s_mask = s_mask_all[wv];
+    int *f_node = f_node_all[wv];
+    uint64_t *f_mask = f_mask_all[wv];
+    bool fail = false;
+
+    // the frontier, in the order the ORDERED walk visits it
+    const NodeRec root = nodes[T.root];
+    const int cut = (root.topo.y + 31) / 32;
+    int F = 0, sp = 1;
+    if (lane == 0) { s_node[0] = T.root; s_meta[0] = 0; s_mask[0] = vmask; }
+    __builtin_amdgcn_wave_barrier();
+    while (sp > 0) {
+        --sp;
+        __builtin_amdgcn_wave_barrier();
+        const int node = uniform_i32(s_node[sp]);
+        const int depth = uniform_i32(s_meta[sp]);
+        const uint64_t mask = uniform_u64(s_mask[sp]);
+        if (node < 0 || node >= a.max_nodes) { fail = true; break; }
+        const NodeRec nd = nodes[node];
+        if (!lbvh_top(nd, depth, cut)) {
+            if (F >= kMaxFrontier) { fail = true; break; }  // unreachable: depth <= kTopDepth
+            if (lane == 0) { f_node[F] = node; f_mask[F] = mask; }
+            ++F;
+            continue;
+        }
+        const bool act = (mask >> lane) & 1ull;
+        const uint64_t open = mask & ~__ballot(act && lbvh_accepts(nd.mac, me.x, me.y, me.z));
+        if (open == 0) continue;
+        if (sp + 2 > kStack) { fail = true; break; }
+        if (lane == 0) {
+            s_node[sp] = nd.topo.z + 1; s_meta[sp] = depth + 1; s_mask[sp] = open;
+            s_node[sp + 1] = nd.topo.z; s_meta[sp + 1] = depth + 1; s_mask[sp + 1] = open;
+        }
+        __builtin_amdgcn_wave_barrier();
+        sp += 2;
+    }
+    __builtin_amdgcn_wave_barrier();
+    uint32_t cpp = 0, cnp = 0;
+    const int64_t stride = a.split_stride;
+    for (int i = k; i < F && !fail; i += K) {
+        const int f = uniform_i32(f_node[i]);
+        const uint64_t m = uniform_u64(f_mask[i]);
+        float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+        ordered_descent<MAXD, QUAD, false>(a, f, m, cut, me.x, me.y, me.z, T.G, T.eps2, s_node, s_meta, s_mask,
+                                           s_ring_all[wv], rx, ry, rz, cpp, cnp, fail, tgt);
+        if (valid && ((m >> lane) & 1ull)) {
+            const int64_t o = (int64_t)i * 3 * stride + tgt;
+            a.split_part_f[o] = rx;
+            a.split_part_f[o + stride] = ry;
+            a.split_part_f[o + 2 * stride] = rz;
+        }
+    }
+    if (fail && lane == 0) atomicOr(a.err, 2u);
+    const uint32_t wpp = wave_sum(valid ? cpp : 0u), wnp = wave_sum(valid ? cnp : 0u);
+    if (lane == 0) {
+        atomicAdd(&a.counts[0], (unsigned long long)wpp);
+        atomicAdd(&a.counts[1], (unsigned long long)wnp);
+    }
+}
+
+// The ORDERED split's combine: the walk from the root over the top nodes,
+// with each frontier subtree's stored result added where the walk meets it.
+template <int MAXD, bool QUAD>
+__global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_lbvh_ordered_combine(WalkArgs a) {
+    constexpr int kStack = 2 * MAXD + 4;
+    __shared__ int s_node_all[kOrderedWaves][kStack];
+    __shared__ int s_meta_all[kOrderedWaves][kStack];
+    __shared__ uint64_t s_mask_all[kOrderedWaves][kStack];
+    __shared__ float s_ring_all[kOrderedWaves][kSaveRing * 3 * kWave];
+
+    const int lane = lane_id(), wv = uniform_i32(threadIdx.x / kWave);
+    const WaveTask T = wave_task<false>(a, (int64_t)xcd_block(blockIdx.x, gridDim.x) * kOrderedWaves + wv);
+    const int64_t tgt = T.base + lane;
+    const bool valid = tgt < T.end;
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0) return;
+    const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const int cut = (a.nodes[T.root].topo.y + 31) / 32;
+    float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+    uint32_t cpp = 0, cnp = 0;
+    bool fail = false;
+    ordered_descent<MAXD, QUAD, true>(a, T.root, vmask, cut, me.x, me.y, me.z, T.G, T.eps2, s_node_all[wv],
+                                      s_meta_all[wv], s_mask_all[wv], s_ring_all[wv], rx, ry, rz, cpp, cnp, fail,
+                                      tgt);
     if (fail && lane == 0) atomicOr(a.err, 2u);
     if (valid) {
         const uint32_t o = a.perm[tgt];
@@ -337,14 +498,6 @@ __device__ __forceinline__ void visit_pair(const NodeRec &r0, const NodeRec &r1,
 // splitting its blocks (the sums meeting in k_walk_split_reduce) give the
 // same bits.  The opening decisions, and so the interaction set, are those
 // of the plain depth-first walk.
-constexpr int kTopDepth = 7;
-constexpr int kMaxFrontier = 1 << kTopDepth;
-constexpr int kFrontierBlock = 8;
-static_assert(1 + kMaxFrontier / kFrontierBlock == kSplitSlots, "split slots: s_top and one per block");
-
-__device__ __forceinline__ bool lbvh_top(const NodeRec &r, int depth, int cut) {
-    return r.topo.z >= 0 && r.topo.y > cut && depth < kTopDepth;
-}
 
 // A lane accepts node nd (slot sums W): its interaction (monopole or
 // quadrupole, qn the node's two moment records).
@@ -1372,6 +1525,23 @@ void launch_walk_lbvh(const WalkArgs &a, int ordered, int max_depth, int stats, 
         if (stats >= 2) launch_fast<2>(a, s);
         else if (stats == 1) launch_fast<1>(a, s);
         else launch_fast<0>(a, s);
+    } else if (a.nsys == 0 && a.split > 1 && a.split_part_f) {
+        constexpr int W = kOrderedWaves;
+        const int64_t groups = (a.nt + kWave - 1) / kWave;
+        const dim3 gs((unsigned)((groups * a.split + W - 1) / W)), gc((unsigned)((groups + W - 1) / W)), t(64 * W);
+        if (a.quad && max_depth <= 32) {
+            hipLaunchKernelGGL((k_walk_lbvh_ordered_split<32, true>), gs, t, 0, s, a);
+            hipLaunchKernelGGL((k_walk_lbvh_ordered_combine<32, true>), gc, t, 0, s, a);
+        } else if (a.quad) {
+            hipLaunchKernelGGL((k_walk_lbvh_ordered_split<64, true>), gs, t, 0, s, a);
+            hipLaunchKernelGGL((k_walk_lbvh_ordered_combine<64, true>), gc, t, 0, s, a);
+        } else if (max_depth <= 32) {
+            hipLaunchKernelGGL((k_walk_lbvh_ordered_split<32, false>), gs, t, 0, s, a);
+            hipLaunchKernelGGL((k_walk_lbvh_ordered_combine<32, false>), gc, t, 0, s, a);
+        } else {
+            hipLaunchKernelGGL((k_walk_lbvh_ordered_split<64, false>), gs, t, 0, s, a);
+            hipLaunchKernelGGL((k_walk_lbvh_ordered_combine<64, false>), gc, t, 0, s, a);
+        }
     } else {
         constexpr int W = kOrderedWaves;
         const unsigned blocks = (unsigned)((walk_waves(a) + W - 1) / W);
