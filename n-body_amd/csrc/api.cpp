@@ -228,8 +228,9 @@ bool is_lbvh(int solver) { return solver == NBG_LINEAR_BVH || solver == NBG_LINE
 // frontier holds at most 8 blocks, so it uses at most 8 waves.
 int split_of(const nbg_sys *s) {
     const bool lbvh = is_lbvh(s->prm.solver);
+    const bool tree = lbvh || s->prm.solver == NBG_BARNES_HUT || s->prm.solver == NBG_BARNES_HUT_QUADRUPOLE;
     if (s->prm.sum_mode != NBG_SUM_FAST) {
-        if (!lbvh || s->n > kOrderedSplitMaxBodies) return 1;
+        if (!tree || s->n > kOrderedSplitMaxBodies) return 1;
         return s->split_req > 0 ? s->split_req : split_waves(s->n);
     }
     if (!lbvh && !(is_octree(s->prm.solver) && s->prm.solver != NBG_IMPLICIT_FMM)) return 1;

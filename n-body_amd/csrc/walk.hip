@@ -1083,11 +1083,23 @@ __device__ __forceinline__ uint64_t oct_eval_ordered(const OctRec &nd, const flo
     return 0;
 }
 
-template <int MAXL, bool QUAD, bool FOREST>
-__global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_oct_ordered(WalkArgs a) {
-    __shared__ int s_node_all[kOrderedWaves][MAXL];
-    __shared__ int s_pos_all[kOrderedWaves][MAXL];  // next child index | next octant << 8
-    __shared__ uint64_t s_mask_all[kOrderedWaves][MAXL];
+// The top of an octree for the ORDERED split: internal nodes at depth 0 or 1
+// holding more than `cut` bodies.  Their other children are the frontier
+// (at most 8 + 64 nodes).
+__device__ __forceinline__ bool oct_top_ordered(const OctRec &r, int depth, int cut) {
+    return r.topo.z >= 0 && r.topo.y > cut && depth < 2;
+}
+
+// The ORDERED octree descent from node `start` for the lanes of `smask`: the
+// node's result into r (assigned, as the reference's recursion returns it),
+// counts into cpp / cnp.  COMBINE: a child that is not a top node (with the
+// root's cut) is a frontier node: its result is taken from split_part_f
+// (slot fi, counted in visit order) instead of being evaluated.
+template <int MAXL, bool QUAD, bool COMBINE>
+__device__ __forceinline__ void oct_ordered_descent(const WalkArgs &a, int start, uint64_t smask, int cut, float px,
+                                                    float py, float pz, float G, float eps2, int *s_node, int *s_pos,
+                                                    uint64_t *s_mask, float &rx, float &ry, float &rz, uint32_t &cpp,
+                                                    uint32_t &cnp, bool &fail, int64_t tgt) {
     // Per lane and level, the bracket state (R, A, B; 3 floats each).  The
     // top frame's state lives in registers (cur); the frames below it are
     // parked in a private array indexed by the wave-uniform level (scratch
@@ -1095,37 +1107,31 @@ __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_oct_ordered(WalkArg
     // completes, so the child visits in between touch registers only.
     float s_st[MAXL * 9];
     float cur[9];
-
     const float4 *__restrict__ bodies = a.bodies;
     const OctRec *__restrict__ nodes = a.onodes;
-    const int lane = lane_id(), wv = uniform_i32(threadIdx.x / kWave);
-    int *s_node = s_node_all[wv];
-    int *s_pos = s_pos_all[wv];
-    uint64_t *s_mask = s_mask_all[wv];
-    const WaveTask T = wave_task<FOREST>(a, (int64_t)xcd_block(blockIdx.x, gridDim.x) * kOrderedWaves + wv);
-    const int64_t tgt = T.base + lane;
-    const bool valid = tgt < T.end;
-    const uint64_t vmask = __ballot(valid);
-    if (vmask == 0) return;
-    if (FOREST && (T.root < 0 || T.root >= a.max_nodes)) {
-        if (lane == 0) atomicOr(a.err, 2u);
-        return;
-    }
-
-    const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float px = me.x, py = me.y, pz = me.z;
-    const float G = T.G, eps2 = T.eps2;
-    float rx = 0.0f, ry = 0.0f, rz = 0.0f;
-    uint32_t cpp = 0, cnp = 0;
-    bool fail = false;
-
+    const int lane = lane_id();
+    int fi = 0;
+    auto frontier = [&](uint64_t O) {
+        if (fi >= kOrderedSplitSlots) { fail = true; return; }
+        if ((O >> lane) & 1ull) {
+            const int64_t o = (int64_t)fi * 3 * a.split_stride + tgt;
+            rx = a.split_part_f[o];
+            ry = a.split_part_f[o + a.split_stride];
+            rz = a.split_part_f[o + 2 * a.split_stride];
+        }
+        ++fi;
+    };
     int sp = 0;
     {
-        const OctRec root = nodes[T.root];
-        const uint64_t O = oct_eval_ordered<QUAD>(root, a.quad + 2 * T.root, vmask, lane, px, py, pz, G, eps2,
+        const OctRec root = nodes[start];
+        if (COMBINE && !oct_top_ordered(root, 0, cut)) {
+            frontier(smask);
+            return;
+        }
+        const uint64_t O = oct_eval_ordered<QUAD>(root, a.quad + 2 * start, smask, lane, px, py, pz, G, eps2,
                                                   bodies, a.n, rx, ry, rz, cpp, cnp, fail);
         if (O) {
-            if (lane == 0) { s_node[0] = T.root; s_pos[0] = 0; s_mask[0] = O; }
+            if (lane == 0) { s_node[0] = start; s_pos[0] = 0; s_mask[0] = O; }
             sp = 1;
         }
     }
@@ -1163,6 +1169,10 @@ __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_oct_ordered(WalkArg
         }
         if (lane == 0) s_pos[top] = (j + 1) | (q << 8);
         const OctRec cr = nodes[cb + j];
+        if (COMBINE && !oct_top_ordered(cr, top + 1, cut)) {
+            frontier(O);
+            continue;
+        }
         const uint64_t open = oct_eval_ordered<QUAD>(cr, a.quad + 2 * (cb + j), O, lane, px, py, pz, G, eps2, bodies,
                                                      a.n, rx, ry, rz, cpp, cnp, fail);
         if (open) {
@@ -1175,6 +1185,30 @@ __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_oct_ordered(WalkArg
             ++sp;
         }
     }
+}
+
+// The ORDERED octree walk, one wave per 64 targets from the root.
+template <int MAXL, bool QUAD, bool FOREST>
+__global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_oct_ordered(WalkArgs a) {
+    __shared__ int s_node_all[kOrderedWaves][MAXL];
+    __shared__ int s_pos_all[kOrderedWaves][MAXL];  // next child index | next octant << 8
+    __shared__ uint64_t s_mask_all[kOrderedWaves][MAXL];
+    const int lane = lane_id(), wv = uniform_i32(threadIdx.x / kWave);
+    const WaveTask T = wave_task<FOREST>(a, (int64_t)xcd_block(blockIdx.x, gridDim.x) * kOrderedWaves + wv);
+    const int64_t tgt = T.base + lane;
+    const bool valid = tgt < T.end;
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0) return;
+    if (FOREST && (T.root < 0 || T.root >= a.max_nodes)) {
+        if (lane == 0) atomicOr(a.err, 2u);
+        return;
+    }
+    const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+    uint32_t cpp = 0, cnp = 0;
+    bool fail = false;
+    oct_ordered_descent<MAXL, QUAD, false>(a, T.root, vmask, 0, me.x, me.y, me.z, T.G, T.eps2, s_node_all[wv],
+                                           s_pos_all[wv], s_mask_all[wv], rx, ry, rz, cpp, cnp, fail, tgt);
     if (fail && lane == 0) atomicOr(a.err, 2u);
     if (valid) {
         const uint32_t o = a.perm[tgt];
@@ -1187,6 +1221,141 @@ __global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_oct_ordered(WalkArg
     }
 }
 
+// ORDERED octree split (one system): a.split waves per group of 64 targets.
+// Each lists the frontier in the ORDERED walk's visit order (masks: the lanes
+// that evaluate the node), and wave k computes the results of entries
+// i with i mod a.split == k, as the walk would (each node's result is
+// assigned, not accumulated), for k_walk_oct_ordered_combine.
+template <int MAXL, bool QUAD>
+__global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_oct_ordered_split(WalkArgs a) {
+    __shared__ int s_node_all[kOrderedWaves][MAXL];
+    __shared__ int s_pos_all[kOrderedWaves][MAXL];
+    __shared__ uint64_t s_mask_all[kOrderedWaves][MAXL];
+    __shared__ int f_node_all[kOrderedWaves][kOrderedSplitSlots];
+    __shared__ uint64_t f_mask_all[kOrderedWaves][kOrderedSplitSlots];
+    const int lane = lane_id(), wv = uniform_i32(threadIdx.x / kWave);
+    const int K = a.split;
+    const int64_t gw = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kOrderedWaves + wv;
+    const int64_t grp = gw / K;
+    const int k = (int)(gw - grp * K);
+    const WaveTask T = wave_task<false>(a, grp);
+    const int64_t tgt = T.base + lane;
+    const bool valid = tgt < T.end;
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0) return;
+    const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const OctRec *__restrict__ nodes = a.onodes;
+    int *s_node = s_node_all[wv];
+    int *s_pos = s_pos_all[wv];
+    uint64_t *s_mask = s_mask_all[wv];
+    int *f_node = f_node_all[wv];
+    uint64_t *f_mask = f_mask_all[wv];
+    bool fail = false;
+
+    // the frontier, in the order the ORDERED walk meets it: frames over the
+    // top nodes, children in index order, a top child's frame walked before
+    // its later siblings
+    const OctRec root = nodes[T.root];
+    const int cut = (root.topo.y + 31) / 32;
+    int F = 0, sp = 0;
+    auto add = [&](int node, uint64_t m) {
+        if (F >= kOrderedSplitSlots) { fail = true; return; }
+        if (lane == 0) { f_node[F] = node; f_mask[F] = m; }
+        ++F;
+    };
+    if (!oct_top_ordered(root, 0, cut)) {
+        add(T.root, vmask);
+    } else {
+        const bool act = (vmask >> lane) & 1ull;
+        const uint64_t O = vmask & ~__ballot(act && oct_accepts(root, me.x, me.y, me.z));
+        if (O) {
+            if (lane == 0) { s_node[0] = T.root; s_pos[0] = 0; s_mask[0] = O; }
+            sp = 1;
+        }
+    }
+    while (sp > 0 && !fail) {
+        __builtin_amdgcn_wave_barrier();
+        const int top = sp - 1;
+        const int X = uniform_i32(s_node[top]);
+        const int j = uniform_i32(s_pos[top]);
+        const uint64_t O = uniform_u64(s_mask[top]);
+        if (X < 0 || X >= a.max_nodes) { fail = true; break; }
+        const OctRec xr = nodes[X];
+        const int cb = xr.topo.z, nch = oct_nchild(xr);
+        if (cb < 0 || nch < 1 || nch > 8 || (int64_t)cb + nch > a.max_nodes) { fail = true; break; }
+        if (j == nch) {
+            sp = top;
+            continue;
+        }
+        if (lane == 0) s_pos[top] = j + 1;
+        const OctRec cr = nodes[cb + j];
+        if (!oct_top_ordered(cr, top + 1, cut)) {
+            add(cb + j, O);
+            continue;
+        }
+        const bool act = (O >> lane) & 1ull;
+        const uint64_t open = O & ~__ballot(act && oct_accepts(cr, me.x, me.y, me.z));
+        if (open) {
+            if (sp >= MAXL) { fail = true; break; }
+            if (lane == 0) { s_node[sp] = cb + j; s_pos[sp] = 0; s_mask[sp] = open; }
+            ++sp;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    uint32_t cpp = 0, cnp = 0;
+    const int64_t stride = a.split_stride;
+    for (int i = k; i < F && !fail; i += K) {
+        const int f = uniform_i32(f_node[i]);
+        const uint64_t m = uniform_u64(f_mask[i]);
+        float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+        oct_ordered_descent<MAXL, QUAD, false>(a, f, m, cut, me.x, me.y, me.z, T.G, T.eps2, s_node, s_pos, s_mask, rx,
+                                               ry, rz, cpp, cnp, fail, tgt);
+        if (valid && ((m >> lane) & 1ull)) {
+            const int64_t o = (int64_t)i * 3 * stride + tgt;
+            a.split_part_f[o] = rx;
+            a.split_part_f[o + stride] = ry;
+            a.split_part_f[o + 2 * stride] = rz;
+        }
+    }
+    if (fail && lane == 0) atomicOr(a.err, 2u);
+    const uint32_t wpp = wave_sum(valid ? cpp : 0u), wnp = wave_sum(valid ? cnp : 0u);
+    if (lane == 0) {
+        atomicAdd(&a.counts[0], (unsigned long long)wpp);
+        atomicAdd(&a.counts[1], (unsigned long long)wnp);
+    }
+}
+
+// The ORDERED octree split's combine: the walk over the top nodes, each
+// frontier child's stored result applied to its parent's bracket.
+template <int MAXL, bool QUAD>
+__global__ __launch_bounds__(64 * kOrderedWaves) void k_walk_oct_ordered_combine(WalkArgs a) {
+    __shared__ int s_node_all[kOrderedWaves][MAXL];
+    __shared__ int s_pos_all[kOrderedWaves][MAXL];
+    __shared__ uint64_t s_mask_all[kOrderedWaves][MAXL];
+    const int lane = lane_id(), wv = uniform_i32(threadIdx.x / kWave);
+    const WaveTask T = wave_task<false>(a, (int64_t)xcd_block(blockIdx.x, gridDim.x) * kOrderedWaves + wv);
+    const int64_t tgt = T.base + lane;
+    const bool valid = tgt < T.end;
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0) return;
+    const float4 me = valid ? a.targets[tgt] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const int cut = (a.onodes[T.root].topo.y + 31) / 32;
+    float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+    uint32_t cpp = 0, cnp = 0;
+    bool fail = false;
+    oct_ordered_descent<MAXL, QUAD, true>(a, T.root, vmask, cut, me.x, me.y, me.z, T.G, T.eps2, s_node_all[wv],
+                                          s_pos_all[wv], s_mask_all[wv], rx, ry, rz, cpp, cnp, fail, tgt);
+    if (fail && lane == 0) atomicOr(a.err, 2u);
+    if (valid) {
+        const uint32_t o = a.perm[tgt];
+        if ((int64_t)o < a.nt) { a.ax[o] = rx; a.ay[o] = ry; a.az[o] = rz; }
+    }
+    const uint32_t wpp = wave_sum(valid ? cpp : 0u), wnp = wave_sum(valid ? cnp : 0u);
+    if (lane == 0) {
+        atomicAdd(&a.counts[0], (unsigned long long)wpp);
+        atomicAdd(&a.counts[1], (unsigned long long)wnp);
+    }
+}
 
 // The octree's top walk: the root, then (for a tree of at most
 // kSplitMaxBodies bodies) its child group when every child in it is internal
@@ -1454,6 +1623,24 @@ void launch_walk_oct(const WalkArgs &a, int ordered, int stats, hipStream_t s) {
         if (stats >= 2) launch_oct_fast<2>(a, s);
         else if (stats == 1) launch_oct_fast<1>(a, s);
         else launch_oct_fast<0>(a, s);
+    } else if (a.nsys == 0 && a.split > 1 && a.split_part_f) {
+        constexpr int W = kOrderedWaves;
+        const int64_t groups = (a.nt + kWave - 1) / kWave;
+        const dim3 gs((unsigned)((groups * a.split + W - 1) / W)), gc((unsigned)((groups + W - 1) / W)), t(64 * W);
+        const bool shallow = a.levels <= 18;
+        if (a.quad && shallow) {
+            hipLaunchKernelGGL((k_walk_oct_ordered_split<18, true>), gs, t, 0, s, a);
+            hipLaunchKernelGGL((k_walk_oct_ordered_combine<18, true>), gc, t, 0, s, a);
+        } else if (a.quad) {
+            hipLaunchKernelGGL((k_walk_oct_ordered_split<kOctLevels, true>), gs, t, 0, s, a);
+            hipLaunchKernelGGL((k_walk_oct_ordered_combine<kOctLevels, true>), gc, t, 0, s, a);
+        } else if (shallow) {
+            hipLaunchKernelGGL((k_walk_oct_ordered_split<18, false>), gs, t, 0, s, a);
+            hipLaunchKernelGGL((k_walk_oct_ordered_combine<18, false>), gc, t, 0, s, a);
+        } else {
+            hipLaunchKernelGGL((k_walk_oct_ordered_split<kOctLevels, false>), gs, t, 0, s, a);
+            hipLaunchKernelGGL((k_walk_oct_ordered_combine<kOctLevels, false>), gc, t, 0, s, a);
+        }
     } else {
         constexpr int W = kOrderedWaves;
         const unsigned blocks = (unsigned)((walk_waves(a) + W - 1) / W);

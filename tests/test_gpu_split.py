@@ -89,19 +89,24 @@ def test_forest_member_equals_alone_on_both_sides_of_the_split_limit(nbg, gpu_ct
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), f"n={b.n}"
 
 
+@pytest.mark.parametrize("tree", ["lbvh", "octree"])
 @pytest.mark.parametrize("quad", [False, True])
 @pytest.mark.parametrize("n,kind,bits", [(1, "plummer", 63), (65, "cube", 30), (1000, "plummer", 63),
                                          (4099, "cube", 30), (10000, "plummer", 63), (16384, "plummer", 30)])
-def test_ordered_split_bitexact(nbg, orc, gpu_ctx, n, kind, bits, quad):
-    """The ORDERED linear-BVH walk split over 1 to 16 waves per group (each
-    frontier subtree walked from zero, the results added back in the walk's
-    order) gives the oracle's bits and counts for every split."""
-    print(f"[ordered split] n={n} {kind} bits={bits} quad={quad}", flush=True)
+def test_ordered_split_bitexact(nbg, orc, gpu_ctx, n, kind, bits, quad, tree):
+    """The ORDERED walks split over 1 to 16 waves per group (each frontier
+    subtree's result computed from zero, as the reference's recursion
+    returns it, and used where the walk meets it) give the oracle's bits and
+    counts for every split."""
+    print(f"[ordered split] n={n} {kind} bits={bits} quad={quad} {tree}", flush=True)
     b = nbg.plummer(n, seed=5000 + n) if kind == "plummer" else nbg.cube(n, seed=5000 + n, side=2.0)
-    solver = nbg.LINEAR_BVH_QUADRUPOLE if quad else nbg.LINEAR_BVH
+    if tree == "lbvh":
+        solver, kind_id = (nbg.LINEAR_BVH_QUADRUPOLE, 4) if quad else (nbg.LINEAR_BVH, 1)
+    else:
+        solver, kind_id = (nbg.BARNES_HUT_QUADRUPOLE, 3) if quad else (nbg.BARNES_HUT, 0)
     p = _params(nbg, solver, bits=bits)
     p.sum_mode = nbg.SUM_ORDERED
-    t = orc.Tree(4 if quad else 1, b.x, b.y, b.z, b.m, theta=0.5, leaf=16, bits=bits, eps2=1e-4)
+    t = orc.Tree(kind_id, b.x, b.y, b.z, b.m, theta=0.5, leaf=16, bits=bits, eps2=1e-4)
     want, pp, npc = t.accel(b.x, b.y, b.z, counts=True)
     s = nbg.System(gpu_ctx, p, n)
     s.set_state(b)
