@@ -18,7 +18,10 @@
 // (activeTree.h:66-76).  Each lane's pending partial sums of the open depths
 // (the depth is wave-uniform) sit in an LDS ring for the deepest levels and a
 // private array for the rest (linear BVH), or in registers for the top frame
-// and a private array below it (octree).
+// and a private array below it (octree).  A subtree's result does not depend
+// on where it is computed, so a small system's walk can be split: frontier
+// subtrees walked from zero by several waves, their results used where the
+// walk over the top nodes meets them, with unchanged bits.
 //
 // FAST: the same opening decisions (the MAC is still exact) and so the same
 // interaction set, evaluated two siblings (linear BVH) or one child group
