@@ -199,7 +199,7 @@ int nbg_sys_walk_stats(nbg_sys *sys, uint64_t *out4);
 /* Waves per group of 64 targets in the tree walks (FAST: linear BVH with one
  * target per lane, octree; ORDERED: linear BVH; no reference counterpart: a
  * schedule choice).  0 (the default) chooses by size: systems of at most
- * 65,536 bodies (ORDERED: 16,384) fill the GPU with up to 16 waves per group
+ * 65,536 bodies fill the GPU with up to 16 waves per group
  * (FAST octree: 8), larger ones use 1.  Results are the same bits for every
  * choice: FAST sums each target's interactions per fixed block of subtrees and
  * adds the blocks in one order (the forest and the several-targets-per-lane

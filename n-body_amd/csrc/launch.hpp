@@ -282,7 +282,7 @@ constexpr int64_t kSplitMaxBodies = 65536;
 // ORDERED split: one result per frontier subtree (at most kOrderedSplitSlots)
 // per target, for systems of at most kOrderedSplitMaxBodies bodies.
 constexpr int kOrderedSplitSlots = 128;
-constexpr int64_t kOrderedSplitMaxBodies = 16384;
+constexpr int64_t kOrderedSplitMaxBodies = 65536;
 // Split waves per group for a system of n bodies (1: the chip is full).
 inline int split_waves(int64_t n) {
     if (n > kSplitMaxBodies || n < 1) return 1;

@@ -92,7 +92,8 @@ def test_forest_member_equals_alone_on_both_sides_of_the_split_limit(nbg, gpu_ct
 @pytest.mark.parametrize("tree", ["lbvh", "octree"])
 @pytest.mark.parametrize("quad", [False, True])
 @pytest.mark.parametrize("n,kind,bits", [(1, "plummer", 63), (65, "cube", 30), (1000, "plummer", 63),
-                                         (4099, "cube", 30), (10000, "plummer", 63), (16384, "plummer", 30)])
+                                         (4099, "cube", 30), (10000, "plummer", 63), (16384, "plummer", 30),
+                                         (65536, "cube", 63)])
 def test_ordered_split_bitexact(nbg, orc, gpu_ctx, n, kind, bits, quad, tree):
     """The ORDERED walks split over 1 to 16 waves per group (each frontier
     subtree's result computed from zero, as the reference's recursion
