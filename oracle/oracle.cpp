@@ -16,6 +16,9 @@
 //   * octree partition (Octree.h:33-61 uses std::partition): stable, so the
 //     bodies of a leaf appear in ascending original index.
 // These two orders are the only freedoms; everything else is the reference's.
+// Parity unpinned: these two orders (only the reference binary could pin them,
+// and it is itself nondeterministic there); they move summation order inside
+// COM and leaf sums, never the set of interactions.
 //
 // Parity pins (see tests/test_oracle.py): reference KATs gravity.cpp:13-26,
 // centerOfMassSummary.cpp:12-47, boundingBox.cpp:12-32; closed forms for
