@@ -363,6 +363,16 @@ int nbg_batch_autotune_grid(nbg_batch *b, const int32_t *leaf_sizes, int32_t kl,
  * autotuner sets it too.  Results do not depend on it. */
 int nbg_batch_set_targets_per_lane(nbg_batch *b, int32_t tpl);
 
+/* Waves per group of 64 targets in the batch's FAST walks (no reference
+ * counterpart: a schedule choice, as nbg_sys_set_walk_split).  0 (the
+ * default) chooses by size: a forest of at most 65,536 bodies in all, walked
+ * with one target per lane and with at least half of its bodies in systems of
+ * 4,096 bodies or more, splits each group over up to 16 waves (octree: 8);
+ * 1 never splits.  Each member's results are the same bits for every
+ * choice.  A FAST batch keeps 408 bytes per body, for up to 65,536 bodies,
+ * of split sums in HBM. */
+int nbg_batch_set_walk_split(nbg_batch *b, int32_t waves);
+
 /* ---- serving: continuous batching over a forest ----
  *
  * No reference counterpart.  Jobs (a linear-BVH system, 30-bit keys, plus a

@@ -48,7 +48,7 @@ EXPORTS = (
     "nbg_batch_bytes", "nbg_batch_create", "nbg_batch_destroy", "nbg_batch_add", "nbg_batch_remove",
     "nbg_batch_set_params", "nbg_batch_info", "nbg_batch_get", "nbg_batch_update_accelerations", "nbg_batch_step",
     "nbg_batch_synchronize", "nbg_batch_counts", "nbg_batch_enable_counts", "nbg_batch_use_graph",
-    "nbg_batch_autotune", "nbg_batch_autotune_grid", "nbg_batch_set_targets_per_lane", "nbg_batch_nonfinite",
+    "nbg_batch_autotune", "nbg_batch_autotune_grid", "nbg_batch_set_targets_per_lane", "nbg_batch_set_walk_split", "nbg_batch_nonfinite",
     "nbg_scenario_load_json", "nbg_scenario_load_tipsy", "nbg_scenario_count", "nbg_scenario_get", "nbg_scenario_free",
     "nbg_server_create", "nbg_server_create_express", "nbg_server_destroy", "nbg_server_capacity", "nbg_server_submit", "nbg_server_wait",
     "nbg_server_stats", "nbg_server_checkpoint", "nbg_server_suspend", "nbg_server_resume",
@@ -138,6 +138,7 @@ def lib() -> ctypes.CDLL:
             "nbg_batch_autotune_grid": (ctypes.c_int, [_vp, _i32p, ctypes.c_int32, _i32p, ctypes.c_int32,
                                                        ctypes.c_int32, _i32p, _i32p, _f32p]),
             "nbg_batch_set_targets_per_lane": (ctypes.c_int, [_vp, ctypes.c_int32]),
+            "nbg_batch_set_walk_split": (ctypes.c_int, [_vp, ctypes.c_int32]),
             "nbg_scenario_load_json": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
             "nbg_scenario_load_tipsy": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
             "nbg_scenario_count": (ctypes.c_int64, [_vp]),
@@ -541,6 +542,12 @@ class Batch:
 
     def set_targets_per_lane(self, tpl: int):
         _check(lib().nbg_batch_set_targets_per_lane(self.handle, int(tpl)))
+
+    def set_walk_split(self, waves: int = 0):
+        """Waves per group of 64 targets in the FAST walks: 0 by size (forests
+        of at most 65,536 bodies, at least half of them in systems of 4,096 or
+        more, split), 1 off, or 2, 4, 8, 16."""
+        _check(lib().nbg_batch_set_walk_split(self.handle, int(waves)))
 
     def autotune(self, leaf_sizes=(8, 16, 32, 64), repeats: int = 2):
         ls = np.asarray(leaf_sizes, np.int32)

@@ -94,6 +94,13 @@ struct nbg_batch {
     // targets per lane of the FAST linear-BVH walk (the autotuner's
     // bodies-per-wavefront choice: 64 x tpl); the other walks use 1
     int32_t tpl = 1;
+    // split FAST walks of small forests (several waves per group of 64
+    // targets): per-target sums for up to kSplitMaxBodies bodies and one
+    // block count per wave; split_req 0 = by size, 1 = off
+    double *split_part = nullptr;
+    int32_t *split_nb = nullptr;
+    int64_t split_bodies = 0, split_nb_cap = 0;
+    int32_t split_req = 0;
 
     float *x() { return st[0]; }
     float *y() { return st[1]; }
@@ -123,6 +130,8 @@ int64_t bytes_for(int64_t cap_n, int64_t cap_s) {
     b += 4 * sort_status_words(n, 64);                        // sort status
     b += (int64_t)sizeof(NodeRec) * 2 * n + 2 * 16 * 2 * n + 4 * 2 * n;  // nodes, bmin/bmax, flags
     b += 3 * 8 * (s + 1) + 2 * 4 * s + 16 * s + (int64_t)sizeof(MacParam) * s + 24 * s + 4 * 32 + 4 * s;
+    const int64_t ns = std::min(n, kSplitMaxBodies);
+    b += 8 * kSplitSlots * 3 * ns + 4 * (ns / 64 + s + 1);   // split walk sums, block counts
     return b;
 }
 
@@ -145,7 +154,7 @@ void free_batch(nbg_batch *b) {
                                 b->kgamma, b->kparent, b->ksplit, b->krank, b->leaf_start, b->glist, b->scan_ws,
                                 b->sort_status, b->nodes, b->bmin, b->bmax, b->flags, b->sys_first, b->wave_first,
                                 b->old_first, b->sys_root, b->leaf_tab, b->sys_prm, b->mp_tab, b->bbox6, b->misc,
-                                b->quad, b->bad};
+                                b->quad, b->bad, b->split_part, b->split_nb};
     for (float *p : b->st) ptrs.push_back(p);
     for (float *p : b->stage) ptrs.push_back(p);
     for (void *p : ptrs)
@@ -174,6 +183,39 @@ int check_member_params(const nbg_params *p) {
 int walk_tpl(const nbg_batch *b) {
     const bool lbvh = b->solver == NBG_LINEAR_BVH || b->solver == NBG_LINEAR_BVH_QUADRUPOLE;
     return lbvh && b->sum_mode == NBG_SUM_FAST ? b->tpl : 1;
+}
+
+// Waves per group of the forest's FAST walk: a forest of at most
+// kSplitMaxBodies bodies in all (one target per lane) fills the chip by
+// splitting each group's frontier blocks over up to 16 waves (the octree
+// over 8), as a single small system does.  By size, only when at least half
+// of the bodies are in systems of kSplitMinMember bodies or more: a tiny
+// system's tree has too few frontier blocks to share, and its extra waves
+// cost more than they save.  The block sums, and so the bits, do not depend
+// on the split.
+constexpr int64_t kSplitMinMember = 4096;
+
+int forest_split(const nbg_batch *b) {
+    if (b->sum_mode != NBG_SUM_FAST || !b->split_part || b->split_req == 1 || walk_tpl(b) != 1) return 1;
+    if (b->n > b->split_bodies || b->nwaves > b->split_nb_cap) return 1;
+    if (b->split_req == 0) {
+        int64_t big = 0;
+        for (const Member &m : b->sys)
+            if (m.n >= kSplitMinMember) big += m.n;
+        if (2 * big < b->n) return 1;
+    }
+    int k = b->split_req > 1 ? b->split_req : split_waves_groups(b->nwaves);
+    if (is_octree_solver(b->solver)) k = std::min(k, 8);
+    return k;
+}
+
+void set_split(const nbg_batch *b, WalkArgs &wa) {
+    wa.split = forest_split(b);
+    if (wa.split > 1) {
+        wa.split_part = b->split_part;
+        wa.split_nb = b->split_nb;
+        wa.split_stride = b->n;
+    }
 }
 
 int ensure_quad(nbg_batch *b, int64_t slots) {
@@ -331,6 +373,7 @@ void enqueue_pass_octree(nbg_batch *b) {
     wa.sys_root = b->sys_root;
     wa.sys_prm = b->sys_prm;
     wa.quad = quad;
+    set_split(b, wa);
     const bool ordered = b->sum_mode == NBG_SUM_ORDERED;
     launch_walk_oct(wa, ordered, (b->counting || ordered) ? 1 : 0, s);
 }
@@ -386,6 +429,7 @@ void enqueue_pass(nbg_batch *b) {
     wa.sys_prm = b->sys_prm;
     wa.quad = quad;
     wa.tpl = walk_tpl(b);
+    set_split(b, wa);
     const bool ordered = b->sum_mode == NBG_SUM_ORDERED;
     launch_walk_lbvh(wa, ordered, 32, (b->counting || ordered) ? 1 : 0, s);
 }
@@ -463,6 +507,12 @@ int nbg_batch_create(nbg_ctx *ctx, int64_t max_bodies, int32_t max_systems, int3
     A(&b->lv_tab, S);
     A(&b->misc, 32);
     A(&b->bad, S);
+    if (sum_mode == NBG_SUM_FAST) {
+        b->split_bodies = std::min(n, kSplitMaxBodies);
+        b->split_nb_cap = b->split_bodies / 64 + S + 1;
+        A(&b->split_part, (int64_t)kSplitSlots * 3 * b->split_bodies);
+        A(&b->split_nb, b->split_nb_cap);
+    }
     if (r == NBG_OK) {
         hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipMemsetAsync(b->misc, 0, 32 * 4, b->stream);
@@ -747,6 +797,15 @@ int nbg_batch_autotune(nbg_batch *b, const int32_t *leaf_sizes, int32_t k, int32
     if (!b) return fail(NBG_ERR_ARG, "batch is NULL");
     const int32_t tpl = b->tpl;
     return nbg_batch_autotune_grid(b, leaf_sizes, k, &tpl, 1, repeats, best, nullptr, ms_out);
+}
+
+int nbg_batch_set_walk_split(nbg_batch *b, int32_t waves) {
+    if (!b) return fail(NBG_ERR_ARG, "batch is NULL");
+    if (waves != 0 && waves != 1 && waves != 2 && waves != 4 && waves != 8 && waves != 16)
+        return fail(NBG_ERR_ARG, "walk split: 0 (by size), 1, 2, 4, 8 or 16 waves");
+    b->split_req = waves;
+    b->tables_dirty = true;
+    return NBG_OK;
 }
 
 int nbg_batch_set_targets_per_lane(nbg_batch *b, int32_t tpl) {

@@ -283,13 +283,16 @@ constexpr int64_t kSplitMaxBodies = 65536;
 // per target, for systems of at most kOrderedSplitMaxBodies bodies.
 constexpr int kOrderedSplitSlots = 128;
 constexpr int64_t kOrderedSplitMaxBodies = 65536;
-// Split waves per group for a system of n bodies (1: the chip is full).
-inline int split_waves(int64_t n) {
-    if (n > kSplitMaxBodies || n < 1) return 1;
-    const int64_t groups = (n + 63) / 64;
+// Split waves per group for a walk of `groups` groups of 64 targets, and
+// for a system of n bodies (1: the chip is full).
+inline int split_waves_groups(int64_t groups) {
     int k = 1;
     while (k < 16 && groups * k * 2 <= 8192) k *= 2;
     return k;
+}
+inline int split_waves(int64_t n) {
+    if (n > kSplitMaxBodies || n < 1) return 1;
+    return split_waves_groups((n + 63) / 64);
 }
 
 // Barnes-Hut traversal over the LBVH.  ordered != 0 selects the

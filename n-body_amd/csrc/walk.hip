@@ -699,12 +699,12 @@ __device__ __forceinline__ void walk_subtree(const NodeRec *__restrict__ nodes, 
 // FAST linear-BVH walk.  TPL targets per lane: the wave serves 64 TPL
 // targets from its task's base, slot t of lane l being target base + 64 t
 // + l.  FOREST: the wave's system from the forest's wave table (wave_task).
-// SPLIT (one system, TPL 1): a.split waves serve each group of 64 targets;
-// wave k of a group walks the blocks b with b mod a.split == k (wave 0 also
-// the top nodes) and stores their sums for k_walk_split_reduce.
+// SPLIT (TPL 1, one system or a forest): a.split waves serve each group of
+// 64 targets; wave k of a group walks the blocks b with b mod a.split == k
+// (wave 0 also the top nodes) and stores their sums for k_walk_split_reduce.
 template <int WAVES, bool SOFT, int STATS, bool QUAD, int TPL, bool FOREST, bool SPLIT>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_fast(WalkArgs a) {
-    static_assert(!SPLIT || (TPL == 1 && !FOREST), "split groups are one-slot single-system walks");
+    static_assert(!SPLIT || TPL == 1, "split groups are one-slot walks");
     __shared__ int s_node[WAVES][kStackFast];
     __shared__ uint64_t s_mask[WAVES][kStackFast * TPL];
     __shared__ float4 s_leaf[WAVES][kLeafScratch];
@@ -991,12 +991,27 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_lbvh_plain(WalkArgs a) {
 }
 
 // The split walk's sums, ((s_top + B_0) + B_1) + ... for each target in
-// FP64, times G, rounded once and stored in output order.
+// FP64, times G, rounded once and stored in output order.  FOREST: the
+// target's group and G from its system (the forest's waves start at each
+// system's first body).
+template <bool FOREST>
 __global__ __launch_bounds__(256) void k_walk_split_reduce(WalkArgs a) {
     const int64_t tgt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tgt >= a.nt) return;
     const int64_t stride = a.split_stride;
-    int nb = a.split_nb[tgt / kWave];
+    int64_t grp = tgt / kWave;
+    double g = (double)a.G;
+    if (FOREST) {
+        int lo = 0, hi = a.nsys;  // sys_first[lo] <= tgt < sys_first[lo + 1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (a.sys_first[mid] <= tgt) lo = mid;
+            else hi = mid;
+        }
+        grp = a.wave_first[lo] + (tgt - a.sys_first[lo]) / kWave;
+        g = (double)a.sys_prm[lo].x;
+    }
+    int nb = a.split_nb[grp];
     if (nb < 0 || nb > kMaxFrontier / kFrontierBlock) {
         atomicOr(a.err, 2u);
         nb = 0;
@@ -1010,7 +1025,6 @@ __global__ __launch_bounds__(256) void k_walk_split_reduce(WalkArgs a) {
     }
     const uint32_t o = a.perm[tgt];
     if ((int64_t)o < a.nt) {
-        const double g = (double)a.G;
         a.ax[o] = (float)(S[0] * g);
         a.ay[o] = (float)(S[1] * g);
         a.az[o] = (float)(S[2] * g);
@@ -1419,12 +1433,11 @@ __device__ int oct_top_walk(const OctRec *__restrict__ nodes, const float4 *__re
 }
 
 // FAST octree walk.  FOREST: the wave's system from the forest's wave table.
-// SPLIT (one system): a.split waves per group of 64 targets, wave k walking
+// SPLIT: a.split waves per group of 64 targets, wave k walking
 // the blocks b with b mod a.split == k (wave 0 also the top nodes), the sums
 // meeting in k_walk_split_reduce.
 template <int WAVES, bool SOFT, int STATS, bool QUAD, bool FOREST, bool SPLIT>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_oct_fast(WalkArgs a) {
-    static_assert(!(SPLIT && FOREST), "split groups are single-system walks");
     __shared__ int s_grp[WAVES][kStackOct];       // first child | nchild << 28
     __shared__ uint64_t s_mask[WAVES][kStackOct];
     __shared__ float4 s_leaf[WAVES][kLeafScratch];
@@ -1599,7 +1612,12 @@ static void launch_oct_fast_q(const WalkArgs &a, hipStream_t s) {
     constexpr int W = 4;
     const unsigned blocks = (unsigned)((walk_waves(a) + W - 1) / W);
     const dim3 g(blocks), t(64 * W);
-    if (a.nsys > 0) {
+    if (a.nsys > 0 && a.split > 1 && a.split_part && a.split_nb) {
+        const dim3 gs((unsigned)((a.nwaves * a.split + W - 1) / W));
+        if (a.eps2 > 0.0f) hipLaunchKernelGGL((k_walk_oct_fast<W, true, STATS, QUAD, true, true>), gs, t, 0, s, a);
+        else hipLaunchKernelGGL((k_walk_oct_fast<W, false, STATS, QUAD, true, true>), gs, t, 0, s, a);
+        hipLaunchKernelGGL(k_walk_split_reduce<true>, dim3((unsigned)((a.nt + 255) / 256)), dim3(256), 0, s, a);
+    } else if (a.nsys > 0) {
         if (a.eps2 > 0.0f) hipLaunchKernelGGL((k_walk_oct_fast<W, true, STATS, QUAD, true, false>), g, t, 0, s, a);
         else hipLaunchKernelGGL((k_walk_oct_fast<W, false, STATS, QUAD, true, false>), g, t, 0, s, a);
     } else if (a.split > 1 && a.split_part && a.split_nb) {
@@ -1607,7 +1625,7 @@ static void launch_oct_fast_q(const WalkArgs &a, hipStream_t s) {
         const dim3 gs((unsigned)((waves + W - 1) / W));
         if (a.eps2 > 0.0f) hipLaunchKernelGGL((k_walk_oct_fast<W, true, STATS, QUAD, false, true>), gs, t, 0, s, a);
         else hipLaunchKernelGGL((k_walk_oct_fast<W, false, STATS, QUAD, false, true>), gs, t, 0, s, a);
-        hipLaunchKernelGGL(k_walk_split_reduce, dim3((unsigned)((a.nt + 255) / 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_walk_split_reduce<false>, dim3((unsigned)((a.nt + 255) / 256)), dim3(256), 0, s, a);
     } else {
         if (a.eps2 > 0.0f) hipLaunchKernelGGL((k_walk_oct_fast<W, true, STATS, QUAD, false, false>), g, t, 0, s, a);
         else hipLaunchKernelGGL((k_walk_oct_fast<W, false, STATS, QUAD, false, false>), g, t, 0, s, a);
@@ -1680,6 +1698,11 @@ static void launch_fast_q(const WalkArgs &a, hipStream_t s) {
     const int tpl = a.tpl == 2 || a.tpl == 4 ? a.tpl : 1;
     if (a.nsys > 0) {
         // a forest's wave table counts waves of 64 tpl targets
+        if (tpl == 1 && a.split > 1 && a.split_part && a.split_nb) {
+            launch_lbvh_fast_k<STATS, QUAD, 1, true, true>(a, a.nwaves * a.split, s);
+            hipLaunchKernelGGL(k_walk_split_reduce<true>, dim3((unsigned)((a.nt + 255) / 256)), dim3(256), 0, s, a);
+            return;
+        }
         if (tpl == 2) launch_lbvh_fast_k<STATS, QUAD, 2, true, false>(a, a.nwaves, s);
         else if (tpl == 4) launch_lbvh_fast_k<STATS, QUAD, 4, true, false>(a, a.nwaves, s);
         else launch_lbvh_fast_k<STATS, QUAD, 1, true, false>(a, a.nwaves, s);
@@ -1690,7 +1713,7 @@ static void launch_fast_q(const WalkArgs &a, hipStream_t s) {
     if (tpl == 4) return launch_lbvh_fast_k<STATS, QUAD, 4, false, false>(a, (groups + 3) / 4, s);
     if (a.split > 1 && a.split_part && a.split_nb) {
         launch_lbvh_fast_k<STATS, QUAD, 1, false, true>(a, groups * a.split, s);
-        hipLaunchKernelGGL(k_walk_split_reduce, dim3((unsigned)((a.nt + 255) / 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_walk_split_reduce<false>, dim3((unsigned)((a.nt + 255) / 256)), dim3(256), 0, s, a);
         return;
     }
     if (a.n > kSplitMaxBodies) {

@@ -88,7 +88,7 @@ def test_serving_backfill(nbg, gpu_ctx):
     budget = nbg.Batch.bytes_for(10000, 8)
     srv = nbg.Server(gpu_ctx, budget, max_systems=8)
     cap, _ = srv.capacity()
-    first = srv.submit(_params(nbg, 0.5), nbg.plummer(cap // 2 + 100, seed=40), 300)
+    first = srv.submit(_params(nbg, 0.5), nbg.plummer(cap // 2 + 100, seed=40), 3000)
     time.sleep(0.2)  # the first job is running
     big = srv.submit(_params(nbg, 0.5), nbg.plummer(cap // 2 + 100, seed=41), 2)   # does not fit beside it
     small = srv.submit(_params(nbg, 0.5), nbg.plummer(500, seed=42), 2)            # fits beside it

@@ -63,6 +63,10 @@ def test_split_argument(nbg, gpu_ctx):
     with pytest.raises(nbg.NbgError):
         s.set_walk_split(3)
     s.close()
+    B = nbg.Batch(gpu_ctx, 1000, 4)
+    with pytest.raises(nbg.NbgError):
+        B.set_walk_split(32)
+    B.close()
 
 
 @pytest.mark.parametrize("tree", ["lbvh", "octree"])
@@ -87,6 +91,43 @@ def test_forest_member_equals_alone_on_both_sides_of_the_split_limit(nbg, gpu_ct
         want = s.update_accelerations()
         s.close()
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), f"n={b.n}"
+
+
+@pytest.mark.parametrize("tree", ["lbvh", "octree"])
+@pytest.mark.parametrize("quad", [False, True])
+def test_small_forest_split_same_bits(nbg, gpu_ctx, tree, quad):
+    """A FAST forest of at most 65,536 bodies in all splits each group over
+    several waves (the member's frontier blocks per wave, the sums met in
+    one reduce): every member keeps its bits and the counts for 1 to 16
+    waves, and equals the same system alone."""
+    if tree == "lbvh":
+        solver = nbg.LINEAR_BVH_QUADRUPOLE if quad else nbg.LINEAR_BVH
+    else:
+        solver = nbg.BARNES_HUT_QUADRUPOLE if quad else nbg.BARNES_HUT
+    sizes = [1, 2, 65, 700, 5000, 20000, 129, 31000]
+    systems = [nbg.plummer(m, seed=90 + i) if i % 2 else nbg.cube(m, seed=90 + i, side=2.0)
+               for i, m in enumerate(sizes)]
+    assert sum(sizes) <= 65536
+    p = _params(nbg, solver, bits=30)
+    runs = []
+    for k in (1, 2, 4, 8, 16, 0):
+        print(f"[forest split] {tree} quad={quad} waves={k}", flush=True)
+        B = nbg.Batch(gpu_ctx, 65536, 16, sum_mode=nbg.SUM_FAST)
+        ids = [B.add(p, b) for b in systems]
+        B.set_walk_split(k)
+        B.update_accelerations()
+        runs.append(([B.get(sid, accelerations=True)[1] for sid in ids], B.counts()))
+        B.close()
+    for k, (accs, c) in zip((2, 4, 8, 16, 0), runs[1:]):
+        assert c == runs[0][1], f"split {k}"
+        for b, a, a1 in zip(systems, accs, runs[0][0]):
+            assert np.array_equal(a.view(np.uint32), a1.view(np.uint32)), f"split {k}, n={b.n}"
+    for b, a in zip(systems, runs[-1][0]):
+        s = nbg.System(gpu_ctx, p, b.n)
+        s.set_state(b)
+        want = s.update_accelerations()
+        s.close()
+        assert np.array_equal(a.view(np.uint32), want.view(np.uint32)), f"alone, n={b.n}"
 
 
 @pytest.mark.parametrize("tree", ["lbvh", "octree"])
