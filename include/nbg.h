@@ -199,8 +199,8 @@ int nbg_sys_walk_stats(nbg_sys *sys, uint64_t *out4);
 /* Waves per group of 64 targets in the tree walks (FAST: linear BVH with one
  * target per lane, octree; ORDERED: linear BVH; no reference counterpart: a
  * schedule choice).  0 (the default) chooses by size: systems of at most
- * 65,536 bodies fill the GPU with up to 16 waves per group
- * (FAST octree: 8), larger ones use 1.  Results are the same bits for every
+ * 65,536 bodies fill the GPU with up to 16 waves per group, within 4,096
+ * waves in all (FAST: linear BVH 4, octree 8), larger ones use 1.  Results are the same bits for every
  * choice: FAST sums each target's interactions per fixed block of subtrees and
  * adds the blocks in one order (the forest and the several-targets-per-lane
  * walks use the same blocks); ORDERED computes each frontier subtree's result
@@ -367,7 +367,7 @@ int nbg_batch_set_targets_per_lane(nbg_batch *b, int32_t tpl);
  * counterpart: a schedule choice, as nbg_sys_set_walk_split).  0 (the
  * default) chooses by size: a forest of at most 65,536 bodies in all, walked
  * with one target per lane and with at least half of its bodies in systems of
- * 4,096 bodies or more, splits each group over up to 16 waves (octree: 8);
+ * 4,096 bodies or more, splits each group over up to 4 waves (octree: 8);
  * 1 never splits.  Each member's results are the same bits for every
  * choice.  A FAST batch keeps 408 bytes per body, for up to 65,536 bodies,
  * of split sums in HBM. */

@@ -187,8 +187,8 @@ int walk_tpl(const nbg_batch *b) {
 
 // Waves per group of the forest's FAST walk: a forest of at most
 // kSplitMaxBodies bodies in all (one target per lane) fills the chip by
-// splitting each group's frontier blocks over up to 16 waves (the octree
-// over 8), as a single small system does.  By size, only when at least half
+// splitting each group's frontier blocks over several waves (by size up
+// to 4 for the linear BVH, 8 for the octree), as a single small system does.  By size, only when at least half
 // of the bodies are in systems of kSplitMinMember bodies or more: a tiny
 // system's tree has too few frontier blocks to share, and its extra waves
 // cost more than they save.  The block sums, and so the bits, do not depend
@@ -206,6 +206,7 @@ int forest_split(const nbg_batch *b) {
     }
     int k = b->split_req > 1 ? b->split_req : split_waves_groups(b->nwaves);
     if (is_octree_solver(b->solver)) k = std::min(k, 8);
+    else if (b->split_req == 0) k = std::min(k, kSplitAutoLbvh);
     return k;
 }
 

@@ -279,15 +279,19 @@ struct WalkArgs {
 constexpr int kSplitSlots = 17;
 // Largest system whose FAST linear-BVH walk splits its groups over waves.
 constexpr int64_t kSplitMaxBodies = 65536;
+// Waves per group the FAST linear-BVH walk takes when chosen by size.
+constexpr int kSplitAutoLbvh = 4;
 // ORDERED split: one result per frontier subtree (at most kOrderedSplitSlots)
 // per target, for systems of at most kOrderedSplitMaxBodies bodies.
 constexpr int kOrderedSplitSlots = 128;
 constexpr int64_t kOrderedSplitMaxBodies = 65536;
 // Split waves per group for a walk of `groups` groups of 64 targets, and
-// for a system of n bodies (1: the chip is full).
+// for a system of n bodies (1: the chip is full): the most waves, up to 16
+// per group, that keep the launch within 4,096 waves (16 per CU; measured:
+// wider launches of the split walks run slower from 30k bodies up).
 inline int split_waves_groups(int64_t groups) {
     int k = 1;
-    while (k < 16 && groups * k * 2 <= 8192) k *= 2;
+    while (k < 16 && groups * k * 2 <= 4096) k *= 2;
     return k;
 }
 inline int split_waves(int64_t n) {
